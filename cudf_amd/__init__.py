@@ -1,0 +1,14 @@
+"""cudf_amd: an MI355X-native hash groupby-aggregate with cudf's API.
+
+The compute path is libcudf_amd.so (hand-written HIP kernels for gfx950 behind
+the C ABI in include/mgb_groupby.h). This package is its Python binding,
+shaped like pylibcudf's groupby."""
+from . import aggregation, hashing
+from .column import Column, Table, pack_validity, unpack_validity
+from .groupby import GroupBy, GroupByRequest
+from .types import Kind, NullPolicy, Sorted, TypeId
+
+__all__ = [
+    "aggregation", "hashing", "Column", "Table", "GroupBy", "GroupByRequest", "Kind", "NullPolicy",
+    "Sorted", "TypeId", "pack_validity", "unpack_validity",
+]

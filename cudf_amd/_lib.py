@@ -1,0 +1,148 @@
+"""ctypes binding of libcudf_amd.so (the C ABI declared in include/mgb_groupby.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (``make -C
+cudf_amd/csrc``). Loading fails loudly when it is missing: there is no CPU or
+PyTorch fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libcudf_amd.so")
+
+MGB_SUCCESS = 0
+MGB_ERROR_LOGIC = 1
+MGB_ERROR_UNSUPPORTED = 2
+MGB_ERROR_HIP = 3
+MGB_ERROR_ALLOC = 4
+
+
+class mgb_column(ctypes.Structure):
+    _fields_ = [
+        ("type_id", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("size", ctypes.c_int64),
+        ("data", ctypes.c_void_p),
+        ("null_mask", ctypes.c_void_p),
+        ("offset", ctypes.c_int64),
+        ("null_count", ctypes.c_int64),
+    ]
+
+
+class mgb_aggregation(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("ddof", ctypes.c_int32)]
+
+
+class mgb_request(ctypes.Structure):
+    _fields_ = [
+        ("values", mgb_column),
+        ("aggs", ctypes.POINTER(mgb_aggregation)),
+        ("num_aggs", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p)
+DEALLOC_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p)
+
+
+class mgb_allocator(ctypes.Structure):
+    _fields_ = [("allocate", ALLOC_FN), ("deallocate", DEALLOC_FN), ("ctx", ctypes.c_void_p)]
+
+
+class mgb_out_column(ctypes.Structure):
+    _fields_ = [
+        ("type_id", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("size", ctypes.c_int64),
+        ("data", ctypes.c_void_p),
+        ("null_mask", ctypes.c_void_p),
+        ("null_count", ctypes.c_int64),
+        ("data_bytes", ctypes.c_size_t),
+        ("mask_bytes", ctypes.c_size_t),
+    ]
+
+
+# Every symbol include/mgb_groupby.h declares, with its ctypes signature.
+SIGNATURES = {
+    "mgb_version": (ctypes.c_int32, []),
+    "mgb_last_error_message": (ctypes.c_char_p, []),
+    "mgb_target_type": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32]),
+    "mgb_groupby_aggregate": (
+        ctypes.c_int,
+        [
+            ctypes.POINTER(mgb_column), ctypes.c_int32, ctypes.c_int32,
+            ctypes.POINTER(mgb_request), ctypes.c_int32,
+            ctypes.POINTER(mgb_out_column), ctypes.POINTER(mgb_out_column),
+            ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.POINTER(mgb_allocator),
+        ],
+    ),
+    "mgb_murmurhash3_x86_32": (
+        ctypes.c_int,
+        [ctypes.POINTER(mgb_column), ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p],
+    ),
+    "mgb_spark_murmurhash3_x86_32": (
+        ctypes.c_int,
+        [ctypes.POINTER(mgb_column), ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p],
+    ),
+    "mgb_profiling_enable": (ctypes.c_int, [ctypes.c_int32]),
+    "mgb_profiling_read": (
+        ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)],
+    ),
+    "mgb_release_workspace": (ctypes.c_int, [ctypes.c_void_p]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class LibraryNotBuilt(RuntimeError):
+    pass
+
+
+def load():
+    """Returns the loaded library (raises LibraryNotBuilt if it was never built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise LibraryNotBuilt(
+                    f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                    "(make -C cudf_amd/csrc); there is no fallback implementation"
+                )
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+        return _lib
+
+
+class CudfError(Exception):
+    """Base of the errors raised from library status codes."""
+
+
+class LogicError(CudfError, ValueError):
+    """cudf::logic_error: invalid argument or type/aggregation pair."""
+
+
+class HipError(CudfError, RuntimeError):
+    """cudf::cuda_error: a HIP runtime call failed."""
+
+
+class AllocError(CudfError, MemoryError):
+    """rmm::bad_alloc: an allocation failed."""
+
+
+def check(status: int) -> None:
+    if status == MGB_SUCCESS:
+        return
+    msg = load().mgb_last_error_message().decode()
+    if status == MGB_ERROR_HIP:
+        raise HipError(msg)
+    if status == MGB_ERROR_ALLOC:
+        raise AllocError(msg)
+    raise LogicError(msg)

@@ -1,0 +1,297 @@
+// The cudf C++ API for the hash groupby path (cudf::groupby::groupby,
+// aggregation factories, cudf::hashing row hashers) implemented over the C
+// ABI of mgb_groupby.h. Argument checks and their exceptions follow
+// cpp/src/groupby/groupby.cu:173-219; results are columns whose buffers come
+// from the caller's memory resource.
+#include <cudf/aggregation.hpp>
+#include <cudf/groupby.hpp>
+#include <cudf/hashing.hpp>
+#include <cudf/utilities/error.hpp>
+
+#include <rmm/device_buffer.hpp>
+
+#include "mgb_groupby.h"
+
+#include <string>
+#include <vector>
+
+namespace cudf {
+
+std::size_t size_of(data_type t)
+{
+  switch (t.id()) {
+    case type_id::INT8: case type_id::UINT8: case type_id::BOOL8: return 1;
+    case type_id::INT16: case type_id::UINT16: return 2;
+    case type_id::INT32: case type_id::UINT32: case type_id::FLOAT32: case type_id::TIMESTAMP_DAYS:
+    case type_id::DURATION_DAYS: case type_id::DECIMAL32: return 4;
+    case type_id::INT64: case type_id::UINT64: case type_id::FLOAT64: case type_id::TIMESTAMP_SECONDS:
+    case type_id::TIMESTAMP_MILLISECONDS: case type_id::TIMESTAMP_MICROSECONDS: case type_id::TIMESTAMP_NANOSECONDS:
+    case type_id::DURATION_SECONDS: case type_id::DURATION_MILLISECONDS: case type_id::DURATION_MICROSECONDS:
+    case type_id::DURATION_NANOSECONDS: case type_id::DECIMAL64: return 8;
+    case type_id::DECIMAL128: return 16;
+    default: CUDF_FAIL("size_of: not a fixed-width type");
+  }
+}
+
+rmm::cuda_stream_view const get_default_stream() { return rmm::cuda_stream_view{}; }
+
+namespace detail {
+
+// One concrete aggregation type serving every factory; `param` carries ddof
+// (VARIANCE / STD) or n (NTH_ELEMENT).
+class simple_aggregation final : public groupby_aggregation,
+                                 public groupby_scan_aggregation,
+                                 public reduce_aggregation,
+                                 public rolling_aggregation {
+ public:
+  explicit simple_aggregation(aggregation::Kind k, size_type param = 0) : aggregation(k), _param{param} {}
+  [[nodiscard]] std::unique_ptr<aggregation> clone() const override
+  {
+    return std::make_unique<simple_aggregation>(kind, _param);
+  }
+  [[nodiscard]] bool is_equal(aggregation const& other) const override
+  {
+    auto const* o = dynamic_cast<simple_aggregation const*>(&other);
+    return o && o->kind == kind && o->_param == _param;
+  }
+  [[nodiscard]] size_type param() const { return _param; }
+
+ private:
+  size_type _param;
+};
+
+size_type aggregation_param(aggregation const& a)
+{
+  auto const* s = dynamic_cast<simple_aggregation const*>(&a);
+  return s ? s->param() : 0;
+}
+
+[[noreturn]] void throw_status(mgb_status st)
+{
+  std::string msg = mgb_last_error_message();
+  switch (st) {
+    case MGB_ERROR_HIP: throw cudf::cuda_error("CUDA error: " + msg);
+    case MGB_ERROR_ALLOC: throw rmm::bad_alloc(msg);
+    default: throw cudf::logic_error(msg);
+  }
+}
+
+// Bridges mgb_allocator to an rmm device_memory_resource. Exceptions from the
+// resource are turned into a NULL return (the C side reports MGB_ERROR_ALLOC).
+struct mr_allocator {
+  static void* allocate(void* ctx, size_t bytes, void* stream)
+  {
+    try {
+      return static_cast<rmm::mr::device_memory_resource*>(ctx)->allocate(
+        bytes, rmm::cuda_stream_view{static_cast<hipStream_t>(stream)});
+    } catch (...) {
+      return nullptr;
+    }
+  }
+  static void deallocate(void* ctx, void* p, size_t bytes, void* stream)
+  {
+    static_cast<rmm::mr::device_memory_resource*>(ctx)->deallocate(
+      p, bytes, rmm::cuda_stream_view{static_cast<hipStream_t>(stream)});
+  }
+  static mgb_allocator make(rmm::mr::device_memory_resource* mr)
+  {
+    return mgb_allocator{&mr_allocator::allocate, &mr_allocator::deallocate, mr};
+  }
+};
+
+mgb_column to_mgb(column_view const& c)
+{
+  mgb_column m{};
+  m.type_id = static_cast<int32_t>(c.type().id());
+  m.size = c.size();
+  m.data = c.head();
+  m.null_mask = c.null_mask();
+  m.offset = c.offset();
+  m.null_count = c.nullable() ? c.null_count() : 0;
+  return m;
+}
+
+std::unique_ptr<column> adopt(mgb_out_column const& o, rmm::cuda_stream_view stream,
+                              rmm::mr::device_memory_resource* mr)
+{
+  rmm::device_buffer data{o.data, o.data ? o.data_bytes : 0, stream, mr};
+  rmm::device_buffer mask{o.null_mask, o.null_mask ? o.mask_bytes : 0, stream, mr};
+  return std::make_unique<column>(data_type{static_cast<type_id>(o.type_id)}, static_cast<size_type>(o.size),
+                                  std::move(data), std::move(mask), static_cast<size_type>(o.null_count));
+}
+
+}  // namespace detail
+
+#define MGB_FACTORY(name, KIND)                                                   \
+  template <typename Base>                                                        \
+  std::unique_ptr<Base> name()                                                    \
+  {                                                                               \
+    return std::make_unique<detail::simple_aggregation>(aggregation::KIND);       \
+  }                                                                               \
+  template std::unique_ptr<aggregation> name<aggregation>();                      \
+  template std::unique_ptr<groupby_aggregation> name<groupby_aggregation>();      \
+  template std::unique_ptr<reduce_aggregation> name<reduce_aggregation>();        \
+  template std::unique_ptr<rolling_aggregation> name<rolling_aggregation>();
+
+MGB_FACTORY(make_sum_aggregation, SUM)
+MGB_FACTORY(make_product_aggregation, PRODUCT)
+MGB_FACTORY(make_min_aggregation, MIN)
+MGB_FACTORY(make_max_aggregation, MAX)
+MGB_FACTORY(make_sum_of_squares_aggregation, SUM_OF_SQUARES)
+MGB_FACTORY(make_mean_aggregation, MEAN)
+MGB_FACTORY(make_argmax_aggregation, ARGMAX)
+MGB_FACTORY(make_argmin_aggregation, ARGMIN)
+#undef MGB_FACTORY
+
+// COUNT(EXCLUDE) -> COUNT_VALID, COUNT(INCLUDE) -> COUNT_ALL
+// (cpp/src/aggregation/aggregation.cpp:471-476)
+template <typename Base>
+std::unique_ptr<Base> make_count_aggregation(null_policy null_handling)
+{
+  return std::make_unique<detail::simple_aggregation>(
+    null_handling == null_policy::INCLUDE ? aggregation::COUNT_ALL : aggregation::COUNT_VALID);
+}
+template std::unique_ptr<aggregation> make_count_aggregation<aggregation>(null_policy);
+template std::unique_ptr<groupby_aggregation> make_count_aggregation<groupby_aggregation>(null_policy);
+template std::unique_ptr<reduce_aggregation> make_count_aggregation<reduce_aggregation>(null_policy);
+template std::unique_ptr<rolling_aggregation> make_count_aggregation<rolling_aggregation>(null_policy);
+
+template <typename Base>
+std::unique_ptr<Base> make_variance_aggregation(size_type ddof)
+{
+  return std::make_unique<detail::simple_aggregation>(aggregation::VARIANCE, ddof);
+}
+template std::unique_ptr<aggregation> make_variance_aggregation<aggregation>(size_type);
+template std::unique_ptr<groupby_aggregation> make_variance_aggregation<groupby_aggregation>(size_type);
+
+template <typename Base>
+std::unique_ptr<Base> make_std_aggregation(size_type ddof)
+{
+  return std::make_unique<detail::simple_aggregation>(aggregation::STD, ddof);
+}
+template std::unique_ptr<aggregation> make_std_aggregation<aggregation>(size_type);
+template std::unique_ptr<groupby_aggregation> make_std_aggregation<groupby_aggregation>(size_type);
+
+template <typename Base>
+std::unique_ptr<Base> make_nunique_aggregation(null_policy)
+{
+  return std::make_unique<detail::simple_aggregation>(aggregation::NUNIQUE);
+}
+template std::unique_ptr<aggregation> make_nunique_aggregation<aggregation>(null_policy);
+template std::unique_ptr<groupby_aggregation> make_nunique_aggregation<groupby_aggregation>(null_policy);
+
+template <typename Base>
+std::unique_ptr<Base> make_nth_element_aggregation(size_type n, null_policy)
+{
+  return std::make_unique<detail::simple_aggregation>(aggregation::NTH_ELEMENT, n);
+}
+template std::unique_ptr<aggregation> make_nth_element_aggregation<aggregation>(size_type, null_policy);
+template std::unique_ptr<groupby_aggregation> make_nth_element_aggregation<groupby_aggregation>(size_type,
+                                                                                                null_policy);
+
+namespace groupby {
+
+groupby::groupby(table_view const& keys, null_policy null_handling, sorted keys_are_sorted,
+                 std::vector<order> const& column_order, std::vector<null_order> const& null_precedence)
+  : _keys{keys},
+    _include_null_keys{null_handling},
+    _keys_are_sorted{keys_are_sorted},
+    _column_order{column_order},
+    _null_precedence{null_precedence}
+{
+}
+
+groupby::~groupby() = default;
+
+std::pair<std::unique_ptr<table>, std::vector<aggregation_result>> groupby::aggregate(
+  host_span<aggregation_request const> requests, rmm::mr::device_memory_resource* mr)
+{
+  return aggregate(requests, cudf::get_default_stream(), mr);
+}
+
+std::pair<std::unique_ptr<table>, std::vector<aggregation_result>> groupby::aggregate(
+  host_span<aggregation_request const> requests, rmm::cuda_stream_view stream, rmm::mr::device_memory_resource* mr)
+{
+  // cpp/src/groupby/groupby.cu:208-214
+  for (auto const& r : requests)
+    CUDF_EXPECTS(r.values.size() == _keys.num_rows(), "Size mismatch between request values and groupby keys.");
+  for (auto const& r : requests)
+    for (auto const& a : r.aggregations)
+      CUDF_EXPECTS(mgb_target_type(static_cast<int32_t>(r.values.type().id()), a->kind) >= 0,
+                   "Invalid type/aggregation combination.");
+
+  std::vector<mgb_column> keys;
+  for (auto const& c : _keys) keys.push_back(detail::to_mgb(c));
+  std::vector<std::vector<mgb_aggregation>> aggs(requests.size());
+  std::vector<mgb_request> reqs(requests.size());
+  std::size_t n_out = 0;
+  for (std::size_t i = 0; i < requests.size(); ++i) {
+    for (auto const& a : requests[i].aggregations)
+      aggs[i].push_back(mgb_aggregation{static_cast<int32_t>(a->kind), detail::aggregation_param(*a)});
+    reqs[i].values = detail::to_mgb(requests[i].values);
+    reqs[i].aggs = aggs[i].data();
+    reqs[i].num_aggs = static_cast<int32_t>(aggs[i].size());
+    n_out += aggs[i].size();
+  }
+  std::vector<mgb_out_column> out_keys(keys.size());
+  std::vector<mgb_out_column> out_results(n_out);
+  int64_t groups = 0;
+  mgb_allocator alloc = detail::mr_allocator::make(mr);
+  mgb_status st = mgb_groupby_aggregate(keys.data(), static_cast<int32_t>(keys.size()),
+                                        _include_null_keys == null_policy::INCLUDE ? MGB_NULL_INCLUDE
+                                                                                   : MGB_NULL_EXCLUDE,
+                                        reqs.data(), static_cast<int32_t>(reqs.size()), out_keys.data(),
+                                        out_results.data(), &groups, stream.value(), &alloc);
+  if (st != MGB_SUCCESS) detail::throw_status(st);
+
+  std::vector<std::unique_ptr<column>> key_cols;
+  for (auto const& o : out_keys) key_cols.push_back(detail::adopt(o, stream, mr));
+  std::vector<aggregation_result> results(requests.size());
+  std::size_t j = 0;
+  for (std::size_t i = 0; i < requests.size(); ++i)
+    for (std::size_t a = 0; a < aggs[i].size(); ++a, ++j)
+      results[i].results.push_back(detail::adopt(out_results[j], stream, mr));
+  return {std::make_unique<table>(std::move(key_cols)), std::move(results)};
+}
+
+}  // namespace groupby
+
+namespace hashing {
+
+namespace {
+template <typename Fn>
+std::unique_ptr<column> hash_table(table_view const& input, uint32_t seed, rmm::cuda_stream_view stream,
+                                   rmm::mr::device_memory_resource* mr, type_id out_type, Fn fn)
+{
+  std::vector<mgb_column> cols;
+  for (auto const& c : input) cols.push_back(detail::to_mgb(c));
+  size_type rows = input.num_rows();
+  rmm::device_buffer data{static_cast<std::size_t>(rows) * 4, stream, mr};
+  if (rows > 0) {
+    mgb_status st = fn(cols.data(), static_cast<int32_t>(cols.size()), seed, data.data(), stream.value());
+    if (st != MGB_SUCCESS) detail::throw_status(st);
+  }
+  return std::make_unique<column>(data_type{out_type}, rows, std::move(data), rmm::device_buffer{}, 0);
+}
+}  // namespace
+
+std::unique_ptr<column> murmurhash3_x86_32(table_view const& input, uint32_t seed, rmm::cuda_stream_view stream,
+                                           rmm::mr::device_memory_resource* mr)
+{
+  return hash_table(input, seed, stream, mr, type_id::UINT32,
+                    [](mgb_column const* c, int32_t n, uint32_t s, void* out, hipStream_t st) {
+                      return mgb_murmurhash3_x86_32(c, n, s, static_cast<uint32_t*>(out), st);
+                    });
+}
+
+std::unique_ptr<column> spark_murmurhash3_x86_32(table_view const& input, uint32_t seed,
+                                                 rmm::cuda_stream_view stream, rmm::mr::device_memory_resource* mr)
+{
+  return hash_table(input, seed, stream, mr, type_id::INT32,
+                    [](mgb_column const* c, int32_t n, uint32_t s, void* out, hipStream_t st) {
+                      return mgb_spark_murmurhash3_x86_32(c, n, s, static_cast<int32_t*>(out), st);
+                    });
+}
+
+}  // namespace hashing
+}  // namespace cudf

@@ -1,0 +1,127 @@
+"""pylibcudf-shaped groupby over the C ABI (mirrors
+python/cudf/cudf/_lib/pylibcudf/groupby.pyx:80-157): ``GroupBy(keys,
+null_handling, keys_are_sorted).aggregate([GroupByRequest(values, aggs)])``
+returns ``(unique_keys: Table, [Table of results per request])``."""
+from __future__ import annotations
+
+import ctypes
+import threading
+from typing import Dict, List, Sequence
+
+import torch
+
+from . import _lib
+from .aggregation import Aggregation
+from .column import Column, Table
+from .types import NullPolicy, Sorted
+
+
+def _to_mgb(c: Column) -> _lib.mgb_column:
+    m = _lib.mgb_column()
+    m.type_id = int(c.type_id)
+    m.size = c.size
+    m.data = c.data_ptr or None
+    m.null_mask = c.mask_ptr or None
+    m.offset = c.offset
+    m.null_count = c.null_count if c.mask is not None else 0
+    return m
+
+
+class _TorchAllocator:
+    """mgb_allocator backed by torch's caching allocator: each buffer is a
+    uint8 tensor kept alive here until the result columns take it over."""
+
+    def __init__(self, device: torch.device):
+        self.device = device
+        self.live: Dict[int, torch.Tensor] = {}
+        self.lock = threading.Lock()
+        self._alloc = _lib.ALLOC_FN(self._allocate)
+        self._dealloc = _lib.DEALLOC_FN(self._deallocate)
+        self.struct = _lib.mgb_allocator(self._alloc, self._dealloc, None)
+
+    def _allocate(self, ctx, nbytes, stream):
+        try:
+            t = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+        except Exception:  # out of memory: the library reports MGB_ERROR_ALLOC
+            return None
+        with self.lock:
+            self.live[t.data_ptr()] = t
+        return t.data_ptr()
+
+    def _deallocate(self, ctx, ptr, nbytes, stream):
+        with self.lock:
+            self.live.pop(ptr, None)
+
+    def take(self, ptr: int):
+        if not ptr:
+            return None
+        with self.lock:
+            return self.live.pop(ptr)
+
+
+class GroupByRequest:
+    """Values to aggregate and the aggregations to compute on them."""
+
+    def __init__(self, values: Column, aggregations: Sequence[Aggregation]):
+        self.values = values
+        self.aggregations = list(aggregations)
+
+
+class GroupBy:
+    """Group values by keys and compute aggregations (cudf::groupby::groupby)."""
+
+    def __init__(self, keys: Table, null_handling: NullPolicy = NullPolicy.EXCLUDE,
+                 keys_are_sorted: Sorted = Sorted.NO):
+        self.keys = keys
+        self.null_handling = NullPolicy(null_handling)
+        self.keys_are_sorted = Sorted(keys_are_sorted)
+
+    def aggregate(self, requests: List[GroupByRequest], stream: torch.cuda.Stream = None):
+        lib = _lib.load()
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        dev = None
+        for c in self.keys.columns:
+            if c.data is not None:
+                dev = c.data.device
+                break
+        if dev is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        nk = self.keys.num_columns()
+        keys_arr = (_lib.mgb_column * max(nk, 1))(*[_to_mgb(c) for c in self.keys.columns])
+        agg_arrays = []
+        reqs = (_lib.mgb_request * max(len(requests), 1))()
+        n_out = 0
+        for i, r in enumerate(requests):
+            arr = (_lib.mgb_aggregation * max(len(r.aggregations), 1))(
+                *[_lib.mgb_aggregation(int(a.kind), int(a.ddof)) for a in r.aggregations])
+            agg_arrays.append(arr)
+            reqs[i].values = _to_mgb(r.values)
+            reqs[i].aggs = ctypes.cast(arr, ctypes.POINTER(_lib.mgb_aggregation))
+            reqs[i].num_aggs = len(r.aggregations)
+            n_out += len(r.aggregations)
+        out_keys = (_lib.mgb_out_column * max(nk, 1))()
+        out_res = (_lib.mgb_out_column * max(n_out, 1))()
+        groups = ctypes.c_int64(0)
+        alloc = _TorchAllocator(dev)
+        st = lib.mgb_groupby_aggregate(keys_arr, nk, int(self.null_handling), reqs, len(requests),
+                                       out_keys, out_res, ctypes.byref(groups),
+                                       ctypes.c_void_p(stream.cuda_stream), ctypes.byref(alloc.struct))
+        _lib.check(st)
+
+        def wrap(o: _lib.mgb_out_column) -> Column:
+            data = alloc.take(o.data)
+            mask_t = alloc.take(o.null_mask)
+            mask = mask_t.view(torch.int32) if mask_t is not None else None
+            return Column(o.type_id, o.size, data, mask, o.null_count)
+
+        key_table = Table([wrap(out_keys[i]) for i in range(nk)])
+        results = []
+        j = 0
+        for r in requests:
+            cols = []
+            for _ in r.aggregations:
+                cols.append(wrap(out_res[j]))
+                j += 1
+            results.append(Table(cols))
+        return key_table, results

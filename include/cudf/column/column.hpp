@@ -1,0 +1,51 @@
+// Owning device column (subset of cpp/include/cudf/column/column.hpp).
+#pragma once
+
+#include <cudf/column/column_view.hpp>
+#include <cudf/types.hpp>
+
+#include <rmm/device_buffer.hpp>
+
+#include <memory>
+
+namespace cudf {
+
+class column {
+ public:
+  column() = default;
+  ~column() = default;
+  column(column const&) = delete;
+  column& operator=(column const&) = delete;
+  column(column&&) noexcept = default;
+  column& operator=(column&&) noexcept = default;
+
+  column(data_type dtype, size_type size, rmm::device_buffer&& data, rmm::device_buffer&& null_mask,
+         size_type null_count)
+    : _type{dtype}, _size{size}, _data{std::move(data)}, _null_mask{std::move(null_mask)},
+      _null_count{_null_mask.is_empty() ? 0 : null_count}
+  {
+  }
+
+  [[nodiscard]] data_type type() const noexcept { return _type; }
+  [[nodiscard]] size_type size() const noexcept { return _size; }
+  [[nodiscard]] size_type null_count() const noexcept { return _null_count; }
+  [[nodiscard]] bool nullable() const noexcept { return !_null_mask.is_empty(); }
+  [[nodiscard]] bool has_nulls() const noexcept { return _null_count > 0; }
+
+  [[nodiscard]] column_view view() const
+  {
+    return column_view{_type, _size, _data.data(),
+                       nullable() ? static_cast<bitmask_type const*>(_null_mask.data()) : nullptr,
+                       _null_count};
+  }
+  operator column_view() const { return view(); }
+
+ private:
+  data_type _type{type_id::EMPTY};
+  size_type _size{0};
+  rmm::device_buffer _data{};
+  rmm::device_buffer _null_mask{};
+  size_type _null_count{0};
+};
+
+}  // namespace cudf

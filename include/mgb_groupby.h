@@ -1,0 +1,189 @@
+/*
+ * mgb_groupby.h -- C ABI of the MI355X hash groupby-aggregate library
+ * (libcudf_amd.so).
+ *
+ * Plain C: device pointers, sizes and status codes only. The C++ drop-in
+ * (cudf::groupby::groupby, include/cudf/groupby.hpp) and the Python binding
+ * (cudf_amd.groupby) are both thin layers over these entry points.
+ *
+ * Conventions shared by every entry point
+ *   - Columns are described by mgb_column: a type id numbered exactly like
+ *     cudf::type_id (cpp/include/cudf/types.hpp:201-233), a device data pointer
+ *     to element 0 of the parent buffer, an element offset, a row count and an
+ *     optional validity bitmask (32-bit words, LSB first, bit = 1 means valid,
+ *     bit index = offset + row), as in cudf::column_view
+ *     (cpp/include/cudf/column/column_view.hpp:53-225).
+ *   - Aggregation kinds are numbered exactly like cudf::aggregation::Kind
+ *     (cpp/include/cudf/aggregation.hpp:85-122).
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream). All
+ *     device work is ordered on it.
+ *   - Every entry point returns an mgb_status. On failure,
+ *     mgb_last_error_message() returns a description that stays valid until
+ *     the calling thread's next call into the library.
+ */
+#ifndef MGB_GROUPBY_H
+#define MGB_GROUPBY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The C++ layer maps LOGIC and UNSUPPORTED to cudf::logic_error,
+ * HIP to cudf::cuda_error and ALLOC to rmm::bad_alloc
+ * (cpp/include/cudf/utilities/error.hpp:63-126). */
+typedef enum mgb_status {
+  MGB_SUCCESS = 0,
+  MGB_ERROR_LOGIC = 1,       /* invalid argument: size mismatch, bad type/agg pair */
+  MGB_ERROR_UNSUPPORTED = 2, /* valid in cudf but outside this library (e.g. sort-only aggs) */
+  MGB_ERROR_HIP = 3,         /* a HIP runtime call failed */
+  MGB_ERROR_ALLOC = 4        /* the allocator returned NULL */
+} mgb_status;
+
+/* cudf::type_id subset handled here (same numeric values). */
+enum {
+  MGB_TYPE_EMPTY = 0,
+  MGB_TYPE_INT8 = 1, MGB_TYPE_INT16 = 2, MGB_TYPE_INT32 = 3, MGB_TYPE_INT64 = 4,
+  MGB_TYPE_UINT8 = 5, MGB_TYPE_UINT16 = 6, MGB_TYPE_UINT32 = 7, MGB_TYPE_UINT64 = 8,
+  MGB_TYPE_FLOAT32 = 9, MGB_TYPE_FLOAT64 = 10, MGB_TYPE_BOOL8 = 11,
+  MGB_TYPE_TIMESTAMP_DAYS = 12, MGB_TYPE_TIMESTAMP_SECONDS = 13, MGB_TYPE_TIMESTAMP_MILLISECONDS = 14,
+  MGB_TYPE_TIMESTAMP_MICROSECONDS = 15, MGB_TYPE_TIMESTAMP_NANOSECONDS = 16,
+  MGB_TYPE_DURATION_DAYS = 17, MGB_TYPE_DURATION_SECONDS = 18, MGB_TYPE_DURATION_MILLISECONDS = 19,
+  MGB_TYPE_DURATION_MICROSECONDS = 20, MGB_TYPE_DURATION_NANOSECONDS = 21
+};
+
+/* cudf::aggregation::Kind subset handled here (same numeric values). */
+enum {
+  MGB_AGG_SUM = 0, MGB_AGG_PRODUCT = 1, MGB_AGG_MIN = 2, MGB_AGG_MAX = 3,
+  MGB_AGG_COUNT_VALID = 4, MGB_AGG_COUNT_ALL = 5, MGB_AGG_SUM_OF_SQUARES = 8,
+  MGB_AGG_MEAN = 9, MGB_AGG_M2 = 10, MGB_AGG_VARIANCE = 11, MGB_AGG_STD = 12,
+  MGB_AGG_ARGMAX = 15, MGB_AGG_ARGMIN = 16
+};
+
+/* cudf::null_policy (cpp/include/cudf/types.hpp). */
+enum { MGB_NULL_EXCLUDE = 0, MGB_NULL_INCLUDE = 1 };
+
+/* Read-only column view (cudf::column_view for fixed-width types). */
+typedef struct mgb_column {
+  int32_t type_id;
+  int32_t reserved;
+  int64_t size;              /* rows */
+  const void* data;          /* device pointer to element 0 of the parent buffer */
+  const uint32_t* null_mask; /* device pointer or NULL (no nulls) */
+  int64_t offset;            /* first row, in elements; also the first mask bit */
+  int64_t null_count;        /* nulls in [offset, offset+size); -1 = unknown */
+} mgb_column;
+
+/* One aggregation: kind plus its parameter (ddof for VARIANCE / STD, the
+ * null_policy for COUNT is expressed by COUNT_VALID vs COUNT_ALL). */
+typedef struct mgb_aggregation {
+  int32_t kind;
+  int32_t ddof;
+} mgb_aggregation;
+
+/* cudf::groupby::aggregation_request (cpp/include/cudf/groupby.hpp:59-62). */
+typedef struct mgb_request {
+  mgb_column values;
+  const mgb_aggregation* aggs;
+  int32_t num_aggs;
+  int32_t reserved;
+} mgb_request;
+
+/* Device allocator for the returned buffers (the role of the `mr` argument of
+ * cudf::groupby::groupby::aggregate). Buffers come back owned by the caller,
+ * who releases them with `deallocate`. */
+typedef struct mgb_allocator {
+  void* (*allocate)(void* ctx, size_t bytes, void* stream);
+  void (*deallocate)(void* ctx, void* ptr, size_t bytes, void* stream);
+  void* ctx;
+} mgb_allocator;
+
+/* An owned result column written by the library. */
+typedef struct mgb_out_column {
+  int32_t type_id;
+  int32_t reserved;
+  int64_t size;
+  void* data;          /* allocated with the caller's allocator; NULL when size == 0 */
+  uint32_t* null_mask; /* allocated with the caller's allocator, or NULL */
+  int64_t null_count;
+  size_t data_bytes;   /* bytes allocated for data (for deallocate) */
+  size_t mask_bytes;   /* bytes allocated for null_mask */
+} mgb_out_column;
+
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int32_t mgb_version(void);
+
+/* Message of the calling thread's most recent failure ("" if none). */
+const char* mgb_last_error_message(void);
+
+/* Result type id of (value type, aggregation kind), or -1 when the pair is not
+ * a valid aggregation (cudf::detail::target_type / is_valid_aggregation,
+ * cpp/include/cudf/detail/aggregation/aggregation.hpp:1168-1424,1610-1623). */
+int32_t mgb_target_type(int32_t value_type_id, int32_t agg_kind);
+
+/*
+ * Hash groupby-aggregate. Replaces cudf::groupby::groupby::aggregate
+ * (cpp/include/cudf/groupby.hpp:185-197, cpp/src/groupby/groupby.cu:195-219,
+ * hash path cpp/src/groupby/hash/groupby.cu:548-665).
+ *
+ *   keys, num_keys        the key table (all columns the same size)
+ *   null_policy           MGB_NULL_EXCLUDE drops rows with any null key;
+ *                         MGB_NULL_INCLUDE groups null keys together
+ *   requests              value columns and their aggregations
+ *   out_keys              num_keys entries: the unique key rows
+ *   out_results           one entry per aggregation, requests in order and
+ *                         each request's aggregations in order
+ *   out_num_groups        number of groups (rows of every output)
+ *
+ * Output row order is arbitrary (as in cudf). Errors: MGB_ERROR_LOGIC for a
+ * size mismatch or an invalid type/aggregation pair (groupby.cu:208-212,
+ * 173-190), MGB_ERROR_UNSUPPORTED for aggregations cudf serves only from its
+ * sort-based path.
+ */
+mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_keys, int32_t null_policy,
+                                 const mgb_request* requests, int32_t num_requests,
+                                 mgb_out_column* out_keys, mgb_out_column* out_results,
+                                 int64_t* out_num_groups, void* stream, const mgb_allocator* alloc);
+
+/*
+ * Row hash of a table with cudf's MurmurHash3_x86_32 row hasher: per column
+ * murmur3 of the element (floats with -0.0 -> +0.0 and NaN -> canonical NaN,
+ * null -> 0xFFFFFFFF), folded with hash_combine from `seed`. Replaces
+ * cudf::hashing::murmurhash3_x86_32(table_view, seed) for fixed-width columns
+ * (cpp/include/cudf/hashing.hpp:90-94, cpp/src/hash/murmurhash3_x86_32.cu).
+ * `out` is a device array of num_rows uint32.
+ */
+mgb_status mgb_murmurhash3_x86_32(const mgb_column* columns, int32_t num_columns, uint32_t seed,
+                                  uint32_t* out, void* stream);
+
+/*
+ * Spark-compatible row hash: columns hashed in order with each result seeding
+ * the next, nulls leaving the running hash unchanged, small integers widened
+ * to 4 bytes. Replaces cudf::hashing::spark_murmurhash3_x86_32(table_view,
+ * seed) for fixed-width columns (cpp/include/cudf/hashing.hpp:130-134,
+ * cpp/src/hash/spark_murmurhash3_x86_32.cu). `out` is num_rows int32.
+ */
+mgb_status mgb_spark_murmurhash3_x86_32(const mgb_column* columns, int32_t num_columns, uint32_t seed,
+                                        int32_t* out, void* stream);
+
+/*
+ * Measurement hooks (no counterpart in cudf). When enabled, every launch of
+ * a named kernel ("aggregate", "fill", "finalize") is bracketed with HIP
+ * events on its stream. mgb_profiling_read synchronises those events and
+ * returns the number of launches and their summed duration since the last
+ * read, then forgets them.
+ */
+mgb_status mgb_profiling_enable(int32_t enable);
+mgb_status mgb_profiling_read(const char* kernel, int64_t* launches, double* total_ms);
+
+/* Releases the device workspace cached for `stream` (the library keeps one per
+ * stream so that repeated calls do not allocate). */
+mgb_status mgb_release_workspace(void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGB_GROUPBY_H */

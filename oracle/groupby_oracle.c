@@ -1,0 +1,506 @@
+/*
+ * groupby_oracle.c -- CPU restatement of libcudf 24.04's hash groupby-aggregate
+ * path and of the MurmurHash3_x86_32 row hashers it uses.
+ *
+ * THIS IS TEST INFRASTRUCTURE. It is the checker that the HIP product path is
+ * compared against. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product library never links or calls it.
+ *
+ * What it restates (citations relative to the cudf source tree):
+ *   - MurmurHash3_x86_32 (cudf variant):
+ *       cpp/include/cudf/hashing/detail/murmurhash3_x86_32.cuh:41-130
+ *   - Spark variant (signed tail bytes, small ints widened to 4 bytes,
+ *     seed chaining over columns, nulls keep the running hash):
+ *       cpp/src/hash/spark_murmurhash3_x86_32.cu:42-300
+ *   - hash_combine: cpp/include/cudf/hashing/detail/hashing.hpp:90-93
+ *   - NaN / -0.0 normalisation: cpp/include/cudf/hashing/detail/hash_functions.cuh:29-47
+ *   - row hash = fold of hash_combine over key columns starting at the seed,
+ *     null element -> 0xFFFFFFFF:
+ *       cpp/include/cudf/table/experimental/row_operators.cuh:1799-1883
+ *   - row equality with NaN == NaN and null == null:
+ *       cpp/include/cudf/table/experimental/row_operators.cuh:1285-1364
+ *   - the hash groupby driver: key-row validity (bitmask_and of the key
+ *     columns when nulls are excluded), insert-or-find over row indices, one
+ *     update per flattened aggregation into sparse targets indexed by the
+ *     group's representative row, compaction of the populated keys and the
+ *     MEAN = SUM / COUNT_VALID finalisation:
+ *       cpp/src/groupby/hash/groupby.cu:351-383 (flatten), 424-454 (sparse
+ *       targets), 460-504 (single pass), 510-520 (populated keys),
+ *       287-306 (MEAN), 548-620 (driver)
+ *   - element update rules and identities:
+ *       cpp/include/cudf/detail/aggregation/aggregation.cuh:131-434
+ *       cpp/include/cudf/detail/utilities/device_operators.cuh:63-200
+ *       cpp/include/cudf/fixed_point/temporary.hpp:62-72 (min / max ops)
+ *   - target types: cpp/include/cudf/detail/aggregation/aggregation.hpp:1168-1424
+ *
+ * The rows are processed in order, so float sums are the naive left-to-right
+ * sums and float MIN/MAX follow the CAS-loop operator `old < x ? old : x`
+ * applied in row order.
+ *
+ * Pinning: the murmur core is checked against SMHasher's published
+ * verification value for MurmurHash3_x86_32 (0xB0F57EE3) and against the
+ * Spark golden vectors in cpp/tests/hashing/spark_murmurhash3_x86_32_test.cpp;
+ * hash_combine against the list-column vectors of
+ * cpp/tests/hashing/murmurhash3_x86_32_test.cpp:123-160; the groupby against
+ * the literal fixtures of the cpp/tests/groupby test files, brute force and pandas.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* cudf::type_id numbering (cpp/include/cudf/types.hpp:201-233). */
+enum {
+  T_EMPTY = 0, T_INT8, T_INT16, T_INT32, T_INT64, T_UINT8, T_UINT16, T_UINT32, T_UINT64,
+  T_FLOAT32, T_FLOAT64, T_BOOL8, T_TS_D, T_TS_S, T_TS_MS, T_TS_US, T_TS_NS,
+  T_DUR_D, T_DUR_S, T_DUR_MS, T_DUR_US, T_DUR_NS
+};
+
+/* cudf::aggregation::Kind numbering (cpp/include/cudf/aggregation.hpp:85-122). */
+enum {
+  K_SUM = 0, K_PRODUCT = 1, K_MIN = 2, K_MAX = 3, K_COUNT_VALID = 4, K_COUNT_ALL = 5,
+  K_SUM_OF_SQUARES = 8, K_MEAN = 9, K_M2 = 10, K_VARIANCE = 11, K_STD = 12,
+  K_ARGMAX = 15, K_ARGMIN = 16
+};
+
+typedef struct {
+  int32_t type_id;
+  const void* data;
+  const uint32_t* mask; /* LSB-first words, bit = 1 valid; NULL = all valid */
+} mgo_col;
+
+static int type_size(int t)
+{
+  switch (t) {
+    case T_INT8: case T_UINT8: case T_BOOL8: return 1;
+    case T_INT16: case T_UINT16: return 2;
+    case T_INT32: case T_UINT32: case T_FLOAT32: case T_TS_D: case T_DUR_D: return 4;
+    case T_INT64: case T_UINT64: case T_FLOAT64: case T_TS_S: case T_TS_MS: case T_TS_US:
+    case T_TS_NS: case T_DUR_S: case T_DUR_MS: case T_DUR_US: case T_DUR_NS: return 8;
+    default: return 0;
+  }
+}
+static int is_float(int t) { return t == T_FLOAT32 || t == T_FLOAT64; }
+static int is_signed_int(int t) { return t == T_INT8 || t == T_INT16 || t == T_INT32 || t == T_INT64; }
+static int is_unsigned_int(int t) { return t == T_UINT8 || t == T_UINT16 || t == T_UINT32 || t == T_UINT64; }
+static int is_integral(int t) { return is_signed_int(t) || is_unsigned_int(t) || t == T_BOOL8; }
+static int is_timestamp(int t) { return t >= T_TS_D && t <= T_TS_NS; }
+static int is_duration(int t) { return t >= T_DUR_D && t <= T_DUR_NS; }
+
+static int bit_is_set(const uint32_t* m, int64_t i) { return m == NULL || ((m[i >> 5] >> (i & 31)) & 1u); }
+
+/* ---------------------------------------------------------------- murmur3 */
+
+static uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+static uint32_t fmix32(uint32_t h)
+{
+  h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+  return h;
+}
+static uint32_t mix_k1(uint32_t k1) { k1 *= 0xcc9e2d51u; k1 = rotl32(k1, 15); return k1 * 0x1b873593u; }
+
+/* cudf / SMHasher MurmurHash3_x86_32 over bytes (murmurhash3_x86_32.cuh:93-121). */
+uint32_t mgo_murmur3_x86_32(const void* key, int32_t len, uint32_t seed)
+{
+  const uint8_t* d = (const uint8_t*)key;
+  int32_t nblocks = len / 4;
+  uint32_t h = seed;
+  for (int32_t i = 0; i < nblocks; ++i) {
+    uint32_t k1 = (uint32_t)d[4 * i] | ((uint32_t)d[4 * i + 1] << 8) |
+                  ((uint32_t)d[4 * i + 2] << 16) | ((uint32_t)d[4 * i + 3] << 24);
+    h ^= mix_k1(k1);
+    h = rotl32(h, 13);
+    h = h * 5u + 0xe6546b64u;
+  }
+  const uint8_t* tail = d + 4 * nblocks;
+  uint32_t k1 = 0;
+  switch (len & 3) {
+    case 3: k1 ^= (uint32_t)tail[2] << 16; /* fall through */
+    case 2: k1 ^= (uint32_t)tail[1] << 8;  /* fall through */
+    case 1: k1 ^= tail[0]; h ^= mix_k1(k1);
+  }
+  h ^= (uint32_t)len;
+  return fmix32(h);
+}
+
+/* Spark variant: tail bytes are sign-extended and mixed one at a time
+ * (spark_murmurhash3_x86_32.cu:78-96). */
+uint32_t mgo_spark_murmur3_x86_32(const void* key, int32_t len, uint32_t seed)
+{
+  const uint8_t* d = (const uint8_t*)key;
+  int32_t nblocks = len / 4;
+  uint32_t h = seed;
+  for (int32_t i = 0; i < nblocks; ++i) {
+    uint32_t k1 = (uint32_t)d[4 * i] | ((uint32_t)d[4 * i + 1] << 8) |
+                  ((uint32_t)d[4 * i + 2] << 16) | ((uint32_t)d[4 * i + 3] << 24);
+    h ^= mix_k1(k1);
+    h = rotl32(h, 13);
+    h = h * 5u + 0xe6546b64u;
+  }
+  for (int32_t i = 4 * nblocks; i < len; ++i) {
+    uint32_t k1 = (uint32_t)(int32_t)(int8_t)d[i];
+    h ^= mix_k1(k1);
+    h = rotl32(h, 13);
+    h = h * 5u + 0xe6546b64u;
+  }
+  h ^= (uint32_t)len;
+  return fmix32(h);
+}
+
+/* Boost-style combine (hashing.hpp:90-93). */
+uint32_t mgo_hash_combine(uint32_t lhs, uint32_t rhs)
+{
+  return lhs ^ (rhs + 0x9e3779b9u + (lhs << 6) + (lhs >> 2));
+}
+
+/* Element hash of the cudf variant: floats normalised (-0 -> +0, NaN -> qNaN),
+ * bool hashed as one byte, every other type over its storage bytes. */
+static uint32_t element_hash(int t, const void* base, int64_t i, uint32_t seed)
+{
+  int sz = type_size(t);
+  const uint8_t* p = (const uint8_t*)base + (size_t)i * sz;
+  if (t == T_FLOAT32) {
+    float f; memcpy(&f, p, 4);
+    if (f == 0.0f) f = 0.0f;
+    if (isnan(f)) f = NAN;
+    return mgo_murmur3_x86_32(&f, 4, seed);
+  }
+  if (t == T_FLOAT64) {
+    double f; memcpy(&f, p, 8);
+    if (f == 0.0) f = 0.0;
+    if (isnan(f)) f = (double)NAN;
+    return mgo_murmur3_x86_32(&f, 8, seed);
+  }
+  if (t == T_BOOL8) { uint8_t b = p[0] != 0; return mgo_murmur3_x86_32(&b, 1, seed); }
+  return mgo_murmur3_x86_32(p, sz, seed);
+}
+
+static uint32_t row_hash(const mgo_col* cols, int32_t ncols, int64_t i, uint32_t seed, int check_nulls)
+{
+  uint32_t h = seed;
+  for (int32_t c = 0; c < ncols; ++c) {
+    uint32_t e = (check_nulls && !bit_is_set(cols[c].mask, i)) ? 0xFFFFFFFFu
+                                                                : element_hash(cols[c].type_id, cols[c].data, i, seed);
+    h = mgo_hash_combine(h, e);
+  }
+  return h;
+}
+
+/* cudf::hashing::murmurhash3_x86_32(table, seed) for flat columns. */
+int mgo_murmurhash3_x86_32_table(const mgo_col* cols, int32_t ncols, int64_t nrows, uint32_t seed, uint32_t* out)
+{
+  int check_nulls = 0;
+  for (int32_t c = 0; c < ncols; ++c) {
+    if (type_size(cols[c].type_id) == 0) return 1;
+    if (cols[c].mask) check_nulls = 1;
+  }
+  for (int64_t i = 0; i < nrows; ++i) out[i] = row_hash(cols, ncols, i, seed, check_nulls);
+  return 0;
+}
+
+/* cudf::hashing::spark_murmurhash3_x86_32(table, seed) for flat columns. */
+int mgo_spark_murmurhash3_x86_32_table(const mgo_col* cols, int32_t ncols, int64_t nrows, uint32_t seed, int32_t* out)
+{
+  for (int32_t c = 0; c < ncols; ++c)
+    if (type_size(cols[c].type_id) == 0) return 1;
+  for (int64_t i = 0; i < nrows; ++i) {
+    uint32_t h = seed;
+    for (int32_t c = 0; c < ncols; ++c) {
+      if (!bit_is_set(cols[c].mask, i)) continue; /* null keeps the running hash */
+      int t = cols[c].type_id, sz = type_size(t);
+      const uint8_t* p = (const uint8_t*)cols[c].data + (size_t)i * sz;
+      uint32_t w;
+      switch (t) {
+        case T_BOOL8: case T_UINT8: w = p[0]; if (t == T_BOOL8) w = p[0]; h = mgo_spark_murmur3_x86_32(&w, 4, h); break;
+        case T_INT8: w = (uint32_t)(int32_t)(int8_t)p[0]; h = mgo_spark_murmur3_x86_32(&w, 4, h); break;
+        case T_INT16: { int16_t s; memcpy(&s, p, 2); w = (uint32_t)(int32_t)s; h = mgo_spark_murmur3_x86_32(&w, 4, h); break; }
+        case T_UINT16: { uint16_t s; memcpy(&s, p, 2); w = s; h = mgo_spark_murmur3_x86_32(&w, 4, h); break; }
+        case T_FLOAT32: { float f; memcpy(&f, p, 4); if (isnan(f)) f = NAN; h = mgo_spark_murmur3_x86_32(&f, 4, h); break; }
+        case T_FLOAT64: { double f; memcpy(&f, p, 8); if (isnan(f)) f = (double)NAN; h = mgo_spark_murmur3_x86_32(&f, 8, h); break; }
+        default: h = mgo_spark_murmur3_x86_32(p, sz, h);
+      }
+    }
+    out[i] = (int32_t)h;
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------- groupby */
+
+static double load_f(int t, const void* base, int64_t i)
+{
+  if (t == T_FLOAT32) return (double)((const float*)base)[i];
+  return ((const double*)base)[i];
+}
+/* Integer load widened to int64 (uint64 values are carried in the same bits). */
+static int64_t load_i(int t, const void* base, int64_t i)
+{
+  switch (type_size(t)) {
+    case 1: return is_signed_int(t) ? (int64_t)((const int8_t*)base)[i] : (int64_t)((const uint8_t*)base)[i];
+    case 2: return is_signed_int(t) ? (int64_t)((const int16_t*)base)[i] : (int64_t)((const uint16_t*)base)[i];
+    case 4: return (is_signed_int(t) || is_timestamp(t) || is_duration(t)) ? (int64_t)((const int32_t*)base)[i]
+                                                                          : (int64_t)((const uint32_t*)base)[i];
+    default: return ((const int64_t*)base)[i];
+  }
+}
+
+static int elements_equal(int t, const void* base, int64_t a, int64_t b)
+{
+  if (is_float(t)) {
+    double x = load_f(t, base, a), y = load_f(t, base, b);
+    return (isnan(x) && isnan(y)) ? 1 : x == y;
+  }
+  int sz = type_size(t);
+  if (t == T_BOOL8) return (((const uint8_t*)base)[a] != 0) == (((const uint8_t*)base)[b] != 0);
+  return memcmp((const uint8_t*)base + (size_t)a * sz, (const uint8_t*)base + (size_t)b * sz, sz) == 0;
+}
+
+static int rows_equal(const mgo_col* k, int32_t nk, int64_t a, int64_t b)
+{
+  for (int32_t c = 0; c < nk; ++c) {
+    int va = bit_is_set(k[c].mask, a), vb = bit_is_set(k[c].mask, b);
+    if (va != vb) return 0;
+    if (!va) continue; /* null == null (null_equality::EQUAL) */
+    if (!elements_equal(k[c].type_id, k[c].data, a, b)) return 0;
+  }
+  return 1;
+}
+
+/* Target type of (source type, kind); -1 when the pair is invalid
+ * (aggregation.hpp:1168-1424). */
+int mgo_target_type(int t, int kind)
+{
+  if (type_size(t) == 0) return -1;
+  switch (kind) {
+    case K_MIN: case K_MAX: return t;
+    case K_COUNT_VALID: case K_COUNT_ALL: return T_INT32;
+    case K_ARGMIN: case K_ARGMAX: return T_INT32;
+    case K_SUM:
+      if (is_integral(t)) return T_INT64;
+      if (is_float(t) || is_duration(t)) return t;
+      return -1;
+    case K_PRODUCT: case K_SUM_OF_SQUARES:
+      if (is_integral(t)) return T_INT64;
+      if (is_float(t)) return t;
+      return -1;
+    case K_MEAN:
+      if (is_duration(t)) return t;
+      if (is_timestamp(t)) return -1;
+      return T_FLOAT64;
+    case K_M2: case K_VARIANCE: case K_STD:
+      if (is_timestamp(t)) return -1;
+      return T_FLOAT64;
+    default: return -1;
+  }
+}
+
+/* Per-group scratch for one request. Integers accumulate in int64 (uint64 in
+ * the same bits, wrapping like the device's two's complement atomics), floats
+ * in the source type for SUM/PRODUCT and MIN/MAX, MEAN's sum in its SUM target. */
+typedef struct {
+  int64_t* i;
+  double* f;
+  float* f32;
+  int64_t* cnt;
+  double* m2;
+  uint8_t* valid;
+} acc_t;
+
+/*
+ * keys/values: host columns; requests: (value column index, kind, ddof).
+ * Outputs: out_rep_rows[g] = representative (first) row of group g, in order of
+ * first appearance; out_data[j] / out_valid[j] = request j's result per group
+ * (capacity num_rows each). Returns 0, or 1 for an invalid type/aggregation pair.
+ */
+int mgo_groupby_aggregate(const mgo_col* keys, int32_t n_keys, int64_t num_rows, int32_t include_null_keys,
+                          const mgo_col* values, int32_t n_values, const int32_t* req_col,
+                          const int32_t* req_kind, const int32_t* req_ddof, int32_t n_req,
+                          int64_t* out_rep_rows, int64_t* out_num_groups, void* const* out_data,
+                          uint8_t* const* out_valid)
+{
+  (void)n_values;
+  for (int32_t j = 0; j < n_req; ++j)
+    if (mgo_target_type(values[req_col[j]].type_id, req_kind[j]) < 0) return 1;
+
+  int keys_have_nulls = 0;
+  for (int32_t c = 0; c < n_keys; ++c) keys_have_nulls |= keys[c].mask != NULL;
+  int skip_null_rows = keys_have_nulls && !include_null_keys;
+
+  /* open addressing over row indices, linear probing, empty = -1 */
+  int64_t cap = 16;
+  while (cap < 2 * num_rows) cap <<= 1;
+  int64_t* slots = (int64_t*)malloc(sizeof(int64_t) * cap);
+  int64_t* group_of_row = (int64_t*)malloc(sizeof(int64_t) * (num_rows ? num_rows : 1));
+  for (int64_t s = 0; s < cap; ++s) slots[s] = -1;
+
+  int64_t ng = 0;
+  for (int64_t i = 0; i < num_rows; ++i) {
+    group_of_row[i] = -1;
+    int valid = 1;
+    if (skip_null_rows)
+      for (int32_t c = 0; c < n_keys; ++c) valid &= bit_is_set(keys[c].mask, i);
+    if (!valid) continue;
+    uint32_t h = row_hash(keys, n_keys, i, 0u, keys_have_nulls);
+    int64_t s = h & (cap - 1);
+    for (;;) {
+      if (slots[s] < 0) { slots[s] = i; out_rep_rows[ng] = i; group_of_row[i] = ng++; break; }
+      if (rows_equal(keys, n_keys, slots[s], i)) { group_of_row[i] = group_of_row[slots[s]]; break; }
+      s = (s + 1) & (cap - 1);
+    }
+  }
+  free(slots);
+  *out_num_groups = ng;
+
+  size_t gsz = (size_t)(ng ? ng : 1);
+  acc_t a;
+  a.i = (int64_t*)malloc(sizeof(int64_t) * gsz);
+  a.f = (double*)malloc(sizeof(double) * gsz);
+  a.f32 = (float*)malloc(sizeof(float) * gsz);
+  a.cnt = (int64_t*)malloc(sizeof(int64_t) * gsz);
+  a.m2 = (double*)malloc(sizeof(double) * gsz);
+  a.valid = (uint8_t*)malloc(gsz);
+
+  for (int32_t j = 0; j < n_req; ++j) {
+    const mgo_col* v = &values[req_col[j]];
+    int t = v->type_id, kind = req_kind[j];
+    int tt = mgo_target_type(t, kind);
+    int fl = is_float(t);
+    for (int64_t g = 0; g < ng; ++g) {
+      a.cnt[g] = 0; a.valid[g] = 0; a.m2[g] = 0.0;
+      /* identities (device_operators.cuh:63-200) */
+      if (kind == K_PRODUCT) { a.i[g] = 1; a.f[g] = 1.0; a.f32[g] = 1.0f; }
+      else if (kind == K_MIN) { a.i[g] = 0; a.f[g] = INFINITY; a.f32[g] = INFINITY; }
+      else if (kind == K_MAX) { a.i[g] = 0; a.f[g] = -INFINITY; a.f32[g] = -INFINITY; }
+      else if (kind == K_ARGMIN || kind == K_ARGMAX) { a.i[g] = -1; }
+      else { a.i[g] = 0; a.f[g] = 0.0; a.f32[g] = 0.0f; }
+    }
+    for (int64_t r = 0; r < num_rows; ++r) {
+      int64_t g = group_of_row[r];
+      if (g < 0) continue;
+      int vv = bit_is_set(v->mask, r);
+      if (kind == K_COUNT_ALL) { a.cnt[g]++; continue; }
+      if (!vv) continue; /* null source rows are skipped (aggregation.cuh:144,193,242,408) */
+      a.cnt[g]++;
+      switch (kind) {
+        case K_COUNT_VALID: break;
+        case K_SUM: case K_MEAN: case K_VARIANCE: case K_STD: case K_M2:
+          if (t == T_FLOAT32) a.f32[g] += (float)load_f(t, v->data, r);
+          else if (fl) a.f[g] += load_f(t, v->data, r);
+          else a.i[g] = (int64_t)((uint64_t)a.i[g] + (uint64_t)load_i(t, v->data, r));
+          break;
+        case K_SUM_OF_SQUARES:
+          if (t == T_FLOAT32) { float x = (float)load_f(t, v->data, r); a.f32[g] += x * x; }
+          else if (fl) { double x = load_f(t, v->data, r); a.f[g] += x * x; }
+          else { uint64_t x = (uint64_t)load_i(t, v->data, r); a.i[g] = (int64_t)((uint64_t)a.i[g] + x * x); }
+          break;
+        case K_PRODUCT:
+          if (t == T_FLOAT32) a.f32[g] *= (float)load_f(t, v->data, r);
+          else if (fl) a.f[g] *= load_f(t, v->data, r);
+          else a.i[g] = (int64_t)((uint64_t)a.i[g] * (uint64_t)load_i(t, v->data, r));
+          break;
+        case K_MIN: case K_MAX: {
+          int is_min = kind == K_MIN;
+          if (t == T_FLOAT32) {
+            float x = (float)load_f(t, v->data, r), o = a.f32[g];
+            a.f32[g] = is_min ? (o < x ? o : x) : (o > x ? o : x);
+          } else if (fl) {
+            double x = load_f(t, v->data, r), o = a.f[g];
+            a.f[g] = is_min ? (o < x ? o : x) : (o > x ? o : x);
+          } else {
+            int64_t x = load_i(t, v->data, r);
+            if (a.cnt[g] == 1) a.i[g] = x;
+            else if (is_unsigned_int(t)) {
+              uint64_t o = (uint64_t)a.i[g], ux = (uint64_t)x;
+              a.i[g] = (int64_t)(is_min ? (o < ux ? o : ux) : (o > ux ? o : ux));
+            } else {
+              int64_t o = a.i[g];
+              a.i[g] = is_min ? (o < x ? o : x) : (o > x ? o : x);
+            }
+          }
+          break;
+        }
+        case K_ARGMIN: case K_ARGMAX: {
+          /* row index of the min / max valid element; ties keep the smaller index
+           * (aggregation.cuh:356-394 compares values then row indices) */
+          int64_t cur = a.i[g];
+          if (cur < 0) { a.i[g] = r; break; }
+          int better;
+          if (fl) {
+            double x = load_f(t, v->data, r), o = load_f(t, v->data, cur);
+            better = kind == K_ARGMIN ? (x < o) : (x > o);
+          } else if (is_unsigned_int(t)) {
+            uint64_t x = (uint64_t)load_i(t, v->data, r), o = (uint64_t)load_i(t, v->data, cur);
+            better = kind == K_ARGMIN ? (x < o) : (x > o);
+          } else {
+            int64_t x = load_i(t, v->data, r), o = load_i(t, v->data, cur);
+            better = kind == K_ARGMIN ? (x < o) : (x > o);
+          }
+          if (better) a.i[g] = r;
+          break;
+        }
+      }
+    }
+    /* second pass for M2 / VARIANCE / STD (multi_pass_kernels.cuh:35-112) */
+    if (kind == K_M2 || kind == K_VARIANCE || kind == K_STD) {
+      for (int64_t r = 0; r < num_rows; ++r) {
+        int64_t g = group_of_row[r];
+        if (g < 0 || !bit_is_set(v->mask, r)) continue;
+        double sum = (t == T_FLOAT32) ? (double)a.f32[g] : fl ? a.f[g] : (double)a.i[g];
+        double mean = sum / (double)a.cnt[g];
+        double x = fl ? load_f(t, v->data, r) : (double)load_i(t, v->data, r);
+        a.m2[g] += (x - mean) * (x - mean);
+      }
+    }
+    /* finalise into the dense result */
+    for (int64_t g = 0; g < ng; ++g) {
+      int has = a.cnt[g] > 0;
+      uint8_t ok = 1;
+      switch (kind) {
+        case K_COUNT_VALID: case K_COUNT_ALL: ((int32_t*)out_data[j])[g] = (int32_t)a.cnt[g]; break;
+        case K_MEAN:
+          ok = has;
+          if (is_duration(t)) ((int64_t*)out_data[j])[g] = has ? a.i[g] / a.cnt[g] : 0;
+          else {
+            double sum = (t == T_FLOAT32) ? (double)a.f32[g] : fl ? a.f[g] : (double)a.i[g];
+            ((double*)out_data[j])[g] = has ? sum / (double)a.cnt[g] : 0.0;
+          }
+          if (is_duration(t) && type_size(t) == 4) ((int32_t*)out_data[j])[g] = (int32_t)(has ? a.i[g] / a.cnt[g] : 0);
+          break;
+        case K_M2: case K_VARIANCE: case K_STD: {
+          int ddof = req_ddof ? req_ddof[j] : 1;
+          double m2 = a.m2[g], n = (double)a.cnt[g];
+          double res = kind == K_M2 ? m2 : m2 / (n - ddof);
+          if (kind == K_STD) res = sqrt(res);
+          ok = kind == K_M2 ? has : (a.cnt[g] > ddof);
+          ((double*)out_data[j])[g] = ok ? res : 0.0;
+          break;
+        }
+        case K_ARGMIN: case K_ARGMAX:
+          ok = a.i[g] >= 0;
+          ((int32_t*)out_data[j])[g] = (int32_t)a.i[g];
+          break;
+        default: {
+          ok = has;
+          if (tt == T_FLOAT32) ((float*)out_data[j])[g] = (kind == K_MIN || kind == K_MAX || kind == K_SUM ||
+                                                          kind == K_PRODUCT || kind == K_SUM_OF_SQUARES)
+                                                             ? a.f32[g] : 0.0f;
+          else if (tt == T_FLOAT64) ((double*)out_data[j])[g] = a.f[g];
+          else {
+            int sz = type_size(tt);
+            int64_t x = a.i[g];
+            void* o = out_data[j];
+            if (sz == 1) ((int8_t*)o)[g] = (int8_t)x;
+            else if (sz == 2) ((int16_t*)o)[g] = (int16_t)x;
+            else if (sz == 4) ((int32_t*)o)[g] = (int32_t)x;
+            else ((int64_t*)o)[g] = x;
+          }
+        }
+      }
+      if (kind == K_COUNT_VALID || kind == K_COUNT_ALL) ok = 1;
+      out_valid[j][g] = ok;
+    }
+  }
+  free(a.i); free(a.f); free(a.f32); free(a.cnt); free(a.m2); free(a.valid);
+  free(group_of_row);
+  return 0;
+}

@@ -1,0 +1,183 @@
+"""GPU parity: the HIP path through the C ABI against the reference's literal
+fixtures and against the oracle on seeded random inputs."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import golden_cases as G
+from tests.helpers import abs_sums, assert_same, gpu_groupby, oracle_groupby, sorted_result
+
+pytestmark = pytest.mark.gpu
+
+CASES = G.load_cases()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
+def test_reference_fixture(gpu, case):
+    kt, kv, kvalid = case["keys"]
+    keys, res = gpu_groupby([(kt, kv, kvalid)], [(case["vals"], [case["kind"]])],
+                            include_null_keys=case["include_null_keys"])
+    G.check_case(case, keys, res)
+
+
+def _random_case(rng, n, key_type, card, val_types, null_frac, kinds, key_null_frac=0.0):
+    if key_type in (O.FLOAT32, O.FLOAT64):
+        pool = rng.standard_normal(card).astype(O.NP[key_type])
+    else:
+        info = np.iinfo(O.NP[key_type])
+        pool = rng.integers(max(info.min, -2 ** 62), min(info.max, 2 ** 62), card, dtype=np.int64,
+                            endpoint=True).astype(O.NP[key_type])
+    keys = pool[rng.integers(0, card, n)]
+    kvalid = (rng.random(n) >= key_null_frac) if key_null_frac > 0 else None
+    reqs = []
+    for vt in val_types:
+        if vt in (O.FLOAT32, O.FLOAT64):
+            v = rng.standard_normal(n).astype(O.NP[vt])
+        elif vt == O.BOOL8:
+            v = rng.integers(0, 2, n).astype(np.uint8)
+        else:
+            info = np.iinfo(O.NP[vt])
+            lo, hi = max(info.min, -1_000_000), min(info.max, 1_000_000)
+            v = rng.integers(lo, hi, n, endpoint=True).astype(O.NP[vt])
+        valid = (rng.random(n) >= null_frac) if null_frac > 0 else None
+        reqs.append(((vt, v, valid), kinds))
+    return [(key_type, keys, kvalid)], reqs
+
+
+def _compare(keys, reqs, include_null_keys=False):
+    gk, gr = gpu_groupby(keys, reqs, include_null_keys)
+    ok, orr = oracle_groupby(keys, reqs, include_null_keys)
+    gk, gr = sorted_result(gk, gr)
+    ok, orr = sorted_result(ok, orr)
+    sums = abs_sums(keys, reqs, include_null_keys)
+    assert len(gk[0][0]) == len(ok[0][0]), "group count differs"
+    for c, ((gkv, gkvalid), (okv, okvalid)) in enumerate(zip(gk, ok)):
+        assert_same((gkv, gkvalid), (okv, okvalid), None, what=f"key column {c}")
+    for r, (gblk, oblk) in enumerate(zip(gr, orr)):
+        sk, s = sums[r]
+        s_sorted = s[O.sort_order(sk)]
+        # float32 sources (their SUM and the SUM inside MEAN are float32 in cudf)
+        # are held to the float32 bar; everything else to its result type's bar
+        rtol = 1e-5 if reqs[r][0][0] == O.FLOAT32 else None
+        for a, (g, o) in enumerate(zip(gblk, oblk)):
+            assert_same(g, o, None, abs_sum=s_sorted, rtol=rtol, what=f"request {r} agg {reqs[r][1][a]}")
+
+
+ALL_KINDS = [O.SUM, O.MIN, O.MAX, O.COUNT_VALID, O.COUNT_ALL, O.MEAN, O.SUM_OF_SQUARES]
+
+
+@pytest.mark.parametrize("key_type", [O.INT8, O.INT16, O.INT32, O.INT64, O.UINT32, O.UINT64, O.FLOAT32, O.FLOAT64])
+def test_key_types(gpu, key_type):
+    rng = np.random.default_rng(100 + key_type)
+    card = 100 if key_type in (O.INT8,) else 5000
+    keys, reqs = _random_case(rng, 50_000, key_type, card, [O.INT32, O.FLOAT64], 0.1, ALL_KINDS)
+    _compare(keys, reqs)
+
+
+@pytest.mark.parametrize("val_type", [O.INT8, O.INT16, O.INT32, O.INT64, O.UINT8, O.UINT16, O.UINT32, O.UINT64,
+                                      O.FLOAT32, O.FLOAT64, O.BOOL8])
+def test_value_types(gpu, val_type):
+    rng = np.random.default_rng(200 + val_type)
+    kinds = [O.SUM, O.MIN, O.MAX, O.COUNT_VALID, O.COUNT_ALL, O.MEAN]
+    keys, reqs = _random_case(rng, 40_000, O.INT32, 3000, [val_type], 0.2, kinds)
+    _compare(keys, reqs)
+
+
+@pytest.mark.parametrize("card", [1, 16, 1000, 100_000])
+def test_cardinality(gpu, card):
+    rng = np.random.default_rng(300 + card)
+    keys, reqs = _random_case(rng, 200_000, O.INT64, card, [O.INT32, O.FLOAT64, O.FLOAT32], 0.05,
+                              [O.SUM, O.COUNT_VALID, O.MIN, O.MAX, O.MEAN])
+    _compare(keys, reqs)
+
+
+def test_null_keys_exclude_and_include(gpu):
+    rng = np.random.default_rng(11)
+    keys, reqs = _random_case(rng, 30_000, O.INT32, 500, [O.INT64], 0.1, ALL_KINDS, key_null_frac=0.1)
+    _compare(keys, reqs, include_null_keys=False)
+    _compare(keys, reqs, include_null_keys=True)
+
+
+def test_composite_keys(gpu):
+    rng = np.random.default_rng(12)
+    n = 100_000
+    a = rng.integers(0, 300, n).astype(np.int32)
+    b = rng.integers(0, 50, n).astype(np.int64) * 1_000_000_007
+    c = rng.integers(0, 3, n).astype(np.float64)
+    v = rng.standard_normal(n)
+    keys = [(O.INT32, a, None), (O.INT64, b, None), (O.FLOAT64, c, rng.random(n) > 0.05)]
+    reqs = [((O.FLOAT64, v, rng.random(n) > 0.1), [O.SUM, O.COUNT_VALID, O.MAX, O.MEAN])]
+    _compare(keys, reqs)
+    _compare(keys, reqs, include_null_keys=True)
+
+
+def test_sentinel_key_values(gpu):
+    # INT64_MIN is the empty-slot header of single integer keys; it must still group.
+    n = 10_000
+    rng = np.random.default_rng(13)
+    keys = rng.choice(np.array([np.iinfo(np.int64).min, -1, 0, 1, np.iinfo(np.int64).max], dtype=np.int64), n)
+    v = rng.integers(-5, 5, n).astype(np.int32)
+    _compare([(O.INT64, keys, None)], [((O.INT32, v, None), [O.SUM, O.COUNT_VALID, O.MIN, O.MAX])])
+    ukeys = keys.view(np.uint64)
+    _compare([(O.UINT64, ukeys, None)], [((O.INT32, v, None), [O.SUM, O.COUNT_VALID])])
+
+
+def test_float_key_normalisation(gpu):
+    # -0.0 == +0.0 and NaN == NaN as keys (row_operators.cuh:1285-1315)
+    keys = np.array([0.0, -0.0, np.nan, -np.nan, 1.5, 1.5, np.nan], dtype=np.float64)
+    v = np.arange(7, dtype=np.int64)
+    gk, gr = gpu_groupby([(O.FLOAT64, keys, None)], [((O.INT64, v, None), [O.SUM, O.COUNT_VALID])])
+    assert len(gk[0][0]) == 3
+    got = {("nan" if np.isnan(k) else float(k)): (int(s), int(c))
+           for k, s, c in zip(gk[0][0], gr[0][0][0], gr[0][1][0])}
+    assert got == {0.0: (1, 2), "nan": (2 + 3 + 6, 3), 1.5: (9, 2)}
+
+
+def test_duplicate_requests(gpu):
+    # the same column in two requests and the same agg twice (keys_tests.cpp:367-421)
+    keys = np.array([1, 2, 3, 1, 2, 2, 1, 3, 3, 2], dtype=np.int32)
+    vals = np.arange(10, dtype=np.int32)
+    gk, gr = gpu_groupby([(O.INT32, keys, None)],
+                         [((O.INT32, vals, None), [O.SUM, O.SUM]), ((O.INT32, vals, None), [O.SUM, O.MEAN])])
+    order = np.argsort(gk[0][0])
+    for blk in gr:
+        assert list(blk[0][0][order]) == [9, 19, 17]
+    assert list(gr[0][1][0][order]) == [9, 19, 17]
+    assert np.allclose(gr[1][1][0][order], [3.0, 4.75, 17 / 3])
+
+
+def test_nan_values_min_max_terminate(gpu):
+    # min_tests.cpp:524-547 / max_tests.cpp:527-551: NaN values must not hang
+    for t in (O.FLOAT32, O.FLOAT64):
+        keys = np.array([1, 1], dtype=np.int32)
+        v = np.array([np.nan, np.nan], dtype=O.NP[t])
+        gk, gr = gpu_groupby([(O.INT32, keys, None)], [((t, v, None), [O.MIN, O.MAX])])
+        assert len(gk[0][0]) == 1
+        assert np.isnan(gr[0][0][0][0]) and np.isnan(gr[0][1][0][0])
+
+
+def test_nan_skipped_when_other_values(gpu):
+    # float MIN/MAX ignore NaN unless every valid value of the group is NaN (pandas skipna)
+    keys = np.array([1, 1, 1, 2, 2], dtype=np.int32)
+    v = np.array([np.nan, 3.0, -1.0, np.nan, np.nan], dtype=np.float64)
+    gk, gr = gpu_groupby([(O.INT32, keys, None)], [((O.FLOAT64, v, None), [O.MIN, O.MAX])])
+    order = np.argsort(gk[0][0])
+    mn, mx = gr[0][0][0][order], gr[0][1][0][order]
+    assert mn[0] == -1.0 and mx[0] == 3.0
+    assert np.isnan(mn[1]) and np.isnan(mx[1])
+
+
+def test_c2_shape_slice(gpu):
+    """2M rows of the 100M-row benchmark's distribution (int64 keys over 200K
+    distinct values, int32/f64/f32 values 5% null, 5 aggs each)."""
+    rng = np.random.default_rng(42)
+    n = 2_000_000
+    uniq = np.unique(rng.integers(-2 ** 63, 2 ** 63 - 1, 200_000, dtype=np.int64))
+    keys = uniq[rng.integers(0, uniq.size, n)]
+    kinds = [O.SUM, O.COUNT_VALID, O.MIN, O.MAX, O.MEAN]
+    reqs = [((O.INT32, rng.integers(-1_000_000, 1_000_001, n).astype(np.int32), rng.random(n) >= 0.05), kinds),
+            ((O.FLOAT64, rng.standard_normal(n), rng.random(n) >= 0.05), kinds),
+            ((O.FLOAT32, rng.standard_normal(n).astype(np.float32), rng.random(n) >= 0.05), kinds)]
+    _compare([(O.INT64, keys, None)], reqs)
