@@ -149,7 +149,7 @@ def main():
     torch.cuda.synchronize()
 
     lib.mgb_profiling_enable(1)
-    for name in (b"aggregate", b"fill", b"finalize"):
+    for name in (b"aggregate", b"fill", b"finalize", b"partition_hist", b"partition_scatter"):
         n, ms = ctypes.c_int64(), ctypes.c_double()
         lib.mgb_profiling_read(name, ctypes.byref(n), ctypes.byref(ms))  # drop stale records
     stream = torch.cuda.current_stream()
@@ -164,7 +164,7 @@ def main():
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
     kern = {}
-    for name in ("aggregate", "fill", "finalize"):
+    for name in ("aggregate", "fill", "finalize", "partition_hist", "partition_scatter"):
         n, ms = ctypes.c_int64(), ctypes.c_double()
         _lib.check(lib.mgb_profiling_read(name.encode(), ctypes.byref(n), ctypes.byref(ms)))
         kern[name] = (n.value, ms.value)

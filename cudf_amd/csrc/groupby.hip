@@ -1,27 +1,29 @@
 // MI355X hash groupby-aggregate: kernels and the C ABI driver.
 //
-// Design (replacing cpp/src/groupby/hash/groupby.cu:424-620 of libcudf):
-//   * One open-addressing table of fixed-stride records in HBM. A record is an
-//     8-byte header followed by the group's accumulators (array of structs),
-//     so a row touches one record instead of one slot per flattened
-//     aggregation column.
-//   * The header is claimed with a single 64-bit CAS and is never rewritten.
-//     It holds the key itself for a single integer-like key column, or
-//     (row hash << 32 | representative row) otherwise; equality then gathers
-//     the representative row's key elements from the (read-only) input. No
-//     other field has to be published before a second row may use the group,
-//     so there is no spin on a half-built slot and no agent-scope
-//     release/acquire (which on gfx950 means an L2 writeback / invalidate).
-//   * Accumulators start at their identities (one fill of the record array)
-//     and are updated with device atomics; float MIN/MAX use order-preserving
-//     unsigned codes so they are exact and order independent.
-//   * A finalize pass compacts occupied records in index order (block counts,
-//     a one-workgroup scan, then a ballot-based intra-block prefix) and writes
-//     SoA result columns directly, including MEAN and the validity bytes that
-//     a pack kernel turns into cudf bitmasks.
-//   * Table capacity comes from a sampled distinct-count estimate (Chao1 over
-//     a stratified sample); an overflow flag makes the driver rebuild with a
-//     4x larger table.
+// The path replaces cpp/src/groupby/hash/groupby.cu:424-620 of libcudf. It
+// has three executions, chosen per call from a sampled distinct-count
+// estimate (G) and the size of one group's accumulator record:
+//
+//   * partitioned (G above what one workgroup's LDS holds): the rows are hash
+//     partitioned (histogram, scan, scatter) into P = 2^k partitions whose
+//     groups fit one workgroup's LDS; each partition is then aggregated by one
+//     workgroup in an LDS hash table with LDS atomics and written out as
+//     final results. No global atomics touch the per-row path.
+//   * direct (small G): each workgroup aggregates a contiguous slice of the
+//     input into its own LDS table, then folds its groups into a small global
+//     table with device atomics (a handful per group and workgroup).
+//   * global (composite keys compared with nulls, or too many aggregations
+//     for LDS): one global open-addressing table of records updated with
+//     device atomics per row.
+//
+// Records (global or LDS) are an 8-byte header followed by the group's
+// accumulators. The header is claimed with one 64-bit CAS and never
+// rewritten: it is the key itself for a single integer-like key, otherwise
+// (row hash << 32 | representative row). Nothing else has to be published
+// before another row may use the group, so there is no spin on half-built
+// slots and no agent-scope release/acquire (an L2 writeback / invalidate on
+// gfx950). Float MIN/MAX use order-preserving unsigned codes so they are
+// exact and independent of update order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,146 +34,23 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
-#include "groupby_kernels.hpp"
+#include "groupby_device.hpp"
 #include "mgb_groupby.h"
 
 namespace mgb {
 
+constexpr int kLdsBlock = 1024;     // threads of an LDS aggregation workgroup
+constexpr int kScanItems = 16;      // u32 scan: items per thread (x256 threads)
+constexpr int kScanTile = 256 * kScanItems;
+constexpr int kMaxRecordWords = 64;
+
 // =================================================================== kernels
-
-__device__ __forceinline__ bool rows_equal(const AggParams& p, int64_t a, int64_t b)
-{
-  for (int c = 0; c < p.num_keys; ++c) {
-    const KeyCol& k = p.keys[c];
-    if (p.compare_key_nulls) {
-      bool va = bit_valid(k.mask, k.bit_off + a), vb = bit_valid(k.mask, k.bit_off + b);
-      if (va != vb) return false;
-      if (!va) continue;
-    }
-    uint64_t x = load_raw(k.data, k.width, a), y = load_raw(k.data, k.width, b);
-    if (k.vclass == VC_F32) {
-      float fx = __uint_as_float((uint32_t)x), fy = __uint_as_float((uint32_t)y);
-      if (!((isnan(fx) && isnan(fy)) || fx == fy)) return false;
-    } else if (k.vclass == VC_F64) {
-      double fx = __longlong_as_double((long long)x), fy = __longlong_as_double((long long)y);
-      if (!((isnan(fx) && isnan(fy)) || fx == fy)) return false;
-    } else if (k.vclass == VC_BOOL) {
-      if ((x != 0) != (y != 0)) return false;
-    } else if (x != y) {
-      return false;
-    }
-  }
-  return true;
-}
-
-__device__ __forceinline__ uint32_t row_hash(const KeyCol* keys, int num_keys, int64_t row, bool check_nulls)
-{
-  uint32_t h = 0;  // DEFAULT_HASH_SEED
-  for (int c = 0; c < num_keys; ++c) {
-    const KeyCol& k = keys[c];
-    uint32_t e;
-    if (check_nulls && !bit_valid(k.mask, k.bit_off + row)) {
-      e = 0xffffffffu;
-    } else {
-      uint64_t raw = normalize_float_bits(load_raw(k.data, k.width, row), k.vclass);
-      e = murmur3_bits(raw, k.width, 0);
-    }
-    h = hash_combine(h, e);
-  }
-  return h;
-}
-
-__device__ __forceinline__ int64_t slot_of(uint32_t h, int64_t capacity)
-{
-  return (int64_t)(((uint64_t)h * (uint64_t)capacity) >> 32);
-}
-
-__device__ __forceinline__ void flag_overflow(unsigned long long* counters)
-{
-  atom_max(&counters[1], 1ull);
-}
-
-// Returns the record of `row`'s group, inserting the group if needed, or
-// nullptr when the table is over its group budget (overflow flag set).
-__device__ __forceinline__ uint8_t* find_or_insert(const AggParams& p, int64_t row)
-{
-  uint64_t key;
-  uint32_t h;
-  if (p.key_mode == KEY_INT) {
-    const KeyCol& k = p.keys[0];
-    uint64_t raw = load_raw(k.data, k.width, row);
-    key = (uint64_t)widen_int(raw, k.width, k.vclass);
-    if (key == kEmptyIntKey) {
-      uint8_t* rec = p.table + p.capacity * (int64_t)p.stride;
-      unsigned long long* hp = reinterpret_cast<unsigned long long*>(rec);
-      if (*hp == 0ull && atom_cas(hp, 0ull, 1ull) == 0ull) atom_add(&p.counters[0], 1ull);
-      return rec;
-    }
-    h = hash_combine(0u, murmur3_bits(normalize_float_bits(raw, k.vclass), k.width, 0));
-  } else {
-    h = row_hash(p.keys, p.num_keys, row, p.compare_key_nulls != 0);
-    key = ((uint64_t)h << 32) | (uint64_t)row;
-  }
-  const uint64_t empty = p.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
-  int64_t idx = slot_of(h, p.capacity);
-  for (int64_t probe = 0; probe < p.capacity; ++probe) {
-    uint8_t* rec = p.table + idx * (int64_t)p.stride;
-    unsigned long long* hp = reinterpret_cast<unsigned long long*>(rec);
-    uint64_t cur = *hp;  // a stale read can only be EMPTY; the CAS below resolves it
-    for (;;) {
-      if (cur == empty) {
-        if (__hip_atomic_load(&p.counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
-            (unsigned long long)p.max_groups) {
-          flag_overflow(p.counters);
-          return nullptr;
-        }
-        uint64_t seen = atom_cas(hp, empty, key);
-        if (seen == empty) {
-          atom_add(&p.counters[0], 1ull);
-          return rec;
-        }
-        cur = seen;
-        continue;  // re-examine the value that won the slot
-      }
-      if (p.key_mode == KEY_INT) {
-        if (cur == key) return rec;
-      } else if ((uint32_t)(cur >> 32) == h && rows_equal(p, (int64_t)(cur & 0xffffffffull), row)) {
-        return rec;
-      }
-      break;
-    }
-    idx = (idx + 1 == p.capacity) ? 0 : idx + 1;
-  }
-  flag_overflow(p.counters);
-  return nullptr;
-}
-
-__global__ __launch_bounds__(kBlock) void aggregate_kernel(const AggParams p)
-{
-  const int64_t step = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < p.num_rows; row += step) {
-    if (p.skip_null_keys) {
-      bool ok = true;
-      for (int c = 0; c < p.num_keys; ++c) ok = ok && bit_valid(p.keys[c].mask, p.keys[c].bit_off + row);
-      if (!ok) continue;
-    }
-    uint8_t* rec = find_or_insert(p, row);
-    if (rec == nullptr) continue;
-    if (p.rows_op >= 0) apply_op(rec, OP_ROWS, p.ops[p.rows_op].off, 0, 4, VC_UNSIGNED);
-    for (int c = 0; c < p.num_vals; ++c) {
-      const ValCol& v = p.vals[c];
-      if (!bit_valid(v.mask, v.bit_off + row)) continue;
-      uint64_t raw = load_raw(v.data, v.width, row);
-      for (int o = v.op_begin; o < v.op_end; ++o) apply_op(rec, p.ops[o].kind, p.ops[o].off, raw, v.width, v.vclass);
-    }
-  }
-}
 
 // Writes `tmpl` (header + identities) into every record; record `capacity`
 // gets `special_hdr` as header.
-constexpr int kMaxRecordWords = 64;
 struct FillParams {
   uint64_t* table;
   int64_t num_records;
@@ -191,6 +70,18 @@ __global__ __launch_bounds__(kBlock) void fill_table_kernel(const FillParams p)
     uint64_t v = p.tmpl[w];
     if (w == 0 && r == p.num_records - 1) v = p.special_hdr;
     p.table[i] = v;
+  }
+}
+
+// Per-row insert-or-find and atomic update in the global table.
+__global__ __launch_bounds__(kBlock) void aggregate_kernel(const AggParams p)
+{
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < p.num_rows; row += step) {
+    if (!key_row_valid(p, row)) continue;
+    const RowKey rk = row_key(p, row);
+    uint8_t* rec = global_find_or_insert(p, rk, row);
+    if (rec) apply_row(p, rec, row);
   }
 }
 
@@ -238,16 +129,8 @@ __global__ __launch_bounds__(1024) void scan_blocks_kernel(int64_t* counts, int6
   if (threadIdx.x == 0) *total = (unsigned long long)carry;
 }
 
-__device__ __forceinline__ void store_width(void* out, int64_t pos, int width, uint64_t bits)
-{
-  switch (width) {
-    case 1: reinterpret_cast<uint8_t*>(out)[pos] = (uint8_t)bits; break;
-    case 2: reinterpret_cast<uint16_t*>(out)[pos] = (uint16_t)bits; break;
-    case 4: reinterpret_cast<uint32_t*>(out)[pos] = (uint32_t)bits; break;
-    default: reinterpret_cast<uint64_t*>(out)[pos] = bits; break;
-  }
-}
-
+// Compacts the occupied records of the global table in index order and
+// writes keys and results.
 __global__ __launch_bounds__(kBlock) void finalize_kernel(const FinParams p)
 {
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -260,67 +143,10 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(const FinParams p)
   if (!occ) return;
   int64_t pos = p.block_offsets[blockIdx.x] + __popcll(ballot & ((1ull << lane) - 1ull));
   for (unsigned w = 0; w < wave; ++w) pos += wave_counts[w];
-
   const uint8_t* rec = p.table + r * (int64_t)p.stride;
   const uint64_t hdr = *reinterpret_cast<const uint64_t*>(rec);
-  if (p.key_mode == KEY_INT) {
-    uint64_t k = (r == p.capacity) ? kEmptyIntKey : hdr;
-    store_width(p.key_out[0], pos, p.keys[0].width, k);
-  } else {
-    const int64_t rep = (int64_t)(hdr & 0xffffffffull);
-    for (int c = 0; c < p.num_keys; ++c) {
-      const KeyCol& k = p.keys[c];
-      store_width(p.key_out[c], pos, k.width, load_raw(k.data, k.width, rep));
-      if (p.key_valid[c]) p.key_valid[c][pos] = bit_valid(k.mask, k.bit_off + rep) ? 1 : 0;
-    }
-  }
-  for (int o = 0; o < p.num_outs; ++o) {
-    const OutSpec& s = p.outs[o];
-    const uint32_t cnt = *reinterpret_cast<const uint32_t*>(rec + s.count_off);
-    const uint8_t* f = rec + s.field_off;
-    uint64_t bits = 0;
-    switch (s.fin) {
-      case FIN_COUNT: bits = cnt; break;
-      case FIN_INT:
-        bits = (s.out_width == 8) ? *reinterpret_cast<const uint64_t*>(f) : *reinterpret_cast<const uint32_t*>(f);
-        break;
-      case FIN_F64: bits = *reinterpret_cast<const uint64_t*>(f); break;
-      case FIN_F64_TO_F32: bits = __float_as_uint((float)*reinterpret_cast<const double*>(f)); break;
-      case FIN_F32_CODE: {
-        uint32_t c = *reinterpret_cast<const uint32_t*>(f);
-        bits = (c == kF32MinNaN || c == kF32MaxNaN) ? 0x7fc00000u : f32_decode(c);
-        break;
-      }
-      case FIN_F64_CODE: {
-        uint64_t c = *reinterpret_cast<const uint64_t*>(f);
-        bits = (c == kF64MinNaN || c == kF64MaxNaN) ? 0x7ff8000000000000ull : f64_decode(c);
-        break;
-      }
-      case FIN_MEAN_I64: {
-        double m = cnt ? (double)*reinterpret_cast<const int64_t*>(f) / (double)cnt : 0.0;
-        bits = (uint64_t)__double_as_longlong(m);
-        break;
-      }
-      case FIN_MEAN_F64: {
-        double m = cnt ? *reinterpret_cast<const double*>(f) / (double)cnt : 0.0;
-        bits = (uint64_t)__double_as_longlong(m);
-        break;
-      }
-      case FIN_MEAN_F32: {
-        double m = cnt ? (double)(float)*reinterpret_cast<const double*>(f) / (double)cnt : 0.0;
-        bits = (uint64_t)__double_as_longlong(m);
-        break;
-      }
-      case FIN_MEAN_DURATION: {
-        int64_t m = cnt ? *reinterpret_cast<const int64_t*>(f) / (int64_t)cnt : 0;
-        bits = (uint64_t)m;
-        break;
-      }
-      default: break;
-    }
-    store_width(s.out, pos, s.out_width, bits);
-    if (s.valid) s.valid[pos] = cnt != 0;
-  }
+  finalize_keys(hdr, r == p.capacity, p.key_mode, p.keys, p.num_keys, pos, *p.out);
+  finalize_record(rec, pos, *p.out);
 }
 
 // Packs validity bytes into a cudf bitmask (words beyond n are zeroed) and
@@ -354,8 +180,6 @@ struct SampleParams {
   unsigned long long* stats;  // [0] distinct, [1] f1, [2] f2, [3] sampled rows
 };
 
-__device__ __forceinline__ bool sample_rows_equal(const AggParams& p, int64_t a, int64_t b) { return rows_equal(p, a, b); }
-
 __global__ __launch_bounds__(kBlock) void sample_insert_kernel(const SampleParams s)
 {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -363,34 +187,20 @@ __global__ __launch_bounds__(kBlock) void sample_insert_kernel(const SampleParam
   const AggParams& p = s.agg;
   const int64_t span = p.num_rows / s.samples;
   int64_t row = j * span + (span > 1 ? (int64_t)(fmix32((uint32_t)j * 0x9e3779b9u + 1u) % (uint32_t)span) : 0);
-  if (row >= p.num_rows) return;
-  if (p.skip_null_keys) {
-    for (int c = 0; c < p.num_keys; ++c)
-      if (!bit_valid(p.keys[c].mask, p.keys[c].bit_off + row)) return;
-  }
+  if (row >= p.num_rows || !key_row_valid(p, row)) return;
   atom_add(&s.stats[3], 1ull);
-  uint64_t key;
-  uint32_t h;
-  if (p.key_mode == KEY_INT) {
-    const KeyCol& k = p.keys[0];
-    uint64_t raw = load_raw(k.data, k.width, row);
-    key = (uint64_t)widen_int(raw, k.width, k.vclass);
-    if (key == kEmptyIntKey) {
-      atom_add(&s.cnt[s.scap], 1u);
-      return;
-    }
-    h = hash_combine(0u, murmur3_bits(normalize_float_bits(raw, k.vclass), k.width, 0));
-  } else {
-    h = row_hash(p.keys, p.num_keys, row, p.compare_key_nulls != 0);
-    key = ((uint64_t)h << 32) | (uint64_t)row;
+  const RowKey rk = row_key(p, row);
+  if (rk.special) {
+    atom_add(&s.cnt[s.scap], 1u);
+    return;
   }
   const uint64_t empty = p.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
-  int64_t idx = slot_of(h, s.scap);
+  int64_t idx = reduce_range(rk.hash, s.scap);
   for (int64_t probe = 0; probe < s.scap; ++probe) {
     uint64_t cur = s.hdr[idx];
     for (;;) {
       if (cur == empty) {
-        uint64_t seen = atom_cas(&s.hdr[idx], empty, key);
+        uint64_t seen = atom_cas(&s.hdr[idx], empty, rk.key);
         if (seen == empty) {
           atom_add(&s.cnt[idx], 1u);
           return;
@@ -399,8 +209,9 @@ __global__ __launch_bounds__(kBlock) void sample_insert_kernel(const SampleParam
         continue;
       }
       bool eq = p.key_mode == KEY_INT
-                  ? cur == key
-                  : ((uint32_t)(cur >> 32) == h && sample_rows_equal(p, (int64_t)(cur & 0xffffffffull), row));
+                  ? cur == rk.key
+                  : ((uint32_t)(cur >> 32) == rk.hash &&
+                     rows_equal(p.keys, p.num_keys, p.compare_key_nulls != 0, (int64_t)(cur & 0xffffffffull), row));
       if (eq) {
         atom_add(&s.cnt[idx], 1u);
         return;
@@ -431,6 +242,349 @@ __global__ __launch_bounds__(kBlock) void fill_u64_kernel(unsigned long long* p,
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) p[i] = v;
 }
 
+// ---- hash partitioning
+
+__device__ __forceinline__ uint32_t partition_of(uint32_t hash, int log2p) { return log2p ? hash >> (32 - log2p) : 0u; }
+
+// Rows per partition of each workgroup's contiguous slice -> hist[p][block].
+__global__ __launch_bounds__(kBlock) void part_hist_kernel(const PartParams pp)
+{
+  extern __shared__ uint32_t counts[];
+  const AggParams& p = pp.src;
+  const int P = 1 << pp.log2p;
+  for (int i = threadIdx.x; i < P; i += kBlock) counts[i] = 0;
+  __syncthreads();
+  const int64_t begin = (int64_t)blockIdx.x * pp.chunk;
+  const int64_t end = begin + pp.chunk < p.num_rows ? begin + pp.chunk : p.num_rows;
+  for (int64_t row = begin + threadIdx.x; row < end; row += kBlock) {
+    if (!key_row_valid(p, row)) continue;
+    atomicAdd(&counts[partition_of(row_key(p, row).hash, pp.log2p)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < P; i += kBlock) pp.hist[(int64_t)i * pp.nblocks + blockIdx.x] = counts[i];
+}
+
+// Writes one row as a partition record: key and value columns at their
+// record offsets (native width), value validity as bits of a flags word.
+__device__ __forceinline__ void write_record(const PartParams& pp, uint8_t* rec, int64_t row)
+{
+  const AggParams& p = pp.src;
+  for (int c = 0; c < p.num_keys; ++c) {
+    const KeyCol& k = p.keys[c];
+    store_width(rec + pp.key_off[c], 0, k.width, load_at(k.data, k.width, k.stride, row));
+  }
+  uint32_t flags = 0;
+  for (int c = 0; c < p.num_vals; ++c) {
+    const ValCol& v = p.vals[c];
+    store_width(rec + pp.val_off[c], 0, v.width, load_at(v.data, v.width, v.stride, row));
+    if (v.flag_bit >= 0 && bit_valid(v.mask, v.bit_off + row)) flags |= 1u << v.flag_bit;
+  }
+  if (pp.flag_bytes) store_width(rec + pp.flags_off, 0, pp.flag_bytes, flags);
+}
+
+// Copies every row with a valid key to its partition's range of this
+// workgroup (offsets[p][block] onward) as a record.
+__global__ __launch_bounds__(kBlock) void part_scatter_kernel(const PartParams pp)
+{
+  extern __shared__ uint32_t cursor[];
+  const AggParams& p = pp.src;
+  const int P = 1 << pp.log2p;
+  for (int i = threadIdx.x; i < P; i += kBlock) cursor[i] = pp.offsets[(int64_t)i * pp.nblocks + blockIdx.x];
+  __syncthreads();
+  const int64_t begin = (int64_t)blockIdx.x * pp.chunk;
+  const int64_t end = begin + pp.chunk < p.num_rows ? begin + pp.chunk : p.num_rows;
+  for (int64_t row = begin + threadIdx.x; row < end; row += kBlock) {
+    if (!key_row_valid(p, row)) continue;
+    const uint32_t part = partition_of(row_key(p, row).hash, pp.log2p);
+    const int64_t dst = atomicAdd(&cursor[part], 1u);
+    write_record(pp, pp.out + dst * (int64_t)pp.rec_bytes, row);
+  }
+}
+
+// Exclusive scan of one value per thread across a 1024-thread workgroup
+// (16 wave64); returns the thread's prefix, and the total through *total.
+// `tmp` holds 16 words of LDS.
+__device__ __forceinline__ uint32_t block_exclusive_scan_1024(uint32_t v, uint32_t* tmp, uint32_t* total)
+{
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= (unsigned)d) x += y;
+  }
+  if (lane == 63) tmp[wave] = x;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t w = lane < 16 ? tmp[lane] : 0u;
+    for (int d = 1; d < 16; d <<= 1) {
+      uint32_t y = __shfl_up(w, d, 64);
+      if (lane >= (unsigned)d) w += y;
+    }
+    if (lane < 16) tmp[lane] = w;  // inclusive wave totals
+  }
+  __syncthreads();
+  const uint32_t before = wave ? tmp[wave - 1] : 0u;
+  *total = tmp[15];
+  __syncthreads();
+  return before + x - v;
+}
+
+constexpr int kScatterItems = 4;                        // rows per thread per tile (at most)
+constexpr int kScatterTile = kLdsBlock * kScatterItems;  // 4096 rows
+
+// Tiled scatter: each tile of rows is counting-sorted by partition in LDS
+// (one LDS atomic per row gives its rank inside its partition's run) and
+// staged as records; consecutive threads then copy consecutive 8-byte words
+// of the sorted records, so each partition's run leaves as one contiguous
+// stretch. LDS: per-partition cursor and tile count, partition id per
+// staged row, the staged records.
+__global__ __launch_bounds__(kLdsBlock) void part_scatter_tiled_kernel(const PartParams pp)
+{
+  extern __shared__ __align__(16) uint8_t smem[];
+  const AggParams& p = pp.src;
+  const int P = 1 << pp.log2p;
+  const int tile_rows_max = pp.tile_rows;             // multiple of kLdsBlock, <= kScatterTile
+  const int rec_words = pp.rec_bytes / 8;
+  uint32_t* cursor = reinterpret_cast<uint32_t*>(smem);
+  uint32_t* tcount = cursor + P;
+  uint32_t* scan_tmp = tcount + P;  // 16 words
+  uint16_t* pid = reinterpret_cast<uint16_t*>(scan_tmp + 16);
+  uint64_t* stage = reinterpret_cast<uint64_t*>(
+    (reinterpret_cast<uintptr_t>(pid + kScatterTile) + 15) & ~uintptr_t(15));
+  uint64_t* out = reinterpret_cast<uint64_t*>(pp.out);
+  for (int i = threadIdx.x; i < P; i += kLdsBlock) {
+    cursor[i] = pp.offsets[(int64_t)i * pp.nblocks + blockIdx.x];
+    tcount[i] = 0;
+  }
+  __syncthreads();
+  const int64_t begin = (int64_t)blockIdx.x * pp.chunk;
+  const int64_t end = begin + pp.chunk < p.num_rows ? begin + pp.chunk : p.num_rows;
+  const int items = tile_rows_max / kLdsBlock;
+  for (int64_t tile = begin; tile < end; tile += tile_rows_max) {
+    uint32_t part[kScatterItems], rank[kScatterItems];
+    for (int k = 0; k < kScatterItems; ++k) {
+      part[k] = 0xffffffffu;
+      if (k >= items) continue;
+      const int64_t row = tile + k * kLdsBlock + threadIdx.x;
+      if (row < end && key_row_valid(p, row)) {
+        part[k] = partition_of(row_key(p, row).hash, pp.log2p);
+        rank[k] = atomicAdd(&tcount[part[k]], 1u);
+      }
+    }
+    __syncthreads();
+    // tile-local start of each partition's run
+    uint32_t tile_rows = 0;
+    for (int base = 0; base < P; base += kLdsBlock) {
+      const int i = base + threadIdx.x;
+      const uint32_t c = i < P ? tcount[i] : 0u;
+      uint32_t chunk_total;
+      const uint32_t ex = block_exclusive_scan_1024(c, scan_tmp, &chunk_total);
+      if (i < P) tcount[i] = tile_rows + ex;  // now the run start
+      tile_rows += chunk_total;
+      __syncthreads();
+    }
+    // stage rows in partition order
+    for (int k = 0; k < kScatterItems; ++k) {
+      if (part[k] == 0xffffffffu) continue;
+      const int64_t row = tile + k * kLdsBlock + threadIdx.x;
+      const uint32_t pos = tcount[part[k]] + rank[k];
+      pid[pos] = (uint16_t)part[k];
+      write_record(pp, reinterpret_cast<uint8_t*>(stage) + (int64_t)pos * pp.rec_bytes, row);
+    }
+    __syncthreads();
+    // copy out word by word: the words of one partition's run are contiguous
+    const uint32_t words = tile_rows * (uint32_t)rec_words;
+    for (uint32_t w = threadIdx.x; w < words; w += kLdsBlock) {
+      const uint32_t pos = w / (uint32_t)rec_words;
+      const uint32_t q = pid[pos];
+      const int64_t dst = (int64_t)cursor[q] + (pos - tcount[q]);
+      out[dst * rec_words + (w - pos * (uint32_t)rec_words)] = stage[w];
+    }
+    __syncthreads();
+    // advance cursors by this tile's runs and clear the counts
+    for (int i = threadIdx.x; i < P; i += kLdsBlock) {
+      const uint32_t next = (i + 1 < P) ? tcount[i + 1] : tile_rows;
+      cursor[i] += next - tcount[i];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < P; i += kLdsBlock) tcount[i] = 0;
+    __syncthreads();
+  }
+}
+
+// ---- exclusive scan of u32 counts (values and total below 2^32)
+
+__global__ __launch_bounds__(256) void scan_reduce_u32_kernel(const uint32_t* in, int64_t n, uint32_t* bsum)
+{
+  __shared__ uint32_t red[256];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  uint32_t s = 0;
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) s += in[base + k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = 128; d > 0; d >>= 1) {
+    if (threadIdx.x < (unsigned)d) red[threadIdx.x] += red[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void scan_bsums_u32_kernel(uint32_t* bsum, int64_t n, unsigned long long* total)
+{
+  __shared__ uint32_t buf[1024];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < n; base += 1024) {
+    int64_t i = base + threadIdx.x;
+    uint32_t v = i < n ? bsum[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+      uint32_t t = threadIdx.x >= (unsigned)d ? buf[threadIdx.x - d] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < n) bsum[i] = carry + buf[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += buf[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = (unsigned long long)carry;
+}
+
+__global__ __launch_bounds__(256) void scan_apply_u32_kernel(const uint32_t* in, int64_t n, const uint32_t* bsum,
+                                                            uint32_t* out)
+{
+  __shared__ uint32_t buf[256];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0u;
+    s += v[k];
+  }
+  buf[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    uint32_t t = threadIdx.x >= (unsigned)d ? buf[threadIdx.x - d] : 0;
+    __syncthreads();
+    buf[threadIdx.x] += t;
+    __syncthreads();
+  }
+  uint32_t run = bsum[blockIdx.x] + buf[threadIdx.x] - s;
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+}
+
+// ---- LDS aggregation of one work item
+
+__global__ __launch_bounds__(kLdsBlock) void lds_agg_kernel(const LdsParams lp)
+{
+  extern __shared__ __align__(16) uint8_t lds[];
+  const AggParams& p = lp.src;
+  LdsTable t;
+  t.rec = lds;
+  t.slots = lp.slots;
+  t.stride = p.stride;
+  t.cap = lp.cap;
+  int32_t* ctr = reinterpret_cast<int32_t*>(lds + (int64_t)(lp.slots + 1) * p.stride);
+  t.count = ctr;
+  t.overflow = ctr + 1;
+  int32_t* scan = ctr + 4;  // kLdsBlock ints
+
+  uint64_t* w64 = reinterpret_cast<uint64_t*>(lds);
+  const int total_words = (lp.slots + 1) * lp.words;
+  for (int i = threadIdx.x; i < total_words; i += kLdsBlock) {
+    const int wi = i % lp.words;
+    uint64_t v = lp.tmpl[wi];
+    if (wi == 0 && i / lp.words == lp.slots) v = 0ull;  // the special record starts unoccupied
+    w64[i] = v;
+  }
+  if (threadIdx.x < 4) ctr[threadIdx.x] = 0;
+  __syncthreads();
+
+  int64_t begin, end;
+  if (lp.direct) {
+    begin = (int64_t)blockIdx.x * lp.rows_per_item;
+    end = begin + lp.rows_per_item < p.num_rows ? begin + lp.rows_per_item : p.num_rows;
+  } else {
+    const int P = 1 << lp.log2p;
+    begin = lp.offsets[(int64_t)blockIdx.x * lp.nblocks];
+    end = (int)blockIdx.x + 1 < P ? (int64_t)lp.offsets[(int64_t)(blockIdx.x + 1) * lp.nblocks]
+                                  : (int64_t)*lp.total_rows;
+  }
+  for (int64_t row = begin + threadIdx.x; row < end; row += kLdsBlock) {
+    if (!lp.direct && __hip_atomic_load(t.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+    if (lp.direct && !key_row_valid(p, row)) continue;
+    const RowKey rk = row_key(p, row);
+    const uint32_t slot_hash = rk.hash << lp.log2p;
+    uint8_t* rec = lds_find_or_insert(t, p, rk, slot_hash, row);
+    if (rec) {
+      apply_row(p, rec, row);
+    } else if (lp.direct) {
+      uint8_t* g = global_find_or_insert(p, rk, row);
+      if (g) apply_row(p, g, row);
+    }
+  }
+  __syncthreads();
+
+  if (lp.direct) {
+    // fold this item's groups into the global table
+    for (int i = threadIdx.x; i <= lp.slots; i += kLdsBlock) {
+      if (!lds_occupied(t, i, p.key_mode)) continue;
+      const uint8_t* rec = lds_record(t, i);
+      const uint64_t hdr = *reinterpret_cast<const uint64_t*>(rec);
+      RowKey rk;
+      int64_t rep = 0;
+      if (p.key_mode == KEY_INT) {
+        const KeyCol& k = p.keys[0];
+        rk.special = i == lp.slots;
+        rk.key = rk.special ? kEmptyIntKey : hdr;
+        const uint64_t raw = k.width == 8 ? rk.key : (rk.key & ((1ull << (8 * k.width)) - 1ull));
+        rk.hash = hash_combine(0u, murmur3_bits(normalize_float_bits(raw, k.vclass), k.width, 0));
+      } else {
+        rep = (int64_t)(hdr & 0xffffffffull);
+        rk = row_key(p, rep);
+      }
+      uint8_t* g = global_find_or_insert(p, rk, rep);
+      if (g) merge_record(p, g, rec);
+    }
+    return;
+  }
+
+  if (*t.overflow) {
+    if (threadIdx.x == 0) atom_max(&lp.out_counters[1], 1ull);
+    return;
+  }
+  int mine = 0;
+  for (int i = threadIdx.x; i <= lp.slots; i += kLdsBlock) mine += lds_occupied(t, i, p.key_mode) ? 1 : 0;
+  scan[threadIdx.x] = mine;
+  __syncthreads();
+  for (int d = 1; d < kLdsBlock; d <<= 1) {
+    int v = threadIdx.x >= (unsigned)d ? scan[threadIdx.x - d] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  if (threadIdx.x == kLdsBlock - 1)
+    ctr[2] = (int32_t)atomicAdd(&lp.out_counters[0], (unsigned long long)scan[kLdsBlock - 1]);
+  __syncthreads();
+  int64_t pos = (int64_t)ctr[2] + scan[threadIdx.x] - mine;
+  const OutTable& ot = *lp.out;
+  for (int i = threadIdx.x; i <= lp.slots; i += kLdsBlock) {
+    if (!lds_occupied(t, i, p.key_mode)) continue;
+    const uint8_t* rec = lds_record(t, i);
+    finalize_keys(*reinterpret_cast<const uint64_t*>(rec), i == lp.slots, p.key_mode, p.keys, p.num_keys, pos, ot);
+    finalize_record(rec, pos, ot);
+    ++pos;
+  }
+}
+
 // ---- table hashing entry points
 
 struct HashParams {
@@ -453,7 +607,7 @@ __global__ __launch_bounds__(kBlock) void murmur_table_kernel(const HashParams p
     if (p.check_nulls && !bit_valid(k.mask, k.bit_off + row)) {
       e = 0xffffffffu;
     } else {
-      e = murmur3_bits(normalize_float_bits(load_raw(k.data, k.width, row), k.vclass), k.width, p.seed);
+      e = murmur3_bits(normalize_float_bits(load_at(k.data, k.width, k.stride, row), k.vclass), k.width, p.seed);
     }
     h = hash_combine(h, e);
   }
@@ -468,7 +622,7 @@ __global__ __launch_bounds__(kBlock) void spark_murmur_table_kernel(const HashPa
   for (int c = 0; c < p.num_cols; ++c) {
     const KeyCol& k = p.cols[c];
     if (!bit_valid(k.mask, k.bit_off + row)) continue;  // a null keeps the running hash
-    uint64_t raw = load_raw(k.data, k.width, row);
+    uint64_t raw = load_at(k.data, k.width, k.stride, row);
     if (k.width < 4) {
       // bool / int8 / uint8 / int16 / uint16 are hashed as 4-byte integers
       uint32_t w = k.vclass == VC_SIGNED ? (uint32_t)(int32_t)widen_int(raw, k.width, k.vclass)
@@ -627,16 +781,24 @@ struct DeviceBuffer {
   }
 };
 
+constexpr size_t kStagingBytes = 64 * 1024;  // pinned parameter staging (kUploadSlots slots)
+constexpr int kUploadSlots = 8;
+
 struct Workspace {
   std::mutex mu;
-  DeviceBuffer main;   // table, counters, block counts, sample table
-  DeviceBuffer outs;   // validity bytes of the outputs
+  DeviceBuffer main;    // tables, partitioned columns, histograms, counters, sample table
+  DeviceBuffer outs;    // temporary output columns
+  DeviceBuffer params;  // kernel parameter blocks (output tables, identity templates)
   unsigned long long* pinned = nullptr;  // host readback area
+  uint8_t* staging = nullptr;            // pinned source of parameter uploads
+  int upload_slot = 0;                   // next staging / params slot of the current call
   ~Workspace()
   {
     if (main.ptr) (void)hipFree(main.ptr);
     if (outs.ptr) (void)hipFree(outs.ptr);
+    if (params.ptr) (void)hipFree(params.ptr);
     if (pinned) (void)hipHostFree(pinned);
+    if (staging) (void)hipHostFree(staging);
   }
 };
 
@@ -654,6 +816,7 @@ Workspace* workspace_for(hipStream_t s)
 }
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+constexpr size_t align_up_c(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 constexpr int kMaxPinned = 1024;  // host readback slots (counters, null counts)
 
@@ -688,7 +851,8 @@ struct PlanCol {
   int type;
   bool nullable;
   std::vector<std::pair<int, int>> ops;  // (op kind, byte offset)
-  int count_off = -1;       // OP_COUNT field, or the rows field when not nullable
+  int count_off = -1;       // the row-count field
+  int nulls_off = -1;       // OP_NULLS field when the column has nulls
   int op_offset(int kind) const
   {
     for (auto& o : ops)
@@ -743,6 +907,7 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
     kc.bit_off = k.offset;
     kc.type_id = k.type_id;
     kc.width = w;
+    kc.stride = w;
     kc.vclass = value_class(k.type_id);
     plan.key_has_nulls |= kc.mask != nullptr;
     plan.keys.push_back(kc);
@@ -785,6 +950,7 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
       pc.col.mask = pc.nullable ? v.null_mask : nullptr;
       pc.col.bit_off = v.offset;
       pc.col.width = w;
+      pc.col.stride = w;
       pc.col.vclass = value_class(v.type_id);
       plan.cols.push_back(pc);
     } else {
@@ -811,10 +977,10 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
       case MGB_AGG_COUNT_ALL: need_rows = true; break;
       default: break;
     }
-    if (wt.kind != MGB_AGG_COUNT_ALL) {
-      if (pc.nullable) add_op(pc, OP_COUNT);
-      else need_rows = true;
-    }
+    // COUNT_VALID = rows - nulls: one shared row count per group plus a null
+    // count per nullable column (updated only for null values)
+    need_rows = true;
+    if (wt.kind != MGB_AGG_COUNT_ALL && pc.nullable) add_op(pc, OP_NULLS);
   }
 
   // record layout: 8-byte header, 8-byte fields, then 4-byte fields
@@ -829,7 +995,10 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
   plan.stride = (int)align_up((size_t)off, 8);
   if (plan.stride > kMaxRecordWords * 8)
     return fail(MGB_ERROR_UNSUPPORTED, "too many distinct aggregations for one groupby call");
-  for (auto& pc : plan.cols) pc.count_off = pc.nullable ? pc.op_offset(OP_COUNT) : plan.rows_off;
+  for (auto& pc : plan.cols) {
+    pc.count_off = plan.rows_off;
+    pc.nulls_off = pc.nullable ? pc.op_offset(OP_NULLS) : -1;
+  }
 
   // outputs
   for (auto& wt : wants) {
@@ -841,6 +1010,7 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
     o.out_type = target_type(t, wt.kind);
     OutSpec& s = o.spec;
     s.count_off = pc.count_off;
+    s.nulls_off = pc.nulls_off;
     s.out_width = type_width(o.out_type);
     s.valid = reinterpret_cast<uint8_t*>(1);  // nullable marker; real pointer set at finalize
     switch (wt.kind) {
@@ -851,6 +1021,7 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
       case MGB_AGG_COUNT_ALL:
         s.fin = FIN_COUNT;
         s.count_off = plan.rows_off;
+        s.nulls_off = -1;
         s.valid = nullptr;
         break;
       case MGB_AGG_SUM: case MGB_AGG_PRODUCT: case MGB_AGG_SUM_OF_SQUARES: {
@@ -923,70 +1094,267 @@ struct Allocs {
 
 }  // namespace
 
-// Runs the full groupby for a validated plan. Exposed inside the library so
-// the batched executor can reuse it.
-mgb_status run_groupby(const Plan& plan, int64_t num_rows, mgb_out_column* out_keys, mgb_out_column* out_results,
-                       int64_t* out_num_groups, hipStream_t stream, const mgb_allocator* alloc);
 
-mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mgb_out_column* out_results,
-                       int64_t* out_num_groups, hipStream_t stream, const mgb_allocator* alloc)
+namespace {
+
+// Temporary output columns in the workspace (filled by finalize / LDS
+// kernels, then copied into the caller's allocations).
+struct TempOuts {
+  std::vector<uint8_t*> key_data, key_valid, out_data, out_valid;
+  unsigned long long* null_counts = nullptr;
+  int64_t rows = 0;  // capacity in groups
+};
+
+mgb_status layout_temp(Workspace* ws, const Plan& plan, int64_t rows, hipStream_t stream, TempOuts& t,
+                       OutTable& ot)
 {
-  Workspace* ws = workspace_for(stream);
-  std::lock_guard<std::mutex> lk(ws->mu);
-  if (!ws->pinned)
-    MGB_HIP(hipHostMalloc((void**)&ws->pinned, kMaxPinned * sizeof(unsigned long long), hipHostMallocDefault));
-  const int cus = device_cu_count();
-
-  AggParams base{};
-  base.num_rows = R;
-  base.key_mode = plan.key_mode;
-  base.num_keys = (int)plan.keys.size();
-  base.skip_null_keys = plan.skip_null_keys;
-  base.compare_key_nulls = plan.compare_key_nulls;
-  base.stride = plan.stride;
-  for (size_t c = 0; c < plan.keys.size(); ++c) base.keys[c] = plan.keys[c];
-
-  // ---- capacity from a sampled estimate
-  const int64_t kSample = 1 << 16;
-  int64_t capacity;
-  if (R <= 4 * kSample) {
-    capacity = std::max<int64_t>(64, 2 * R);
-  } else {
-    int64_t scap = 4 * kSample;
-    size_t need = align_up((size_t)(scap + 1) * 8, 256) + align_up((size_t)(scap + 1) * 4, 256) + 256;
-    MGB_HIP(ws->main.reserve(need, stream));
-    SampleParams sp{};
-    sp.agg = base;
-    sp.hdr = static_cast<unsigned long long*>(ws->main.ptr);
-    sp.cnt = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws->main.ptr) + align_up((size_t)(scap + 1) * 8, 256));
-    sp.stats = reinterpret_cast<unsigned long long*>(reinterpret_cast<uint8_t*>(sp.cnt) +
-                                                     align_up((size_t)(scap + 1) * 4, 256));
-    sp.scap = scap;
-    sp.samples = kSample;
-    uint64_t empty = plan.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
-    fill_u64_kernel<<<256, kBlock, 0, stream>>>(sp.hdr, scap, empty);
-    MGB_HIP(hipMemsetAsync(sp.cnt, 0, (size_t)(scap + 1) * 4, stream));
-    MGB_HIP(hipMemsetAsync(sp.stats, 0, 4 * sizeof(unsigned long long), stream));
-    sample_insert_kernel<<<(unsigned)((kSample + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp);
-    sample_stats_kernel<<<(unsigned)((scap + 1 + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp.cnt, scap + 1, sp.stats);
-    MGB_HIP(hipGetLastError());
-    MGB_HIP(hipMemcpyAsync(ws->pinned, sp.stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-    MGB_HIP(hipStreamSynchronize(stream));
-    int64_t est = estimate_groups(ws->pinned[0], ws->pinned[1], ws->pinned[2], ws->pinned[3], R);
-    capacity = std::max<int64_t>(64, 2 * est + 1024);
+  const int nk = (int)plan.keys.size(), no = (int)plan.outs.size();
+  if (no > kMaxOuts) return fail(MGB_ERROR_UNSUPPORTED, "more than 64 aggregations in one groupby call");
+  const size_t n = (size_t)std::max<int64_t>(rows, 1);
+  size_t off = 0;
+  std::vector<size_t> koff(nk), kvoff(nk, SIZE_MAX), ooff(no), ovoff(no, SIZE_MAX);
+  for (int c = 0; c < nk; ++c) {
+    koff[c] = off;
+    off += align_up(n * plan.keys[c].width, 256);
+    if (plan.compare_key_nulls && plan.keys[c].mask) {
+      kvoff[c] = off;
+      off += align_up(n, 256);
+    }
   }
+  for (int o = 0; o < no; ++o) {
+    ooff[o] = off;
+    off += align_up(n * plan.outs[o].spec.out_width, 256);
+    if (plan.outs[o].spec.valid) {
+      ovoff[o] = off;
+      off += align_up(n, 256);
+    }
+  }
+  size_t nc_off = off;
+  off += align_up((size_t)(nk + no) * 8, 256);
+  MGB_HIP(ws->outs.reserve(off, stream));
+  uint8_t* b = static_cast<uint8_t*>(ws->outs.ptr);
+  t = TempOuts{};
+  t.rows = rows;
+  ot = OutTable{};
+  for (int c = 0; c < nk; ++c) {
+    t.key_data.push_back(b + koff[c]);
+    t.key_valid.push_back(kvoff[c] == SIZE_MAX ? nullptr : b + kvoff[c]);
+    ot.key_out[c] = t.key_data[c];
+    ot.key_valid[c] = t.key_valid[c];
+  }
+  ot.num_outs = no;
+  for (int o = 0; o < no; ++o) {
+    t.out_data.push_back(b + ooff[o]);
+    t.out_valid.push_back(ovoff[o] == SIZE_MAX ? nullptr : b + ovoff[o]);
+    ot.outs[o] = plan.outs[o].spec;
+    ot.outs[o].out = t.out_data[o];
+    ot.outs[o].valid = t.out_valid[o];
+  }
+  t.null_counts = reinterpret_cast<unsigned long long*>(b + nc_off);
+  return MGB_SUCCESS;
+}
 
-  // ---- build the table (retry with a larger one on overflow)
-  int64_t groups = 0;
+// Copies an OutTable and the identity template to the device (through the
+// pinned staging area, ordered on `stream`).
+mgb_status upload_params(Workspace* ws, const OutTable& ot, const uint64_t* tmpl, int words, hipStream_t stream,
+                         const OutTable** d_ot, const uint64_t** d_tmpl)
+{
+  const size_t ot_bytes = align_up(sizeof(OutTable), 256);
+  const size_t slot_bytes = kStagingBytes / kUploadSlots;
+  static_assert(align_up_c(sizeof(OutTable), 256) + kMaxRecordWords * 8 <= kStagingBytes / kUploadSlots,
+                "parameter slot too small");
+  if (ws->upload_slot >= kUploadSlots) return fail(MGB_ERROR_UNSUPPORTED, "too many parameter uploads in one call");
+  const int slot = ws->upload_slot++;
+  MGB_HIP(ws->params.reserve(kStagingBytes, stream));
+  uint8_t* host = ws->staging + slot * slot_bytes;
+  uint8_t* dev = static_cast<uint8_t*>(ws->params.ptr) + slot * slot_bytes;
+  std::memcpy(host, &ot, sizeof(OutTable));
+  std::memcpy(host + ot_bytes, tmpl, (size_t)words * 8);
+  MGB_HIP(hipMemcpyAsync(dev, host, ot_bytes + (size_t)words * 8, hipMemcpyHostToDevice, stream));
+  *d_ot = reinterpret_cast<const OutTable*>(dev);
+  *d_tmpl = reinterpret_cast<const uint64_t*>(dev + ot_bytes);
+  return MGB_SUCCESS;
+}
+
+// Allocates the caller's columns for `groups` rows, copies the temporary
+// outputs into them and builds their null masks.
+mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int64_t groups, mgb_out_column* out_keys,
+                        mgb_out_column* out_results, hipStream_t stream, const mgb_allocator* alloc)
+{
+  Allocs al{alloc, stream, {}};
+  const int nk = (int)plan.keys.size(), no = (int)plan.outs.size();
+  auto fill_col = [&](mgb_out_column& oc, int type_id, int width, const uint8_t* data, const uint8_t* valid,
+                      bool nullable) -> mgb_status {
+    std::memset(&oc, 0, sizeof(oc));
+    oc.type_id = type_id;
+    oc.size = groups;
+    if (groups == 0) return MGB_SUCCESS;
+    oc.data_bytes = (size_t)groups * width;
+    oc.data = al.get(oc.data_bytes);
+    if (!oc.data) return fail(MGB_ERROR_ALLOC, "allocation of an output column failed");
+    MGB_HIP(hipMemcpyAsync(oc.data, data, oc.data_bytes, hipMemcpyDeviceToDevice, stream));
+    if (nullable && valid) {
+      oc.mask_bytes = (size_t)bitmask_bytes(groups);
+      oc.null_mask = static_cast<uint32_t*>(al.get(oc.mask_bytes));
+      if (!oc.null_mask) return fail(MGB_ERROR_ALLOC, "allocation of an output null mask failed");
+    }
+    return MGB_SUCCESS;
+  };
+  mgb_status st = MGB_SUCCESS;
+  for (int c = 0; c < nk && st == MGB_SUCCESS; ++c)
+    st = fill_col(out_keys[c], plan.keys[c].type_id, plan.keys[c].width, t.key_data[c], t.key_valid[c],
+                  t.key_valid[c] != nullptr);
+  for (int o = 0; o < no && st == MGB_SUCCESS; ++o)
+    st = fill_col(out_results[o], plan.outs[o].out_type, plan.outs[o].spec.out_width, t.out_data[o], t.out_valid[o],
+                  t.out_valid[o] != nullptr);
+  if (st != MGB_SUCCESS) {
+    al.release_all();
+    return st;
+  }
+  if (groups > 0) {
+    MGB_HIP(hipMemsetAsync(t.null_counts, 0, (size_t)(nk + no) * 8, stream));
+    const int64_t words = bitmask_bytes(groups) / 4;
+    for (int j = 0; j < no + nk; ++j) {
+      mgb_out_column& oc = j < no ? out_results[j] : out_keys[j - no];
+      const uint8_t* valid = j < no ? t.out_valid[j] : t.key_valid[j - no];
+      if (!oc.null_mask) continue;
+      pack_validity_kernel<<<(unsigned)((words + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+        valid, groups, oc.null_mask, words, t.null_counts + j);
+    }
+    MGB_HIP(hipGetLastError());
+    MGB_HIP(hipMemcpyAsync(ws->pinned, t.null_counts, (size_t)(no + nk) * 8, hipMemcpyDeviceToHost, stream));
+    MGB_HIP(hipStreamSynchronize(stream));
+    for (int o = 0; o < no; ++o) out_results[o].null_count = out_results[o].null_mask ? (int64_t)ws->pinned[o] : 0;
+    for (int c = 0; c < nk; ++c) out_keys[c].null_count = out_keys[c].null_mask ? (int64_t)ws->pinned[no + c] : 0;
+  }
+  return MGB_SUCCESS;
+}
+
+// Fills AggParams' value columns and ops from the plan; false when they do
+// not fit one launch.
+bool single_launch_ops(const Plan& plan, AggParams& ap)
+{
+  ap.num_vals = 0;
+  ap.num_ops = 0;
+  ap.rows_op = -1;
+  if (plan.rows_off >= 0) {
+    ap.ops[0] = OpDesc{OP_ROWS, plan.rows_off};
+    ap.rows_op = 0;
+    ap.num_ops = 1;
+  }
+  int flag_bit = 0;
+  for (const PlanCol& pc : plan.cols) {
+    if (ap.num_vals >= kMaxVals || ap.num_ops + (int)pc.ops.size() > kMaxOps) return false;
+    ValCol vc = pc.col;
+    vc.flag_bit = pc.nullable ? flag_bit++ : -1;
+    vc.nulls_off = pc.nulls_off;
+    vc.op_begin = ap.num_ops;
+    for (auto& o : pc.ops) ap.ops[ap.num_ops++] = OpDesc{o.first, o.second};
+    vc.op_end = ap.num_ops;
+    ap.vals[ap.num_vals++] = vc;
+  }
+  return true;
+}
+
+size_t lds_limit()
+{
+  static size_t lim = [] {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+      v = 64 * 1024;
+    return (size_t)std::min(v, 160 * 1024);
+  }();
+  return lim;
+}
+
+struct LdsGeom {
+  int slots = 0, cap = 0;
+  size_t bytes = 0;
+  bool ok = false;
+};
+
+LdsGeom lds_geometry(int stride)
+{
+  LdsGeom g;
+  const size_t reserve = 16 + (size_t)kLdsBlock * 4;
+  const size_t lim = lds_limit();
+  if (lim <= reserve + 2 * (size_t)stride) return g;
+  int64_t slots = (int64_t)((lim - reserve) / (size_t)stride) - 1;
+  g.slots = (int)slots;
+  g.cap = (int)(slots * 0.62);
+  g.bytes = (size_t)(slots + 1) * stride + reserve;
+  g.ok = slots >= 256;
+  return g;
+}
+
+enum class Path { Auto, Global, Direct, Partitioned };
+
+Path forced_path()
+{
+  const char* e = std::getenv("MGB_GROUPBY_PATH");
+  if (!e) return Path::Auto;
+  std::string s(e);
+  if (s == "global") return Path::Global;
+  if (s == "direct") return Path::Direct;
+  if (s == "partitioned") return Path::Partitioned;
+  return Path::Auto;
+}
+
+int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_t stream, mgb_status& st)
+{
+  st = MGB_SUCCESS;
+  const int64_t kSample = 1 << 16;
+  const int64_t scap = 4 * kSample;
+  const int64_t samples = R <= scap / 2 ? R : kSample;  // small inputs are counted exactly
+  size_t need = align_up((size_t)(scap + 1) * 8, 256) + align_up((size_t)(scap + 1) * 4, 256) + 256;
+  hipError_t e = ws->main.reserve(need, stream);
+  if (e != hipSuccess) {
+    st = fail(MGB_ERROR_HIP, std::string("workspace: ") + hipGetErrorString(e));
+    return R;
+  }
+  SampleParams sp{};
+  sp.agg = base;
+  sp.hdr = static_cast<unsigned long long*>(ws->main.ptr);
+  sp.cnt = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws->main.ptr) + align_up((size_t)(scap + 1) * 8, 256));
+  sp.stats = reinterpret_cast<unsigned long long*>(reinterpret_cast<uint8_t*>(sp.cnt) +
+                                                   align_up((size_t)(scap + 1) * 4, 256));
+  sp.scap = scap;
+  sp.samples = samples;
+  const uint64_t empty = base.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
+  fill_u64_kernel<<<256, kBlock, 0, stream>>>(sp.hdr, scap, empty);
+  (void)hipMemsetAsync(sp.cnt, 0, (size_t)(scap + 1) * 4, stream);
+  (void)hipMemsetAsync(sp.stats, 0, 4 * sizeof(unsigned long long), stream);
+  sample_insert_kernel<<<(unsigned)((samples + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp);
+  sample_stats_kernel<<<(unsigned)((scap + 1 + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp.cnt, scap + 1, sp.stats);
+  e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ws->pinned, sp.stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    st = fail(MGB_ERROR_HIP, std::string("cardinality sample: ") + hipGetErrorString(e));
+    return R;
+  }
+  return estimate_groups(ws->pinned[0], ws->pinned[1], ws->pinned[2], ws->pinned[3], R);
+}
+
+// Global-table executions (global and direct paths): build the table, then
+// compact it into temporary outputs.
+mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int64_t R, int64_t G_est, bool direct,
+                     const LdsGeom& geom, hipStream_t stream, TempOuts& temp, int64_t& groups)
+{
+  const int cus = device_cu_count();
+  int64_t capacity = std::max<int64_t>(64, 2 * G_est + 1024);
+  capacity = std::min<int64_t>(capacity, std::max<int64_t>(2 * R + 64, 64));
   int64_t nblocks = 0;
-  size_t table_bytes = 0, block_off = 0, counters_off = 0;
+  size_t block_off = 0;
   for (int attempt = 0;; ++attempt) {
     if (capacity > (int64_t)0x7fffffff) return fail(MGB_ERROR_UNSUPPORTED, "groupby table would exceed 2^31 records");
     nblocks = (capacity + 1 + kBlock - 1) / kBlock;
-    table_bytes = align_up((size_t)(capacity + 1) * plan.stride, 256);
-    counters_off = table_bytes;
+    const size_t table_bytes = align_up((size_t)(capacity + 1) * plan.stride, 256);
+    const size_t counters_off = table_bytes;
     block_off = counters_off + 256;
-    size_t need = block_off + align_up((size_t)nblocks * 8, 256);
+    const size_t need = block_off + align_up((size_t)nblocks * 8, 256);
     MGB_HIP(ws->main.reserve(need, stream));
     uint8_t* wsb = static_cast<uint8_t*>(ws->main.ptr);
 
@@ -996,8 +1364,8 @@ mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mg
     fp.words = plan.stride / 8;
     fp.special_hdr = plan.key_mode == KEY_INT ? 0ull : kEmptyRowRef;
     record_template(plan, plan.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef, fp.tmpl);
-    int64_t fill_words = fp.num_records * fp.words;
-    unsigned fill_grid = (unsigned)std::min<int64_t>((fill_words + kBlock - 1) / kBlock, (int64_t)cus * 32);
+    const int64_t fill_words = fp.num_records * fp.words;
+    const unsigned fill_grid = (unsigned)std::min<int64_t>((fill_words + kBlock - 1) / kBlock, (int64_t)cus * 32);
     {
       ProfScope prof("fill", stream);
       fill_table_kernel<<<fill_grid, kBlock, 0, stream>>>(fp);
@@ -1010,39 +1378,73 @@ mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mg
     ap.capacity = capacity;
     ap.counters = counters;
     ap.max_groups = std::max<int64_t>(1, capacity * 3 / 4);
-    // launch in batches of value columns / ops
-    size_t ci = 0;
-    bool first = true;
-    do {
-      AggParams lp = ap;
-      lp.num_vals = 0;
-      lp.num_ops = 0;
-      lp.rows_op = -1;
-      if (first && plan.rows_off >= 0) {
-        lp.ops[0] = OpDesc{OP_ROWS, plan.rows_off};
-        lp.rows_op = 0;
-        lp.num_ops = 1;
-      }
-      while (ci < plan.cols.size() && lp.num_vals < kMaxVals &&
-             lp.num_ops + (int)plan.cols[ci].ops.size() <= kMaxOps) {
-        const PlanCol& pc = plan.cols[ci];
-        ValCol vc = pc.col;
-        vc.op_begin = lp.num_ops;
-        for (auto& o : pc.ops) lp.ops[lp.num_ops++] = OpDesc{o.first, o.second};
-        vc.op_end = lp.num_ops;
-        lp.vals[lp.num_vals++] = vc;
-        ++ci;
-      }
-      if (lp.num_vals == 0 && ci < plan.cols.size())
-        return fail(MGB_ERROR_UNSUPPORTED, "a value column needs more accumulator ops than one launch holds");
-      int64_t grid = std::min<int64_t>((R + kBlock - 1) / kBlock, (int64_t)cus * 16);
+    if (direct) {
+      LdsParams lp{};
+      lp.src = ap;
+      if (!single_launch_ops(plan, lp.src)) return fail(MGB_ERROR_UNSUPPORTED, "plan does not fit the LDS path");
+      lp.src.flags = nullptr;
+      for (int c = 0; c < lp.src.num_vals; ++c) lp.src.vals[c].flag_bit = -1;
+      const int64_t items = std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 2, (R + 16383) / 16384));
+      lp.rows_per_item = (R + items - 1) / items;
+      uint64_t tmpl[kMaxRecordWords];
+      record_template(plan, plan.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef, tmpl);
+      OutTable dummy{};
+      const OutTable* d_ot = nullptr;
+      const uint64_t* d_tmpl = nullptr;
+      mgb_status st = upload_params(ws, dummy, tmpl, plan.stride / 8, stream, &d_ot, &d_tmpl);
+      if (st != MGB_SUCCESS) return st;
+      lp.tmpl = d_tmpl;
+      lp.out = d_ot;
+      lp.slots = geom.slots;
+      lp.cap = geom.cap;
+      lp.words = plan.stride / 8;
+      lp.direct = 1;
+      lp.log2p = 0;
+      MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)geom.bytes));
       {
         ProfScope prof("aggregate", stream);
-        aggregate_kernel<<<(unsigned)std::max<int64_t>(grid, 1), kBlock, 0, stream>>>(lp);
+        lds_agg_kernel<<<(unsigned)items, kLdsBlock, geom.bytes, stream>>>(lp);
       }
       MGB_HIP(hipGetLastError());
-      first = false;
-    } while (ci < plan.cols.size());
+    } else {
+      // launch in batches of value columns / ops
+      size_t ci = 0;
+      bool first = true;
+      do {
+        AggParams lp = ap;
+        lp.num_vals = 0;
+        lp.num_ops = 0;
+        lp.rows_op = -1;
+        lp.flags = nullptr;
+        if (first && plan.rows_off >= 0) {
+          lp.ops[0] = OpDesc{OP_ROWS, plan.rows_off};
+          lp.rows_op = 0;
+          lp.num_ops = 1;
+        }
+        while (ci < plan.cols.size() && lp.num_vals < kMaxVals &&
+               lp.num_ops + (int)plan.cols[ci].ops.size() <= kMaxOps) {
+          const PlanCol& pc = plan.cols[ci];
+          ValCol vc = pc.col;
+          vc.flag_bit = -1;
+          vc.nulls_off = pc.nulls_off;
+          vc.op_begin = lp.num_ops;
+          for (auto& o : pc.ops) lp.ops[lp.num_ops++] = OpDesc{o.first, o.second};
+          vc.op_end = lp.num_ops;
+          lp.vals[lp.num_vals++] = vc;
+          ++ci;
+        }
+        if (lp.num_vals == 0 && ci < plan.cols.size())
+          return fail(MGB_ERROR_UNSUPPORTED, "a value column needs more accumulator ops than one launch holds");
+        const int64_t grid = std::min<int64_t>((R + kBlock - 1) / kBlock, (int64_t)cus * 16);
+        {
+          ProfScope prof("aggregate", stream);
+          aggregate_kernel<<<(unsigned)std::max<int64_t>(grid, 1), kBlock, 0, stream>>>(lp);
+        }
+        MGB_HIP(hipGetLastError());
+        first = false;
+      } while (ci < plan.cols.size());
+    }
 
     int64_t* block_counts = reinterpret_cast<int64_t*>(wsb + block_off);
     count_occupied_kernel<<<(unsigned)nblocks, kBlock, 0, stream>>>(wsb, capacity, plan.stride, plan.key_mode,
@@ -1058,102 +1460,248 @@ mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mg
     if (attempt >= 6) return fail(MGB_ERROR_HIP, "groupby table kept overflowing");
     capacity = std::min<int64_t>(capacity * 4, std::max<int64_t>(2 * R + 64, 64));
   }
-  *out_num_groups = groups;
 
-  // ---- outputs
-  Allocs al{alloc, stream, {}};
-  const int nk = (int)plan.keys.size();
-  const int no = (int)plan.outs.size();
-  if (no + nk > kMaxPinned) return fail(MGB_ERROR_UNSUPPORTED, "too many output columns in one groupby call");
-  size_t vbytes = align_up((size_t)std::max<int64_t>(groups, 1), 256);
-  int n_valid = no + nk;
-  size_t nc_bytes = align_up((size_t)n_valid * 8, 256);
-  MGB_HIP(ws->outs.reserve(vbytes * (size_t)n_valid + nc_bytes, stream));
-  uint8_t* vb = static_cast<uint8_t*>(ws->outs.ptr);
-  unsigned long long* null_counts = reinterpret_cast<unsigned long long*>(vb + vbytes * (size_t)n_valid);
-  MGB_HIP(hipMemsetAsync(null_counts, 0, nc_bytes, stream));
+  OutTable ot{};
+  mgb_status st = layout_temp(ws, plan, groups, stream, temp, ot);
+  if (st != MGB_SUCCESS) return st;
+  if (groups == 0) return MGB_SUCCESS;
+  uint64_t tmpl[kMaxRecordWords] = {};
+  const OutTable* d_ot = nullptr;
+  const uint64_t* d_tmpl = nullptr;
+  st = upload_params(ws, ot, tmpl, 1, stream, &d_ot, &d_tmpl);
+  if (st != MGB_SUCCESS) return st;
+  FinParams f{};
+  f.table = static_cast<const uint8_t*>(ws->main.ptr);
+  f.capacity = capacity;
+  f.stride = plan.stride;
+  f.key_mode = plan.key_mode;
+  f.num_keys = (int)plan.keys.size();
+  f.block_offsets = reinterpret_cast<const int64_t*>(static_cast<uint8_t*>(ws->main.ptr) + block_off);
+  f.out = d_ot;
+  for (size_t c = 0; c < plan.keys.size(); ++c) f.keys[c] = plan.keys[c];
+  {
+    ProfScope prof("finalize", stream);
+    finalize_kernel<<<(unsigned)nblocks, kBlock, 0, stream>>>(f);
+  }
+  MGB_HIP(hipGetLastError());
+  return MGB_SUCCESS;
+}
 
-  FinParams fin{};
-  fin.table = static_cast<const uint8_t*>(ws->main.ptr);
-  fin.capacity = capacity;
-  fin.stride = plan.stride;
-  fin.key_mode = plan.key_mode;
-  fin.num_keys = nk;
-  fin.block_offsets = reinterpret_cast<const int64_t*>(static_cast<uint8_t*>(ws->main.ptr) + block_off);
-  for (int c = 0; c < nk; ++c) {
-    const KeyCol& k = plan.keys[c];
-    fin.keys[c] = k;
-    mgb_out_column& oc = out_keys[c];
-    std::memset(&oc, 0, sizeof(oc));
-    oc.type_id = k.type_id;
-    oc.size = groups;
-    if (groups > 0) {
-      oc.data_bytes = (size_t)groups * k.width;
-      oc.data = al.get(oc.data_bytes);
-      if (!oc.data) { al.release_all(); return fail(MGB_ERROR_ALLOC, "allocation of a key column failed"); }
-      if (plan.compare_key_nulls && k.mask) {
-        oc.mask_bytes = (size_t)bitmask_bytes(groups);
-        oc.null_mask = static_cast<uint32_t*>(al.get(oc.mask_bytes));
-        if (!oc.null_mask) { al.release_all(); return fail(MGB_ERROR_ALLOC, "allocation of a key mask failed"); }
-        fin.key_valid[c] = vb + vbytes * (size_t)(no + c);
-      }
+// Partitioned execution: hash-partition the rows, then aggregate every
+// partition in LDS. Returns MGB_SUCCESS with `done` false when the estimate
+// asks for more partitions than supported (the caller then uses the global
+// path).
+mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& base, int64_t R, int64_t G_est,
+                           const LdsGeom& geom, hipStream_t stream, TempOuts& temp, int64_t& groups, bool& done)
+{
+  done = false;
+  const int cus = device_cu_count();
+  AggParams src = base;
+  if (!single_launch_ops(plan, src)) return MGB_SUCCESS;
+  int nullable = 0;
+  for (int c = 0; c < src.num_vals; ++c) nullable += src.vals[c].flag_bit >= 0;
+  const int flag_bytes = nullable == 0 ? 0 : nullable <= 8 ? 1 : nullable <= 16 ? 2 : 4;
+  const double m_target = std::max(geom.cap * 0.5, geom.cap - 4.5 * std::sqrt((double)geom.cap));
+  int64_t est = std::max<int64_t>(G_est, 1);
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    int log2p = 1;
+    while ((double)(1ll << log2p) * m_target < (double)est) ++log2p;
+    if (log2p > 15) return MGB_SUCCESS;
+    const int64_t P = 1ll << log2p;
+    int64_t nw = std::min<int64_t>((int64_t)cus * 4, std::max<int64_t>(64, (1ll << 22) / P));
+    nw = std::max<int64_t>(1, std::min<int64_t>(nw, (R + 4095) / 4096));
+    const int64_t chunk = (R + nw - 1) / nw;
+    const int64_t nh = P * nw;
+    const int64_t nb = (nh + kScanTile - 1) / kScanTile;
+
+    // record layout of a partitioned row: 8-byte fields, then 4, 2 and 1-byte
+    // fields, then the validity bits; the size is a multiple of 8
+    std::vector<int> koff(plan.keys.size()), voff(src.num_vals);
+    int rec = 0;
+    for (int w : {8, 4, 2, 1}) {
+      for (size_t c = 0; c < plan.keys.size(); ++c)
+        if (plan.keys[c].width == w) { koff[c] = rec; rec += w; }
+      for (int c = 0; c < src.num_vals; ++c)
+        if (src.vals[c].width == w) { voff[c] = rec; rec += w; }
     }
-    fin.key_out[c] = oc.data;
-  }
-  for (int o = 0; o < no; ++o) {
-    const Plan::Out& po = plan.outs[o];
-    mgb_out_column& oc = out_results[o];
-    std::memset(&oc, 0, sizeof(oc));
-    oc.type_id = po.out_type;
-    oc.size = groups;
-    bool nullable = po.spec.valid != nullptr;
-    if (groups > 0) {
-      oc.data_bytes = (size_t)groups * type_width(po.out_type);
-      oc.data = al.get(oc.data_bytes);
-      if (!oc.data) { al.release_all(); return fail(MGB_ERROR_ALLOC, "allocation of a result column failed"); }
-      if (nullable) {
-        oc.mask_bytes = (size_t)bitmask_bytes(groups);
-        oc.null_mask = static_cast<uint32_t*>(al.get(oc.mask_bytes));
-        if (!oc.null_mask) { al.release_all(); return fail(MGB_ERROR_ALLOC, "allocation of a result mask failed"); }
-      }
+    const int flags_rec_off = (int)align_up((size_t)rec, (size_t)std::max(flag_bytes, 1));
+    rec = flags_rec_off + flag_bytes;
+    const int rec_bytes = (int)align_up((size_t)rec, 8);
+
+    // workspace: partitioned records, hist, offsets, block sums, counters
+    size_t off = 0;
+    const size_t recs_off = off;
+    off += align_up((size_t)R * rec_bytes, 256);
+    const size_t hist_off = off;
+    off += align_up((size_t)nh * 4, 256);
+    const size_t offs_off = off;
+    off += align_up((size_t)nh * 4, 256);
+    const size_t bsum_off = off;
+    off += align_up((size_t)nb * 4, 256);
+    const size_t ctr_off = off;
+    off += 256;
+    MGB_HIP(ws->main.reserve(off, stream));
+    uint8_t* wsb = static_cast<uint8_t*>(ws->main.ptr);
+    unsigned long long* ctr = reinterpret_cast<unsigned long long*>(wsb + ctr_off);  // [0] total rows [1] out [2] ovf
+    MGB_HIP(hipMemsetAsync(ctr, 0, 256, stream));
+
+    // every group has a row, and an item holds at most slots + 1 groups (the
+    // cap check races with concurrent inserts, the slot count does not)
+    const int64_t U = std::min<int64_t>(R, P * (int64_t)(geom.slots + 1));
+    OutTable ot{};
+    mgb_status st = layout_temp(ws, plan, U, stream, temp, ot);
+    if (st != MGB_SUCCESS) return st;
+    uint64_t tmpl[kMaxRecordWords];
+    record_template(plan, plan.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef, tmpl);
+    const OutTable* d_ot = nullptr;
+    const uint64_t* d_tmpl = nullptr;
+    st = upload_params(ws, ot, tmpl, plan.stride / 8, stream, &d_ot, &d_tmpl);
+    if (st != MGB_SUCCESS) return st;
+
+    PartParams pp{};
+    pp.src = src;
+    pp.chunk = chunk;
+    pp.log2p = log2p;
+    pp.nblocks = (int32_t)nw;
+    pp.hist = reinterpret_cast<uint32_t*>(wsb + hist_off);
+    pp.offsets = reinterpret_cast<const uint32_t*>(wsb + offs_off);
+    pp.out = wsb + recs_off;
+    pp.rec_bytes = rec_bytes;
+    pp.flag_bytes = flag_bytes;
+    pp.flags_off = flags_rec_off;
+    for (size_t c = 0; c < plan.keys.size(); ++c) pp.key_off[c] = koff[c];
+    for (int c = 0; c < src.num_vals; ++c) pp.val_off[c] = voff[c];
+    const size_t part_lds = (size_t)P * 4;
+    MGB_HIP(hipFuncSetAttribute((const void*)part_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)part_lds));
+    MGB_HIP(hipFuncSetAttribute((const void*)part_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)part_lds));
+    {
+      ProfScope prof("partition_hist", stream);
+      part_hist_kernel<<<(unsigned)nw, kBlock, part_lds, stream>>>(pp);
     }
-  }
-  if (groups > 0) {
-    for (int base_o = 0; base_o < no || base_o == 0; base_o += kMaxOuts) {
-      FinParams f = fin;
-      f.num_outs = std::min(kMaxOuts, no - base_o);
-      for (int o = 0; o < f.num_outs; ++o) {
-        const Plan::Out& po = plan.outs[base_o + o];
-        f.outs[o] = po.spec;
-        f.outs[o].out = out_results[base_o + o].data;
-        f.outs[o].valid = po.spec.valid ? vb + vbytes * (size_t)(base_o + o) : nullptr;
-      }
-      if (base_o > 0) {
-        // keys were written by the first launch; later launches only write results
-        for (int c = 0; c < nk; ++c) f.key_valid[c] = nullptr;
-      }
-      {
-        ProfScope prof("finalize", stream);
-        finalize_kernel<<<(unsigned)nblocks, kBlock, 0, stream>>>(f);
-      }
-      MGB_HIP(hipGetLastError());
-      if (no <= kMaxOuts) break;
-    }
-    for (int j = 0; j < no + nk; ++j) {
-      uint32_t* mask = j < no ? out_results[j].null_mask : out_keys[j - no].null_mask;
-      if (!mask) continue;
-      int64_t words = bitmask_bytes(groups) / 4;
-      pack_validity_kernel<<<(unsigned)((words + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
-        vb + vbytes * (size_t)j, groups, mask, words, null_counts + j);
+    uint32_t* bsum = reinterpret_cast<uint32_t*>(wsb + bsum_off);
+    scan_reduce_u32_kernel<<<(unsigned)nb, 256, 0, stream>>>(pp.hist, nh, bsum);
+    scan_bsums_u32_kernel<<<1, 1024, 0, stream>>>(bsum, nb, ctr + 0);
+    scan_apply_u32_kernel<<<(unsigned)nb, 256, 0, stream>>>(pp.hist, nh, bsum, reinterpret_cast<uint32_t*>(wsb + offs_off));
+    // the largest tile (a multiple of the workgroup size) whose records fit LDS
+    const size_t fixed_lds = (size_t)P * 8 + 16 * 4 + (size_t)kScatterTile * 2 + 16;
+    int tile = 0;
+    for (int t = kScatterTile; t >= kLdsBlock; t -= kLdsBlock)
+      if (fixed_lds + (size_t)t * rec_bytes <= lds_limit()) { tile = t; break; }
+    pp.tile_rows = tile;
+    const size_t tiled_lds = fixed_lds + (size_t)tile * rec_bytes;
+    if (tile > 0) {
+      MGB_HIP(hipFuncSetAttribute((const void*)part_scatter_tiled_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)tiled_lds));
+      ProfScope prof("partition_scatter", stream);
+      part_scatter_tiled_kernel<<<(unsigned)nw, kLdsBlock, tiled_lds, stream>>>(pp);
+    } else {
+      ProfScope prof("partition_scatter", stream);
+      part_scatter_kernel<<<(unsigned)nw, kBlock, part_lds, stream>>>(pp);
     }
     MGB_HIP(hipGetLastError());
-    MGB_HIP(hipMemcpyAsync(ws->pinned, null_counts, (size_t)(no + nk) * 8, hipMemcpyDeviceToHost, stream));
+
+    LdsParams lp{};
+    lp.src = src;
+    lp.src.num_rows = R;
+    lp.src.skip_null_keys = 0;
+    uint8_t* recs = wsb + recs_off;
+    lp.src.flags = flag_bytes ? recs + flags_rec_off : nullptr;
+    lp.src.flag_bytes = flag_bytes;
+    lp.src.flag_stride = rec_bytes;
+    for (size_t c = 0; c < plan.keys.size(); ++c) {
+      lp.src.keys[c].data = recs + koff[c];
+      lp.src.keys[c].stride = rec_bytes;
+      lp.src.keys[c].mask = nullptr;
+      lp.src.keys[c].bit_off = 0;
+    }
+    for (int c = 0; c < src.num_vals; ++c) {
+      lp.src.vals[c].data = recs + voff[c];
+      lp.src.vals[c].stride = rec_bytes;
+      lp.src.vals[c].mask = nullptr;
+      lp.src.vals[c].bit_off = 0;
+    }
+    lp.offsets = pp.offsets;
+    lp.total_rows = ctr + 0;
+    lp.tmpl = d_tmpl;
+    lp.out = d_ot;
+    lp.out_counters = ctr + 1;
+    lp.nblocks = (int32_t)nw;
+    lp.log2p = log2p;
+    lp.slots = geom.slots;
+    lp.cap = geom.cap;
+    lp.words = plan.stride / 8;
+    lp.direct = 0;
+    MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)geom.bytes));
+    {
+      ProfScope prof("aggregate", stream);
+      lds_agg_kernel<<<(unsigned)P, kLdsBlock, geom.bytes, stream>>>(lp);
+    }
+    MGB_HIP(hipGetLastError());
+    MGB_HIP(hipMemcpyAsync(ws->pinned, ctr, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     MGB_HIP(hipStreamSynchronize(stream));
-    for (int o = 0; o < no; ++o) out_results[o].null_count = (int64_t)ws->pinned[o];
-    for (int c = 0; c < nk; ++c) out_keys[c].null_count = (int64_t)ws->pinned[no + c];
+    if (ws->pinned[2] == 0) {
+      groups = (int64_t)ws->pinned[1];
+      temp.rows = groups;
+      done = true;
+      return MGB_SUCCESS;
+    }
+    est *= 4;  // a partition held more groups than LDS: repartition finer
   }
-  al.live.clear();  // ownership passes to the caller
   return MGB_SUCCESS;
+}
+
+}  // namespace
+
+mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mgb_out_column* out_results,
+                       int64_t* out_num_groups, hipStream_t stream, const mgb_allocator* alloc)
+{
+  Workspace* ws = workspace_for(stream);
+  std::lock_guard<std::mutex> lk(ws->mu);
+  if (!ws->pinned) {
+    MGB_HIP(hipHostMalloc((void**)&ws->pinned, kMaxPinned * sizeof(unsigned long long), hipHostMallocDefault));
+    MGB_HIP(hipHostMalloc((void**)&ws->staging, kStagingBytes, hipHostMallocDefault));
+  }
+
+  ws->upload_slot = 0;
+
+  AggParams base{};
+  base.num_rows = R;
+  base.key_mode = plan.key_mode;
+  base.num_keys = (int)plan.keys.size();
+  base.skip_null_keys = plan.skip_null_keys;
+  base.compare_key_nulls = plan.compare_key_nulls;
+  base.stride = plan.stride;
+  for (size_t c = 0; c < plan.keys.size(); ++c) base.keys[c] = plan.keys[c];
+
+  mgb_status st = MGB_SUCCESS;
+  const int64_t G_est = sample_estimate(ws, base, R, stream, st);
+  if (st != MGB_SUCCESS) return st;
+
+  const LdsGeom geom = lds_geometry(plan.stride);
+  AggParams probe = base;
+  const bool lds_ok = geom.ok && single_launch_ops(plan, probe) && !plan.compare_key_nulls &&
+                      (int)plan.outs.size() <= kMaxOuts;
+  Path path = forced_path();
+  if (!lds_ok) path = Path::Global;
+  if (path == Path::Auto) path = (G_est <= geom.cap / 2) ? Path::Direct : Path::Partitioned;
+
+  TempOuts temp;
+  int64_t groups = 0;
+  bool done = false;
+  if (path == Path::Partitioned) {
+    st = run_partitioned(ws, plan, base, R, G_est, geom, stream, temp, groups, done);
+    if (st != MGB_SUCCESS) return st;
+    if (!done) path = Path::Global;
+  }
+  if (!done) {
+    st = run_table(ws, plan, base, R, G_est, path == Path::Direct, geom, stream, temp, groups);
+    if (st != MGB_SUCCESS) return st;
+  }
+  *out_num_groups = groups;
+  return emit_outputs(ws, plan, temp, groups, out_keys, out_results, stream, alloc);
 }
 
 mgb_status make_hash_params(const mgb_column* cols, int32_t n, HashParams& hp, int64_t& rows)
@@ -1172,6 +1720,7 @@ mgb_status make_hash_params(const mgb_column* cols, int32_t n, HashParams& hp, i
     k.bit_off = cols[c].offset;
     k.type_id = cols[c].type_id;
     k.width = w;
+    k.stride = w;
     k.vclass = value_class(cols[c].type_id);
     hp.check_nulls |= k.mask != nullptr;
   }

@@ -1,0 +1,380 @@
+// Device routines shared by the groupby kernels: row access, row hashing and
+// equality, the global record table, the LDS record table, and finalisation
+// of a record into output columns.
+#pragma once
+
+#include "groupby_kernels.hpp"
+
+namespace mgb {
+
+// ------------------------------------------------------------- row access
+
+__device__ __forceinline__ uint32_t load_flags(const uint8_t* flags, int flag_bytes, int flag_stride, int64_t row)
+{
+  return (uint32_t)load_at(flags, flag_bytes, flag_stride, row);
+}
+
+__device__ __forceinline__ bool value_valid(const AggParams& p, const ValCol& v, int64_t row, uint32_t flags)
+{
+  if (v.mask) return bit_valid(v.mask, v.bit_off + row);
+  if (v.flag_bit >= 0) return (flags >> v.flag_bit) & 1u;
+  return true;
+}
+
+__device__ __forceinline__ bool key_row_valid(const AggParams& p, int64_t row)
+{
+  if (!p.skip_null_keys) return true;
+  for (int c = 0; c < p.num_keys; ++c)
+    if (!bit_valid(p.keys[c].mask, p.keys[c].bit_off + row)) return false;
+  return true;
+}
+
+__device__ __forceinline__ bool rows_equal(const KeyCol* keys, int num_keys, bool compare_nulls, int64_t a, int64_t b)
+{
+  for (int c = 0; c < num_keys; ++c) {
+    const KeyCol& k = keys[c];
+    if (compare_nulls) {
+      bool va = bit_valid(k.mask, k.bit_off + a), vb = bit_valid(k.mask, k.bit_off + b);
+      if (va != vb) return false;
+      if (!va) continue;
+    }
+    uint64_t x = load_at(k.data, k.width, k.stride, a), y = load_at(k.data, k.width, k.stride, b);
+    if (k.vclass == VC_F32) {
+      float fx = __uint_as_float((uint32_t)x), fy = __uint_as_float((uint32_t)y);
+      if (!((isnan(fx) && isnan(fy)) || fx == fy)) return false;
+    } else if (k.vclass == VC_F64) {
+      double fx = __longlong_as_double((long long)x), fy = __longlong_as_double((long long)y);
+      if (!((isnan(fx) && isnan(fy)) || fx == fy)) return false;
+    } else if (k.vclass == VC_BOOL) {
+      if ((x != 0) != (y != 0)) return false;
+    } else if (x != y) {
+      return false;
+    }
+  }
+  return true;
+}
+
+// cudf's row hash: hash_combine fold of per-column MurmurHash3_x86_32 from
+// seed 0, nulls hashing to 0xFFFFFFFF (row_operators.cuh:1799-1883).
+__device__ __forceinline__ uint32_t row_hash(const KeyCol* keys, int num_keys, int64_t row, bool check_nulls)
+{
+  uint32_t h = 0;
+  for (int c = 0; c < num_keys; ++c) {
+    const KeyCol& k = keys[c];
+    uint32_t e;
+    if (check_nulls && !bit_valid(k.mask, k.bit_off + row)) {
+      e = 0xffffffffu;
+    } else {
+      e = murmur3_bits(normalize_float_bits(load_at(k.data, k.width, k.stride, row), k.vclass), k.width, 0);
+    }
+    h = hash_combine(h, e);
+  }
+  return h;
+}
+
+// The group identity of a row: the 64-bit header value stored in its record
+// and its row hash. For KEY_INT the header is the sign-extended key.
+struct RowKey {
+  uint64_t key;
+  uint32_t hash;
+  bool special;   // the 8-byte key equal to the empty header (kEmptyIntKey)
+};
+
+__device__ __forceinline__ RowKey row_key(const AggParams& p, int64_t row)
+{
+  RowKey r;
+  if (p.key_mode == KEY_INT) {
+    const KeyCol& k = p.keys[0];
+    uint64_t raw = load_at(k.data, k.width, k.stride, row);
+    r.key = (uint64_t)widen_int(raw, k.width, k.vclass);
+    r.hash = hash_combine(0u, murmur3_bits(normalize_float_bits(raw, k.vclass), k.width, 0));
+    r.special = r.key == kEmptyIntKey;
+  } else {
+    r.hash = row_hash(p.keys, p.num_keys, row, p.compare_key_nulls != 0);
+    r.key = ((uint64_t)r.hash << 32) | (uint64_t)row;
+    r.special = false;
+  }
+  return r;
+}
+
+__device__ __forceinline__ int64_t reduce_range(uint32_t h, int64_t n) { return (int64_t)(((uint64_t)h * (uint64_t)n) >> 32); }
+
+// ------------------------------------------------------------- global table
+
+__device__ __forceinline__ void flag_overflow(unsigned long long* counters) { atom_max(&counters[1], 1ull); }
+
+// Record of the group of (rk, row) in the global table, inserting it if
+// needed; nullptr when the table is over its group budget (overflow flag set).
+__device__ __forceinline__ uint8_t* global_find_or_insert(const AggParams& p, const RowKey& rk, int64_t row)
+{
+  if (rk.special) {
+    uint8_t* rec = p.table + p.capacity * (int64_t)p.stride;
+    unsigned long long* hp = reinterpret_cast<unsigned long long*>(rec);
+    if (*hp == 0ull && atom_cas(hp, 0ull, 1ull) == 0ull) atom_add(&p.counters[0], 1ull);
+    return rec;
+  }
+  const uint64_t empty = p.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
+  int64_t idx = reduce_range(rk.hash, p.capacity);
+  for (int64_t probe = 0; probe < p.capacity; ++probe) {
+    uint8_t* rec = p.table + idx * (int64_t)p.stride;
+    unsigned long long* hp = reinterpret_cast<unsigned long long*>(rec);
+    uint64_t cur = *hp;  // a stale read can only be EMPTY; the CAS below resolves it
+    for (;;) {
+      if (cur == empty) {
+        if (__hip_atomic_load(&p.counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+            (unsigned long long)p.max_groups) {
+          flag_overflow(p.counters);
+          return nullptr;
+        }
+        uint64_t seen = atom_cas(hp, empty, rk.key);
+        if (seen == empty) {
+          atom_add(&p.counters[0], 1ull);
+          return rec;
+        }
+        cur = seen;
+        continue;  // re-examine the value that won the slot
+      }
+      if (p.key_mode == KEY_INT) {
+        if (cur == rk.key) return rec;
+      } else if ((uint32_t)(cur >> 32) == rk.hash &&
+                 rows_equal(p.keys, p.num_keys, p.compare_key_nulls != 0, (int64_t)(cur & 0xffffffffull), row)) {
+        return rec;
+      }
+      break;
+    }
+    idx = (idx + 1 == p.capacity) ? 0 : idx + 1;
+  }
+  flag_overflow(p.counters);
+  return nullptr;
+}
+
+// All ops of one row on a record (global or LDS).
+__device__ __forceinline__ void apply_row(const AggParams& p, uint8_t* rec, int64_t row)
+{
+  uint32_t flags = p.flags ? load_flags(p.flags, p.flag_bytes, p.flag_stride, row) : 0u;
+  if (p.rows_op >= 0) apply_op(rec, OP_ROWS, p.ops[p.rows_op].off, 0, 4, VC_UNSIGNED);
+  for (int c = 0; c < p.num_vals; ++c) {
+    const ValCol& v = p.vals[c];
+    if (!value_valid(p, v, row, flags)) {
+      if (v.nulls_off >= 0) atom_add(reinterpret_cast<uint32_t*>(rec + v.nulls_off), 1u);
+      continue;
+    }
+    uint64_t raw = load_at(v.data, v.width, v.stride, row);
+    for (int o = v.op_begin; o < v.op_end; ++o) apply_op(rec, p.ops[o].kind, p.ops[o].off, raw, v.width, v.vclass);
+  }
+}
+
+// Folds one partial record (`src`) into `dst` with device atomics.
+__device__ __forceinline__ void merge_record(const AggParams& p, uint8_t* dst, const uint8_t* src)
+{
+  for (int o = 0; o < p.num_ops; ++o) {
+    const int kind = p.ops[o].kind, off = p.ops[o].off;
+    const uint8_t* s = src + off;
+    uint8_t* d = dst + off;
+    switch (kind) {
+      case OP_ROWS: case OP_COUNT: case OP_NULLS: {
+        uint32_t v = *reinterpret_cast<const uint32_t*>(s);
+        if (v) atom_add(reinterpret_cast<uint32_t*>(d), v);
+        break;
+      }
+      case OP_SUM_I64: case OP_SUMSQ_I64:
+        atom_add(reinterpret_cast<unsigned long long*>(d), *reinterpret_cast<const unsigned long long*>(s));
+        break;
+      case OP_SUM_F64: case OP_SUMSQ_F64:
+        atom_add(reinterpret_cast<double*>(d), *reinterpret_cast<const double*>(s));
+        break;
+      case OP_PROD_I64: case OP_PROD_F64: {
+        unsigned long long* dp = reinterpret_cast<unsigned long long*>(d);
+        unsigned long long x = *reinterpret_cast<const unsigned long long*>(s);
+        if (x == op_identity(kind)) break;
+        unsigned long long old = __hip_atomic_load(dp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (;;) {
+          unsigned long long nxt;
+          if (kind == OP_PROD_I64) {
+            nxt = old * x;
+          } else {
+            nxt = (unsigned long long)__double_as_longlong(__longlong_as_double((long long)old) *
+                                                          __longlong_as_double((long long)x));
+          }
+          unsigned long long seen = atom_cas(dp, old, nxt);
+          if (seen == old) break;
+          old = seen;
+        }
+        break;
+      }
+      case OP_MIN_I32: atom_min(reinterpret_cast<int32_t*>(d), *reinterpret_cast<const int32_t*>(s)); break;
+      case OP_MAX_I32: atom_max(reinterpret_cast<int32_t*>(d), *reinterpret_cast<const int32_t*>(s)); break;
+      case OP_MIN_U32: case OP_MIN_F32:
+        atom_min(reinterpret_cast<uint32_t*>(d), *reinterpret_cast<const uint32_t*>(s));
+        break;
+      case OP_MAX_U32: case OP_MAX_F32:
+        atom_max(reinterpret_cast<uint32_t*>(d), *reinterpret_cast<const uint32_t*>(s));
+        break;
+      case OP_MIN_I64: atom_min(reinterpret_cast<long long*>(d), *reinterpret_cast<const long long*>(s)); break;
+      case OP_MAX_I64: atom_max(reinterpret_cast<long long*>(d), *reinterpret_cast<const long long*>(s)); break;
+      case OP_MIN_U64: case OP_MIN_F64:
+        atom_min(reinterpret_cast<unsigned long long*>(d), *reinterpret_cast<const unsigned long long*>(s));
+        break;
+      case OP_MAX_U64: case OP_MAX_F64:
+        atom_max(reinterpret_cast<unsigned long long*>(d), *reinterpret_cast<const unsigned long long*>(s));
+        break;
+      default: break;
+    }
+  }
+}
+
+// ------------------------------------------------------------- LDS table
+
+// LDS layout of a work item: `slots` records, then one record for the 8-byte
+// key equal to the empty header, then two counters.
+struct LdsTable {
+  uint8_t* rec;
+  int32_t slots;
+  int32_t stride;
+  int32_t cap;
+  int32_t* count;       // groups inserted
+  int32_t* overflow;    // set when a new group did not fit
+};
+
+__device__ __forceinline__ uint8_t* lds_record(const LdsTable& t, int64_t i) { return t.rec + i * (int64_t)t.stride; }
+
+// Record of the row's group in the LDS table, inserting it if needed; nullptr
+// when the table is full (overflow set). `slot_hash` must be independent of
+// the bits that chose the partition.
+__device__ __forceinline__ uint8_t* lds_find_or_insert(const LdsTable& t, const AggParams& p, const RowKey& rk,
+                                                       uint32_t slot_hash, int64_t row)
+{
+  if (rk.special) {
+    uint8_t* rec = lds_record(t, t.slots);
+    unsigned long long* hp = reinterpret_cast<unsigned long long*>(rec);
+    unsigned long long expect = 0ull;
+    if (__hip_atomic_compare_exchange_strong(hp, &expect, 1ull, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP))
+      atomicAdd(t.count, 1);
+    return rec;
+  }
+  const uint64_t empty = p.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
+  int32_t idx = (int32_t)reduce_range(slot_hash, t.slots);
+  for (int32_t probe = 0; probe < t.slots; ++probe) {
+    uint8_t* rec = lds_record(t, idx);
+    unsigned long long* hp = reinterpret_cast<unsigned long long*>(rec);
+    uint64_t cur = __hip_atomic_load(hp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (;;) {
+      if (cur == empty) {
+        if (__hip_atomic_load(t.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= t.cap) {
+          *t.overflow = 1;
+          return nullptr;
+        }
+        unsigned long long expect = empty;
+        if (__hip_atomic_compare_exchange_strong(hp, &expect, (unsigned long long)rk.key, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+          atomicAdd(t.count, 1);
+          return rec;
+        }
+        cur = expect;
+        continue;
+      }
+      if (p.key_mode == KEY_INT) {
+        if (cur == rk.key) return rec;
+      } else if ((uint32_t)(cur >> 32) == rk.hash &&
+                 rows_equal(p.keys, p.num_keys, p.compare_key_nulls != 0, (int64_t)(cur & 0xffffffffull), row)) {
+        return rec;
+      }
+      break;
+    }
+    idx = (idx + 1 == t.slots) ? 0 : idx + 1;
+  }
+  *t.overflow = 1;
+  return nullptr;
+}
+
+__device__ __forceinline__ bool lds_occupied(const LdsTable& t, int32_t i, int key_mode)
+{
+  uint64_t hdr = *reinterpret_cast<const uint64_t*>(lds_record(t, i));
+  if (i == t.slots) return key_mode == KEY_INT && hdr == 1ull;
+  return hdr != (key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef);
+}
+
+// ------------------------------------------------------------- finalisation
+
+__device__ __forceinline__ void store_width(void* out, int64_t pos, int width, uint64_t bits)
+{
+  switch (width) {
+    case 1: reinterpret_cast<uint8_t*>(out)[pos] = (uint8_t)bits; break;
+    case 2: reinterpret_cast<uint16_t*>(out)[pos] = (uint16_t)bits; break;
+    case 4: reinterpret_cast<uint32_t*>(out)[pos] = (uint32_t)bits; break;
+    default: reinterpret_cast<uint64_t*>(out)[pos] = bits; break;
+  }
+}
+
+// Writes the requested results of one group's record at output row `pos`.
+__device__ __forceinline__ void finalize_record(const uint8_t* rec, int64_t pos, const OutTable& ot)
+{
+  for (int o = 0; o < ot.num_outs; ++o) {
+    const OutSpec& s = ot.outs[o];
+    uint32_t cnt = *reinterpret_cast<const uint32_t*>(rec + s.count_off);
+    if (s.nulls_off >= 0) cnt -= *reinterpret_cast<const uint32_t*>(rec + s.nulls_off);
+    const uint8_t* f = rec + s.field_off;
+    uint64_t bits = 0;
+    switch (s.fin) {
+      case FIN_COUNT: bits = cnt; break;
+      case FIN_INT:
+        bits = (s.out_width == 8) ? *reinterpret_cast<const uint64_t*>(f) : *reinterpret_cast<const uint32_t*>(f);
+        break;
+      case FIN_F64: bits = *reinterpret_cast<const uint64_t*>(f); break;
+      case FIN_F64_TO_F32: bits = __float_as_uint((float)*reinterpret_cast<const double*>(f)); break;
+      case FIN_F32_CODE: {
+        uint32_t c = *reinterpret_cast<const uint32_t*>(f);
+        bits = (c == kF32MinNaN || c == kF32MaxNaN) ? 0x7fc00000u : f32_decode(c);
+        break;
+      }
+      case FIN_F64_CODE: {
+        uint64_t c = *reinterpret_cast<const uint64_t*>(f);
+        bits = (c == kF64MinNaN || c == kF64MaxNaN) ? 0x7ff8000000000000ull : f64_decode(c);
+        break;
+      }
+      case FIN_MEAN_I64: {
+        double m = cnt ? (double)*reinterpret_cast<const int64_t*>(f) / (double)cnt : 0.0;
+        bits = (uint64_t)__double_as_longlong(m);
+        break;
+      }
+      case FIN_MEAN_F64: {
+        double m = cnt ? *reinterpret_cast<const double*>(f) / (double)cnt : 0.0;
+        bits = (uint64_t)__double_as_longlong(m);
+        break;
+      }
+      case FIN_MEAN_F32: {
+        double m = cnt ? (double)(float)*reinterpret_cast<const double*>(f) / (double)cnt : 0.0;
+        bits = (uint64_t)__double_as_longlong(m);
+        break;
+      }
+      case FIN_MEAN_DURATION: {
+        int64_t m = cnt ? *reinterpret_cast<const int64_t*>(f) / (int64_t)cnt : 0;
+        bits = (uint64_t)m;
+        break;
+      }
+      default: break;
+    }
+    store_width(s.out, pos, s.out_width, bits);
+    if (s.valid) s.valid[pos] = cnt != 0;
+  }
+}
+
+// Writes the key columns of a group whose header is `hdr` (`special`: the
+// 8-byte key equal to the empty header) at output row `pos`.
+__device__ __forceinline__ void finalize_keys(uint64_t hdr, bool special, int key_mode, const KeyCol* keys,
+                                              int num_keys, int64_t pos, const OutTable& ot)
+{
+  if (key_mode == KEY_INT) {
+    store_width(ot.key_out[0], pos, keys[0].width, special ? kEmptyIntKey : hdr);
+    return;
+  }
+  const int64_t rep = (int64_t)(hdr & 0xffffffffull);
+  for (int c = 0; c < num_keys; ++c) {
+    const KeyCol& k = keys[c];
+    store_width(ot.key_out[c], pos, k.width, load_at(k.data, k.width, k.stride, rep));
+    if (ot.key_valid[c]) ot.key_valid[c][pos] = bit_valid(k.mask, k.bit_off + rep) ? 1 : 0;
+  }
+}
+
+}  // namespace mgb

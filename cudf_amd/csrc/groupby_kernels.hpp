@@ -1,18 +1,21 @@
-// Kernel parameter blocks and launch helpers shared by the groupby driver
-// (groupby.hip) and the batched / served executors.
+// Kernel parameter blocks of the groupby kernels (passed by value).
 #pragma once
 
 #include "mgb_device.hpp"
 
 namespace mgb {
 
-// Everything the aggregate kernel needs, passed by value as a kernel argument.
+// A set of rows to aggregate (the input columns, or the partitioned copy of
+// them), the accumulator ops, and the global record table.
 struct AggParams {
   int64_t num_rows;
   uint8_t* table;                 // (capacity + 1) records of `stride` bytes
   int64_t capacity;               // records in the hashed region
   unsigned long long* counters;   // [0] groups, [1] overflow flag
   int64_t max_groups;             // inserting more sets the overflow flag
+  const uint8_t* flags;           // per-row validity bits of the partitioned copy, or nullptr
+  int32_t flag_bytes;             // 1, 2 or 4
+  int32_t flag_stride;            // bytes between the flags of consecutive rows
   int32_t stride;
   int32_t key_mode;
   int32_t num_keys;
@@ -42,14 +45,25 @@ enum FinKind : int32_t {
 
 struct OutSpec {
   void* out;              // result data (out_width bytes per group)
-  uint8_t* valid;         // validity byte per group (workspace), nullptr = never null
+  uint8_t* valid;         // validity byte per group, nullptr = never null
   int32_t fin;
   int32_t out_width;
   int32_t field_off;      // primary field
-  int32_t count_off;      // count field deciding validity / the MEAN divisor
+  int32_t count_off;      // row-count field (COUNT_ALL) or count of valid values
+  int32_t nulls_off;      // OP_NULLS field subtracted from count_off, or -1
+  int32_t pad;
 };
 
-constexpr int kMaxOuts = 32;    // outputs per finalize launch
+constexpr int kMaxOuts = 64;    // outputs per call on the LDS paths / per finalize launch
+
+// Where finalised groups are written (device-resident, see OutTable).
+struct OutTable {
+  void* key_out[kMaxKeys];        // key columns
+  uint8_t* key_valid[kMaxKeys];   // key validity bytes (null keys grouped), or nullptr
+  int32_t num_outs;
+  int32_t pad;
+  OutSpec outs[kMaxOuts];
+};
 
 struct FinParams {
   const uint8_t* table;
@@ -57,12 +71,45 @@ struct FinParams {
   int32_t stride;
   int32_t key_mode;
   int32_t num_keys;
-  int32_t num_outs;
+  int32_t pad;
   const int64_t* block_offsets;   // exclusive prefix of occupied records per block
-  KeyCol keys[kMaxKeys];          // generic mode: gather source
-  void* key_out[kMaxKeys];        // key columns out
-  uint8_t* key_valid[kMaxKeys];   // generic mode with null keys: validity bytes
-  OutSpec outs[kMaxOuts];
+  const OutTable* out;            // device memory
+  KeyCol keys[kMaxKeys];          // representative-row gather source
+};
+
+// Hash partitioning of the input rows (histogram and scatter passes).
+struct PartParams {
+  AggParams src;                  // input rows; keys / vals / skip_null_keys
+  int64_t chunk;                  // rows per workgroup
+  int32_t log2p;                  // 2^log2p partitions
+  int32_t nblocks;                // workgroups of both passes
+  uint32_t* hist;                 // [P][nblocks] row counts
+  const uint32_t* offsets;        // [P][nblocks] exclusive prefix of hist
+  uint8_t* out;                   // partitioned rows: records of rec_bytes
+  int32_t rec_bytes;              // multiple of 8
+  int32_t flag_bytes;             // 0, 1, 2 or 4
+  int32_t flags_off;              // offset of the validity bits in a record
+  int32_t tile_rows;              // rows staged per tile (tiled scatter)
+  int32_t key_off[kMaxKeys];      // offset of each key column in a record
+  int32_t val_off[kMaxVals];      // offset of each value column in a record
+};
+
+// LDS-resident aggregation of one work item (a partition of the partitioned
+// copy, or a contiguous slice of the input rows).
+struct LdsParams {
+  AggParams src;                  // rows, ops; table/capacity/counters: the global table (direct mode)
+  const uint32_t* offsets;        // partitioned mode: [P][nblocks] prefix; partition p starts at offsets[p*nblocks]
+  const unsigned long long* total_rows;  // partitioned mode: rows in the partitioned copy
+  int64_t rows_per_item;          // direct mode
+  const uint64_t* tmpl;           // identity record (device memory)
+  const OutTable* out;            // partitioned mode: outputs
+  unsigned long long* out_counters;  // [0] groups written, [1] LDS overflow flag
+  int32_t nblocks;                // partition pass workgroups
+  int32_t log2p;
+  int32_t slots;                  // LDS records (plus one for the 8-byte empty key)
+  int32_t cap;                    // groups an item may hold
+  int32_t words;                  // record words
+  int32_t direct;                 // 1: merge into the global table, 0: write outputs
 };
 
 }  // namespace mgb

@@ -42,6 +42,7 @@ enum AccOp : int32_t {
   OP_MIN_I64, OP_MAX_I64, OP_MIN_U64, OP_MAX_U64,
   OP_MIN_F32, OP_MAX_F32,  // u32 order-preserving codes
   OP_MIN_F64, OP_MAX_F64,  // u64 order-preserving codes
+  OP_NULLS,       // u32  += 1 for every null value (the row count minus it is COUNT_VALID)
   OP_NUM
 };
 
@@ -52,7 +53,7 @@ struct KeyCol {
   int32_t type_id;
   int32_t width;            // bytes
   int32_t vclass;
-  int32_t pad;
+  int32_t stride;           // bytes from one row to the next (width for a plain column)
 };
 
 struct ValCol {
@@ -63,6 +64,10 @@ struct ValCol {
   int32_t vclass;
   int32_t op_begin;         // ops[op_begin, op_end) read this column
   int32_t op_end;
+  int32_t flag_bit;         // validity from bit `flag_bit` of the row's flags word, or -1
+  int32_t nulls_off;        // OP_NULLS field of this column, or -1
+  int32_t stride;           // bytes from one row to the next (width for a plain column)
+  int32_t pad;
 };
 
 struct OpDesc {
@@ -137,6 +142,18 @@ __device__ __forceinline__ uint64_t load_raw(const uint8_t* base, int width, int
     case 2: return reinterpret_cast<const uint16_t*>(base)[row];
     case 4: return reinterpret_cast<const uint32_t*>(base)[row];
     default: return reinterpret_cast<const uint64_t*>(base)[row];
+  }
+}
+
+// Element `row` of a column whose rows are `stride` bytes apart.
+__device__ __forceinline__ uint64_t load_at(const uint8_t* base, int width, int stride, int64_t row)
+{
+  const uint8_t* q = base + row * (int64_t)stride;
+  switch (width) {
+    case 1: return *q;
+    case 2: return *reinterpret_cast<const uint16_t*>(q);
+    case 4: return *reinterpret_cast<const uint32_t*>(q);
+    default: return *reinterpret_cast<const uint64_t*>(q);
   }
 }
 
@@ -347,7 +364,7 @@ __device__ __host__ __forceinline__ int op_field_bytes(int kind)
 {
   switch (kind) {
     case OP_ROWS: case OP_COUNT: case OP_MIN_I32: case OP_MAX_I32: case OP_MIN_U32: case OP_MAX_U32:
-    case OP_MIN_F32: case OP_MAX_F32: return 4;
+    case OP_MIN_F32: case OP_MAX_F32: case OP_NULLS: return 4;
     default: return 8;
   }
 }

@@ -46,6 +46,20 @@ def _random_case(rng, n, key_type, card, val_types, null_frac, kinds, key_null_f
     return [(key_type, keys, kvalid)], reqs
 
 
+PATHS = ["auto", "global", "direct", "partitioned"]
+
+
+@pytest.fixture(params=PATHS)
+def path(request, monkeypatch):
+    """Runs a test once per execution of the groupby (the library reads
+    MGB_GROUPBY_PATH on every call; "auto" picks by the cardinality estimate)."""
+    if request.param != "auto":
+        monkeypatch.setenv("MGB_GROUPBY_PATH", request.param)
+    else:
+        monkeypatch.delenv("MGB_GROUPBY_PATH", raising=False)
+    return request.param
+
+
 def _compare(keys, reqs, include_null_keys=False):
     gk, gr = gpu_groupby(keys, reqs, include_null_keys)
     ok, orr = oracle_groupby(keys, reqs, include_null_keys)
@@ -69,7 +83,7 @@ ALL_KINDS = [O.SUM, O.MIN, O.MAX, O.COUNT_VALID, O.COUNT_ALL, O.MEAN, O.SUM_OF_S
 
 
 @pytest.mark.parametrize("key_type", [O.INT8, O.INT16, O.INT32, O.INT64, O.UINT32, O.UINT64, O.FLOAT32, O.FLOAT64])
-def test_key_types(gpu, key_type):
+def test_key_types(gpu, path, key_type):
     rng = np.random.default_rng(100 + key_type)
     card = 100 if key_type in (O.INT8,) else 5000
     keys, reqs = _random_case(rng, 50_000, key_type, card, [O.INT32, O.FLOAT64], 0.1, ALL_KINDS)
@@ -78,7 +92,7 @@ def test_key_types(gpu, key_type):
 
 @pytest.mark.parametrize("val_type", [O.INT8, O.INT16, O.INT32, O.INT64, O.UINT8, O.UINT16, O.UINT32, O.UINT64,
                                       O.FLOAT32, O.FLOAT64, O.BOOL8])
-def test_value_types(gpu, val_type):
+def test_value_types(gpu, path, val_type):
     rng = np.random.default_rng(200 + val_type)
     kinds = [O.SUM, O.MIN, O.MAX, O.COUNT_VALID, O.COUNT_ALL, O.MEAN]
     keys, reqs = _random_case(rng, 40_000, O.INT32, 3000, [val_type], 0.2, kinds)
@@ -86,21 +100,21 @@ def test_value_types(gpu, val_type):
 
 
 @pytest.mark.parametrize("card", [1, 16, 1000, 100_000])
-def test_cardinality(gpu, card):
+def test_cardinality(gpu, path, card):
     rng = np.random.default_rng(300 + card)
     keys, reqs = _random_case(rng, 200_000, O.INT64, card, [O.INT32, O.FLOAT64, O.FLOAT32], 0.05,
                               [O.SUM, O.COUNT_VALID, O.MIN, O.MAX, O.MEAN])
     _compare(keys, reqs)
 
 
-def test_null_keys_exclude_and_include(gpu):
+def test_null_keys_exclude_and_include(gpu, path):
     rng = np.random.default_rng(11)
     keys, reqs = _random_case(rng, 30_000, O.INT32, 500, [O.INT64], 0.1, ALL_KINDS, key_null_frac=0.1)
     _compare(keys, reqs, include_null_keys=False)
     _compare(keys, reqs, include_null_keys=True)
 
 
-def test_composite_keys(gpu):
+def test_composite_keys(gpu, path):
     rng = np.random.default_rng(12)
     n = 100_000
     a = rng.integers(0, 300, n).astype(np.int32)
@@ -113,7 +127,7 @@ def test_composite_keys(gpu):
     _compare(keys, reqs, include_null_keys=True)
 
 
-def test_sentinel_key_values(gpu):
+def test_sentinel_key_values(gpu, path):
     # INT64_MIN is the empty-slot header of single integer keys; it must still group.
     n = 10_000
     rng = np.random.default_rng(13)
@@ -169,7 +183,7 @@ def test_nan_skipped_when_other_values(gpu):
     assert np.isnan(mn[1]) and np.isnan(mx[1])
 
 
-def test_c2_shape_slice(gpu):
+def test_c2_shape_slice(gpu, path):
     """2M rows of the 100M-row benchmark's distribution (int64 keys over 200K
     distinct values, int32/f64/f32 values 5% null, 5 aggs each)."""
     rng = np.random.default_rng(42)
