@@ -26,6 +26,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "groupby rows/s & % HBM peak (100M rows); batched q/s, p50/p99 at fixed QPS"
+
+
+def log(msg: str) -> None:
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -78,6 +83,98 @@ def c2_requests(cols):
     return [ca.GroupByRequest(c, [agg.sum(), agg.count(), agg.min(), agg.max(), agg.mean()]) for c in cols]
 
 
+def zipf_ranks(n_rows: int, n_keys: int, s: float, g, device: str = "cuda"):
+    """Ranks in [0, n_keys) with P(rank r) ~ (r+1)^-s, by inverting the
+    continuous approximation of the Zipf CDF."""
+    import torch
+
+    u = torch.rand(n_rows, generator=g, device=device, dtype=torch.float64)
+    a = 1.0 - s
+    x = (1.0 + u * (float(n_keys) ** a - 1.0)) ** (1.0 / a)
+    return torch.clamp(x.floor().to(torch.int64) - 1, 0, n_keys - 1)
+
+
+def make_workload(name: str, rows: int, seed: int, device: str = "cuda"):
+    """(key columns, requests, input bytes per row, output bytes per group, description)."""
+    import torch
+
+    import cudf_amd as ca
+    from cudf_amd import aggregation as agg
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    if name == "c2":
+        key_col, cols, distinct = make_c2(rows, 1_000_000, seed, device)
+        return ([key_col], c2_requests(cols), 8 + 4 + 8 + 4 + 3 / 8, 96 + 12 / 8,
+                f"groupby-aggregate, {rows:,} rows, int64 key over {distinct:,} distinct values, int32/f64/f32 "
+                "values 5% null, sum/count/min/max/mean per value column")
+    if name == "c1":
+        keys = torch.randint(0, 1000, (rows,), generator=g, device=device, dtype=torch.int32)
+        vals = torch.rand(rows, generator=g, device=device, dtype=torch.float64)
+        req = [ca.GroupByRequest(ca.Column.from_torch(vals, ca.TypeId.FLOAT64),
+                                 [agg.sum(), agg.count(), agg.mean(), agg.min(), agg.max()])]
+        return ([ca.Column.from_torch(keys, ca.TypeId.INT32)], req, 12, 4 + 8 + 4 + 8 + 8 + 8,
+                f"groupby-aggregate, {rows:,} rows, int32 key in [0,1000), f64 value in [0,1), "
+                "sum/count/mean/min/max")
+    if name == "c3a":
+        n_keys = 50_000_000
+        ka = torch.randint(-2 ** 31, 2 ** 31 - 1, (n_keys,), generator=g, device=device, dtype=torch.int32)
+        kb = torch.randint(-2 ** 63, 2 ** 63 - 1, (n_keys,), generator=g, device=device, dtype=torch.int64)
+        perm = torch.randperm(n_keys, generator=g, device=device)
+        idx = perm[zipf_ranks(rows, n_keys, 1.1, g, device)]
+        del perm
+        a, b = ka[idx], kb[idx]
+        del idx, ka, kb
+        vals = torch.randn(rows, generator=g, device=device, dtype=torch.float64)
+        req = [ca.GroupByRequest(ca.Column.from_torch(vals, ca.TypeId.FLOAT64), [agg.sum(), agg.count(), agg.max()])]
+        return ([ca.Column.from_torch(a, ca.TypeId.INT32), ca.Column.from_torch(b, ca.TypeId.INT64)], req, 20,
+                4 + 8 + 8 + 4 + 8,
+                f"groupby-aggregate, {rows:,} rows, composite (int32, int64) key Zipf s=1.1 over 50M distinct tuples, "
+                "f64 value, sum/count/max")
+    if name == "c3b":
+        uniq = torch.randint(-2 ** 31, 2 ** 31 - 1, (16,), generator=g, device=device, dtype=torch.int32)
+        keys = uniq[torch.randint(0, 16, (rows,), generator=g, device=device)]
+        vals = torch.randn(rows, generator=g, device=device, dtype=torch.float64)
+        req = [ca.GroupByRequest(ca.Column.from_torch(vals, ca.TypeId.FLOAT64), [agg.sum(), agg.count(), agg.max()])]
+        return ([ca.Column.from_torch(keys, ca.TypeId.INT32)], req, 12, 4 + 8 + 4 + 8,
+                f"groupby-aggregate, {rows:,} rows, int32 key with 16 distinct values, f64 value, sum/count/max")
+    raise SystemExit(f"unknown workload {name}")
+
+
+DEFAULT_ROWS = {"c1": 1_000_000, "c2": 100_000_000, "c3a": 200_000_000, "c3b": 200_000_000}
+
+
+def cpu_baseline_generic(name: str, sample_rows: int, seed: int, reps: int = 3):
+    """pandas groupby-agg on a bounded sample drawn like the workload."""
+    import numpy as np
+    import pandas as pd
+
+    rng = np.random.default_rng(seed)
+    if name == "c1":
+        df = pd.DataFrame({"key": rng.integers(0, 1000, sample_rows).astype(np.int32), "val": rng.random(sample_rows)})
+        run = lambda: df.groupby("key", sort=False)["val"].agg(["sum", "count", "mean", "min", "max"])
+    elif name == "c3a":
+        n_keys = 50_000_000
+        u = rng.random(sample_rows)
+        r = np.clip(np.floor((1.0 + u * (n_keys ** -0.1 - 1.0)) ** -10.0).astype(np.int64) - 1, 0, n_keys - 1)
+        df = pd.DataFrame({"a": (r * 2654435761 % 2 ** 31).astype(np.int32), "b": r * 0x9E3779B97F4A7C15 % 2 ** 61,
+                           "val": rng.standard_normal(sample_rows)})
+        run = lambda: df.groupby(["a", "b"], sort=False)["val"].agg(["sum", "count", "max"])
+    else:  # c3b
+        uniq = rng.integers(-2 ** 31, 2 ** 31 - 1, 16).astype(np.int32)
+        df = pd.DataFrame({"key": uniq[rng.integers(0, 16, sample_rows)], "val": rng.standard_normal(sample_rows)})
+        run = lambda: df.groupby("key", sort=False)["val"].agg(["sum", "count", "max"])
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    return {"value": sample_rows / t, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"pandas {pd.__version__} groupby-agg on {sample_rows:,} rows drawn like the workload, "
+                      f"median of {reps}, single-threaded; host os.cpu_count()={os.cpu_count()}"}
+
+
 def cpu_baseline_c2(sample_rows: int, distinct: int, seed: int, reps: int = 3):
     """pandas groupby-agg on a bounded sample of the config-2 distribution."""
     import numpy as np
@@ -117,12 +214,25 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--rows", type=int, default=100_000_000)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3a", "c3b"])
+    ap.add_argument("--rows", type=int, default=None)
     ap.add_argument("--distinct", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-sample-rows", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 1 GiB buffer 3 times (reference kernel for FETCH_SIZE / WRITE_SIZE scaling)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": 1 << 30, "bytes_written": 1 << 30}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU and "
@@ -137,10 +247,13 @@ def main():
 
     torch.cuda.set_device(0)
     lib = _lib.load()
-    key_col, cols, distinct = make_c2(args.rows, args.distinct, args.seed)
+    rows = args.rows or DEFAULT_ROWS[args.workload]
+    args.rows = rows
+    log(f"generating workload {args.workload} ({rows:,} rows)")
+    key_cols, reqs, in_bytes_per_row, out_bytes_per_group, description = make_workload(args.workload, rows, args.seed)
     torch.cuda.synchronize()
-    reqs = c2_requests(cols)
-    gb = ca.GroupBy(ca.Table([key_col]))
+    log("warmup")
+    gb = ca.GroupBy(ca.Table(key_cols))
     groups = 0
     for _ in range(args.warmup):
         k, r = gb.aggregate(reqs)
@@ -148,8 +261,9 @@ def main():
         del k, r
     torch.cuda.synchronize()
 
+    log(f"timing {args.steps} steps")
     lib.mgb_profiling_enable(1)
-    for name in (b"aggregate", b"fill", b"finalize", b"partition_hist", b"partition_scatter"):
+    for name in (b"aggregate", b"fill", b"finalize", b"partition_hist", b"partition_scatter", b"estimate"):
         n, ms = ctypes.c_int64(), ctypes.c_double()
         lib.mgb_profiling_read(name, ctypes.byref(n), ctypes.byref(ms))  # drop stale records
     stream = torch.cuda.current_stream()
@@ -164,7 +278,7 @@ def main():
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
     kern = {}
-    for name in ("aggregate", "fill", "finalize", "partition_hist", "partition_scatter"):
+    for name in ("aggregate", "fill", "finalize", "partition_hist", "partition_scatter", "estimate"):
         n, ms = ctypes.c_int64(), ctypes.c_double()
         _lib.check(lib.mgb_profiling_read(name.encode(), ctypes.byref(n), ctypes.byref(ms)))
         kern[name] = (n.value, ms.value)
@@ -172,11 +286,10 @@ def main():
 
     R = args.rows
     ms_per_step = total_ms / args.steps
-    in_bytes_per_row = 8 + 4 + 8 + 4 + 3 / 8
-    in_bytes = R * in_bytes_per_row
-    # per group: key 8 + int32 col (sum 8, count 4, min 4, max 4, mean 8) + f64 col (8, 4, 8, 8, 8)
+    # c2 per group: key 8 + int32 col (sum 8, count 4, min 4, max 4, mean 8) + f64 col (8, 4, 8, 8, 8)
     # + f32 col (4, 4, 4, 4, 8) = 96 bytes, plus 12 validity bits
-    out_bytes = groups * 96 + groups * 12 / 8
+    in_bytes = R * in_bytes_per_row
+    out_bytes = groups * out_bytes_per_group
     agg_n, agg_ms = kern["aggregate"]
     agg_avg_ms = agg_ms / max(agg_n, 1)
     achieved = in_bytes / (agg_avg_ms * 1e-3) / 1e9
@@ -184,7 +297,11 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline_c2(min(args.cpu_sample_rows, R), args.distinct, args.seed)
+        log("cpu baseline")
+        if args.workload == "c2":
+            cpu = cpu_baseline_c2(min(args.cpu_sample_rows, R), args.distinct, args.seed)
+        else:
+            cpu = cpu_baseline_generic(args.workload, min(args.cpu_sample_rows, R), args.seed)
 
     line = {
         "metric": METRIC,
@@ -197,13 +314,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int64 key; int32/f64/f32 values (f64 accumulation)",
+        "dtype": "int key(s); int/float values; int64 and f64 accumulation",
         "data": "synthetic (seeded on device)",
-        "config": {
-            "workload": "groupby-aggregate, 100M rows, int64 key over 1M distinct values, int32/f64/f32 values "
-                        "5% null, sum/count/min/max/mean per value column",
-            "rows": R, "distinct_keys": distinct, "groups": groups, "parallelism": "single GPU",
-        },
+        "config": {"workload": description, "rows": R, "groups": groups, "parallelism": "single GPU"},
         "roofline": {
             "bound": "hbm",
             "kernel": "aggregate_kernel",

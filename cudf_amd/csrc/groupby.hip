@@ -143,6 +143,7 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(const FinParams p)
   if (!occ) return;
   int64_t pos = p.block_offsets[blockIdx.x] + __popcll(ballot & ((1ull << lane) - 1ull));
   for (unsigned w = 0; w < wave; ++w) pos += wave_counts[w];
+  if (p.pos_base) pos += (int64_t)*p.pos_base;
   const uint8_t* rec = p.table + r * (int64_t)p.stride;
   const uint64_t hdr = *reinterpret_cast<const uint64_t*>(rec);
   finalize_keys(hdr, r == p.capacity, p.key_mode, p.keys, p.num_keys, pos, *p.out);
@@ -233,6 +234,58 @@ __global__ __launch_bounds__(kBlock) void sample_stats_kernel(const uint32_t* cn
     if (d) atom_add(&stats[0], (unsigned long long)d);
     if (f1) atom_add(&stats[1], (unsigned long long)f1);
     if (f2) atom_add(&stats[2], (unsigned long long)f2);
+  }
+}
+
+// ---- cardinality estimate: HyperLogLog over every row's key hash
+
+constexpr int kHllBits = 14;
+constexpr int kHllRegs = 1 << kHllBits;
+
+// Per-workgroup registers: register = top 14 bits of the row hash, value =
+// leading zeros + 1 of a bijective remix of the hash.
+__global__ __launch_bounds__(kLdsBlock) void hll_block_kernel(const AggParams p, uint8_t* block_regs)
+{
+  extern __shared__ uint32_t regs[];
+  for (int i = threadIdx.x; i < kHllRegs; i += kLdsBlock) regs[i] = 0;
+  __syncthreads();
+  const int64_t step = (int64_t)gridDim.x * kLdsBlock;
+  for (int64_t row = (int64_t)blockIdx.x * kLdsBlock + threadIdx.x; row < p.num_rows; row += step) {
+    if (!key_row_valid(p, row)) continue;
+    const uint32_t h = row_key(p, row).hash;
+    const uint32_t rank = (uint32_t)__clz(fmix32(h ^ 0x9e3779b9u)) + 1u;
+    uint32_t* r = &regs[h >> (32 - kHllBits)];
+    // registers saturate quickly: most rows only read
+    if (rank > __hip_atomic_load(r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMax(r, rank);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHllRegs; i += kLdsBlock)
+    block_regs[(int64_t)blockIdx.x * kHllRegs + i] = (uint8_t)regs[i];
+}
+
+// Max over workgroups per register, then per block the sum of 2^-M and the
+// number of empty registers.
+__global__ __launch_bounds__(256) void hll_reduce_kernel(const uint8_t* block_regs, int nb, double* sums,
+                                                        uint32_t* zeros)
+{
+  __shared__ double s[256];
+  __shared__ uint32_t z[256];
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  uint32_t m = 0;
+  for (int b = 0; b < nb; ++b) m = max(m, (uint32_t)block_regs[(int64_t)b * kHllRegs + j]);
+  s[threadIdx.x] = ldexp(1.0, -(int)m);
+  z[threadIdx.x] = m == 0;
+  __syncthreads();
+  for (int d = 128; d > 0; d >>= 1) {
+    if (threadIdx.x < (unsigned)d) {
+      s[threadIdx.x] += s[threadIdx.x + d];
+      z[threadIdx.x] += z[threadIdx.x + d];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sums[blockIdx.x] = s[0];
+    zeros[blockIdx.x] = z[0];
   }
 }
 
@@ -349,8 +402,10 @@ __global__ __launch_bounds__(kLdsBlock) void part_scatter_tiled_kernel(const Par
   uint32_t* tcount = cursor + P;
   uint32_t* scan_tmp = tcount + P;  // 16 words
   uint16_t* pid = reinterpret_cast<uint16_t*>(scan_tmp + 16);
-  uint64_t* stage = reinterpret_cast<uint64_t*>(
-    (reinterpret_cast<uintptr_t>(pid + kScatterTile) + 15) & ~uintptr_t(15));
+  // staged records start 16-byte aligned after the partition ids (offset
+  // computed from smem itself so the compiler keeps the LDS address space)
+  const int stage_off = ((P * 8 + 16 * 4 + kScatterTile * 2) + 15) & ~15;
+  uint64_t* stage = reinterpret_cast<uint64_t*>(smem + stage_off);
   uint64_t* out = reinterpret_cast<uint64_t*>(pp.out);
   for (int i = threadIdx.x; i < P; i += kLdsBlock) {
     cursor[i] = pp.offsets[(int64_t)i * pp.nblocks + blockIdx.x];
@@ -481,18 +536,77 @@ __global__ __launch_bounds__(256) void scan_apply_u32_kernel(const uint32_t* in,
   }
 }
 
+// A row whose group did not fit the LDS table: device atomics on the global
+// table. Kept out of line so that the LDS path's record pointer stays an LDS
+// pointer (ds_* atomics) in the caller.
+__device__ __forceinline__ void spill_row(const AggParams& p, const RowKey& rk, int64_t row)
+{
+  uint8_t* g = global_find_or_insert(p, rk, row);
+  if (g) apply_row(p, g, row);
+}
+
 // ---- LDS aggregation of one work item
 
+// Work items of the partitioned execution: partition p (rows
+// [offsets[p*nblocks], offsets[(p+1)*nblocks])) is cut into
+// ceil(rows / max_item_rows) items. count[p] = its number of items;
+// count[P] = 0 so that an exclusive scan of P+1 entries ends with the total.
+__global__ __launch_bounds__(kBlock) void count_items_kernel(const uint32_t* offsets, int nblocks, int P,
+                                                             const unsigned long long* total_rows,
+                                                             int64_t max_item_rows, uint32_t* count)
+{
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p > P) return;
+  if (p == P) {
+    count[P] = 0;
+    return;
+  }
+  const int64_t b = offsets[(int64_t)p * nblocks];
+  const int64_t e = p + 1 < P ? (int64_t)offsets[(int64_t)(p + 1) * nblocks] : (int64_t)*total_rows;
+  count[p] = (uint32_t)((e - b + max_item_rows - 1) / max_item_rows);
+}
+
+// One workgroup aggregates one work item in an LDS table. Rows whose group
+// does not fit the table go to the global table. At the end the item either
+// writes its groups as final output rows (a partition handled by this item
+// alone, without spill), or folds them into the global table (direct mode,
+// partitions split over several items, or an item that spilled), whose
+// groups are finalised afterwards.
 __global__ __launch_bounds__(kLdsBlock) void lds_agg_kernel(const LdsParams lp)
 {
   extern __shared__ __align__(16) uint8_t lds[];
   const AggParams& p = lp.src;
+
+  int64_t begin, end;
+  bool merge = true;
+  if (lp.direct) {
+    begin = (int64_t)blockIdx.x * lp.rows_per_item;
+    end = begin + lp.rows_per_item < p.num_rows ? begin + lp.rows_per_item : p.num_rows;
+  } else {
+    const int P = 1 << lp.log2p;
+    const uint32_t b = blockIdx.x;
+    if (b >= lp.item_offsets[P]) return;  // the grid is an upper bound on the items
+    int lo = 0, hi = P - 1;                // last partition whose first item is <= b
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (lp.item_offsets[mid] <= b) lo = mid;
+      else hi = mid - 1;
+    }
+    const int part = lo;
+    const uint32_t n_items = lp.item_offsets[part + 1] - lp.item_offsets[part];
+    const int64_t ps = lp.offsets[(int64_t)part * lp.nblocks];
+    const int64_t pe = part + 1 < P ? (int64_t)lp.offsets[(int64_t)(part + 1) * lp.nblocks] : (int64_t)*lp.total_rows;
+    begin = ps + (int64_t)(b - lp.item_offsets[part]) * lp.max_item_rows;
+    end = begin + lp.max_item_rows < pe ? begin + lp.max_item_rows : pe;
+    merge = n_items > 1;
+  }
+
   LdsTable t;
   t.rec = lds;
   t.slots = lp.slots;
-  t.stride = p.stride;
+  t.stride = lp.words * 8;  // LDS record stride (the record padded to 8 x odd bytes)
   t.cap = lp.cap;
-  int32_t* ctr = reinterpret_cast<int32_t*>(lds + (int64_t)(lp.slots + 1) * p.stride);
+  int32_t* ctr = reinterpret_cast<int32_t*>(lds + (int64_t)(lp.slots + 1) * t.stride);
   t.count = ctr;
   t.overflow = ctr + 1;
   int32_t* scan = ctr + 4;  // kLdsBlock ints
@@ -508,32 +622,42 @@ __global__ __launch_bounds__(kLdsBlock) void lds_agg_kernel(const LdsParams lp)
   if (threadIdx.x < 4) ctr[threadIdx.x] = 0;
   __syncthreads();
 
-  int64_t begin, end;
-  if (lp.direct) {
-    begin = (int64_t)blockIdx.x * lp.rows_per_item;
-    end = begin + lp.rows_per_item < p.num_rows ? begin + lp.rows_per_item : p.num_rows;
-  } else {
-    const int P = 1 << lp.log2p;
-    begin = lp.offsets[(int64_t)blockIdx.x * lp.nblocks];
-    end = (int)blockIdx.x + 1 < P ? (int64_t)lp.offsets[(int64_t)(blockIdx.x + 1) * lp.nblocks]
-                                  : (int64_t)*lp.total_rows;
-  }
-  for (int64_t row = begin + threadIdx.x; row < end; row += kLdsBlock) {
-    if (!lp.direct && __hip_atomic_load(t.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-    if (lp.direct && !key_row_valid(p, row)) continue;
-    const RowKey rk = row_key(p, row);
-    const uint32_t slot_hash = rk.hash << lp.log2p;
-    uint8_t* rec = lds_find_or_insert(t, p, rk, slot_hash, row);
-    if (rec) {
-      apply_row(p, rec, row);
-    } else if (lp.direct) {
-      uint8_t* g = global_find_or_insert(p, rk, row);
-      if (g) apply_row(p, g, row);
+  // Software pipelined in groups of kGroup rows per thread: the loads of the
+  // next group (key, flags, first kPre values of each row) are issued before
+  // the current group's probes and LDS atomics, keeping ~kGroup rows per
+  // thread in flight against HBM latency at one workgroup per CU.
+  constexpr int kGroup = 2;
+  constexpr int64_t kSpan = (int64_t)kGroup * kLdsBlock;
+  RowPre cur[kGroup], nxt[kGroup];
+  const int64_t first = begin + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < kGroup; ++u)
+    if (first + u * kLdsBlock < end) preload_row(p, first + u * kLdsBlock, cur[u]);
+  for (int64_t base = first; base < end; base += kSpan) {
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u)
+      if (base + kSpan + u * kLdsBlock < end) preload_row(p, base + kSpan + u * kLdsBlock, nxt[u]);
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const int64_t row = base + u * kLdsBlock;
+      if (row >= end) break;
+      if (lp.direct && !key_row_valid(p, row)) continue;
+      const RowKey rk = row_key_pre(p, row, cur[u]);
+      const uint32_t slot_hash = rk.hash << lp.log2p;
+      uint8_t* lrec = lds_find_or_insert(t, p, rk, slot_hash, row);
+      if (lrec) {
+        apply_row_pre(p, lrec, row, cur[u]);  // LDS atomics
+      } else {
+        // spill: a separate call keeps the two pointers' address spaces apart
+        spill_row(p, rk, row);
+      }
     }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) cur[u] = nxt[u];
   }
   __syncthreads();
 
-  if (lp.direct) {
+  if (merge || *t.overflow) {
     // fold this item's groups into the global table
     for (int i = threadIdx.x; i <= lp.slots; i += kLdsBlock) {
       if (!lds_occupied(t, i, p.key_mode)) continue;
@@ -557,10 +681,6 @@ __global__ __launch_bounds__(kLdsBlock) void lds_agg_kernel(const LdsParams lp)
     return;
   }
 
-  if (*t.overflow) {
-    if (threadIdx.x == 0) atom_max(&lp.out_counters[1], 1ull);
-    return;
-  }
   int mine = 0;
   for (int i = threadIdx.x; i <= lp.slots; i += kLdsBlock) mine += lds_occupied(t, i, p.key_mode) ? 1 : 0;
   scan[threadIdx.x] = mine;
@@ -993,7 +1113,7 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
     for (auto& o : pc.ops)
       if (op_field_bytes(o.first) == 4) { o.second = off; off += 4; }
   plan.stride = (int)align_up((size_t)off, 8);
-  if (plan.stride > kMaxRecordWords * 8)
+  if ((plan.stride + 7) / 8 + 1 > kMaxRecordWords)  // room for the LDS padding word
     return fail(MGB_ERROR_UNSUPPORTED, "too many distinct aggregations for one groupby call");
   for (auto& pc : plan.cols) {
     pc.count_off = plan.rows_off;
@@ -1063,16 +1183,6 @@ void record_template(const Plan& plan, uint64_t header, uint64_t* words)
       std::memcpy(bytes.data() + o.second, &id, (size_t)op_field_bytes(o.first));
     }
   std::memcpy(words, bytes.data(), (size_t)plan.stride);
-}
-
-// Chao1 distinct-count estimate from a sample's frequency counts.
-int64_t estimate_groups(uint64_t d, uint64_t f1, uint64_t f2, uint64_t sampled, int64_t rows)
-{
-  if (sampled == 0) return 1;
-  if ((int64_t)sampled >= rows) return (int64_t)d;
-  double est = (double)d + (double)f1 * (double)(f1 > 0 ? f1 - 1 : 0) / (2.0 * (double)(f2 + 1));
-  est = std::min(est, (double)rows);
-  return std::max<int64_t>((int64_t)d, (int64_t)std::ceil(est));
 }
 
 struct Allocs {
@@ -1274,9 +1384,20 @@ struct LdsGeom {
   bool ok = false;
 };
 
+// LDS record stride: the record padded to 8 x (odd number) bytes, so that the
+// same field of records at random slots spreads over 32 of the 64 LDS banks
+// (an 80-byte stride = 20 dwords would reach only 16 of them).
+int lds_stride_of(int stride)
+{
+  int words = (stride + 7) / 8;
+  if (words % 2 == 0) ++words;
+  return words * 8;
+}
+
 LdsGeom lds_geometry(int stride)
 {
   LdsGeom g;
+  stride = lds_stride_of(stride);
   const size_t reserve = 16 + (size_t)kLdsBlock * 4;
   const size_t lim = lds_limit();
   if (lim <= reserve + 2 * (size_t)stride) return g;
@@ -1301,12 +1422,62 @@ Path forced_path()
   return Path::Auto;
 }
 
+// HyperLogLog estimate of the number of distinct key rows (standard error
+// ~1.04/sqrt(2^14) = 0.8%), with linear counting for small cardinalities.
+int64_t hll_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_t stream, mgb_status& st)
+{
+  st = MGB_SUCCESS;
+  const int cus = device_cu_count();
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(2 * cus, (R + 65535) / 65536));
+  const int nred = kHllRegs / 256;
+  const size_t regs_bytes = align_up((size_t)nb * kHllRegs, 256);
+  const size_t need = regs_bytes + align_up((size_t)nred * 8, 256) + align_up((size_t)nred * 4, 256);
+  hipError_t e = ws->main.reserve(need, stream);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)hll_block_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kHllRegs * 4);
+  if (e != hipSuccess) {
+    st = fail(MGB_ERROR_HIP, std::string("workspace: ") + hipGetErrorString(e));
+    return R;
+  }
+  uint8_t* regs = static_cast<uint8_t*>(ws->main.ptr);
+  double* sums = reinterpret_cast<double*>(regs + regs_bytes);
+  uint32_t* zeros = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(sums) + align_up((size_t)nred * 8, 256));
+  {
+    ProfScope prof("estimate", stream);
+    hll_block_kernel<<<(unsigned)nb, kLdsBlock, kHllRegs * 4, stream>>>(base, regs);
+    hll_reduce_kernel<<<(unsigned)nred, 256, 0, stream>>>(regs, nb, sums, zeros);
+  }
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(ws->pinned, sums, (size_t)nred * 8, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ws->pinned + nred, zeros, (size_t)nred * 4, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    st = fail(MGB_ERROR_HIP, std::string("cardinality estimate: ") + hipGetErrorString(e));
+    return R;
+  }
+  double sum = 0.0;
+  uint64_t z = 0;
+  const double* hs = reinterpret_cast<const double*>(ws->pinned);
+  const uint32_t* hz = reinterpret_cast<const uint32_t*>(ws->pinned + nred);
+  for (int i = 0; i < nred; ++i) {
+    sum += hs[i];
+    z += hz[i];
+  }
+  const double m = (double)kHllRegs;
+  double est = (0.7213 / (1.0 + 1.079 / m)) * m * m / sum;
+  if (est <= 2.5 * m && z > 0) est = m * std::log(m / (double)z);
+  return std::max<int64_t>(1, std::min<int64_t>(R, (int64_t)std::ceil(est)));
+}
+
 int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_t stream, mgb_status& st)
 {
   st = MGB_SUCCESS;
   const int64_t kSample = 1 << 16;
   const int64_t scap = 4 * kSample;
-  const int64_t samples = R <= scap / 2 ? R : kSample;  // small inputs are counted exactly
+  // small inputs are counted exactly; larger ones are sampled
+  const int64_t samples = R <= scap / 2 ? R : kSample;
   size_t need = align_up((size_t)(scap + 1) * 8, 256) + align_up((size_t)(scap + 1) * 4, 256) + 256;
   hipError_t e = ws->main.reserve(need, stream);
   if (e != hipSuccess) {
@@ -1335,7 +1506,12 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
     st = fail(MGB_ERROR_HIP, std::string("cardinality sample: ") + hipGetErrorString(e));
     return R;
   }
-  return estimate_groups(ws->pinned[0], ws->pinned[1], ws->pinned[2], ws->pinned[3], R);
+  const uint64_t d = ws->pinned[0], f1 = ws->pinned[1], sampled = ws->pinned[3];
+  if ((int64_t)sampled >= R || samples == R) return (int64_t)d;
+  // a saturated sample (every sampled key seen at least twice, few distinct
+  // keys) has found all groups; otherwise count distinct keys over all rows
+  if (f1 == 0 && d * 8 <= sampled) return (int64_t)d;
+  return hll_estimate(ws, base, R, stream, st);
 }
 
 // Global-table executions (global and direct paths): build the table, then
@@ -1386,18 +1562,18 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
       for (int c = 0; c < lp.src.num_vals; ++c) lp.src.vals[c].flag_bit = -1;
       const int64_t items = std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 2, (R + 16383) / 16384));
       lp.rows_per_item = (R + items - 1) / items;
-      uint64_t tmpl[kMaxRecordWords];
+      uint64_t tmpl[kMaxRecordWords] = {};
       record_template(plan, plan.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef, tmpl);
       OutTable dummy{};
       const OutTable* d_ot = nullptr;
       const uint64_t* d_tmpl = nullptr;
-      mgb_status st = upload_params(ws, dummy, tmpl, plan.stride / 8, stream, &d_ot, &d_tmpl);
+      mgb_status st = upload_params(ws, dummy, tmpl, lds_stride_of(plan.stride) / 8, stream, &d_ot, &d_tmpl);
       if (st != MGB_SUCCESS) return st;
       lp.tmpl = d_tmpl;
       lp.out = d_ot;
       lp.slots = geom.slots;
       lp.cap = geom.cap;
-      lp.words = plan.stride / 8;
+      lp.words = lds_stride_of(plan.stride) / 8;
       lp.direct = 1;
       lp.log2p = 0;
       MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1488,9 +1664,11 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
 }
 
 // Partitioned execution: hash-partition the rows, then aggregate every
-// partition in LDS. Returns MGB_SUCCESS with `done` false when the estimate
-// asks for more partitions than supported (the caller then uses the global
-// path).
+// partition in LDS. Partitions larger than one work item are split over
+// several items whose groups meet in a small global table (as do the groups
+// of an item that outgrows its LDS table). Returns MGB_SUCCESS with `done`
+// false when the estimate asks for more partitions than supported (the
+// caller then uses the global path).
 mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& base, int64_t R, int64_t G_est,
                            const LdsGeom& geom, hipStream_t stream, TempOuts& temp, int64_t& groups, bool& done)
 {
@@ -1513,6 +1691,15 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     const int64_t chunk = (R + nw - 1) / nw;
     const int64_t nh = P * nw;
     const int64_t nb = (nh + kScanTile - 1) / kScanTile;
+    const int64_t nb_items = (P + 1 + kScanTile - 1) / kScanTile;
+    // work items: partitions longer than max_item_rows are split
+    const int64_t max_item_rows = std::max<int64_t>(65536, 4 * ((R + P - 1) / P));
+    const int64_t item_grid = P + (R + max_item_rows - 1) / max_item_rows;
+    // global table for the groups of split or spilled items
+    const int64_t split_bound = (R + max_item_rows - 1) / max_item_rows + P / 16 + 1;
+    int64_t gcap = 2 * std::min<int64_t>(R, split_bound * (int64_t)(geom.slots + 1)) + 1024;
+    gcap = std::min<int64_t>(gcap, 2 * R + 64);
+    const int64_t gblocks = (gcap + 1 + kBlock - 1) / kBlock;
 
     // record layout of a partitioned row: 8-byte fields, then 4, 2 and 1-byte
     // fields, then the validity bits; the size is a multiple of 8
@@ -1528,7 +1715,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     rec = flags_rec_off + flag_bytes;
     const int rec_bytes = (int)align_up((size_t)rec, 8);
 
-    // workspace: partitioned records, hist, offsets, block sums, counters
+    // workspace: records, hist, offsets, block sums, item counts/offsets, global table, counters
     size_t off = 0;
     const size_t recs_off = off;
     off += align_up((size_t)R * rec_bytes, 256);
@@ -1538,24 +1725,36 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     off += align_up((size_t)nh * 4, 256);
     const size_t bsum_off = off;
     off += align_up((size_t)nb * 4, 256);
+    const size_t icnt_off = off;
+    off += align_up((size_t)(P + 1) * 4, 256);
+    const size_t ioff_off = off;
+    off += align_up((size_t)(P + 1) * 4, 256);
+    const size_t ibsum_off = off;
+    off += align_up((size_t)nb_items * 4, 256);
+    const size_t gtab_off = off;
+    off += align_up((size_t)(gcap + 1) * plan.stride, 256);
+    const size_t gblk_off = off;
+    off += align_up((size_t)gblocks * 8, 256);
     const size_t ctr_off = off;
     off += 256;
     MGB_HIP(ws->main.reserve(off, stream));
     uint8_t* wsb = static_cast<uint8_t*>(ws->main.ptr);
-    unsigned long long* ctr = reinterpret_cast<unsigned long long*>(wsb + ctr_off);  // [0] total rows [1] out [2] ovf
+    // ctr: [0] partitioned rows [1] groups written by items [2] unused [3] items
+    //      [4] global groups [5] global overflow [6] global occupied
+    unsigned long long* ctr = reinterpret_cast<unsigned long long*>(wsb + ctr_off);
     MGB_HIP(hipMemsetAsync(ctr, 0, 256, stream));
 
-    // every group has a row, and an item holds at most slots + 1 groups (the
-    // cap check races with concurrent inserts, the slot count does not)
-    const int64_t U = std::min<int64_t>(R, P * (int64_t)(geom.slots + 1));
+    // every group has a row; items write at most P * (slots + 1) groups, the
+    // global table holds at most gcap
+    const int64_t U = std::min<int64_t>(R, P * (int64_t)(geom.slots + 1) + gcap);
     OutTable ot{};
     mgb_status st = layout_temp(ws, plan, U, stream, temp, ot);
     if (st != MGB_SUCCESS) return st;
-    uint64_t tmpl[kMaxRecordWords];
+    uint64_t tmpl[kMaxRecordWords] = {};
     record_template(plan, plan.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef, tmpl);
     const OutTable* d_ot = nullptr;
     const uint64_t* d_tmpl = nullptr;
-    st = upload_params(ws, ot, tmpl, plan.stride / 8, stream, &d_ot, &d_tmpl);
+    st = upload_params(ws, ot, tmpl, lds_stride_of(plan.stride) / 8, stream, &d_ot, &d_tmpl);
     if (st != MGB_SUCCESS) return st;
 
     PartParams pp{};
@@ -1587,8 +1786,11 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     // the largest tile (a multiple of the workgroup size) whose records fit LDS
     const size_t fixed_lds = (size_t)P * 8 + 16 * 4 + (size_t)kScatterTile * 2 + 16;
     int tile = 0;
-    for (int t = kScatterTile; t >= kLdsBlock; t -= kLdsBlock)
-      if (fixed_lds + (size_t)t * rec_bytes <= lds_limit()) { tile = t; break; }
+    int tile_max = kScatterTile;
+    if (const char* e = std::getenv("MGB_SCATTER_TILE"))  // tuning override (rows, multiple of 1024)
+      tile_max = std::max(kLdsBlock, std::min(kScatterTile, std::atoi(e) / kLdsBlock * kLdsBlock));
+    for (int tt = tile_max; tt >= kLdsBlock; tt -= kLdsBlock)
+      if (fixed_lds + (size_t)tt * rec_bytes <= lds_limit()) { tile = tt; break; }
     pp.tile_rows = tile;
     const size_t tiled_lds = fixed_lds + (size_t)tile * rec_bytes;
     if (tile > 0) {
@@ -1602,10 +1804,39 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     }
     MGB_HIP(hipGetLastError());
 
+    // work items per partition
+    uint32_t* icnt = reinterpret_cast<uint32_t*>(wsb + icnt_off);
+    uint32_t* ioff = reinterpret_cast<uint32_t*>(wsb + ioff_off);
+    uint32_t* ibsum = reinterpret_cast<uint32_t*>(wsb + ibsum_off);
+    count_items_kernel<<<(unsigned)((P + 1 + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+      pp.offsets, (int)nw, (int)P, ctr + 0, max_item_rows, icnt);
+    scan_reduce_u32_kernel<<<(unsigned)nb_items, 256, 0, stream>>>(icnt, P + 1, ibsum);
+    scan_bsums_u32_kernel<<<1, 1024, 0, stream>>>(ibsum, nb_items, ctr + 3);
+    scan_apply_u32_kernel<<<(unsigned)nb_items, 256, 0, stream>>>(icnt, P + 1, ibsum, ioff);
+
+    // the global table
+    uint8_t* gtab = wsb + gtab_off;
+    FillParams fp{};
+    fp.table = reinterpret_cast<uint64_t*>(gtab);
+    fp.num_records = gcap + 1;
+    fp.words = plan.stride / 8;
+    fp.special_hdr = plan.key_mode == KEY_INT ? 0ull : kEmptyRowRef;
+    record_template(plan, plan.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef, fp.tmpl);
+    {
+      const int64_t fill_words = fp.num_records * fp.words;
+      const unsigned fill_grid = (unsigned)std::min<int64_t>((fill_words + kBlock - 1) / kBlock, (int64_t)cus * 32);
+      ProfScope prof("fill", stream);
+      fill_table_kernel<<<fill_grid, kBlock, 0, stream>>>(fp);
+    }
+
     LdsParams lp{};
     lp.src = src;
     lp.src.num_rows = R;
     lp.src.skip_null_keys = 0;
+    lp.src.table = gtab;
+    lp.src.capacity = gcap;
+    lp.src.counters = ctr + 4;
+    lp.src.max_groups = std::max<int64_t>(1, gcap * 3 / 4);
     uint8_t* recs = wsb + recs_off;
     lp.src.flags = flag_bytes ? recs + flags_rec_off : nullptr;
     lp.src.flag_bytes = flag_bytes;
@@ -1631,24 +1862,47 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     lp.log2p = log2p;
     lp.slots = geom.slots;
     lp.cap = geom.cap;
-    lp.words = plan.stride / 8;
+    lp.words = lds_stride_of(plan.stride) / 8;
     lp.direct = 0;
+    lp.item_offsets = ioff;
+    lp.max_item_rows = max_item_rows;
     MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)geom.bytes));
     {
       ProfScope prof("aggregate", stream);
-      lds_agg_kernel<<<(unsigned)P, kLdsBlock, geom.bytes, stream>>>(lp);
+      lds_agg_kernel<<<(unsigned)item_grid, kLdsBlock, geom.bytes, stream>>>(lp);
     }
     MGB_HIP(hipGetLastError());
-    MGB_HIP(hipMemcpyAsync(ws->pinned, ctr, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+
+    // finalise the global table's groups after the items' rows
+    int64_t* gblk = reinterpret_cast<int64_t*>(wsb + gblk_off);
+    count_occupied_kernel<<<(unsigned)gblocks, kBlock, 0, stream>>>(gtab, gcap, plan.stride, plan.key_mode, gblk);
+    scan_blocks_kernel<<<1, 1024, 0, stream>>>(gblk, gblocks, ctr + 6);
+    FinParams f{};
+    f.table = gtab;
+    f.capacity = gcap;
+    f.stride = plan.stride;
+    f.key_mode = plan.key_mode;
+    f.num_keys = (int)plan.keys.size();
+    f.block_offsets = gblk;
+    f.pos_base = ctr + 1;
+    f.out = d_ot;
+    for (size_t c = 0; c < plan.keys.size(); ++c) f.keys[c] = lp.src.keys[c];
+    {
+      ProfScope prof("finalize", stream);
+      finalize_kernel<<<(unsigned)gblocks, kBlock, 0, stream>>>(f);
+    }
+    MGB_HIP(hipGetLastError());
+    MGB_HIP(hipMemcpyAsync(ws->pinned, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     MGB_HIP(hipStreamSynchronize(stream));
-    if (ws->pinned[2] == 0) {
-      groups = (int64_t)ws->pinned[1];
+    if (ws->pinned[5] == 0) {
+      groups = (int64_t)(ws->pinned[1] + ws->pinned[6]);
       temp.rows = groups;
       done = true;
       return MGB_SUCCESS;
     }
-    est *= 4;  // a partition held more groups than LDS: repartition finer
+    est *= 4;  // the spill table overflowed: repartition finer
+    ws->upload_slot = std::max(0, ws->upload_slot - 1);
   }
   return MGB_SUCCESS;
 }

@@ -113,9 +113,15 @@ __device__ __forceinline__ uint8_t* global_find_or_insert(const AggParams& p, co
     if (*hp == 0ull && atom_cas(hp, 0ull, 1ull) == 0ull) atom_add(&p.counters[0], 1ull);
     return rec;
   }
+  // once the table has overflowed the call is repeated with a larger one:
+  // stop work on this attempt at once
+  if (__hip_atomic_load(&p.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return nullptr;
   const uint64_t empty = p.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
   int64_t idx = reduce_range(rk.hash, p.capacity);
   for (int64_t probe = 0; probe < p.capacity; ++probe) {
+    if ((probe & 63) == 63 &&
+        __hip_atomic_load(&p.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return nullptr;
     uint8_t* rec = p.table + idx * (int64_t)p.stride;
     unsigned long long* hp = reinterpret_cast<unsigned long long*>(rec);
     uint64_t cur = *hp;  // a stale read can only be EMPTY; the CAS below resolves it
@@ -160,6 +166,58 @@ __device__ __forceinline__ void apply_row(const AggParams& p, uint8_t* rec, int6
       continue;
     }
     uint64_t raw = load_at(v.data, v.width, v.stride, row);
+    for (int o = v.op_begin; o < v.op_end; ++o) apply_op(rec, p.ops[o].kind, p.ops[o].off, raw, v.width, v.vclass);
+  }
+}
+
+// Loads of one row issued ahead of its processing.
+constexpr int kPre = 4;
+struct RowPre {
+  uint64_t key;          // first key column (raw bits)
+  uint32_t flags;
+  uint64_t vals[kPre];   // raw bits of the first kPre value columns
+};
+
+__device__ __forceinline__ void preload_row(const AggParams& p, int64_t row, RowPre& r)
+{
+  const KeyCol& k = p.keys[0];
+  r.key = load_at(k.data, k.width, k.stride, row);
+  r.flags = p.flags ? load_flags(p.flags, p.flag_bytes, p.flag_stride, row) : 0u;
+#pragma unroll
+  for (int c = 0; c < kPre; ++c)
+    if (c < p.num_vals) r.vals[c] = load_at(p.vals[c].data, p.vals[c].width, p.vals[c].stride, row);
+}
+
+__device__ __forceinline__ RowKey row_key_pre(const AggParams& p, int64_t row, const RowPre& pre)
+{
+  if (p.key_mode != KEY_INT) return row_key(p, row);
+  const KeyCol& k = p.keys[0];
+  RowKey r;
+  r.key = (uint64_t)widen_int(pre.key, k.width, k.vclass);
+  r.hash = hash_combine(0u, murmur3_bits(normalize_float_bits(pre.key, k.vclass), k.width, 0));
+  r.special = r.key == kEmptyIntKey;
+  return r;
+}
+
+// apply_row with the row's loads already done (columns >= kPre load here).
+__device__ __forceinline__ void apply_row_pre(const AggParams& p, uint8_t* rec, int64_t row, const RowPre& pre)
+{
+  if (p.rows_op >= 0) apply_op(rec, OP_ROWS, p.ops[p.rows_op].off, 0, 4, VC_UNSIGNED);
+  for (int c = 0; c < p.num_vals; ++c) {
+    const ValCol& v = p.vals[c];
+    if (!value_valid(p, v, row, pre.flags)) {
+      if (v.nulls_off >= 0) atom_add(reinterpret_cast<uint32_t*>(rec + v.nulls_off), 1u);
+      continue;
+    }
+    uint64_t raw;
+    if (c < kPre) {
+      raw = 0;
+#pragma unroll
+      for (int q = 0; q < kPre; ++q)
+        if (q == c) raw = pre.vals[q];
+    } else {
+      raw = load_at(v.data, v.width, v.stride, row);
+    }
     for (int o = v.op_begin; o < v.op_end; ++o) apply_op(rec, p.ops[o].kind, p.ops[o].off, raw, v.width, v.vclass);
   }
 }

@@ -73,6 +73,7 @@ struct FinParams {
   int32_t num_keys;
   int32_t pad;
   const int64_t* block_offsets;   // exclusive prefix of occupied records per block
+  const unsigned long long* pos_base;  // added to every output row, or nullptr
   const OutTable* out;            // device memory
   KeyCol keys[kMaxKeys];          // representative-row gather source
 };
@@ -110,6 +111,8 @@ struct LdsParams {
   int32_t cap;                    // groups an item may hold
   int32_t words;                  // record words
   int32_t direct;                 // 1: merge into the global table, 0: write outputs
+  const uint32_t* item_offsets;   // partitioned mode: [P+1] first work item of each partition
+  int64_t max_item_rows;          // partitioned mode: rows per work item at most
 };
 
 }  // namespace mgb

@@ -211,7 +211,8 @@ int mgo_spark_murmurhash3_x86_32_table(const mgo_col* cols, int32_t ncols, int64
       const uint8_t* p = (const uint8_t*)cols[c].data + (size_t)i * sz;
       uint32_t w;
       switch (t) {
-        case T_BOOL8: case T_UINT8: w = p[0]; if (t == T_BOOL8) w = p[0]; h = mgo_spark_murmur3_x86_32(&w, 4, h); break;
+        case T_BOOL8: w = p[0] != 0; h = mgo_spark_murmur3_x86_32(&w, 4, h); break;  /* bool reads as 0/1 */
+        case T_UINT8: w = p[0]; h = mgo_spark_murmur3_x86_32(&w, 4, h); break;
         case T_INT8: w = (uint32_t)(int32_t)(int8_t)p[0]; h = mgo_spark_murmur3_x86_32(&w, 4, h); break;
         case T_INT16: { int16_t s; memcpy(&s, p, 2); w = (uint32_t)(int32_t)s; h = mgo_spark_murmur3_x86_32(&w, 4, h); break; }
         case T_UINT16: { uint16_t s; memcpy(&s, p, 2); w = s; h = mgo_spark_murmur3_x86_32(&w, 4, h); break; }

@@ -1,0 +1,113 @@
+"""The C-ABI boundary on the CPU: libcudf_amd.so loads and exports every
+entry point include/mgb_groupby.h declares, and argument errors are reported
+before any device work (cpp/src/groupby/groupby.cu:208-216)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cudf_amd import _lib
+from cudf_amd.column import pack_validity, unpack_validity
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "mgb_groupby.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mgb_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert "mgb_groupby_aggregate" in names and len(names) >= 8
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES), "the Python binding must declare every C entry point"
+
+
+def test_version():
+    assert _lib.load().mgb_version() >= 100
+
+
+@pytest.mark.parametrize("vt", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 4, 5, 8, 9, 10, 11, 12, 15, 16])
+def test_target_type_matches_oracle(vt, kind):
+    assert _lib.load().mgb_target_type(vt, kind) == O.target_type(vt, kind)
+
+
+def _col(type_id, size, ptr=0x1000):
+    c = _lib.mgb_column()
+    c.type_id = type_id
+    c.size = size
+    c.data = ptr
+    c.null_count = 0
+    return c
+
+
+def _call(keys, reqs, nout):
+    lib = _lib.load()
+    ka = (_lib.mgb_column * max(len(keys), 1))(*keys)
+    ra = (_lib.mgb_request * max(len(reqs), 1))()
+    keep = []
+    for i, (vals, kinds) in enumerate(reqs):
+        arr = (_lib.mgb_aggregation * len(kinds))(*[_lib.mgb_aggregation(k, 1) for k in kinds])
+        keep.append(arr)
+        ra[i].values = vals
+        ra[i].aggs = ctypes.cast(arr, ctypes.POINTER(_lib.mgb_aggregation))
+        ra[i].num_aggs = len(kinds)
+    ok = (_lib.mgb_out_column * max(len(keys), 1))()
+    orr = (_lib.mgb_out_column * max(nout, 1))()
+    g = ctypes.c_int64(-1)
+    calls = []
+    alloc = _lib.mgb_allocator(_lib.ALLOC_FN(lambda c, n, s: calls.append(n) or None),
+                               _lib.DEALLOC_FN(lambda c, p, n, s: None), None)
+    st = lib.mgb_groupby_aggregate(ka, len(keys), 0, ra, len(reqs), ok, orr, ctypes.byref(g), None,
+                                   ctypes.byref(alloc))
+    return st, g.value, ok, orr, calls
+
+
+def test_size_mismatch_is_a_logic_error():
+    st, *_ = _call([_col(3, 10)], [(_col(3, 9), [0])], 1)
+    assert st == _lib.MGB_ERROR_LOGIC
+    assert b"Size mismatch" in _lib.load().mgb_last_error_message()
+
+
+def test_invalid_type_aggregation_is_a_logic_error():
+    st, *_ = _call([_col(3, 10)], [(_col(12, 10), [0])], 1)  # SUM of TIMESTAMP_DAYS
+    assert st == _lib.MGB_ERROR_LOGIC
+    assert b"Invalid type/aggregation" in _lib.load().mgb_last_error_message()
+    with pytest.raises(_lib.LogicError):
+        _lib.check(st)
+
+
+def test_empty_input_returns_typed_empty_columns_without_allocating():
+    st, g, ok, orr, calls = _call([_col(3, 0, 0)], [(_col(9, 0, 0), [0, 4, 9])], 3)
+    assert st == _lib.MGB_SUCCESS and g == 0 and calls == []
+    assert ok[0].type_id == 3 and ok[0].size == 0
+    assert [orr[i].type_id for i in range(3)] == [9, 3, 10]  # SUM f32 -> f32, COUNT -> int32, MEAN -> f64
+
+
+def test_null_arguments_rejected():
+    lib = _lib.load()
+    g = ctypes.c_int64()
+    st = lib.mgb_groupby_aggregate(None, 1, 0, None, 0, None, None, ctypes.byref(g), None, None)
+    assert st == _lib.MGB_ERROR_LOGIC
+
+
+@pytest.mark.parametrize("n", [0, 1, 31, 32, 33, 1000])
+def test_validity_pack_roundtrip(n):
+    rng = np.random.default_rng(n)
+    v = rng.random(n) > 0.3
+    words = pack_validity(v)
+    assert (words.size * 4) % 64 == 0
+    assert np.array_equal(unpack_validity(words, n), v)
+    # LSB-first: bit i of word i//32
+    for i in range(min(n, 40)):
+        assert ((int(words[i // 32]) >> (i % 32)) & 1) == int(v[i])

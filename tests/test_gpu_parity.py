@@ -195,3 +195,48 @@ def test_c2_shape_slice(gpu, path):
             ((O.FLOAT64, rng.standard_normal(n), rng.random(n) >= 0.05), kinds),
             ((O.FLOAT32, rng.standard_normal(n).astype(np.float32), rng.random(n) >= 0.05), kinds)]
     _compare([(O.INT64, keys, None)], reqs)
+
+
+def _zipf_ranks(rng, n, n_keys, s):
+    u = rng.random(n)
+    return np.clip(np.floor((1.0 + u * (n_keys ** (1 - s) - 1.0)) ** (1.0 / (1 - s))).astype(np.int64) - 1,
+                   0, n_keys - 1)
+
+
+def test_zipf_composite_keys(gpu, path):
+    """The shape-stress workload at 2M rows: composite (int32, int64) keys,
+    Zipf s=1.1 over 500K tuples (the hottest key holds ~12% of the rows, so
+    its partition is split over several work items), f64 sum/count/max."""
+    rng = np.random.default_rng(31)
+    n, n_keys = 2_000_000, 500_000
+    ta = rng.integers(-2 ** 31, 2 ** 31 - 1, n_keys).astype(np.int32)
+    tb = rng.integers(-2 ** 62, 2 ** 62, n_keys).astype(np.int64)
+    perm = rng.permutation(n_keys)
+    idx = perm[_zipf_ranks(rng, n, n_keys, 1.1)]
+    keys = [(O.INT32, ta[idx], None), (O.INT64, tb[idx], None)]
+    reqs = [((O.FLOAT64, rng.standard_normal(n), None), [O.SUM, O.COUNT_VALID, O.MAX])]
+    _compare(keys, reqs)
+
+
+def test_hot_key_partition_split(gpu, path):
+    """One int64 key holds 40% of 1M rows: far more than one work item, so
+    the partition is aggregated by several workgroups whose partial groups
+    meet in the global table."""
+    rng = np.random.default_rng(32)
+    n = 1_000_000
+    keys = rng.integers(0, 50_000, n).astype(np.int64)
+    keys[rng.random(n) < 0.4] = 123456789
+    reqs = [((O.INT64, rng.integers(-1000, 1000, n).astype(np.int64), rng.random(n) > 0.1),
+             [O.SUM, O.COUNT_VALID, O.MIN, O.MAX, O.MEAN]),
+            ((O.FLOAT32, rng.standard_normal(n).astype(np.float32), None), [O.SUM, O.MIN, O.MAX, O.COUNT_ALL])]
+    _compare([(O.INT64, keys, None)], reqs)
+
+
+def test_low_cardinality_16_keys(gpu, path):
+    """The LDS-path workload at 4M rows: int32 key with 16 distinct values."""
+    rng = np.random.default_rng(33)
+    n = 4_000_000
+    uniq = rng.integers(-2 ** 31, 2 ** 31 - 1, 16).astype(np.int32)
+    keys = uniq[rng.integers(0, 16, n)]
+    reqs = [((O.FLOAT64, rng.standard_normal(n), None), [O.SUM, O.COUNT_VALID, O.MAX])]
+    _compare([(O.INT32, keys, None)], reqs)
