@@ -37,28 +37,14 @@
 #include <tuple>
 #include <vector>
 
-#include "groupby_device.hpp"
-#include "mgb_groupby.h"
+#include "groupby_host.hpp"
+#include "lds_agg.hpp"
 
 namespace mgb {
 
-constexpr int kLdsBlock = 1024;     // threads of an LDS aggregation workgroup
-constexpr int kScanItems = 16;      // u32 scan: items per thread (x256 threads)
-constexpr int kScanTile = 256 * kScanItems;
-constexpr int kMaxRecordWords = 64;
 
 // =================================================================== kernels
 
-// Writes `tmpl` (header + identities) into every record; record `capacity`
-// gets `special_hdr` as header.
-struct FillParams {
-  uint64_t* table;
-  int64_t num_records;
-  int32_t words;
-  int32_t pad;
-  uint64_t special_hdr;
-  uint64_t tmpl[kMaxRecordWords];
-};
 
 __global__ __launch_bounds__(kBlock) void fill_table_kernel(const FillParams p)
 {
@@ -309,9 +295,20 @@ __global__ __launch_bounds__(kBlock) void part_hist_kernel(const PartParams pp)
   __syncthreads();
   const int64_t begin = (int64_t)blockIdx.x * pp.chunk;
   const int64_t end = begin + pp.chunk < p.num_rows ? begin + pp.chunk : p.num_rows;
-  for (int64_t row = begin + threadIdx.x; row < end; row += kBlock) {
-    if (!key_row_valid(p, row)) continue;
-    atomicAdd(&counts[partition_of(row_key(p, row).hash, pp.log2p)], 1u);
+  constexpr int kItems = 4;  // rows per thread per step, hashed column by column
+  for (int64_t row0 = begin + threadIdx.x; row0 < end; row0 += (int64_t)kItems * kBlock) {
+    bool present[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      present[k] = row0 + k * kBlock < end;
+      if (present[k] && p.skip_null_keys) present[k] = key_row_valid(p, row0 + k * kBlock);
+    }
+    uint32_t h[kItems];
+    uint64_t key0[kItems];
+    hash_rows(p, row0, kBlock, present, h, key0);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+      if (present[k]) atomicAdd(&counts[partition_of(h[k], pp.log2p)], 1u);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < P; i += kBlock) pp.hist[(int64_t)i * pp.nblocks + blockIdx.x] = counts[i];
@@ -389,8 +386,12 @@ constexpr int kScatterTile = kLdsBlock * kScatterItems;  // 4096 rows
 // (one LDS atomic per row gives its rank inside its partition's run) and
 // staged as records; consecutive threads then copy consecutive 8-byte words
 // of the sorted records, so each partition's run leaves as one contiguous
-// stretch. LDS: per-partition cursor and tile count, partition id per
-// staged row, the staged records.
+// stretch. A thread's rows of a tile are handled column by column: the key
+// columns and the first kScatterPreVals value columns of all of them are
+// loaded before the ranking and the scan, so one memory latency covers the
+// tile. LDS: per-partition cursor and tile count, partition id per staged
+// row, the staged records.
+constexpr int kScatterPreVals = 3;
 __global__ __launch_bounds__(kLdsBlock) void part_scatter_tiled_kernel(const PartParams pp)
 {
   extern __shared__ __align__(16) uint8_t smem[];
@@ -405,7 +406,8 @@ __global__ __launch_bounds__(kLdsBlock) void part_scatter_tiled_kernel(const Par
   // staged records start 16-byte aligned after the partition ids (offset
   // computed from smem itself so the compiler keeps the LDS address space)
   const int stage_off = ((P * 8 + 16 * 4 + kScatterTile * 2) + 15) & ~15;
-  uint64_t* stage = reinterpret_cast<uint64_t*>(smem + stage_off);
+  uint8_t* stage_bytes = smem + stage_off;
+  const uint64_t* stage = reinterpret_cast<const uint64_t*>(smem + stage_off);
   uint64_t* out = reinterpret_cast<uint64_t*>(pp.out);
   for (int i = threadIdx.x; i < P; i += kLdsBlock) {
     cursor[i] = pp.offsets[(int64_t)i * pp.nblocks + blockIdx.x];
@@ -415,16 +417,41 @@ __global__ __launch_bounds__(kLdsBlock) void part_scatter_tiled_kernel(const Par
   const int64_t begin = (int64_t)blockIdx.x * pp.chunk;
   const int64_t end = begin + pp.chunk < p.num_rows ? begin + pp.chunk : p.num_rows;
   const int items = tile_rows_max / kLdsBlock;
+  const int64_t step = kLdsBlock;
   for (int64_t tile = begin; tile < end; tile += tile_rows_max) {
-    uint32_t part[kScatterItems], rank[kScatterItems];
-    for (int k = 0; k < kScatterItems; ++k) {
-      part[k] = 0xffffffffu;
-      if (k >= items) continue;
-      const int64_t row = tile + k * kLdsBlock + threadIdx.x;
-      if (row < end && key_row_valid(p, row)) {
-        part[k] = partition_of(row_key(p, row).hash, pp.log2p);
-        rank[k] = atomicAdd(&tcount[part[k]], 1u);
+    const int64_t row0 = tile + threadIdx.x;
+    bool present[kScatterItems];
+#pragma unroll
+    for (int k = 0; k < kScatterItems; ++k) present[k] = k < items && row0 + k * step < end;
+    if (p.skip_null_keys) {
+#pragma unroll
+      for (int k = 0; k < kScatterItems; ++k)
+        if (present[k]) present[k] = key_row_valid(p, row0 + k * step);
+    }
+    uint32_t h[kScatterItems];
+    uint64_t key0[kScatterItems];
+    hash_rows(p, row0, step, present, h, key0);
+    uint64_t vpre[kScatterPreVals][kScatterItems];
+#pragma unroll
+    for (int c = 0; c < kScatterPreVals; ++c)
+      if (c < p.num_vals)
+        load_rows(p.vals[c].data, p.vals[c].width, p.vals[c].stride, VC_UNSIGNED, row0, step, present, vpre[c]);
+    uint32_t flags[kScatterItems] = {};
+    if (pp.flag_bytes) {
+#pragma unroll 1
+      for (int c = 0; c < p.num_vals; ++c) {
+        const ValCol& v = p.vals[c];
+        if (v.flag_bit < 0) continue;
+#pragma unroll
+        for (int k = 0; k < kScatterItems; ++k)
+          if (present[k] && bit_valid(v.mask, v.bit_off + row0 + k * step)) flags[k] |= 1u << v.flag_bit;
       }
+    }
+    uint32_t part[kScatterItems], rank[kScatterItems];
+#pragma unroll
+    for (int k = 0; k < kScatterItems; ++k) {
+      part[k] = partition_of(h[k], pp.log2p);
+      rank[k] = present[k] ? atomicAdd(&tcount[part[k]], 1u) : 0u;
     }
     __syncthreads();
     // tile-local start of each partition's run
@@ -438,22 +465,51 @@ __global__ __launch_bounds__(kLdsBlock) void part_scatter_tiled_kernel(const Par
       tile_rows += chunk_total;
       __syncthreads();
     }
-    // stage rows in partition order
+    // stage rows in partition order, column by column
+    uint32_t pos[kScatterItems];
+#pragma unroll
     for (int k = 0; k < kScatterItems; ++k) {
-      if (part[k] == 0xffffffffu) continue;
-      const int64_t row = tile + k * kLdsBlock + threadIdx.x;
-      const uint32_t pos = tcount[part[k]] + rank[k];
-      pid[pos] = (uint16_t)part[k];
-      write_record(pp, reinterpret_cast<uint8_t*>(stage) + (int64_t)pos * pp.rec_bytes, row);
+      pos[k] = present[k] ? tcount[part[k]] + rank[k] : 0u;
+      if (present[k]) pid[pos[k]] = (uint16_t)part[k];
+    }
+#pragma unroll 1
+    for (int c = 0; c < p.num_keys; ++c) {
+      const KeyCol& kc = p.keys[c];
+      uint64_t x[kScatterItems];
+      if (c == 0) {
+#pragma unroll
+        for (int k = 0; k < kScatterItems; ++k) x[k] = key0[k];
+      } else {
+        load_rows(kc.data, kc.width, kc.stride, VC_UNSIGNED, row0, step, present, x);
+      }
+      store_rows(stage_bytes, pos, pp.rec_bytes, pp.key_off[c], kc.width, x, present);
+    }
+#pragma unroll 1
+    for (int c = 0; c < p.num_vals; ++c) {
+      const ValCol& v = p.vals[c];
+      uint64_t x[kScatterItems];
+      switch (c) {
+        case 0: copy_rows(vpre[0], x); break;
+        case 1: copy_rows(vpre[1], x); break;
+        case 2: copy_rows(vpre[2], x); break;
+        default: load_rows(v.data, v.width, v.stride, VC_UNSIGNED, row0, step, present, x); break;
+      }
+      store_rows(stage_bytes, pos, pp.rec_bytes, pp.val_off[c], v.width, x, present);
+    }
+    if (pp.flag_bytes) {
+      uint64_t f[kScatterItems];
+#pragma unroll
+      for (int k = 0; k < kScatterItems; ++k) f[k] = flags[k];
+      store_rows(stage_bytes, pos, pp.rec_bytes, pp.flags_off, pp.flag_bytes, f, present);
     }
     __syncthreads();
     // copy out word by word: the words of one partition's run are contiguous
     const uint32_t words = tile_rows * (uint32_t)rec_words;
     for (uint32_t w = threadIdx.x; w < words; w += kLdsBlock) {
-      const uint32_t pos = w / (uint32_t)rec_words;
-      const uint32_t q = pid[pos];
-      const int64_t dst = (int64_t)cursor[q] + (pos - tcount[q]);
-      out[dst * rec_words + (w - pos * (uint32_t)rec_words)] = stage[w];
+      const uint32_t r = (uint32_t)(((uint64_t)w * pp.rec_magic) >> 32);  // w / rec_words
+      const uint32_t q = pid[r];
+      const int64_t dst = (int64_t)cursor[q] + (r - tcount[q]);
+      out[dst * rec_words + (w - r * (uint32_t)rec_words)] = stage[w];
     }
     __syncthreads();
     // advance cursors by this tile's runs and clear the counts
@@ -536,14 +592,6 @@ __global__ __launch_bounds__(256) void scan_apply_u32_kernel(const uint32_t* in,
   }
 }
 
-// A row whose group did not fit the LDS table: device atomics on the global
-// table. Kept out of line so that the LDS path's record pointer stays an LDS
-// pointer (ds_* atomics) in the caller.
-__device__ __forceinline__ void spill_row(const AggParams& p, const RowKey& rk, int64_t row)
-{
-  uint8_t* g = global_find_or_insert(p, rk, row);
-  if (g) apply_row(p, g, row);
-}
 
 // ---- LDS aggregation of one work item
 
@@ -601,108 +649,11 @@ __global__ __launch_bounds__(kLdsBlock) void lds_agg_kernel(const LdsParams lp)
     merge = n_items > 1;
   }
 
-  LdsTable t;
-  t.rec = lds;
-  t.slots = lp.slots;
-  t.stride = lp.words * 8;  // LDS record stride (the record padded to 8 x odd bytes)
-  t.cap = lp.cap;
-  int32_t* ctr = reinterpret_cast<int32_t*>(lds + (int64_t)(lp.slots + 1) * t.stride);
-  t.count = ctr;
-  t.overflow = ctr + 1;
-  int32_t* scan = ctr + 4;  // kLdsBlock ints
-
-  uint64_t* w64 = reinterpret_cast<uint64_t*>(lds);
-  const int total_words = (lp.slots + 1) * lp.words;
-  for (int i = threadIdx.x; i < total_words; i += kLdsBlock) {
-    const int wi = i % lp.words;
-    uint64_t v = lp.tmpl[wi];
-    if (wi == 0 && i / lp.words == lp.slots) v = 0ull;  // the special record starts unoccupied
-    w64[i] = v;
-  }
-  if (threadIdx.x < 4) ctr[threadIdx.x] = 0;
-  __syncthreads();
-
-  // Software pipelined in groups of kGroup rows per thread: the loads of the
-  // next group (key, flags, first kPre values of each row) are issued before
-  // the current group's probes and LDS atomics, keeping ~kGroup rows per
-  // thread in flight against HBM latency at one workgroup per CU.
-  constexpr int kGroup = 2;
-  constexpr int64_t kSpan = (int64_t)kGroup * kLdsBlock;
-  RowPre cur[kGroup], nxt[kGroup];
-  const int64_t first = begin + threadIdx.x;
-#pragma unroll
-  for (int u = 0; u < kGroup; ++u)
-    if (first + u * kLdsBlock < end) preload_row(p, first + u * kLdsBlock, cur[u]);
-  for (int64_t base = first; base < end; base += kSpan) {
-#pragma unroll
-    for (int u = 0; u < kGroup; ++u)
-      if (base + kSpan + u * kLdsBlock < end) preload_row(p, base + kSpan + u * kLdsBlock, nxt[u]);
-#pragma unroll
-    for (int u = 0; u < kGroup; ++u) {
-      const int64_t row = base + u * kLdsBlock;
-      if (row >= end) break;
-      if (lp.direct && !key_row_valid(p, row)) continue;
-      const RowKey rk = row_key_pre(p, row, cur[u]);
-      const uint32_t slot_hash = rk.hash << lp.log2p;
-      uint8_t* lrec = lds_find_or_insert(t, p, rk, slot_hash, row);
-      if (lrec) {
-        apply_row_pre(p, lrec, row, cur[u]);  // LDS atomics
-      } else {
-        // spill: a separate call keeps the two pointers' address spaces apart
-        spill_row(p, rk, row);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kGroup; ++u) cur[u] = nxt[u];
-  }
-  __syncthreads();
-
-  if (merge || *t.overflow) {
-    // fold this item's groups into the global table
-    for (int i = threadIdx.x; i <= lp.slots; i += kLdsBlock) {
-      if (!lds_occupied(t, i, p.key_mode)) continue;
-      const uint8_t* rec = lds_record(t, i);
-      const uint64_t hdr = *reinterpret_cast<const uint64_t*>(rec);
-      RowKey rk;
-      int64_t rep = 0;
-      if (p.key_mode == KEY_INT) {
-        const KeyCol& k = p.keys[0];
-        rk.special = i == lp.slots;
-        rk.key = rk.special ? kEmptyIntKey : hdr;
-        const uint64_t raw = k.width == 8 ? rk.key : (rk.key & ((1ull << (8 * k.width)) - 1ull));
-        rk.hash = hash_combine(0u, murmur3_bits(normalize_float_bits(raw, k.vclass), k.width, 0));
-      } else {
-        rep = (int64_t)(hdr & 0xffffffffull);
-        rk = row_key(p, rep);
-      }
-      uint8_t* g = global_find_or_insert(p, rk, rep);
-      if (g) merge_record(p, g, rec);
-    }
-    return;
-  }
-
-  int mine = 0;
-  for (int i = threadIdx.x; i <= lp.slots; i += kLdsBlock) mine += lds_occupied(t, i, p.key_mode) ? 1 : 0;
-  scan[threadIdx.x] = mine;
-  __syncthreads();
-  for (int d = 1; d < kLdsBlock; d <<= 1) {
-    int v = threadIdx.x >= (unsigned)d ? scan[threadIdx.x - d] : 0;
-    __syncthreads();
-    scan[threadIdx.x] += v;
-    __syncthreads();
-  }
-  if (threadIdx.x == kLdsBlock - 1)
-    ctr[2] = (int32_t)atomicAdd(&lp.out_counters[0], (unsigned long long)scan[kLdsBlock - 1]);
-  __syncthreads();
-  int64_t pos = (int64_t)ctr[2] + scan[threadIdx.x] - mine;
-  const OutTable& ot = *lp.out;
-  for (int i = threadIdx.x; i <= lp.slots; i += kLdsBlock) {
-    if (!lds_occupied(t, i, p.key_mode)) continue;
-    const uint8_t* rec = lds_record(t, i);
-    finalize_keys(*reinterpret_cast<const uint64_t*>(rec), i == lp.slots, p.key_mode, p.keys, p.num_keys, pos, ot);
-    finalize_record(rec, pos, ot);
-    ++pos;
-  }
+  const LdsTable t = lds_table_at(lds, lp.slots, lp.words, lp.cap);
+  lds_table_init(t, lp.tmpl, lp.words);
+  lds_aggregate_rows(p, t, lds, begin, end, lp.log2p, lp.direct != 0);
+  if (merge || *t.overflow) lds_fold_to_global(p, t);
+  else lds_write_groups(p, t, &lp.out_counters[0], 0, *lp.out);
 }
 
 // ---- table hashing entry points
@@ -764,8 +715,6 @@ __global__ __launch_bounds__(kBlock) void spark_murmur_table_kernel(const HashPa
 
 // =================================================================== host
 
-namespace {
-
 thread_local std::string g_last_error;
 
 mgb_status fail(mgb_status s, std::string msg)
@@ -773,12 +722,6 @@ mgb_status fail(mgb_status s, std::string msg)
   g_last_error = std::move(msg);
   return s;
 }
-
-#define MGB_HIP(call)                                                                             \
-  do {                                                                                            \
-    hipError_t e_ = (call);                                                                       \
-    if (e_ != hipSuccess) return fail(MGB_ERROR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
 int type_width(int t)
 {
@@ -810,8 +753,6 @@ int value_class(int t)
   return VC_SIGNED;  // signed ints, timestamps, durations
 }
 
-}  // namespace
-
 // cudf::detail::target_type (aggregation.hpp:1168-1424) for fixed-width types.
 int32_t target_type(int32_t t, int32_t kind)
 {
@@ -839,67 +780,12 @@ int32_t target_type(int32_t t, int32_t kind)
   }
 }
 
-namespace {
-
-// ------------------------------------------------------------ profiling
-
-struct ProfRecord {
-  std::string kernel;
-  hipEvent_t start, stop;
-};
 std::mutex g_prof_mu;
 bool g_prof_enabled = false;
 std::vector<ProfRecord> g_prof;
 
-// Brackets one kernel launch with events when profiling is on.
-struct ProfScope {
-  hipEvent_t start = nullptr, stop = nullptr;
-  hipStream_t stream;
-  const char* name;
-  ProfScope(const char* kernel, hipStream_t s) : stream{s}, name{kernel}
-  {
-    std::lock_guard<std::mutex> lk(g_prof_mu);
-    if (!g_prof_enabled) return;
-    if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
-      start = stop = nullptr;
-      return;
-    }
-    (void)hipEventRecord(start, stream);
-  }
-  ~ProfScope()
-  {
-    if (!start) return;
-    (void)hipEventRecord(stop, stream);
-    std::lock_guard<std::mutex> lk(g_prof_mu);
-    g_prof.push_back(ProfRecord{name, start, stop});
-  }
-};
 
-// ------------------------------------------------------------ workspace
 
-struct DeviceBuffer {
-  void* ptr = nullptr;
-  size_t bytes = 0;
-  hipError_t reserve(size_t need, hipStream_t stream)
-  {
-    if (need <= bytes) return hipSuccess;
-    if (ptr) {
-      hipError_t e = hipStreamSynchronize(stream);
-      if (e != hipSuccess) return e;
-      (void)hipFree(ptr);
-      ptr = nullptr;
-      bytes = 0;
-    }
-    size_t want = std::max(need, bytes + bytes / 4);
-    hipError_t e = hipMalloc(&ptr, want);
-    if (e != hipSuccess) {
-      ptr = nullptr;
-      return e;
-    }
-    bytes = want;
-    return hipSuccess;
-  }
-};
 
 constexpr size_t kStagingBytes = 64 * 1024;  // pinned parameter staging (kUploadSlots slots)
 constexpr int kUploadSlots = 8;
@@ -935,8 +821,6 @@ Workspace* workspace_for(hipStream_t s)
   return w.get();
 }
 
-size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
-constexpr size_t align_up_c(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 constexpr int kMaxPinned = 1024;  // host readback slots (counters, null counts)
 
@@ -951,53 +835,7 @@ int device_cu_count()
   return cus;
 }
 
-int64_t bitmask_bytes(int64_t n) { return (int64_t)align_up((size_t)((n + 31) / 32) * 4, 64); }
 
-// ------------------------------------------------------------ plan
-
-struct ValKey {
-  const void* data;
-  const uint32_t* mask;
-  int64_t offset, size;
-  int32_t type;
-  bool operator<(const ValKey& o) const
-  {
-    return std::tie(data, mask, offset, size, type) < std::tie(o.data, o.mask, o.offset, o.size, o.type);
-  }
-};
-
-struct PlanCol {
-  ValCol col;               // op_begin/op_end filled when batching
-  int type;
-  bool nullable;
-  std::vector<std::pair<int, int>> ops;  // (op kind, byte offset)
-  int count_off = -1;       // the row-count field
-  int nulls_off = -1;       // OP_NULLS field when the column has nulls
-  int op_offset(int kind) const
-  {
-    for (auto& o : ops)
-      if (o.first == kind) return o.second;
-    return -1;
-  }
-};
-
-struct Plan {
-  std::vector<KeyCol> keys;
-  int key_mode = KEY_INT;
-  bool skip_null_keys = false;
-  bool compare_key_nulls = false;
-  bool key_has_nulls = false;
-  std::vector<PlanCol> cols;
-  int rows_off = -1;
-  int stride = 8;
-  struct Out {
-    int col;        // index into cols
-    int kind;       // aggregation kind
-    int out_type;
-    OutSpec spec;
-  };
-  std::vector<Out> outs;
-};
 
 int sum_op_for(int t) { return is_float_t(t) ? OP_SUM_F64 : OP_SUM_I64; }
 
@@ -1185,27 +1023,9 @@ void record_template(const Plan& plan, uint64_t header, uint64_t* words)
   std::memcpy(words, bytes.data(), (size_t)plan.stride);
 }
 
-struct Allocs {
-  const mgb_allocator* alloc;
-  hipStream_t stream;
-  std::vector<std::pair<void*, size_t>> live;
-  void* get(size_t bytes)
-  {
-    void* p = alloc->allocate(alloc->ctx, bytes, (void*)stream);
-    if (p) live.push_back({p, bytes});
-    return p;
-  }
-  void release_all()
-  {
-    for (auto& l : live) alloc->deallocate(alloc->ctx, l.first, l.second, (void*)stream);
-    live.clear();
-  }
-};
-
-}  // namespace
 
 
-namespace {
+
 
 // Temporary output columns in the workspace (filled by finalize / LDS
 // kernels, then copied into the caller's allocations).
@@ -1378,11 +1198,6 @@ size_t lds_limit()
   return lim;
 }
 
-struct LdsGeom {
-  int slots = 0, cap = 0;
-  size_t bytes = 0;
-  bool ok = false;
-};
 
 // LDS record stride: the record padded to 8 x (odd number) bytes, so that the
 // same field of records at random slots spreads over 32 of the 64 LDS banks
@@ -1766,6 +1581,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     pp.offsets = reinterpret_cast<const uint32_t*>(wsb + offs_off);
     pp.out = wsb + recs_off;
     pp.rec_bytes = rec_bytes;
+    pp.rec_magic = ((1ull << 32) + (uint64_t)(rec_bytes / 8) - 1) / (uint64_t)(rec_bytes / 8);
     pp.flag_bytes = flag_bytes;
     pp.flags_off = flags_rec_off;
     for (size_t c = 0; c < plan.keys.size(); ++c) pp.key_off[c] = koff[c];
@@ -1907,7 +1723,6 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
   return MGB_SUCCESS;
 }
 
-}  // namespace
 
 mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mgb_out_column* out_results,
                        int64_t* out_num_groups, hipStream_t stream, const mgb_allocator* alloc)

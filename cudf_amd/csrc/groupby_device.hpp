@@ -353,6 +353,310 @@ __device__ __forceinline__ bool lds_occupied(const LdsTable& t, int32_t i, int k
   return hdr != (key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef);
 }
 
+// ------------------------------------------------------------- batched LDS updates
+// A thread takes N rows at a time (row0 + u * step) and updates their LDS
+// records column by column: the dispatch on a column's width, class and ops
+// is uniform across the wavefront and is paid once per column for all N rows
+// instead of once per row and field.
+
+constexpr uint32_t kRecNone = 0xffffffffu;   // row not aggregated (past the end, or a dropped null key)
+constexpr uint32_t kRecSpill = 0xfffffffeu;  // the row's group did not fit the LDS table
+
+template <typename T>
+__device__ __forceinline__ void lds_add(T* p, T v)
+{
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void lds_min(T* p, T v)
+{
+  __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void lds_max(T* p, T v)
+{
+  __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Elements row0 + u * step (u < N, where present[u]) of a column, in the
+// canonical form the ops take: integers widened to 64 bits (sign- or
+// zero-extended, bool -> 0/1), floats as their raw bits. Absent rows read 0.
+template <int N>
+__device__ __forceinline__ void load_rows(const uint8_t* data, int width, int stride, int vclass, int64_t row0,
+                                          int64_t step, const bool (&present)[N], uint64_t (&x)[N])
+{
+  const bool sgn = vclass == VC_SIGNED;
+  switch (width) {
+    case 1:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        const uint8_t b = present[u] ? data[(row0 + u * step) * stride] : 0;
+        x[u] = sgn ? (uint64_t)(int64_t)(int8_t)b : vclass == VC_BOOL ? (uint64_t)(b != 0) : (uint64_t)b;
+      }
+      break;
+    case 2:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        const uint16_t b = present[u] ? *reinterpret_cast<const uint16_t*>(data + (row0 + u * step) * stride) : 0;
+        x[u] = sgn ? (uint64_t)(int64_t)(int16_t)b : (uint64_t)b;
+      }
+      break;
+    case 4:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        const uint32_t b = present[u] ? *reinterpret_cast<const uint32_t*>(data + (row0 + u * step) * stride) : 0u;
+        x[u] = sgn ? (uint64_t)(int64_t)(int32_t)b : (uint64_t)b;
+      }
+      break;
+    default:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        x[u] = present[u] ? *reinterpret_cast<const uint64_t*>(data + (row0 + u * step) * stride) : 0ull;
+      break;
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void copy_rows(const uint64_t (&from)[N], uint64_t (&to)[N])
+{
+#pragma unroll
+  for (int u = 0; u < N; ++u) to[u] = from[u];
+}
+
+// Writes the low `width` bytes of x[u] at base + pos[u] * rec_bytes + off for
+// the present rows.
+template <int N>
+__device__ __forceinline__ void store_rows(uint8_t* base, const uint32_t (&pos)[N], int rec_bytes, int off, int width,
+                                           const uint64_t (&x)[N], const bool (&present)[N])
+{
+  switch (width) {
+    case 1:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (present[u]) base[pos[u] * rec_bytes + off] = (uint8_t)x[u];
+      break;
+    case 2:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (present[u]) *reinterpret_cast<uint16_t*>(base + pos[u] * rec_bytes + off) = (uint16_t)x[u];
+      break;
+    case 4:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (present[u]) *reinterpret_cast<uint32_t*>(base + pos[u] * rec_bytes + off) = (uint32_t)x[u];
+      break;
+    default:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (present[u]) *reinterpret_cast<uint64_t*>(base + pos[u] * rec_bytes + off) = x[u];
+      break;
+  }
+}
+
+// Row hashes (row_hash / row_key) of N rows, one key column at a time; the
+// raw bits of the first key column are kept in key0.
+template <int N>
+__device__ __forceinline__ void hash_rows(const AggParams& p, int64_t row0, int64_t step, const bool (&present)[N],
+                                          uint32_t (&h)[N], uint64_t (&key0)[N])
+{
+  const bool check_nulls = p.key_mode != KEY_INT && p.compare_key_nulls != 0;
+#pragma unroll
+  for (int u = 0; u < N; ++u) h[u] = 0u;
+#pragma unroll 1
+  for (int c = 0; c < p.num_keys; ++c) {
+    const KeyCol& k = p.keys[c];
+    uint64_t raw[N];
+    load_rows(k.data, k.width, k.stride, VC_UNSIGNED, row0, step, present, raw);
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      uint32_t e;
+      if (check_nulls && present[u] && !bit_valid(k.mask, k.bit_off + row0 + u * step)) e = 0xffffffffu;
+      else e = murmur3_bits(normalize_float_bits(raw[u], k.vclass), k.width, 0);
+      h[u] = hash_combine(h[u], e);
+    }
+    if (c == 0) {
+#pragma unroll
+      for (int u = 0; u < N; ++u) key0[u] = raw[u];
+    }
+  }
+}
+
+__device__ __forceinline__ double canon_f64(uint64_t x, int vclass)
+{
+  return vclass == VC_F32 ? (double)__uint_as_float((uint32_t)x) : __longlong_as_double((long long)x);
+}
+
+// One accumulator op on the LDS records of N rows (ok[u]: the row has a
+// record and a valid value). x[u] is the canonical value (see load_rows).
+template <int N>
+__device__ __forceinline__ void lds_op_rows(uint8_t* lds, const uint32_t (&loff)[N], const uint64_t (&x)[N],
+                                            const bool (&ok)[N], int kind, int off, int vclass)
+{
+  switch (kind) {
+    case OP_ROWS: case OP_COUNT:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_add(reinterpret_cast<uint32_t*>(lds + loff[u] + off), 1u);
+      break;
+    case OP_SUM_I64:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_add(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), (unsigned long long)x[u]);
+      break;
+    case OP_SUM_F64:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_add(reinterpret_cast<double*>(lds + loff[u] + off), canon_f64(x[u], vclass));
+      break;
+    case OP_SUMSQ_I64:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_add(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), (unsigned long long)(x[u] * x[u]));
+      break;
+    case OP_SUMSQ_F64:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        if (!ok[u]) continue;
+        double sq;
+        if (vclass == VC_F32) {
+          const float f = __uint_as_float((uint32_t)x[u]);
+          sq = (double)(f * f);
+        } else {
+          const double d = __longlong_as_double((long long)x[u]);
+          sq = d * d;
+        }
+        lds_add(reinterpret_cast<double*>(lds + loff[u] + off), sq);
+      }
+      break;
+    case OP_PROD_I64: case OP_PROD_F64:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        if (!ok[u]) continue;
+        unsigned long long* q = reinterpret_cast<unsigned long long*>(lds + loff[u] + off);
+        unsigned long long old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (;;) {
+          unsigned long long nxt;
+          if (kind == OP_PROD_I64) {
+            nxt = old * (unsigned long long)x[u];
+          } else if (vclass == VC_F32) {
+            nxt = (unsigned long long)__double_as_longlong(
+                (double)((float)__longlong_as_double((long long)old) * __uint_as_float((uint32_t)x[u])));
+          } else {
+            nxt = (unsigned long long)__double_as_longlong(__longlong_as_double((long long)old) *
+                                                          __longlong_as_double((long long)x[u]));
+          }
+          unsigned long long expect = old;
+          if (__hip_atomic_compare_exchange_strong(q, &expect, nxt, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP))
+            break;
+          old = expect;
+        }
+      }
+      break;
+    case OP_MIN_I32:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_min(reinterpret_cast<int32_t*>(lds + loff[u] + off), (int32_t)x[u]);
+      break;
+    case OP_MAX_I32:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_max(reinterpret_cast<int32_t*>(lds + loff[u] + off), (int32_t)x[u]);
+      break;
+    case OP_MIN_U32:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_min(reinterpret_cast<uint32_t*>(lds + loff[u] + off), (uint32_t)x[u]);
+      break;
+    case OP_MAX_U32:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_max(reinterpret_cast<uint32_t*>(lds + loff[u] + off), (uint32_t)x[u]);
+      break;
+    case OP_MIN_I64:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_min(reinterpret_cast<long long*>(lds + loff[u] + off), (long long)x[u]);
+      break;
+    case OP_MAX_I64:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_max(reinterpret_cast<long long*>(lds + loff[u] + off), (long long)x[u]);
+      break;
+    case OP_MIN_U64:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_min(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), (unsigned long long)x[u]);
+      break;
+    case OP_MAX_U64:
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        if (ok[u]) lds_max(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), (unsigned long long)x[u]);
+      break;
+    case OP_MIN_F32:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        if (!ok[u]) continue;
+        const uint32_t b = (uint32_t)x[u];
+        const bool nan = (b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu);
+        lds_min(reinterpret_cast<uint32_t*>(lds + loff[u] + off), nan ? kF32MinNaN : f32_code(b));
+      }
+      break;
+    case OP_MAX_F32:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        if (!ok[u]) continue;
+        const uint32_t b = (uint32_t)x[u];
+        const bool nan = (b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu);
+        lds_max(reinterpret_cast<uint32_t*>(lds + loff[u] + off), nan ? kF32MaxNaN : f32_code(b));
+      }
+      break;
+    case OP_MIN_F64:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        if (!ok[u]) continue;
+        const unsigned long long b = x[u];
+        const bool nan = (b & 0x7ff0000000000000ull) == 0x7ff0000000000000ull && (b & 0x000fffffffffffffull);
+        lds_min(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), nan ? kF64MinNaN : f64_code(b));
+      }
+      break;
+    case OP_MAX_F64:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        if (!ok[u]) continue;
+        const unsigned long long b = x[u];
+        const bool nan = (b & 0x7ff0000000000000ull) == 0x7ff0000000000000ull && (b & 0x000fffffffffffffull);
+        lds_max(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), nan ? kF64MaxNaN : f64_code(b));
+      }
+      break;
+    default: break;  // OP_NULLS: updated for null values by the caller
+  }
+}
+
+// All ops of one value column on the LDS records of N rows. Validity comes
+// from the column's null mask (input rows) or from bit `flag_bit` of the
+// row's flags word (partitioned records).
+template <int N>
+__device__ __forceinline__ void lds_column_rows(const AggParams& p, uint8_t* lds, const ValCol& v,
+                                                const uint32_t (&loff)[N], const uint64_t (&x)[N],
+                                                const uint32_t (&flags)[N], int64_t row0, int64_t step)
+{
+  bool ok[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    ok[u] = loff[u] < kRecSpill;
+    if (!ok[u]) continue;
+    if (v.mask) ok[u] = bit_valid(v.mask, v.bit_off + row0 + u * step);
+    else if (v.flag_bit >= 0) ok[u] = (flags[u] >> v.flag_bit) & 1u;
+  }
+  if (v.nulls_off >= 0) {
+#pragma unroll
+    for (int u = 0; u < N; ++u)
+      if (loff[u] < kRecSpill && !ok[u]) lds_add(reinterpret_cast<uint32_t*>(lds + loff[u] + v.nulls_off), 1u);
+  }
+  for (int o = v.op_begin; o < v.op_end; ++o) lds_op_rows(lds, loff, x, ok, p.ops[o].kind, p.ops[o].off, v.vclass);
+}
+
 // ------------------------------------------------------------- finalisation
 
 __device__ __forceinline__ void store_width(void* out, int64_t pos, int width, uint64_t bits)

@@ -91,6 +91,7 @@ struct PartParams {
   int32_t flag_bytes;             // 0, 1, 2 or 4
   int32_t flags_off;              // offset of the validity bits in a record
   int32_t tile_rows;              // rows staged per tile (tiled scatter)
+  uint64_t rec_magic;             // ceil(2^32 / (rec_bytes / 8)): word index -> record index by multiply-shift
   int32_t key_off[kMaxKeys];      // offset of each key column in a record
   int32_t val_off[kMaxVals];      // offset of each value column in a record
 };

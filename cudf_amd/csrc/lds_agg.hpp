@@ -1,0 +1,242 @@
+// Device routines of LDS aggregation work items (one workgroup of kLdsBlock
+// threads per item): set-up of the item's LDS table, aggregation of a range
+// of rows into it, and the two ways an item ends: folding its groups into a
+// global table, or writing them as output rows.
+#pragma once
+
+#include "groupby_host.hpp"
+
+namespace mgb {
+
+// A row whose group did not fit the LDS table: device atomics on the global
+// table. A call of its own, so that the LDS path's record pointer and this
+// global one never meet in one variable (which would make both flat).
+__device__ __forceinline__ void spill_row(const AggParams& p, const RowKey& rk, int64_t row)
+{
+  uint8_t* g = global_find_or_insert(p, rk, row);
+  if (g) apply_row(p, g, row);
+}
+
+// The LDS table of a work item: `slots` records of `words` 8-byte words, one
+// record for the 8-byte key equal to the empty header, then counters
+// (count, overflow, a 64-bit output base) and kLdsBlock ints of scan space.
+__device__ __forceinline__ LdsTable lds_table_at(uint8_t* lds, int slots, int words, int cap)
+{
+  LdsTable t;
+  t.rec = lds;
+  t.slots = slots;
+  t.stride = words * 8;  // LDS record stride (the record padded to 8 x odd bytes)
+  t.cap = cap;
+  int32_t* ctr = reinterpret_cast<int32_t*>(lds + (int64_t)(slots + 1) * t.stride);
+  t.count = ctr;
+  t.overflow = ctr + 1;
+  return t;
+}
+
+// LDS bytes of a table of `slots` records of `words` words.
+__host__ __device__ constexpr size_t lds_table_bytes(int slots, int words)
+{
+  return (size_t)(slots + 1) * words * 8 + 16 + (size_t)kLdsBlock * 4;
+}
+
+// Writes the identity record into every slot (the special record starts
+// unoccupied) and clears the counters.
+__device__ __forceinline__ void lds_table_init(const LdsTable& t, const uint64_t* tmpl, int words)
+{
+  uint64_t* w64 = reinterpret_cast<uint64_t*>(t.rec);
+  const int total_words = (t.slots + 1) * words;
+  for (int i = threadIdx.x; i < total_words; i += kLdsBlock) {
+    const int wi = i % words;
+    uint64_t v = tmpl[wi];
+    if (wi == 0 && i / words == t.slots) v = 0ull;
+    w64[i] = v;
+  }
+  if (threadIdx.x < 4) t.count[threadIdx.x] = 0;
+  __syncthreads();
+}
+
+// Aggregates rows [begin, end) of `p` into the LDS table. `log2p` is the
+// number of hash bits that chose the rows' partition (the slot comes from the
+// bits below them); rows with a null key are dropped when `check_keys`.
+// Ends with a workgroup barrier.
+__device__ __forceinline__ void lds_aggregate_rows(const AggParams& p, const LdsTable& t, uint8_t* lds, int64_t begin,
+                                                   int64_t end, int log2p, bool check_keys)
+{
+  // Rows are taken kRows per thread at a time (rows base + u * kLdsBlock):
+  // the loads of the key, the flags and the first kPreCols value columns of
+  // all of them are issued together, then the rows are probed (the home slot
+  // headers of all rows read together, the full probe only on a miss), then
+  // each value column updates the rows' records (lds_column_rows). Rows whose
+  // group does not fit go to the global table afterwards.
+  constexpr int kRows = 4;
+  constexpr int kPreCols = 3;  // the switch below picks pre[0..2]
+  const int64_t step = kLdsBlock;
+  const KeyCol& k0 = p.keys[0];
+  const bool int_key = p.key_mode == KEY_INT;
+  for (int64_t base = begin + threadIdx.x; base < end; base += (int64_t)kRows * kLdsBlock) {
+    bool present[kRows];
+#pragma unroll
+    for (int u = 0; u < kRows; ++u) present[u] = base + u * step < end;
+    uint64_t key[kRows];
+    if (int_key) load_rows(k0.data, k0.width, k0.stride, VC_UNSIGNED, base, step, present, key);
+    uint32_t flags[kRows];
+    {
+      uint64_t f[kRows];
+      if (p.flags) load_rows(p.flags, p.flag_bytes, p.flag_stride, VC_UNSIGNED, base, step, present, f);
+#pragma unroll
+      for (int u = 0; u < kRows; ++u) flags[u] = p.flags ? (uint32_t)f[u] : 0u;
+    }
+    uint64_t pre[kPreCols][kRows];
+#pragma unroll
+    for (int c = 0; c < kPreCols; ++c)
+      if (c < p.num_vals)
+        load_rows(p.vals[c].data, p.vals[c].width, p.vals[c].stride, p.vals[c].vclass, base, step, present, pre[c]);
+    if (check_keys && p.skip_null_keys) {
+#pragma unroll
+      for (int u = 0; u < kRows; ++u)
+        if (present[u]) present[u] = key_row_valid(p, base + u * step);
+    }
+
+    // probe: the home slot and the next one of every row are read together;
+    // rows found in neither (new groups, longer probe runs, row-reference
+    // keys) take the full probe one row at a time
+    uint32_t loff[kRows];
+    uint32_t miss = 0;
+    if (int_key) {
+      uint64_t kk[kRows], h0[kRows], h1[kRows];
+      uint32_t i0[kRows], i1[kRows];
+#pragma unroll
+      for (int u = 0; u < kRows; ++u) {
+        kk[u] = (uint64_t)widen_int(key[u], k0.width, k0.vclass);
+        const uint32_t hsh = hash_combine(0u, murmur3_bits(normalize_float_bits(key[u], k0.vclass), k0.width, 0));
+        i0[u] = (uint32_t)reduce_range(hsh << log2p, t.slots);
+        i1[u] = i0[u] + 1 == (uint32_t)t.slots ? 0u : i0[u] + 1;
+        h0[u] = h1[u] = kEmptyIntKey;
+        if (present[u]) {
+          h0[u] = __hip_atomic_load(reinterpret_cast<unsigned long long*>(lds_record(t, i0[u])), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+          h1[u] = __hip_atomic_load(reinterpret_cast<unsigned long long*>(lds_record(t, i1[u])), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kRows; ++u) {
+        loff[u] = kRecNone;
+        if (!present[u]) continue;
+        if (kk[u] != kEmptyIntKey && h0[u] == kk[u]) loff[u] = i0[u] * (uint32_t)t.stride;
+        else if (kk[u] != kEmptyIntKey && h0[u] != kEmptyIntKey && h1[u] == kk[u]) loff[u] = i1[u] * (uint32_t)t.stride;
+        else miss |= 1u << u;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kRows; ++u) {
+        loff[u] = kRecNone;
+        if (present[u]) miss |= 1u << u;
+      }
+    }
+#pragma unroll 1
+    while (miss) {
+      const int u = __builtin_ctz(miss);
+      miss &= miss - 1;
+      const int64_t row = base + u * step;
+      const RowKey rk = row_key(p, row);
+      uint8_t* r = lds_find_or_insert(t, p, rk, rk.hash << log2p, row);
+      const uint32_t v = r ? (uint32_t)(r - lds) : kRecSpill;
+#pragma unroll
+      for (int q = 0; q < kRows; ++q)
+        if (q == u) loff[q] = v;
+    }
+
+    // updates
+    if (p.rows_op >= 0) {
+      const int off = p.ops[p.rows_op].off;
+#pragma unroll
+      for (int u = 0; u < kRows; ++u)
+        if (loff[u] < kRecSpill) lds_add(reinterpret_cast<uint32_t*>(lds + loff[u] + off), 1u);
+    }
+    // one call site for all columns keeps the kernel's code small; the
+    // preloaded values are picked by a uniform switch
+#pragma unroll 1
+    for (int c = 0; c < p.num_vals; ++c) {
+      uint64_t x[kRows];
+      switch (c) {
+        case 0: copy_rows(pre[0], x); break;
+        case 1: copy_rows(pre[1], x); break;
+        case 2: copy_rows(pre[2], x); break;
+        default:
+          load_rows(p.vals[c].data, p.vals[c].width, p.vals[c].stride, p.vals[c].vclass, base, step, present, x);
+          break;
+      }
+      lds_column_rows(p, lds, p.vals[c], loff, x, flags, base, step);
+    }
+
+    // groups that did not fit: the global table (a separate call keeps the
+    // two pointers' address spaces apart)
+    uint32_t spill = 0;
+#pragma unroll
+    for (int u = 0; u < kRows; ++u)
+      if (loff[u] == kRecSpill) spill |= 1u << u;
+#pragma unroll 1
+    while (spill) {
+      const int u = __builtin_ctz(spill);
+      spill &= spill - 1;
+      spill_row(p, row_key(p, base + u * step), base + u * step);
+    }
+  }
+  __syncthreads();
+}
+
+// Folds every group of the LDS table into p's global table (device atomics).
+__device__ __forceinline__ void lds_fold_to_global(const AggParams& p, const LdsTable& t)
+{
+  for (int i = threadIdx.x; i <= t.slots; i += kLdsBlock) {
+    if (!lds_occupied(t, i, p.key_mode)) continue;
+    const uint8_t* rec = lds_record(t, i);
+    const uint64_t hdr = *reinterpret_cast<const uint64_t*>(rec);
+    RowKey rk;
+    int64_t rep = 0;
+    if (p.key_mode == KEY_INT) {
+      const KeyCol& k = p.keys[0];
+      rk.special = i == t.slots;
+      rk.key = rk.special ? kEmptyIntKey : hdr;
+      const uint64_t raw = k.width == 8 ? rk.key : (rk.key & ((1ull << (8 * k.width)) - 1ull));
+      rk.hash = hash_combine(0u, murmur3_bits(normalize_float_bits(raw, k.vclass), k.width, 0));
+    } else {
+      rep = (int64_t)(hdr & 0xffffffffull);
+      rk = row_key(p, rep);
+    }
+    uint8_t* g = global_find_or_insert(p, rk, rep);
+    if (g) merge_record(p, g, rec);
+  }
+}
+
+// Writes every group of the LDS table as an output row: rows pos_base + n
+// onward, n taken from *out_counter (one atomic per workgroup).
+__device__ __forceinline__ void lds_write_groups(const AggParams& p, const LdsTable& t,
+                                                 unsigned long long* out_counter, int64_t pos_base, const OutTable& ot)
+{
+  int32_t* scan = t.count + 4;  // kLdsBlock ints
+  unsigned long long* base = reinterpret_cast<unsigned long long*>(t.count + 2);
+  int mine = 0;
+  for (int i = threadIdx.x; i <= t.slots; i += kLdsBlock) mine += lds_occupied(t, i, p.key_mode) ? 1 : 0;
+  scan[threadIdx.x] = mine;
+  __syncthreads();
+  for (int d = 1; d < kLdsBlock; d <<= 1) {
+    int v = threadIdx.x >= (unsigned)d ? scan[threadIdx.x - d] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  if (threadIdx.x == kLdsBlock - 1) *base = atomicAdd(out_counter, (unsigned long long)scan[kLdsBlock - 1]);
+  __syncthreads();
+  int64_t pos = pos_base + (int64_t)*base + scan[threadIdx.x] - mine;
+  for (int i = threadIdx.x; i <= t.slots; i += kLdsBlock) {
+    if (!lds_occupied(t, i, p.key_mode)) continue;
+    const uint8_t* rec = lds_record(t, i);
+    finalize_keys(*reinterpret_cast<const uint64_t*>(rec), i == t.slots, p.key_mode, p.keys, p.num_keys, pos, ot);
+    finalize_record(rec, pos, ot);
+    ++pos;
+  }
+}
+
+}  // namespace mgb
