@@ -5,10 +5,11 @@ the C ABI in include/mgb_groupby.h). This package is its Python binding,
 shaped like pylibcudf's groupby."""
 from . import aggregation, hashing
 from .column import Column, Table, pack_validity, unpack_validity
+from .batch import GroupByBatch
 from .groupby import GroupBy, GroupByRequest
 from .types import Kind, NullPolicy, Sorted, TypeId
 
 __all__ = [
-    "aggregation", "hashing", "Column", "Table", "GroupBy", "GroupByRequest", "Kind", "NullPolicy",
+    "aggregation", "hashing", "Column", "Table", "GroupBy", "GroupByBatch", "GroupByRequest", "Kind", "NullPolicy",
     "Sorted", "TypeId", "pack_validity", "unpack_validity",
 ]

@@ -65,6 +65,21 @@ class mgb_out_column(ctypes.Structure):
     ]
 
 
+class mgb_batch_options(ctypes.Structure):
+    _fields_ = [
+        ("force_mode", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+        ("direct_item_rows", ctypes.c_int64),
+        ("estimate_scale", ctypes.c_double),
+    ]
+
+
+class mgb_batch_stats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "num_queries", "direct_queries", "global_queries", "partitioned_queries", "num_items",
+        "output_rows_reserved", "device_bytes", "reruns")]
+
+
 # Every symbol include/mgb_groupby.h declares, with its ctypes signature.
 SIGNATURES = {
     "mgb_version": (ctypes.c_int32, []),
@@ -92,6 +107,30 @@ SIGNATURES = {
         ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)],
     ),
     "mgb_release_workspace": (ctypes.c_int, [ctypes.c_void_p]),
+    "mgb_batch_create": (
+        ctypes.c_int,
+        [
+            ctypes.POINTER(mgb_column), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(mgb_request), ctypes.c_int32,
+            ctypes.c_int32, ctypes.POINTER(mgb_batch_options), ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+        ],
+    ),
+    "mgb_batch_execute": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "mgb_batch_finish": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "mgb_batch_result": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(mgb_column), ctypes.POINTER(mgb_column),
+         ctypes.POINTER(ctypes.c_int64)],
+    ),
+    "mgb_batch_copy_result": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(mgb_out_column), ctypes.POINTER(mgb_out_column),
+         ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.POINTER(mgb_allocator)],
+    ),
+    "mgb_batch_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(mgb_batch_stats)]),
+    "mgb_batch_query_mode": (
+        ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)],
+    ),
+    "mgb_batch_destroy": (None, [ctypes.c_void_p]),
 }
 
 _lib = None

@@ -170,58 +170,6 @@ __device__ __forceinline__ void apply_row(const AggParams& p, uint8_t* rec, int6
   }
 }
 
-// Loads of one row issued ahead of its processing.
-constexpr int kPre = 4;
-struct RowPre {
-  uint64_t key;          // first key column (raw bits)
-  uint32_t flags;
-  uint64_t vals[kPre];   // raw bits of the first kPre value columns
-};
-
-__device__ __forceinline__ void preload_row(const AggParams& p, int64_t row, RowPre& r)
-{
-  const KeyCol& k = p.keys[0];
-  r.key = load_at(k.data, k.width, k.stride, row);
-  r.flags = p.flags ? load_flags(p.flags, p.flag_bytes, p.flag_stride, row) : 0u;
-#pragma unroll
-  for (int c = 0; c < kPre; ++c)
-    if (c < p.num_vals) r.vals[c] = load_at(p.vals[c].data, p.vals[c].width, p.vals[c].stride, row);
-}
-
-__device__ __forceinline__ RowKey row_key_pre(const AggParams& p, int64_t row, const RowPre& pre)
-{
-  if (p.key_mode != KEY_INT) return row_key(p, row);
-  const KeyCol& k = p.keys[0];
-  RowKey r;
-  r.key = (uint64_t)widen_int(pre.key, k.width, k.vclass);
-  r.hash = hash_combine(0u, murmur3_bits(normalize_float_bits(pre.key, k.vclass), k.width, 0));
-  r.special = r.key == kEmptyIntKey;
-  return r;
-}
-
-// apply_row with the row's loads already done (columns >= kPre load here).
-__device__ __forceinline__ void apply_row_pre(const AggParams& p, uint8_t* rec, int64_t row, const RowPre& pre)
-{
-  if (p.rows_op >= 0) apply_op(rec, OP_ROWS, p.ops[p.rows_op].off, 0, 4, VC_UNSIGNED);
-  for (int c = 0; c < p.num_vals; ++c) {
-    const ValCol& v = p.vals[c];
-    if (!value_valid(p, v, row, pre.flags)) {
-      if (v.nulls_off >= 0) atom_add(reinterpret_cast<uint32_t*>(rec + v.nulls_off), 1u);
-      continue;
-    }
-    uint64_t raw;
-    if (c < kPre) {
-      raw = 0;
-#pragma unroll
-      for (int q = 0; q < kPre; ++q)
-        if (q == c) raw = pre.vals[q];
-    } else {
-      raw = load_at(v.data, v.width, v.stride, row);
-    }
-    for (int o = v.op_begin; o < v.op_end; ++o) apply_op(rec, p.ops[o].kind, p.ops[o].off, raw, v.width, v.vclass);
-  }
-}
-
 // Folds one partial record (`src`) into `dst` with device atomics.
 __device__ __forceinline__ void merge_record(const AggParams& p, uint8_t* dst, const uint8_t* src)
 {
@@ -386,32 +334,35 @@ __device__ __forceinline__ void load_rows(const uint8_t* data, int width, int st
                                           int64_t step, const bool (&present)[N], uint64_t (&x)[N])
 {
   const bool sgn = vclass == VC_SIGNED;
+  const global_t<const uint8_t>* g = to_global(data);
   switch (width) {
     case 1:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
-        const uint8_t b = present[u] ? data[(row0 + u * step) * stride] : 0;
+        const uint8_t b = present[u] ? g[(row0 + u * step) * stride] : 0;
         x[u] = sgn ? (uint64_t)(int64_t)(int8_t)b : vclass == VC_BOOL ? (uint64_t)(b != 0) : (uint64_t)b;
       }
       break;
     case 2:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
-        const uint16_t b = present[u] ? *reinterpret_cast<const uint16_t*>(data + (row0 + u * step) * stride) : 0;
+        const uint16_t b =
+            present[u] ? *reinterpret_cast<const global_t<uint16_t>*>(g + (row0 + u * step) * stride) : 0;
         x[u] = sgn ? (uint64_t)(int64_t)(int16_t)b : (uint64_t)b;
       }
       break;
     case 4:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
-        const uint32_t b = present[u] ? *reinterpret_cast<const uint32_t*>(data + (row0 + u * step) * stride) : 0u;
+        const uint32_t b =
+            present[u] ? *reinterpret_cast<const global_t<uint32_t>*>(g + (row0 + u * step) * stride) : 0u;
         x[u] = sgn ? (uint64_t)(int64_t)(int32_t)b : (uint64_t)b;
       }
       break;
     default:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        x[u] = present[u] ? *reinterpret_cast<const uint64_t*>(data + (row0 + u * step) * stride) : 0ull;
+        x[u] = present[u] ? *reinterpret_cast<const global_t<uint64_t>*>(g + (row0 + u * step) * stride) : 0ull;
       break;
   }
 }
@@ -661,11 +612,12 @@ __device__ __forceinline__ void lds_column_rows(const AggParams& p, uint8_t* lds
 
 __device__ __forceinline__ void store_width(void* out, int64_t pos, int width, uint64_t bits)
 {
+  global_t<uint8_t>* g = to_global(static_cast<uint8_t*>(out));
   switch (width) {
-    case 1: reinterpret_cast<uint8_t*>(out)[pos] = (uint8_t)bits; break;
-    case 2: reinterpret_cast<uint16_t*>(out)[pos] = (uint16_t)bits; break;
-    case 4: reinterpret_cast<uint32_t*>(out)[pos] = (uint32_t)bits; break;
-    default: reinterpret_cast<uint64_t*>(out)[pos] = bits; break;
+    case 1: g[pos] = (uint8_t)bits; break;
+    case 2: reinterpret_cast<global_t<uint16_t>*>(g)[pos] = (uint16_t)bits; break;
+    case 4: reinterpret_cast<global_t<uint32_t>*>(g)[pos] = (uint32_t)bits; break;
+    default: reinterpret_cast<global_t<uint64_t>*>(g)[pos] = bits; break;
   }
 }
 
@@ -718,7 +670,7 @@ __device__ __forceinline__ void finalize_record(const uint8_t* rec, int64_t pos,
       default: break;
     }
     store_width(s.out, pos, s.out_width, bits);
-    if (s.valid) s.valid[pos] = cnt != 0;
+    if (s.valid) to_global(s.valid)[pos] = cnt != 0;
   }
 }
 
@@ -735,7 +687,7 @@ __device__ __forceinline__ void finalize_keys(uint64_t hdr, bool special, int ke
   for (int c = 0; c < num_keys; ++c) {
     const KeyCol& k = keys[c];
     store_width(ot.key_out[c], pos, k.width, load_at(k.data, k.width, k.stride, rep));
-    if (ot.key_valid[c]) ot.key_valid[c][pos] = bit_valid(k.mask, k.bit_off + rep) ? 1 : 0;
+    if (ot.key_valid[c]) to_global(ot.key_valid[c])[pos] = bit_valid(k.mask, k.bit_off + rep) ? 1 : 0;
   }
 }
 

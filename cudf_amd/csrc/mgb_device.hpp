@@ -135,31 +135,44 @@ __device__ __host__ __forceinline__ uint32_t hash_combine(uint32_t lhs, uint32_t
 
 // ----------------------------------------------------------------- loads
 
+// Global-memory pointers. Column and output pointers read from memory (not
+// from kernel arguments) are flat pointers to the compiler, and accesses
+// through them flat accesses that also hold the LDS counter; loads and
+// stores of column data go through to_global(...) instead.
+template <typename T>
+using global_t = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ global_t<T>* to_global(T* p)
+{
+  return (global_t<T>*)p;
+}
+
 __device__ __forceinline__ uint64_t load_raw(const uint8_t* base, int width, int64_t row)
 {
+  const global_t<const uint8_t>* g = to_global(base);
   switch (width) {
-    case 1: return base[row];
-    case 2: return reinterpret_cast<const uint16_t*>(base)[row];
-    case 4: return reinterpret_cast<const uint32_t*>(base)[row];
-    default: return reinterpret_cast<const uint64_t*>(base)[row];
+    case 1: return g[row];
+    case 2: return reinterpret_cast<const global_t<uint16_t>*>(g)[row];
+    case 4: return reinterpret_cast<const global_t<uint32_t>*>(g)[row];
+    default: return reinterpret_cast<const global_t<uint64_t>*>(g)[row];
   }
 }
 
 // Element `row` of a column whose rows are `stride` bytes apart.
 __device__ __forceinline__ uint64_t load_at(const uint8_t* base, int width, int stride, int64_t row)
 {
-  const uint8_t* q = base + row * (int64_t)stride;
+  const global_t<const uint8_t>* q = to_global(base) + row * (int64_t)stride;
   switch (width) {
     case 1: return *q;
-    case 2: return *reinterpret_cast<const uint16_t*>(q);
-    case 4: return *reinterpret_cast<const uint32_t*>(q);
-    default: return *reinterpret_cast<const uint64_t*>(q);
+    case 2: return *reinterpret_cast<const global_t<uint16_t>*>(q);
+    case 4: return *reinterpret_cast<const global_t<uint32_t>*>(q);
+    default: return *reinterpret_cast<const global_t<uint64_t>*>(q);
   }
 }
 
 __device__ __forceinline__ bool bit_valid(const uint32_t* mask, int64_t bit)
 {
-  return mask == nullptr || ((mask[bit >> 5] >> (bit & 31)) & 1u);
+  return mask == nullptr || ((to_global(mask)[bit >> 5] >> (bit & 31)) & 1u);
 }
 
 // Raw bits widened to a 64-bit integer (sign- or zero-extended, bool -> 0/1).

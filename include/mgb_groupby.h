@@ -182,6 +182,72 @@ mgb_status mgb_profiling_read(const char* kernel, int64_t* launches, double* tot
  * stream so that repeated calls do not allocate). */
 mgb_status mgb_release_workspace(void* stream);
 
+
+/* ------------------------------------------------------------------------
+ * Batched execution (no counterpart in the reference: cudf runs one
+ * groupby per call). Many independent groupby-aggregate queries that share
+ * one schema -- the same key types, value types, null-ness and aggregations
+ * -- are planned together and run as one HIP graph: per query the planner
+ * estimates the number of groups from a sample and picks the LDS-direct,
+ * global-atomic or hash-partitioned execution, the LDS table size and the
+ * work-item size by a cost model; the work items of all queries are then
+ * taken from one queue by a persistent kernel.
+ *
+ * Lifecycle: mgb_batch_create (plans; synchronises `stream` once), then any
+ * number of mgb_batch_execute (enqueues the graph; does not wait) each
+ * followed by mgb_batch_finish (waits, reads the group counts, re-runs on
+ * the plain path any query whose estimate proved too low), then
+ * mgb_batch_result per query, then mgb_batch_destroy. The input columns
+ * must stay valid and unchanged for the batch's lifetime. Results are views
+ * into buffers the batch owns, valid until the next execute or destroy.
+ * One batch must not be executed on two streams at once.
+ * ------------------------------------------------------------------------ */
+typedef struct mgb_batch mgb_batch;
+
+typedef struct mgb_batch_options {
+  int32_t force_mode;        /* -1: chosen per query; 0: LDS direct; 1: global atomics; 2: partitioned */
+  int32_t use_graph;         /* 1: capture the execution in a HIP graph; 0: plain launches */
+  int64_t direct_item_rows;  /* rows per direct / global work item; 0: chosen per query */
+  double estimate_scale;     /* multiplies every distinct-count estimate (1.0; below 1 only to test re-runs) */
+} mgb_batch_options;
+
+typedef struct mgb_batch_stats {
+  int64_t num_queries;
+  int64_t direct_queries;
+  int64_t global_queries;
+  int64_t partitioned_queries;
+  int64_t num_items;             /* work items of the persistent kernel */
+  int64_t output_rows_reserved;  /* rows of the concatenated output columns */
+  int64_t device_bytes;          /* device memory held by the batch */
+  int64_t reruns;                /* queries re-run on the plain path by the last finish */
+} mgb_batch_stats;
+
+/* keys: num_queries * num_keys columns, query-major; requests:
+ * num_queries * num_requests, query-major (request r of every query carries
+ * the same aggregations). options may be NULL. */
+mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys, int32_t null_policy,
+                            const mgb_request* requests, int32_t num_requests, int32_t num_queries,
+                            const mgb_batch_options* options, void* stream, mgb_batch** out);
+mgb_status mgb_batch_execute(mgb_batch* batch, void* stream);
+mgb_status mgb_batch_finish(mgb_batch* batch, void* stream);
+/* Views of query `query`'s result: out_keys[num_keys], out_results[one per
+ * aggregation, in request order]. A view's rows start at its `offset`
+ * (element offset of data, bit offset of null_mask); null_count is -1 when
+ * not counted. */
+mgb_status mgb_batch_result(const mgb_batch* batch, int32_t query, mgb_column* out_keys, mgb_column* out_results,
+                            int64_t* num_groups);
+/* Query `query`'s result copied into caller-owned columns (allocated with
+ * `alloc`, validity re-packed from bit 0, null counts exact), as
+ * mgb_groupby_aggregate returns them. Synchronises `stream`. */
+mgb_status mgb_batch_copy_result(const mgb_batch* batch, int32_t query, mgb_out_column* out_keys,
+                                 mgb_out_column* out_results, int64_t* num_groups, void* stream,
+                                 const mgb_allocator* alloc);
+mgb_status mgb_batch_get_stats(const mgb_batch* batch, mgb_batch_stats* out);
+/* The execution chosen for a query (0 direct, 1 global, 2 partitioned) and
+ * its distinct-count estimate. */
+mgb_status mgb_batch_query_mode(const mgb_batch* batch, int32_t query, int32_t* mode, int64_t* estimate);
+void mgb_batch_destroy(mgb_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif
