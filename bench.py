@@ -144,6 +144,108 @@ def make_workload(name: str, rows: int, seed: int, device: str = "cuda"):
 DEFAULT_ROWS = {"c1": 1_000_000, "c2": 100_000_000, "c3a": 200_000_000, "c3b": 200_000_000}
 
 
+def make_c4(num_queries: int, seed: int, device: str = "cuda", max_rows: float = 1e7):
+    """Batched workload: per query rows log-uniform in [1e3, max_rows], groups
+    log-uniform in [10, 1e5] (at most the rows), int32 keys from that many
+    distinct values, float64 N(0,1) values; sum/count/mean."""
+    import numpy as np
+    import torch
+
+    import cudf_amd as ca
+    from cudf_amd import aggregation as agg
+
+    rng = np.random.default_rng(seed)
+    rows = np.rint(10 ** rng.uniform(3, np.log10(max_rows), num_queries)).astype(np.int64)
+    card = np.minimum(rows, np.rint(10 ** rng.uniform(1, 5, num_queries))).astype(np.int64)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    queries = []
+    for q in range(num_queries):
+        idx = torch.randint(0, int(card[q]), (int(rows[q]),), generator=g, device=device, dtype=torch.int64)
+        a = int(rng.integers(1, 2 ** 31)) * 2 + 1       # odd: a bijection of the 32-bit integers
+        b = int(rng.integers(0, 2 ** 32))
+        k = (idx * a + b) % (2 ** 32)
+        keys = torch.where(k >= 2 ** 31, k - 2 ** 32, k).to(torch.int32)
+        del idx, k
+        vals = torch.randn(int(rows[q]), generator=g, device=device, dtype=torch.float64)
+        queries.append((ca.Table([ca.Column.from_torch(keys, ca.TypeId.INT32)]),
+                        [ca.GroupByRequest(ca.Column.from_torch(vals, ca.TypeId.FLOAT64),
+                                           [agg.sum(), agg.count(), agg.mean()])]))
+    return queries, rows, card
+
+
+def bench_batched(args):
+    """4096 independent queries: one planned batch (one HIP graph per
+    execution) against the same queries as sequential plain calls."""
+    import numpy as np
+    import torch
+
+    import cudf_amd as ca
+
+    Q = args.queries
+    log(f"generating {Q} batched queries")
+    queries, rows, card = make_c4(Q, args.seed, max_rows=args.max_query_rows)
+    torch.cuda.synchronize()
+    total_rows = int(rows.sum())
+    log(f"{total_rows:,} rows in total; planning the batch")
+    t0 = time.perf_counter()
+    batch = ca.GroupByBatch(queries)
+    plan_s = time.perf_counter() - t0
+    stats = batch.stats()
+    log(f"plan {plan_s * 1e3:.1f} ms: {stats}")
+    for _ in range(args.warmup):
+        batch.run()
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch.run()   # execute (graph launch) + finish (wait, group counts)
+        times.append(time.perf_counter() - t0)
+    t_batch = statistics.median(times)
+    groups = [batch.num_groups(q) for q in range(Q)]
+    log(f"batched: median {t_batch * 1e3:.2f} ms per batch of {Q}")
+
+    # the same queries one plain groupby call at a time
+    n_base = min(Q, args.baseline_queries)
+    log(f"per-query baseline over {n_base} queries")
+    for q in range(min(8, n_base)):
+        ca.GroupBy(queries[q][0]).aggregate(queries[q][1])
+    torch.cuda.synchronize()
+    lat = []
+    t_all = time.perf_counter()
+    for q in range(n_base):
+        t0 = time.perf_counter()
+        k, r = ca.GroupBy(queries[q][0]).aggregate(queries[q][1])
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+        del k, r
+    t_base = time.perf_counter() - t_all
+    base_qps = n_base / t_base
+    in_bytes = 12.0 * total_rows
+    return {
+        "queries": Q,
+        "rows_total": total_rows,
+        "batch_ms": t_batch * 1e3,
+        "batch_ms_all": [round(t * 1e3, 3) for t in times],
+        "plan_ms": plan_s * 1e3,
+        "qps": Q / t_batch,
+        "rows_per_s": total_rows / t_batch,
+        "hbm_frac": in_bytes / t_batch / 1e9 / HBM_PEAK_GBS,
+        "groups_total": int(sum(groups)),
+        "modes": {"lds_direct": stats["direct_queries"], "global": stats["global_queries"],
+                  "partitioned": stats["partitioned_queries"]},
+        "reruns": batch.stats()["reruns"],
+        "per_query_baseline": {
+            "queries": n_base,
+            "qps": base_qps,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        },
+        "speedup_vs_per_query": (Q / t_batch) / base_qps,
+        "latency_note": "every query of a batch completes with the batch: its latency is batch_ms",
+    }
+
+
 def cpu_baseline_generic(name: str, sample_rows: int, seed: int, reps: int = 3):
     """pandas groupby-agg on a bounded sample drawn like the workload."""
     import numpy as np
@@ -214,7 +316,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3a", "c3b"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3a", "c3b", "c4"])
+    ap.add_argument("--queries", type=int, default=4096, help="c4: queries per batch")
+    ap.add_argument("--max-query-rows", type=float, default=1e7, help="c4: largest query (rows)")
+    ap.add_argument("--baseline-queries", type=int, default=4096, help="c4: queries timed one plain call each")
     ap.add_argument("--rows", type=int, default=None)
     ap.add_argument("--distinct", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=42)
@@ -247,6 +352,17 @@ def main():
 
     torch.cuda.set_device(0)
     lib = _lib.load()
+    if args.workload == "c4":
+        res = bench_batched(args)
+        print(json.dumps({
+            "metric": METRIC, "value": res["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["batch_ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "int32 key; f64 value; f64 accumulation", "data": "synthetic (seeded on device)",
+            "config": {"workload": f"batched groupby: {args.queries} queries, rows log-uniform [1e3,"
+                                   f"{args.max_query_rows:.0e}], groups log-uniform [10,1e5], int32 key, f64 value, "
+                                   "sum/count/mean", "parallelism": "single GPU"},
+            "batched": res}), flush=True)
+        return
     rows = args.rows or DEFAULT_ROWS[args.workload]
     args.rows = rows
     log(f"generating workload {args.workload} ({rows:,} rows)")
