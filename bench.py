@@ -174,6 +174,131 @@ def make_c4(num_queries: int, seed: int, device: str = "cuda", max_rows: float =
     return queries, rows, card
 
 
+def make_c5_tables(num_tables: int, seed: int, device: str = "cuda", max_rows: float = 5e7):
+    """Serving tables: rows log-uniform in [1e4, max_rows], groups log-uniform in
+    [10, 1e6] (at most the rows), int64 key, float64 N(0,1) and int32 uniform
+    [-1e6, 1e6] values, each 2% null."""
+    import numpy as np
+    import torch
+
+    import cudf_amd as ca
+
+    rng = np.random.default_rng(seed)
+    rows = np.rint(10 ** rng.uniform(4, np.log10(max_rows), num_tables)).astype(np.int64)
+    card = np.minimum(rows, np.rint(10 ** rng.uniform(1, 6, num_tables))).astype(np.int64)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    tables = []
+    for t in range(num_tables):
+        n = int(rows[t])
+        pool = torch.randint(-2 ** 63, 2 ** 63 - 1, (int(card[t]),), generator=g, device=device, dtype=torch.int64)
+        keys = pool[torch.randint(0, int(card[t]), (n,), generator=g, device=device)]
+        f64 = torch.randn(n, generator=g, device=device, dtype=torch.float64)
+        i32 = torch.randint(-1_000_000, 1_000_001, (n,), generator=g, device=device, dtype=torch.int32)
+        cols = [ca.Column.from_torch(keys, ca.TypeId.INT64)]
+        for v, tid in ((f64, ca.TypeId.FLOAT64), (i32, ca.TypeId.INT32)):
+            valid = torch.rand(n, generator=g, device=device) >= 0.02
+            cols.append(ca.Column.from_torch(v, tid, pack_bits(valid), int((~valid).sum().item())))
+        tables.append(cols)
+    return tables, rows, card
+
+
+def serve_run(tables, rate: float, n_queries: int, batching: bool, seed: int, budget: int,
+              max_inflight: int = 2, max_batch: int = 256):
+    """Poisson arrivals at `rate` queries/s on random tables; returns latencies and throughput."""
+    import threading
+
+    import numpy as np
+
+    import cudf_amd as ca
+    from cudf_amd import aggregation as agg
+
+    aggs = [agg.sum(), agg.count(), agg.min(), agg.max(), agg.mean()]
+    srv = ca.Server(hbm_budget_bytes=budget, max_inflight=max_inflight, max_batch_queries=max_batch,
+                    batching=batching)
+    tids = [srv.add_table(t) for t in tables]
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n_queries)
+    picks = rng.integers(0, len(tids), n_queries)
+    tickets = [None] * n_queries
+    lat = [0.0] * n_queries
+    done_t = [0.0] * n_queries
+    submitted = threading.Semaphore(0)
+
+    def collect():
+        for i in range(n_queries):
+            submitted.acquire()
+            _, lat[i] = srv.wait(tickets[i], 600_000)
+            done_t[i] = time.perf_counter()
+            srv.release(tickets[i])
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    due = t0
+    for i in range(n_queries):
+        due += gaps[i]
+        while True:
+            now = time.perf_counter()
+            if now >= due:
+                break
+            if due - now > 0.0005:
+                time.sleep(due - now - 0.0003)
+        tickets[i] = srv.submit(tids[int(picks[i])], [0], [(1, aggs), (2, aggs)])
+        submitted.release()
+    t_submitted = time.perf_counter() - t0
+    th.join()
+    elapsed = max(done_t) - t0
+    st = srv.stats()
+    srv.close()
+    lat_ms = np.asarray(lat)
+    return {
+        "offered_qps": rate,
+        "queries": n_queries,
+        "achieved_qps": n_queries / elapsed,
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "submit_span_s": t_submitted,
+        "batches": st["batches"],
+        "mean_batch_queries": st["mean_batch_queries"],
+        "deferred": st["deferred"],
+        "reruns": st["reruns"],
+        "peak_held_gb": st["peak_held_bytes"] / 1e9,
+        "resident_gb": st["resident_bytes"] / 1e9,
+    }
+
+
+def bench_serving(args):
+    """Fixed-QPS sweeps of the batching server and of the per-query baseline;
+    each side's highest offered rate whose p99 stays within the target."""
+    import torch
+
+    log(f"generating {args.tables} serving tables")
+    tables, rows, card = make_c5_tables(args.tables, args.seed, max_rows=args.max_table_rows)
+    torch.cuda.synchronize()
+    budget = int(args.hbm_budget_gb * (1 << 30))
+    out = {"tables": args.tables, "table_rows_total": int(rows.sum()), "hbm_budget_gb": args.hbm_budget_gb,
+           "p99_target_ms": args.p99_target_ms, "runs": {"batched": [], "per_query": []}}
+    for mode, rates in (("per_query", args.base_rates), ("batched", args.batch_rates)):
+        for rate in rates:
+            n = int(min(args.serve_max_queries, max(args.serve_min_queries, rate * args.serve_seconds)))
+            log(f"serving {mode}: {rate} q/s, {n} queries")
+            r = serve_run(tables, rate, n, mode == "batched", args.seed + int(rate), budget)
+            log(f"  p50 {r['p50_ms']:.2f} ms  p99 {r['p99_ms']:.2f} ms  achieved {r['achieved_qps']:.0f} q/s  "
+                f"batches {r['batches']} (mean {r['mean_batch_queries']:.1f})")
+            out["runs"][mode].append(r)
+            if r["p99_ms"] > args.p99_target_ms or r["achieved_qps"] < 0.8 * rate:
+                break
+    for mode in ("batched", "per_query"):
+        ok = [r for r in out["runs"][mode] if r["p99_ms"] <= args.p99_target_ms and r["achieved_qps"] >= 0.8 * r["offered_qps"]]
+        out[f"{mode}_qps_at_target"] = max((r["achieved_qps"] for r in ok), default=0.0)
+        out[f"{mode}_p99_at_target"] = min((r["p99_ms"] for r in ok if r["achieved_qps"] == out[f"{mode}_qps_at_target"]),
+                                           default=None)
+    b, p = out["batched_qps_at_target"], out["per_query_qps_at_target"]
+    out["speedup_at_equal_p99"] = b / p if p else None
+    return out
+
+
 def bench_batched(args):
     """4096 independent queries: one planned batch (one HIP graph per
     execution) against the same queries as sequential plain calls."""
@@ -316,7 +441,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3a", "c3b", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3a", "c3b", "c4", "c5"])
+    ap.add_argument("--tables", type=int, default=32, help="c5: resident tables")
+    ap.add_argument("--max-table-rows", type=float, default=5e7, help="c5: largest table (rows)")
+    ap.add_argument("--hbm-budget-gb", type=float, default=200.0, help="c5: HBM budget of the server (GiB)")
+    ap.add_argument("--p99-target-ms", type=float, default=100.0, help="c5: p99 latency target")
+    ap.add_argument("--base-rates", type=float, nargs="*", default=[50, 100, 200, 400, 800, 1600])
+    ap.add_argument("--batch-rates", type=float, nargs="*", default=[200, 400, 800, 1600, 3200, 6400, 12800])
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-min-queries", type=int, default=200)
+    ap.add_argument("--serve-max-queries", type=int, default=4000)
     ap.add_argument("--queries", type=int, default=4096, help="c4: queries per batch")
     ap.add_argument("--max-query-rows", type=float, default=1e7, help="c4: largest query (rows)")
     ap.add_argument("--baseline-queries", type=int, default=4096, help="c4: queries timed one plain call each")
@@ -362,6 +496,19 @@ def main():
                                    f"{args.max_query_rows:.0e}], groups log-uniform [10,1e5], int32 key, f64 value, "
                                    "sum/count/mean", "parallelism": "single GPU"},
             "batched": res}), flush=True)
+        return
+    if args.workload == "c5":
+        res = bench_serving(args)
+        print(json.dumps({
+            "metric": METRIC, "value": res["batched_qps_at_target"], "unit": "queries/s at p99 target",
+            "n_gpus": 1, "steps": None, "warmup": None, "ms_per_step": None, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int64 key; f64 + int32 values; f64/int64 accumulation",
+            "data": "synthetic (seeded on device)",
+            "config": {"workload": f"serving: Poisson arrivals over {args.tables} resident tables, rows log-uniform "
+                                   f"[1e4,{args.max_table_rows:.0e}], groups log-uniform [10,1e6], int64 key, f64 + "
+                                   "int32 values 2% null, sum/count/min/max/mean, HBM budget "
+                                   f"{args.hbm_budget_gb:.0f} GiB", "parallelism": "single GPU"},
+            "serving": res}), flush=True)
         return
     rows = args.rows or DEFAULT_ROWS[args.workload]
     args.rows = rows

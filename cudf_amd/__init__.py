@@ -7,9 +7,10 @@ from . import aggregation, hashing
 from .column import Column, Table, pack_validity, unpack_validity
 from .batch import GroupByBatch
 from .groupby import GroupBy, GroupByRequest
+from .serving import Server
 from .types import Kind, NullPolicy, Sorted, TypeId
 
 __all__ = [
-    "aggregation", "hashing", "Column", "Table", "GroupBy", "GroupByBatch", "GroupByRequest", "Kind", "NullPolicy",
+    "aggregation", "hashing", "Column", "Table", "GroupBy", "GroupByBatch", "GroupByRequest", "Server", "Kind", "NullPolicy",
     "Sorted", "TypeId", "pack_validity", "unpack_validity",
 ]

@@ -80,6 +80,30 @@ class mgb_batch_stats(ctypes.Structure):
         "output_rows_reserved", "device_bytes", "reruns")]
 
 
+class mgb_server_options(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_int64),
+        ("max_inflight", ctypes.c_int32),
+        ("max_batch_queries", ctypes.c_int32),
+        ("batching", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class mgb_server_request(ctypes.Structure):
+    _fields_ = [
+        ("column", ctypes.c_int32),
+        ("num_aggs", ctypes.c_int32),
+        ("aggs", ctypes.POINTER(mgb_aggregation)),
+    ]
+
+
+class mgb_server_stats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "submitted", "completed", "failed", "batches", "deferred", "reruns", "peak_held_bytes", "resident_bytes",
+        "held_bytes", "pending")] + [("mean_batch_queries", ctypes.c_double)]
+
+
 # Every symbol include/mgb_groupby.h declares, with its ctypes signature.
 SIGNATURES = {
     "mgb_version": (ctypes.c_int32, []),
@@ -131,6 +155,28 @@ SIGNATURES = {
         ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)],
     ),
     "mgb_batch_destroy": (None, [ctypes.c_void_p]),
+    "mgb_server_create": (ctypes.c_int, [ctypes.POINTER(mgb_server_options), ctypes.POINTER(ctypes.c_void_p)]),
+    "mgb_server_add_table": (
+        ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(mgb_column), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
+    ),
+    "mgb_server_submit": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32,
+         ctypes.POINTER(mgb_server_request), ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)],
+    ),
+    "mgb_server_wait": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_int64),
+         ctypes.POINTER(ctypes.c_double)],
+    ),
+    "mgb_server_take_result": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(mgb_out_column), ctypes.POINTER(mgb_out_column),
+         ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.POINTER(mgb_allocator)],
+    ),
+    "mgb_server_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "mgb_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(mgb_server_stats)]),
+    "mgb_server_destroy": (None, [ctypes.c_void_p]),
 }
 
 _lib = None

@@ -339,7 +339,8 @@ struct mgb_batch {
   unsigned long long* null_counts = nullptr;
   std::vector<uint8_t*> key_out, out_data, out_valid;
   std::vector<uint32_t*> out_mask;
-  unsigned long long* host_counters = nullptr;  // pinned
+  std::vector<unsigned long long> host_counters;
+  hipStream_t alloc_stream = nullptr;        // stream-ordered allocation and release of `dev`
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -362,8 +363,9 @@ struct mgb_batch {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
-    if (host_counters) (void)hipHostFree(host_counters);
-    if (dev) (void)hipFree(dev);
+    // stream-ordered release: no device-wide synchronisation (a batch may be
+    // destroyed while other batches run)
+    if (dev) (void)hipFreeAsync(dev, alloc_stream);
   }
   void release_fallbacks()
   {
@@ -681,7 +683,8 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
     }
   }
   const size_t o_nulls = take(sizeof(unsigned long long) * (no + 1));
-  MGB_HIP(hipMalloc(&b->dev, off));
+  b->alloc_stream = stream;
+  MGB_HIP(hipMallocAsync(&b->dev, off, stream));
   uint8_t* base = static_cast<uint8_t*>(b->dev);
   b->counters = reinterpret_cast<unsigned long long*>(base + o_counters);
   b->null_counts = reinterpret_cast<unsigned long long*>(base + o_nulls);
@@ -807,7 +810,7 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
                            stream));
   MGB_HIP(hipMemcpyAsync(base + o_tmpl, tmpl, sizeof(tmpl), hipMemcpyHostToDevice, stream));
   MGB_HIP(hipMemcpyAsync(base + o_ot, &ot, sizeof(OutTable), hipMemcpyHostToDevice, stream));
-  MGB_HIP(hipHostMalloc((void**)&b->host_counters, sizeof(unsigned long long) * (4 * (size_t)Q + 8)));
+  b->host_counters.assign(4 * (size_t)Q + 8, 0ull);
   MGB_HIP(hipStreamSynchronize(stream));
 
   // launch geometry
@@ -910,12 +913,12 @@ extern "C" mgb_status mgb_batch_finish(mgb_batch* b, void* stream_v)
   if (!b) return fail(MGB_ERROR_LOGIC, "batch is NULL");
   hipStream_t stream = static_cast<hipStream_t>(stream_v);
   const int Q = b->Q;
-  MGB_HIP(hipMemcpyAsync(b->host_counters, b->counters, sizeof(unsigned long long) * 4 * Q, hipMemcpyDeviceToHost,
-                         stream));
+  MGB_HIP(hipMemcpyAsync(b->host_counters.data(), b->counters, sizeof(unsigned long long) * 4 * Q,
+                         hipMemcpyDeviceToHost, stream));
   MGB_HIP(hipStreamSynchronize(stream));
   int64_t reruns = 0;
   for (int q = 0; q < Q; ++q) {
-    const unsigned long long* c = b->host_counters + 4 * (size_t)q;
+    const unsigned long long* c = b->host_counters.data() + 4 * (size_t)q;
     b->groups[q] = (int64_t)c[2];
     b->fallback[q] = c[1] != 0 || (int64_t)c[2] > b->qp[q].out_cap;
     if (!b->fallback[q]) continue;

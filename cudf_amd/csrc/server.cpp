@@ -1,0 +1,603 @@
+// Serving loop over device-resident tables: queries are submitted from any
+// host thread, a scheduler thread forms batches of the queries that are
+// waiting (continuous batching: whatever arrived while earlier batches ran),
+// admits each batch against an HBM budget, and runs up to max_inflight
+// batches at once, each on its own stream, so that their device work
+// overlaps. A batch is planned and executed by the batch engine
+// (mgb_batch_*); with batching off every query runs alone on the plain path,
+// in arrival order (the per-query baseline).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "groupby_host.hpp"
+
+using namespace mgb;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point t0, Clock::time_point t1)
+{
+  return std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
+struct Table {
+  std::vector<mgb_column> cols;
+  int64_t bytes = 0;
+};
+
+struct Query {
+  int64_t ticket = 0;
+  int32_t table = 0;
+  int32_t null_policy = 0;
+  std::vector<mgb_column> keys;   // the table's columns, resolved at submission
+  std::vector<mgb_column> vals;
+  std::vector<std::vector<mgb_aggregation>> aggs;
+  std::string signature;  // queries of one batch share it
+  Clock::time_point arrival;
+};
+
+// A batch held until every one of its results has been taken or released.
+struct HeldBatch {
+  mgb_batch* batch = nullptr;
+  int64_t bytes = 0;
+  int remaining = 0;
+  ~HeldBatch() { mgb_batch_destroy(batch); }
+};
+
+struct Done {
+  mgb_status status = MGB_SUCCESS;
+  std::string error;
+  int64_t groups = 0;
+  double latency_ms = 0;
+  std::shared_ptr<HeldBatch> batch;  // batched result, or
+  int32_t index = 0;                 // the query's index in it
+  std::vector<mgb_out_column> keys, results;  // plain-path result (owned)
+  int64_t bytes = 0;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  bool busy = false;
+  std::atomic<bool> finished{false};
+  std::shared_ptr<HeldBatch> batch;
+  std::vector<Query> queries;
+  struct mgb_server* server = nullptr;
+};
+
+void* dev_alloc(void*, size_t bytes, void*)
+{
+  void* p = nullptr;
+  return hipMalloc(&p, bytes) == hipSuccess ? p : nullptr;
+}
+void dev_free(void*, void* p, size_t, void*) { (void)hipFree(p); }
+
+void free_out(std::vector<mgb_out_column>& cols)
+{
+  for (auto& c : cols) {
+    if (c.data) (void)hipFree(c.data);
+    if (c.null_mask) (void)hipFree(c.null_mask);
+    c.data = nullptr;
+    c.null_mask = nullptr;
+  }
+}
+
+}  // namespace
+
+struct mgb_server {
+  mgb_server_options opt{};
+  std::mutex mu;
+  std::condition_variable cv;        // scheduler wake-ups
+  std::condition_variable done_cv;   // waiters
+  bool stop = false;
+  std::vector<Table> tables;
+  int64_t resident_bytes = 0;
+  int64_t held_bytes = 0;            // batches and plain results not yet taken
+  std::deque<Query> pending;
+  std::map<int64_t, Done> done;
+  int64_t next_ticket = 1;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::thread scheduler;
+  mgb_server_stats stats{};
+  std::vector<double> batch_sizes;
+
+  void run();
+  void retire(Slot& s);
+  bool launch(Slot& s);
+  void fail_query(const Query& q, mgb_status st, const std::string& msg);
+};
+
+namespace {
+
+// Host callback after a slot's batch (a runtime thread; takes no lock, so
+// that no HIP call waiting for this callback can wait on a lock holder; the
+// scheduler's timed wait covers a wake-up that races its predicate check).
+void slot_done(void* arg)
+{
+  Slot* s = static_cast<Slot*>(arg);
+  s->finished.store(true);
+  s->server->cv.notify_all();
+}
+
+std::string signature_of(const Query& q)
+{
+  // tables with the same column types and null-ness batch together
+  std::string sig = std::to_string(q.null_policy) + "|";
+  for (const mgb_column& k : q.keys)
+    sig += "k" + std::to_string(k.type_id) + (k.null_mask && k.null_count ? "n" : "") + ",";
+  for (size_t r = 0; r < q.vals.size(); ++r) {
+    const mgb_column& c = q.vals[r];
+    sig += "v" + std::to_string(c.type_id) + (c.null_mask && c.null_count ? "n" : "") + ":";
+    for (auto& a : q.aggs[r]) sig += std::to_string(a.kind) + "." + std::to_string(a.ddof) + ",";
+  }
+  return sig;
+}
+
+}  // namespace
+
+void mgb_server::fail_query(const Query& q, mgb_status st, const std::string& msg)
+{
+  Done d;
+  d.status = st;
+  d.error = msg;
+  d.latency_ms = ms_since(q.arrival, Clock::now());
+  done[q.ticket] = std::move(d);
+  ++stats.failed;
+}
+
+// Completed slot -> results (the scheduler thread, lock not held).
+void mgb_server::retire(Slot& s)
+{
+  std::vector<Done> out(s.queries.size());
+  if (s.batch) {
+    mgb_status st = mgb_batch_finish(s.batch->batch, s.stream);
+    std::string err = st == MGB_SUCCESS ? "" : mgb_last_error_message();
+    mgb_batch_stats bs{};
+    (void)mgb_batch_get_stats(s.batch->batch, &bs);
+    const auto now = Clock::now();
+    for (size_t i = 0; i < s.queries.size(); ++i) {
+      out[i].status = st;
+      out[i].error = err;
+      out[i].latency_ms = ms_since(s.queries[i].arrival, now);
+      out[i].batch = s.batch;
+      out[i].index = (int32_t)i;
+      if (st == MGB_SUCCESS) (void)mgb_batch_result(s.batch->batch, (int32_t)i, nullptr, nullptr, &out[i].groups);
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    stats.reruns += bs.reruns;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  for (size_t i = 0; i < s.queries.size(); ++i) {
+    if (!s.batch) continue;
+    done[s.queries[i].ticket] = std::move(out[i]);
+    ++stats.completed;
+  }
+  s.batch.reset();
+  s.queries.clear();
+  s.busy = false;
+  s.finished.store(false);
+  done_cv.notify_all();
+}
+
+// Forms and starts one batch on an idle slot; false when nothing could start.
+bool mgb_server::launch(Slot& s)
+{
+  std::vector<Query> take;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (pending.empty()) return false;
+    const std::string sig = pending.front().signature;
+    const int limit = opt.batching ? std::max(1, opt.max_batch_queries) : 1;
+    for (auto it = pending.begin(); it != pending.end() && (int)take.size() < limit;) {
+      if (it->signature == sig) {
+        take.push_back(std::move(*it));
+        it = pending.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  if (!opt.batching) {
+    // the per-query baseline: the plain path, synchronously on this slot's stream
+    Query& q = take[0];
+    std::vector<mgb_column>& keys = q.keys;
+    std::vector<mgb_request> reqs(q.vals.size());
+    int nout = 0;
+    for (size_t r = 0; r < q.vals.size(); ++r) {
+      reqs[r].values = q.vals[r];
+      reqs[r].aggs = q.aggs[r].data();
+      reqs[r].num_aggs = (int32_t)q.aggs[r].size();
+      nout += reqs[r].num_aggs;
+    }
+    Done d;
+    d.keys.assign(keys.size(), mgb_out_column{});
+    d.results.assign(nout, mgb_out_column{});
+    mgb_allocator al{dev_alloc, dev_free, nullptr};
+    d.status = mgb_groupby_aggregate(keys.data(), (int32_t)keys.size(), q.null_policy, reqs.data(),
+                                     (int32_t)reqs.size(), d.keys.data(), d.results.data(), &d.groups, s.stream, &al);
+    if (d.status != MGB_SUCCESS) d.error = mgb_last_error_message();
+    d.latency_ms = ms_since(q.arrival, Clock::now());
+    for (auto& c : d.keys) d.bytes += (int64_t)(c.data_bytes + c.mask_bytes);
+    for (auto& c : d.results) d.bytes += (int64_t)(c.data_bytes + c.mask_bytes);
+    std::lock_guard<std::mutex> lk(mu);
+    held_bytes += d.bytes;
+    ++stats.completed;
+    ++stats.batches;
+    batch_sizes.push_back(1);
+    done[q.ticket] = std::move(d);
+    done_cv.notify_all();
+    return true;
+  }
+  // continuous batching: plan the waiting queries as one batch; halve it
+  // until it fits the HBM budget next to what is resident and held
+  while (!take.empty()) {
+    const int Q = (int)take.size();
+    const int nk = (int)take[0].keys.size(), nr = (int)take[0].vals.size();
+    std::vector<mgb_column> keys((size_t)Q * nk);
+    std::vector<mgb_request> reqs((size_t)Q * nr);
+    for (int q = 0; q < Q; ++q) {
+      for (int k = 0; k < nk; ++k) keys[(size_t)q * nk + k] = take[q].keys[k];
+      for (int r = 0; r < nr; ++r) {
+        reqs[(size_t)q * nr + r].values = take[q].vals[r];
+        reqs[(size_t)q * nr + r].aggs = take[q].aggs[r].data();
+        reqs[(size_t)q * nr + r].num_aggs = (int32_t)take[q].aggs[r].size();
+      }
+    }
+    // a served batch runs once: plain launches (capturing a graph for a
+    // single execution only adds its capture and instantiation)
+    mgb_batch_options bo{};
+    bo.force_mode = -1;
+    bo.use_graph = 0;
+    bo.direct_item_rows = 0;
+    bo.estimate_scale = 1.0;
+    mgb_batch* b = nullptr;
+    mgb_status st = mgb_batch_create(keys.data(), nk, take[0].null_policy, reqs.data(), nr, Q, &bo, s.stream, &b);
+    if (st != MGB_SUCCESS) {
+      const std::string msg = mgb_last_error_message();
+      std::lock_guard<std::mutex> lk(mu);
+      for (auto& q : take) fail_query(q, st, msg);
+      done_cv.notify_all();
+      return true;
+    }
+    mgb_batch_stats bs{};
+    (void)mgb_batch_get_stats(b, &bs);
+    bool fits;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      fits = resident_bytes + held_bytes + bs.device_bytes <= opt.hbm_budget_bytes;
+      if (fits) held_bytes += bs.device_bytes;
+    }
+    if (!fits) {
+      mgb_batch_destroy(b);
+      std::lock_guard<std::mutex> lk(mu);
+      ++stats.deferred;
+      if (Q == 1) {
+        bool idle = true;
+        for (auto& o : slots) idle = idle && !o->busy;
+        if (idle && held_bytes == 0) {
+          fail_query(take[0], MGB_ERROR_ALLOC,
+                     "query does not fit the HBM budget (" + std::to_string(bs.device_bytes) + " bytes next to " +
+                         std::to_string(resident_bytes) + " resident, budget " +
+                         std::to_string(opt.hbm_budget_bytes) + ")");
+          done_cv.notify_all();
+          return true;
+        }
+        pending.push_front(std::move(take[0]));  // wait for memory to be released
+        return false;
+      }
+      // put the second half back at the front, in order
+      for (int q = Q - 1; q >= Q / 2; --q) pending.push_front(std::move(take[q]));
+      take.resize(Q / 2);
+      continue;
+    }
+    st = mgb_batch_execute(b, s.stream);
+    if (st == MGB_SUCCESS && hipLaunchHostFunc(s.stream, slot_done, &s) != hipSuccess) st = MGB_ERROR_HIP;
+    if (st != MGB_SUCCESS) {
+      const std::string msg = mgb_last_error_message();
+      mgb_batch_destroy(b);
+      std::lock_guard<std::mutex> lk(mu);
+      held_bytes -= bs.device_bytes;
+      for (auto& q : take) fail_query(q, st, msg);
+      done_cv.notify_all();
+      return true;
+    }
+    auto hb = std::make_shared<HeldBatch>();
+    hb->batch = b;
+    hb->bytes = bs.device_bytes;
+    hb->remaining = Q;
+    std::lock_guard<std::mutex> lk(mu);
+    s.batch = hb;
+    s.queries = std::move(take);
+    s.busy = true;
+    ++stats.batches;
+    batch_sizes.push_back(Q);
+    stats.peak_held_bytes = std::max(stats.peak_held_bytes, held_bytes);
+    return true;
+  }
+  return false;
+}
+
+void mgb_server::run()
+{
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait_for(lk, std::chrono::milliseconds(2), [&] {
+        if (stop) return true;
+        for (auto& s : slots)
+          if (s->busy && s->finished.load()) return true;
+        bool idle = false;
+        for (auto& s : slots) idle = idle || !s->busy;
+        return idle && !pending.empty();
+      });
+      if (stop) {
+        bool busy = false;
+        for (auto& s : slots) busy = busy || s->busy;
+        if (!busy) return;
+      }
+    }
+    for (auto& s : slots)
+      if (s->busy && s->finished.load()) retire(*s);
+    for (auto& s : slots) {
+      if (s->busy) continue;
+      if (!launch(*s)) break;
+    }
+  }
+}
+
+extern "C" mgb_status mgb_server_create(const mgb_server_options* options, mgb_server** out)
+{
+  if (!out) return fail(MGB_ERROR_LOGIC, "out is NULL");
+  *out = nullptr;
+  std::unique_ptr<mgb_server> s(new mgb_server());
+  if (options) s->opt = *options;
+  if (s->opt.hbm_budget_bytes <= 0) s->opt.hbm_budget_bytes = 200ll << 30;
+  if (s->opt.max_inflight <= 0) s->opt.max_inflight = 2;
+  if (s->opt.max_batch_queries <= 0) s->opt.max_batch_queries = 256;
+  if (!s->opt.batching) s->opt.max_inflight = 1;
+  for (int i = 0; i < s->opt.max_inflight; ++i) {
+    auto slot = std::make_unique<Slot>();
+    slot->server = s.get();
+    MGB_HIP(hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking));
+    s->slots.push_back(std::move(slot));
+  }
+  mgb_server* raw = s.get();
+  s->scheduler = std::thread([raw] { raw->run(); });
+  *out = s.release();
+  return MGB_SUCCESS;
+}
+
+extern "C" mgb_status mgb_server_add_table(mgb_server* s, const mgb_column* columns, int32_t num_columns,
+                                           int32_t* table_id)
+{
+  if (!s || !columns || num_columns <= 0 || !table_id) return fail(MGB_ERROR_LOGIC, "bad table");
+  Table t;
+  t.cols.assign(columns, columns + num_columns);
+  for (auto& c : t.cols) {
+    if (c.size != t.cols[0].size) return fail(MGB_ERROR_LOGIC, "table columns differ in size");
+    t.bytes += c.size * (int64_t)type_width(c.type_id) + (c.null_mask ? bitmask_bytes(c.size) : 0);
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->resident_bytes + t.bytes > s->opt.hbm_budget_bytes)
+    return fail(MGB_ERROR_ALLOC, "table does not fit the HBM budget");
+  s->resident_bytes += t.bytes;
+  *table_id = (int32_t)s->tables.size();
+  s->tables.push_back(std::move(t));
+  return MGB_SUCCESS;
+}
+
+extern "C" mgb_status mgb_server_submit(mgb_server* s, int32_t table_id, const int32_t* key_columns, int32_t num_keys,
+                                        int32_t null_policy, const mgb_server_request* requests,
+                                        int32_t num_requests, int64_t* ticket)
+{
+  if (!s || !ticket || !key_columns || num_keys <= 0 || num_requests < 0 || (num_requests && !requests))
+    return fail(MGB_ERROR_LOGIC, "bad query");
+  Query q;
+  q.arrival = Clock::now();
+  q.null_policy = null_policy;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (table_id < 0 || table_id >= (int32_t)s->tables.size()) return fail(MGB_ERROR_LOGIC, "unknown table");
+    const int ncol = (int)s->tables[table_id].cols.size();
+    for (int k = 0; k < num_keys; ++k)
+      if (key_columns[k] < 0 || key_columns[k] >= ncol) return fail(MGB_ERROR_LOGIC, "key column out of range");
+    for (int r = 0; r < num_requests; ++r)
+      if (requests[r].column < 0 || requests[r].column >= ncol || requests[r].num_aggs < 0 ||
+          (requests[r].num_aggs && !requests[r].aggs))
+        return fail(MGB_ERROR_LOGIC, "request out of range");
+    q.table = table_id;
+    const Table& t = s->tables[table_id];
+    for (int k = 0; k < num_keys; ++k) q.keys.push_back(t.cols[key_columns[k]]);
+    for (int r = 0; r < num_requests; ++r) {
+      q.vals.push_back(t.cols[requests[r].column]);
+      q.aggs.emplace_back(requests[r].aggs, requests[r].aggs + requests[r].num_aggs);
+    }
+    q.signature = signature_of(q);
+    q.ticket = s->next_ticket++;
+    *ticket = q.ticket;
+    s->pending.push_back(std::move(q));
+    ++s->stats.submitted;
+  }
+  s->cv.notify_all();
+  return MGB_SUCCESS;
+}
+
+extern "C" mgb_status mgb_server_wait(mgb_server* s, int64_t ticket, double timeout_ms, int64_t* num_groups,
+                                      double* latency_ms)
+{
+  if (!s) return fail(MGB_ERROR_LOGIC, "server is NULL");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto ready = [&] { return s->done.count(ticket) != 0; };
+  if (timeout_ms < 0) s->done_cv.wait(lk, ready);
+  else if (!s->done_cv.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
+    return fail(MGB_ERROR_LOGIC, "timed out waiting for ticket " + std::to_string(ticket));
+  const Done& d = s->done[ticket];
+  if (num_groups) *num_groups = d.groups;
+  if (latency_ms) *latency_ms = d.latency_ms;
+  if (d.status != MGB_SUCCESS) return fail(d.status, d.error);
+  return MGB_SUCCESS;
+}
+
+namespace {
+
+// What a released ticket held. Freed with no lock held: hipFree waits for
+// the device, and so for the host callbacks of batches still in flight.
+struct Released {
+  std::shared_ptr<HeldBatch> batch;
+  std::vector<mgb_out_column> keys, results;
+  ~Released()
+  {
+    batch.reset();  // the last reference destroys the batch
+    free_out(keys);
+    free_out(results);
+  }
+};
+
+// Takes a finished ticket out of the map (caller holds s->mu); the memory
+// accounting changes at once, the memory itself is freed by the caller.
+std::unique_ptr<Released> release_locked(mgb_server* s, std::map<int64_t, Done>::iterator it)
+{
+  auto r = std::make_unique<Released>();
+  Done& d = it->second;
+  if (d.batch) {
+    if (--d.batch->remaining == 0) s->held_bytes -= d.batch->bytes;
+    r->batch = std::move(d.batch);
+  } else {
+    s->held_bytes -= d.bytes;
+    r->keys = std::move(d.keys);
+    r->results = std::move(d.results);
+  }
+  s->done.erase(it);
+  s->cv.notify_all();  // memory may have been released
+  return r;
+}
+
+}  // namespace
+
+extern "C" mgb_status mgb_server_take_result(mgb_server* s, int64_t ticket, mgb_out_column* out_keys,
+                                             mgb_out_column* out_results, int64_t* num_groups, void* stream,
+                                             const mgb_allocator* alloc)
+{
+  if (!s || !alloc) return fail(MGB_ERROR_LOGIC, "NULL argument");
+  std::shared_ptr<HeldBatch> hb;
+  int32_t index = 0;
+  std::vector<mgb_out_column> own_keys, own_results;
+  std::unique_ptr<Released> freed;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->done.find(ticket);
+    if (it == s->done.end()) return fail(MGB_ERROR_LOGIC, "ticket not finished or already taken");
+    if (it->second.status != MGB_SUCCESS) {
+      const mgb_status st = it->second.status;
+      const std::string err = it->second.error;
+      freed = release_locked(s, it);
+      lk.unlock();
+      freed.reset();
+      return fail(st, err);
+    }
+    hb = it->second.batch;
+    index = it->second.index;
+    if (!hb) {
+      own_keys = it->second.keys;
+      own_results = it->second.results;
+      if (num_groups) *num_groups = it->second.groups;
+    }
+  }
+  mgb_status st = MGB_SUCCESS;
+  if (hb) {
+    st = mgb_batch_copy_result(hb->batch, index, out_keys, out_results, num_groups, stream, alloc);
+  } else {
+    // plain-path result: copy into the caller's allocations
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    auto copy = [&](const mgb_out_column& src, mgb_out_column& dst) -> mgb_status {
+      dst = src;
+      dst.data = nullptr;
+      dst.null_mask = nullptr;
+      if (src.data) {
+        dst.data = alloc->allocate(alloc->ctx, src.data_bytes, stream);
+        if (!dst.data) return fail(MGB_ERROR_ALLOC, "allocation of a result column failed");
+        MGB_HIP(hipMemcpyAsync(dst.data, src.data, src.data_bytes, hipMemcpyDeviceToDevice, hs));
+      }
+      if (src.null_mask) {
+        dst.null_mask = static_cast<uint32_t*>(alloc->allocate(alloc->ctx, src.mask_bytes, stream));
+        if (!dst.null_mask) return fail(MGB_ERROR_ALLOC, "allocation of a result null mask failed");
+        MGB_HIP(hipMemcpyAsync(dst.null_mask, src.null_mask, src.mask_bytes, hipMemcpyDeviceToDevice, hs));
+      }
+      return MGB_SUCCESS;
+    };
+    for (size_t c = 0; c < own_keys.size() && st == MGB_SUCCESS; ++c) st = copy(own_keys[c], out_keys[c]);
+    for (size_t c = 0; c < own_results.size() && st == MGB_SUCCESS; ++c) st = copy(own_results[c], out_results[c]);
+    if (st == MGB_SUCCESS && hipStreamSynchronize(hs) != hipSuccess) st = fail(MGB_ERROR_HIP, "stream sync failed");
+  }
+  hb.reset();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->done.find(ticket);
+    if (it != s->done.end()) freed = release_locked(s, it);
+  }
+  freed.reset();
+  return st;
+}
+
+extern "C" mgb_status mgb_server_release(mgb_server* s, int64_t ticket)
+{
+  if (!s) return fail(MGB_ERROR_LOGIC, "server is NULL");
+  std::unique_ptr<Released> freed;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->done.find(ticket);
+    if (it == s->done.end()) return fail(MGB_ERROR_LOGIC, "ticket not finished or already taken");
+    freed = release_locked(s, it);
+  }
+  freed.reset();
+  return MGB_SUCCESS;
+}
+
+extern "C" mgb_status mgb_server_get_stats(mgb_server* s, mgb_server_stats* out)
+{
+  if (!s || !out) return fail(MGB_ERROR_LOGIC, "NULL argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->stats;
+  out->resident_bytes = s->resident_bytes;
+  out->held_bytes = s->held_bytes;
+  out->pending = (int64_t)s->pending.size();
+  double sum = 0;
+  for (double b : s->batch_sizes) sum += b;
+  out->mean_batch_queries = s->batch_sizes.empty() ? 0.0 : sum / (double)s->batch_sizes.size();
+  return MGB_SUCCESS;
+}
+
+extern "C" void mgb_server_destroy(mgb_server* s)
+{
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  if (s->scheduler.joinable()) s->scheduler.join();
+  std::vector<std::unique_ptr<Released>> freed;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    while (!s->done.empty()) freed.push_back(release_locked(s, s->done.begin()));
+  }
+  freed.clear();
+  for (auto& slot : s->slots) {
+    if (slot->stream) {
+      (void)hipStreamSynchronize(slot->stream);
+      (void)hipStreamDestroy(slot->stream);
+    }
+  }
+  delete s;
+}

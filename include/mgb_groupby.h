@@ -248,6 +248,76 @@ mgb_status mgb_batch_get_stats(const mgb_batch* batch, mgb_batch_stats* out);
 mgb_status mgb_batch_query_mode(const mgb_batch* batch, int32_t query, int32_t* mode, int64_t* estimate);
 void mgb_batch_destroy(mgb_batch* batch);
 
+
+/* ------------------------------------------------------------------------
+ * Serving (no counterpart in the reference). A server keeps tables resident
+ * on the device and answers groupby queries on them submitted from any host
+ * thread. A scheduler thread batches the queries that are waiting
+ * (continuous batching: whatever arrived while earlier batches ran), admits
+ * each batch against the HBM budget (resident tables + results not yet taken
+ * + the batch's own memory; a batch that does not fit is halved, a query
+ * that does not fit waits for memory to be released), and runs up to
+ * max_inflight batches at once, one stream each, so their device work
+ * overlaps. With batching = 0 every query runs alone on the plain path, in
+ * arrival order (the per-query baseline). Results do not depend on which
+ * batch a query ran in, beyond floating-point summation order.
+ * ------------------------------------------------------------------------ */
+typedef struct mgb_server mgb_server;
+
+typedef struct mgb_server_options {
+  int64_t hbm_budget_bytes;   /* 0: 200 GiB */
+  int32_t max_inflight;       /* batches executing at once (0: 2) */
+  int32_t max_batch_queries;  /* queries per batch at most (0: 256) */
+  int32_t batching;           /* 1: continuous batching; 0: per-query plain path */
+  int32_t reserved;
+} mgb_server_options;
+
+/* One request of a query: a column of the table and its aggregations. */
+typedef struct mgb_server_request {
+  int32_t column;
+  int32_t num_aggs;
+  const mgb_aggregation* aggs;
+} mgb_server_request;
+
+typedef struct mgb_server_stats {
+  int64_t submitted;
+  int64_t completed;
+  int64_t failed;
+  int64_t batches;
+  int64_t deferred;           /* batches halved or postponed by the HBM budget */
+  int64_t reruns;             /* queries re-run on the plain path after an underestimate */
+  int64_t peak_held_bytes;
+  int64_t resident_bytes;
+  int64_t held_bytes;
+  int64_t pending;
+  double mean_batch_queries;
+} mgb_server_stats;
+
+mgb_status mgb_server_create(const mgb_server_options* options, mgb_server** out);
+/* Registers a device-resident table; its columns stay owned by the caller and
+ * must outlive the server. Returns MGB_ERROR_ALLOC past the budget. */
+mgb_status mgb_server_add_table(mgb_server* server, const mgb_column* columns, int32_t num_columns,
+                                int32_t* table_id);
+/* Queues a query: group table `table_id` by columns key_columns[num_keys]. */
+mgb_status mgb_server_submit(mgb_server* server, int32_t table_id, const int32_t* key_columns, int32_t num_keys,
+                             int32_t null_policy, const mgb_server_request* requests, int32_t num_requests,
+                             int64_t* ticket);
+/* Blocks until the ticket's query has finished (timeout_ms < 0: no limit);
+ * reports its group count and its latency from submission to completion. */
+mgb_status mgb_server_wait(mgb_server* server, int64_t ticket, double timeout_ms, int64_t* num_groups,
+                           double* latency_ms);
+/* Copies a finished query's result into caller-owned columns (allocated with
+ * `alloc`, as mgb_groupby_aggregate returns them) and releases the ticket. */
+mgb_status mgb_server_take_result(mgb_server* server, int64_t ticket, mgb_out_column* out_keys,
+                                  mgb_out_column* out_results, int64_t* num_groups, void* stream,
+                                  const mgb_allocator* alloc);
+/* Releases a finished ticket without copying its result. */
+mgb_status mgb_server_release(mgb_server* server, int64_t ticket);
+mgb_status mgb_server_get_stats(mgb_server* server, mgb_server_stats* out);
+/* Stops the scheduler after the batches in flight; queries still queued are
+ * dropped. Call after every wait has returned. */
+void mgb_server_destroy(mgb_server* server);
+
 #ifdef __cplusplus
 }
 #endif
