@@ -33,6 +33,11 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 HBM_PEAK_GBS = 8000.0
 
+# library profiling scopes (mgb_profiling_read) and the kernels they time
+PROFILED = ("aggregate", "lds_aggregate", "fill", "finalize", "partition_hist", "partition_scatter", "estimate")
+KERNEL_SYMBOL = {"aggregate": "mgb::aggregate_kernel", "lds_aggregate": "mgb::lds_agg_kernel",
+                 "partition_scatter": "mgb::part_scatter_tiled_kernel", "partition_hist": "mgb::part_hist_kernel"}
+
 
 def pack_bits(valid):
     """bool tensor [n] -> int32 bitmask words (LSB first), padded to 64 bytes."""
@@ -384,7 +389,8 @@ def cpu_baseline_generic(name: str, sample_rows: int, seed: int, reps: int = 3):
         n_keys = 50_000_000
         u = rng.random(sample_rows)
         r = np.clip(np.floor((1.0 + u * (n_keys ** -0.1 - 1.0)) ** -10.0).astype(np.int64) - 1, 0, n_keys - 1)
-        df = pd.DataFrame({"a": (r * 2654435761 % 2 ** 31).astype(np.int32), "b": r * 0x9E3779B97F4A7C15 % 2 ** 61,
+        df = pd.DataFrame({"a": (r * 2654435761 % 2 ** 31).astype(np.int32),
+                           "b": (r.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15) % np.uint64(2 ** 61)).astype(np.int64),
                            "val": rng.standard_normal(sample_rows)})
         run = lambda: df.groupby(["a", "b"], sort=False)["val"].agg(["sum", "count", "max"])
     else:  # c3b
@@ -436,6 +442,70 @@ def cpu_baseline_c2(sample_rows: int, distinct: int, seed: int, reps: int = 3):
     }
 
 
+def summarize_trace(db_path: str, title: str, command: str) -> str:
+    """Text summary of the top_kernels view of a rocprofv3 --kernel-trace --stats database."""
+    import sqlite3
+
+    con = sqlite3.connect(db_path)
+    rows = list(con.execute("select name, total_calls, total_duration, average, percentage from top_kernels"))
+    out = [title, "Command (from the repo root, TMPDIR=/tmp):", "  " + command,
+           "Durations in microseconds (top_kernels view of the rocprofv3 database; torch kernels are data generation).",
+           "", f"{'kernel':60s} {'calls':>7s} {'total_us':>12s} {'avg_us':>10s} {'pct':>6s}"]
+    for name, calls, total, avg, pct in rows[:30]:
+        out.append(f"{name.split('(')[0][:60]:60s} {int(calls):7d} {float(total):12.1f} {float(avg):10.1f} "
+                   f"{float(pct):6.2f}")
+    return "\n".join(out) + "\n"
+
+
+def summarize_pmc(fetch_csv: str, write_csv: str, cal_fetch_csv: str, cal_write_csv: str, title: str,
+                  commands) -> str:
+    """Per-kernel HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE passes,
+    each scaled by what the counter reads for a copy of 1 GiB (read and
+    written once)."""
+    import collections
+    import csv
+
+    def per_kernel(path, counter):
+        acc = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                acc[r["Kernel_Name"].split("(")[0]].append(float(r["Counter_Value"]))
+        return acc
+
+    cal_f = per_kernel(cal_fetch_csv, "FETCH_SIZE")
+    cal_w = per_kernel(cal_write_csv, "WRITE_SIZE")
+    copy_name = next(k for k in cal_f if "copyBuffer" in k or "copy" in k.lower())
+    gib_kib = float(1 << 20)
+    f_scale = gib_kib / statistics.median(cal_f[copy_name])
+    w_scale = gib_kib / statistics.median(cal_w[copy_name])
+    fetch = per_kernel(fetch_csv, "FETCH_SIZE")
+    write = per_kernel(write_csv, "WRITE_SIZE")
+    out = [title, "Commands (from the repo root, TMPDIR=/tmp):"] + ["  " + c for c in commands] + [
+        f"Calibration: a 1 GiB device copy ({copy_name}) reads FETCH_SIZE = {statistics.median(cal_f[copy_name]):.0f} KiB "
+        f"and WRITE_SIZE = {statistics.median(cal_w[copy_name]):.0f} KiB per dispatch: FETCH_SIZE is scaled by "
+        f"{f_scale:.3f}, WRITE_SIZE by {w_scale:.3f}.", "",
+        f"{'kernel':48s} {'dispatches':>10s} {'read_MB':>10s} {'written_MB':>11s}"]
+    for k in sorted(fetch, key=lambda k: -statistics.mean(fetch[k])):
+        if not k.startswith("mgb::"):
+            continue
+        rd = statistics.mean(fetch[k]) * f_scale * 1024 / 1e6
+        wr = statistics.mean(write.get(k, [0.0])) * w_scale * 1024 / 1e6
+        out.append(f"{k[:48]:48s} {len(fetch[k]):10d} {rd:10.1f} {wr:11.1f}")
+    return "\n".join(out) + "\n"
+
+
+def read_traffic(path: str, kernel: str):
+    """HBM bytes per dispatch of `kernel` from a summary written by summarize_pmc, or None."""
+    try:
+        for line in open(path):
+            parts = line.split()
+            if parts and parts[0] == kernel and len(parts) >= 4:
+                return (float(parts[2]) + float(parts[3])) * 1e6
+    except OSError:
+        return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -459,9 +529,23 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-sample-rows", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--summarize-trace", nargs=3, metavar=("RESULTS_DB", "TITLE", "COMMAND"),
+                    help="print a text summary of a rocprofv3 --kernel-trace --stats database and exit")
+    ap.add_argument("--summarize-pmc", nargs=6,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "CAL_FETCH_CSV", "CAL_WRITE_CSV", "TITLE", "COMMANDS"),
+                    help="print per-kernel HBM bytes from FETCH_SIZE / WRITE_SIZE passes and exit "
+                         "(COMMANDS: the profiling commands, separated by ';;')")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "hbm_traffic_groupby_c2_100M_rows.txt"))
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB buffer 3 times (reference kernel for FETCH_SIZE / WRITE_SIZE scaling)")
     args = ap.parse_args()
+    if args.summarize_trace:
+        print(summarize_trace(*args.summarize_trace), end="")
+        return
+    if args.summarize_pmc:
+        a = args.summarize_pmc
+        print(summarize_pmc(a[0], a[1], a[2], a[3], a[4], a[5].split(";;")), end="")
+        return
     if args.calibrate_copy:
         import torch
 
@@ -526,9 +610,9 @@ def main():
 
     log(f"timing {args.steps} steps")
     lib.mgb_profiling_enable(1)
-    for name in (b"aggregate", b"fill", b"finalize", b"partition_hist", b"partition_scatter", b"estimate"):
+    for name in PROFILED:
         n, ms = ctypes.c_int64(), ctypes.c_double()
-        lib.mgb_profiling_read(name, ctypes.byref(n), ctypes.byref(ms))  # drop stale records
+        lib.mgb_profiling_read(name.encode(), ctypes.byref(n), ctypes.byref(ms))  # drop stale records
     stream = torch.cuda.current_stream()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -541,10 +625,11 @@ def main():
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
     kern = {}
-    for name in ("aggregate", "fill", "finalize", "partition_hist", "partition_scatter", "estimate"):
+    for name in PROFILED:
         n, ms = ctypes.c_int64(), ctypes.c_double()
         _lib.check(lib.mgb_profiling_read(name.encode(), ctypes.byref(n), ctypes.byref(ms)))
-        kern[name] = (n.value, ms.value)
+        if n.value:
+            kern[name] = (n.value, ms.value)
     lib.mgb_profiling_enable(0)
 
     R = args.rows
@@ -553,9 +638,14 @@ def main():
     # + f32 col (4, 4, 4, 4, 8) = 96 bytes, plus 12 validity bits
     in_bytes = R * in_bytes_per_row
     out_bytes = groups * out_bytes_per_group
-    agg_n, agg_ms = kern["aggregate"]
-    agg_avg_ms = agg_ms / max(agg_n, 1)
+    # the dominant kernel: the longest of the passes that each read every row
+    # once (its algorithmic bytes: the input columns, read once)
+    dom = max((k for k in ("aggregate", "lds_aggregate", "partition_scatter") if k in kern),
+              key=lambda k: kern[k][1] / kern[k][0])
+    agg_n, agg_ms = kern[dom]
+    agg_avg_ms = agg_ms / agg_n
     achieved = in_bytes / (agg_avg_ms * 1e-3) / 1e9
+    traffic = read_traffic(args.traffic_file, KERNEL_SYMBOL[dom]) if args.workload == "c2" else None
     call_gbs = (in_bytes + out_bytes) / (ms_per_step * 1e-3) / 1e9
 
     cpu = None
@@ -582,12 +672,13 @@ def main():
         "config": {"workload": description, "rows": R, "groups": groups, "parallelism": "single GPU"},
         "roofline": {
             "bound": "hbm",
-            "kernel": "aggregate_kernel",
+            "kernel": KERNEL_SYMBOL[dom],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": os.path.relpath(args.traffic_file, ROOT) if traffic is not None else None,
             "algorithmic_bytes_per_launch": in_bytes,
             "avg_launch_ms": agg_avg_ms,
         },

@@ -1205,7 +1205,7 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
       MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)geom.bytes));
       {
-        ProfScope prof("aggregate", stream);
+        ProfScope prof("lds_aggregate", stream);
         lds_agg_kernel<<<(unsigned)items, kLdsBlock, geom.bytes, stream>>>(lp);
       }
       MGB_HIP(hipGetLastError());
@@ -1496,7 +1496,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)geom.bytes));
     {
-      ProfScope prof("aggregate", stream);
+      ProfScope prof("lds_aggregate", stream);
       lds_agg_kernel<<<(unsigned)item_grid, kLdsBlock, geom.bytes, stream>>>(lp);
     }
     MGB_HIP(hipGetLastError());

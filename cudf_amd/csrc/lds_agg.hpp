@@ -68,7 +68,7 @@ __device__ __forceinline__ void lds_aggregate_rows(const AggParams& p, const Lds
   // headers of all rows read together, the full probe only on a miss), then
   // each value column updates the rows' records (lds_column_rows). Rows whose
   // group does not fit go to the global table afterwards.
-  constexpr int kRows = 4;
+  constexpr int kRows = 3;
   constexpr int kPreCols = 3;  // the switch below picks pre[0..2]
   const int64_t step = kLdsBlock;
   const KeyCol& k0 = p.keys[0];

@@ -73,8 +73,9 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_1024(uint32_t v, uint32
 constexpr int kScatterItems = 4;                        // rows per thread per tile (at most)
 constexpr int kScatterTile = kLdsBlock * kScatterItems;  // 4096 rows
 
-// Tiled scatter of chunk `chunk` (rows [chunk * pp.chunk, +pp.chunk)): each tile of rows is counting-sorted by partition in LDS
-// (one LDS atomic per row gives its rank inside its partition's run) and
+// Tiled scatter of chunk `chunk` (rows [chunk * pp.chunk, +pp.chunk)): each
+// tile of rows is counting-sorted by partition in LDS (one LDS atomic per row
+// gives its rank inside its partition's run) and
 // staged as records; consecutive threads then copy consecutive 8-byte words
 // of the sorted records, so each partition's run leaves as one contiguous
 // stretch. A thread's rows of a tile are handled column by column: the key
