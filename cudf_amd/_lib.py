@@ -71,6 +71,7 @@ class mgb_batch_options(ctypes.Structure):
         ("use_graph", ctypes.c_int32),
         ("direct_item_rows", ctypes.c_int64),
         ("estimate_scale", ctypes.c_double),
+        ("estimates", ctypes.POINTER(ctypes.c_int64)),
     ]
 
 
@@ -101,7 +102,8 @@ class mgb_server_request(ctypes.Structure):
 class mgb_server_stats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int64) for name in (
         "submitted", "completed", "failed", "batches", "deferred", "reruns", "peak_held_bytes", "resident_bytes",
-        "held_bytes", "pending")] + [("mean_batch_queries", ctypes.c_double)]
+        "held_bytes", "pending")] + [(name, ctypes.c_double) for name in (
+            "mean_batch_queries", "host_plan_ms", "host_finish_ms")]
 
 
 # Every symbol include/mgb_groupby.h declares, with its ctypes signature.
@@ -117,6 +119,10 @@ SIGNATURES = {
             ctypes.POINTER(mgb_out_column), ctypes.POINTER(mgb_out_column),
             ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.POINTER(mgb_allocator),
         ],
+    ),
+    "mgb_estimate_groups": (
+        ctypes.c_int,
+        [ctypes.POINTER(mgb_column), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)],
     ),
     "mgb_murmurhash3_x86_32": (
         ctypes.c_int,

@@ -35,7 +35,7 @@ class GroupByBatch:
     def __init__(self, queries: Sequence[Tuple[Table, List[GroupByRequest]]],
                  null_handling: NullPolicy = NullPolicy.EXCLUDE, force_mode: Optional[int] = None,
                  use_graph: bool = True, direct_item_rows: int = 0, estimate_scale: float = 1.0,
-                 stream: Optional[torch.cuda.Stream] = None):
+                 estimates: Optional[Sequence[int]] = None, stream: Optional[torch.cuda.Stream] = None):
         self._lib = _lib.load()
         if not queries:
             raise ValueError("a batch needs at least one query")
@@ -65,6 +65,11 @@ class GroupByBatch:
         opt.use_graph = 1 if use_graph else 0
         opt.direct_item_rows = int(direct_item_rows)
         opt.estimate_scale = float(estimate_scale)
+        if estimates is not None:
+            if len(estimates) != Q:
+                raise ValueError("one estimate per query")
+            self._estimates = (ctypes.c_int64 * Q)(*[int(e) for e in estimates])
+            opt.estimates = ctypes.cast(self._estimates, ctypes.POINTER(ctypes.c_int64))
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         handle = ctypes.c_void_p()
         _lib.check(self._lib.mgb_batch_create(keys, nk, int(NullPolicy(null_handling)), reqs, nreq, Q,

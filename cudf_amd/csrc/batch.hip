@@ -57,14 +57,25 @@ struct BatchItem {
   int32_t query;
   int32_t kind;
   int32_t part;        // BI_PART: partition
-  int32_t pad;
-  int64_t begin, end;  // BI_DIRECT / BI_GLOBAL: rows
+  int32_t nsub;        // BI_PART: the partition is cut into nsub items ...
+  int64_t begin, end;  // BI_DIRECT / BI_GLOBAL: rows; BI_PART: begin = which of them
 };
 
 struct BatchChunk {
   int32_t query;
   int32_t chunk;
 };
+
+// A stretch of one query's global table (records [first, first + count)),
+// the unit of work of the fill and finalize kernels.
+struct BatchSeg {
+  int32_t query;
+  int32_t pad;
+  int64_t first;
+  int64_t count;
+};
+
+constexpr int64_t kSegRecords = 8192;
 
 struct BatchParams {
   const BatchQuery* queries;
@@ -82,6 +93,8 @@ struct BatchParams {
   int32_t part_cap;
   uint32_t* hist;                 // [hist_entries + 1] partition counts; the last entry stays 0
   int64_t hist_entries;
+  const BatchSeg* segs;           // the global tables, cut into stretches
+  int64_t num_segs;
 };
 
 // ---------------------------------------------------------------- kernels
@@ -137,18 +150,19 @@ __global__ __launch_bounds__(kBlock) void batch_sample_kernel(const BatchQuery* 
   if (threadIdx.x == 3) est[(int64_t)blockIdx.x * 4 + 3] = (unsigned long long)S;
 }
 
-// Identity records into every query's global table (one workgroup per query).
+// Identity records into every query's global table (one workgroup per
+// stretch of a table, one thread per record).
 __global__ __launch_bounds__(kBlock) void batch_fill_kernel(const BatchParams bp, uint64_t special_hdr)
 {
-  const BatchQuery& Q = bp.queries[blockIdx.x];
   if (blockIdx.x == 0 && threadIdx.x == 0 && bp.hist) bp.hist[bp.hist_entries] = 0u;
-  const int words = (int)(Q.ap.stride / 8);
-  uint64_t* t = reinterpret_cast<uint64_t*>(Q.ap.table);
-  const int64_t total = (Q.ap.capacity + 1) * words;
-  for (int64_t i = threadIdx.x; i < total; i += kBlock) {
-    const int64_t rec = i / words;
-    const int w = (int)(i - rec * words);
-    t[i] = (w == 0 && rec == Q.ap.capacity) ? special_hdr : bp.tmpl[w];
+  const BatchSeg sg = bp.segs[blockIdx.x];
+  const AggParams& p = bp.queries[sg.query].ap;
+  const int words = (int)(p.stride / 8);
+  global_t<uint64_t>* t = to_global(reinterpret_cast<uint64_t*>(p.table));
+  for (int64_t rec = sg.first + threadIdx.x; rec < sg.first + sg.count; rec += kBlock) {
+    global_t<uint64_t>* r = t + rec * words;
+    r[0] = rec == p.capacity ? special_hdr : bp.tmpl[0];
+    for (int w = 1; w < words; ++w) r[w] = bp.tmpl[w];
   }
 }
 
@@ -200,13 +214,17 @@ __global__ __launch_bounds__(kLdsBlock) void batch_items_kernel(const BatchParam
       int64_t begin = item.begin, end = item.end;
       if (part) {
         const int64_t nchunks = Q.pp.nblocks;
-        begin = bp.offsets[Q.hist_base + (int64_t)item.part * nchunks];
-        end = bp.offsets[Q.hist_base + (int64_t)(item.part + 1) * nchunks];
+        const int64_t ps = bp.offsets[Q.hist_base + (int64_t)item.part * nchunks];
+        const int64_t pe = bp.offsets[Q.hist_base + (int64_t)(item.part + 1) * nchunks];
+        begin = ps + (pe - ps) * item.begin / item.nsub;
+        end = ps + (pe - ps) * (item.begin + 1) / item.nsub;
       }
       const LdsTable t = lds_table_at(lds, part ? bp.part_slots : Q.slots, bp.words, part ? bp.part_cap : Q.cap);
       lds_table_init(t, bp.tmpl, bp.words);
       lds_aggregate_rows(p, t, lds, begin, end, part ? Q.pp.log2p : 0, !part);
-      if (part && !*t.overflow) lds_write_groups(p, t, Q.out_count, Q.out_base, *bp.out);
+      // a partition aggregated whole writes its groups; parts of one fold
+      // them into the query's table, as does an item whose groups overflowed
+      if (part && item.nsub == 1 && !*t.overflow) lds_write_groups(p, t, Q.out_count, Q.out_base, *bp.out);
       else lds_fold_to_global(p, t);
     }
     __syncthreads();  // the next item reuses the LDS
@@ -214,36 +232,42 @@ __global__ __launch_bounds__(kLdsBlock) void batch_items_kernel(const BatchParam
 }
 
 // Groups of every query's global table -> output rows (one workgroup per
-// query; one output-counter atomic per workgroup pass).
+// stretch of a table; one output-counter atomic per workgroup pass).
 __global__ __launch_bounds__(kBlock) void batch_finalize_kernel(const BatchParams bp)
 {
   __shared__ uint32_t scan[kBlock];
   __shared__ unsigned long long base;
-  const BatchQuery& Q = bp.queries[blockIdx.x];
+  const BatchSeg sg = bp.segs[blockIdx.x];
+  const BatchQuery& Q = bp.queries[sg.query];
   // a partitioned query's table holds spilled groups whose representative
   // rows are partitioned records
   const AggParams& p = Q.mode == BM_PARTITIONED ? Q.rec_ap : Q.ap;
-  const int64_t n = p.capacity + 1;
-  for (int64_t i0 = 0; i0 < n; i0 += kBlock) {
+  const int64_t end = sg.first + sg.count;
+  for (int64_t i0 = sg.first; i0 < end; i0 += kBlock) {
     const int64_t i = i0 + threadIdx.x;
     bool occ = false;
-    if (i < n) {
+    if (i < end) {
       const uint64_t hdr = *reinterpret_cast<const uint64_t*>(p.table + i * (int64_t)p.stride);
       if (i == p.capacity) occ = p.key_mode == KEY_INT && hdr == 1ull;
       else occ = hdr != (p.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef);
     }
-    scan[threadIdx.x] = occ ? 1u : 0u;
+    const unsigned long long ballot = __ballot(occ);
+    const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) scan[wave] = (uint32_t)__popcll(ballot);
     __syncthreads();
-    for (int d = 1; d < kBlock; d <<= 1) {
-      const uint32_t v = threadIdx.x >= (unsigned)d ? scan[threadIdx.x - d] : 0u;
-      __syncthreads();
-      scan[threadIdx.x] += v;
-      __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t total = 0;
+      for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t c = scan[w];
+        scan[w] = total;
+        total += c;
+      }
+      base = total ? atomicAdd(Q.out_count, (unsigned long long)total) : 0ull;
     }
-    if (threadIdx.x == kBlock - 1) base = scan[kBlock - 1] ? atomicAdd(Q.out_count, (unsigned long long)scan[kBlock - 1]) : 0ull;
     __syncthreads();
     if (occ) {
-      const int64_t pos = Q.out_base + (int64_t)base + scan[threadIdx.x] - 1;
+      const uint32_t rank = scan[wave] + (uint32_t)__popcll(ballot & ((1ull << lane) - 1ull));
+      const int64_t pos = Q.out_base + (int64_t)base + rank;
       const uint8_t* rec = p.table + i * (int64_t)p.stride;
       finalize_keys(*reinterpret_cast<const uint64_t*>(rec), i == p.capacity, p.key_mode, p.keys, p.num_keys, pos,
                     *bp.out);
@@ -304,6 +328,7 @@ struct QueryPlan {
   int64_t capacity = 0;   // global table records
   int64_t item_rows = 0;  // direct / global items
   int log2p = 0;          // partitioned
+  int nsub = 1;           // partitioned: items per partition
   int64_t nchunks = 0;
   int64_t chunk_rows = 0;
   int64_t hist_base = 0;
@@ -340,6 +365,13 @@ struct mgb_batch {
   std::vector<uint8_t*> key_out, out_data, out_valid;
   std::vector<uint32_t*> out_mask;
   std::vector<unsigned long long> host_counters;
+  // host copies of the uploaded descriptors (kept for the uploads' lifetime)
+  std::vector<BatchQuery> h_queries;
+  std::vector<BatchItem> h_items;
+  std::vector<BatchChunk> h_chunks;
+  std::vector<BatchSeg> h_segs;
+  uint64_t h_tmpl[kMaxRecordWords] = {};
+  OutTable h_ot{};
   hipStream_t alloc_stream = nullptr;        // stream-ordered allocation and release of `dev`
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -352,7 +384,7 @@ struct mgb_batch {
   mgb_batch_stats stats{};
   // launch geometry
   size_t hist_lds = 0, scatter_lds = 0, items_lds = 0;
-  int64_t num_chunks = 0, num_items = 0, hist_entries = 0;
+  int64_t num_chunks = 0, num_items = 0, hist_entries = 0, num_segs = 0;
   int items_grid = 0;
   uint32_t* bsum = nullptr;  // block sums of the histogram scan
   int64_t nbh = 0;
@@ -446,6 +478,17 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
   if (options) opt = *options;
   if (!(opt.estimate_scale > 0.0)) opt.estimate_scale = 1.0;
 
+  // batches allocate stream-ordered from the device's default pool: keep
+  // released memory in the pool instead of returning it at every
+  // synchronisation, so that planning a batch does not map fresh memory
+  {
+    int dev = 0;
+    hipMemPool_t pool = nullptr;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+  }
   std::unique_ptr<mgb_batch> b(new mgb_batch());
   b->Q = num_queries;
   b->num_keys = num_keys;
@@ -491,12 +534,19 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
     if (st != MGB_SUCCESS) return st;
   }
 
-  // ---- distinct-count estimates (one sync)
+  // ---- distinct-count estimates: given, or sampled on the device (one sync)
   const int lds_words = lds_stride_of(plan.stride) / 8;
   const LdsGeom geom = lds_geometry(plan.stride);
   if (!geom.ok) return fail(MGB_ERROR_UNSUPPORTED, "record too large for the LDS executions of a batch");
   b->qp.resize(Q);
-  {
+  if (opt.estimates) {
+    for (int q = 0; q < Q; ++q) {
+      const int64_t R = hq[q].ap.num_rows;
+      const int64_t hi = (int64_t)std::ceil((double)opt.estimates[q] * 1.1 * opt.estimate_scale) + 16;
+      b->qp[q].rows = R;
+      b->qp[q].est = std::max<int64_t>(1, std::min<int64_t>(hi, R));
+    }
+  } else {
     BatchQuery* d_q = nullptr;
     unsigned long long* d_est = nullptr;
     MGB_HIP(hipMallocAsync((void**)&d_q, sizeof(BatchQuery) * Q, stream));
@@ -532,6 +582,15 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
   // its queue position in static LDS
   const int part_slots = geom.slots - 1, part_cap = (int)((geom.slots - 1) * 0.62);
   int max_log2p = 0;
+  // work-item and chunk sizes from the batch's total rows: enough items and
+  // chunks to fill every CU a few times over, whether the batch holds one
+  // query or thousands
+  const double cus = (double)device_cu_count();
+  int64_t total_rows = 0;
+  for (int q = 0; q < Q; ++q) total_rows += b->qp[q].rows;
+  const int64_t fill_rows = std::max<int64_t>(1, total_rows / (int64_t)(4 * cus));
+  const int64_t item_rows = opt.direct_item_rows > 0 ? opt.direct_item_rows
+                                                     : std::max<int64_t>(16384, std::min<int64_t>(262144, fill_rows));
   for (int q = 0; q < Q; ++q) {
     QueryPlan& P = b->qp[q];
     const int64_t R = P.rows, G = P.est;
@@ -539,10 +598,7 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
     int64_t slots = std::max<int64_t>(256, next_pow2(2 * G));
     const bool direct_ok = slots <= part_slots && G <= part_cap;
     if (slots > part_slots) slots = part_slots;
-    const int64_t item_rows = opt.direct_item_rows > 0 ? opt.direct_item_rows
-                                                       : std::max<int64_t>(65536, std::min<int64_t>(262144, R));
     const int64_t items = std::max<int64_t>(1, (R + item_rows - 1) / item_rows);
-    const double cus = (double)device_cu_count();
     const double c_direct = direct_ok ? R * cm.lds_row + items * ((cm.item + slots * cm.slot_init) / cus +
                                                                   std::min<int64_t>(G, item_rows) * nops * cm.atomic)
                                       : 1e30;
@@ -559,16 +615,35 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
     P.cost = mode == BM_DIRECT ? c_direct : mode == BM_GLOBAL ? c_global : c_part;
     P.slots = (int)slots;
     P.cap = (int)std::max<int64_t>(1, (int64_t)(slots * 0.62));
-    P.item_rows = mode == BM_GLOBAL ? std::max<int64_t>(16384, std::min<int64_t>(262144, R)) : item_rows;
+    P.item_rows = item_rows;
     if (mode == BM_PARTITIONED) {
       P.log2p = log2p;
       max_log2p = std::max(max_log2p, log2p);
-      P.chunk_rows = 1 << 19;
-      P.nchunks = std::max<int64_t>(1, (R + P.chunk_rows - 1) / P.chunk_rows);
-      // spill table: room for two overflowing partitions
-      P.capacity = next_pow2(4 * (int64_t)part_cap + 1024);
+      // partitions much longer than a work item are cut into several items
+      // (all of whose groups then go to the query's table)
+      const int64_t part_item_rows = std::max<int64_t>(65536, fill_rows);
+      P.nsub = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (R >> log2p) / part_item_rows));
+      // its table: every group when partitions are cut, else room for two
+      // overflowing partitions
+      P.capacity = P.nsub > 1 ? next_pow2(2 * G + 1024) : next_pow2(4 * (int64_t)part_cap + 1024);
     } else {
       P.capacity = next_pow2(std::max<int64_t>(64, 2 * G + 16));
+    }
+  }
+
+  // histogram / scatter chunks of the partitioned queries (multiples of the
+  // scatter tile)
+  {
+    int64_t part_rows = 0;
+    for (int q = 0; q < Q; ++q)
+      if (b->qp[q].mode == BM_PARTITIONED) part_rows += b->qp[q].rows;
+    int64_t chunk = (int64_t)align_up((size_t)std::max<int64_t>(1, part_rows / (int64_t)(4 * cus)), kScatterTile);
+    chunk = std::max<int64_t>(8 * kScatterTile, std::min<int64_t>(1 << 19, chunk));
+    for (int q = 0; q < Q; ++q) {
+      QueryPlan& P = b->qp[q];
+      if (P.mode != BM_PARTITIONED) continue;
+      P.chunk_rows = chunk;
+      P.nchunks = std::max<int64_t>(1, (P.rows + chunk - 1) / chunk);
     }
   }
 
@@ -629,10 +704,11 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
     if (P.mode == BM_PARTITIONED) {
       for (int64_t c = 0; c < P.nchunks; ++c) chunks.push_back(BatchChunk{q, (int32_t)c});
       const int np = 1 << P.log2p;
-      for (int part = 0; part < np; ++part) {
-        items.push_back(BatchItem{q, BI_PART, part, 0, 0, 0});
-        item_cost.push_back(P.cost / np);
-      }
+      for (int part = 0; part < np; ++part)
+        for (int k = 0; k < P.nsub; ++k) {
+          items.push_back(BatchItem{q, BI_PART, part, P.nsub, k, 0});
+          item_cost.push_back(P.cost / ((double)np * P.nsub));
+        }
     } else {
       for (int64_t r = 0; r < P.rows; r += P.item_rows) {
         items.push_back(BatchItem{q, P.mode == BM_DIRECT ? BI_DIRECT : BI_GLOBAL, 0, 0, r,
@@ -651,6 +727,13 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
   }
   b->num_items = (int64_t)items.size();
   b->num_chunks = (int64_t)chunks.size();
+  std::vector<BatchSeg> segs;
+  for (int q = 0; q < Q; ++q) {
+    const int64_t n = b->qp[q].capacity + 1;
+    for (int64_t f = 0; f < n; f += kSegRecords)
+      segs.push_back(BatchSeg{q, 0, f, std::min<int64_t>(kSegRecords, n - f)});
+  }
+  b->num_segs = (int64_t)segs.size();
 
   // one device allocation
   const int nk = num_keys, no = b->num_outs;
@@ -663,6 +746,7 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
   const size_t o_queries = take(sizeof(BatchQuery) * Q);
   const size_t o_items = take(sizeof(BatchItem) * items.size());
   const size_t o_chunks = take(sizeof(BatchChunk) * chunks.size());
+  const size_t o_segs = take(sizeof(BatchSeg) * segs.size());
   const size_t o_counters = take(sizeof(unsigned long long) * (4 * (size_t)Q + 8));
   const size_t o_tables = take((size_t)table_records * stride);
   const size_t o_hist = take(sizeof(uint32_t) * (size_t)(hist_entries + 1));
@@ -798,20 +882,33 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
   bp.part_slots = part_slots;
   bp.part_cap = part_cap;
   bp.hist = hist_entries ? reinterpret_cast<uint32_t*>(base + o_hist) : nullptr;
+  bp.segs = reinterpret_cast<const BatchSeg*>(base + o_segs);
+  bp.num_segs = (int64_t)segs.size();
   bp.hist_entries = hist_entries;
   b->bsum = reinterpret_cast<uint32_t*>(base + o_bsum);
   b->nbh = nbh;
 
-  MGB_HIP(hipMemcpyAsync(base + o_queries, hq.data(), sizeof(BatchQuery) * Q, hipMemcpyHostToDevice, stream));
-  if (!items.empty())
-    MGB_HIP(hipMemcpyAsync(base + o_items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice, stream));
-  if (!chunks.empty())
-    MGB_HIP(hipMemcpyAsync(base + o_chunks, chunks.data(), sizeof(BatchChunk) * chunks.size(), hipMemcpyHostToDevice,
-                           stream));
-  MGB_HIP(hipMemcpyAsync(base + o_tmpl, tmpl, sizeof(tmpl), hipMemcpyHostToDevice, stream));
-  MGB_HIP(hipMemcpyAsync(base + o_ot, &ot, sizeof(OutTable), hipMemcpyHostToDevice, stream));
+  // the host copies stay with the batch, so that these asynchronous uploads
+  // need no synchronisation here
+  b->h_queries = std::move(hq);
+  b->h_items = std::move(items);
+  b->h_chunks = std::move(chunks);
+  b->h_segs = std::move(segs);
+  std::memcpy(b->h_tmpl, tmpl, sizeof(tmpl));
+  b->h_ot = ot;
+  MGB_HIP(hipMemcpyAsync(base + o_queries, b->h_queries.data(), sizeof(BatchQuery) * Q, hipMemcpyHostToDevice,
+                         stream));
+  if (!b->h_items.empty())
+    MGB_HIP(hipMemcpyAsync(base + o_items, b->h_items.data(), sizeof(BatchItem) * b->h_items.size(),
+                           hipMemcpyHostToDevice, stream));
+  if (!b->h_chunks.empty())
+    MGB_HIP(hipMemcpyAsync(base + o_chunks, b->h_chunks.data(), sizeof(BatchChunk) * b->h_chunks.size(),
+                           hipMemcpyHostToDevice, stream));
+  MGB_HIP(hipMemcpyAsync(base + o_segs, b->h_segs.data(), sizeof(BatchSeg) * b->h_segs.size(), hipMemcpyHostToDevice,
+                         stream));
+  MGB_HIP(hipMemcpyAsync(base + o_tmpl, b->h_tmpl, sizeof(b->h_tmpl), hipMemcpyHostToDevice, stream));
+  MGB_HIP(hipMemcpyAsync(base + o_ot, &b->h_ot, sizeof(OutTable), hipMemcpyHostToDevice, stream));
   b->host_counters.assign(4 * (size_t)Q + 8, 0ull);
-  MGB_HIP(hipStreamSynchronize(stream));
 
   // launch geometry
   b->items_lds = std::max(lds_table_bytes(part_slots, lds_words), (size_t)16);
@@ -855,7 +952,7 @@ mgb_status enqueue_batch(mgb_batch* b, hipStream_t stream)
     fill_u64_kernel<<<(unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 1024)), kBlock, 0,
                       stream>>>(b->counters, n, 0ull);
   }
-  batch_fill_kernel<<<(unsigned)Q, kBlock, 0, stream>>>(bp, b->plan.key_mode == KEY_INT ? 0ull : kEmptyRowRef);
+  batch_fill_kernel<<<(unsigned)b->num_segs, kBlock, 0, stream>>>(bp, b->plan.key_mode == KEY_INT ? 0ull : kEmptyRowRef);
   if (b->num_chunks > 0) {
     batch_hist_kernel<<<(unsigned)b->num_chunks, kBlock, b->hist_lds, stream>>>(bp);
     const int64_t n = b->hist_entries + 1;
@@ -866,7 +963,7 @@ mgb_status enqueue_batch(mgb_batch* b, hipStream_t stream)
     batch_scatter_kernel<<<(unsigned)b->num_chunks, kLdsBlock, b->scatter_lds, stream>>>(bp);
   }
   if (b->num_items > 0) batch_items_kernel<<<(unsigned)b->items_grid, kLdsBlock, b->items_lds, stream>>>(bp);
-  batch_finalize_kernel<<<(unsigned)Q, kBlock, 0, stream>>>(bp);
+  batch_finalize_kernel<<<(unsigned)b->num_segs, kBlock, 0, stream>>>(bp);
   const int no = b->num_outs;
   const int64_t words = bitmask_bytes(std::max<int64_t>(b->total_out, 1)) / 4;
   for (int o = 0; o < no; ++o) {

@@ -1318,12 +1318,16 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     const int64_t nh = P * nw;
     const int64_t nb = (nh + kScanTile - 1) / kScanTile;
     const int64_t nb_items = (P + 1 + kScanTile - 1) / kScanTile;
-    // work items: partitions longer than max_item_rows are split
-    const int64_t max_item_rows = std::max<int64_t>(65536, 4 * ((R + P - 1) / P));
+    // work items: partitions longer than max_item_rows are split, both for
+    // skew (four times the mean partition) and for parallelism (few large
+    // partitions would leave most CUs idle)
+    const int64_t par_rows = (R + 2 * (int64_t)cus - 1) / (2 * (int64_t)cus);
+    const int64_t max_item_rows = std::max<int64_t>(65536, std::min<int64_t>(4 * ((R + P - 1) / P), par_rows));
     const int64_t item_grid = P + (R + max_item_rows - 1) / max_item_rows;
-    // global table for the groups of split or spilled items
+    // global table for the groups of split or spilled items: no more than the
+    // items could hold, nor than the groups there are
     const int64_t split_bound = (R + max_item_rows - 1) / max_item_rows + P / 16 + 1;
-    int64_t gcap = 2 * std::min<int64_t>(R, split_bound * (int64_t)(geom.slots + 1)) + 1024;
+    int64_t gcap = 2 * std::min<int64_t>({R, split_bound * (int64_t)(geom.slots + 1), 2 * est}) + 1024;
     gcap = std::min<int64_t>(gcap, 2 * R + 64);
     const int64_t gblocks = (gcap + 1 + kBlock - 1) / kBlock;
 
@@ -1661,6 +1665,40 @@ extern "C" mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_
   mgb_status st = build_plan(keys, num_keys, null_policy, requests, num_requests, plan);
   if (st != MGB_SUCCESS) return st;
   return run_groupby(plan, R, out_keys, out_results, out_num_groups, static_cast<hipStream_t>(stream), alloc);
+}
+
+extern "C" mgb_status mgb_estimate_groups(const mgb_column* keys, int32_t num_keys, int32_t null_policy, void* stream_v,
+                                          int64_t* estimate)
+{
+  if (!estimate) return fail(MGB_ERROR_LOGIC, "estimate is NULL");
+  if (num_keys <= 0 || !keys) return fail(MGB_ERROR_LOGIC, "no key columns");
+  const int64_t R = keys[0].size;
+  for (int c = 1; c < num_keys; ++c)
+    if (keys[c].size != R) return fail(MGB_ERROR_LOGIC, "key columns differ in size");
+  Plan plan;
+  mgb_status st = build_plan(keys, num_keys, null_policy, nullptr, 0, plan);
+  if (st != MGB_SUCCESS) return st;
+  if (R == 0) {
+    *estimate = 0;
+    return MGB_SUCCESS;
+  }
+  hipStream_t stream = static_cast<hipStream_t>(stream_v);
+  Workspace* ws = workspace_for(stream);
+  std::lock_guard<std::mutex> lk(ws->mu);
+  if (!ws->pinned) {
+    MGB_HIP(hipHostMalloc((void**)&ws->pinned, kMaxPinned * sizeof(unsigned long long), hipHostMallocDefault));
+    MGB_HIP(hipHostMalloc((void**)&ws->staging, kStagingBytes, hipHostMallocDefault));
+  }
+  AggParams base{};
+  base.num_rows = R;
+  base.key_mode = plan.key_mode;
+  base.num_keys = (int)plan.keys.size();
+  base.skip_null_keys = plan.skip_null_keys;
+  base.compare_key_nulls = plan.compare_key_nulls;
+  base.stride = plan.stride;
+  for (size_t c = 0; c < plan.keys.size(); ++c) base.keys[c] = plan.keys[c];
+  *estimate = sample_estimate(ws, base, R, stream, st);
+  return st;
 }
 
 extern "C" mgb_status mgb_murmurhash3_x86_32(const mgb_column* columns, int32_t num_columns, uint32_t seed,

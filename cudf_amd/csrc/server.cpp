@@ -43,6 +43,7 @@ struct Query {
   int32_t null_policy = 0;
   std::vector<mgb_column> keys;   // the table's columns, resolved at submission
   std::vector<mgb_column> vals;
+  std::string group_key;          // table and key columns: the cache key of its group-count estimate
   std::vector<std::vector<mgb_aggregation>> aggs;
   std::string signature;  // queries of one batch share it
   Clock::time_point arrival;
@@ -76,18 +77,21 @@ struct Slot {
   struct mgb_server* server = nullptr;
 };
 
-void* dev_alloc(void*, size_t bytes, void*)
+// Results of the plain path: stream-ordered allocations from the default
+// pool (hipFree would wait for the whole device).
+void* dev_alloc(void*, size_t bytes, void* stream)
 {
   void* p = nullptr;
-  return hipMalloc(&p, bytes) == hipSuccess ? p : nullptr;
+  return hipMallocAsync(&p, bytes, static_cast<hipStream_t>(stream)) == hipSuccess ? p : nullptr;
 }
-void dev_free(void*, void* p, size_t, void*) { (void)hipFree(p); }
+void dev_free(void*, void* p, size_t, void* stream) { (void)hipFreeAsync(p, static_cast<hipStream_t>(stream)); }
 
+// Released after the query's work and its copies have completed.
 void free_out(std::vector<mgb_out_column>& cols)
 {
   for (auto& c : cols) {
-    if (c.data) (void)hipFree(c.data);
-    if (c.null_mask) (void)hipFree(c.null_mask);
+    if (c.data) (void)hipFreeAsync(c.data, nullptr);
+    if (c.null_mask) (void)hipFreeAsync(c.null_mask, nullptr);
     c.data = nullptr;
     c.null_mask = nullptr;
   }
@@ -111,6 +115,7 @@ struct mgb_server {
   std::thread scheduler;
   mgb_server_stats stats{};
   std::vector<double> batch_sizes;
+  std::map<std::string, int64_t> estimates;  // distinct key rows per (table, key columns); tables are immutable
 
   void run();
   void retire(Slot& s);
@@ -120,13 +125,15 @@ struct mgb_server {
 
 namespace {
 
-// Host callback after a slot's batch (a runtime thread; takes no lock, so
-// that no HIP call waiting for this callback can wait on a lock holder; the
-// scheduler's timed wait covers a wake-up that races its predicate check).
+// Host callback after a slot's batch (a runtime thread). It takes the
+// server's lock only to order its wake-up after the scheduler's predicate
+// check; no HIP call is ever made under that lock, so nothing holding it can
+// be waiting for this callback.
 void slot_done(void* arg)
 {
   Slot* s = static_cast<Slot*>(arg);
   s->finished.store(true);
+  { std::lock_guard<std::mutex> lk(s->server->mu); }
   s->server->cv.notify_all();
 }
 
@@ -159,6 +166,7 @@ void mgb_server::fail_query(const Query& q, mgb_status st, const std::string& ms
 // Completed slot -> results (the scheduler thread, lock not held).
 void mgb_server::retire(Slot& s)
 {
+  const auto t0 = Clock::now();
   std::vector<Done> out(s.queries.size());
   if (s.batch) {
     mgb_status st = mgb_batch_finish(s.batch->batch, s.stream);
@@ -177,16 +185,17 @@ void mgb_server::retire(Slot& s)
     std::lock_guard<std::mutex> lk(mu);
     stats.reruns += bs.reruns;
   }
+  std::shared_ptr<HeldBatch> last = std::move(s.batch);  // released below, outside the lock
   std::lock_guard<std::mutex> lk(mu);
   for (size_t i = 0; i < s.queries.size(); ++i) {
-    if (!s.batch) continue;
+    if (!last) continue;
     done[s.queries[i].ticket] = std::move(out[i]);
     ++stats.completed;
   }
-  s.batch.reset();
   s.queries.clear();
   s.busy = false;
   s.finished.store(false);
+  stats.host_finish_ms += ms_since(t0, Clock::now());
   done_cv.notify_all();
 }
 
@@ -254,6 +263,21 @@ bool mgb_server::launch(Slot& s)
         reqs[(size_t)q * nr + r].num_aggs = (int32_t)take[q].aggs[r].size();
       }
     }
+    // group counts of resident tables: estimated once per table and key
+    // columns (one synchronisation, the first time), then planned from the
+    // cache with no device round trip
+    std::vector<int64_t> est(Q);
+    for (int q = 0; q < Q; ++q) {
+      auto it = estimates.find(take[q].group_key);  // only this thread touches the cache
+      if (it == estimates.end()) {
+        int64_t e = 0;
+        if (mgb_estimate_groups(take[q].keys.data(), (int32_t)take[q].keys.size(), take[q].null_policy, s.stream,
+                                &e) != MGB_SUCCESS)
+          e = take[q].keys.empty() ? 1 : take[q].keys[0].size;
+        it = estimates.emplace(take[q].group_key, e).first;
+      }
+      est[q] = it->second;
+    }
     // a served batch runs once: plain launches (capturing a graph for a
     // single execution only adds its capture and instantiation)
     mgb_batch_options bo{};
@@ -261,8 +285,15 @@ bool mgb_server::launch(Slot& s)
     bo.use_graph = 0;
     bo.direct_item_rows = 0;
     bo.estimate_scale = 1.0;
+    bo.estimates = est.data();
     mgb_batch* b = nullptr;
+    const auto t_plan = Clock::now();
     mgb_status st = mgb_batch_create(keys.data(), nk, take[0].null_policy, reqs.data(), nr, Q, &bo, s.stream, &b);
+    const double plan_ms = ms_since(t_plan, Clock::now());
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stats.host_plan_ms += plan_ms;
+    }
     if (st != MGB_SUCCESS) {
       const std::string msg = mgb_last_error_message();
       std::lock_guard<std::mutex> lk(mu);
@@ -333,7 +364,7 @@ void mgb_server::run()
   for (;;) {
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv.wait_for(lk, std::chrono::milliseconds(2), [&] {
+      cv.wait_for(lk, std::chrono::milliseconds(1), [&] {
         if (stop) return true;
         for (auto& s : slots)
           if (s->busy && s->finished.load()) return true;
@@ -418,7 +449,11 @@ extern "C" mgb_status mgb_server_submit(mgb_server* s, int32_t table_id, const i
         return fail(MGB_ERROR_LOGIC, "request out of range");
     q.table = table_id;
     const Table& t = s->tables[table_id];
-    for (int k = 0; k < num_keys; ++k) q.keys.push_back(t.cols[key_columns[k]]);
+    q.group_key = std::to_string(table_id) + "|" + std::to_string(null_policy) + "|";
+    for (int k = 0; k < num_keys; ++k) {
+      q.keys.push_back(t.cols[key_columns[k]]);
+      q.group_key += std::to_string(key_columns[k]) + ",";
+    }
     for (int r = 0; r < num_requests; ++r) {
       q.vals.push_back(t.cols[requests[r].column]);
       q.aggs.emplace_back(requests[r].aggs, requests[r].aggs + requests[r].num_aggs);

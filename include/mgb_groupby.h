@@ -155,6 +155,13 @@ mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_keys, int32
  * (cpp/include/cudf/hashing.hpp:90-94, cpp/src/hash/murmurhash3_x86_32.cu).
  * `out` is a device array of num_rows uint32.
  */
+/* Estimated number of distinct key rows (groups) of a key table: exact up
+ * to 131072 rows, otherwise from a sample, or HyperLogLog over every row when
+ * the sample does not settle it (the plain path's own estimate). Synchronises
+ * `stream`. No counterpart in the reference. */
+mgb_status mgb_estimate_groups(const mgb_column* keys, int32_t num_keys, int32_t null_policy, void* stream,
+                               int64_t* estimate);
+
 mgb_status mgb_murmurhash3_x86_32(const mgb_column* columns, int32_t num_columns, uint32_t seed,
                                   uint32_t* out, void* stream);
 
@@ -209,6 +216,8 @@ typedef struct mgb_batch_options {
   int32_t use_graph;         /* 1: capture the execution in a HIP graph; 0: plain launches */
   int64_t direct_item_rows;  /* rows per direct / global work item; 0: chosen per query */
   double estimate_scale;     /* multiplies every distinct-count estimate (1.0; below 1 only to test re-runs) */
+  const int64_t* estimates;  /* distinct key rows per query (e.g. cached from mgb_estimate_groups for
+                                resident tables), or NULL: sampled on the device (one synchronisation) */
 } mgb_batch_options;
 
 typedef struct mgb_batch_stats {
@@ -291,6 +300,8 @@ typedef struct mgb_server_stats {
   int64_t held_bytes;
   int64_t pending;
   double mean_batch_queries;
+  double host_plan_ms;        /* scheduler time spent planning batches */
+  double host_finish_ms;      /* scheduler time spent finishing batches */
 } mgb_server_stats;
 
 mgb_status mgb_server_create(const mgb_server_options* options, mgb_server** out);

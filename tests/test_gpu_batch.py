@@ -81,6 +81,19 @@ def test_batch_modes(gpu, mode):
         _check(batch, q, keys, reqs)
 
 
+@pytest.mark.parametrize("mode", [None, 2], ids=["auto", "partitioned"])
+def test_batch_long_partitions_are_split(gpu, mode):
+    # few groups over many rows: a handful of partitions, each cut into
+    # several work items whose groups meet in the query's table
+    host, dev = _batch_inputs(18, shapes=[(2_000_000, 3000), (1_500_000, 2200), (40000, 40)],
+                              val_types=(O.FLOAT64, O.INT32), null_frac=0.02)
+    batch = ca.GroupByBatch(dev, force_mode=mode)
+    print("split modes", [batch.mode(q) for q in range(len(dev))], batch.stats(), flush=True)
+    batch.run()
+    for q, (keys, reqs) in enumerate(host):
+        _check(batch, q, keys, reqs)
+
+
 def test_batch_auto_picks_by_shape(gpu):
     host, dev = _batch_inputs(12)
     batch = ca.GroupByBatch(dev)
@@ -120,6 +133,28 @@ def test_batch_rerun_after_underestimate(gpu):
     batch = ca.GroupByBatch(dev, force_mode=0, estimate_scale=0.01)
     batch.run()
     assert batch.stats()["reruns"] >= 1
+    for q, (keys, reqs) in enumerate(host):
+        _check(batch, q, keys, reqs)
+
+
+def test_batch_with_given_estimates(gpu):
+    # estimates from mgb_estimate_groups (the cache a server keeps for resident
+    # tables) replace the device sample
+    import ctypes
+    from cudf_amd import _lib
+    host, dev = _batch_inputs(19)
+    lib = _lib.load()
+    ests = []
+    for kt, _ in dev:
+        arr = (_lib.mgb_column * 1)(ca.groupby._to_mgb(kt.columns[0]))
+        e = ctypes.c_int64()
+        _lib.check(lib.mgb_estimate_groups(arr, 1, 0, ctypes.c_void_p(0), ctypes.byref(e)))
+        ests.append(e.value)
+    for (keys, _), e in zip(host, ests):
+        exact = len(np.unique(keys[0][1]))
+        assert abs(e - exact) <= max(2, 0.1 * exact), (e, exact)
+    batch = ca.GroupByBatch(dev, estimates=ests)
+    batch.run()
     for q, (keys, reqs) in enumerate(host):
         _check(batch, q, keys, reqs)
 

@@ -232,6 +232,20 @@ def test_hot_key_partition_split(gpu, path):
     _compare([(O.INT64, keys, None)], reqs)
 
 
+def test_few_partitions_many_rows_split(gpu, path):
+    """3M rows over 2,500 groups: a few partitions of about a million rows
+    each, cut into work items for parallelism; their partial groups meet in
+    the global table."""
+    rng = np.random.default_rng(34)
+    n = 3_000_000
+    pool = rng.integers(-2 ** 62, 2 ** 62, 2500, dtype=np.int64)
+    keys = pool[rng.integers(0, 2500, n)]
+    reqs = [((O.FLOAT64, rng.standard_normal(n), rng.random(n) > 0.02), [O.SUM, O.COUNT_VALID, O.MIN, O.MAX, O.MEAN]),
+            ((O.INT32, rng.integers(-1_000_000, 1_000_000, n).astype(np.int32), rng.random(n) > 0.02),
+             [O.SUM, O.MIN, O.MAX, O.MEAN])]
+    _compare([(O.INT64, keys, None)], reqs)
+
+
 def test_low_cardinality_16_keys(gpu, path):
     """The LDS-path workload at 4M rows: int32 key with 16 distinct values."""
     rng = np.random.default_rng(33)
