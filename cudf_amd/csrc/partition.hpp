@@ -75,14 +75,14 @@ constexpr int kScatterTile = kLdsBlock * kScatterItems;  // 4096 rows
 
 // Tiled scatter of chunk `chunk` (rows [chunk * pp.chunk, +pp.chunk)): each
 // tile of rows is counting-sorted by partition in LDS (one LDS atomic per row
-// gives its rank inside its partition's run) and
-// staged as records; consecutive threads then copy consecutive 8-byte words
-// of the sorted records, so each partition's run leaves as one contiguous
-// stretch. A thread's rows of a tile are handled column by column: the key
-// columns and the first kScatterPreVals value columns of all of them are
-// loaded before the ranking and the scan, so one memory latency covers the
-// tile. LDS: per-partition cursor and tile count, partition id per staged
-// row, the staged records.
+// gives its rank inside its partition's run) and staged as records;
+// consecutive threads then copy consecutive 8-byte words of the sorted
+// records, so each partition's run leaves as one contiguous stretch. A
+// thread's rows of a tile are handled column by column. The loads of the next
+// tile (the first two key columns, the first kScatterPreVals value columns and
+// their null-mask words) are issued before the current tile's copy-out, so that
+// reading the input overlaps writing the records. LDS: per-partition cursor
+// and tile count, partition id per staged row, the staged records.
 __device__ __forceinline__ void scatter_chunk(const PartParams& pp, int64_t chunk, uint8_t* smem)
 {
   const AggParams& p = pp.src;
@@ -108,28 +108,72 @@ __device__ __forceinline__ void scatter_chunk(const PartParams& pp, int64_t chun
   const int64_t end = begin + pp.chunk < p.num_rows ? begin + pp.chunk : p.num_rows;
   const int items = tile_rows_max / kLdsBlock;
   const int64_t step = kLdsBlock;
-  for (int64_t tile = begin; tile < end; tile += tile_rows_max) {
+  // up to two key columns without null grouping: the hash comes from the
+  // prefetched keys alone
+  bool pre_keys = p.num_keys <= 2;
+  for (int c = 0; c < p.num_keys && c < 2; ++c) pre_keys = pre_keys && !(p.compare_key_nulls && p.keys[c].mask);
+
+  // the thread's rows of one tile, as loaded
+  bool present[kScatterItems];
+  uint64_t kpre[2][kScatterItems];
+  uint64_t vpre[kScatterPreVals][kScatterItems];
+  uint32_t mword[kScatterPreVals][kScatterItems];
+  auto fetch = [&](int64_t tile) {
     const int64_t row0 = tile + threadIdx.x;
-    bool present[kScatterItems];
 #pragma unroll
     for (int k = 0; k < kScatterItems; ++k) present[k] = k < items && row0 + k * step < end;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      if (c < p.num_keys)
+        load_rows(p.keys[c].data, p.keys[c].width, p.keys[c].stride, VC_UNSIGNED, row0, step, present, kpre[c]);
+#pragma unroll
+    for (int c = 0; c < kScatterPreVals; ++c) {
+      if (c >= p.num_vals) continue;
+      const ValCol& v = p.vals[c];
+      load_rows(v.data, v.width, v.stride, VC_UNSIGNED, row0, step, present, vpre[c]);
+#pragma unroll
+      for (int k = 0; k < kScatterItems; ++k)
+        mword[c][k] = (v.mask && present[k]) ? to_global(v.mask)[(v.bit_off + row0 + k * step) >> 5] : ~0u;
+    }
+  };
+
+  if (begin < end) fetch(begin);
+  for (int64_t tile = begin; tile < end; tile += tile_rows_max) {
+    const int64_t row0 = tile + threadIdx.x;
     if (p.skip_null_keys) {
 #pragma unroll
       for (int k = 0; k < kScatterItems; ++k)
         if (present[k]) present[k] = key_row_valid(p, row0 + k * step);
     }
     uint32_t h[kScatterItems];
-    uint64_t key0[kScatterItems];
-    hash_rows(p, row0, step, present, h, key0);
-    uint64_t vpre[kScatterPreVals][kScatterItems];
+    if (pre_keys) {
 #pragma unroll
-    for (int c = 0; c < kScatterPreVals; ++c)
-      if (c < p.num_vals)
-        load_rows(p.vals[c].data, p.vals[c].width, p.vals[c].stride, VC_UNSIGNED, row0, step, present, vpre[c]);
+      for (int k = 0; k < kScatterItems; ++k) h[k] = 0u;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        if (c >= p.num_keys) continue;
+        const KeyCol& kc = p.keys[c];
+#pragma unroll
+        for (int k = 0; k < kScatterItems; ++k)
+          h[k] = hash_combine(h[k], murmur3_bits(normalize_float_bits(kpre[c][k], kc.vclass), kc.width, 0));
+      }
+    } else {
+      uint64_t ignored[kScatterItems];
+      hash_rows(p, row0, step, present, h, ignored);
+    }
     uint32_t flags[kScatterItems] = {};
     if (pp.flag_bytes) {
+#pragma unroll
+      for (int c = 0; c < kScatterPreVals; ++c) {
+        if (c >= p.num_vals || p.vals[c].flag_bit < 0) continue;
+        const ValCol& v = p.vals[c];
+#pragma unroll
+        for (int k = 0; k < kScatterItems; ++k)
+          if (present[k] && ((mword[c][k] >> ((v.bit_off + row0 + k * step) & 31)) & 1u))
+            flags[k] |= 1u << v.flag_bit;
+      }
 #pragma unroll 1
-      for (int c = 0; c < p.num_vals; ++c) {
+      for (int c = kScatterPreVals; c < p.num_vals; ++c) {
         const ValCol& v = p.vals[c];
         if (v.flag_bit < 0) continue;
 #pragma unroll
@@ -166,11 +210,10 @@ __device__ __forceinline__ void scatter_chunk(const PartParams& pp, int64_t chun
     for (int c = 0; c < p.num_keys; ++c) {
       const KeyCol& kc = p.keys[c];
       uint64_t x[kScatterItems];
-      if (c == 0) {
-#pragma unroll
-        for (int k = 0; k < kScatterItems; ++k) x[k] = key0[k];
-      } else {
-        load_rows(kc.data, kc.width, kc.stride, VC_UNSIGNED, row0, step, present, x);
+      switch (c) {
+        case 0: copy_rows(kpre[0], x); break;
+        case 1: copy_rows(kpre[1], x); break;
+        default: load_rows(kc.data, kc.width, kc.stride, VC_UNSIGNED, row0, step, present, x); break;
       }
       store_rows(stage_bytes, pos, pp.rec_bytes, pp.key_off[c], kc.width, x, present);
     }
@@ -193,6 +236,8 @@ __device__ __forceinline__ void scatter_chunk(const PartParams& pp, int64_t chun
       store_rows(stage_bytes, pos, pp.rec_bytes, pp.flags_off, pp.flag_bytes, f, present);
     }
     __syncthreads();
+    // the next tile's loads fly while this one is copied out
+    if (tile + tile_rows_max < end) fetch(tile + tile_rows_max);
     // copy out word by word: the words of one partition's run are contiguous
     const uint32_t words = tile_rows * (uint32_t)rec_words;
     for (uint32_t w = threadIdx.x; w < words; w += kLdsBlock) {
