@@ -25,7 +25,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -39,7 +41,10 @@ namespace mgb {
 enum BatchMode : int32_t { BM_DIRECT = 0, BM_GLOBAL = 1, BM_PARTITIONED = 2 };
 enum BatchItemKind : int32_t { BI_DIRECT = 0, BI_GLOBAL = 1, BI_PART = 2 };
 
-// Per-query device descriptor.
+// Per-query device descriptor. The kernels take the descriptor arrays as
+// const __restrict__ arguments: read-only for the whole launch, so that their
+// uniform fields are fetched with scalar loads (through the constant cache)
+// rather than with a vector load per access.
 struct BatchQuery {
   AggParams ap;        // the query's input columns and ops; table / capacity / counters: its global table
   AggParams rec_ap;    // partitioned: the same ops over the partitioned records (same global table)
@@ -152,11 +157,12 @@ __global__ __launch_bounds__(kBlock) void batch_sample_kernel(const BatchQuery* 
 
 // Identity records into every query's global table (one workgroup per
 // stretch of a table, one thread per record).
-__global__ __launch_bounds__(kBlock) void batch_fill_kernel(const BatchParams bp, uint64_t special_hdr)
+__global__ __launch_bounds__(kBlock) void batch_fill_kernel(const BatchParams bp, const BatchQuery* __restrict__ queries,
+                                                          const BatchSeg* __restrict__ segs, uint64_t special_hdr)
 {
   if (blockIdx.x == 0 && threadIdx.x == 0 && bp.hist) bp.hist[bp.hist_entries] = 0u;
-  const BatchSeg sg = bp.segs[blockIdx.x];
-  const AggParams& p = bp.queries[sg.query].ap;
+  const BatchSeg sg = segs[blockIdx.x];
+  const AggParams& p = queries[sg.query].ap;
   const int words = (int)(p.stride / 8);
   global_t<uint64_t>* t = to_global(reinterpret_cast<uint64_t*>(p.table));
   for (int64_t rec = sg.first + threadIdx.x; rec < sg.first + sg.count; rec += kBlock) {
@@ -166,18 +172,20 @@ __global__ __launch_bounds__(kBlock) void batch_fill_kernel(const BatchParams bp
   }
 }
 
-__global__ __launch_bounds__(kBlock) void batch_hist_kernel(const BatchParams bp)
+__global__ __launch_bounds__(kBlock) void batch_hist_kernel(const BatchQuery* __restrict__ queries,
+                                                          const BatchChunk* __restrict__ chunks)
 {
   extern __shared__ uint32_t counts[];
-  const BatchChunk c = bp.chunks[blockIdx.x];
-  hist_chunk(bp.queries[c.query].pp, c.chunk, counts);
+  const BatchChunk c = chunks[blockIdx.x];
+  hist_chunk(queries[c.query].pp, c.chunk, counts);
 }
 
-__global__ __launch_bounds__(kLdsBlock) void batch_scatter_kernel(const BatchParams bp)
+__global__ __launch_bounds__(kLdsBlock) void batch_scatter_kernel(const BatchQuery* __restrict__ queries,
+                                                                 const BatchChunk* __restrict__ chunks)
 {
   extern __shared__ __align__(16) uint8_t smem[];
-  const BatchChunk c = bp.chunks[blockIdx.x];
-  scatter_chunk(bp.queries[c.query].pp, c.chunk, smem);
+  const BatchChunk c = chunks[blockIdx.x];
+  scatter_chunk(queries[c.query].pp, c.chunk, smem);
 }
 
 // Rows [begin, end) into p's global table with device atomics, one row per
@@ -192,7 +200,8 @@ __device__ __forceinline__ void global_rows(const AggParams& p, int64_t begin, i
 }
 
 // Persistent workgroups take items from the queue until it is empty.
-__global__ __launch_bounds__(kLdsBlock) void batch_items_kernel(const BatchParams bp)
+__global__ __launch_bounds__(kLdsBlock) void batch_items_kernel(const BatchParams bp, const BatchQuery* __restrict__ queries,
+                                                               const BatchItem* __restrict__ items)
 {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ int64_t next;
@@ -203,8 +212,8 @@ __global__ __launch_bounds__(kLdsBlock) void batch_items_kernel(const BatchParam
     const int64_t it = (int64_t)__builtin_amdgcn_readfirstlane((int)next);
     __syncthreads();
     if (it >= bp.num_items) break;
-    const BatchItem item = bp.items[it];
-    const BatchQuery& Q = bp.queries[item.query];
+    const BatchItem item = items[it];
+    const BatchQuery& Q = queries[item.query];
     if (item.kind == BI_GLOBAL) {
       global_rows(Q.ap, item.begin, item.end);
     } else {
@@ -233,12 +242,13 @@ __global__ __launch_bounds__(kLdsBlock) void batch_items_kernel(const BatchParam
 
 // Groups of every query's global table -> output rows (one workgroup per
 // stretch of a table; one output-counter atomic per workgroup pass).
-__global__ __launch_bounds__(kBlock) void batch_finalize_kernel(const BatchParams bp)
+__global__ __launch_bounds__(kBlock) void batch_finalize_kernel(const BatchParams bp, const BatchQuery* __restrict__ queries,
+                                                              const BatchSeg* __restrict__ segs)
 {
   __shared__ uint32_t scan[kBlock];
   __shared__ unsigned long long base;
-  const BatchSeg sg = bp.segs[blockIdx.x];
-  const BatchQuery& Q = bp.queries[sg.query];
+  const BatchSeg sg = segs[blockIdx.x];
+  const BatchQuery& Q = queries[sg.query];
   // a partitioned query's table holds spilled groups whose representative
   // rows are partitioned records
   const AggParams& p = Q.mode == BM_PARTITIONED ? Q.rec_ap : Q.ap;
@@ -347,6 +357,47 @@ int64_t next_pow2(int64_t x)
 
 }  // namespace
 
+namespace {
+
+// Reusable pinned host blocks (never returned to the driver: hipHostFree
+// waits for the whole device). A batch stages its descriptors in one block
+// and uploads them with one asynchronous copy, and reads its counters back
+// through it, so planning and finishing a batch never wait for other work on
+// the device.
+struct PinnedPool {
+  std::mutex mu;
+  std::multimap<size_t, void*> free;
+} g_pinned;
+
+void* pinned_get(size_t bytes, size_t* got)
+{
+  size_t sz = 4096;
+  while (sz < bytes) sz <<= 1;
+  {
+    std::lock_guard<std::mutex> lk(g_pinned.mu);
+    auto it = g_pinned.free.lower_bound(sz);
+    if (it != g_pinned.free.end() && it->first <= 4 * sz) {
+      void* p = it->second;
+      *got = it->first;
+      g_pinned.free.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess) return nullptr;
+  *got = sz;
+  return p;
+}
+
+void pinned_put(void* p, size_t bytes)
+{
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(g_pinned.mu);
+  g_pinned.free.emplace(bytes, p);
+}
+
+}  // namespace
+
 struct mgb_batch {
   int Q = 0;
   int num_keys = 0, num_requests = 0, num_outs = 0;
@@ -364,14 +415,9 @@ struct mgb_batch {
   unsigned long long* null_counts = nullptr;
   std::vector<uint8_t*> key_out, out_data, out_valid;
   std::vector<uint32_t*> out_mask;
-  std::vector<unsigned long long> host_counters;
-  // host copies of the uploaded descriptors (kept for the uploads' lifetime)
-  std::vector<BatchQuery> h_queries;
-  std::vector<BatchItem> h_items;
-  std::vector<BatchChunk> h_chunks;
-  std::vector<BatchSeg> h_segs;
-  uint64_t h_tmpl[kMaxRecordWords] = {};
-  OutTable h_ot{};
+  void* pinned = nullptr;                    // staged descriptors, then the counters read back
+  size_t pinned_bytes = 0;
+  unsigned long long* host_counters = nullptr;  // in `pinned`
   hipStream_t alloc_stream = nullptr;        // stream-ordered allocation and release of `dev`
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -398,6 +444,7 @@ struct mgb_batch {
     // stream-ordered release: no device-wide synchronisation (a batch may be
     // destroyed while other batches run)
     if (dev) (void)hipFreeAsync(dev, alloc_stream);
+    pinned_put(pinned, pinned_bytes);
   }
   void release_fallbacks()
   {
@@ -481,12 +528,22 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
   // batches allocate stream-ordered from the device's default pool: keep
   // released memory in the pool instead of returning it at every
   // synchronisation, so that planning a batch does not map fresh memory
+  // (once per device: changing the attribute trims the pool, which can take
+  // tens of milliseconds)
   {
+    static std::mutex pool_mu;
+    static std::vector<int> pool_done;
     int dev = 0;
-    hipMemPool_t pool = nullptr;
-    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
-      uint64_t keep = UINT64_MAX;
-      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    if (hipGetDevice(&dev) == hipSuccess) {
+      std::lock_guard<std::mutex> lk(pool_mu);
+      if (std::find(pool_done.begin(), pool_done.end(), dev) == pool_done.end()) {
+        pool_done.push_back(dev);
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+          uint64_t keep = UINT64_MAX;
+          (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+      }
     }
   }
   std::unique_ptr<mgb_batch> b(new mgb_batch());
@@ -743,10 +800,14 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
     off += align_up(std::max<size_t>(bytes, 1), 256);
     return o;
   };
+  // the uploaded descriptors first: one copy of [0, upload_bytes)
   const size_t o_queries = take(sizeof(BatchQuery) * Q);
   const size_t o_items = take(sizeof(BatchItem) * items.size());
   const size_t o_chunks = take(sizeof(BatchChunk) * chunks.size());
   const size_t o_segs = take(sizeof(BatchSeg) * segs.size());
+  const size_t o_tmpl = take(sizeof(uint64_t) * kMaxRecordWords);
+  const size_t o_ot = take(sizeof(OutTable));
+  const size_t upload_bytes = off;
   const size_t o_counters = take(sizeof(unsigned long long) * (4 * (size_t)Q + 8));
   const size_t o_tables = take((size_t)table_records * stride);
   const size_t o_hist = take(sizeof(uint32_t) * (size_t)(hist_entries + 1));
@@ -754,8 +815,6 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
   const int64_t nbh = std::max<int64_t>(1, (hist_entries + 1 + kScanTile - 1) / kScanTile);
   const size_t o_bsum = take(sizeof(uint32_t) * (size_t)nbh);
   const size_t o_recs = take((size_t)recs_total * rec_bytes);
-  const size_t o_tmpl = take(sizeof(uint64_t) * kMaxRecordWords);
-  const size_t o_ot = take(sizeof(OutTable));
   std::vector<size_t> o_key(nk), o_out(no), o_valid(no, SIZE_MAX), o_mask(no, SIZE_MAX);
   const size_t nout = (size_t)std::max<int64_t>(out_total, 1);
   for (int c = 0; c < nk; ++c) o_key[c] = take(nout * plan.keys[c].width);
@@ -888,36 +947,39 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
   b->bsum = reinterpret_cast<uint32_t*>(base + o_bsum);
   b->nbh = nbh;
 
-  // the host copies stay with the batch, so that these asynchronous uploads
-  // need no synchronisation here
-  b->h_queries = std::move(hq);
-  b->h_items = std::move(items);
-  b->h_chunks = std::move(chunks);
-  b->h_segs = std::move(segs);
-  std::memcpy(b->h_tmpl, tmpl, sizeof(tmpl));
-  b->h_ot = ot;
-  MGB_HIP(hipMemcpyAsync(base + o_queries, b->h_queries.data(), sizeof(BatchQuery) * Q, hipMemcpyHostToDevice,
-                         stream));
-  if (!b->h_items.empty())
-    MGB_HIP(hipMemcpyAsync(base + o_items, b->h_items.data(), sizeof(BatchItem) * b->h_items.size(),
-                           hipMemcpyHostToDevice, stream));
-  if (!b->h_chunks.empty())
-    MGB_HIP(hipMemcpyAsync(base + o_chunks, b->h_chunks.data(), sizeof(BatchChunk) * b->h_chunks.size(),
-                           hipMemcpyHostToDevice, stream));
-  MGB_HIP(hipMemcpyAsync(base + o_segs, b->h_segs.data(), sizeof(BatchSeg) * b->h_segs.size(), hipMemcpyHostToDevice,
-                         stream));
-  MGB_HIP(hipMemcpyAsync(base + o_tmpl, b->h_tmpl, sizeof(b->h_tmpl), hipMemcpyHostToDevice, stream));
-  MGB_HIP(hipMemcpyAsync(base + o_ot, &b->h_ot, sizeof(OutTable), hipMemcpyHostToDevice, stream));
-  b->host_counters.assign(4 * (size_t)Q + 8, 0ull);
+  // stage every descriptor in one pinned block and upload it with one
+  // asynchronous copy; the block stays with the batch (the copy may still be
+  // in flight when this returns); its tail receives the counters
+  {
+    const size_t counters_bytes = sizeof(unsigned long long) * (4 * (size_t)Q + 8);
+    const size_t need = align_up(upload_bytes, 256) + counters_bytes;
+    b->pinned = pinned_get(need, &b->pinned_bytes);
+    if (!b->pinned) return fail(MGB_ERROR_ALLOC, "pinned staging allocation failed");
+    uint8_t* h = static_cast<uint8_t*>(b->pinned);
+    std::memcpy(h + o_queries, hq.data(), sizeof(BatchQuery) * Q);
+    if (!items.empty()) std::memcpy(h + o_items, items.data(), sizeof(BatchItem) * items.size());
+    if (!chunks.empty()) std::memcpy(h + o_chunks, chunks.data(), sizeof(BatchChunk) * chunks.size());
+    if (!segs.empty()) std::memcpy(h + o_segs, segs.data(), sizeof(BatchSeg) * segs.size());
+    std::memcpy(h + o_tmpl, tmpl, sizeof(tmpl));
+    std::memcpy(h + o_ot, &ot, sizeof(OutTable));
+    MGB_HIP(hipMemcpyAsync(base, h, upload_bytes, hipMemcpyHostToDevice, stream));
+    b->host_counters = reinterpret_cast<unsigned long long*>(h + align_up(upload_bytes, 256));
+  }
 
   // launch geometry
   b->items_lds = std::max(lds_table_bytes(part_slots, lds_words), (size_t)16);
-  MGB_HIP(hipFuncSetAttribute((const void*)batch_items_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)b->items_lds));
-  if (b->num_chunks) {
-    MGB_HIP(hipFuncSetAttribute((const void*)batch_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)b->scatter_lds));
-  }
+  // the largest dynamic LDS either kernel may use, set once per process
+  static std::once_flag lds_once;
+  static hipError_t lds_err = hipSuccess;
+  std::call_once(lds_once, [] {
+    const int lim = (int)lds_limit();
+    // (the items kernel also holds 8 bytes of static LDS)
+    lds_err = hipFuncSetAttribute((const void*)batch_items_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  lim - 64);
+    if (lds_err == hipSuccess)
+      lds_err = hipFuncSetAttribute((const void*)batch_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  });
+  MGB_HIP(lds_err);
   b->items_grid = (int)std::max<int64_t>(1, std::min<int64_t>(b->num_items, (int64_t)device_cu_count()));
 
   // statistics
@@ -952,18 +1014,20 @@ mgb_status enqueue_batch(mgb_batch* b, hipStream_t stream)
     fill_u64_kernel<<<(unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 1024)), kBlock, 0,
                       stream>>>(b->counters, n, 0ull);
   }
-  batch_fill_kernel<<<(unsigned)b->num_segs, kBlock, 0, stream>>>(bp, b->plan.key_mode == KEY_INT ? 0ull : kEmptyRowRef);
+  batch_fill_kernel<<<(unsigned)b->num_segs, kBlock, 0, stream>>>(bp, bp.queries, bp.segs,
+                                                                   b->plan.key_mode == KEY_INT ? 0ull : kEmptyRowRef);
   if (b->num_chunks > 0) {
-    batch_hist_kernel<<<(unsigned)b->num_chunks, kBlock, b->hist_lds, stream>>>(bp);
+    batch_hist_kernel<<<(unsigned)b->num_chunks, kBlock, b->hist_lds, stream>>>(bp.queries, bp.chunks);
     const int64_t n = b->hist_entries + 1;
     uint32_t* offs = const_cast<uint32_t*>(bp.offsets);
     scan_reduce_u32_kernel<<<(unsigned)b->nbh, 256, 0, stream>>>(bp.hist, n, b->bsum);
     scan_bsums_u32_kernel<<<1, 1024, 0, stream>>>(b->bsum, b->nbh, b->counters + 4 * (size_t)Q + 1);
     scan_apply_u32_kernel<<<(unsigned)b->nbh, 256, 0, stream>>>(bp.hist, n, b->bsum, offs);
-    batch_scatter_kernel<<<(unsigned)b->num_chunks, kLdsBlock, b->scatter_lds, stream>>>(bp);
+    batch_scatter_kernel<<<(unsigned)b->num_chunks, kLdsBlock, b->scatter_lds, stream>>>(bp.queries, bp.chunks);
   }
-  if (b->num_items > 0) batch_items_kernel<<<(unsigned)b->items_grid, kLdsBlock, b->items_lds, stream>>>(bp);
-  batch_finalize_kernel<<<(unsigned)b->num_segs, kBlock, 0, stream>>>(bp);
+  if (b->num_items > 0)
+    batch_items_kernel<<<(unsigned)b->items_grid, kLdsBlock, b->items_lds, stream>>>(bp, bp.queries, bp.items);
+  batch_finalize_kernel<<<(unsigned)b->num_segs, kBlock, 0, stream>>>(bp, bp.queries, bp.segs);
   const int no = b->num_outs;
   const int64_t words = bitmask_bytes(std::max<int64_t>(b->total_out, 1)) / 4;
   for (int o = 0; o < no; ++o) {
@@ -1010,12 +1074,12 @@ extern "C" mgb_status mgb_batch_finish(mgb_batch* b, void* stream_v)
   if (!b) return fail(MGB_ERROR_LOGIC, "batch is NULL");
   hipStream_t stream = static_cast<hipStream_t>(stream_v);
   const int Q = b->Q;
-  MGB_HIP(hipMemcpyAsync(b->host_counters.data(), b->counters, sizeof(unsigned long long) * 4 * Q,
-                         hipMemcpyDeviceToHost, stream));
+  MGB_HIP(hipMemcpyAsync(b->host_counters, b->counters, sizeof(unsigned long long) * 4 * Q, hipMemcpyDeviceToHost,
+                         stream));
   MGB_HIP(hipStreamSynchronize(stream));
   int64_t reruns = 0;
   for (int q = 0; q < Q; ++q) {
-    const unsigned long long* c = b->host_counters.data() + 4 * (size_t)q;
+    const unsigned long long* c = b->host_counters + 4 * (size_t)q;
     b->groups[q] = (int64_t)c[2];
     b->fallback[q] = c[1] != 0 || (int64_t)c[2] > b->qp[q].out_cap;
     if (!b->fallback[q]) continue;

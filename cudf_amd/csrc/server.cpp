@@ -314,18 +314,17 @@ bool mgb_server::launch(Slot& s)
       std::lock_guard<std::mutex> lk(mu);
       ++stats.deferred;
       if (Q == 1) {
-        bool idle = true;
-        for (auto& o : slots) idle = idle && !o->busy;
-        if (idle && held_bytes == 0) {
-          fail_query(take[0], MGB_ERROR_ALLOC,
-                     "query does not fit the HBM budget (" + std::to_string(bs.device_bytes) + " bytes next to " +
-                         std::to_string(resident_bytes) + " resident, budget " +
-                         std::to_string(opt.hbm_budget_bytes) + ")");
-          done_cv.notify_all();
-          return true;
+        // fails only when it could not fit even with nothing else held
+        if (resident_bytes + bs.device_bytes <= opt.hbm_budget_bytes) {
+          pending.push_front(std::move(take[0]));  // wait for memory to be released
+          return false;
         }
-        pending.push_front(std::move(take[0]));  // wait for memory to be released
-        return false;
+        fail_query(take[0], MGB_ERROR_ALLOC,
+                   "query does not fit the HBM budget (" + std::to_string(bs.device_bytes) + " bytes next to " +
+                       std::to_string(resident_bytes) + " resident, budget " + std::to_string(opt.hbm_budget_bytes) +
+                       ")");
+        done_cv.notify_all();
+        return true;
       }
       // put the second half back at the front, in order
       for (int q = Q - 1; q >= Q / 2; --q) pending.push_front(std::move(take[q]));
