@@ -80,7 +80,10 @@ struct BatchSeg {
   int64_t count;
 };
 
-constexpr int64_t kSegRecords = 8192;
+// records per stretch of the fill / finalize kernels: between one and eight
+// workgroup passes, so that a batch of one large table still spreads over
+// every CU
+constexpr int64_t kSegMin = 1024, kSegMax = 8192;
 
 struct BatchParams {
   const BatchQuery* queries;
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(kLdsBlock) void batch_items_kernel(const BatchParam
       // a partition aggregated whole writes its groups; parts of one fold
       // them into the query's table, as does an item whose groups overflowed
       if (part && item.nsub == 1 && !*t.overflow) lds_write_groups(p, t, Q.out_count, Q.out_base, *bp.out);
-      else lds_fold_to_global(p, t);
+      else lds_fold_to_global(p, t, part ? (int)item.begin : 0, part ? item.nsub : 1);
     }
     __syncthreads();  // the next item reuses the LDS
   }
@@ -648,6 +651,10 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
   const int64_t fill_rows = std::max<int64_t>(1, total_rows / (int64_t)(4 * cus));
   const int64_t item_rows = opt.direct_item_rows > 0 ? opt.direct_item_rows
                                                      : std::max<int64_t>(16384, std::min<int64_t>(262144, fill_rows));
+  // partitions of a partitioned query: enough for its groups, and more when
+  // its rows are many, up to one work item of part_item_rows per partition
+  // while each keeps 64 groups or more
+  const int64_t part_item_rows = std::max<int64_t>(65536, fill_rows);
   for (int q = 0; q < Q; ++q) {
     QueryPlan& P = b->qp[q];
     const int64_t R = P.rows, G = P.est;
@@ -662,6 +669,7 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
     const double c_global = R * nops * cm.atomic + G * cm.atomic;
     int log2p = 1;
     while ((int64_t)(part_cap * 0.75) * (1ll << log2p) < G && log2p < 12) ++log2p;
+    while (log2p < 12 && (R >> log2p) > part_item_rows && (G >> (log2p + 1)) >= 64) ++log2p;
     const double c_part = R * cm.part_row + (1ll << log2p) * (cm.item + part_slots * cm.slot_init) / cus;
     int mode = c_direct <= c_global && c_direct <= c_part ? BM_DIRECT : c_global <= c_part ? BM_GLOBAL : BM_PARTITIONED;
     if (opt.force_mode == BM_DIRECT && direct_ok) mode = BM_DIRECT;
@@ -676,9 +684,8 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
     if (mode == BM_PARTITIONED) {
       P.log2p = log2p;
       max_log2p = std::max(max_log2p, log2p);
-      // partitions much longer than a work item are cut into several items
-      // (all of whose groups then go to the query's table)
-      const int64_t part_item_rows = std::max<int64_t>(65536, fill_rows);
+      // partitions still much longer than a work item are cut into several
+      // items (all of whose groups then go to the query's table)
       P.nsub = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (R >> log2p) / part_item_rows));
       // its table: every group when partitions are cut, else room for two
       // overflowing partitions
@@ -785,10 +792,15 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
   b->num_items = (int64_t)items.size();
   b->num_chunks = (int64_t)chunks.size();
   std::vector<BatchSeg> segs;
-  for (int q = 0; q < Q; ++q) {
-    const int64_t n = b->qp[q].capacity + 1;
-    for (int64_t f = 0; f < n; f += kSegRecords)
-      segs.push_back(BatchSeg{q, 0, f, std::min<int64_t>(kSegRecords, n - f)});
+  {
+    int64_t records = 0;
+    for (int q = 0; q < Q; ++q) records += b->qp[q].capacity + 1;
+    const int64_t seg = std::max<int64_t>(
+        kSegMin, std::min<int64_t>(kSegMax, (int64_t)align_up((size_t)(records / (int64_t)(4 * cus)), kBlock)));
+    for (int q = 0; q < Q; ++q) {
+      const int64_t n = b->qp[q].capacity + 1;
+      for (int64_t f = 0; f < n; f += seg) segs.push_back(BatchSeg{q, 0, f, std::min<int64_t>(seg, n - f)});
+    }
   }
   b->num_segs = (int64_t)segs.size();
 

@@ -438,6 +438,7 @@ __global__ __launch_bounds__(kLdsBlock) void lds_agg_kernel(const LdsParams lp)
 
   int64_t begin, end;
   bool merge = true;
+  int fold_part = 0, fold_parts = 1;
   if (lp.direct) {
     begin = (int64_t)blockIdx.x * lp.rows_per_item;
     end = begin + lp.rows_per_item < p.num_rows ? begin + lp.rows_per_item : p.num_rows;
@@ -458,12 +459,14 @@ __global__ __launch_bounds__(kLdsBlock) void lds_agg_kernel(const LdsParams lp)
     begin = ps + (int64_t)(b - lp.item_offsets[part]) * lp.max_item_rows;
     end = begin + lp.max_item_rows < pe ? begin + lp.max_item_rows : pe;
     merge = n_items > 1;
+    fold_part = (int)(b - lp.item_offsets[part]);
+    fold_parts = (int)n_items;
   }
 
   const LdsTable t = lds_table_at(lds, lp.slots, lp.words, lp.cap);
   lds_table_init(t, lp.tmpl, lp.words);
   lds_aggregate_rows(p, t, lds, begin, end, lp.log2p, lp.direct != 0);
-  if (merge || *t.overflow) lds_fold_to_global(p, t);
+  if (merge || *t.overflow) lds_fold_to_global(p, t, fold_part, fold_parts);
   else lds_write_groups(p, t, &lp.out_counters[0], 0, *lp.out);
 }
 
@@ -1311,6 +1314,13 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     int log2p = 1;
     while ((double)(1ll << log2p) * m_target < (double)est) ++log2p;
     if (log2p > 15) return MGB_SUCCESS;
+    // more partitions than the groups need when the rows are many, so that a
+    // partition of mean size is one work item (the items of a partition cut
+    // in several fold all of its groups into the global table), while every
+    // partition keeps 64 groups or more (fewer put many lanes of a wave on
+    // one LDS record; such partitions are cut instead, and fold few groups)
+    const int64_t item_target = std::max<int64_t>(65536, (R + 2 * (int64_t)cus - 1) / (2 * (int64_t)cus));
+    while (log2p < 12 && (R >> log2p) > item_target && (est >> (log2p + 1)) >= 64) ++log2p;
     const int64_t P = 1ll << log2p;
     int64_t nw = std::min<int64_t>((int64_t)cus * 4, std::max<int64_t>(64, (1ll << 22) / P));
     nw = std::max<int64_t>(1, std::min<int64_t>(nw, (R + 4095) / 4096));

@@ -187,9 +187,15 @@ __device__ __forceinline__ void lds_aggregate_rows(const AggParams& p, const Lds
 }
 
 // Folds every group of the LDS table into p's global table (device atomics).
-__device__ __forceinline__ void lds_fold_to_global(const AggParams& p, const LdsTable& t)
+// `part` of `nparts`: items that cover parts of one partition hold mostly the
+// same groups in the same slots and tend to finish together; each starts its
+// pass at a different slot, so that their atomics on one record do not meet.
+__device__ __forceinline__ void lds_fold_to_global(const AggParams& p, const LdsTable& t, int part = 0, int nparts = 1)
 {
-  for (int i = threadIdx.x; i <= t.slots; i += kLdsBlock) {
+  const int n = t.slots + 1;
+  const int first = nparts > 1 ? (int)(((int64_t)n * part) / nparts) : 0;
+  for (int j = threadIdx.x; j < n; j += kLdsBlock) {
+    const int i = j + first < n ? j + first : j + first - n;
     if (!lds_occupied(t, i, p.key_mode)) continue;
     const uint8_t* rec = lds_record(t, i);
     const uint64_t hdr = *reinterpret_cast<const uint64_t*>(rec);

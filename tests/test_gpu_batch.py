@@ -83,9 +83,10 @@ def test_batch_modes(gpu, mode):
 
 @pytest.mark.parametrize("mode", [None, 2], ids=["auto", "partitioned"])
 def test_batch_long_partitions_are_split(gpu, mode):
-    # few groups over many rows: a handful of partitions, each cut into
-    # several work items whose groups meet in the query's table
-    host, dev = _batch_inputs(18, shapes=[(2_000_000, 3000), (1_500_000, 2200), (40000, 40)],
+    # few groups over many rows: partitions keep 64 groups or more, so there
+    # are few of them, each cut into several work items whose groups meet in
+    # the query's table
+    host, dev = _batch_inputs(18, shapes=[(4_000_000, 1000), (3_000_000, 800), (40000, 40)],
                               val_types=(O.FLOAT64, O.INT32), null_frac=0.02)
     batch = ca.GroupByBatch(dev, force_mode=mode)
     print("split modes", [batch.mode(q) for q in range(len(dev))], batch.stats(), flush=True)

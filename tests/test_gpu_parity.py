@@ -233,9 +233,9 @@ def test_hot_key_partition_split(gpu, path):
 
 
 def test_few_partitions_many_rows_split(gpu, path):
-    """3M rows over 2,500 groups: a few partitions of about a million rows
-    each, cut into work items for parallelism; their partial groups meet in
-    the global table."""
+    """3M rows over 2,500 groups: partitions are kept at 64 groups or more,
+    so each of about 94K rows is cut into two work items, whose partial
+    groups meet in the global table."""
     rng = np.random.default_rng(34)
     n = 3_000_000
     pool = rng.integers(-2 ** 62, 2 ** 62, 2500, dtype=np.int64)
