@@ -208,7 +208,7 @@ def make_c5_tables(num_tables: int, seed: int, device: str = "cuda", max_rows: f
     return tables, rows, card
 
 
-def serve_run(tables, rate: float, n_queries: int, batching: bool, seed: int, budget: int,
+def serve_run(tables, rate: float, n_queries: int, batching: bool, seed: int, budget: int, reserve: int = 0,
               max_inflight: int = 2, max_batch: int = 256):
     """Poisson arrivals at `rate` queries/s on random tables; returns latencies and throughput."""
     import threading
@@ -220,7 +220,7 @@ def serve_run(tables, rate: float, n_queries: int, batching: bool, seed: int, bu
 
     aggs = [agg.sum(), agg.count(), agg.min(), agg.max(), agg.mean()]
     srv = ca.Server(hbm_budget_bytes=budget, max_inflight=max_inflight, max_batch_queries=max_batch,
-                    batching=batching)
+                    batching=batching, reserve_bytes=reserve)
     tids = [srv.add_table(t) for t in tables]
     # one query per table first, not timed: the steady state of a server
     # whose tables are resident (workspaces sized, code loaded)
@@ -278,6 +278,7 @@ def serve_run(tables, rate: float, n_queries: int, batching: bool, seed: int, bu
         "host_plan_ms_per_batch": st["host_plan_ms"] / max(st["batches"], 1),
         "host_finish_ms_per_batch": st["host_finish_ms"] / max(st["batches"], 1),
         "resident_gb": st["resident_bytes"] / 1e9,
+        "server_device_gb": st["device_bytes"] / 1e9,
     }
 
 
@@ -296,7 +297,8 @@ def bench_serving(args):
         for rate in rates:
             n = int(min(args.serve_max_queries, max(args.serve_min_queries, rate * args.serve_seconds)))
             log(f"serving {mode}: {rate} q/s, {n} queries")
-            r = serve_run(tables, rate, n, mode == "batched", args.seed + int(rate), budget)
+            r = serve_run(tables, rate, n, mode == "batched", args.seed + int(rate), budget,
+                          reserve=int(args.server_reserve_gb * (1 << 30)))
             log(f"  p50 {r['p50_ms']:.2f} ms  p99 {r['p99_ms']:.2f} ms  achieved {r['achieved_qps']:.0f} q/s  "
                 f"batches {r['batches']} (mean {r['mean_batch_queries']:.1f}); host plan "
                 f"{r['host_plan_ms_per_batch']:.2f} ms, finish {r['host_finish_ms_per_batch']:.2f} ms per batch")
@@ -525,8 +527,10 @@ def main():
     ap.add_argument("--max-table-rows", type=float, default=5e7, help="c5: largest table (rows)")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0, help="c5: HBM budget of the server (GiB)")
     ap.add_argument("--p99-target-ms", type=float, default=100.0, help="c5: p99 latency target")
-    ap.add_argument("--base-rates", type=float, nargs="*", default=[50, 100, 200, 400, 800, 1600])
-    ap.add_argument("--batch-rates", type=float, nargs="*", default=[200, 400, 800, 1600, 3200, 6400, 12800])
+    ap.add_argument("--server-reserve-gb", type=float, default=64.0,
+                    help="c5: device memory the server takes at creation for batches and results (GiB)")
+    ap.add_argument("--base-rates", type=float, nargs="*", default=[400, 800, 1200, 1600, 2000, 2400])
+    ap.add_argument("--batch-rates", type=float, nargs="*", default=[800, 1600, 2400, 3200, 4000, 4800, 5600, 6400])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-min-queries", type=int, default=200)
     ap.add_argument("--serve-max-queries", type=int, default=4000)

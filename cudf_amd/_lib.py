@@ -72,6 +72,7 @@ class mgb_batch_options(ctypes.Structure):
         ("direct_item_rows", ctypes.c_int64),
         ("estimate_scale", ctypes.c_double),
         ("estimates", ctypes.POINTER(ctypes.c_int64)),
+        ("alloc", ctypes.POINTER(mgb_allocator)),
     ]
 
 
@@ -88,6 +89,7 @@ class mgb_server_options(ctypes.Structure):
         ("max_batch_queries", ctypes.c_int32),
         ("batching", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("reserve_bytes", ctypes.c_int64),
     ]
 
 
@@ -103,7 +105,7 @@ class mgb_server_stats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int64) for name in (
         "submitted", "completed", "failed", "batches", "deferred", "reruns", "peak_held_bytes", "resident_bytes",
         "held_bytes", "pending")] + [(name, ctypes.c_double) for name in (
-            "mean_batch_queries", "host_plan_ms", "host_finish_ms")]
+            "mean_batch_queries", "host_plan_ms", "host_finish_ms")] + [("device_bytes", ctypes.c_int64)]
 
 
 # Every symbol include/mgb_groupby.h declares, with its ctypes signature.

@@ -422,6 +422,9 @@ struct mgb_batch {
   size_t pinned_bytes = 0;
   unsigned long long* host_counters = nullptr;  // in `pinned`
   hipStream_t alloc_stream = nullptr;        // stream-ordered allocation and release of `dev`
+  size_t dev_bytes = 0;
+  bool own_alloc = false;                    // `dev` and re-run results from the caller's allocator
+  mgb_allocator alloc{};
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -446,7 +449,10 @@ struct mgb_batch {
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
     // stream-ordered release: no device-wide synchronisation (a batch may be
     // destroyed while other batches run)
-    if (dev) (void)hipFreeAsync(dev, alloc_stream);
+    if (dev) {
+      if (own_alloc) alloc.deallocate(alloc.ctx, dev, dev_bytes, alloc_stream);
+      else (void)hipFreeAsync(dev, alloc_stream);
+    }
     pinned_put(pinned, pinned_bytes);
   }
   void release_fallbacks()
@@ -454,8 +460,13 @@ struct mgb_batch {
     for (auto* v : {&fb_keys, &fb_results})
       for (auto& cols : *v)
         for (auto& c : cols) {
-          if (c.data) (void)hipFree(c.data);
-          if (c.null_mask) (void)hipFree(c.null_mask);
+          if (own_alloc) {
+            if (c.data) alloc.deallocate(alloc.ctx, c.data, c.data_bytes, alloc_stream);
+            if (c.null_mask) alloc.deallocate(alloc.ctx, c.null_mask, c.mask_bytes, alloc_stream);
+          } else {
+            if (c.data) (void)hipFree(c.data);
+            if (c.null_mask) (void)hipFree(c.null_mask);
+          }
           c.data = nullptr;
           c.null_mask = nullptr;
         }
@@ -839,7 +850,16 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
   }
   const size_t o_nulls = take(sizeof(unsigned long long) * (no + 1));
   b->alloc_stream = stream;
-  MGB_HIP(hipMallocAsync(&b->dev, off, stream));
+  b->dev_bytes = off;
+  if (opt.alloc) {
+    if (!opt.alloc->allocate || !opt.alloc->deallocate) return fail(MGB_ERROR_LOGIC, "allocator without callbacks");
+    b->alloc = *opt.alloc;
+    b->own_alloc = true;
+    b->dev = b->alloc.allocate(b->alloc.ctx, off, stream);
+    if (!b->dev) return fail(MGB_ERROR_ALLOC, "allocation of " + std::to_string(off) + " bytes for the batch failed");
+  } else {
+    MGB_HIP(hipMallocAsync(&b->dev, off, stream));
+  }
   uint8_t* base = static_cast<uint8_t*>(b->dev);
   b->counters = reinterpret_cast<unsigned long long*>(base + o_counters);
   b->null_counts = reinterpret_cast<unsigned long long*>(base + o_nulls);
@@ -1105,7 +1125,7 @@ extern "C" mgb_status mgb_batch_finish(mgb_batch* b, void* stream_v)
     }
     b->fb_keys[q].assign(b->num_keys, mgb_out_column{});
     b->fb_results[q].assign(b->num_outs, mgb_out_column{});
-    mgb_allocator al{fb_alloc, fb_free, nullptr};
+    const mgb_allocator al = b->own_alloc ? b->alloc : mgb_allocator{fb_alloc, fb_free, nullptr};
     int64_t g = 0;
     mgb_status st = mgb_groupby_aggregate(&b->keys[(size_t)q * b->num_keys], b->num_keys, b->null_policy,
                                           reqs.data(), b->num_requests, b->fb_keys[q].data(), b->fb_results[q].data(),

@@ -68,6 +68,139 @@ struct Done {
   int64_t bytes = 0;
 };
 
+// Device memory of batches and results: chunks taken from the device once
+// and kept until the server is destroyed, carved first-fit in 256-byte
+// units. A block comes back only when nothing on the device uses it any
+// more (a batch is released after its work and the copies of its results
+// have completed), so it can be handed out again at once.
+class Arena {
+ public:
+  explicit Arena(int64_t limit) : limit_(limit) {}
+  ~Arena()
+  {
+    for (auto& c : chunks_) (void)hipFree(c.base);
+  }
+  // One chunk of `bytes`, taken now.
+  bool reserve(size_t bytes)
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    return add_chunk(align_up(bytes, kUnit));
+  }
+  void* allocate(size_t bytes)
+  {
+    const size_t n = align_up(std::max<size_t>(bytes, 1), kUnit);
+    std::lock_guard<std::mutex> lk(mu_);
+    if (void* p = take(n)) return p;
+    // a new chunk: the request, and at least the memory held so far (up to
+    // 8 GiB), so that chunks are few
+    size_t sz = std::max(align_up(n, kGrow), std::min<size_t>(total_, 8ull << 30));
+    if ((int64_t)(total_ + sz) > limit_) sz = n;
+    // near the limit: chunks with nothing in use go back first (hipFree
+    // waits for the device; only under memory pressure)
+    if ((int64_t)(total_ + sz) > limit_) drop_idle_chunks();
+    if ((int64_t)(total_ + sz) > limit_ || !add_chunk(sz)) return nullptr;
+    return take(n);
+  }
+  void deallocate(void* p)
+  {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    const size_t n = it->second;
+    live_.erase(it);
+    for (auto& c : chunks_) {
+      uint8_t* u = static_cast<uint8_t*>(p);
+      if (u < c.base || u >= c.base + c.size) continue;
+      size_t off = (size_t)(u - c.base), len = n;
+      auto next = c.free.lower_bound(off);
+      if (next != c.free.end() && off + len == next->first) {
+        len += next->second;
+        next = c.free.erase(next);
+      }
+      if (next != c.free.begin()) {
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == off) {
+          prev->second += len;
+          return;
+        }
+      }
+      c.free.emplace(off, len);
+      return;
+    }
+  }
+  int64_t total()
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    return (int64_t)total_;
+  }
+  bool in_use()
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    return !live_.empty();
+  }
+  void set_limit(int64_t limit)
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    limit_ = limit;
+  }
+
+ private:
+  static constexpr size_t kUnit = 256, kGrow = 256ull << 20;
+  struct Chunk {
+    uint8_t* base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free;  // offset -> length
+  };
+  bool add_chunk(size_t sz)
+  {
+    void* p = nullptr;
+    if (sz == 0 || hipMalloc(&p, sz) != hipSuccess) return false;
+    Chunk c;
+    c.base = static_cast<uint8_t*>(p);
+    c.size = sz;
+    c.free.emplace(0, sz);
+    chunks_.push_back(std::move(c));
+    total_ += sz;
+    return true;
+  }
+  void drop_idle_chunks()
+  {
+    for (size_t i = 0; i < chunks_.size();) {
+      Chunk& c = chunks_[i];
+      if (c.free.size() == 1 && c.free.begin()->second == c.size) {
+        (void)hipFree(c.base);
+        total_ -= c.size;
+        chunks_.erase(chunks_.begin() + (std::ptrdiff_t)i);
+      } else {
+        ++i;
+      }
+    }
+  }
+  void* take(size_t n)
+  {
+    for (auto& c : chunks_)
+      for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+        if (it->second < n) continue;
+        const size_t off = it->first, len = it->second;
+        c.free.erase(it);
+        if (len > n) c.free.emplace(off + n, len - n);
+        void* p = c.base + off;
+        live_.emplace(p, n);
+        return p;
+      }
+    return nullptr;
+  }
+  std::mutex mu_;
+  std::vector<Chunk> chunks_;
+  std::map<void*, size_t> live_;
+  size_t total_ = 0;
+  int64_t limit_;
+};
+
+void* arena_alloc(void* ctx, size_t bytes, void*) { return static_cast<Arena*>(ctx)->allocate(bytes); }
+void arena_free(void* ctx, void* p, size_t, void*) { static_cast<Arena*>(ctx)->deallocate(p); }
+
 struct Slot {
   hipStream_t stream = nullptr;
   bool busy = false;
@@ -77,21 +210,12 @@ struct Slot {
   struct mgb_server* server = nullptr;
 };
 
-// Results of the plain path: stream-ordered allocations from the default
-// pool (hipFree would wait for the whole device).
-void* dev_alloc(void*, size_t bytes, void* stream)
-{
-  void* p = nullptr;
-  return hipMallocAsync(&p, bytes, static_cast<hipStream_t>(stream)) == hipSuccess ? p : nullptr;
-}
-void dev_free(void*, void* p, size_t, void* stream) { (void)hipFreeAsync(p, static_cast<hipStream_t>(stream)); }
-
 // Released after the query's work and its copies have completed.
-void free_out(std::vector<mgb_out_column>& cols)
+void free_out(Arena* arena, std::vector<mgb_out_column>& cols)
 {
   for (auto& c : cols) {
-    if (c.data) (void)hipFreeAsync(c.data, nullptr);
-    if (c.null_mask) (void)hipFreeAsync(c.null_mask, nullptr);
+    arena->deallocate(c.data);
+    arena->deallocate(c.null_mask);
     c.data = nullptr;
     c.null_mask = nullptr;
   }
@@ -116,6 +240,9 @@ struct mgb_server {
   mgb_server_stats stats{};
   std::vector<double> batch_sizes;
   std::map<std::string, int64_t> estimates;  // distinct key rows per (table, key columns); tables are immutable
+  bool wait_memory = false;                   // the next query waits for memory to be released
+  std::unique_ptr<Arena> arena;               // memory of batches and plain-path results
+  mgb_allocator alloc{};                      // over `arena`
 
   void run();
   void retire(Slot& s);
@@ -232,9 +359,8 @@ bool mgb_server::launch(Slot& s)
     Done d;
     d.keys.assign(keys.size(), mgb_out_column{});
     d.results.assign(nout, mgb_out_column{});
-    mgb_allocator al{dev_alloc, dev_free, nullptr};
     d.status = mgb_groupby_aggregate(keys.data(), (int32_t)keys.size(), q.null_policy, reqs.data(),
-                                     (int32_t)reqs.size(), d.keys.data(), d.results.data(), &d.groups, s.stream, &al);
+                                     (int32_t)reqs.size(), d.keys.data(), d.results.data(), &d.groups, s.stream, &alloc);
     if (d.status != MGB_SUCCESS) d.error = mgb_last_error_message();
     d.latency_ms = ms_since(q.arrival, Clock::now());
     for (auto& c : d.keys) d.bytes += (int64_t)(c.data_bytes + c.mask_bytes);
@@ -250,6 +376,7 @@ bool mgb_server::launch(Slot& s)
   }
   // continuous batching: plan the waiting queries as one batch; halve it
   // until it fits the HBM budget next to what is resident and held
+  bool retried = false;
   while (!take.empty()) {
     const int Q = (int)take.size();
     const int nk = (int)take[0].keys.size(), nr = (int)take[0].vals.size();
@@ -286,6 +413,7 @@ bool mgb_server::launch(Slot& s)
     bo.direct_item_rows = 0;
     bo.estimate_scale = 1.0;
     bo.estimates = est.data();
+    bo.alloc = &alloc;
     mgb_batch* b = nullptr;
     const auto t_plan = Clock::now();
     mgb_status st = mgb_batch_create(keys.data(), nk, take[0].null_policy, reqs.data(), nr, Q, &bo, s.stream, &b);
@@ -294,35 +422,45 @@ bool mgb_server::launch(Slot& s)
       std::lock_guard<std::mutex> lk(mu);
       stats.host_plan_ms += plan_ms;
     }
-    if (st != MGB_SUCCESS) {
+    if (st != MGB_SUCCESS && st != MGB_ERROR_ALLOC) {
       const std::string msg = mgb_last_error_message();
       std::lock_guard<std::mutex> lk(mu);
       for (auto& q : take) fail_query(q, st, msg);
       done_cv.notify_all();
       return true;
     }
+    // no memory for the batch now (the chunks are taken and may not grow):
+    // like a batch over the budget
+    const bool no_memory = st == MGB_ERROR_ALLOC;
     mgb_batch_stats bs{};
-    (void)mgb_batch_get_stats(b, &bs);
-    bool fits;
-    {
+    if (!no_memory) (void)mgb_batch_get_stats(b, &bs);
+    bool fits = false;
+    if (!no_memory) {
       std::lock_guard<std::mutex> lk(mu);
       fits = resident_bytes + held_bytes + bs.device_bytes <= opt.hbm_budget_bytes;
       if (fits) held_bytes += bs.device_bytes;
     }
     if (!fits) {
-      mgb_batch_destroy(b);
+      if (b) mgb_batch_destroy(b);
       std::lock_guard<std::mutex> lk(mu);
       ++stats.deferred;
       if (Q == 1) {
         // fails only when it could not fit even with nothing else held
-        if (resident_bytes + bs.device_bytes <= opt.hbm_budget_bytes) {
+        if (no_memory ? arena->in_use() : resident_bytes + bs.device_bytes <= opt.hbm_budget_bytes) {
           pending.push_front(std::move(take[0]));  // wait for memory to be released
+          wait_memory = true;
           return false;
         }
+        if (no_memory && !retried) {
+          // the last block in use may have come back since the attempt
+          retried = true;
+          continue;
+        }
         fail_query(take[0], MGB_ERROR_ALLOC,
-                   "query does not fit the HBM budget (" + std::to_string(bs.device_bytes) + " bytes next to " +
-                       std::to_string(resident_bytes) + " resident, budget " + std::to_string(opt.hbm_budget_bytes) +
-                       ")");
+                   no_memory ? "no device memory for the query"
+                             : "query does not fit the HBM budget (" + std::to_string(bs.device_bytes) +
+                                   " bytes next to " + std::to_string(resident_bytes) + " resident, budget " +
+                                   std::to_string(opt.hbm_budget_bytes) + ")");
         done_cv.notify_all();
         return true;
       }
@@ -369,7 +507,7 @@ void mgb_server::run()
           if (s->busy && s->finished.load()) return true;
         bool idle = false;
         for (auto& s : slots) idle = idle || !s->busy;
-        return idle && !pending.empty();
+        return idle && !pending.empty() && !wait_memory;
       });
       if (stop) {
         bool busy = false;
@@ -396,6 +534,11 @@ extern "C" mgb_status mgb_server_create(const mgb_server_options* options, mgb_s
   if (s->opt.max_inflight <= 0) s->opt.max_inflight = 2;
   if (s->opt.max_batch_queries <= 0) s->opt.max_batch_queries = 256;
   if (!s->opt.batching) s->opt.max_inflight = 1;
+  s->arena = std::make_unique<Arena>(s->opt.hbm_budget_bytes);
+  s->alloc = mgb_allocator{arena_alloc, arena_free, s->arena.get()};
+  if (s->opt.reserve_bytes > 0 &&
+      (s->opt.reserve_bytes > s->opt.hbm_budget_bytes || !s->arena->reserve((size_t)s->opt.reserve_bytes)))
+    return fail(MGB_ERROR_ALLOC, "the reserve does not fit the budget or the device");
   for (int i = 0; i < s->opt.max_inflight; ++i) {
     auto slot = std::make_unique<Slot>();
     slot->server = s.get();
@@ -419,9 +562,11 @@ extern "C" mgb_status mgb_server_add_table(mgb_server* s, const mgb_column* colu
     t.bytes += c.size * (int64_t)type_width(c.type_id) + (c.null_mask ? bitmask_bytes(c.size) : 0);
   }
   std::lock_guard<std::mutex> lk(s->mu);
-  if (s->resident_bytes + t.bytes > s->opt.hbm_budget_bytes)
+  // the server's own device memory counts against the budget too
+  if (s->resident_bytes + t.bytes + s->arena->total() > s->opt.hbm_budget_bytes)
     return fail(MGB_ERROR_ALLOC, "table does not fit the HBM budget");
   s->resident_bytes += t.bytes;
+  s->arena->set_limit(s->opt.hbm_budget_bytes - s->resident_bytes);
   *table_id = (int32_t)s->tables.size();
   s->tables.push_back(std::move(t));
   return MGB_SUCCESS;
@@ -488,13 +633,20 @@ namespace {
 // What a released ticket held. Freed with no lock held: hipFree waits for
 // the device, and so for the host callbacks of batches still in flight.
 struct Released {
+  mgb_server* server = nullptr;
   std::shared_ptr<HeldBatch> batch;
   std::vector<mgb_out_column> keys, results;
   ~Released()
   {
     batch.reset();  // the last reference destroys the batch
-    free_out(keys);
-    free_out(results);
+    free_out(server->arena.get(), keys);
+    free_out(server->arena.get(), results);
+    // the memory is back: a query waiting for it may start
+    {
+      std::lock_guard<std::mutex> lk(server->mu);
+      server->wait_memory = false;
+    }
+    server->cv.notify_all();
   }
 };
 
@@ -503,6 +655,7 @@ struct Released {
 std::unique_ptr<Released> release_locked(mgb_server* s, std::map<int64_t, Done>::iterator it)
 {
   auto r = std::make_unique<Released>();
+  r->server = s;
   Done& d = it->second;
   if (d.batch) {
     if (--d.batch->remaining == 0) s->held_bytes -= d.batch->bytes;
@@ -513,7 +666,7 @@ std::unique_ptr<Released> release_locked(mgb_server* s, std::map<int64_t, Done>:
     r->results = std::move(d.results);
   }
   s->done.erase(it);
-  s->cv.notify_all();  // memory may have been released
+  s->cv.notify_all();  // the budget may admit a waiting batch
   return r;
 }
 
@@ -606,6 +759,7 @@ extern "C" mgb_status mgb_server_get_stats(mgb_server* s, mgb_server_stats* out)
   out->resident_bytes = s->resident_bytes;
   out->held_bytes = s->held_bytes;
   out->pending = (int64_t)s->pending.size();
+  out->device_bytes = s->arena->total();
   double sum = 0;
   for (double b : s->batch_sizes) sum += b;
   out->mean_batch_queries = s->batch_sizes.empty() ? 0.0 : sum / (double)s->batch_sizes.size();

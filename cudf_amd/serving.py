@@ -26,10 +26,11 @@ from .types import NullPolicy
 
 class Server:
     def __init__(self, hbm_budget_bytes: int = 200 << 30, max_inflight: int = 2, max_batch_queries: int = 256,
-                 batching: bool = True, device: Optional[torch.device] = None):
+                 batching: bool = True, reserve_bytes: int = 0, device: Optional[torch.device] = None):
         self._lib = _lib.load()
         opt = _lib.mgb_server_options()
         opt.hbm_budget_bytes = int(hbm_budget_bytes)
+        opt.reserve_bytes = int(reserve_bytes)
         opt.max_inflight = int(max_inflight)
         opt.max_batch_queries = int(max_batch_queries)
         opt.batching = 1 if batching else 0

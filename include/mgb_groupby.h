@@ -218,6 +218,8 @@ typedef struct mgb_batch_options {
   double estimate_scale;     /* multiplies every distinct-count estimate (1.0; below 1 only to test re-runs) */
   const int64_t* estimates;  /* distinct key rows per query (e.g. cached from mgb_estimate_groups for
                                 resident tables), or NULL: sampled on the device (one synchronisation) */
+  const mgb_allocator* alloc; /* the batch's device memory (one allocation, freed by mgb_batch_destroy on
+                                 the create stream), or NULL: stream-ordered from the device's pool */
 } mgb_batch_options;
 
 typedef struct mgb_batch_stats {
@@ -267,7 +269,10 @@ void mgb_batch_destroy(mgb_batch* batch);
  * + the batch's own memory; a batch that does not fit is halved, a query
  * that does not fit waits for memory to be released), and runs up to
  * max_inflight batches at once, one stream each, so their device work
- * overlaps. With batching = 0 every query runs alone on the plain path, in
+ * overlaps. Batches and results take their memory from the server's own
+ * device chunks, allocated at creation (reserve_bytes) or when first needed
+ * and kept until mgb_server_destroy: new device memory is slow to map and
+ * clear, and would stall the work in flight. With batching = 0 every query runs alone on the plain path, in
  * arrival order (the per-query baseline). Results do not depend on which
  * batch a query ran in, beyond floating-point summation order.
  * ------------------------------------------------------------------------ */
@@ -279,6 +284,8 @@ typedef struct mgb_server_options {
   int32_t max_batch_queries;  /* queries per batch at most (0: 256) */
   int32_t batching;           /* 1: continuous batching; 0: per-query plain path */
   int32_t reserved;
+  int64_t reserve_bytes;      /* device memory for batches and results taken at creation (0: none);
+                                 the server grows it on demand up to the budget and keeps it */
 } mgb_server_options;
 
 /* One request of a query: a column of the table and its aggregations. */
@@ -302,6 +309,7 @@ typedef struct mgb_server_stats {
   double mean_batch_queries;
   double host_plan_ms;        /* scheduler time spent planning batches */
   double host_finish_ms;      /* scheduler time spent finishing batches */
+  int64_t device_bytes;       /* device memory the server holds for batches and results */
 } mgb_server_stats;
 
 mgb_status mgb_server_create(const mgb_server_options* options, mgb_server** out);

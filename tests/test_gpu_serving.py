@@ -177,6 +177,26 @@ def test_hbm_budget_admission(gpu):
     srv.close()
 
 
+def test_reserved_memory(gpu):
+    rng = np.random.default_rng(26)
+    host, dev = _tables(rng, 3, max_rows=100_000)
+    with pytest.raises(ca._lib.AllocError):
+        ca.Server(hbm_budget_bytes=64 << 20, reserve_bytes=128 << 20)
+    srv = ca.Server(reserve_bytes=256 << 20, max_inflight=2, max_batch_queries=4)
+    assert srv.stats()["device_bytes"] == 256 << 20
+    tids = [srv.add_table(t) for t in dev]
+    picks = rng.integers(0, len(tids), 16)
+    tickets = [_submit(srv, tids[t]) for t in picks]
+    got = _drain(srv, tickets)
+    for tk, t in zip(tickets, picks):
+        _same(got[tk], _alone(dev[t]), host[t], f"ticket {tk}")
+    st = srv.stats()
+    print("reserve stats", st, flush=True)
+    # every batch was carved from the reserved chunk: nothing more was taken
+    assert st["device_bytes"] == 256 << 20 and st["held_bytes"] == 0, st
+    srv.close()
+
+
 def test_baseline_mode_matches(gpu):
     rng = np.random.default_rng(24)
     host, dev = _tables(rng, 3, max_rows=50_000)
