@@ -942,6 +942,8 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
       ra.flags = flag_bytes ? recs + flags_rec_off : nullptr;
       ra.flag_bytes = flag_bytes;
       ra.flag_stride = rec_bytes;
+      ra.rec_words = rec_bytes <= 32 ? rec_bytes / 8 : 0;
+      ra.rec_base = recs;
       for (int c = 0; c < ra.num_keys; ++c) {
         ra.keys[c].data = recs + koff[c];
         ra.keys[c].stride = rec_bytes;

@@ -1482,6 +1482,8 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     lp.src.flags = flag_bytes ? recs + flags_rec_off : nullptr;
     lp.src.flag_bytes = flag_bytes;
     lp.src.flag_stride = rec_bytes;
+    lp.src.rec_words = rec_bytes <= 32 ? rec_bytes / 8 : 0;
+    lp.src.rec_base = recs;
     for (size_t c = 0; c < plan.keys.size(); ++c) {
       lp.src.keys[c].data = recs + koff[c];
       lp.src.keys[c].stride = rec_bytes;

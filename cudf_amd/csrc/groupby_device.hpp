@@ -367,6 +367,78 @@ __device__ __forceinline__ void load_rows(const uint8_t* data, int width, int st
   }
 }
 
+// The canonical form (see load_rows) of the low `width` bytes of raw.
+__device__ __forceinline__ uint64_t widen_raw(uint64_t raw, int width, int vclass)
+{
+  const bool sgn = vclass == VC_SIGNED;
+  switch (width) {
+    case 1: {
+      const uint8_t b = (uint8_t)raw;
+      return sgn ? (uint64_t)(int64_t)(int8_t)b : vclass == VC_BOOL ? (uint64_t)(b != 0) : (uint64_t)b;
+    }
+    case 2: {
+      const uint16_t b = (uint16_t)raw;
+      return sgn ? (uint64_t)(int64_t)(int16_t)b : (uint64_t)b;
+    }
+    case 4: {
+      const uint32_t b = (uint32_t)raw;
+      return sgn ? (uint64_t)(int64_t)(int32_t)b : (uint64_t)b;
+    }
+    default:
+      return raw;
+  }
+}
+
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+// One record of up to four 8-byte words, held in registers. A vector value
+// and not an array: its words are picked by a uniform offset
+// (extract_rows), which on an array becomes an indexed copy in scratch
+// memory; on a vector it is a register select.
+typedef uint64_t RecWords __attribute__((ext_vector_type(4)));
+
+// Whole records row0 + u * step (u < N, where present[u]) of `words` (<= 4)
+// 8-byte words each into rw[u]; absent rows and words past the record read 0.
+template <int N>
+__device__ __forceinline__ void load_recs(const uint8_t* base, int words, int64_t row0, int64_t step,
+                                          const bool (&present)[N], RecWords (&rw)[N])
+{
+  if (words == 4) {
+    // 32-byte records of a 16-byte aligned array: two 16-byte loads each
+    const global_t<const u64x2>* g = to_global(reinterpret_cast<const u64x2*>(base));
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      u64x2 a = {0ull, 0ull}, b = {0ull, 0ull};
+      if (present[u]) {
+        a = g[(row0 + u * step) * 2];
+        b = g[(row0 + u * step) * 2 + 1];
+      }
+      rw[u] = __builtin_shufflevector(a, b, 0, 1, 2, 3);
+    }
+  } else {
+    const global_t<const uint64_t>* g = to_global(reinterpret_cast<const uint64_t*>(base));
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      const global_t<const uint64_t>* r = g + (row0 + u * step) * words;
+      const bool ok = present[u];
+      const uint64_t w0 = ok ? r[0] : 0ull;
+      const uint64_t w1 = ok && words > 1 ? r[1] : 0ull;
+      const uint64_t w2 = ok && words > 2 ? r[2] : 0ull;
+      rw[u] = RecWords{w0, w1, w2, 0ull};
+    }
+  }
+}
+
+// The field at byte `off` (naturally aligned, so inside one 8-byte word) of
+// records rw[u], in canonical form.
+template <int N>
+__device__ __forceinline__ void extract_rows(const RecWords (&rw)[N], int off, int width, int vclass, uint64_t (&x)[N])
+{
+  const int wi = (off >> 3) & 3, sh = (off & 7) * 8;
+#pragma unroll
+  for (int u = 0; u < N; ++u) x[u] = widen_raw(rw[u][wi] >> sh, width, vclass);
+}
+
 template <int N>
 __device__ __forceinline__ void copy_rows(const uint64_t (&from)[N], uint64_t (&to)[N])
 {

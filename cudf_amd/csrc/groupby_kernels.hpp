@@ -24,6 +24,12 @@ struct AggParams {
   int32_t num_vals;
   int32_t num_ops;
   int32_t rows_op;                // index of the OP_ROWS op in ops[], or -1
+  // partitioned records of at most 32 bytes: row r is the record at
+  // rec_base + r * 8 * rec_words, read whole (the key, value and flags
+  // columns point into it); 0: rows are read column by column
+  int32_t rec_words = 0;
+  int32_t rec_pad = 0;
+  const uint8_t* rec_base = nullptr;
   KeyCol keys[kMaxKeys];
   ValCol vals[kMaxVals];
   OpDesc ops[kMaxOps];

@@ -55,21 +55,20 @@ __device__ __forceinline__ void lds_table_init(const LdsTable& t, const uint64_t
   __syncthreads();
 }
 
-// Aggregates rows [begin, end) of `p` into the LDS table. `log2p` is the
-// number of hash bits that chose the rows' partition (the slot comes from the
-// bits below them); rows with a null key are dropped when `check_keys`.
-// Ends with a workgroup barrier.
-__device__ __forceinline__ void lds_aggregate_rows(const AggParams& p, const LdsTable& t, uint8_t* lds, int64_t begin,
-                                                   int64_t end, int log2p, bool check_keys)
+// The row loop of lds_aggregate_rows; kRec: rows are whole records
+// (p.rec_words), loaded with one or two wide loads each.
+template <bool kRec>
+__device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, const LdsTable& t, uint8_t* lds,
+                                                        int64_t begin, int64_t end, int log2p, bool check_keys)
 {
   // Rows are taken kRows per thread at a time (rows base + u * kLdsBlock):
-  // the loads of the key, the flags and the first kPreCols value columns of
-  // all of them are issued together, then the rows are probed (the home slot
+  // the loads of the key, the flags and the first kPreCols value columns (or
+  // the whole records) of all of them are issued together, then the rows are probed (the home slot
   // headers of all rows read together, the full probe only on a miss), then
   // each value column updates the rows' records (lds_column_rows). Rows whose
   // group does not fit go to the global table afterwards.
   constexpr int kRows = 3;
-  constexpr int kPreCols = 3;  // the switch below picks pre[0..2]
+  constexpr int kPreCols = kRec ? 0 : 3;  // the switch below picks pre[0..2]
   const int64_t step = kLdsBlock;
   const KeyCol& k0 = p.keys[0];
   const bool int_key = p.key_mode == KEY_INT;
@@ -78,19 +77,31 @@ __device__ __forceinline__ void lds_aggregate_rows(const AggParams& p, const Lds
 #pragma unroll
     for (int u = 0; u < kRows; ++u) present[u] = base + u * step < end;
     uint64_t key[kRows];
-    if (int_key) load_rows(k0.data, k0.width, k0.stride, VC_UNSIGNED, base, step, present, key);
     uint32_t flags[kRows];
-    {
+    // records: every field of the rows comes from registers (rw)
+    RecWords rw[kRec ? kRows : 1];
+    uint64_t pre[kRec ? 1 : kPreCols][kRows];
+    if constexpr (kRec) {
+      load_recs(p.rec_base, p.rec_words, base, step, present, rw);
+      if (int_key) extract_rows(rw, (int)(k0.data - p.rec_base), k0.width, VC_UNSIGNED, key);
       uint64_t f[kRows];
-      if (p.flags) load_rows(p.flags, p.flag_bytes, p.flag_stride, VC_UNSIGNED, base, step, present, f);
+      if (p.flags) extract_rows(rw, (int)(p.flags - p.rec_base), p.flag_bytes, VC_UNSIGNED, f);
 #pragma unroll
       for (int u = 0; u < kRows; ++u) flags[u] = p.flags ? (uint32_t)f[u] : 0u;
-    }
-    uint64_t pre[kPreCols][kRows];
+    } else {
+      if (int_key) load_rows(k0.data, k0.width, k0.stride, VC_UNSIGNED, base, step, present, key);
+      {
+        uint64_t f[kRows];
+        if (p.flags) load_rows(p.flags, p.flag_bytes, p.flag_stride, VC_UNSIGNED, base, step, present, f);
 #pragma unroll
-    for (int c = 0; c < kPreCols; ++c)
-      if (c < p.num_vals)
-        load_rows(p.vals[c].data, p.vals[c].width, p.vals[c].stride, p.vals[c].vclass, base, step, present, pre[c]);
+        for (int u = 0; u < kRows; ++u) flags[u] = p.flags ? (uint32_t)f[u] : 0u;
+      }
+#pragma unroll
+      for (int c = 0; c < kPreCols; ++c)
+        if (c < p.num_vals)
+          load_rows(p.vals[c].data, p.vals[c].width, p.vals[c].stride, p.vals[c].vclass, base, step, present,
+                    pre[c]);
+    }
     if (check_keys && p.skip_null_keys) {
 #pragma unroll
       for (int u = 0; u < kRows; ++u)
@@ -159,13 +170,17 @@ __device__ __forceinline__ void lds_aggregate_rows(const AggParams& p, const Lds
 #pragma unroll 1
     for (int c = 0; c < p.num_vals; ++c) {
       uint64_t x[kRows];
-      switch (c) {
-        case 0: copy_rows(pre[0], x); break;
-        case 1: copy_rows(pre[1], x); break;
-        case 2: copy_rows(pre[2], x); break;
-        default:
-          load_rows(p.vals[c].data, p.vals[c].width, p.vals[c].stride, p.vals[c].vclass, base, step, present, x);
-          break;
+      if constexpr (kRec) {
+        extract_rows(rw, (int)(p.vals[c].data - p.rec_base), p.vals[c].width, p.vals[c].vclass, x);
+      } else {
+        switch (c) {
+          case 0: copy_rows(pre[0], x); break;
+          case 1: copy_rows(pre[1], x); break;
+          case 2: copy_rows(pre[2], x); break;
+          default:
+            load_rows(p.vals[c].data, p.vals[c].width, p.vals[c].stride, p.vals[c].vclass, base, step, present, x);
+            break;
+        }
       }
       lds_column_rows(p, lds, p.vals[c], loff, x, flags, base, step);
     }
@@ -184,6 +199,17 @@ __device__ __forceinline__ void lds_aggregate_rows(const AggParams& p, const Lds
     }
   }
   __syncthreads();
+}
+
+// Aggregates rows [begin, end) of `p` into the LDS table. `log2p` is the
+// number of hash bits that chose the rows' partition (the slot comes from the
+// bits below them); rows with a null key are dropped when `check_keys`.
+// Ends with a workgroup barrier.
+__device__ __forceinline__ void lds_aggregate_rows(const AggParams& p, const LdsTable& t, uint8_t* lds, int64_t begin,
+                                                   int64_t end, int log2p, bool check_keys)
+{
+  if (p.rec_words > 0) lds_aggregate_rows_impl<true>(p, t, lds, begin, end, log2p, check_keys);
+  else lds_aggregate_rows_impl<false>(p, t, lds, begin, end, log2p, check_keys);
 }
 
 // Folds every group of the LDS table into p's global table (device atomics).
