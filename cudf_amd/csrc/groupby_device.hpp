@@ -511,35 +511,36 @@ __device__ __forceinline__ double canon_f64(uint64_t x, int vclass)
 
 // One accumulator op on the LDS records of N rows (ok[u]: the row has a
 // record and a valid value). x[u] is the canonical value (see load_rows).
+// Rows that are not ok update the thread's dummy word `dum` (LDS byte
+// offset, 8-byte aligned, never read) instead of branching around the update.
 template <int N>
 __device__ __forceinline__ void lds_op_rows(uint8_t* lds, const uint32_t (&loff)[N], const uint64_t (&x)[N],
-                                            const bool (&ok)[N], int kind, int off, int vclass)
+                                            const bool (&ok)[N], int kind, int off, int vclass, uint32_t dum)
 {
   switch (kind) {
     case OP_ROWS: case OP_COUNT:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_add(reinterpret_cast<uint32_t*>(lds + loff[u] + off), 1u);
+        lds_add(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + off : dum)), 1u);
       break;
     case OP_SUM_I64:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_add(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), (unsigned long long)x[u]);
+        lds_add(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + off : dum)), (unsigned long long)x[u]);
       break;
     case OP_SUM_F64:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_add(reinterpret_cast<double*>(lds + loff[u] + off), canon_f64(x[u], vclass));
+        lds_add(reinterpret_cast<double*>(lds + (ok[u] ? loff[u] + off : dum)), canon_f64(x[u], vclass));
       break;
     case OP_SUMSQ_I64:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_add(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), (unsigned long long)(x[u] * x[u]));
+        lds_add(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + off : dum)), (unsigned long long)(x[u] * x[u]));
       break;
     case OP_SUMSQ_F64:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
-        if (!ok[u]) continue;
         double sq;
         if (vclass == VC_F32) {
           const float f = __uint_as_float((uint32_t)x[u]);
@@ -548,7 +549,7 @@ __device__ __forceinline__ void lds_op_rows(uint8_t* lds, const uint32_t (&loff)
           const double d = __longlong_as_double((long long)x[u]);
           sq = d * d;
         }
-        lds_add(reinterpret_cast<double*>(lds + loff[u] + off), sq);
+        lds_add(reinterpret_cast<double*>(lds + (ok[u] ? loff[u] + off : dum)), sq);
       }
       break;
     case OP_PROD_I64: case OP_PROD_F64:
@@ -579,77 +580,73 @@ __device__ __forceinline__ void lds_op_rows(uint8_t* lds, const uint32_t (&loff)
     case OP_MIN_I32:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_min(reinterpret_cast<int32_t*>(lds + loff[u] + off), (int32_t)x[u]);
+        lds_min(reinterpret_cast<int32_t*>(lds + (ok[u] ? loff[u] + off : dum)), (int32_t)x[u]);
       break;
     case OP_MAX_I32:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_max(reinterpret_cast<int32_t*>(lds + loff[u] + off), (int32_t)x[u]);
+        lds_max(reinterpret_cast<int32_t*>(lds + (ok[u] ? loff[u] + off : dum)), (int32_t)x[u]);
       break;
     case OP_MIN_U32:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_min(reinterpret_cast<uint32_t*>(lds + loff[u] + off), (uint32_t)x[u]);
+        lds_min(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + off : dum)), (uint32_t)x[u]);
       break;
     case OP_MAX_U32:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_max(reinterpret_cast<uint32_t*>(lds + loff[u] + off), (uint32_t)x[u]);
+        lds_max(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + off : dum)), (uint32_t)x[u]);
       break;
     case OP_MIN_I64:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_min(reinterpret_cast<long long*>(lds + loff[u] + off), (long long)x[u]);
+        lds_min(reinterpret_cast<long long*>(lds + (ok[u] ? loff[u] + off : dum)), (long long)x[u]);
       break;
     case OP_MAX_I64:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_max(reinterpret_cast<long long*>(lds + loff[u] + off), (long long)x[u]);
+        lds_max(reinterpret_cast<long long*>(lds + (ok[u] ? loff[u] + off : dum)), (long long)x[u]);
       break;
     case OP_MIN_U64:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_min(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), (unsigned long long)x[u]);
+        lds_min(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + off : dum)), (unsigned long long)x[u]);
       break;
     case OP_MAX_U64:
 #pragma unroll
       for (int u = 0; u < N; ++u)
-        if (ok[u]) lds_max(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), (unsigned long long)x[u]);
+        lds_max(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + off : dum)), (unsigned long long)x[u]);
       break;
     case OP_MIN_F32:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
-        if (!ok[u]) continue;
         const uint32_t b = (uint32_t)x[u];
         const bool nan = (b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu);
-        lds_min(reinterpret_cast<uint32_t*>(lds + loff[u] + off), nan ? kF32MinNaN : f32_code(b));
+        lds_min(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + off : dum)), nan ? kF32MinNaN : f32_code(b));
       }
       break;
     case OP_MAX_F32:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
-        if (!ok[u]) continue;
         const uint32_t b = (uint32_t)x[u];
         const bool nan = (b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu);
-        lds_max(reinterpret_cast<uint32_t*>(lds + loff[u] + off), nan ? kF32MaxNaN : f32_code(b));
+        lds_max(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + off : dum)), nan ? kF32MaxNaN : f32_code(b));
       }
       break;
     case OP_MIN_F64:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
-        if (!ok[u]) continue;
         const unsigned long long b = x[u];
         const bool nan = (b & 0x7ff0000000000000ull) == 0x7ff0000000000000ull && (b & 0x000fffffffffffffull);
-        lds_min(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), nan ? kF64MinNaN : f64_code(b));
+        lds_min(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + off : dum)), nan ? kF64MinNaN : f64_code(b));
       }
       break;
     case OP_MAX_F64:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
-        if (!ok[u]) continue;
         const unsigned long long b = x[u];
         const bool nan = (b & 0x7ff0000000000000ull) == 0x7ff0000000000000ull && (b & 0x000fffffffffffffull);
-        lds_max(reinterpret_cast<unsigned long long*>(lds + loff[u] + off), nan ? kF64MaxNaN : f64_code(b));
+        lds_max(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + off : dum)), nan ? kF64MaxNaN : f64_code(b));
       }
       break;
     default: break;  // OP_NULLS: updated for null values by the caller
@@ -662,7 +659,7 @@ __device__ __forceinline__ void lds_op_rows(uint8_t* lds, const uint32_t (&loff)
 template <int N>
 __device__ __forceinline__ void lds_column_rows(const AggParams& p, uint8_t* lds, const ValCol& v,
                                                 const uint32_t (&loff)[N], const uint64_t (&x)[N],
-                                                const uint32_t (&flags)[N], int64_t row0, int64_t step)
+                                                const uint32_t (&flags)[N], int64_t row0, int64_t step, uint32_t dum)
 {
   bool ok[N];
 #pragma unroll
@@ -675,9 +672,19 @@ __device__ __forceinline__ void lds_column_rows(const AggParams& p, uint8_t* lds
   if (v.nulls_off >= 0) {
 #pragma unroll
     for (int u = 0; u < N; ++u)
-      if (loff[u] < kRecSpill && !ok[u]) lds_add(reinterpret_cast<uint32_t*>(lds + loff[u] + v.nulls_off), 1u);
+      lds_add(reinterpret_cast<uint32_t*>(lds + (loff[u] < kRecSpill && !ok[u] ? loff[u] + v.nulls_off : dum)), 1u);
   }
-  for (int o = v.op_begin; o < v.op_end; ++o) lds_op_rows(lds, loff, x, ok, p.ops[o].kind, p.ops[o].off, v.vclass);
+  // the column's op descriptors are read together, before any update: a
+  // scalar load inside the op loop would wait for every LDS update issued
+  // before it
+  OpDesc cop[kMaxColOps];
+  const OpDesc* ops = p.ops + v.op_begin;
+#pragma unroll
+  for (int o = 0; o < kMaxColOps; ++o) cop[o] = ops[o];
+  const int nops = v.op_end - v.op_begin;
+#pragma unroll
+  for (int o = 0; o < kMaxColOps; ++o)
+    if (o < nops) lds_op_rows(lds, loff, x, ok, cop[o].kind, cop[o].off, v.vclass, dum);
 }
 
 // ------------------------------------------------------------- finalisation

@@ -32,7 +32,7 @@ struct AggParams {
   const uint8_t* rec_base = nullptr;
   KeyCol keys[kMaxKeys];
   ValCol vals[kMaxVals];
-  OpDesc ops[kMaxOps];
+  OpDesc ops[kMaxOps + kMaxColOps];  // the last kMaxColOps unused: a column's window of ops is read whole
 };
 
 // How one requested aggregation is produced from record fields.

@@ -72,6 +72,9 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
   const int64_t step = kLdsBlock;
   const KeyCol& k0 = p.keys[0];
   const bool int_key = p.key_mode == KEY_INT;
+  // the thread's dummy word for updates of rows that have none: in the scan
+  // space after the table, unused until the groups are written out
+  const uint32_t dum = (uint32_t)(reinterpret_cast<uint8_t*>(t.count + 4) - lds) + (threadIdx.x & 63u) * 8u;
   for (int64_t base = begin + threadIdx.x; base < end; base += (int64_t)kRows * kLdsBlock) {
     bool present[kRows];
 #pragma unroll
@@ -163,7 +166,7 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
       const int off = p.ops[p.rows_op].off;
 #pragma unroll
       for (int u = 0; u < kRows; ++u)
-        if (loff[u] < kRecSpill) lds_add(reinterpret_cast<uint32_t*>(lds + loff[u] + off), 1u);
+        lds_add(reinterpret_cast<uint32_t*>(lds + (loff[u] < kRecSpill ? loff[u] + off : dum)), 1u);
     }
     // one call site for all columns keeps the kernel's code small; the
     // preloaded values are picked by a uniform switch
@@ -182,7 +185,7 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
             break;
         }
       }
-      lds_column_rows(p, lds, p.vals[c], loff, x, flags, base, step);
+      lds_column_rows(p, lds, p.vals[c], loff, x, flags, base, step, dum);
     }
 
     // groups that did not fit: the global table (a separate call keeps the
