@@ -157,6 +157,73 @@ __global__ __launch_bounds__(kBlock) void pack_validity_kernel(const uint8_t* va
   if (nulls) atom_add(null_count, (unsigned long long)nulls);
 }
 
+// ---- result columns: copy and validity packing in one launch
+
+// One result column: `bytes` of data copied from the temporary outputs and,
+// when `mask` is set, `rows` validity bytes packed into `words` bitmask words
+// (bits past the rows 0) with the nulls added to *nulls.
+struct EmitCol {
+  const uint8_t* src;
+  uint8_t* dst;
+  int64_t bytes;
+  const uint8_t* valid;
+  uint32_t* mask;
+  int64_t words;
+  unsigned long long* nulls;
+};
+constexpr int kEmitCols = 32;
+struct EmitParams {
+  EmitCol cols[kEmitCols];
+  int64_t rows;
+};
+
+// blockIdx.y: the column; the blocks along x stride over its units.
+__global__ __launch_bounds__(kBlock) void emit_columns_kernel(const EmitParams ep)
+{
+  __shared__ unsigned int block_nulls;
+  const EmitCol& c = ep.cols[blockIdx.y];
+  const int64_t first = (int64_t)blockIdx.x * kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+  const global_t<const uint8_t>* src = to_global(c.src);
+  global_t<uint8_t>* dst = to_global(c.dst);
+  int64_t tail = 0;
+  if ((((uintptr_t)c.src | (uintptr_t)c.dst) & 15) == 0) {
+    const int64_t n16 = c.bytes >> 4;
+    const global_t<const u64x2>* s16 = reinterpret_cast<const global_t<const u64x2>*>(src);
+    global_t<u64x2>* d16 = reinterpret_cast<global_t<u64x2>*>(dst);
+    for (int64_t i = first; i < n16; i += stride) d16[i] = s16[i];
+    tail = n16 << 4;
+  }
+  for (int64_t i = tail + first; i < c.bytes; i += stride) dst[i] = src[i];
+  if (!c.mask) return;
+  if (threadIdx.x == 0) block_nulls = 0;
+  __syncthreads();
+  unsigned int nulls = 0;
+  const global_t<const uint8_t>* v = to_global(c.valid);
+  for (int64_t w = first; w < c.words; w += stride) {
+    uint32_t bits = 0;
+    const int64_t r0 = w * 32;
+    if (r0 + 32 <= ep.rows) {
+      // 32 validity bytes as two 16-byte loads (the array is 256-byte aligned)
+      const global_t<const u64x2>* v16 = reinterpret_cast<const global_t<const u64x2>*>(v + r0);
+      const u64x2 a = v16[0], b = v16[1];
+      const uint64_t q[4] = {a.x, a.y, b.x, b.y};
+#pragma unroll
+      for (int k = 0; k < 32; ++k)
+        if ((q[k >> 3] >> ((k & 7) * 8)) & 0xffu) bits |= 1u << k;
+      nulls += 32u - (unsigned)__popc(bits);
+    } else {
+      for (int k = 0; k < 32 && r0 + k < ep.rows; ++k) {
+        if (v[r0 + k]) bits |= 1u << k;
+        else ++nulls;
+      }
+    }
+    to_global(c.mask)[w] = bits;
+  }
+  if (nulls) atomicAdd(&block_nulls, nulls);
+  __syncthreads();
+  if (threadIdx.x == 0 && block_nulls) atom_add(c.nulls, (unsigned long long)block_nulls);
+}
+
 // ---- cardinality estimate: stratified sample into a small table
 
 struct SampleParams {
@@ -251,28 +318,51 @@ __global__ __launch_bounds__(kLdsBlock) void hll_block_kernel(const AggParams p,
 }
 
 // Max over workgroups per register, then per block the sum of 2^-M and the
-// number of empty registers.
+// number of empty registers. A workgroup of 256 threads covers 128 registers
+// (32 words of 4 byte-registers): its 8 groups of 32 threads each take every
+// 8th workgroup's words, and the 8 partial maxima meet in LDS.
+constexpr int kHllReduceRegs = 128;
 __global__ __launch_bounds__(256) void hll_reduce_kernel(const uint8_t* block_regs, int nb, double* sums,
                                                         uint32_t* zeros)
 {
-  __shared__ double s[256];
-  __shared__ uint32_t z[256];
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  uint32_t m = 0;
-  for (int b = 0; b < nb; ++b) m = max(m, (uint32_t)block_regs[(int64_t)b * kHllRegs + j]);
-  s[threadIdx.x] = ldexp(1.0, -(int)m);
-  z[threadIdx.x] = m == 0;
-  __syncthreads();
-  for (int d = 128; d > 0; d >>= 1) {
-    if (threadIdx.x < (unsigned)d) {
-      s[threadIdx.x] += s[threadIdx.x + d];
-      z[threadIdx.x] += z[threadIdx.x + d];
-    }
-    __syncthreads();
+  __shared__ uint32_t part[8][32][4];
+  __shared__ double s[32];
+  __shared__ uint32_t z[32];
+  const int wi = threadIdx.x & 31, sl = threadIdx.x >> 5;
+  const int64_t word = (int64_t)blockIdx.x * 32 + wi;  // u32 word of the register array
+  const global_t<const uint32_t>* w32 = to_global(reinterpret_cast<const uint32_t*>(block_regs));
+  uint32_t m[4] = {0u, 0u, 0u, 0u};
+  for (int b = sl; b < nb; b += 8) {
+    const uint32_t x = w32[(int64_t)b * (kHllRegs / 4) + word];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) m[k] = max(m[k], (x >> (8 * k)) & 0xffu);
   }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) part[sl][wi][k] = m[k];
+  __syncthreads();
+  if (sl == 0) {
+    double sum = 0.0;
+    uint32_t zero = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uint32_t r = 0;
+      for (int j = 0; j < 8; ++j) r = max(r, part[j][wi][k]);
+      sum += ldexp(1.0, -(int)r);
+      zero += r == 0;
+    }
+    s[wi] = sum;
+    z[wi] = zero;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    sums[blockIdx.x] = s[0];
-    zeros[blockIdx.x] = z[0];
+    double sum = 0.0;
+    uint32_t zero = 0;
+    for (int j = 0; j < 32; ++j) {
+      sum += s[j];
+      zero += z[j];
+    }
+    sums[blockIdx.x] = sum;
+    zeros[blockIdx.x] = zero;
   }
 }
 
@@ -936,7 +1026,6 @@ mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int6
     oc.data_bytes = (size_t)groups * width;
     oc.data = al.get(oc.data_bytes);
     if (!oc.data) return fail(MGB_ERROR_ALLOC, "allocation of an output column failed");
-    MGB_HIP(hipMemcpyAsync(oc.data, data, oc.data_bytes, hipMemcpyDeviceToDevice, stream));
     if (nullable && valid) {
       oc.mask_bytes = (size_t)bitmask_bytes(groups);
       oc.null_mask = static_cast<uint32_t*>(al.get(oc.mask_bytes));
@@ -958,12 +1047,27 @@ mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int6
   if (groups > 0) {
     MGB_HIP(hipMemsetAsync(t.null_counts, 0, (size_t)(nk + no) * 8, stream));
     const int64_t words = bitmask_bytes(groups) / 4;
-    for (int j = 0; j < no + nk; ++j) {
-      mgb_out_column& oc = j < no ? out_results[j] : out_keys[j - no];
-      const uint8_t* valid = j < no ? t.out_valid[j] : t.key_valid[j - no];
-      if (!oc.null_mask) continue;
-      pack_validity_kernel<<<(unsigned)((words + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
-        valid, groups, oc.null_mask, words, t.null_counts + j);
+    // every column's copy and validity in one launch (per kEmitCols columns)
+    for (int j0 = 0; j0 < no + nk; j0 += kEmitCols) {
+      EmitParams ep{};
+      ep.rows = groups;
+      const int n = std::min(kEmitCols, no + nk - j0);
+      int64_t units = words;
+      for (int i = 0; i < n; ++i) {
+        const int j = j0 + i;
+        const mgb_out_column& oc = j < no ? out_results[j] : out_keys[j - no];
+        EmitCol& c = ep.cols[i];
+        c.src = j < no ? t.out_data[j] : t.key_data[j - no];
+        c.dst = static_cast<uint8_t*>(oc.data);
+        c.bytes = (int64_t)oc.data_bytes;
+        c.valid = j < no ? t.out_valid[j] : t.key_valid[j - no];
+        c.mask = oc.null_mask;
+        c.words = oc.null_mask ? words : 0;
+        c.nulls = t.null_counts + j;
+        units = std::max(units, (c.bytes + 15) / 16);
+      }
+      const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((units + kBlock - 1) / kBlock, 1024));
+      emit_columns_kernel<<<dim3(gx, (unsigned)n), kBlock, 0, stream>>>(ep);
     }
     MGB_HIP(hipGetLastError());
     MGB_HIP(hipMemcpyAsync(ws->pinned, t.null_counts, (size_t)(no + nk) * 8, hipMemcpyDeviceToHost, stream));
@@ -1058,7 +1162,7 @@ int64_t hll_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_
   st = MGB_SUCCESS;
   const int cus = device_cu_count();
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(2 * cus, (R + 65535) / 65536));
-  const int nred = kHllRegs / 256;
+  const int nred = kHllRegs / kHllReduceRegs;
   const size_t regs_bytes = align_up((size_t)nb * kHllRegs, 256);
   const size_t need = regs_bytes + align_up((size_t)nred * 8, 256) + align_up((size_t)nred * 4, 256);
   hipError_t e = ws->main.reserve(need, stream);
