@@ -81,20 +81,32 @@ __device__ __forceinline__ bool record_occupied(const uint8_t* table, int64_t r,
   return hdr != (key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef);
 }
 
+// table_groups (or nullptr): the table's group counter; an empty table is
+// not scanned (this kernel, scan_blocks_kernel and finalize_kernel).
 __global__ __launch_bounds__(kBlock) void count_occupied_kernel(const uint8_t* table, int64_t capacity,
                                                                 int32_t stride, int32_t key_mode,
-                                                                int64_t* block_counts)
+                                                                int64_t* block_counts,
+                                                                const unsigned long long* table_groups)
 {
+  if (table_groups && *table_groups == 0) {
+    if (threadIdx.x == 0) block_counts[blockIdx.x] = 0;
+    return;
+  }
   int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   int c = __syncthreads_count(record_occupied(table, r, capacity, stride, key_mode));
   if (threadIdx.x == 0) block_counts[blockIdx.x] = c;
 }
 
 // Exclusive scan of n block counts in place by one workgroup; total -> *total.
-__global__ __launch_bounds__(1024) void scan_blocks_kernel(int64_t* counts, int64_t n, unsigned long long* total)
+__global__ __launch_bounds__(1024) void scan_blocks_kernel(int64_t* counts, int64_t n, unsigned long long* total,
+                                                          const unsigned long long* table_groups)
 {
   __shared__ int64_t buf[1024];
   __shared__ int64_t carry;
+  if (table_groups && *table_groups == 0) {  // every count is 0
+    if (threadIdx.x == 0) *total = 0ull;
+    return;
+  }
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   for (int64_t base = 0; base < n; base += 1024) {
@@ -120,6 +132,7 @@ __global__ __launch_bounds__(1024) void scan_blocks_kernel(int64_t* counts, int6
 // writes keys and results.
 __global__ __launch_bounds__(kBlock) void finalize_kernel(const FinParams p)
 {
+  if (p.table_groups && *p.table_groups == 0) return;
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const bool occ = record_occupied(p.table, r, p.capacity, p.stride, p.key_mode);
   const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -297,24 +310,62 @@ constexpr int kHllBits = 14;
 constexpr int kHllRegs = 1 << kHllBits;
 
 // Per-workgroup registers: register = top 14 bits of the row hash, value =
-// leading zeros + 1 of a bijective remix of the hash.
-__global__ __launch_bounds__(kLdsBlock) void hll_block_kernel(const AggParams p, uint8_t* block_regs)
+// leading zeros + 1 of a bijective remix of the hash. The same pass counts
+// each chunk's rows per fine bucket (top kFineBits bits of the hash):
+// partitions of up to 2^kFineBits take their histogram from these counts
+// (hist_coarsen_kernel) instead of a second pass over the keys. Workgroup b
+// covers rows [b * chunk, +chunk), the chunks of the partitioned execution.
+constexpr int kFineBits = 12;
+constexpr int kFineBuckets = 1 << kFineBits;
+__global__ __launch_bounds__(kLdsBlock) void hll_hist_kernel(const AggParams p, uint8_t* block_regs, uint32_t* fine,
+                                                            int64_t chunk, int32_t nw)
 {
-  extern __shared__ uint32_t regs[];
-  for (int i = threadIdx.x; i < kHllRegs; i += kLdsBlock) regs[i] = 0;
+  extern __shared__ uint32_t regs[];  // kHllRegs registers, then kFineBuckets counts
+  uint32_t* counts = regs + kHllRegs;
+  for (int i = threadIdx.x; i < kHllRegs + kFineBuckets; i += kLdsBlock) regs[i] = 0;
   __syncthreads();
-  const int64_t step = (int64_t)gridDim.x * kLdsBlock;
-  for (int64_t row = (int64_t)blockIdx.x * kLdsBlock + threadIdx.x; row < p.num_rows; row += step) {
-    if (!key_row_valid(p, row)) continue;
-    const uint32_t h = row_key(p, row).hash;
-    const uint32_t rank = (uint32_t)__clz(fmix32(h ^ 0x9e3779b9u)) + 1u;
-    uint32_t* r = &regs[h >> (32 - kHllBits)];
-    // registers saturate quickly: most rows only read
-    if (rank > __hip_atomic_load(r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMax(r, rank);
+  const int64_t begin = (int64_t)blockIdx.x * chunk;
+  const int64_t end = begin + chunk < p.num_rows ? begin + chunk : p.num_rows;
+  constexpr int kItems = 4;  // rows per thread per step, hashed column by column
+  for (int64_t row0 = begin + threadIdx.x; row0 < end; row0 += (int64_t)kItems * kLdsBlock) {
+    bool present[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      present[k] = row0 + k * kLdsBlock < end;
+      if (present[k] && p.skip_null_keys) present[k] = key_row_valid(p, row0 + k * kLdsBlock);
+    }
+    uint32_t h[kItems];
+    uint64_t key0[kItems];
+    hash_rows(p, row0, kLdsBlock, present, h, key0);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      if (!present[k]) continue;
+      const uint32_t rank = (uint32_t)__clz(fmix32(h[k] ^ 0x9e3779b9u)) + 1u;
+      uint32_t* r = &regs[h[k] >> (32 - kHllBits)];
+      // registers saturate quickly: most rows only read
+      if (rank > __hip_atomic_load(r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMax(r, rank);
+      atomicAdd(&counts[h[k] >> (32 - kFineBits)], 1u);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kHllRegs; i += kLdsBlock)
     block_regs[(int64_t)blockIdx.x * kHllRegs + i] = (uint8_t)regs[i];
+  for (int i = threadIdx.x; i < kFineBuckets; i += kLdsBlock) fine[(int64_t)i * nw + blockIdx.x] = counts[i];
+}
+
+// Histogram of 2^log2p partitions (log2p <= kFineBits) from the fine bucket
+// counts: hist[p * nw + c] = sum of the partition's buckets in chunk c.
+__global__ __launch_bounds__(kBlock) void hist_coarsen_kernel(const uint32_t* fine, int32_t nw, int32_t log2p,
+                                                             uint32_t* hist)
+{
+  const int64_t n = ((int64_t)1 << log2p) * nw;
+  const int per = 1 << (kFineBits - log2p);
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t part = i / nw, c = i - part * nw;
+    uint32_t sum = 0;
+    for (int j = 0; j < per; ++j) sum += fine[(part * per + j) * nw + c];
+    hist[i] = sum;
+  }
 }
 
 // Max over workgroups per register, then per block the sum of 2^-M and the
@@ -699,6 +750,9 @@ struct Workspace {
   DeviceBuffer main;    // tables, partitioned columns, histograms, counters, sample table
   DeviceBuffer outs;    // temporary output columns
   DeviceBuffer params;  // kernel parameter blocks (output tables, identity templates)
+  DeviceBuffer fine;    // fine bucket counts per chunk of the last key pass (hll_hist_kernel)
+  bool fine_valid = false;  // `fine` holds the counts of this call's keys
+  int64_t fine_nw = 0, fine_chunk = 0;
   unsigned long long* pinned = nullptr;  // host readback area
   uint8_t* staging = nullptr;            // pinned source of parameter uploads
   int upload_slot = 0;                   // next staging / params slot of the current call
@@ -707,6 +761,7 @@ struct Workspace {
     if (main.ptr) (void)hipFree(main.ptr);
     if (outs.ptr) (void)hipFree(outs.ptr);
     if (params.ptr) (void)hipFree(params.ptr);
+    if (fine.ptr) (void)hipFree(fine.ptr);
     if (pinned) (void)hipHostFree(pinned);
     if (staging) (void)hipHostFree(staging);
   }
@@ -1155,20 +1210,32 @@ Path forced_path()
   return Path::Auto;
 }
 
+// Chunks (workgroups) of the histogram and scatter passes over R rows into
+// 2^log2p partitions: up to four per CU, fewer when the partitions are many
+// (the histogram holds 2^log2p counts per chunk), at least 4096 rows each.
+int64_t partition_chunks(int64_t R, int log2p, int cus)
+{
+  int64_t nw = std::min<int64_t>((int64_t)cus * 4, std::max<int64_t>(64, (1ll << 22) >> log2p));
+  return std::max<int64_t>(1, std::min<int64_t>(nw, (R + 4095) / 4096));
+}
+
 // HyperLogLog estimate of the number of distinct key rows (standard error
 // ~1.04/sqrt(2^14) = 0.8%), with linear counting for small cardinalities.
 int64_t hll_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_t stream, mgb_status& st)
 {
   st = MGB_SUCCESS;
   const int cus = device_cu_count();
-  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(2 * cus, (R + 65535) / 65536));
+  // one workgroup per chunk of the partitioned execution (partition_chunks)
+  const int64_t nb = partition_chunks(R, kFineBits, cus);
+  const int64_t chunk = (R + nb - 1) / nb;
   const int nred = kHllRegs / kHllReduceRegs;
   const size_t regs_bytes = align_up((size_t)nb * kHllRegs, 256);
   const size_t need = regs_bytes + align_up((size_t)nred * 8, 256) + align_up((size_t)nred * 4, 256);
+  const size_t lds = (size_t)(kHllRegs + kFineBuckets) * 4;
   hipError_t e = ws->main.reserve(need, stream);
+  if (e == hipSuccess) e = ws->fine.reserve((size_t)kFineBuckets * nb * 4, stream);
   if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)hll_block_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            kHllRegs * 4);
+    e = hipFuncSetAttribute((const void*)hll_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) {
     st = fail(MGB_ERROR_HIP, std::string("workspace: ") + hipGetErrorString(e));
     return R;
@@ -1178,9 +1245,13 @@ int64_t hll_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_
   uint32_t* zeros = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(sums) + align_up((size_t)nred * 8, 256));
   {
     ProfScope prof("estimate", stream);
-    hll_block_kernel<<<(unsigned)nb, kLdsBlock, kHllRegs * 4, stream>>>(base, regs);
-    hll_reduce_kernel<<<(unsigned)nred, 256, 0, stream>>>(regs, nb, sums, zeros);
+    hll_hist_kernel<<<(unsigned)nb, kLdsBlock, lds, stream>>>(base, regs, static_cast<uint32_t*>(ws->fine.ptr), chunk,
+                                                              (int32_t)nb);
+    hll_reduce_kernel<<<(unsigned)nred, 256, 0, stream>>>(regs, (int)nb, sums, zeros);
   }
+  ws->fine_valid = true;
+  ws->fine_nw = nb;
+  ws->fine_chunk = chunk;
   e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(ws->pinned, sums, (size_t)nred * 8, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess)
@@ -1357,8 +1428,8 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
 
     int64_t* block_counts = reinterpret_cast<int64_t*>(wsb + block_off);
     count_occupied_kernel<<<(unsigned)nblocks, kBlock, 0, stream>>>(wsb, capacity, plan.stride, plan.key_mode,
-                                                                     block_counts);
-    scan_blocks_kernel<<<1, 1024, 0, stream>>>(block_counts, nblocks, counters + 2);
+                                                                     block_counts, nullptr);
+    scan_blocks_kernel<<<1, 1024, 0, stream>>>(block_counts, nblocks, counters + 2, nullptr);
     MGB_HIP(hipGetLastError());
     MGB_HIP(hipMemcpyAsync(ws->pinned, counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     MGB_HIP(hipStreamSynchronize(stream));
@@ -1426,8 +1497,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     const int64_t item_target = std::max<int64_t>(65536, (R + 2 * (int64_t)cus - 1) / (2 * (int64_t)cus));
     while (log2p < 12 && (R >> log2p) > item_target && (est >> (log2p + 1)) >= 64) ++log2p;
     const int64_t P = 1ll << log2p;
-    int64_t nw = std::min<int64_t>((int64_t)cus * 4, std::max<int64_t>(64, (1ll << 22) / P));
-    nw = std::max<int64_t>(1, std::min<int64_t>(nw, (R + 4095) / 4096));
+    const int64_t nw = partition_chunks(R, log2p, cus);
     const int64_t chunk = (R + nw - 1) / nw;
     const int64_t nh = P * nw;
     const int64_t nb = (nh + kScanTile - 1) / kScanTile;
@@ -1520,7 +1590,13 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
                                 (int)part_lds));
     MGB_HIP(hipFuncSetAttribute((const void*)part_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)part_lds));
-    {
+    if (ws->fine_valid && log2p <= kFineBits && ws->fine_nw == nw && ws->fine_chunk == chunk) {
+      // the estimate's pass counted the rows per fine bucket of these chunks
+      ProfScope prof("partition_hist", stream);
+      const unsigned grid = (unsigned)std::min<int64_t>((nh + kBlock - 1) / kBlock, (int64_t)cus * 8);
+      hist_coarsen_kernel<<<grid, kBlock, 0, stream>>>(static_cast<const uint32_t*>(ws->fine.ptr), (int32_t)nw, log2p,
+                                                      pp.hist);
+    } else {
       ProfScope prof("partition_hist", stream);
       part_hist_kernel<<<(unsigned)nw, kBlock, part_lds, stream>>>(pp);
     }
@@ -1623,10 +1699,12 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
 
     // finalise the global table's groups after the items' rows
     int64_t* gblk = reinterpret_cast<int64_t*>(wsb + gblk_off);
-    count_occupied_kernel<<<(unsigned)gblocks, kBlock, 0, stream>>>(gtab, gcap, plan.stride, plan.key_mode, gblk);
-    scan_blocks_kernel<<<1, 1024, 0, stream>>>(gblk, gblocks, ctr + 6);
+    count_occupied_kernel<<<(unsigned)gblocks, kBlock, 0, stream>>>(gtab, gcap, plan.stride, plan.key_mode, gblk,
+                                                                     ctr + 4);
+    scan_blocks_kernel<<<1, 1024, 0, stream>>>(gblk, gblocks, ctr + 6, ctr + 4);
     FinParams f{};
     f.table = gtab;
+    f.table_groups = ctr + 4;
     f.capacity = gcap;
     f.stride = plan.stride;
     f.key_mode = plan.key_mode;
@@ -1677,6 +1755,7 @@ mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mg
   for (size_t c = 0; c < plan.keys.size(); ++c) base.keys[c] = plan.keys[c];
 
   mgb_status st = MGB_SUCCESS;
+  ws->fine_valid = false;
   const int64_t G_est = sample_estimate(ws, base, R, stream, st);
   if (st != MGB_SUCCESS) return st;
 

@@ -80,6 +80,7 @@ struct FinParams {
   int32_t pad;
   const int64_t* block_offsets;   // exclusive prefix of occupied records per block
   const unsigned long long* pos_base;  // added to every output row, or nullptr
+  const unsigned long long* table_groups;  // the table's group counter (0: nothing to write), or nullptr
   const OutTable* out;            // device memory
   KeyCol keys[kMaxKeys];          // representative-row gather source
 };
