@@ -369,21 +369,21 @@ __global__ __launch_bounds__(kBlock) void hist_coarsen_kernel(const uint32_t* fi
 }
 
 // Max over workgroups per register, then per block the sum of 2^-M and the
-// number of empty registers. A workgroup of 256 threads covers 128 registers
-// (32 words of 4 byte-registers): its 8 groups of 32 threads each take every
-// 8th workgroup's words, and the 8 partial maxima meet in LDS.
+// number of empty registers. A workgroup of 1024 threads covers 128 registers
+// (32 words of 4 byte-registers): its 32 groups of 32 threads each take every
+// 32nd workgroup's words, and the 32 partial maxima meet in LDS.
 constexpr int kHllReduceRegs = 128;
-__global__ __launch_bounds__(256) void hll_reduce_kernel(const uint8_t* block_regs, int nb, double* sums,
-                                                        uint32_t* zeros)
+__global__ __launch_bounds__(1024) void hll_reduce_kernel(const uint8_t* block_regs, int nb, double* sums,
+                                                         uint32_t* zeros)
 {
-  __shared__ uint32_t part[8][32][4];
+  __shared__ uint32_t part[32][32][4];
   __shared__ double s[32];
   __shared__ uint32_t z[32];
   const int wi = threadIdx.x & 31, sl = threadIdx.x >> 5;
   const int64_t word = (int64_t)blockIdx.x * 32 + wi;  // u32 word of the register array
   const global_t<const uint32_t>* w32 = to_global(reinterpret_cast<const uint32_t*>(block_regs));
   uint32_t m[4] = {0u, 0u, 0u, 0u};
-  for (int b = sl; b < nb; b += 8) {
+  for (int b = sl; b < nb; b += 32) {
     const uint32_t x = w32[(int64_t)b * (kHllRegs / 4) + word];
 #pragma unroll
     for (int k = 0; k < 4; ++k) m[k] = max(m[k], (x >> (8 * k)) & 0xffu);
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void hll_reduce_kernel(const uint8_t* block_re
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       uint32_t r = 0;
-      for (int j = 0; j < 8; ++j) r = max(r, part[j][wi][k]);
+      for (int j = 0; j < 32; ++j) r = max(r, part[j][wi][k]);
       sum += ldexp(1.0, -(int)r);
       zero += r == 0;
     }
@@ -1133,6 +1133,29 @@ mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int6
   return MGB_SUCCESS;
 }
 
+// The LDS path's view of a column's ops (ValCol::sum_off ... lds_generic).
+void column_program(ValCol& vc, const std::vector<std::pair<int, int>>& ops)
+{
+  vc.sum_off = vc.min_off = vc.max_off = -1;
+  vc.mm_kind = -1;
+  vc.lds_generic = 0;
+  for (auto& o : ops) {
+    switch (o.first) {
+      case OP_SUM_I64: case OP_SUM_F64: vc.sum_off = o.second; break;
+      case OP_MIN_I32: case OP_MIN_U32: case OP_MIN_I64: case OP_MIN_U64: case OP_MIN_F32: case OP_MIN_F64:
+        vc.min_off = o.second;
+        vc.mm_kind = o.first;
+        break;
+      case OP_MAX_I32: case OP_MAX_U32: case OP_MAX_I64: case OP_MAX_U64: case OP_MAX_F32: case OP_MAX_F64:
+        vc.max_off = o.second;
+        vc.mm_kind = o.first - 1;  // the matching MIN kind
+        break;
+      case OP_NULLS: break;
+      default: vc.lds_generic = 1; break;
+    }
+  }
+}
+
 // Fills AggParams' value columns and ops from the plan; false when they do
 // not fit one launch.
 bool single_launch_ops(const Plan& plan, AggParams& ap)
@@ -1154,6 +1177,7 @@ bool single_launch_ops(const Plan& plan, AggParams& ap)
     vc.op_begin = ap.num_ops;
     for (auto& o : pc.ops) ap.ops[ap.num_ops++] = OpDesc{o.first, o.second};
     vc.op_end = ap.num_ops;
+    column_program(vc, pc.ops);
     ap.vals[ap.num_vals++] = vc;
   }
   return true;
@@ -1247,7 +1271,7 @@ int64_t hll_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_
     ProfScope prof("estimate", stream);
     hll_hist_kernel<<<(unsigned)nb, kLdsBlock, lds, stream>>>(base, regs, static_cast<uint32_t*>(ws->fine.ptr), chunk,
                                                               (int32_t)nb);
-    hll_reduce_kernel<<<(unsigned)nred, 256, 0, stream>>>(regs, (int)nb, sums, zeros);
+    hll_reduce_kernel<<<(unsigned)nred, 1024, 0, stream>>>(regs, (int)nb, sums, zeros);
   }
   ws->fine_valid = true;
   ws->fine_nw = nb;
@@ -1411,6 +1435,7 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
           vc.op_begin = lp.num_ops;
           for (auto& o : pc.ops) lp.ops[lp.num_ops++] = OpDesc{o.first, o.second};
           vc.op_end = lp.num_ops;
+          column_program(vc, pc.ops);
           lp.vals[lp.num_vals++] = vc;
           ++ci;
         }

@@ -653,6 +653,85 @@ __device__ __forceinline__ void lds_op_rows(uint8_t* lds, const uint32_t (&loff)
   }
 }
 
+// SUM, MIN and MAX of one column (ValCol::sum_off ...) on the LDS records of
+// N rows: one dispatch on the column's class for all three, the min/max key
+// (order-preserving code for floats) computed once for both.
+template <int N>
+__device__ __forceinline__ void lds_column_program(uint8_t* lds, const ValCol& v, const uint32_t (&loff)[N],
+                                                   const uint64_t (&x)[N], const bool (&ok)[N], uint32_t dum)
+{
+  if (v.sum_off >= 0) {
+    if (v.vclass == VC_F32 || v.vclass == VC_F64) {
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        lds_add(reinterpret_cast<double*>(lds + (ok[u] ? loff[u] + v.sum_off : dum)), canon_f64(x[u], v.vclass));
+    } else {
+#pragma unroll
+      for (int u = 0; u < N; ++u)
+        lds_add(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + v.sum_off : dum)),
+                (unsigned long long)x[u]);
+    }
+  }
+  if (v.mm_kind < 0) return;
+  const bool has_min = v.min_off >= 0, has_max = v.max_off >= 0;
+  switch (v.mm_kind) {
+    case OP_MIN_I32:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        if (has_min) lds_min(reinterpret_cast<int32_t*>(lds + (ok[u] ? loff[u] + v.min_off : dum)), (int32_t)x[u]);
+        if (has_max) lds_max(reinterpret_cast<int32_t*>(lds + (ok[u] ? loff[u] + v.max_off : dum)), (int32_t)x[u]);
+      }
+      break;
+    case OP_MIN_U32:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        if (has_min) lds_min(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + v.min_off : dum)), (uint32_t)x[u]);
+        if (has_max) lds_max(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + v.max_off : dum)), (uint32_t)x[u]);
+      }
+      break;
+    case OP_MIN_I64:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        if (has_min) lds_min(reinterpret_cast<long long*>(lds + (ok[u] ? loff[u] + v.min_off : dum)), (long long)x[u]);
+        if (has_max) lds_max(reinterpret_cast<long long*>(lds + (ok[u] ? loff[u] + v.max_off : dum)), (long long)x[u]);
+      }
+      break;
+    case OP_MIN_U64:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        if (has_min)
+          lds_min(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + v.min_off : dum)),
+                  (unsigned long long)x[u]);
+        if (has_max)
+          lds_max(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + v.max_off : dum)),
+                  (unsigned long long)x[u]);
+      }
+      break;
+    case OP_MIN_F32:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        const uint32_t b = (uint32_t)x[u];
+        const bool nan = (b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu);
+        const uint32_t code = f32_code(b);
+        if (has_min) lds_min(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + v.min_off : dum)), nan ? kF32MinNaN : code);
+        if (has_max) lds_max(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + v.max_off : dum)), nan ? kF32MaxNaN : code);
+      }
+      break;
+    default:  // OP_MIN_F64
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        const unsigned long long b = x[u];
+        const bool nan = (b & 0x7ff0000000000000ull) == 0x7ff0000000000000ull && (b & 0x000fffffffffffffull);
+        const unsigned long long code = f64_code(b);
+        if (has_min)
+          lds_min(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + v.min_off : dum)), nan ? kF64MinNaN : code);
+        if (has_max)
+          lds_max(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + v.max_off : dum)), nan ? kF64MaxNaN : code);
+      }
+      break;
+  }
+}
+
 // All ops of one value column on the LDS records of N rows. Validity comes
 // from the column's null mask (input rows) or from bit `flag_bit` of the
 // row's flags word (partitioned records).
@@ -673,6 +752,10 @@ __device__ __forceinline__ void lds_column_rows(const AggParams& p, uint8_t* lds
 #pragma unroll
     for (int u = 0; u < N; ++u)
       lds_add(reinterpret_cast<uint32_t*>(lds + (loff[u] < kRecSpill && !ok[u] ? loff[u] + v.nulls_off : dum)), 1u);
+  }
+  if (!v.lds_generic) {
+    lds_column_program(lds, v, loff, x, ok, dum);
+    return;
   }
   // the column's op descriptors are read together, before any update: a
   // scalar load inside the op loop would wait for every LDS update issued

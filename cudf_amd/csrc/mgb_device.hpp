@@ -68,7 +68,15 @@ struct ValCol {
   int32_t flag_bit;         // validity from bit `flag_bit` of the row's flags word, or -1
   int32_t nulls_off;        // OP_NULLS field of this column, or -1
   int32_t stride;           // bytes from one row to the next (width for a plain column)
-  int32_t pad;
+  // the ops as the LDS path applies them (column_program): SUM, MIN and MAX
+  // fields (-1: none), the MIN op kind naming the class of both, and 1 when
+  // the column has other ops (PRODUCT, SUM_OF_SQUARES), which take the
+  // general op-by-op path
+  int32_t sum_off;
+  int32_t min_off;
+  int32_t max_off;
+  int32_t mm_kind;
+  int32_t lds_generic;
 };
 
 struct OpDesc {
