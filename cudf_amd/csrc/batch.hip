@@ -956,6 +956,7 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
         ra.vals[c].mask = nullptr;
         ra.vals[c].bit_off = 0;
         ra.vals[c].flag_bit = pp.src.vals[c].flag_bit;
+        if (c < kProgramCols) ra.prog[c] = pack_program(ra.vals[c], voff[c]);
       }
     }
   }

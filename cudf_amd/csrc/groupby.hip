@@ -1180,6 +1180,9 @@ bool single_launch_ops(const Plan& plan, AggParams& ap)
     column_program(vc, pc.ops);
     ap.vals[ap.num_vals++] = vc;
   }
+  ap.program_cols = ap.num_vals <= kProgramCols;
+  for (int c = 0; c < ap.num_vals; ++c)
+    if (ap.vals[c].lds_generic) ap.program_cols = 0;
   return true;
 }
 
@@ -1700,6 +1703,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       lp.src.vals[c].stride = rec_bytes;
       lp.src.vals[c].mask = nullptr;
       lp.src.vals[c].bit_off = 0;
+      if (c < kProgramCols) lp.src.prog[c] = pack_program(lp.src.vals[c], voff[c]);
     }
     lp.offsets = pp.offsets;
     lp.total_rows = ctr + 0;

@@ -653,11 +653,44 @@ __device__ __forceinline__ void lds_op_rows(uint8_t* lds, const uint32_t (&loff)
   }
 }
 
+// One value column's program for records, packed into 64 bits: LDS field
+// offsets + 1 (0: none) of SUM (bits 0-9), MIN (10-19), MAX (20-29) and the
+// null count (30-39), the MIN op kind (40-44), flag bit + 1 (45-49), value
+// class (50-52), width (53-56) and the value's byte offset in the record
+// (57-62).
+__host__ __device__ inline uint64_t pack_program(const ValCol& v, int rec_off)
+{
+  auto f10 = [](int off) { return (uint64_t)(off + 1) & 1023u; };
+  return f10(v.sum_off) | f10(v.min_off) << 10 | f10(v.max_off) << 20 | f10(v.nulls_off) << 30 |
+         (uint64_t)(v.mm_kind & 31) << 40 | (uint64_t)((v.flag_bit + 1) & 31) << 45 | (uint64_t)(v.vclass & 7) << 50 |
+         (uint64_t)(v.width & 15) << 53 | (uint64_t)(rec_off & 63) << 57;
+}
+
+// A packed program's fields.
+struct Program {
+  int sum_off, min_off, max_off, nulls_off, mm_kind, flag_bit, vclass, width, rec_off;
+};
+__device__ __forceinline__ Program unpack_program(uint64_t pg)
+{
+  Program g;
+  g.sum_off = (int)(pg & 1023u) - 1;
+  g.min_off = (int)((pg >> 10) & 1023u) - 1;
+  g.max_off = (int)((pg >> 20) & 1023u) - 1;
+  g.nulls_off = (int)((pg >> 30) & 1023u) - 1;
+  const int mm = (int)((pg >> 40) & 31u);
+  g.mm_kind = mm == 31 ? -1 : mm;
+  g.flag_bit = (int)((pg >> 45) & 31u) - 1;
+  g.vclass = (int)((pg >> 50) & 7u);
+  g.width = (int)((pg >> 53) & 15u);
+  g.rec_off = (int)((pg >> 57) & 63u);
+  return g;
+}
+
 // SUM, MIN and MAX of one column (ValCol::sum_off ...) on the LDS records of
 // N rows: one dispatch on the column's class for all three, the min/max key
 // (order-preserving code for floats) computed once for both.
-template <int N>
-__device__ __forceinline__ void lds_column_program(uint8_t* lds, const ValCol& v, const uint32_t (&loff)[N],
+template <int N, typename V>
+__device__ __forceinline__ void lds_column_program(uint8_t* lds, const V& v, const uint32_t (&loff)[N],
                                                    const uint64_t (&x)[N], const bool (&ok)[N], uint32_t dum)
 {
   if (v.sum_off >= 0) {

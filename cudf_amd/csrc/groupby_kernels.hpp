@@ -28,7 +28,11 @@ struct AggParams {
   // rec_base + r * 8 * rec_words, read whole (the key, value and flags
   // columns point into it); 0: rows are read column by column
   int32_t rec_words = 0;
-  int32_t rec_pad = 0;
+  int32_t program_cols = 0;       // 1: at most kProgramCols value columns, none with ops beyond SUM/MIN/MAX
+  // record mode with program_cols: each value column's program in one word
+  // (pack_program), so the row loop keeps a few registers per column
+  uint64_t prog[kProgramCols] = {};
+
   const uint8_t* rec_base = nullptr;
   KeyCol keys[kMaxKeys];
   ValCol vals[kMaxVals];

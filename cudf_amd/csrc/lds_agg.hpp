@@ -71,26 +71,78 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
   constexpr int kPreCols = kRec ? 0 : 3;  // the switch below picks pre[0..2]
   const int64_t step = kLdsBlock;
   const KeyCol& k0 = p.keys[0];
-  const bool int_key = p.key_mode == KEY_INT;
   // the thread's dummy word for updates of rows that have none: in the scan
   // space after the table, unused until the groups are written out
   const uint32_t dum = (uint32_t)(reinterpret_cast<uint8_t*>(t.count + 4) - lds) + (threadIdx.x & 63u) * 8u;
+  // Records: the uniform values of the row loop are held in vector
+  // registers and made scalar at each use (v_readfirstlane). Held in scalar
+  // registers, the compiler reloads them from memory when it runs short of
+  // those, and each such load waits for every LDS update in flight.
+  uint32_t pin_a = 0, pin_b = 0, pin_c = 0;
+  uint64_t pin_base = 0;
+  uint64_t pin_prog[kRec ? kProgramCols : 1] = {};
+  if constexpr (kRec) {
+    const int has_flags = p.flags != nullptr;
+    pin_a = (uint32_t)(k0.width & 15) | (uint32_t)(k0.vclass & 7) << 4 |
+            (uint32_t)((k0.data - p.rec_base) & 63) << 7 |
+            (uint32_t)(has_flags ? (p.flags - p.rec_base) & 63 : 0) << 13 | (uint32_t)(p.flag_bytes & 7) << 19 |
+            (uint32_t)has_flags << 22 | (uint32_t)(p.key_mode == KEY_INT) << 23 | (uint32_t)(p.rec_words & 7) << 24;
+    pin_b = (uint32_t)t.slots | (uint32_t)t.stride << 16;
+    pin_c = (uint32_t)((p.rows_op >= 0 ? p.ops[p.rows_op].off : -1) + 1) | (uint32_t)(p.num_vals & 31) << 10 |
+            (uint32_t)p.program_cols << 15;
+    pin_base = (uint64_t)p.rec_base;
+    asm volatile("" : "+v"(pin_a), "+v"(pin_b), "+v"(pin_c), "+v"(pin_base));
+#pragma unroll
+    for (int c = 0; c < kProgramCols; ++c) {
+      pin_prog[c] = p.prog[c];
+      asm volatile("" : "+v"(pin_prog[c]));
+    }
+  }
   for (int64_t base = begin + threadIdx.x; base < end; base += (int64_t)kRows * kLdsBlock) {
     bool present[kRows];
 #pragma unroll
     for (int u = 0; u < kRows; ++u) present[u] = base + u * step < end;
+    // this iteration's uniform values
+    int kw, kvc, slots, stride, rows_off, nvals;
+    bool int_key, prog_cols;
+    if constexpr (kRec) {
+      const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_a);
+      const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_b);
+      const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_c);
+      kw = (int)(a & 15);
+      kvc = (int)((a >> 4) & 7);
+      int_key = (a >> 23) & 1;
+      slots = (int)(b & 0xffff);
+      stride = (int)(b >> 16);
+      rows_off = (int)(c & 1023) - 1;
+      nvals = (int)((c >> 10) & 31);
+      prog_cols = (c >> 15) & 1;
+    } else {
+      kw = k0.width;
+      kvc = k0.vclass;
+      int_key = p.key_mode == KEY_INT;
+      slots = t.slots;
+      stride = t.stride;
+      rows_off = p.rows_op >= 0 ? p.ops[p.rows_op].off : -1;
+      nvals = p.num_vals;
+      prog_cols = p.program_cols;
+    }
     uint64_t key[kRows];
     uint32_t flags[kRows];
     // records: every field of the rows comes from registers (rw)
     RecWords rw[kRec ? kRows : 1];
     uint64_t pre[kRec ? 1 : kPreCols][kRows];
     if constexpr (kRec) {
-      load_recs(p.rec_base, p.rec_words, base, step, present, rw);
-      if (int_key) extract_rows(rw, (int)(k0.data - p.rec_base), k0.width, VC_UNSIGNED, key);
+      const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_a);
+      const uint64_t rb = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pin_base) |
+                          (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pin_base >> 32)) << 32;
+      load_recs(reinterpret_cast<const uint8_t*>(rb), (int)((a >> 24) & 7), base, step, present, rw);
+      if (int_key) extract_rows(rw, (int)((a >> 7) & 63), kw, VC_UNSIGNED, key);
+      const bool has_flags = (a >> 22) & 1;
       uint64_t f[kRows];
-      if (p.flags) extract_rows(rw, (int)(p.flags - p.rec_base), p.flag_bytes, VC_UNSIGNED, f);
+      if (has_flags) extract_rows(rw, (int)((a >> 13) & 63), (int)((a >> 19) & 7), VC_UNSIGNED, f);
 #pragma unroll
-      for (int u = 0; u < kRows; ++u) flags[u] = p.flags ? (uint32_t)f[u] : 0u;
+      for (int u = 0; u < kRows; ++u) flags[u] = has_flags ? (uint32_t)f[u] : 0u;
     } else {
       if (int_key) load_rows(k0.data, k0.width, k0.stride, VC_UNSIGNED, base, step, present, key);
       {
@@ -121,24 +173,24 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
       uint32_t i0[kRows], i1[kRows];
 #pragma unroll
       for (int u = 0; u < kRows; ++u) {
-        kk[u] = (uint64_t)widen_int(key[u], k0.width, k0.vclass);
-        const uint32_t hsh = hash_combine(0u, murmur3_bits(normalize_float_bits(key[u], k0.vclass), k0.width, 0));
-        i0[u] = (uint32_t)reduce_range(hsh << log2p, t.slots);
-        i1[u] = i0[u] + 1 == (uint32_t)t.slots ? 0u : i0[u] + 1;
+        kk[u] = (uint64_t)widen_int(key[u], kw, kvc);
+        const uint32_t hsh = hash_combine(0u, murmur3_bits(normalize_float_bits(key[u], kvc), kw, 0));
+        i0[u] = (uint32_t)reduce_range(hsh << log2p, slots);
+        i1[u] = i0[u] + 1 == (uint32_t)slots ? 0u : i0[u] + 1;
         h0[u] = h1[u] = kEmptyIntKey;
         if (present[u]) {
-          h0[u] = __hip_atomic_load(reinterpret_cast<unsigned long long*>(lds_record(t, i0[u])), __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-          h1[u] = __hip_atomic_load(reinterpret_cast<unsigned long long*>(lds_record(t, i1[u])), __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+          h0[u] = __hip_atomic_load(reinterpret_cast<unsigned long long*>(t.rec + i0[u] * (uint32_t)stride),
+                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          h1[u] = __hip_atomic_load(reinterpret_cast<unsigned long long*>(t.rec + i1[u] * (uint32_t)stride),
+                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       }
 #pragma unroll
       for (int u = 0; u < kRows; ++u) {
         loff[u] = kRecNone;
         if (!present[u]) continue;
-        if (kk[u] != kEmptyIntKey && h0[u] == kk[u]) loff[u] = i0[u] * (uint32_t)t.stride;
-        else if (kk[u] != kEmptyIntKey && h0[u] != kEmptyIntKey && h1[u] == kk[u]) loff[u] = i1[u] * (uint32_t)t.stride;
+        if (kk[u] != kEmptyIntKey && h0[u] == kk[u]) loff[u] = i0[u] * (uint32_t)stride;
+        else if (kk[u] != kEmptyIntKey && h0[u] != kEmptyIntKey && h1[u] == kk[u]) loff[u] = i1[u] * (uint32_t)stride;
         else miss |= 1u << u;
       }
     } else {
@@ -162,16 +214,45 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
     }
 
     // updates
-    if (p.rows_op >= 0) {
-      const int off = p.ops[p.rows_op].off;
+    if (rows_off >= 0) {
 #pragma unroll
       for (int u = 0; u < kRows; ++u)
-        lds_add(reinterpret_cast<uint32_t*>(lds + (loff[u] < kRecSpill ? loff[u] + off : dum)), 1u);
+        lds_add(reinterpret_cast<uint32_t*>(lds + (loff[u] < kRecSpill ? loff[u] + rows_off : dum)), 1u);
+    }
+    // records whose columns all take the column program: the column loop
+    // unrolled, so that no column's descriptor is loaded inside the row loop
+    // (a scalar load there would wait for every LDS update issued before it)
+    bool unrolled = false;
+    if constexpr (kRec) {
+      if (prog_cols) {
+        unrolled = true;
+#pragma unroll
+        for (int c = 0; c < kProgramCols; ++c) {
+          if (c >= nvals) break;
+          const uint64_t pg = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pin_prog[c]) |
+                              (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pin_prog[c] >> 32))
+                                  << 32;
+          const Program v = unpack_program(pg);
+          uint64_t x[kRows];
+          extract_rows(rw, v.rec_off, v.width, v.vclass, x);
+          bool ok[kRows];
+#pragma unroll
+          for (int u = 0; u < kRows; ++u)
+            ok[u] = loff[u] < kRecSpill && (v.flag_bit < 0 || ((flags[u] >> v.flag_bit) & 1u));
+          if (v.nulls_off >= 0) {
+#pragma unroll
+            for (int u = 0; u < kRows; ++u)
+              lds_add(reinterpret_cast<uint32_t*>(lds + (loff[u] < kRecSpill && !ok[u] ? loff[u] + v.nulls_off : dum)),
+                      1u);
+          }
+          lds_column_program(lds, v, loff, x, ok, dum);
+        }
+      }
     }
     // one call site for all columns keeps the kernel's code small; the
     // preloaded values are picked by a uniform switch
 #pragma unroll 1
-    for (int c = 0; c < p.num_vals; ++c) {
+    for (int c = 0; c < (unrolled ? 0 : p.num_vals); ++c) {
       uint64_t x[kRows];
       if constexpr (kRec) {
         extract_rows(rw, (int)(p.vals[c].data - p.rec_base), p.vals[c].width, p.vals[c].vclass, x);

@@ -15,6 +15,7 @@ constexpr int kMaxKeys = 8;     // key columns handled per launch
 constexpr int kMaxVals = 16;    // value columns per aggregate launch
 constexpr int kMaxOps = 48;     // accumulator ops per aggregate launch
 constexpr int kMaxColOps = 8;   // accumulator ops of one value column (distinct kinds: at most 7)
+constexpr int kProgramCols = 4; // value columns the LDS row loop handles unrolled
 constexpr int kBlock = 256;     // threads per workgroup (4 wave64)
 
 // Header of an empty record. Single integer keys store the key itself in the
