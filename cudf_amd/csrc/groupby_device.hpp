@@ -397,46 +397,73 @@ typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 // memory; on a vector it is a register select.
 typedef uint64_t RecWords __attribute__((ext_vector_type(4)));
 
-// Whole records row0 + u * step (u < N, where present[u]) of `words` (<= 4)
-// 8-byte words each into rw[u]; absent rows and words past the record read 0.
+// Whole records row0 + u * step (u < N) of `words` (<= 4) 8-byte words each
+// into rw[u]; words past the record read 0. Rows past the range (not
+// present) load the last present row instead of branching around the load:
+// `last` is a row known to be in range.
 template <int N>
 __device__ __forceinline__ void load_recs(const uint8_t* base, int words, int64_t row0, int64_t step,
-                                          const bool (&present)[N], RecWords (&rw)[N])
+                                          const bool (&present)[N], int64_t last, RecWords (&rw)[N])
 {
   if (words == 4) {
     // 32-byte records of a 16-byte aligned array: two 16-byte loads each
     const global_t<const u64x2>* g = to_global(reinterpret_cast<const u64x2*>(base));
 #pragma unroll
     for (int u = 0; u < N; ++u) {
-      u64x2 a = {0ull, 0ull}, b = {0ull, 0ull};
-      if (present[u]) {
-        a = g[(row0 + u * step) * 2];
-        b = g[(row0 + u * step) * 2 + 1];
-      }
+      const int64_t r = present[u] ? row0 + u * step : last;
+      const u64x2 a = g[r * 2], b = g[r * 2 + 1];
       rw[u] = __builtin_shufflevector(a, b, 0, 1, 2, 3);
     }
   } else {
     const global_t<const uint64_t>* g = to_global(reinterpret_cast<const uint64_t*>(base));
 #pragma unroll
     for (int u = 0; u < N; ++u) {
-      const global_t<const uint64_t>* r = g + (row0 + u * step) * words;
-      const bool ok = present[u];
-      const uint64_t w0 = ok ? r[0] : 0ull;
-      const uint64_t w1 = ok && words > 1 ? r[1] : 0ull;
-      const uint64_t w2 = ok && words > 2 ? r[2] : 0ull;
+      const global_t<const uint64_t>* r = g + (present[u] ? row0 + u * step : last) * words;
+      const uint64_t w0 = r[0];
+      const uint64_t w1 = words > 1 ? r[1] : 0ull;
+      const uint64_t w2 = words > 2 ? r[2] : 0ull;
       rw[u] = RecWords{w0, w1, w2, 0ull};
     }
   }
 }
 
 // The field at byte `off` (naturally aligned, so inside one 8-byte word) of
-// records rw[u], in canonical form.
+// records rw[u], in canonical form. One dispatch on the width for all rows.
 template <int N>
 __device__ __forceinline__ void extract_rows(const RecWords (&rw)[N], int off, int width, int vclass, uint64_t (&x)[N])
 {
   const int wi = (off >> 3) & 3, sh = (off & 7) * 8;
+  uint64_t w[N];
 #pragma unroll
-  for (int u = 0; u < N; ++u) x[u] = widen_raw(rw[u][wi] >> sh, width, vclass);
+  for (int u = 0; u < N; ++u) w[u] = rw[u][wi] >> sh;
+  const bool sgn = vclass == VC_SIGNED;
+  switch (width) {
+    case 1:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        const uint8_t b = (uint8_t)w[u];
+        x[u] = sgn ? (uint64_t)(int64_t)(int8_t)b : vclass == VC_BOOL ? (uint64_t)(b != 0) : (uint64_t)b;
+      }
+      break;
+    case 2:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        const uint16_t b = (uint16_t)w[u];
+        x[u] = sgn ? (uint64_t)(int64_t)(int16_t)b : (uint64_t)b;
+      }
+      break;
+    case 4:
+#pragma unroll
+      for (int u = 0; u < N; ++u) {
+        const uint32_t b = (uint32_t)w[u];
+        x[u] = sgn ? (uint64_t)(int64_t)(int32_t)b : (uint64_t)b;
+      }
+      break;
+    default:
+#pragma unroll
+      for (int u = 0; u < N; ++u) x[u] = w[u];
+      break;
+  }
 }
 
 template <int N>
@@ -706,62 +733,97 @@ __device__ __forceinline__ void lds_column_program(uint8_t* lds, const V& v, con
     }
   }
   if (v.mm_kind < 0) return;
-  const bool has_min = v.min_off >= 0, has_max = v.max_off >= 0;
+  // the min/max key of every row, then the MIN and the MAX updates
+  uint64_t k[N];
   switch (v.mm_kind) {
-    case OP_MIN_I32:
-#pragma unroll
-      for (int u = 0; u < N; ++u) {
-        if (has_min) lds_min(reinterpret_cast<int32_t*>(lds + (ok[u] ? loff[u] + v.min_off : dum)), (int32_t)x[u]);
-        if (has_max) lds_max(reinterpret_cast<int32_t*>(lds + (ok[u] ? loff[u] + v.max_off : dum)), (int32_t)x[u]);
-      }
-      break;
-    case OP_MIN_U32:
-#pragma unroll
-      for (int u = 0; u < N; ++u) {
-        if (has_min) lds_min(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + v.min_off : dum)), (uint32_t)x[u]);
-        if (has_max) lds_max(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + v.max_off : dum)), (uint32_t)x[u]);
-      }
-      break;
-    case OP_MIN_I64:
-#pragma unroll
-      for (int u = 0; u < N; ++u) {
-        if (has_min) lds_min(reinterpret_cast<long long*>(lds + (ok[u] ? loff[u] + v.min_off : dum)), (long long)x[u]);
-        if (has_max) lds_max(reinterpret_cast<long long*>(lds + (ok[u] ? loff[u] + v.max_off : dum)), (long long)x[u]);
-      }
-      break;
-    case OP_MIN_U64:
-#pragma unroll
-      for (int u = 0; u < N; ++u) {
-        if (has_min)
-          lds_min(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + v.min_off : dum)),
-                  (unsigned long long)x[u]);
-        if (has_max)
-          lds_max(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + v.max_off : dum)),
-                  (unsigned long long)x[u]);
-      }
-      break;
     case OP_MIN_F32:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
         const uint32_t b = (uint32_t)x[u];
-        const bool nan = (b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu);
-        const uint32_t code = f32_code(b);
-        if (has_min) lds_min(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + v.min_off : dum)), nan ? kF32MinNaN : code);
-        if (has_max) lds_max(reinterpret_cast<uint32_t*>(lds + (ok[u] ? loff[u] + v.max_off : dum)), nan ? kF32MaxNaN : code);
+        k[u] = f32_code(b);
       }
       break;
-    default:  // OP_MIN_F64
+    case OP_MIN_F64:
 #pragma unroll
-      for (int u = 0; u < N; ++u) {
-        const unsigned long long b = x[u];
-        const bool nan = (b & 0x7ff0000000000000ull) == 0x7ff0000000000000ull && (b & 0x000fffffffffffffull);
-        const unsigned long long code = f64_code(b);
-        if (has_min)
-          lds_min(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + v.min_off : dum)), nan ? kF64MinNaN : code);
-        if (has_max)
-          lds_max(reinterpret_cast<unsigned long long*>(lds + (ok[u] ? loff[u] + v.max_off : dum)), nan ? kF64MaxNaN : code);
-      }
+      for (int u = 0; u < N; ++u) k[u] = f64_code(x[u]);
       break;
+    default:
+#pragma unroll
+      for (int u = 0; u < N; ++u) k[u] = x[u];
+      break;
+  }
+  for (int which = 0; which < 2; ++which) {
+    const int off = which == 0 ? v.min_off : v.max_off;
+    if (off < 0) continue;
+    uint32_t a[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) a[u] = ok[u] ? loff[u] + off : dum;
+    switch (v.mm_kind * 2 + which) {
+      case OP_MIN_I32 * 2:
+#pragma unroll
+        for (int u = 0; u < N; ++u) lds_min(reinterpret_cast<int32_t*>(lds + a[u]), (int32_t)k[u]);
+        break;
+      case OP_MIN_I32 * 2 + 1:
+#pragma unroll
+        for (int u = 0; u < N; ++u) lds_max(reinterpret_cast<int32_t*>(lds + a[u]), (int32_t)k[u]);
+        break;
+      case OP_MIN_U32 * 2:
+#pragma unroll
+        for (int u = 0; u < N; ++u) lds_min(reinterpret_cast<uint32_t*>(lds + a[u]), (uint32_t)k[u]);
+        break;
+      case OP_MIN_U32 * 2 + 1:
+#pragma unroll
+        for (int u = 0; u < N; ++u) lds_max(reinterpret_cast<uint32_t*>(lds + a[u]), (uint32_t)k[u]);
+        break;
+      case OP_MIN_I64 * 2:
+#pragma unroll
+        for (int u = 0; u < N; ++u) lds_min(reinterpret_cast<long long*>(lds + a[u]), (long long)k[u]);
+        break;
+      case OP_MIN_I64 * 2 + 1:
+#pragma unroll
+        for (int u = 0; u < N; ++u) lds_max(reinterpret_cast<long long*>(lds + a[u]), (long long)k[u]);
+        break;
+      case OP_MIN_U64 * 2:
+#pragma unroll
+        for (int u = 0; u < N; ++u) lds_min(reinterpret_cast<unsigned long long*>(lds + a[u]), (unsigned long long)k[u]);
+        break;
+      case OP_MIN_U64 * 2 + 1:
+#pragma unroll
+        for (int u = 0; u < N; ++u) lds_max(reinterpret_cast<unsigned long long*>(lds + a[u]), (unsigned long long)k[u]);
+        break;
+      case OP_MIN_F32 * 2:
+#pragma unroll
+        for (int u = 0; u < N; ++u) {
+          const uint32_t b = (uint32_t)x[u];
+          const bool nan = (b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu);
+          lds_min(reinterpret_cast<uint32_t*>(lds + a[u]), nan ? kF32MinNaN : (uint32_t)k[u]);
+        }
+        break;
+      case OP_MIN_F32 * 2 + 1:
+#pragma unroll
+        for (int u = 0; u < N; ++u) {
+          const uint32_t b = (uint32_t)x[u];
+          const bool nan = (b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu);
+          lds_max(reinterpret_cast<uint32_t*>(lds + a[u]), nan ? kF32MaxNaN : (uint32_t)k[u]);
+        }
+        break;
+      case OP_MIN_F64 * 2:
+#pragma unroll
+        for (int u = 0; u < N; ++u) {
+          const unsigned long long b = x[u];
+          const bool nan = (b & 0x7ff0000000000000ull) == 0x7ff0000000000000ull && (b & 0x000fffffffffffffull);
+          lds_min(reinterpret_cast<unsigned long long*>(lds + a[u]), nan ? kF64MinNaN : (unsigned long long)k[u]);
+        }
+        break;
+      default:  // OP_MIN_F64 * 2 + 1
+#pragma unroll
+        for (int u = 0; u < N; ++u) {
+          const unsigned long long b = x[u];
+          const bool nan = (b & 0x7ff0000000000000ull) == 0x7ff0000000000000ull && (b & 0x000fffffffffffffull);
+          lds_max(reinterpret_cast<unsigned long long*>(lds + a[u]), nan ? kF64MaxNaN : (unsigned long long)k[u]);
+        }
+        break;
+    }
   }
 }
 

@@ -136,7 +136,7 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
       const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_a);
       const uint64_t rb = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pin_base) |
                           (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pin_base >> 32)) << 32;
-      load_recs(reinterpret_cast<const uint8_t*>(rb), (int)((a >> 24) & 7), base, step, present, rw);
+      load_recs(reinterpret_cast<const uint8_t*>(rb), (int)((a >> 24) & 7), base, step, present, base, rw);
       if (int_key) extract_rows(rw, (int)((a >> 7) & 63), kw, VC_UNSIGNED, key);
       const bool has_flags = (a >> 22) & 1;
       uint64_t f[kRows];
@@ -168,14 +168,17 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
     // keys) take the full probe one row at a time
     uint32_t loff[kRows];
     uint32_t miss = 0;
+    // int keys: the key and hash of each row, for the full probe of a miss
+    uint64_t kk[kRows];
+    uint32_t hsh[kRows];
     if (int_key) {
-      uint64_t kk[kRows], h0[kRows], h1[kRows];
+      uint64_t h0[kRows], h1[kRows];
       uint32_t i0[kRows], i1[kRows];
 #pragma unroll
       for (int u = 0; u < kRows; ++u) {
         kk[u] = (uint64_t)widen_int(key[u], kw, kvc);
-        const uint32_t hsh = hash_combine(0u, murmur3_bits(normalize_float_bits(key[u], kvc), kw, 0));
-        i0[u] = (uint32_t)reduce_range(hsh << log2p, slots);
+        hsh[u] = hash_combine(0u, murmur3_bits(normalize_float_bits(key[u], kvc), kw, 0));
+        i0[u] = (uint32_t)reduce_range(hsh[u] << log2p, slots);
         i1[u] = i0[u] + 1 == (uint32_t)slots ? 0u : i0[u] + 1;
         h0[u] = h1[u] = kEmptyIntKey;
         if (present[u]) {
@@ -205,7 +208,19 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
       const int u = __builtin_ctz(miss);
       miss &= miss - 1;
       const int64_t row = base + u * step;
-      const RowKey rk = row_key(p, row);
+      RowKey rk;
+      if (int_key) {
+        // from the probe's registers (no second read of the key)
+#pragma unroll
+        for (int q = 0; q < kRows; ++q)
+          if (q == u) {
+            rk.key = kk[q];
+            rk.hash = hsh[q];
+          }
+        rk.special = rk.key == kEmptyIntKey;
+      } else {
+        rk = row_key(p, row);
+      }
       uint8_t* r = lds_find_or_insert(t, p, rk, rk.hash << log2p, row);
       const uint32_t v = r ? (uint32_t)(r - lds) : kRecSpill;
 #pragma unroll
