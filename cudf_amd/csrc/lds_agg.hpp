@@ -177,8 +177,14 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
 #pragma unroll
       for (int u = 0; u < kRows; ++u) {
         kk[u] = (uint64_t)widen_int(key[u], kw, kvc);
-        hsh[u] = hash_combine(0u, murmur3_bits(normalize_float_bits(key[u], kvc), kw, 0));
-        i0[u] = (uint32_t)reduce_range(hsh[u] << log2p, slots);
+        // the slot hash: any function of the key independent of the bits that
+        // chose the partition. Records (partitioned rows) take a Fibonacci
+        // (multiplicative) hash of the integer key, cheaper than the row hash
+        // (the partition came from the row hash's top bits); other rows take
+        // the row hash, whose bits below the partition's pick the slot.
+        if constexpr (kRec) hsh[u] = (uint32_t)((kk[u] * 0x9E3779B97F4A7C15ull) >> 32);
+        else hsh[u] = hash_combine(0u, murmur3_bits(normalize_float_bits(key[u], kvc), kw, 0)) << log2p;
+        i0[u] = (uint32_t)reduce_range(hsh[u], slots);
         i1[u] = i0[u] + 1 == (uint32_t)slots ? 0u : i0[u] + 1;
         h0[u] = h1[u] = kEmptyIntKey;
         if (present[u]) {
@@ -209,19 +215,23 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
       miss &= miss - 1;
       const int64_t row = base + u * step;
       RowKey rk;
+      uint32_t slot_hash;
       if (int_key) {
-        // from the probe's registers (no second read of the key)
+        // from the probe's registers (no second read of the key); rk.hash
+        // is not read for integer keys
 #pragma unroll
         for (int q = 0; q < kRows; ++q)
           if (q == u) {
             rk.key = kk[q];
-            rk.hash = hsh[q];
+            slot_hash = hsh[q];
           }
+        rk.hash = slot_hash;
         rk.special = rk.key == kEmptyIntKey;
       } else {
         rk = row_key(p, row);
+        slot_hash = rk.hash << log2p;
       }
-      uint8_t* r = lds_find_or_insert(t, p, rk, rk.hash << log2p, row);
+      uint8_t* r = lds_find_or_insert(t, p, rk, slot_hash, row);
       const uint32_t v = r ? (uint32_t)(r - lds) : kRecSpill;
 #pragma unroll
       for (int q = 0; q < kRows; ++q)
