@@ -387,6 +387,62 @@ def bench_batched(args):
     }
 
 
+def oracle_sample(name: str, sample_rows: int, distinct: int, seed: int):
+    """Host columns of a bounded sample drawn like the workload, for the
+    oracle: (keys, requests)."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(seed)
+    n = sample_rows
+    kinds5 = [O.SUM, O.COUNT_VALID, O.MIN, O.MAX, O.MEAN]
+    if name == "c2":
+        uniq = np.unique(rng.integers(-2 ** 63, 2 ** 63 - 1, distinct, dtype=np.int64))
+        keys = [O.HostColumn(O.INT64, uniq[rng.integers(0, uniq.size, n)])]
+        reqs = [(O.HostColumn(O.INT32, rng.integers(-1_000_000, 1_000_001, n).astype(np.int32), rng.random(n) >= 0.05),
+                 kinds5),
+                (O.HostColumn(O.FLOAT64, rng.standard_normal(n), rng.random(n) >= 0.05), kinds5),
+                (O.HostColumn(O.FLOAT32, rng.standard_normal(n).astype(np.float32), rng.random(n) >= 0.05), kinds5)]
+    elif name == "c1":
+        keys = [O.HostColumn(O.INT32, rng.integers(0, 1000, n).astype(np.int32))]
+        reqs = [(O.HostColumn(O.FLOAT64, rng.random(n)), kinds5)]
+    elif name == "c3a":
+        n_keys = 50_000_000
+        u = rng.random(n)
+        r = np.clip(np.floor((1.0 + u * (n_keys ** -0.1 - 1.0)) ** -10.0).astype(np.int64) - 1, 0, n_keys - 1)
+        keys = [O.HostColumn(O.INT32, (r * 2654435761 % 2 ** 31).astype(np.int32)),
+                O.HostColumn(O.INT64, (r.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+                                       % np.uint64(2 ** 61)).astype(np.int64))]
+        reqs = [(O.HostColumn(O.FLOAT64, rng.standard_normal(n)), [O.SUM, O.COUNT_VALID, O.MAX])]
+    else:  # c3b
+        uniq = rng.integers(-2 ** 31, 2 ** 31 - 1, 16).astype(np.int32)
+        keys = [O.HostColumn(O.INT32, uniq[rng.integers(0, 16, n)])]
+        reqs = [(O.HostColumn(O.FLOAT64, rng.standard_normal(n)), [O.SUM, O.COUNT_VALID, O.MAX])]
+    return keys, reqs
+
+
+def cpu_baseline_oracle(name: str, sample_rows: int, distinct: int, seed: int, pandas_rows: int):
+    """The oracle (the C restatement of the reference's hash groupby,
+    oracle/groupby_oracle.c, one thread) timed on a bounded sample of the
+    workload, with the reference's own CPU path (pandas, the cudf.pandas
+    fallback) timed beside it on a smaller sample."""
+    from oracle import oracle as O
+
+    O.load()
+    keys, reqs = oracle_sample(name, sample_rows, distinct, seed)
+    t0 = time.perf_counter()
+    O.groupby(keys, reqs)
+    t = time.perf_counter() - t0
+    pd_line = (cpu_baseline_c2(pandas_rows, distinct, seed) if name == "c2"
+               else cpu_baseline_generic(name, pandas_rows, seed))
+    return {"value": sample_rows / t, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"the oracle's C groupby (oracle/groupby_oracle.c, one thread) on {sample_rows:,} rows drawn "
+                      f"like the workload, one call of {t:.1f} s; host os.cpu_count()={os.cpu_count()}",
+            "reference_cpu_path": {"value": pd_line["value"], "unit": "rows/s", "cores": 1,
+                                   "sample": pd_line["sample"]}}
+
+
 def cpu_baseline_generic(name: str, sample_rows: int, seed: int, reps: int = 3):
     """pandas groupby-agg on a bounded sample drawn like the workload."""
     import numpy as np
@@ -540,7 +596,10 @@ def main():
     ap.add_argument("--rows", type=int, default=None)
     ap.add_argument("--distinct", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--cpu-sample-rows", type=int, default=10_000_000)
+    ap.add_argument("--cpu-sample-rows", type=int, default=24_000_000,
+                    help="rows of the oracle's timed CPU sample (about 10 s for the 100M-row workload)")
+    ap.add_argument("--pandas-sample-rows", type=int, default=10_000_000,
+                    help="rows of the reference CPU path's (pandas) timed sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--summarize-trace", nargs=3, metavar=("RESULTS_DB", "TITLE", "COMMAND"),
                     help="print a text summary of a rocprofv3 --kernel-trace --stats database and exit")
@@ -664,10 +723,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        if args.workload == "c2":
-            cpu = cpu_baseline_c2(min(args.cpu_sample_rows, R), args.distinct, args.seed)
-        else:
-            cpu = cpu_baseline_generic(args.workload, min(args.cpu_sample_rows, R), args.seed)
+        cpu = cpu_baseline_oracle(args.workload, min(args.cpu_sample_rows, R), args.distinct, args.seed,
+                                  min(args.pandas_sample_rows, R))
 
     line = {
         "metric": METRIC,
