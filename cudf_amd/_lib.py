@@ -48,8 +48,13 @@ ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, c
 DEALLOC_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p)
 
 
+ALLOC_MANY_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_size_t),
+                                 ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p)
+
+
 class mgb_allocator(ctypes.Structure):
-    _fields_ = [("allocate", ALLOC_FN), ("deallocate", DEALLOC_FN), ("ctx", ctypes.c_void_p)]
+    _fields_ = [("allocate", ALLOC_FN), ("deallocate", DEALLOC_FN), ("ctx", ctypes.c_void_p),
+                ("allocate_many", ALLOC_MANY_FN)]
 
 
 class mgb_out_column(ctypes.Structure):

@@ -1072,33 +1072,34 @@ mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int6
 {
   Allocs al{alloc, stream, {}};
   const int nk = (int)plan.keys.size(), no = (int)plan.outs.size();
-  auto fill_col = [&](mgb_out_column& oc, int type_id, int width, const uint8_t* data, const uint8_t* valid,
-                      bool nullable) -> mgb_status {
+  // the columns' sizes, then every buffer in one request
+  std::vector<size_t> sizes;
+  std::vector<void**> dests;
+  auto fill_col = [&](mgb_out_column& oc, int type_id, int width, const uint8_t* valid) {
     std::memset(&oc, 0, sizeof(oc));
     oc.type_id = type_id;
     oc.size = groups;
-    if (groups == 0) return MGB_SUCCESS;
+    if (groups == 0) return;
     oc.data_bytes = (size_t)groups * width;
-    oc.data = al.get(oc.data_bytes);
-    if (!oc.data) return fail(MGB_ERROR_ALLOC, "allocation of an output column failed");
-    if (nullable && valid) {
+    sizes.push_back(oc.data_bytes);
+    dests.push_back(&oc.data);
+    if (valid) {
       oc.mask_bytes = (size_t)bitmask_bytes(groups);
-      oc.null_mask = static_cast<uint32_t*>(al.get(oc.mask_bytes));
-      if (!oc.null_mask) return fail(MGB_ERROR_ALLOC, "allocation of an output null mask failed");
+      sizes.push_back(oc.mask_bytes);
+      dests.push_back(reinterpret_cast<void**>(&oc.null_mask));
     }
-    return MGB_SUCCESS;
   };
-  mgb_status st = MGB_SUCCESS;
-  for (int c = 0; c < nk && st == MGB_SUCCESS; ++c)
-    st = fill_col(out_keys[c], plan.keys[c].type_id, plan.keys[c].width, t.key_data[c], t.key_valid[c],
-                  t.key_valid[c] != nullptr);
-  for (int o = 0; o < no && st == MGB_SUCCESS; ++o)
-    st = fill_col(out_results[o], plan.outs[o].out_type, plan.outs[o].spec.out_width, t.out_data[o], t.out_valid[o],
-                  t.out_valid[o] != nullptr);
-  if (st != MGB_SUCCESS) {
+  for (int c = 0; c < nk; ++c) fill_col(out_keys[c], plan.keys[c].type_id, plan.keys[c].width, t.key_valid[c]);
+  for (int o = 0; o < no; ++o)
+    fill_col(out_results[o], plan.outs[o].out_type, plan.outs[o].spec.out_width, t.out_valid[o]);
+  std::vector<void*> ptrs(sizes.size(), nullptr);
+  if (!al.get_many((int)sizes.size(), sizes.data(), ptrs.data())) {
     al.release_all();
-    return st;
+    for (int c = 0; c < nk; ++c) out_keys[c].data = out_keys[c].null_mask = nullptr;
+    for (int o = 0; o < no; ++o) out_results[o].data = out_results[o].null_mask = nullptr;
+    return fail(MGB_ERROR_ALLOC, "allocation of the output columns failed");
   }
+  for (size_t i = 0; i < ptrs.size(); ++i) *dests[i] = ptrs[i];
   if (groups > 0) {
     MGB_HIP(hipMemsetAsync(t.null_counts, 0, (size_t)(nk + no) * 8, stream));
     const int64_t words = bitmask_bytes(groups) / 4;

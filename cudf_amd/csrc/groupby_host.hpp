@@ -156,6 +156,20 @@ struct Allocs {
     if (p) live.push_back({p, bytes});
     return p;
   }
+  // n buffers (sizes > 0) in one request when the allocator takes one; false
+  // when any allocation failed (those made are in `live`)
+  bool get_many(int n, const size_t* bytes, void** out)
+  {
+    if (n == 0) return true;
+    if (alloc->allocate_many) {
+      if (alloc->allocate_many(alloc->ctx, n, bytes, out, (void*)stream) != 0) return false;
+      for (int i = 0; i < n; ++i) live.push_back({out[i], bytes[i]});
+      return true;
+    }
+    for (int i = 0; i < n; ++i)
+      if (!(out[i] = get(bytes[i]))) return false;
+    return true;
+  }
   void release_all()
   {
     for (auto& l : live) alloc->deallocate(alloc->ctx, l.first, l.second, (void*)stream);

@@ -29,7 +29,9 @@ def _to_mgb(c: Column) -> _lib.mgb_column:
 
 class _TorchAllocator:
     """mgb_allocator backed by torch's caching allocator: each buffer is a
-    uint8 tensor kept alive here until the result columns take it over."""
+    uint8 tensor kept alive here until the result columns take it over. All
+    result buffers of one call come from one tensor (allocate_many): each
+    column holds a view of it, and the tensor is freed with the last view."""
 
     def __init__(self, device: torch.device):
         self.device = device
@@ -37,7 +39,24 @@ class _TorchAllocator:
         self.lock = threading.Lock()
         self._alloc = _lib.ALLOC_FN(self._allocate)
         self._dealloc = _lib.DEALLOC_FN(self._deallocate)
-        self.struct = _lib.mgb_allocator(self._alloc, self._dealloc, None)
+        self._alloc_many = _lib.ALLOC_MANY_FN(self._allocate_many)
+        self.struct = _lib.mgb_allocator(self._alloc, self._dealloc, None, self._alloc_many)
+
+    def _allocate_many(self, ctx, n, sizes, out, stream):
+        offs, total = [], 0
+        for i in range(n):
+            offs.append(total)
+            total += (int(sizes[i]) + 255) // 256 * 256
+        try:
+            t = torch.empty(max(total, 1), dtype=torch.uint8, device=self.device)
+        except Exception:  # out of memory: the library reports MGB_ERROR_ALLOC
+            return 1
+        base = t.data_ptr()
+        with self.lock:
+            for i in range(n):
+                out[i] = base + offs[i]
+                self.live[base + offs[i]] = t[offs[i]: offs[i] + int(sizes[i])]
+        return 0
 
     def _allocate(self, ctx, nbytes, stream):
         try:

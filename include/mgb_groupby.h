@@ -93,11 +93,16 @@ typedef struct mgb_request {
 
 /* Device allocator for the returned buffers (the role of the `mr` argument of
  * cudf::groupby::groupby::aggregate). Buffers come back owned by the caller,
- * who releases them with `deallocate`. */
+ * who releases them with `deallocate`, one buffer at a time.
+ * `allocate_many` (optional, may be NULL; no counterpart in the reference) is
+ * asked for all result buffers of one call at once: n sizes in, n pointers out,
+ * 0 on success. A caller whose allocator is costly per call (an interpreter
+ * callback) sets it; each buffer is still released by its own `deallocate`. */
 typedef struct mgb_allocator {
   void* (*allocate)(void* ctx, size_t bytes, void* stream);
   void (*deallocate)(void* ctx, void* ptr, size_t bytes, void* stream);
   void* ctx;
+  int32_t (*allocate_many)(void* ctx, int32_t n, const size_t* bytes, void** out, void* stream);
 } mgb_allocator;
 
 /* An owned result column written by the library. */
