@@ -1136,6 +1136,8 @@ extern "C" mgb_status mgb_batch_finish(mgb_batch* b, void* stream_v)
     if (st != MGB_SUCCESS) return st;
     b->groups[q] = g;
   }
+  // re-run results are read from any stream afterwards
+  if (reruns) MGB_HIP(hipStreamSynchronize(stream));
   b->stats.reruns = reruns;
   return MGB_SUCCESS;
 }

@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void emit_columns_kernel(const EmitParams e
   }
   if (nulls) atomicAdd(&block_nulls, nulls);
   __syncthreads();
-  if (threadIdx.x == 0 && block_nulls) atom_add(c.nulls, (unsigned long long)block_nulls);
+  if (threadIdx.x == 0 && block_nulls && c.nulls) atom_add(c.nulls, (unsigned long long)block_nulls);
 }
 
 // ---- cardinality estimate: stratified sample into a small table
@@ -782,6 +782,7 @@ Workspace* workspace_for(hipStream_t s)
 
 
 constexpr int kMaxPinned = 1024;  // host readback slots (counters, null counts)
+constexpr int kPinnedNulls = 128; // first slot of null counts read back with the group count
 
 int device_cu_count()
 {
@@ -992,6 +993,10 @@ struct TempOuts {
   std::vector<uint8_t*> key_data, key_valid, out_data, out_valid;
   unsigned long long* null_counts = nullptr;
   int64_t rows = 0;  // capacity in groups
+  // null counts read back with the group count (counted as the groups were
+  // written), so that the outputs are emitted without a second wait
+  bool nulls_known = false;
+  std::vector<int64_t> host_nulls;
 };
 
 mgb_status layout_temp(Workspace* ws, const Plan& plan, int64_t rows, hipStream_t stream, TempOuts& t,
@@ -1101,7 +1106,7 @@ mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int6
   }
   for (size_t i = 0; i < ptrs.size(); ++i) *dests[i] = ptrs[i];
   if (groups > 0) {
-    MGB_HIP(hipMemsetAsync(t.null_counts, 0, (size_t)(nk + no) * 8, stream));
+    if (!t.nulls_known) MGB_HIP(hipMemsetAsync(t.null_counts, 0, (size_t)(nk + no) * 8, stream));
     const int64_t words = bitmask_bytes(groups) / 4;
     // every column's copy and validity in one launch (per kEmitCols columns)
     for (int j0 = 0; j0 < no + nk; j0 += kEmitCols) {
@@ -1119,17 +1124,23 @@ mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int6
         c.valid = j < no ? t.out_valid[j] : t.key_valid[j - no];
         c.mask = oc.null_mask;
         c.words = oc.null_mask ? words : 0;
-        c.nulls = t.null_counts + j;
+        c.nulls = t.nulls_known ? nullptr : t.null_counts + j;
         units = std::max(units, (c.bytes + 15) / 16);
       }
       const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((units + kBlock - 1) / kBlock, 1024));
       emit_columns_kernel<<<dim3(gx, (unsigned)n), kBlock, 0, stream>>>(ep);
     }
     MGB_HIP(hipGetLastError());
-    MGB_HIP(hipMemcpyAsync(ws->pinned, t.null_counts, (size_t)(no + nk) * 8, hipMemcpyDeviceToHost, stream));
-    MGB_HIP(hipStreamSynchronize(stream));
-    for (int o = 0; o < no; ++o) out_results[o].null_count = out_results[o].null_mask ? (int64_t)ws->pinned[o] : 0;
-    for (int c = 0; c < nk; ++c) out_keys[c].null_count = out_keys[c].null_mask ? (int64_t)ws->pinned[no + c] : 0;
+    const int64_t* counts = nullptr;
+    if (t.nulls_known) {
+      counts = t.host_nulls.data();
+    } else {
+      MGB_HIP(hipMemcpyAsync(ws->pinned, t.null_counts, (size_t)(no + nk) * 8, hipMemcpyDeviceToHost, stream));
+      MGB_HIP(hipStreamSynchronize(stream));
+      counts = reinterpret_cast<const int64_t*>(ws->pinned);
+    }
+    for (int o = 0; o < no; ++o) out_results[o].null_count = out_results[o].null_mask ? counts[o] : 0;
+    for (int c = 0; c < nk; ++c) out_keys[c].null_count = out_keys[c].null_mask ? counts[no + c] : 0;
   }
   return MGB_SUCCESS;
 }
@@ -1593,6 +1604,9 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     OutTable ot{};
     mgb_status st = layout_temp(ws, plan, U, stream, temp, ot);
     if (st != MGB_SUCCESS) return st;
+    const int n_nulls = (int)plan.outs.size() + (int)plan.keys.size();
+    ot.nulls = temp.null_counts;
+    MGB_HIP(hipMemsetAsync(temp.null_counts, 0, (size_t)n_nulls * 8, stream));
     uint64_t tmpl[kMaxRecordWords] = {};
     record_template(plan, plan.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef, tmpl);
     const OutTable* d_ot = nullptr;
@@ -1749,10 +1763,14 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     }
     MGB_HIP(hipGetLastError());
     MGB_HIP(hipMemcpyAsync(ws->pinned, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    MGB_HIP(hipMemcpyAsync(ws->pinned + kPinnedNulls, temp.null_counts, (size_t)n_nulls * 8, hipMemcpyDeviceToHost,
+                           stream));
     MGB_HIP(hipStreamSynchronize(stream));
     if (ws->pinned[5] == 0) {
       groups = (int64_t)(ws->pinned[1] + ws->pinned[6]);
       temp.rows = groups;
+      temp.nulls_known = true;
+      temp.host_nulls.assign(ws->pinned + kPinnedNulls, ws->pinned + kPinnedNulls + n_nulls);
       done = true;
       return MGB_SUCCESS;
     }

@@ -878,6 +878,13 @@ __device__ __forceinline__ void store_width(void* out, int64_t pos, int width, u
   }
 }
 
+// Adds the active lanes' `flag`s to *ctr (one atomic per wave).
+__device__ __forceinline__ void count_wave(unsigned long long* ctr, bool flag)
+{
+  const uint64_t m = __ballot(flag);
+  if (m && (threadIdx.x & 63u) == (unsigned)__builtin_ctzll(__ballot(1))) atom_add(ctr, (unsigned long long)__popcll(m));
+}
+
 // Writes the requested results of one group's record at output row `pos`.
 __device__ __forceinline__ void finalize_record(const uint8_t* rec, int64_t pos, const OutTable& ot)
 {
@@ -927,7 +934,10 @@ __device__ __forceinline__ void finalize_record(const uint8_t* rec, int64_t pos,
       default: break;
     }
     store_width(s.out, pos, s.out_width, bits);
-    if (s.valid) to_global(s.valid)[pos] = cnt != 0;
+    if (s.valid) {
+      to_global(s.valid)[pos] = cnt != 0;
+      if (ot.nulls) count_wave(ot.nulls + o, cnt == 0);
+    }
   }
 }
 
@@ -944,7 +954,11 @@ __device__ __forceinline__ void finalize_keys(uint64_t hdr, bool special, int ke
   for (int c = 0; c < num_keys; ++c) {
     const KeyCol& k = keys[c];
     store_width(ot.key_out[c], pos, k.width, load_at(k.data, k.width, k.stride, rep));
-    if (ot.key_valid[c]) to_global(ot.key_valid[c])[pos] = bit_valid(k.mask, k.bit_off + rep) ? 1 : 0;
+    if (ot.key_valid[c]) {
+      const bool valid = bit_valid(k.mask, k.bit_off + rep);
+      to_global(ot.key_valid[c])[pos] = valid ? 1 : 0;
+      if (ot.nulls) count_wave(ot.nulls + ot.num_outs + c, !valid);
+    }
   }
 }
 

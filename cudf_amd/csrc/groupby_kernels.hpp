@@ -72,6 +72,9 @@ struct OutTable {
   uint8_t* key_valid[kMaxKeys];   // key validity bytes (null keys grouped), or nullptr
   int32_t num_outs;
   int32_t pad;
+  // null results counted as they are written: [num_outs] results, then the
+  // keys; nullptr: not counted
+  unsigned long long* nulls;
   OutSpec outs[kMaxOuts];
 };
 

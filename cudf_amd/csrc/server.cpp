@@ -361,6 +361,10 @@ bool mgb_server::launch(Slot& s)
     d.results.assign(nout, mgb_out_column{});
     d.status = mgb_groupby_aggregate(keys.data(), (int32_t)keys.size(), q.null_policy, reqs.data(),
                                      (int32_t)reqs.size(), d.keys.data(), d.results.data(), &d.groups, s.stream, &alloc);
+    // the call returns with its last writes still queued on the stream: the
+    // result is complete (and may be read from any stream) after this wait
+    if (d.status == MGB_SUCCESS && hipStreamSynchronize(s.stream) != hipSuccess)
+      d.status = fail(MGB_ERROR_HIP, "stream sync failed");
     if (d.status != MGB_SUCCESS) d.error = mgb_last_error_message();
     d.latency_ms = ms_since(q.arrival, Clock::now());
     for (auto& c : d.keys) d.bytes += (int64_t)(c.data_bytes + c.mask_bytes);

@@ -142,7 +142,10 @@ int32_t mgb_target_type(int32_t value_type_id, int32_t agg_kind);
  *                         each request's aggregations in order
  *   out_num_groups        number of groups (rows of every output)
  *
- * Output row order is arbitrary (as in cudf). Errors: MGB_ERROR_LOGIC for a
+ * Output row order is arbitrary (as in cudf). The call waits on `stream` for
+ * the group count; the outputs' last writes may still be queued on `stream`
+ * when it returns (stream order, as in cudf): a reader on another stream
+ * waits for `stream` first. Errors: MGB_ERROR_LOGIC for a
  * size mismatch or an invalid type/aggregation pair (groupby.cu:208-212,
  * 173-190), MGB_ERROR_UNSUPPORTED for aggregations cudf serves only from its
  * sort-based path.
