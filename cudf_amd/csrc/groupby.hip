@@ -72,6 +72,18 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const AggParams p)
   }
 }
 
+// Second pass over the rows (VARIANCE / STD / ARGMIN / ARGMAX fields).
+__global__ __launch_bounds__(kBlock) void pass2_kernel(const Pass2Params q)
+{
+  const AggParams& p = q.src;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < p.num_rows; row += step) {
+    if (!key_row_valid(p, row)) continue;
+    uint8_t* rec = global_find(p, row_key(p, row), row);
+    if (rec) pass2_row(q, rec, row);
+  }
+}
+
 __device__ __forceinline__ bool record_occupied(const uint8_t* table, int64_t r, int64_t capacity, int32_t stride,
                                                 int32_t key_mode)
 {
@@ -729,7 +741,8 @@ int32_t target_type(int32_t t, int32_t kind)
       if (is_timestamp_t(t)) return -1;
       return MGB_TYPE_FLOAT64;
     case MGB_AGG_M2: case MGB_AGG_VARIANCE: case MGB_AGG_STD:
-      if (is_timestamp_t(t)) return -1;
+      // numeric sources only (multi_pass_kernels.cuh:59-63)
+      if (!is_integral_t(t) && !is_float_t(t)) return -1;
       return MGB_TYPE_FLOAT64;
     default: return -1;
   }
@@ -849,7 +862,8 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
       if (target_type(v.type_id, kind) < 0) return fail(MGB_ERROR_LOGIC, "Invalid type/aggregation combination.");
       switch (kind) {
         case MGB_AGG_SUM: case MGB_AGG_PRODUCT: case MGB_AGG_MIN: case MGB_AGG_MAX: case MGB_AGG_COUNT_VALID:
-        case MGB_AGG_COUNT_ALL: case MGB_AGG_SUM_OF_SQUARES: case MGB_AGG_MEAN: break;
+        case MGB_AGG_COUNT_ALL: case MGB_AGG_SUM_OF_SQUARES: case MGB_AGG_MEAN: case MGB_AGG_VARIANCE:
+        case MGB_AGG_STD: case MGB_AGG_ARGMIN: case MGB_AGG_ARGMAX: break;
         default:
           return fail(MGB_ERROR_UNSUPPORTED,
                       "aggregation kind " + std::to_string(kind) + " is not served by this hash groupby");
@@ -884,6 +898,12 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
       if (o.first == kind) return;
     pc.ops.push_back({kind, -1});
   };
+  auto add_p2 = [&plan](PlanCol& pc, int kind) {
+    plan.has_pass2 = true;
+    for (auto& o : pc.p2)
+      if (o.first == kind) return;
+    pc.p2.push_back({kind, -1});
+  };
   for (auto& wt : wants) {
     PlanCol& pc = plan.cols[wt.col];
     switch (wt.kind) {
@@ -893,6 +913,20 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
       case MGB_AGG_MIN: add_op(pc, minmax_op_for(pc.type, true)); break;
       case MGB_AGG_MAX: add_op(pc, minmax_op_for(pc.type, false)); break;
       case MGB_AGG_COUNT_ALL: need_rows = true; break;
+      // second-pass aggregations: the SUM (mean) or MIN / MAX of the first
+      // pass, then a field the second pass fills (M2 is shared by every ddof)
+      case MGB_AGG_VARIANCE: case MGB_AGG_STD:
+        add_op(pc, sum_op_for(pc.type));
+        add_p2(pc, OP_M2);
+        break;
+      case MGB_AGG_ARGMIN:
+        add_op(pc, minmax_op_for(pc.type, true));
+        add_p2(pc, OP_ARGMIN_IDX);
+        break;
+      case MGB_AGG_ARGMAX:
+        add_op(pc, minmax_op_for(pc.type, false));
+        add_p2(pc, OP_ARGMAX_IDX);
+        break;
       default: break;
     }
     // COUNT_VALID = rows - nulls: one shared row count per group plus a null
@@ -903,13 +937,19 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
 
   // record layout: 8-byte header, 8-byte fields, then 4-byte fields
   int off = 8;
-  for (auto& pc : plan.cols)
+  for (auto& pc : plan.cols) {
     for (auto& o : pc.ops)
       if (op_field_bytes(o.first) == 8) { o.second = off; off += 8; }
+    for (auto& o : pc.p2)
+      if (op_field_bytes(o.first) == 8) { o.second = off; off += 8; }
+  }
   if (need_rows) { plan.rows_off = off; off += 4; }
-  for (auto& pc : plan.cols)
+  for (auto& pc : plan.cols) {
     for (auto& o : pc.ops)
       if (op_field_bytes(o.first) == 4) { o.second = off; off += 4; }
+    for (auto& o : pc.p2)
+      if (op_field_bytes(o.first) == 4) { o.second = off; off += 4; }
+  }
   plan.stride = (int)align_up((size_t)off, 8);
   if ((plan.stride + 7) / 8 + 1 > kMaxRecordWords)  // room for the LDS padding word
     return fail(MGB_ERROR_UNSUPPORTED, "too many distinct aggregations for one groupby call");
@@ -935,6 +975,15 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
       case MGB_AGG_COUNT_VALID:
         s.fin = FIN_COUNT;
         s.valid = nullptr;
+        break;
+      case MGB_AGG_VARIANCE: case MGB_AGG_STD:
+        s.field_off = pc.p2_offset(OP_M2);
+        s.fin = wt.kind == MGB_AGG_VARIANCE ? FIN_VAR : FIN_STD;
+        s.ddof = wt.ddof;
+        break;
+      case MGB_AGG_ARGMIN: case MGB_AGG_ARGMAX:
+        s.field_off = pc.p2_offset(wt.kind == MGB_AGG_ARGMIN ? OP_ARGMIN_IDX : OP_ARGMAX_IDX);
+        s.fin = FIN_ARGIDX;
         break;
       case MGB_AGG_COUNT_ALL:
         s.fin = FIN_COUNT;
@@ -975,11 +1024,16 @@ void record_template(const Plan& plan, uint64_t header, uint64_t* words)
 {
   std::vector<uint8_t> bytes((size_t)plan.stride, 0);
   std::memcpy(bytes.data(), &header, 8);
-  for (auto& pc : plan.cols)
+  for (auto& pc : plan.cols) {
     for (auto& o : pc.ops) {
       uint64_t id = op_identity(o.first);
       std::memcpy(bytes.data() + o.second, &id, (size_t)op_field_bytes(o.first));
     }
+    for (auto& o : pc.p2) {
+      uint64_t id = op_identity(o.first);
+      std::memcpy(bytes.data() + o.second, &id, (size_t)op_field_bytes(o.first));
+    }
+  }
   std::memcpy(words, bytes.data(), (size_t)plan.stride);
 }
 
@@ -1481,6 +1535,45 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
     capacity = std::min<int64_t>(capacity * 4, std::max<int64_t>(2 * R + 64, 64));
   }
 
+  if (plan.has_pass2 && groups > 0) {
+    // the table is complete: each row finds its group and updates the
+    // second-pass fields, in launches of up to kMaxVals columns
+    Pass2Params q{};
+    q.src = base;
+    q.src.table = static_cast<uint8_t*>(ws->main.ptr);
+    q.src.capacity = capacity;
+    q.src.flags = nullptr;
+    size_t ci = 0;
+    for (;;) {
+      q.src.num_vals = 0;
+      q.num_ops = 0;
+      for (; ci < plan.cols.size(); ++ci) {
+        const PlanCol& pc = plan.cols[ci];
+        if (pc.p2.empty()) continue;
+        if (q.src.num_vals == kMaxVals || q.num_ops + (int)pc.p2.size() > kMaxPass2) break;
+        const int vi = q.src.num_vals++;
+        q.src.vals[vi] = pc.col;
+        for (auto& o : pc.p2) {
+          Pass2Op& op = q.ops[q.num_ops++];
+          op.kind = o.first;
+          op.val = vi;
+          op.off = o.second;
+          op.ref_kind = o.first == OP_M2 ? sum_op_for(pc.type) : minmax_op_for(pc.type, o.first == OP_ARGMIN_IDX);
+          op.ref_off = pc.op_offset(op.ref_kind);
+          op.count_off = plan.rows_off;
+          op.nulls_off = pc.nulls_off;
+        }
+      }
+      if (q.num_ops == 0) break;
+      const int64_t grid = std::min<int64_t>((R + kBlock - 1) / kBlock, (int64_t)cus * 16);
+      {
+        ProfScope prof("pass2", stream);
+        pass2_kernel<<<(unsigned)std::max<int64_t>(grid, 1), kBlock, 0, stream>>>(q);
+      }
+      MGB_HIP(hipGetLastError());
+    }
+  }
+
   OutTable ot{};
   mgb_status st = layout_temp(ws, plan, groups, stream, temp, ot);
   if (st != MGB_SUCCESS) return st;
@@ -1812,7 +1905,8 @@ mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mg
   const bool lds_ok = geom.ok && single_launch_ops(plan, probe) && !plan.compare_key_nulls &&
                       (int)plan.outs.size() <= kMaxOuts;
   Path path = forced_path();
-  if (!lds_ok) path = Path::Global;
+  // the second pass finds each row's group in the global table
+  if (!lds_ok || plan.has_pass2) path = Path::Global;
   if (path == Path::Auto) path = (G_est <= geom.cap / 2) ? Path::Direct : Path::Partitioned;
 
   TempOuts temp;

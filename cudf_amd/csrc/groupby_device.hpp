@@ -154,6 +154,67 @@ __device__ __forceinline__ uint8_t* global_find_or_insert(const AggParams& p, co
   return nullptr;
 }
 
+// Record of the row's group in a completed global table (every group of the
+// rows inserted, so the probe meets the row's key before any empty slot);
+// nullptr if the key is absent.
+__device__ __forceinline__ uint8_t* global_find(const AggParams& p, const RowKey& rk, int64_t row)
+{
+  if (rk.special) return p.table + p.capacity * (int64_t)p.stride;
+  const uint64_t empty = p.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
+  int64_t idx = reduce_range(rk.hash, p.capacity);
+  for (int64_t probe = 0; probe < p.capacity; ++probe) {
+    uint8_t* rec = p.table + idx * (int64_t)p.stride;
+    const uint64_t cur = *reinterpret_cast<const uint64_t*>(rec);
+    if (cur == empty) return nullptr;
+    if (p.key_mode == KEY_INT) {
+      if (cur == rk.key) return rec;
+    } else if ((uint32_t)(cur >> 32) == rk.hash &&
+               rows_equal(p.keys, p.num_keys, p.compare_key_nulls != 0, (int64_t)(cur & 0xffffffffull), row)) {
+      return rec;
+    }
+    idx = (idx + 1 == p.capacity) ? 0 : idx + 1;
+  }
+  return nullptr;
+}
+
+// The second-pass ops of one row on its group's record (global table, whose
+// first-pass fields are final).
+//  OP_M2: adds (x - mean)^2 with mean = SUM / COUNT_VALID, the SUM of a
+//    float32 column taken at float32 precision as its SUM result is
+//    (reference: cpp/src/groupby/hash/multi_pass_kernels.cuh:76-90).
+//  OP_ARGMIN_IDX / OP_ARGMAX_IDX: the smallest row index whose value is the
+//    group's MIN / MAX (the reference's CAS loop in
+//    cpp/include/cudf/detail/aggregation/aggregation.cuh:436-490 returns one
+//    of the tied rows; this is the first of them).
+__device__ __forceinline__ void pass2_row(const Pass2Params& q, uint8_t* rec, int64_t row)
+{
+  const AggParams& p = q.src;
+  for (int i = 0; i < q.num_ops; ++i) {
+    const Pass2Op& o = q.ops[i];
+    const ValCol& v = p.vals[o.val];
+    if (!bit_valid(v.mask, v.bit_off + row)) continue;
+    const uint64_t raw = load_at(v.data, v.width, v.stride, row);
+    if (o.kind == OP_M2) {
+      uint32_t cnt = *reinterpret_cast<const uint32_t*>(rec + o.count_off);
+      if (o.nulls_off >= 0) cnt -= *reinterpret_cast<const uint32_t*>(rec + o.nulls_off);
+      double sum;
+      if (o.ref_kind == OP_SUM_I64) {
+        sum = (double)*reinterpret_cast<const int64_t*>(rec + o.ref_off);
+      } else {
+        sum = *reinterpret_cast<const double*>(rec + o.ref_off);
+        if (v.vclass == VC_F32) sum = (double)(float)sum;
+      }
+      const double d = widen_f64(raw, v.width, v.vclass) - sum / (double)cnt;
+      atom_add(reinterpret_cast<double*>(rec + o.off), d * d);
+    } else {
+      const uint64_t code = minmax_code(o.ref_kind, raw, v.width, v.vclass);
+      const uint64_t best = op_field_bytes(o.ref_kind) == 8 ? *reinterpret_cast<const uint64_t*>(rec + o.ref_off)
+                                                            : *reinterpret_cast<const uint32_t*>(rec + o.ref_off);
+      if (code == best) atom_min(reinterpret_cast<uint32_t*>(rec + o.off), (uint32_t)row);
+    }
+  }
+}
+
 // All ops of one row on a record (global or LDS).
 __device__ __forceinline__ void apply_row(const AggParams& p, uint8_t* rec, int64_t row)
 {
@@ -894,6 +955,7 @@ __device__ __forceinline__ void finalize_record(const uint8_t* rec, int64_t pos,
     if (s.nulls_off >= 0) cnt -= *reinterpret_cast<const uint32_t*>(rec + s.nulls_off);
     const uint8_t* f = rec + s.field_off;
     uint64_t bits = 0;
+    bool ok = cnt != 0;
     switch (s.fin) {
       case FIN_COUNT: bits = cnt; break;
       case FIN_INT:
@@ -931,12 +993,25 @@ __device__ __forceinline__ void finalize_record(const uint8_t* rec, int64_t pos,
         bits = (uint64_t)m;
         break;
       }
+      case FIN_VAR: case FIN_STD: {
+        // valid when the group has more valid values than ddof
+        // (multi_pass_kernels.cuh:80; STD is the square root, groupby.cu:339-347)
+        ok = cnt != 0 && (int64_t)cnt > (int64_t)s.ddof;
+        double r = ok ? *reinterpret_cast<const double*>(f) / (double)((int64_t)cnt - s.ddof) : 0.0;
+        if (s.fin == FIN_STD) r = sqrt(r);
+        bits = (uint64_t)__double_as_longlong(r);
+        break;
+      }
+      case FIN_ARGIDX:
+        // a group without valid values keeps the sentinel -1 (groupby.cu:231-240)
+        bits = cnt ? *reinterpret_cast<const uint32_t*>(f) : 0xffffffffu;
+        break;
       default: break;
     }
     store_width(s.out, pos, s.out_width, bits);
     if (s.valid) {
-      to_global(s.valid)[pos] = cnt != 0;
-      if (ot.nulls) count_wave(ot.nulls + o, cnt == 0);
+      to_global(s.valid)[pos] = ok;
+      if (ot.nulls) count_wave(ot.nulls + o, !ok);
     }
   }
 }

@@ -118,11 +118,18 @@ struct PlanCol {
   int type;
   bool nullable;
   std::vector<std::pair<int, int>> ops;  // (op kind, byte offset)
+  std::vector<std::pair<int, int>> p2;   // second-pass fields (OP_M2, OP_ARG*_IDX), byte offset
   int count_off = -1;       // the row-count field
   int nulls_off = -1;       // OP_NULLS field when the column has nulls
   int op_offset(int kind) const
   {
     for (auto& o : ops)
+      if (o.first == kind) return o.second;
+    return -1;
+  }
+  int p2_offset(int kind) const
+  {
+    for (auto& o : p2)
       if (o.first == kind) return o.second;
     return -1;
   }
@@ -137,6 +144,7 @@ struct Plan {
   std::vector<PlanCol> cols;
   int rows_off = -1;
   int stride = 8;
+  bool has_pass2 = false;   // VARIANCE / STD / ARGMIN / ARGMAX: a second pass over the rows
   struct Out {
     int col;        // index into cols
     int kind;       // aggregation kind

@@ -50,7 +50,10 @@ enum FinKind : int32_t {
   FIN_MEAN_I64,       // i64 sum / u32 count -> double
   FIN_MEAN_F64,       // f64 sum / u32 count -> double
   FIN_MEAN_F32,       // (float)f64 sum / u32 count -> double
-  FIN_MEAN_DURATION   // i64 sum / u32 count -> integer of out_width bytes (truncating)
+  FIN_MEAN_DURATION,  // i64 sum / u32 count -> integer of out_width bytes (truncating)
+  FIN_VAR,            // f64 sum of squared deviations / (count - ddof) -> double, null unless count > ddof
+  FIN_STD,            // square root of FIN_VAR's value
+  FIN_ARGIDX          // u32 row index -> int32 (-1 where the group has no valid value)
 };
 
 struct OutSpec {
@@ -61,7 +64,7 @@ struct OutSpec {
   int32_t field_off;      // primary field
   int32_t count_off;      // row-count field (COUNT_ALL) or count of valid values
   int32_t nulls_off;      // OP_NULLS field subtracted from count_off, or -1
-  int32_t pad;
+  int32_t ddof;           // FIN_VAR / FIN_STD: delta degrees of freedom
 };
 
 constexpr int kMaxOuts = 64;    // outputs per call on the LDS paths / per finalize launch
@@ -128,6 +131,29 @@ struct LdsParams {
   int32_t direct;                 // 1: merge into the global table, 0: write outputs
   const uint32_t* item_offsets;   // partitioned mode: [P+1] first work item of each partition
   int64_t max_item_rows;          // partitioned mode: rows per work item at most
+};
+
+// Second pass over the rows (VARIANCE / STD / ARGMIN / ARGMAX): every row
+// finds its group's record in the completed global table and updates the
+// second-pass fields from the first pass's SUM / MIN / MAX fields.
+constexpr int kMaxPass2 = 16;
+
+struct Pass2Op {
+  int32_t kind;       // OP_M2, OP_ARGMIN_IDX or OP_ARGMAX_IDX
+  int32_t val;        // index into src.vals
+  int32_t off;        // the updated field
+  int32_t ref_off;    // OP_M2: the column's SUM field; ARG*: its MIN / MAX field
+  int32_t ref_kind;   // op kind of the ref field
+  int32_t count_off;  // the row-count field (OP_M2)
+  int32_t nulls_off;  // the column's OP_NULLS field, or -1 (OP_M2)
+  int32_t pad;
+};
+
+struct Pass2Params {
+  AggParams src;      // rows, keys, value columns; table/capacity: the completed global table
+  int32_t num_ops;
+  int32_t pad;
+  Pass2Op ops[kMaxPass2];
 };
 
 }  // namespace mgb

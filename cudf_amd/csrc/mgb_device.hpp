@@ -45,6 +45,11 @@ enum AccOp : int32_t {
   OP_MIN_F32, OP_MAX_F32,  // u32 order-preserving codes
   OP_MIN_F64, OP_MAX_F64,  // u64 order-preserving codes
   OP_NULLS,       // u32  += 1 for every null value (the row count minus it is COUNT_VALID)
+  // second-pass fields, updated by a pass over the rows after the first pass
+  // has completed the group's SUM / MIN / MAX:
+  OP_M2,          // f64  += (x - mean)^2
+  OP_ARGMIN_IDX,  // u32  = min row index whose value equals the group's MIN
+  OP_ARGMAX_IDX,  // u32  = min row index whose value equals the group's MAX
   OP_NUM
 };
 
@@ -361,6 +366,30 @@ __device__ __forceinline__ void apply_op(uint8_t* rec, int kind, int off, uint64
   }
 }
 
+// The field value a MIN / MAX op of `kind` would take from this one value
+// (apply_op's operand): the group's field equals it exactly when the value is
+// the group's minimum / maximum in that op's order (NaN below every number for
+// MAX and above every number for MIN, -0.0 below +0.0).
+__device__ __forceinline__ uint64_t minmax_code(int kind, uint64_t raw, int width, int vclass)
+{
+  switch (kind) {
+    case OP_MIN_I32: case OP_MAX_I32: return (uint32_t)(int32_t)widen_int(raw, width, vclass);
+    case OP_MIN_U32: case OP_MAX_U32: return (uint32_t)widen_int(raw, width, vclass);
+    case OP_MIN_I64: case OP_MAX_I64: return (uint64_t)widen_int(raw, width, vclass);
+    case OP_MIN_U64: case OP_MAX_U64: return raw;
+    case OP_MIN_F32: case OP_MAX_F32: {
+      uint32_t b = (uint32_t)raw;
+      bool nan = (b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu);
+      return nan ? (kind == OP_MIN_F32 ? kF32MinNaN : kF32MaxNaN) : f32_code(b);
+    }
+    case OP_MIN_F64: case OP_MAX_F64: {
+      bool nan = (raw & 0x7ff0000000000000ull) == 0x7ff0000000000000ull && (raw & 0x000fffffffffffffull);
+      return nan ? (kind == OP_MIN_F64 ? kF64MinNaN : kF64MaxNaN) : f64_code(raw);
+    }
+    default: return 0ull;
+  }
+}
+
 // Identity value of an op's field, written by the table fill.
 __device__ __host__ __forceinline__ uint64_t op_identity(int kind)
 {
@@ -379,6 +408,7 @@ __device__ __host__ __forceinline__ uint64_t op_identity(int kind)
     case OP_MAX_F32: return kF32MaxIdentity;
     case OP_MIN_F64: return kF64MinIdentity;
     case OP_MAX_F64: return kF64MaxIdentity;
+    case OP_ARGMIN_IDX: case OP_ARGMAX_IDX: return 0xffffffffull;
     default: return 0ull;
   }
 }
@@ -387,7 +417,7 @@ __device__ __host__ __forceinline__ int op_field_bytes(int kind)
 {
   switch (kind) {
     case OP_ROWS: case OP_COUNT: case OP_MIN_I32: case OP_MAX_I32: case OP_MIN_U32: case OP_MAX_U32:
-    case OP_MIN_F32: case OP_MAX_F32: case OP_NULLS: return 4;
+    case OP_MIN_F32: case OP_MAX_F32: case OP_NULLS: case OP_ARGMIN_IDX: case OP_ARGMAX_IDX: return 4;
     default: return 8;
   }
 }

@@ -46,6 +46,9 @@ struct Query {
   std::string group_key;          // table and key columns: the cache key of its group-count estimate
   std::vector<std::vector<mgb_aggregation>> aggs;
   std::string signature;  // queries of one batch share it
+  // a second-pass aggregation (VARIANCE, STD, ARGMIN, ARGMAX): batches do
+  // not serve it, so the query runs alone on the plain path
+  bool plain_only = false;
   Clock::time_point arrival;
 };
 
@@ -334,7 +337,7 @@ bool mgb_server::launch(Slot& s)
     std::lock_guard<std::mutex> lk(mu);
     if (pending.empty()) return false;
     const std::string sig = pending.front().signature;
-    const int limit = opt.batching ? std::max(1, opt.max_batch_queries) : 1;
+    const int limit = (opt.batching && !pending.front().plain_only) ? std::max(1, opt.max_batch_queries) : 1;
     for (auto it = pending.begin(); it != pending.end() && (int)take.size() < limit;) {
       if (it->signature == sig) {
         take.push_back(std::move(*it));
@@ -344,8 +347,9 @@ bool mgb_server::launch(Slot& s)
       }
     }
   }
-  if (!opt.batching) {
-    // the per-query baseline: the plain path, synchronously on this slot's stream
+  if (!opt.batching || take[0].plain_only) {
+    // the per-query baseline (and the queries batches do not serve): the
+    // plain path, synchronously on this slot's stream
     Query& q = take[0];
     std::vector<mgb_column>& keys = q.keys;
     std::vector<mgb_request> reqs(q.vals.size());
@@ -605,7 +609,13 @@ extern "C" mgb_status mgb_server_submit(mgb_server* s, int32_t table_id, const i
     for (int r = 0; r < num_requests; ++r) {
       q.vals.push_back(t.cols[requests[r].column]);
       q.aggs.emplace_back(requests[r].aggs, requests[r].aggs + requests[r].num_aggs);
+      for (auto& a : q.aggs.back())
+        q.plain_only |= a.kind == MGB_AGG_VARIANCE || a.kind == MGB_AGG_STD || a.kind == MGB_AGG_ARGMIN ||
+                        a.kind == MGB_AGG_ARGMAX;
     }
+    // null keys grouped as a key (null_policy INCLUDE): plain path too
+    if (null_policy == MGB_NULL_INCLUDE)
+      for (auto& k : q.keys) q.plain_only |= k.null_mask != nullptr && k.null_count != 0;
     q.signature = signature_of(q);
     q.ticket = s->next_ticket++;
     *ticket = q.ticket;

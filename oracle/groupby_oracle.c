@@ -289,7 +289,10 @@ int mgo_target_type(int t, int kind)
       if (is_timestamp(t)) return -1;
       return T_FLOAT64;
     case K_M2: case K_VARIANCE: case K_STD:
-      if (is_timestamp(t)) return -1;
+      /* the target is double for every source (aggregation.hpp:1272-1289), but
+       * the hash path's functor accepts numeric sources only
+       * (multi_pass_kernels.cuh:59-71: other types reach CUDF_UNREACHABLE) */
+      if (!is_integral(t) && !is_float(t)) return -1;
       return T_FLOAT64;
     default: return -1;
   }
@@ -401,12 +404,22 @@ int mgo_groupby_aggregate(const mgo_col* keys, int32_t n_keys, int64_t num_rows,
           break;
         case K_MIN: case K_MAX: {
           int is_min = kind == K_MIN;
-          if (t == T_FLOAT32) {
-            float x = (float)load_f(t, v->data, r), o = a.f32[g];
-            a.f32[g] = is_min ? (o < x ? o : x) : (o > x ? o : x);
-          } else if (fl) {
-            double x = load_f(t, v->data, r), o = a.f[g];
-            a.f[g] = is_min ? (o < x ? o : x) : (o > x ? o : x);
+          if (fl) {
+            /* The reference folds old < x ? old : x (device_operators.cuh:117-123,
+             * fixed_point/temporary.hpp:63-72) in the order the atomics land, so a
+             * NaN stands until the next number arrives and tied zeros keep the
+             * last one. This oracle fixes the outcome of the order that brings the
+             * NaN rows first: NaN only when every valid value of the group is NaN,
+             * and -0.0 below +0.0. a.valid[g] marks a number seen. */
+            double x = load_f(t, v->data, r);
+            double o = t == T_FLOAT32 ? (double)a.f32[g] : a.f[g];
+            double res;
+            if (isnan(x)) res = a.valid[g] ? o : x;
+            else if (!a.valid[g]) { res = x; a.valid[g] = 1; }
+            else if (x == o) res = (is_min ? signbit(x) : !signbit(x)) ? x : o;
+            else res = is_min ? (x < o ? x : o) : (x > o ? x : o);
+            if (t == T_FLOAT32) a.f32[g] = (float)res;
+            else a.f[g] = res;
           } else {
             int64_t x = load_i(t, v->data, r);
             if (a.cnt[g] == 1) a.i[g] = x;
@@ -421,14 +434,21 @@ int mgo_groupby_aggregate(const mgo_col* keys, int32_t n_keys, int64_t num_rows,
           break;
         }
         case K_ARGMIN: case K_ARGMAX: {
-          /* row index of the min / max valid element; ties keep the smaller index
-           * (aggregation.cuh:356-394 compares values then row indices) */
+          /* row index of the min / max valid element. The reference's CAS loop
+           * (aggregation.cuh:436-490) keeps whichever tied row arrived first and,
+           * for floats, lets a NaN win only when it arrives first; this oracle
+           * fixes one of those outcomes: ties keep the smaller row index, NaN
+           * counts only when every valid value of the group is NaN, and -0.0
+           * orders below +0.0 (the order of the library's MIN / MAX). */
           int64_t cur = a.i[g];
           if (cur < 0) { a.i[g] = r; break; }
           int better;
           if (fl) {
             double x = load_f(t, v->data, r), o = load_f(t, v->data, cur);
-            better = kind == K_ARGMIN ? (x < o) : (x > o);
+            if (isnan(x)) better = 0;
+            else if (isnan(o)) better = 1;
+            else if (x == o) better = kind == K_ARGMIN ? (signbit(x) && !signbit(o)) : (!signbit(x) && signbit(o));
+            else better = kind == K_ARGMIN ? (x < o) : (x > o);
           } else if (is_unsigned_int(t)) {
             uint64_t x = (uint64_t)load_i(t, v->data, r), o = (uint64_t)load_i(t, v->data, cur);
             better = kind == K_ARGMIN ? (x < o) : (x > o);
@@ -472,7 +492,9 @@ int mgo_groupby_aggregate(const mgo_col* keys, int32_t n_keys, int64_t num_rows,
           double m2 = a.m2[g], n = (double)a.cnt[g];
           double res = kind == K_M2 ? m2 : m2 / (n - ddof);
           if (kind == K_STD) res = sqrt(res);
-          ok = kind == K_M2 ? has : (a.cnt[g] > ddof);
+          /* valid when the group has more valid values than ddof
+           * (multi_pass_kernels.cuh:80) */
+          ok = kind == K_M2 ? has : (has && a.cnt[g] > ddof);
           ((double*)out_data[j])[g] = ok ? res : 0.0;
           break;
         }

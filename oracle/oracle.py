@@ -213,6 +213,8 @@ def sort_order(key_cols) -> np.ndarray:
         v = np.asarray(vals)
         if v.dtype.kind == "f":
             v = np.where(np.isnan(v), np.inf, v)
+        if valid is not None:  # a null's payload is unspecified: nulls tie on it
+            v = np.where(np.asarray(valid, bool), v, np.zeros((), v.dtype))
         sort_keys.append(v)
         if valid is not None:
             sort_keys.append(np.asarray(valid, dtype=np.int8))

@@ -278,6 +278,41 @@ void multi_request_tests()
   CHECK(r2.second.size() == 2 && r2.second[0].results.size() == 2 && r2.second[1].results.size() == 1);
 }
 
+// argmax_tests.cpp:91-113, argmin_tests.cpp:94-117, var_tests.cpp:103-146,
+// std_tests.cpp:103-122 (the second pass over the rows)
+template <typename V>
+void second_pass_tests()
+{
+  wrapper<K> keys(cudf::type_id::INT32, kKeys);
+  std::vector<V> desc{9, 8, 7, 6, 5, 4, 3, 2, 1, 0};
+  wrapper<V> dv(tid<V>(), desc);
+  expect<K, int32_t>(run_single<K, int32_t>(keys.view(), dv.view(),
+                                            cudf::make_argmax_aggregation<cudf::groupby_aggregation>()),
+                     {1, 2, 3}, {0, 1, 2});
+  expect<K, int32_t>(run_single<K, int32_t>(keys.view(), dv.view(),
+                                            cudf::make_argmin_aggregation<cudf::groupby_aggregation>()),
+                     {1, 2, 3}, {6, 9, 8});
+  wrapper<K> ak(cudf::type_id::INT32, kNullKeys, {1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1});
+  std::vector<V> d11{9, 8, 7, 6, 5, 4, 3, 2, 1, 0, 4};
+  wrapper<V> av(tid<V>(), d11, {0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 0});
+  expect<K, int32_t>(run_single<K, int32_t>(ak.view(), av.view(),
+                                            cudf::make_argmax_aggregation<cudf::groupby_aggregation>()),
+                     {1, 2, 3, 4}, {3, 4, 7, 0}, {1, 1, 1, 0});
+  wrapper<K> nk(cudf::type_id::INT32, kNullKeys, kNullKeysValid);
+  auto v11 = iota_vals<V>(11);
+  v11[10] = 3;
+  wrapper<V> nv2(tid<V>(), v11, {0, 1, 1, 1, 1, 0, 1, 1, 1, 1, 1});
+  expect<K, double>(run_single<K, double>(nk.view(), nv2.view(),
+                                          cudf::make_variance_aggregation<cudf::groupby_aggregation>()),
+                    {1, 2, 3, 4}, {4.5, 49. / 3, 18., 0.}, {1, 1, 1, 0});
+  expect<K, double>(run_single<K, double>(nk.view(), nv2.view(),
+                                          cudf::make_variance_aggregation<cudf::groupby_aggregation>(2)),
+                    {1, 2, 3, 4}, {0., 98. / 3, 0., 0.}, {0, 1, 0, 0});
+  expect<K, double>(run_single<K, double>(nk.view(), nv2.view(),
+                                          cudf::make_std_aggregation<cudf::groupby_aggregation>()),
+                    {1, 2, 3, 4}, {3 / std::sqrt(2.), 7 / std::sqrt(3.), 3 * std::sqrt(2.), 0.}, {1, 1, 1, 0});
+}
+
 void error_tests()
 {
   // groupby.cu:208-212: size mismatch; :176-189: invalid type/aggregation
@@ -335,6 +370,8 @@ int main()
     {"mean<double>", mean_tests<double>},        {"minmax_count<int8>", minmax_count_tests<int8_t>},
     {"minmax_count<int32>", minmax_count_tests<int32_t>}, {"minmax_count<int64>", minmax_count_tests<int64_t>},
     {"minmax_count<float>", minmax_count_tests<float>},   {"minmax_count<double>", minmax_count_tests<double>},
+    {"second_pass<int32>", second_pass_tests<int32_t>},   {"second_pass<double>", second_pass_tests<double>},
+    {"second_pass<float>", second_pass_tests<float>},
     {"keys", keys_tests},                        {"multi_request", multi_request_tests},
     {"errors", error_tests},                     {"hashing", hashing_tests},
   };

@@ -14,7 +14,8 @@ TYPES = {"INT8": O.INT8, "INT16": O.INT16, "INT32": O.INT32, "INT64": O.INT64, "
          "UINT16": O.UINT16, "UINT32": O.UINT32, "UINT64": O.UINT64, "FLOAT32": O.FLOAT32,
          "FLOAT64": O.FLOAT64, "BOOL8": O.BOOL8}
 KINDS = {"SUM": O.SUM, "PRODUCT": O.PRODUCT, "MIN": O.MIN, "MAX": O.MAX, "COUNT_VALID": O.COUNT_VALID,
-         "COUNT_ALL": O.COUNT_ALL, "SUM_OF_SQUARES": O.SUM_OF_SQUARES, "MEAN": O.MEAN}
+         "COUNT_ALL": O.COUNT_ALL, "SUM_OF_SQUARES": O.SUM_OF_SQUARES, "MEAN": O.MEAN,
+         "VARIANCE": O.VARIANCE, "STD": O.STD, "ARGMAX": O.ARGMAX, "ARGMIN": O.ARGMIN}
 
 
 def _num(x):
@@ -49,7 +50,7 @@ def load_cases():
             exp_vals = np.array([_num(x) for x in c["expect_vals"]], dtype=np.float64).astype(O.NP[tt])
             exp_valid = np.array(c.get("expect_valid", [1] * len(c["expect_vals"])), bool)
             out.append(dict(
-                id=f"{c['name']}[{kt}->{vt}]", cite=c["cite"], kind=kind,
+                id=f"{c['name']}[{kt}->{vt}]", cite=c["cite"], kind=(kind, c["ddof"]) if "ddof" in c else kind,
                 include_null_keys=c.get("null_policy") == "INCLUDE",
                 keys=(k_t, keys, kvalid), vals=(v_t, vals, vvalid), target=tt,
                 expect_keys=(exp_keys, exp_keys_valid), expect=(exp_vals, exp_valid)))

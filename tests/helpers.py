@@ -16,8 +16,11 @@ TYPE_NAMES = {"INT8": O.INT8, "INT16": O.INT16, "INT32": O.INT32, "INT64": O.INT
               "FLOAT64": O.FLOAT64, "BOOL8": O.BOOL8}
 
 
-def to_agg(kind: int) -> ca.aggregation.Aggregation:
-    return ca.aggregation.Aggregation(ca.Kind(kind))
+def to_agg(kind) -> ca.aggregation.Aggregation:
+    """kind, or (kind, ddof) as the oracle takes it."""
+    if isinstance(kind, tuple):
+        return ca.aggregation.Aggregation(ca.Kind(kind[0]), kind[1])
+    return ca.aggregation.Aggregation(ca.Kind(kind), 1 if kind in (O.VARIANCE, O.STD) else 0)
 
 
 def gpu_groupby(keys, requests, include_null_keys=False, device="cuda"):

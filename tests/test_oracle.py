@@ -155,14 +155,22 @@ def test_int32_sum_widens_to_int64():
     assert res[0][0][0][0] == 3 * (2 ** 31 - 1)
 
 
-def test_float_min_follows_sequential_cas_semantics():
-    # min(old, x) = old < x ? old : x applied in row order from +inf
+def test_float_min_is_an_outcome_of_the_reference_fold():
+    """cudf folds min(old, x) = old < x ? old : x from +inf in the order its
+    atomics land; the oracle's MIN is that fold over the NaN rows first."""
+    def fold(xs):
+        old = np.inf
+        for x in xs:
+            old = old if old < x else x
+        return old
+
     keys = np.zeros(3, np.int32)
-    v = np.array([1.0, np.nan, 0.5])
-    _, res = O.groupby([O.HostColumn(O.INT32, keys)], [(O.HostColumn(O.FLOAT64, v), [O.MIN])])
-    assert res[0][0][0][0] == 0.5
-    _, res = O.groupby([O.HostColumn(O.INT32, keys)], [(O.HostColumn(O.FLOAT64, np.array([1.0, 2.0, np.nan])), [O.MIN])])
-    assert math.isnan(res[0][0][0][0])
+    for v in ([1.0, np.nan, 0.5], [1.0, 2.0, np.nan], [np.nan, np.nan, np.nan], [np.nan, 3.0, -2.0]):
+        v = np.array(v)
+        _, res = O.groupby([O.HostColumn(O.INT32, keys)], [(O.HostColumn(O.FLOAT64, v), [O.MIN])])
+        want = fold(np.concatenate([v[np.isnan(v)], v[~np.isnan(v)]]))
+        got = res[0][0][0][0]
+        assert (math.isnan(got) and math.isnan(want)) or got == want, (v, got, want)
 
 
 def test_target_types_closed_form():
@@ -206,3 +214,123 @@ def test_oracle_vs_pandas(seed):
                 assert (err <= tol * np.maximum(np.abs(b), scale)).all()
             else:
                 assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- second-pass aggregations
+
+
+def _tiny_and_large_groups(rng, n):
+    """int32 keys: about 300 groups of ~17 rows plus groups of 1 and 2 rows."""
+    keys = rng.integers(0, 300, n).astype(np.int32)
+    keys[:20] = np.repeat(np.arange(1000, 1010, dtype=np.int32), 2)
+    keys[20:30] = np.arange(2000, 2010, dtype=np.int32)
+    return keys
+
+
+@pytest.mark.parametrize("vt", [O.INT8, O.INT32, O.INT64, O.UINT16, O.FLOAT32, O.FLOAT64])
+@pytest.mark.parametrize("ddof", [0, 1, 2, 7])
+def test_oracle_variance_std_against_numpy(vt, ddof):
+    """VARIANCE / STD per group = numpy.var / numpy.std with the same ddof;
+    valid exactly when the group has more valid values than ddof."""
+    rng = np.random.default_rng(40 + vt * 10 + ddof)
+    n = 5000
+    keys = _tiny_and_large_groups(rng, n)
+    if vt in (O.FLOAT32, O.FLOAT64):
+        vals = (rng.standard_normal(n) * 100 + 7).astype(O.NP[vt])
+    else:
+        info = np.iinfo(O.NP[vt])
+        vals = rng.integers(max(info.min, -10 ** 6), min(info.max, 10 ** 6), n, endpoint=True).astype(O.NP[vt])
+    valid = rng.random(n) > 0.2
+    ks, res = O.groupby([O.HostColumn(O.INT32, keys)],
+                        [(O.HostColumn(vt, vals, valid), [(O.VARIANCE, ddof), (O.STD, ddof)])])
+    (var, var_ok), (std, std_ok) = res[0]
+    for g, k in enumerate(ks[0][0]):
+        x = vals[(keys == k) & valid].astype(np.float64)
+        ok = len(x) > 0 and len(x) > ddof
+        assert bool(var_ok[g]) == ok and bool(std_ok[g]) == ok, (k, len(x), ddof)
+        if not ok:
+            continue
+        # float32 sources take the mean from a float32 sum, as cudf does
+        rtol = 1e-9 if vt == O.FLOAT32 else 1e-12
+        scale = float(np.mean(x * x)) * len(x) / (len(x) - ddof)
+        assert abs(var[g] - np.var(x, ddof=ddof)) <= rtol * scale, (k, var[g], np.var(x, ddof=ddof))
+        assert abs(std[g] - np.std(x, ddof=ddof)) <= rtol * np.sqrt(scale) + 1e-300, (k, std[g])
+
+
+def _brute_arg(x, rows, is_min, is_float):
+    """Set-based ARGMIN / ARGMAX: the smallest row index among the rows holding
+    the extreme value; NaN only when every value is NaN; -0.0 below +0.0."""
+    if is_float:
+        nan = np.isnan(x)
+        if nan.all():
+            return rows[0]
+        x, rows = x[~nan], rows[~nan]
+    best = x.min() if is_min else x.max()
+    cand = x == best
+    if is_float and best == 0:
+        neg = np.signbit(x) & cand
+        pos = ~np.signbit(x) & cand
+        if is_min and neg.any():
+            cand = neg
+        elif not is_min and pos.any():
+            cand = pos
+    return rows[cand].min()
+
+
+@pytest.mark.parametrize("vt", [O.INT8, O.INT32, O.UINT32, O.INT64, O.UINT64, O.FLOAT32, O.FLOAT64, O.BOOL8])
+def test_oracle_argmin_argmax_brute_force(vt):
+    rng = np.random.default_rng(60 + vt)
+    n = 4000
+    keys = _tiny_and_large_groups(rng, n)
+    if vt in (O.FLOAT32, O.FLOAT64):
+        pool = np.array([-np.inf, -1.5, -0.0, 0.0, 2.5, np.inf, np.nan], dtype=O.NP[vt])
+        vals = pool[rng.integers(0, len(pool), n)]
+        vals[keys == 1003] = np.nan  # an all-NaN group
+    elif vt == O.BOOL8:
+        vals = rng.integers(0, 2, n).astype(np.uint8)
+    else:
+        info = np.iinfo(O.NP[vt])
+        pool = np.array([info.min, info.min + 1, 0, 1, 2, info.max], dtype=O.NP[vt])  # many ties
+        vals = pool[rng.integers(0, len(pool), n)]
+    valid = rng.random(n) > 0.15
+    ks, res = O.groupby([O.HostColumn(O.INT32, keys)], [(O.HostColumn(vt, vals, valid), [O.ARGMIN, O.ARGMAX])])
+    (amin, amin_ok), (amax, amax_ok) = res[0]
+    rows = np.arange(n)
+    is_float = vt in (O.FLOAT32, O.FLOAT64)
+    for g, k in enumerate(ks[0][0]):
+        sel = (keys == k) & valid
+        assert bool(amin_ok[g]) == sel.any() and bool(amax_ok[g]) == sel.any()
+        if not sel.any():
+            continue
+        assert amin[g] == _brute_arg(vals[sel], rows[sel], True, is_float), (k, amin[g])
+        assert amax[g] == _brute_arg(vals[sel], rows[sel], False, is_float), (k, amax[g])
+
+
+@pytest.mark.parametrize("vt", [O.FLOAT32, O.FLOAT64])
+def test_oracle_float_min_max_nan_and_zero(vt):
+    """MIN / MAX over floats: numpy's nanmin / nanmax per group, NaN for a
+    group of NaN only, and the signed zero -0.0 for MIN / +0.0 for MAX where
+    both zeros are the extreme."""
+    rng = np.random.default_rng(80 + vt)
+    n = 4000
+    keys = _tiny_and_large_groups(rng, n)
+    pool = np.array([-np.inf, -1.5, -0.0, 0.0, 2.5, np.inf, np.nan], dtype=O.NP[vt])
+    vals = pool[rng.integers(0, len(pool), n)]
+    vals[keys == 1004] = np.nan
+    valid = rng.random(n) > 0.1
+    ks, res = O.groupby([O.HostColumn(O.INT32, keys)], [(O.HostColumn(vt, vals, valid), [O.MIN, O.MAX])])
+    (mn, mn_ok), (mx, mx_ok) = res[0]
+    for g, k in enumerate(ks[0][0]):
+        x = vals[(keys == k) & valid]
+        assert bool(mn_ok[g]) == (len(x) > 0) and bool(mx_ok[g]) == (len(x) > 0)
+        if not len(x):
+            continue
+        if np.isnan(x).all():
+            assert np.isnan(mn[g]) and np.isnan(mx[g])
+            continue
+        lo, hi = np.nanmin(x), np.nanmax(x)
+        assert mn[g] == lo and mx[g] == hi, (k, mn[g], mx[g], lo, hi)
+        if lo == 0:
+            assert np.signbit(mn[g]) == np.signbit(x[x == 0]).any()
+        if hi == 0:
+            assert np.signbit(mx[g]) == np.signbit(x[x == 0]).all()
