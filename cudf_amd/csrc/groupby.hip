@@ -72,15 +72,15 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const AggParams p)
   }
 }
 
-// Second pass over the rows (VARIANCE / STD / ARGMIN / ARGMAX fields).
-__global__ __launch_bounds__(kBlock) void pass2_kernel(const Pass2Params q)
+// Second pass over the rows (VARIANCE / STD / ARGMIN / ARGMAX fields) on
+// the global table of the global execution.
+__global__ __launch_bounds__(kBlock) void pass2_kernel(const AggParams p, const Pass2Ops* __restrict__ q)
 {
-  const AggParams& p = q.src;
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
   for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < p.num_rows; row += step) {
     if (!key_row_valid(p, row)) continue;
     uint8_t* rec = global_find(p, row_key(p, row), row);
-    if (rec) pass2_row(q, rec, row);
+    if (rec) pass2_update<false>(q, rec, row, 0u, (uint32_t)row);
   }
 }
 
@@ -460,6 +460,7 @@ __device__ __forceinline__ void write_record(const PartParams& pp, uint8_t* rec,
     if (v.flag_bit >= 0 && bit_valid(v.mask, v.bit_off + row)) flags |= 1u << v.flag_bit;
   }
   if (pp.flag_bytes) store_width(rec + pp.flags_off, 0, pp.flag_bytes, flags);
+  if (pp.rowid_off >= 0) store_width(rec + pp.rowid_off, 0, 4, (uint64_t)row);
 }
 
 // Copies every row with a valid key to its partition's range of this
@@ -578,12 +579,43 @@ __global__ __launch_bounds__(kBlock) void count_items_kernel(const uint32_t* off
   count[p] = (uint32_t)((e - b + max_item_rows - 1) / max_item_rows);
 }
 
+// The rows of work item `b` of the partitioned mode: [begin, end) of the
+// partitioned copy; `merge` when its partition is split over several items,
+// which are the `fold_parts` items of that partition (this one is
+// `fold_part`). False when `b` is past the last item.
+__device__ __forceinline__ bool part_item(const LdsParams& lp, uint32_t b, int64_t& begin, int64_t& end, bool& merge,
+                                          int& fold_part, int& fold_parts)
+{
+  const int P = 1 << lp.log2p;
+  if (b >= lp.item_offsets[P]) return false;  // the grid is an upper bound on the items
+  int lo = 0, hi = P - 1;                    // last partition whose first item is <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (lp.item_offsets[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const int part = lo;
+  const uint32_t n_items = lp.item_offsets[part + 1] - lp.item_offsets[part];
+  const int64_t ps = lp.offsets[(int64_t)part * lp.nblocks];
+  const int64_t pe = part + 1 < P ? (int64_t)lp.offsets[(int64_t)(part + 1) * lp.nblocks] : (int64_t)*lp.total_rows;
+  begin = ps + (int64_t)(b - lp.item_offsets[part]) * lp.max_item_rows;
+  end = begin + lp.max_item_rows < pe ? begin + lp.max_item_rows : pe;
+  merge = n_items > 1;
+  fold_part = (int)(b - lp.item_offsets[part]);
+  fold_parts = (int)n_items;
+  return true;
+}
+
 // One workgroup aggregates one work item in an LDS table. Rows whose group
 // does not fit the table go to the global table. At the end the item either
 // writes its groups as final output rows (a partition handled by this item
 // alone, without spill), or folds them into the global table (direct mode,
 // partitions split over several items, or an item that spilled), whose
-// groups are finalised afterwards.
+// groups are finalised afterwards. kPass2 (partitioned mode): an item that
+// writes its groups first runs the second pass over its rows against its LDS
+// table; every item records whether its groups went to the global table
+// (their second pass is pass2_items_kernel's).
+template <bool kPass2>
 __global__ __launch_bounds__(kLdsBlock) void lds_agg_kernel(const LdsParams lp)
 {
   extern __shared__ __align__(16) uint8_t lds[];
@@ -595,32 +627,55 @@ __global__ __launch_bounds__(kLdsBlock) void lds_agg_kernel(const LdsParams lp)
   if (lp.direct) {
     begin = (int64_t)blockIdx.x * lp.rows_per_item;
     end = begin + lp.rows_per_item < p.num_rows ? begin + lp.rows_per_item : p.num_rows;
-  } else {
-    const int P = 1 << lp.log2p;
-    const uint32_t b = blockIdx.x;
-    if (b >= lp.item_offsets[P]) return;  // the grid is an upper bound on the items
-    int lo = 0, hi = P - 1;                // last partition whose first item is <= b
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (lp.item_offsets[mid] <= b) lo = mid;
-      else hi = mid - 1;
-    }
-    const int part = lo;
-    const uint32_t n_items = lp.item_offsets[part + 1] - lp.item_offsets[part];
-    const int64_t ps = lp.offsets[(int64_t)part * lp.nblocks];
-    const int64_t pe = part + 1 < P ? (int64_t)lp.offsets[(int64_t)(part + 1) * lp.nblocks] : (int64_t)*lp.total_rows;
-    begin = ps + (int64_t)(b - lp.item_offsets[part]) * lp.max_item_rows;
-    end = begin + lp.max_item_rows < pe ? begin + lp.max_item_rows : pe;
-    merge = n_items > 1;
-    fold_part = (int)(b - lp.item_offsets[part]);
-    fold_parts = (int)n_items;
+  } else if (!part_item(lp, blockIdx.x, begin, end, merge, fold_part, fold_parts)) {
+    return;
   }
 
   const LdsTable t = lds_table_at(lds, lp.slots, lp.words, lp.cap);
   lds_table_init(t, lp.tmpl, lp.words);
   lds_aggregate_rows(p, t, lds, begin, end, lp.log2p, lp.direct != 0);
-  if (merge || *t.overflow) lds_fold_to_global(p, t, fold_part, fold_parts);
+  const bool fold = merge || *t.overflow;
+  if constexpr (kPass2) {
+    if (threadIdx.x == 0) lp.p2->item_global[blockIdx.x] = fold ? 1u : 0u;
+    if (!fold) lds_pass2_rows(lp.p2->rows, lp.p2, t, begin, end, lp.log2p, false);
+  }
+  if (fold) lds_fold_to_global(p, t, fold_part, fold_parts);
   else lds_write_groups(p, t, &lp.out_counters[0], 0, *lp.out);
+}
+
+// Second pass of the partitioned items whose groups went to the global
+// table (split partitions, spilled items), once every item has folded.
+__global__ __launch_bounds__(kLdsBlock) void pass2_items_kernel(const LdsParams lp)
+{
+  const AggParams& p = lp.src;
+  int64_t begin, end;
+  bool merge;
+  int fold_part, fold_parts;
+  if (!part_item(lp, blockIdx.x, begin, end, merge, fold_part, fold_parts)) return;
+  const Pass2Ops* __restrict__ q = lp.p2;
+  if (!q->item_global[blockIdx.x]) return;
+  for (int64_t row = begin + threadIdx.x; row < end; row += kLdsBlock) {
+    const uint32_t flags = p.flags ? load_flags(p.flags, p.flag_bytes, p.flag_stride, row) : 0u;
+    const uint32_t src = q->rowid ? *reinterpret_cast<const uint32_t*>(q->rowid + row * (int64_t)q->rowid_stride) : 0u;
+    uint8_t* g = global_find(p, row_key(p, row), row);
+    if (g) pass2_update<false>(q, g, row, flags, src);
+  }
+}
+
+// Second pass of the direct mode, once the global table holds every group:
+// each workgroup copies the groups into its LDS table, runs the second pass
+// over its slice of rows there, and folds the second-pass fields back.
+__global__ __launch_bounds__(kLdsBlock) void pass2_direct_kernel(const LdsParams lp)
+{
+  extern __shared__ __align__(16) uint8_t lds[];
+  const AggParams& p = lp.src;
+  const int64_t begin = (int64_t)blockIdx.x * lp.rows_per_item;
+  const int64_t end = begin + lp.rows_per_item < p.num_rows ? begin + lp.rows_per_item : p.num_rows;
+  const LdsTable t = lds_table_at(lds, lp.slots, lp.words, lp.cap);
+  lds_table_init(t, lp.tmpl, lp.words);
+  lds_load_global(p, t);
+  lds_pass2_rows(lp.p2->rows, lp.p2, t, begin, end, 0, true);
+  lds_fold_pass2(lp.p2->rows, lp.p2, t);
 }
 
 // ---- table hashing entry points
@@ -1124,6 +1179,22 @@ mgb_status upload_params(Workspace* ws, const OutTable& ot, const uint64_t* tmpl
   return MGB_SUCCESS;
 }
 
+// Copies the ops of a second pass into a parameter slot of device memory.
+mgb_status upload_pass2(Workspace* ws, const Pass2Ops& q, hipStream_t stream, const Pass2Ops** d)
+{
+  const size_t slot_bytes = kStagingBytes / kUploadSlots;
+  static_assert(sizeof(Pass2Ops) <= kStagingBytes / kUploadSlots, "parameter slot too small");
+  if (ws->upload_slot >= kUploadSlots) return fail(MGB_ERROR_UNSUPPORTED, "too many parameter uploads in one call");
+  const int slot = ws->upload_slot++;
+  MGB_HIP(ws->params.reserve(kStagingBytes, stream));
+  uint8_t* host = ws->staging + slot * slot_bytes;
+  uint8_t* dev = static_cast<uint8_t*>(ws->params.ptr) + slot * slot_bytes;
+  std::memcpy(host, &q, sizeof(Pass2Ops));
+  MGB_HIP(hipMemcpyAsync(dev, host, sizeof(Pass2Ops), hipMemcpyHostToDevice, stream));
+  *d = reinterpret_cast<const Pass2Ops*>(dev);
+  return MGB_SUCCESS;
+}
+
 // Allocates the caller's columns for `groups` rows, copies the temporary
 // outputs into them and builds their null masks.
 mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int64_t groups, mgb_out_column* out_keys,
@@ -1249,6 +1320,44 @@ bool single_launch_ops(const Plan& plan, AggParams& ap)
   ap.program_cols = ap.num_vals <= kProgramCols;
   for (int c = 0; c < ap.num_vals; ++c)
     if (ap.vals[c].lds_generic) ap.program_cols = 0;
+  return true;
+}
+
+// Second-pass op of field `o` of plan column `pc`, over the value column `v`
+// of the rows (its `val`-th).
+Pass2Op pass2_op(const Plan& plan, const PlanCol& pc, const std::pair<int, int>& o, const ValCol& v, int val)
+{
+  Pass2Op op{};
+  op.data = v.data;
+  op.mask = v.mask;
+  op.bit_off = v.bit_off;
+  op.width = v.width;
+  op.vclass = v.vclass;
+  op.stride = v.stride;
+  op.flag_bit = v.mask ? -1 : v.flag_bit;
+  op.kind = o.first;
+  op.val = val;
+  op.off = o.second;
+  op.ref_kind = o.first == OP_M2 ? sum_op_for(pc.type) : minmax_op_for(pc.type, o.first == OP_ARGMIN_IDX);
+  op.ref_off = pc.op_offset(op.ref_kind);
+  op.count_off = plan.rows_off;
+  op.nulls_off = pc.nulls_off;
+  return op;
+}
+
+// The second-pass ops of a plan over `rows`, whose value column c is
+// plan.cols[c] (as single_launch_ops lays them out); false when they exceed
+// one launch.
+bool pass2_ops(const Plan& plan, const AggParams& rows, Pass2Ops& q)
+{
+  q.num_ops = 0;
+  for (size_t c = 0; c < plan.cols.size(); ++c) {
+    const PlanCol& pc = plan.cols[c];
+    for (auto& o : pc.p2) {
+      if (q.num_ops == kMaxPass2) return false;
+      q.ops[q.num_ops++] = pass2_op(plan, pc, o, rows.vals[c], (int)c);
+    }
+  }
   return true;
 }
 
@@ -1421,6 +1530,8 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
   capacity = std::min<int64_t>(capacity, std::max<int64_t>(2 * R + 64, 64));
   int64_t nblocks = 0;
   size_t block_off = 0;
+  LdsParams direct_lp{};  // the direct launch, for its second pass
+  int64_t direct_items = 0;
   for (int attempt = 0;; ++attempt) {
     if (capacity > (int64_t)0x7fffffff) return fail(MGB_ERROR_UNSUPPORTED, "groupby table would exceed 2^31 records");
     nblocks = (capacity + 1 + kBlock - 1) / kBlock;
@@ -1473,13 +1584,15 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
       lp.words = lds_stride_of(plan.stride) / 8;
       lp.direct = 1;
       lp.log2p = 0;
-      MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+      MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)geom.bytes));
       {
         ProfScope prof("lds_aggregate", stream);
-        lds_agg_kernel<<<(unsigned)items, kLdsBlock, geom.bytes, stream>>>(lp);
+        lds_agg_kernel<false><<<(unsigned)items, kLdsBlock, geom.bytes, stream>>>(lp);
       }
       MGB_HIP(hipGetLastError());
+      direct_lp = lp;
+      direct_items = items;
     } else {
       // launch in batches of value columns / ops
       size_t ci = 0;
@@ -1533,42 +1646,51 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
     }
     if (attempt >= 6) return fail(MGB_ERROR_HIP, "groupby table kept overflowing");
     capacity = std::min<int64_t>(capacity * 4, std::max<int64_t>(2 * R + 64, 64));
+    // the attempt's kernels have completed (the counters were read): its
+    // parameter slot is free again
+    if (direct) ws->upload_slot = std::max(0, ws->upload_slot - 1);
   }
 
-  if (plan.has_pass2 && groups > 0) {
+  if (plan.has_pass2 && groups > 0 && direct) {
+    // the table is complete: every workgroup loads it into LDS for the
+    // second pass over its slice
+    LdsParams lp = direct_lp;
+    Pass2Ops q{};
+    if (!pass2_ops(plan, lp.src, q)) return fail(MGB_ERROR_UNSUPPORTED, "too many second-pass aggregations");
+    q.rows = lp.src;
+    mgb_status st = upload_pass2(ws, q, stream, &lp.p2);
+    if (st != MGB_SUCCESS) return st;
+    MGB_HIP(hipFuncSetAttribute((const void*)pass2_direct_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)geom.bytes));
+    {
+      ProfScope prof("pass2", stream);
+      pass2_direct_kernel<<<(unsigned)direct_items, kLdsBlock, geom.bytes, stream>>>(lp);
+    }
+    MGB_HIP(hipGetLastError());
+  } else if (plan.has_pass2 && groups > 0) {
     // the table is complete: each row finds its group and updates the
-    // second-pass fields, in launches of up to kMaxVals columns
-    Pass2Params q{};
-    q.src = base;
-    q.src.table = static_cast<uint8_t*>(ws->main.ptr);
-    q.src.capacity = capacity;
-    q.src.flags = nullptr;
-    size_t ci = 0;
-    for (;;) {
-      q.src.num_vals = 0;
-      q.num_ops = 0;
-      for (; ci < plan.cols.size(); ++ci) {
-        const PlanCol& pc = plan.cols[ci];
-        if (pc.p2.empty()) continue;
-        if (q.src.num_vals == kMaxVals || q.num_ops + (int)pc.p2.size() > kMaxPass2) break;
-        const int vi = q.src.num_vals++;
-        q.src.vals[vi] = pc.col;
-        for (auto& o : pc.p2) {
-          Pass2Op& op = q.ops[q.num_ops++];
-          op.kind = o.first;
-          op.val = vi;
-          op.off = o.second;
-          op.ref_kind = o.first == OP_M2 ? sum_op_for(pc.type) : minmax_op_for(pc.type, o.first == OP_ARGMIN_IDX);
-          op.ref_off = pc.op_offset(op.ref_kind);
-          op.count_off = plan.rows_off;
-          op.nulls_off = pc.nulls_off;
-        }
-      }
-      if (q.num_ops == 0) break;
+    // second-pass fields, in launches of up to kMaxPass2 ops
+    AggParams src = base;
+    src.table = static_cast<uint8_t*>(ws->main.ptr);
+    src.capacity = capacity;
+    src.flags = nullptr;
+    std::vector<Pass2Op> all;
+    for (size_t c = 0; c < plan.cols.size(); ++c) {
+      ValCol v = plan.cols[c].col;
+      v.flag_bit = -1;
+      for (auto& o : plan.cols[c].p2) all.push_back(pass2_op(plan, plan.cols[c], o, v, (int)c));
+    }
+    for (size_t first = 0; first < all.size(); first += kMaxPass2) {
+      Pass2Ops q{};
+      q.num_ops = (int)std::min<size_t>(kMaxPass2, all.size() - first);
+      for (int i = 0; i < q.num_ops; ++i) q.ops[i] = all[first + i];
+      const Pass2Ops* d_q = nullptr;
+      mgb_status st = upload_pass2(ws, q, stream, &d_q);
+      if (st != MGB_SUCCESS) return st;
       const int64_t grid = std::min<int64_t>((R + kBlock - 1) / kBlock, (int64_t)cus * 16);
       {
         ProfScope prof("pass2", stream);
-        pass2_kernel<<<(unsigned)std::max<int64_t>(grid, 1), kBlock, 0, stream>>>(q);
+        pass2_kernel<<<(unsigned)std::max<int64_t>(grid, 1), kBlock, 0, stream>>>(src, d_q);
       }
       MGB_HIP(hipGetLastError());
     }
@@ -1649,14 +1771,19 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     const int64_t gblocks = (gcap + 1 + kBlock - 1) / kBlock;
 
     // record layout of a partitioned row: 8-byte fields, then 4, 2 and 1-byte
-    // fields, then the validity bits; the size is a multiple of 8
+    // fields (the source row index among the 4-byte ones, kept for ARGMIN /
+    // ARGMAX), then the validity bits; the size is a multiple of 8
+    bool need_rowid = false;
+    for (const PlanCol& pc : plan.cols)
+      for (auto& o : pc.p2) need_rowid |= o.first != OP_M2;
     std::vector<int> koff(plan.keys.size()), voff(src.num_vals);
-    int rec = 0;
+    int rec = 0, rowid_off = -1;
     for (int w : {8, 4, 2, 1}) {
       for (size_t c = 0; c < plan.keys.size(); ++c)
         if (plan.keys[c].width == w) { koff[c] = rec; rec += w; }
       for (int c = 0; c < src.num_vals; ++c)
         if (src.vals[c].width == w) { voff[c] = rec; rec += w; }
+      if (w == 4 && need_rowid) { rowid_off = rec; rec += 4; }
     }
     const int flags_rec_off = (int)align_up((size_t)rec, (size_t)std::max(flag_bytes, 1));
     rec = flags_rec_off + flag_bytes;
@@ -1678,6 +1805,8 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     off += align_up((size_t)(P + 1) * 4, 256);
     const size_t ibsum_off = off;
     off += align_up((size_t)nb_items * 4, 256);
+    const size_t iglob_off = off;  // second pass: per item, its groups went global
+    off += plan.has_pass2 ? align_up((size_t)item_grid * 4, 256) : 0;
     const size_t gtab_off = off;
     off += align_up((size_t)(gcap + 1) * plan.stride, 256);
     const size_t gblk_off = off;
@@ -1721,6 +1850,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     pp.flags_off = flags_rec_off;
     for (size_t c = 0; c < plan.keys.size(); ++c) pp.key_off[c] = koff[c];
     for (int c = 0; c < src.num_vals; ++c) pp.val_off[c] = voff[c];
+    pp.rowid_off = rowid_off;
     const size_t part_lds = (size_t)P * 4;
     MGB_HIP(hipFuncSetAttribute((const void*)part_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)part_lds));
@@ -1826,13 +1956,31 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     lp.direct = 0;
     lp.item_offsets = ioff;
     lp.max_item_rows = max_item_rows;
-    MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)geom.bytes));
+    if (plan.has_pass2) {
+      Pass2Ops q{};
+      if (!pass2_ops(plan, lp.src, q)) return MGB_SUCCESS;  // the global execution takes it
+      q.rows = lp.src;
+      q.rowid = need_rowid ? recs + rowid_off : nullptr;
+      q.rowid_stride = rec_bytes;
+      q.item_global = reinterpret_cast<uint32_t*>(wsb + iglob_off);
+      st = upload_pass2(ws, q, stream, &lp.p2);
+      if (st != MGB_SUCCESS) return st;
+    }
+    const void* agg_fn = plan.has_pass2 ? (const void*)lds_agg_kernel<true> : (const void*)lds_agg_kernel<false>;
+    MGB_HIP(hipFuncSetAttribute(agg_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)geom.bytes));
     {
       ProfScope prof("lds_aggregate", stream);
-      lds_agg_kernel<<<(unsigned)item_grid, kLdsBlock, geom.bytes, stream>>>(lp);
+      if (plan.has_pass2) lds_agg_kernel<true><<<(unsigned)item_grid, kLdsBlock, geom.bytes, stream>>>(lp);
+      else lds_agg_kernel<false><<<(unsigned)item_grid, kLdsBlock, geom.bytes, stream>>>(lp);
     }
     MGB_HIP(hipGetLastError());
+    if (plan.has_pass2) {
+      // the items whose groups went to the global table: their second pass
+      // once the table is complete
+      ProfScope prof("pass2", stream);
+      pass2_items_kernel<<<(unsigned)item_grid, kLdsBlock, 0, stream>>>(lp);
+      MGB_HIP(hipGetLastError());
+    }
 
     // finalise the global table's groups after the items' rows
     int64_t* gblk = reinterpret_cast<int64_t*>(wsb + gblk_off);
@@ -1902,11 +2050,11 @@ mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mg
 
   const LdsGeom geom = lds_geometry(plan.stride);
   AggParams probe = base;
+  Pass2Ops probe2{};
   const bool lds_ok = geom.ok && single_launch_ops(plan, probe) && !plan.compare_key_nulls &&
-                      (int)plan.outs.size() <= kMaxOuts;
+                      (int)plan.outs.size() <= kMaxOuts && pass2_ops(plan, probe, probe2);
   Path path = forced_path();
-  // the second pass finds each row's group in the global table
-  if (!lds_ok || plan.has_pass2) path = Path::Global;
+  if (!lds_ok) path = Path::Global;
   if (path == Path::Auto) path = (G_est <= geom.cap / 2) ? Path::Direct : Path::Partitioned;
 
   TempOuts temp;

@@ -177,44 +177,6 @@ __device__ __forceinline__ uint8_t* global_find(const AggParams& p, const RowKey
   return nullptr;
 }
 
-// The second-pass ops of one row on its group's record (global table, whose
-// first-pass fields are final).
-//  OP_M2: adds (x - mean)^2 with mean = SUM / COUNT_VALID, the SUM of a
-//    float32 column taken at float32 precision as its SUM result is
-//    (reference: cpp/src/groupby/hash/multi_pass_kernels.cuh:76-90).
-//  OP_ARGMIN_IDX / OP_ARGMAX_IDX: the smallest row index whose value is the
-//    group's MIN / MAX (the reference's CAS loop in
-//    cpp/include/cudf/detail/aggregation/aggregation.cuh:436-490 returns one
-//    of the tied rows; this is the first of them).
-__device__ __forceinline__ void pass2_row(const Pass2Params& q, uint8_t* rec, int64_t row)
-{
-  const AggParams& p = q.src;
-  for (int i = 0; i < q.num_ops; ++i) {
-    const Pass2Op& o = q.ops[i];
-    const ValCol& v = p.vals[o.val];
-    if (!bit_valid(v.mask, v.bit_off + row)) continue;
-    const uint64_t raw = load_at(v.data, v.width, v.stride, row);
-    if (o.kind == OP_M2) {
-      uint32_t cnt = *reinterpret_cast<const uint32_t*>(rec + o.count_off);
-      if (o.nulls_off >= 0) cnt -= *reinterpret_cast<const uint32_t*>(rec + o.nulls_off);
-      double sum;
-      if (o.ref_kind == OP_SUM_I64) {
-        sum = (double)*reinterpret_cast<const int64_t*>(rec + o.ref_off);
-      } else {
-        sum = *reinterpret_cast<const double*>(rec + o.ref_off);
-        if (v.vclass == VC_F32) sum = (double)(float)sum;
-      }
-      const double d = widen_f64(raw, v.width, v.vclass) - sum / (double)cnt;
-      atom_add(reinterpret_cast<double*>(rec + o.off), d * d);
-    } else {
-      const uint64_t code = minmax_code(o.ref_kind, raw, v.width, v.vclass);
-      const uint64_t best = op_field_bytes(o.ref_kind) == 8 ? *reinterpret_cast<const uint64_t*>(rec + o.ref_off)
-                                                            : *reinterpret_cast<const uint32_t*>(rec + o.ref_off);
-      if (code == best) atom_min(reinterpret_cast<uint32_t*>(rec + o.off), (uint32_t)row);
-    }
-  }
-}
-
 // All ops of one row on a record (global or LDS).
 __device__ __forceinline__ void apply_row(const AggParams& p, uint8_t* rec, int64_t row)
 {
@@ -355,6 +317,32 @@ __device__ __forceinline__ uint8_t* lds_find_or_insert(const LdsTable& t, const 
   return nullptr;
 }
 
+// Record of the row's group in an LDS table that holds it (a probe without
+// insert); nullptr when the key is not in the table.
+__device__ __forceinline__ uint8_t* lds_find(const LdsTable& t, const AggParams& p, const RowKey& rk,
+                                             uint32_t slot_hash, int64_t row)
+{
+  if (rk.special) {
+    uint8_t* rec = lds_record(t, t.slots);
+    return *reinterpret_cast<const uint64_t*>(rec) == 1ull ? rec : nullptr;
+  }
+  const uint64_t empty = p.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
+  int32_t idx = (int32_t)reduce_range(slot_hash, t.slots);
+  for (int32_t probe = 0; probe < t.slots; ++probe) {
+    uint8_t* rec = lds_record(t, idx);
+    const uint64_t cur = *reinterpret_cast<const uint64_t*>(rec);
+    if (cur == empty) return nullptr;
+    if (p.key_mode == KEY_INT) {
+      if (cur == rk.key) return rec;
+    } else if ((uint32_t)(cur >> 32) == rk.hash &&
+               rows_equal(p.keys, p.num_keys, p.compare_key_nulls != 0, (int64_t)(cur & 0xffffffffull), row)) {
+      return rec;
+    }
+    idx = idx + 1 == t.slots ? 0 : idx + 1;
+  }
+  return nullptr;
+}
+
 __device__ __forceinline__ bool lds_occupied(const LdsTable& t, int32_t i, int key_mode)
 {
   uint64_t hdr = *reinterpret_cast<const uint64_t*>(lds_record(t, i));
@@ -385,6 +373,66 @@ template <typename T>
 __device__ __forceinline__ void lds_max(T* p, T v)
 {
   __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Whether a row's value of a second-pass op's column is valid (`flags`: the
+// row's validity bits, partitioned records).
+__device__ __forceinline__ bool pass2_valid(const Pass2Op& o, int64_t row, uint32_t flags)
+{
+  if (o.mask) return bit_valid(o.mask, o.bit_off + row);
+  if (o.flag_bit >= 0) return (flags >> o.flag_bit) & 1u;
+  return true;
+}
+
+// One second-pass op for a valid value (`raw`) of a row on its group's
+// record, whose first-pass fields are final (kLds: an LDS record,
+// workgroup-scope atomics; otherwise a global one); `src_row` is the row's
+// index in the input.
+//  OP_M2: adds (x - mean)^2 with mean = SUM / COUNT_VALID, the SUM of a
+//    float32 column taken at float32 precision as its SUM result is
+//    (reference: cpp/src/groupby/hash/multi_pass_kernels.cuh:76-90).
+//  OP_ARGMIN_IDX / OP_ARGMAX_IDX: the smallest row index whose value is the
+//    group's MIN / MAX (the reference's CAS loop in
+//    cpp/include/cudf/detail/aggregation/aggregation.cuh:436-490 returns one
+//    of the tied rows; this is the first of them).
+template <bool kLds>
+__device__ __forceinline__ void pass2_apply(const Pass2Op& o, uint8_t* rec, uint64_t raw, uint32_t src_row)
+{
+  if (o.kind == OP_M2) {
+    uint32_t cnt = *reinterpret_cast<const uint32_t*>(rec + o.count_off);
+    if (o.nulls_off >= 0) cnt -= *reinterpret_cast<const uint32_t*>(rec + o.nulls_off);
+    double sum;
+    if (o.ref_kind == OP_SUM_I64) {
+      sum = (double)*reinterpret_cast<const int64_t*>(rec + o.ref_off);
+    } else {
+      sum = *reinterpret_cast<const double*>(rec + o.ref_off);
+      if (o.vclass == VC_F32) sum = (double)(float)sum;
+    }
+    const double d = widen_f64(raw, o.width, o.vclass) - sum / (double)cnt;
+    if constexpr (kLds) lds_add(reinterpret_cast<double*>(rec + o.off), d * d);
+    else atom_add(reinterpret_cast<double*>(rec + o.off), d * d);
+  } else {
+    const uint64_t code = minmax_code(o.ref_kind, raw, o.width, o.vclass);
+    const uint64_t best = op_field_bytes(o.ref_kind) == 8 ? *reinterpret_cast<const uint64_t*>(rec + o.ref_off)
+                                                          : *reinterpret_cast<const uint32_t*>(rec + o.ref_off);
+    if (code == best) {
+      if constexpr (kLds) lds_min(reinterpret_cast<uint32_t*>(rec + o.off), src_row);
+      else atom_min(reinterpret_cast<uint32_t*>(rec + o.off), src_row);
+    }
+  }
+}
+
+// Every second-pass op of one row on its group's record.
+template <bool kLds>
+__device__ __forceinline__ void pass2_update(const Pass2Ops* __restrict__ q, uint8_t* rec, int64_t row,
+                                             uint32_t flags, uint32_t src_row)
+{
+  const int n = q->num_ops;
+  for (int i = 0; i < n; ++i) {
+    const Pass2Op& o = q->ops[i];
+    if (!pass2_valid(o, row, flags)) continue;
+    pass2_apply<kLds>(o, rec, load_at(o.data, o.width, o.stride, row), src_row);
+  }
 }
 
 // Elements row0 + u * step (u < N, where present[u]) of a column, in the

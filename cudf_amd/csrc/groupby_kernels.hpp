@@ -111,6 +111,44 @@ struct PartParams {
   uint64_t rec_magic;             // ceil(2^32 / (rec_bytes / 8)): word index -> record index by multiply-shift
   int32_t key_off[kMaxKeys];      // offset of each key column in a record
   int32_t val_off[kMaxVals];      // offset of each value column in a record
+  int32_t rowid_off = -1;         // offset of the source row index (u32) in a record, or -1 (none kept)
+  int32_t pad = 0;
+};
+
+// Second pass over the rows (VARIANCE / STD / ARGMIN / ARGMAX): every row
+// finds its group's record in a completed table (LDS or global) and updates
+// the second-pass fields from the first pass's SUM / MIN / MAX fields.
+constexpr int kMaxPass2 = 16;
+
+struct Pass2Op {
+  const uint8_t* data;  // the value column (an input column, or its field of the partitioned records)
+  const uint32_t* mask; // its null mask, or nullptr
+  int64_t bit_off;
+  int32_t width;
+  int32_t vclass;
+  int32_t stride;       // bytes between consecutive rows' values
+  int32_t flag_bit;     // its validity bit in the row's flags word, or -1
+  int32_t kind;         // OP_M2, OP_ARGMIN_IDX or OP_ARGMAX_IDX
+  int32_t val;          // the value column's index (ops of one column are adjacent)
+  int32_t off;          // the updated field
+  int32_t ref_off;      // OP_M2: the column's SUM field; ARG*: its MIN / MAX field
+  int32_t ref_kind;     // op kind of the ref field
+  int32_t count_off;    // the row-count field (OP_M2)
+  int32_t nulls_off;    // the column's OP_NULLS field, or -1 (OP_M2)
+  int32_t pad;
+};
+
+// The ops of a second pass, read by the kernels from device memory (so that
+// their runtime indexing reads memory instead of copying a parameter block
+// into private memory).
+struct Pass2Ops {
+  AggParams rows;            // the rows and the global table (LDS kernels read them from here, not from their
+                             // parameter block)
+  int32_t num_ops;
+  int32_t rowid_stride;      // bytes between the row indices of consecutive rows
+  const uint8_t* rowid;      // source row index (u32) of each row (partitioned records), or nullptr: the row itself
+  uint32_t* item_global;     // partitioned: per work item, 1 when its groups went to the global table
+  Pass2Op ops[kMaxPass2];
 };
 
 // LDS-resident aggregation of one work item (a partition of the partitioned
@@ -131,29 +169,7 @@ struct LdsParams {
   int32_t direct;                 // 1: merge into the global table, 0: write outputs
   const uint32_t* item_offsets;   // partitioned mode: [P+1] first work item of each partition
   int64_t max_item_rows;          // partitioned mode: rows per work item at most
-};
-
-// Second pass over the rows (VARIANCE / STD / ARGMIN / ARGMAX): every row
-// finds its group's record in the completed global table and updates the
-// second-pass fields from the first pass's SUM / MIN / MAX fields.
-constexpr int kMaxPass2 = 16;
-
-struct Pass2Op {
-  int32_t kind;       // OP_M2, OP_ARGMIN_IDX or OP_ARGMAX_IDX
-  int32_t val;        // index into src.vals
-  int32_t off;        // the updated field
-  int32_t ref_off;    // OP_M2: the column's SUM field; ARG*: its MIN / MAX field
-  int32_t ref_kind;   // op kind of the ref field
-  int32_t count_off;  // the row-count field (OP_M2)
-  int32_t nulls_off;  // the column's OP_NULLS field, or -1 (OP_M2)
-  int32_t pad;
-};
-
-struct Pass2Params {
-  AggParams src;      // rows, keys, value columns; table/capacity: the completed global table
-  int32_t num_ops;
-  int32_t pad;
-  Pass2Op ops[kMaxPass2];
+  const Pass2Ops* p2;             // the second pass (device memory), or nullptr
 };
 
 }  // namespace mgb

@@ -380,4 +380,166 @@ __device__ __forceinline__ void lds_write_groups(const AggParams& p, const LdsTa
   }
 }
 
+// ------------------------------------------------------------- second pass
+
+// The second pass of a row whose group is not in the LDS table: its record
+// in the global table (a call of its own, as spill_row).
+__device__ __forceinline__ void pass2_spill(const AggParams& p, const Pass2Ops* __restrict__ q, const RowKey& rk,
+                                            int64_t row, uint32_t flags, uint32_t src_row)
+{
+  uint8_t* g = global_find(p, rk, row);
+  if (g) pass2_update<false>(q, g, row, flags, src_row);
+}
+
+// The second pass over rows [begin, end) of `p` against the LDS table that
+// their first pass completed. Rows are taken kRows per thread at a time:
+// each finds its group (the same slot hash as lds_aggregate_rows: a
+// Fibonacci hash of integer keys of records, the row hash's bits below the
+// partition's otherwise), then the ops are applied op by op to all of them,
+// each op's descriptor read once and each column's values loaded once. A
+// row whose group is not in the table (direct mode with more groups than the
+// table holds) updates the global table instead. Ends with a workgroup
+// barrier.
+__device__ __forceinline__ void lds_pass2_rows(const AggParams& p, const Pass2Ops* __restrict__ q, const LdsTable& t,
+                                               int64_t begin, int64_t end, int log2p, bool check_keys)
+{
+  constexpr int kRows = 4;
+  const bool rec_hash = p.rec_words > 0;
+  const int nops = q->num_ops;
+  const uint8_t* rowid = q->rowid;
+  const int rowid_stride = q->rowid_stride;
+  for (int64_t base = begin + threadIdx.x; base < end; base += (int64_t)kRows * kLdsBlock) {
+    uint32_t loff[kRows], flags[kRows], src[kRows];
+    bool present[kRows];
+#pragma unroll
+    for (int u = 0; u < kRows; ++u) {
+      const int64_t row = base + (int64_t)u * kLdsBlock;
+      present[u] = row < end && (!check_keys || !p.skip_null_keys || key_row_valid(p, row));
+      loff[u] = kRecNone;
+      flags[u] = 0u;
+      src[u] = (uint32_t)row;
+      if (!present[u]) continue;
+      flags[u] = p.flags ? load_flags(p.flags, p.flag_bytes, p.flag_stride, row) : 0u;
+      if (rowid) src[u] = *reinterpret_cast<const uint32_t*>(rowid + row * (int64_t)rowid_stride);
+      RowKey rk;
+      uint32_t slot_hash;
+      if (p.key_mode == KEY_INT) {
+        const KeyCol& k = p.keys[0];
+        const uint64_t raw = load_at(k.data, k.width, k.stride, row);
+        rk.key = (uint64_t)widen_int(raw, k.width, k.vclass);
+        rk.special = rk.key == kEmptyIntKey;
+        slot_hash = rec_hash ? (uint32_t)((rk.key * 0x9E3779B97F4A7C15ull) >> 32)
+                             : hash_combine(0u, murmur3_bits(normalize_float_bits(raw, k.vclass), k.width, 0)) << log2p;
+        rk.hash = slot_hash;
+      } else {
+        rk = row_key(p, row);
+        slot_hash = rk.hash << log2p;
+      }
+      uint8_t* rec = lds_find(t, p, rk, slot_hash, row);
+      if (rec) {
+        loff[u] = (uint32_t)(rec - t.rec);
+      } else {
+        if (p.key_mode == KEY_INT) {
+          // the global table's probe takes the row hash
+          const KeyCol& k = p.keys[0];
+          rk.hash = hash_combine(0u, murmur3_bits(normalize_float_bits(load_at(k.data, k.width, k.stride, row),
+                                                                       k.vclass), k.width, 0));
+        }
+        pass2_spill(p, q, rk, row, flags[u], src[u]);
+      }
+    }
+    uint64_t x[kRows];
+    int last = -1;
+    for (int i = 0; i < nops; ++i) {
+      const Pass2Op& o = q->ops[i];
+      if (o.val != last) {
+        last = o.val;
+#pragma unroll
+        for (int u = 0; u < kRows; ++u)
+          x[u] = loff[u] != kRecNone ? load_at(o.data, o.width, o.stride, base + (int64_t)u * kLdsBlock) : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < kRows; ++u) {
+        if (loff[u] == kRecNone || !pass2_valid(o, base + (int64_t)u * kLdsBlock, flags[u])) continue;
+        pass2_apply<true>(o, t.rec + loff[u], x[u], src[u]);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Direct mode: copies every group of the completed global table `p.table`
+// into the LDS table (all of them, when they fit; the rest stay global),
+// first-pass fields included. Ends with a workgroup barrier.
+__device__ __forceinline__ void lds_load_global(const AggParams& p, const LdsTable& t)
+{
+  const int words = p.stride / 8;
+  for (int64_t i = threadIdx.x; i <= p.capacity; i += kLdsBlock) {
+    const uint64_t* g = reinterpret_cast<const uint64_t*>(p.table + i * (int64_t)p.stride);
+    const uint64_t hdr = g[0];
+    RowKey rk;
+    int64_t rep = 0;
+    if (i == p.capacity) {
+      if (!(p.key_mode == KEY_INT && hdr == 1ull)) continue;
+      rk.key = kEmptyIntKey;
+      rk.special = true;
+      rk.hash = 0;
+    } else {
+      if (hdr == (p.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef)) continue;
+      rk.special = false;
+      rk.key = hdr;
+      if (p.key_mode == KEY_INT) {
+        const KeyCol& k = p.keys[0];
+        const uint64_t raw = k.width == 8 ? hdr : (hdr & ((1ull << (8 * k.width)) - 1ull));
+        rk.hash = hash_combine(0u, murmur3_bits(normalize_float_bits(raw, k.vclass), k.width, 0));
+      } else {
+        rep = (int64_t)(hdr & 0xffffffffull);
+        rk.hash = (uint32_t)(hdr >> 32);
+      }
+    }
+    uint8_t* rec = lds_find_or_insert(t, p, rk, rk.hash, rep);
+    if (!rec) continue;
+    uint64_t* w = reinterpret_cast<uint64_t*>(rec);
+    for (int j = 1; j < words; ++j) w[j] = g[j];
+  }
+  __syncthreads();
+}
+
+// Folds the second-pass fields of every group of the LDS table into its
+// record of the global table (direct mode).
+__device__ __forceinline__ void lds_fold_pass2(const AggParams& p, const Pass2Ops* __restrict__ q, const LdsTable& t)
+{
+  for (int i = threadIdx.x; i <= t.slots; i += kLdsBlock) {
+    if (!lds_occupied(t, i, p.key_mode)) continue;
+    const uint8_t* rec = lds_record(t, i);
+    const uint64_t hdr = *reinterpret_cast<const uint64_t*>(rec);
+    RowKey rk;
+    int64_t rep = 0;
+    if (p.key_mode == KEY_INT) {
+      const KeyCol& k = p.keys[0];
+      rk.special = i == t.slots;
+      rk.key = rk.special ? kEmptyIntKey : hdr;
+      const uint64_t raw = k.width == 8 ? rk.key : (rk.key & ((1ull << (8 * k.width)) - 1ull));
+      rk.hash = hash_combine(0u, murmur3_bits(normalize_float_bits(raw, k.vclass), k.width, 0));
+    } else {
+      rep = (int64_t)(hdr & 0xffffffffull);
+      rk.special = false;
+      rk.key = hdr;
+      rk.hash = (uint32_t)(hdr >> 32);
+    }
+    uint8_t* g = global_find(p, rk, rep);
+    if (!g) continue;
+    for (int o = 0; o < q->num_ops; ++o) {
+      const Pass2Op& op = q->ops[o];
+      if (op.kind == OP_M2) {
+        const double m2 = *reinterpret_cast<const double*>(rec + op.off);
+        if (m2 != 0.0) atom_add(reinterpret_cast<double*>(g + op.off), m2);
+      } else {
+        const uint32_t idx = *reinterpret_cast<const uint32_t*>(rec + op.off);
+        if (idx != 0xffffffffu) atom_min(reinterpret_cast<uint32_t*>(g + op.off), idx);
+      }
+    }
+  }
+}
+
 }  // namespace mgb

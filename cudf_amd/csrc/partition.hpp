@@ -235,6 +235,12 @@ __device__ __forceinline__ void scatter_chunk(const PartParams& pp, int64_t chun
       for (int k = 0; k < kScatterItems; ++k) f[k] = flags[k];
       store_rows(stage_bytes, pos, pp.rec_bytes, pp.flags_off, pp.flag_bytes, f, present);
     }
+    if (pp.rowid_off >= 0) {
+      uint64_t rid[kScatterItems];
+#pragma unroll
+      for (int k = 0; k < kScatterItems; ++k) rid[k] = (uint64_t)(row0 + k * step);
+      store_rows(stage_bytes, pos, pp.rec_bytes, pp.rowid_off, 4, rid, present);
+    }
     __syncthreads();
     // the next tile's loads fly while this one is copied out
     if (tile + tile_rows_max < end) fetch(tile + tile_rows_max);

@@ -143,15 +143,52 @@ def test_composite_and_null_keys(gpu):
     _compare(keys, reqs, include_null_keys=True)
 
 
-@pytest.mark.parametrize("path", ["partitioned", "direct", "global"])
-def test_any_requested_path_runs_the_second_pass(gpu, monkeypatch, path):
-    # the second pass needs the global table: a forced LDS path gives way
-    monkeypatch.setenv("MGB_GROUPBY_PATH", path)
+PATHS = ["auto", "partitioned", "direct", "global"]
+
+
+@pytest.fixture(params=PATHS)
+def path(request, monkeypatch):
+    """Each execution runs the second pass its own way: the partitioned items
+    in LDS after their first pass (split or spilled items on the global
+    table), the direct workgroups on LDS copies of the completed table, the
+    global execution on the global table."""
+    if request.param != "auto":
+        monkeypatch.setenv("MGB_GROUPBY_PATH", request.param)
+    else:
+        monkeypatch.delenv("MGB_GROUPBY_PATH", raising=False)
+    return request.param
+
+
+def test_every_execution(gpu, path):
     rng = np.random.default_rng(580)
     n = 200_000
     keys = rng.integers(0, 50_000, n).astype(np.int64)
-    reqs = [((O.FLOAT32, _values(rng, O.FLOAT32, n), rng.random(n) > 0.05), SECOND + [O.COUNT_VALID])]
+    reqs = [((O.FLOAT32, _values(rng, O.FLOAT32, n), rng.random(n) > 0.05), SECOND + [O.COUNT_VALID]),
+            ((O.INT32, _values(rng, O.INT32, n, ties=True), rng.random(n) > 0.05), [O.ARGMAX, (O.VARIANCE, 0)])]
     _compare([(O.INT64, keys, None)], reqs)
+
+
+@pytest.mark.parametrize("rows,card", [(3_000_000, 40), (2_000_000, 1000), (400_000, 120_000)])
+def test_few_and_many_groups(gpu, path, rows, card):
+    # few groups over many rows: partitions split over several items (their
+    # second pass on the global table) or the direct execution; many groups:
+    # more than one LDS table holds
+    rng = np.random.default_rng(590 + card)
+    keys = rng.integers(-10 ** 9, 10 ** 9, card)[rng.integers(0, card, rows)].astype(np.int32)
+    reqs = [((O.FLOAT64, _values(rng, O.FLOAT64, rows), rng.random(rows) > 0.02), SECOND),
+            ((O.INT16, _values(rng, O.INT16, rows, ties=True), None), [O.ARGMIN, O.ARGMAX])]
+    _compare([(O.INT32, keys, None)], reqs)
+
+
+def test_row_keys_in_every_execution(gpu, path):
+    # float and composite keys (the row-reference table headers)
+    rng = np.random.default_rng(595)
+    n = 150_000
+    a = rng.integers(0, 300, n).astype(np.float64) / 4
+    b = rng.integers(0, 30, n).astype(np.int16)
+    reqs = [((O.FLOAT64, _values(rng, O.FLOAT64, n), rng.random(n) > 0.1), SECOND)]
+    _compare([(O.FLOAT64, a, None), (O.INT16, b, None)], reqs)
+    _compare([(O.FLOAT64, a, None)], reqs)
 
 
 def test_empty_and_all_null(gpu):
