@@ -1,0 +1,63 @@
+"""The pandas hook on the GPU: ``df.groupby(by).agg(func)`` through
+libcudf_amd.so against pandas itself (the reference's CPU path for this
+call), for the frames of tests/test_pandas_accel.py and larger ones, and the
+installed hook routing served calls to the library."""
+from __future__ import annotations
+
+import numpy as np
+import pandas as pd
+import pytest
+
+from cudf_amd import pandas_accel as PA
+from tests.test_pandas_accel import ALL, frame, same
+
+pytestmark = pytest.mark.gpu
+
+
+def test_list_of_aggregations(gpu):
+    df = frame(1)
+    sub = df[["k", "f64", "f32", "i8", "ni"]]
+    same(PA.groupby_agg(sub, "k", ALL), sub.groupby("k").agg(ALL))
+
+
+def test_dict_spec_idx_and_two_keys(gpu):
+    df = frame(2).drop(columns="k2")
+    spec = {"f64": ["mean", "idxmin", "idxmax", "var", "std"], "i8": "sum", "ni": ["min", "count"]}
+    same(PA.groupby_agg(df, "k", spec), df.groupby("k").agg(spec))
+    df5 = frame(5).drop(columns=["ni"])
+    spec2 = {"f64": ["sum", "std"], "f32": ["min"]}
+    same(PA.groupby_agg(df5, ["k", "k2"], spec2), df5.groupby(["k", "k2"]).agg(spec2))
+
+
+def test_sort_false_and_dropna_false(gpu):
+    df = frame(4, nan_keys=True).drop(columns=["k2", "ni"])
+    spec = {"f64": ["sum", "mean"], "i8": ["max", "count"]}
+    same(PA.groupby_agg(df, "k", spec, sort=False), df.groupby("k", sort=False).agg(spec))
+    same(PA.groupby_agg(df, "k", spec, dropna=False), df.groupby("k", dropna=False).agg(spec))
+
+
+@pytest.mark.parametrize("rows,card", [(2_000_000, 100_000), (1_000_000, 12)])
+def test_large_frames(gpu, rows, card):
+    rng = np.random.default_rng(rows + card)
+    keys = rng.integers(-2 ** 40, 2 ** 40, card)[rng.integers(0, card, rows)]
+    f64 = rng.standard_normal(rows)
+    f64[rng.random(rows) < 0.05] = np.nan
+    df = pd.DataFrame({"key": keys, "f64": f64, "i32": rng.integers(-1000, 1000, rows).astype(np.int32)})
+    spec = {"f64": ["sum", "count", "mean", "min", "max", "std", "idxmax"], "i32": ["sum", "min", "var", "idxmin"]}
+    same(PA.groupby_agg(df, "key", spec), df.groupby("key").agg(spec))
+
+
+def test_installed_hook_runs_on_the_library(gpu):
+    df = frame(8)
+    want = df.groupby("k").agg({"f64": ["sum", "var"], "i8": "max"})
+    PA.install()
+    try:
+        before = PA.stats()
+        got = df.groupby("k").agg({"f64": ["sum", "var"], "i8": "max"})
+        median = df.groupby("k").agg({"f64": "median"})  # not served: pandas runs it
+        after = PA.stats()
+    finally:
+        PA.uninstall()
+    same(got, want)
+    pd.testing.assert_frame_equal(median, df.groupby("k").agg({"f64": "median"}))
+    assert after["gpu"] - before["gpu"] == 1 and after["pandas"] - before["pandas"] == 1
