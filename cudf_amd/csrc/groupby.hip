@@ -1353,6 +1353,11 @@ bool pass2_ops(const Plan& plan, const AggParams& rows, Pass2Ops& q)
   q.num_ops = 0;
   for (size_t c = 0; c < plan.cols.size(); ++c) {
     const PlanCol& pc = plan.cols[c];
+    if (c < (size_t)kProgramCols) {
+      auto f10 = [](int off) { return (uint32_t)(off + 1) & 1023u; };
+      q.p2prog[c] = f10(pc.p2_offset(OP_M2)) | f10(pc.p2_offset(OP_ARGMIN_IDX)) << 10 |
+                    f10(pc.p2_offset(OP_ARGMAX_IDX)) << 20;
+    }
     for (auto& o : pc.p2) {
       if (q.num_ops == kMaxPass2) return false;
       q.ops[q.num_ops++] = pass2_op(plan, pc, o, rows.vals[c], (int)c);

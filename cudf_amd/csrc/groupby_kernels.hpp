@@ -148,6 +148,10 @@ struct Pass2Ops {
   int32_t rowid_stride;      // bytes between the row indices of consecutive rows
   const uint8_t* rowid;      // source row index (u32) of each row (partitioned records), or nullptr: the row itself
   uint32_t* item_global;     // partitioned: per work item, 1 when its groups went to the global table
+  // per value column (the first kProgramCols): its M2, ARGMIN and ARGMAX
+  // field offsets + 1 in bits 0-9, 10-19 and 20-29 (0: none), for the record
+  // loop beside rows.prog
+  uint32_t p2prog[kProgramCols];
   Pass2Op ops[kMaxPass2];
 };
 

@@ -391,6 +391,145 @@ __device__ __forceinline__ void pass2_spill(const AggParams& p, const Pass2Ops* 
   if (g) pass2_update<false>(q, g, row, flags, src_row);
 }
 
+// lds_pass2_rows over whole records (p.rec_words > 0) with an integer key
+// and value columns that all take a column program (p.program_cols): the
+// loop of lds_aggregate_rows_impl<true>. The uniform values are held in
+// vector registers and made scalar at each use, so that no scalar load (which
+// waits for every LDS update in flight) falls inside the loop; each column's
+// first-pass program (p.prog) gives its value, validity bit and SUM / MIN /
+// MAX fields, q->p2prog its second-pass fields. Every group is in the table
+// (a whole partition without spill). Ends with a workgroup barrier.
+__device__ __forceinline__ void lds_pass2_recs(const AggParams& p, const Pass2Ops* __restrict__ q, const LdsTable& t,
+                                               int64_t begin, int64_t end)
+{
+  constexpr int kRows = 3;
+  const int64_t step = kLdsBlock;
+  const KeyCol& k0 = p.keys[0];
+  const int has_flags = p.flags != nullptr;
+  const int has_rowid = q->rowid != nullptr;
+  uint32_t pin_a = (uint32_t)(k0.width & 15) | (uint32_t)(k0.vclass & 7) << 4 |
+                   (uint32_t)((k0.data - p.rec_base) & 63) << 7 |
+                   (uint32_t)(has_flags ? (p.flags - p.rec_base) & 63 : 0) << 13 | (uint32_t)(p.flag_bytes & 7) << 19 |
+                   (uint32_t)has_flags << 22 | (uint32_t)has_rowid << 23 | (uint32_t)(p.rec_words & 7) << 24;
+  uint32_t pin_b = (uint32_t)t.slots | (uint32_t)t.stride << 16;
+  uint32_t pin_c = (uint32_t)((p.rows_op >= 0 ? p.ops[p.rows_op].off : -1) + 1) | (uint32_t)(p.num_vals & 31) << 10 |
+                   (uint32_t)(has_rowid ? (q->rowid - p.rec_base) & 63 : 0) << 15;
+  uint64_t pin_base = (uint64_t)p.rec_base;
+  asm volatile("" : "+v"(pin_a), "+v"(pin_b), "+v"(pin_c), "+v"(pin_base));
+  uint64_t pin_prog[kProgramCols];
+  uint32_t pin_p2[kProgramCols];
+#pragma unroll
+  for (int c = 0; c < kProgramCols; ++c) {
+    pin_prog[c] = p.prog[c];
+    pin_p2[c] = q->p2prog[c];
+    asm volatile("" : "+v"(pin_prog[c]), "+v"(pin_p2[c]));
+  }
+  for (int64_t base = begin + threadIdx.x; base < end; base += (int64_t)kRows * kLdsBlock) {
+    bool present[kRows];
+#pragma unroll
+    for (int u = 0; u < kRows; ++u) present[u] = base + u * step < end;
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_a);
+    const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_b);
+    const uint32_t cc = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_c);
+    const int kw = (int)(a & 15), kvc = (int)((a >> 4) & 7);
+    const int slots = (int)(b & 0xffff), stride = (int)(b >> 16);
+    const int rows_off = (int)(cc & 1023) - 1, nvals = (int)((cc >> 10) & 31);
+    const uint64_t rb = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pin_base) |
+                        (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pin_base >> 32)) << 32;
+    RecWords rw[kRows];
+    load_recs(reinterpret_cast<const uint8_t*>(rb), (int)((a >> 24) & 7), base, step, present, base, rw);
+    uint64_t key[kRows], f[kRows], rid[kRows];
+    extract_rows(rw, (int)((a >> 7) & 63), kw, VC_UNSIGNED, key);
+    if ((a >> 22) & 1) extract_rows(rw, (int)((a >> 13) & 63), (int)((a >> 19) & 7), VC_UNSIGNED, f);
+    else {
+#pragma unroll
+      for (int u = 0; u < kRows; ++u) f[u] = 0ull;
+    }
+    if ((a >> 23) & 1) extract_rows(rw, (int)((cc >> 15) & 63), 4, VC_UNSIGNED, rid);
+    else {
+#pragma unroll
+      for (int u = 0; u < kRows; ++u) rid[u] = (uint64_t)(base + u * step);
+    }
+    // the row's record: the slot the first pass gave its key
+    uint32_t loff[kRows];
+#pragma unroll
+    for (int u = 0; u < kRows; ++u) {
+      loff[u] = kRecNone;
+      if (!present[u]) continue;
+      const uint64_t kk = (uint64_t)widen_int(key[u], kw, kvc);
+      if (kk == kEmptyIntKey) {
+        loff[u] = (uint32_t)slots * (uint32_t)stride;
+        continue;
+      }
+      uint32_t i = (uint32_t)reduce_range((uint32_t)((kk * 0x9E3779B97F4A7C15ull) >> 32), slots);
+      for (int probe = 0; probe < slots; ++probe) {
+        const uint64_t h = *reinterpret_cast<const uint64_t*>(t.rec + i * (uint32_t)stride);
+        if (h == kk) {
+          loff[u] = i * (uint32_t)stride;
+          break;
+        }
+        if (h == kEmptyIntKey) break;
+        i = i + 1 == (uint32_t)slots ? 0u : i + 1;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kProgramCols; ++c) {
+      if (c >= nvals) break;
+      const uint32_t p2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_p2[c]);
+      if (p2 == 0) continue;
+      const uint64_t pg = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pin_prog[c]) |
+                          (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pin_prog[c] >> 32)) << 32;
+      const Program v = unpack_program(pg);
+      uint64_t x[kRows];
+      extract_rows(rw, v.rec_off, v.width, v.vclass, x);
+      bool ok[kRows];
+#pragma unroll
+      for (int u = 0; u < kRows; ++u)
+        ok[u] = loff[u] != kRecNone && (v.flag_bit < 0 || ((f[u] >> v.flag_bit) & 1u));
+      const int m2_off = (int)(p2 & 1023u) - 1;
+      const int amin_off = (int)((p2 >> 10) & 1023u) - 1;
+      const int amax_off = (int)((p2 >> 20) & 1023u) - 1;
+      const bool fl = v.vclass == VC_F32 || v.vclass == VC_F64;
+      if (m2_off >= 0) {
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+          if (!ok[u]) continue;
+          uint8_t* rec = t.rec + loff[u];
+          uint32_t cnt = *reinterpret_cast<const uint32_t*>(rec + rows_off);
+          if (v.nulls_off >= 0) cnt -= *reinterpret_cast<const uint32_t*>(rec + v.nulls_off);
+          double sum;
+          if (fl) {
+            sum = *reinterpret_cast<const double*>(rec + v.sum_off);
+            if (v.vclass == VC_F32) sum = (double)(float)sum;
+          } else {
+            sum = (double)*reinterpret_cast<const int64_t*>(rec + v.sum_off);
+          }
+          const double d = widen_f64(x[u], v.width, v.vclass) - sum / (double)cnt;
+          lds_add(reinterpret_cast<double*>(rec + m2_off), d * d);
+        }
+      }
+#pragma unroll
+      for (int which = 0; which < 2; ++which) {
+        const int arg_off = which ? amax_off : amin_off;
+        if (arg_off < 0) continue;
+        const int kind = v.mm_kind + which;
+        const int ref_off = which ? v.max_off : v.min_off;
+        const bool wide = op_field_bytes(kind) == 8;
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+          if (!ok[u]) continue;
+          uint8_t* rec = t.rec + loff[u];
+          const uint64_t code = minmax_code(kind, x[u], v.width, v.vclass);
+          const uint64_t best = wide ? *reinterpret_cast<const uint64_t*>(rec + ref_off)
+                                     : *reinterpret_cast<const uint32_t*>(rec + ref_off);
+          if (code == best) lds_min(reinterpret_cast<uint32_t*>(rec + arg_off), (uint32_t)rid[u]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // The second pass over rows [begin, end) of `p` against the LDS table that
 // their first pass completed. Rows are taken kRows per thread at a time:
 // each finds its group (the same slot hash as lds_aggregate_rows: a
@@ -403,6 +542,10 @@ __device__ __forceinline__ void pass2_spill(const AggParams& p, const Pass2Ops* 
 __device__ __forceinline__ void lds_pass2_rows(const AggParams& p, const Pass2Ops* __restrict__ q, const LdsTable& t,
                                                int64_t begin, int64_t end, int log2p, bool check_keys)
 {
+  if (p.rec_words > 0 && p.program_cols && p.key_mode == KEY_INT && !check_keys) {
+    lds_pass2_recs(p, q, t, begin, end);
+    return;
+  }
   constexpr int kRows = 4;
   const bool rec_hash = p.rec_words > 0;
   const int nops = q->num_ops;
