@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # library profiling scopes (mgb_profiling_read) and the kernels they time
 PROFILED = ("aggregate", "lds_aggregate", "fill", "finalize", "partition_hist", "partition_scatter", "estimate",
             "pass2")
-KERNEL_SYMBOL = {"aggregate": "mgb::aggregate_kernel", "lds_aggregate": "mgb::lds_agg_kernel",
+KERNEL_SYMBOL = {"aggregate": "mgb::aggregate_kernel", "lds_aggregate": "mgb::lds_agg_kernel<false>",
                  "partition_scatter": "mgb::part_scatter_tiled_kernel", "partition_hist": "mgb::part_hist_kernel",
                  "pass2": "mgb::pass2_kernel"}
 
@@ -571,7 +571,7 @@ def summarize_trace(db_path: str, title: str, command: str) -> str:
            "Durations in microseconds (top_kernels view of the rocprofv3 database; torch kernels are data generation).",
            "", f"{'kernel':60s} {'calls':>7s} {'total_us':>12s} {'avg_us':>10s} {'pct':>6s}"]
     for name, calls, total, avg, pct in rows[:30]:
-        out.append(f"{name.split('(')[0][:60]:60s} {int(calls):7d} {float(total):12.1f} {float(avg):10.1f} "
+        out.append(f"{name.split('(')[0].removeprefix('void ')[:60]:60s} {int(calls):7d} {float(total):12.1f} {float(avg):10.1f} "
                    f"{float(pct):6.2f}")
     return "\n".join(out) + "\n"
 
@@ -588,7 +588,7 @@ def summarize_pmc(fetch_csv: str, write_csv: str, cal_fetch_csv: str, cal_write_
         acc = collections.defaultdict(list)
         for r in csv.DictReader(open(path)):
             if r["Counter_Name"] == counter:
-                acc[r["Kernel_Name"].split("(")[0]].append(float(r["Counter_Value"]))
+                acc[r["Kernel_Name"].split("(")[0].removeprefix("void ")].append(float(r["Counter_Value"]))
         return acc
 
     cal_f = per_kernel(cal_fetch_csv, "FETCH_SIZE")
@@ -782,9 +782,12 @@ def main():
     dom = max((k for k in ("aggregate", "lds_aggregate", "partition_scatter", "pass2") if k in kern),
               key=lambda k: kern[k][1] / kern[k][0])
     agg_n, agg_ms = kern[dom]
+    symbol = KERNEL_SYMBOL[dom]
+    if dom == "lds_aggregate" and args.workload == "c2v":
+        symbol = "mgb::lds_agg_kernel<true>"  # the instance with the second pass
     agg_avg_ms = agg_ms / agg_n
     achieved = in_bytes / (agg_avg_ms * 1e-3) / 1e9
-    traffic = read_traffic(args.traffic_file, KERNEL_SYMBOL[dom]) if args.workload == "c2" else None
+    traffic = read_traffic(args.traffic_file, symbol) if args.workload == "c2" else None
     call_gbs = (in_bytes + out_bytes) / (ms_per_step * 1e-3) / 1e9
 
     cpu = None
@@ -809,7 +812,7 @@ def main():
         "config": {"workload": description, "rows": R, "groups": groups, "parallelism": "single GPU"},
         "roofline": {
             "bound": "hbm",
-            "kernel": KERNEL_SYMBOL[dom],
+            "kernel": symbol,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -177,13 +177,11 @@ __device__ __forceinline__ void lds_aggregate_rows_impl(const AggParams& p, cons
 #pragma unroll
       for (int u = 0; u < kRows; ++u) {
         kk[u] = (uint64_t)widen_int(key[u], kw, kvc);
-        // the slot hash: any function of the key independent of the bits that
-        // chose the partition. Records (partitioned rows) take a Fibonacci
-        // (multiplicative) hash of the integer key, cheaper than the row hash
-        // (the partition came from the row hash's top bits); other rows take
-        // the row hash, whose bits below the partition's pick the slot.
-        if constexpr (kRec) hsh[u] = (uint32_t)((kk[u] * 0x9E3779B97F4A7C15ull) >> 32);
-        else hsh[u] = hash_combine(0u, murmur3_bits(normalize_float_bits(key[u], kvc), kw, 0)) << log2p;
+        // the slot hash: the row hash's bits below the ones that chose the
+        // partition. (A multiplicative hash of the key is cheaper, but keys
+        // in an arithmetic progression modulo 2^32 cluster under it: batches
+        // of such keys ran 6.6 times slower.)
+        hsh[u] = hash_combine(0u, murmur3_bits(normalize_float_bits(key[u], kvc), kw, 0)) << log2p;
         i0[u] = (uint32_t)reduce_range(hsh[u], slots);
         i1[u] = i0[u] + 1 == (uint32_t)slots ? 0u : i0[u] + 1;
         h0[u] = h1[u] = kEmptyIntKey;
@@ -400,7 +398,7 @@ __device__ __forceinline__ void pass2_spill(const AggParams& p, const Pass2Ops* 
 // MAX fields, q->p2prog its second-pass fields. Every group is in the table
 // (a whole partition without spill). Ends with a workgroup barrier.
 __device__ __forceinline__ void lds_pass2_recs(const AggParams& p, const Pass2Ops* __restrict__ q, const LdsTable& t,
-                                               int64_t begin, int64_t end)
+                                               int64_t begin, int64_t end, int log2p)
 {
   constexpr int kRows = 3;
   const int64_t step = kLdsBlock;
@@ -413,7 +411,7 @@ __device__ __forceinline__ void lds_pass2_recs(const AggParams& p, const Pass2Op
                    (uint32_t)has_flags << 22 | (uint32_t)has_rowid << 23 | (uint32_t)(p.rec_words & 7) << 24;
   uint32_t pin_b = (uint32_t)t.slots | (uint32_t)t.stride << 16;
   uint32_t pin_c = (uint32_t)((p.rows_op >= 0 ? p.ops[p.rows_op].off : -1) + 1) | (uint32_t)(p.num_vals & 31) << 10 |
-                   (uint32_t)(has_rowid ? (q->rowid - p.rec_base) & 63 : 0) << 15;
+                   (uint32_t)(has_rowid ? (q->rowid - p.rec_base) & 63 : 0) << 15 | (uint32_t)(log2p & 31) << 21;
   uint64_t pin_base = (uint64_t)p.rec_base;
   asm volatile("" : "+v"(pin_a), "+v"(pin_b), "+v"(pin_c), "+v"(pin_base));
   uint64_t pin_prog[kProgramCols];
@@ -433,7 +431,7 @@ __device__ __forceinline__ void lds_pass2_recs(const AggParams& p, const Pass2Op
     const uint32_t cc = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_c);
     const int kw = (int)(a & 15), kvc = (int)((a >> 4) & 7);
     const int slots = (int)(b & 0xffff), stride = (int)(b >> 16);
-    const int rows_off = (int)(cc & 1023) - 1, nvals = (int)((cc >> 10) & 31);
+    const int rows_off = (int)(cc & 1023) - 1, nvals = (int)((cc >> 10) & 31), shift = (int)((cc >> 21) & 31);
     const uint64_t rb = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pin_base) |
                         (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pin_base >> 32)) << 32;
     RecWords rw[kRows];
@@ -461,7 +459,8 @@ __device__ __forceinline__ void lds_pass2_recs(const AggParams& p, const Pass2Op
         loff[u] = (uint32_t)slots * (uint32_t)stride;
         continue;
       }
-      uint32_t i = (uint32_t)reduce_range((uint32_t)((kk * 0x9E3779B97F4A7C15ull) >> 32), slots);
+      const uint32_t hsh = hash_combine(0u, murmur3_bits(normalize_float_bits(key[u], kvc), kw, 0)) << shift;
+      uint32_t i = (uint32_t)reduce_range(hsh, slots);
       for (int probe = 0; probe < slots; ++probe) {
         const uint64_t h = *reinterpret_cast<const uint64_t*>(t.rec + i * (uint32_t)stride);
         if (h == kk) {
@@ -532,9 +531,8 @@ __device__ __forceinline__ void lds_pass2_recs(const AggParams& p, const Pass2Op
 
 // The second pass over rows [begin, end) of `p` against the LDS table that
 // their first pass completed. Rows are taken kRows per thread at a time:
-// each finds its group (the same slot hash as lds_aggregate_rows: a
-// Fibonacci hash of integer keys of records, the row hash's bits below the
-// partition's otherwise), then the ops are applied op by op to all of them,
+// each finds its group (the same slot as lds_aggregate_rows: the row hash's
+// bits below the partition's), then the ops are applied op by op to all of them,
 // each op's descriptor read once and each column's values loaded once. A
 // row whose group is not in the table (direct mode with more groups than the
 // table holds) updates the global table instead. Ends with a workgroup
@@ -543,11 +541,10 @@ __device__ __forceinline__ void lds_pass2_rows(const AggParams& p, const Pass2Op
                                                int64_t begin, int64_t end, int log2p, bool check_keys)
 {
   if (p.rec_words > 0 && p.program_cols && p.key_mode == KEY_INT && !check_keys) {
-    lds_pass2_recs(p, q, t, begin, end);
+    lds_pass2_recs(p, q, t, begin, end, log2p);
     return;
   }
   constexpr int kRows = 4;
-  const bool rec_hash = p.rec_words > 0;
   const int nops = q->num_ops;
   const uint8_t* rowid = q->rowid;
   const int rowid_stride = q->rowid_stride;
@@ -571,8 +568,7 @@ __device__ __forceinline__ void lds_pass2_rows(const AggParams& p, const Pass2Op
         const uint64_t raw = load_at(k.data, k.width, k.stride, row);
         rk.key = (uint64_t)widen_int(raw, k.width, k.vclass);
         rk.special = rk.key == kEmptyIntKey;
-        slot_hash = rec_hash ? (uint32_t)((rk.key * 0x9E3779B97F4A7C15ull) >> 32)
-                             : hash_combine(0u, murmur3_bits(normalize_float_bits(raw, k.vclass), k.width, 0)) << log2p;
+        slot_hash = hash_combine(0u, murmur3_bits(normalize_float_bits(raw, k.vclass), k.width, 0)) << log2p;
         rk.hash = slot_hash;
       } else {
         rk = row_key(p, row);
