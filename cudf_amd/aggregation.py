@@ -45,6 +45,10 @@ def mean() -> Aggregation:
     return Aggregation(Kind.MEAN)
 
 
+def m2() -> Aggregation:
+    return Aggregation(Kind.M2)
+
+
 def variance(ddof: int = 1) -> Aggregation:
     return Aggregation(Kind.VARIANCE, ddof)
 

@@ -139,6 +139,7 @@ MGB_FACTORY(make_min_aggregation, MIN)
 MGB_FACTORY(make_max_aggregation, MAX)
 MGB_FACTORY(make_sum_of_squares_aggregation, SUM_OF_SQUARES)
 MGB_FACTORY(make_mean_aggregation, MEAN)
+MGB_FACTORY(make_m2_aggregation, M2)
 MGB_FACTORY(make_argmax_aggregation, ARGMAX)
 MGB_FACTORY(make_argmin_aggregation, ARGMIN)
 #undef MGB_FACTORY

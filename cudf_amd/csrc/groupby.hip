@@ -918,7 +918,7 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
       switch (kind) {
         case MGB_AGG_SUM: case MGB_AGG_PRODUCT: case MGB_AGG_MIN: case MGB_AGG_MAX: case MGB_AGG_COUNT_VALID:
         case MGB_AGG_COUNT_ALL: case MGB_AGG_SUM_OF_SQUARES: case MGB_AGG_MEAN: case MGB_AGG_VARIANCE:
-        case MGB_AGG_STD: case MGB_AGG_ARGMIN: case MGB_AGG_ARGMAX: break;
+        case MGB_AGG_STD: case MGB_AGG_M2: case MGB_AGG_ARGMIN: case MGB_AGG_ARGMAX: break;
         default:
           return fail(MGB_ERROR_UNSUPPORTED,
                       "aggregation kind " + std::to_string(kind) + " is not served by this hash groupby");
@@ -970,7 +970,7 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
       case MGB_AGG_COUNT_ALL: need_rows = true; break;
       // second-pass aggregations: the SUM (mean) or MIN / MAX of the first
       // pass, then a field the second pass fills (M2 is shared by every ddof)
-      case MGB_AGG_VARIANCE: case MGB_AGG_STD:
+      case MGB_AGG_VARIANCE: case MGB_AGG_STD: case MGB_AGG_M2:
         add_op(pc, sum_op_for(pc.type));
         add_p2(pc, OP_M2);
         break;
@@ -1035,6 +1035,12 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
         s.field_off = pc.p2_offset(OP_M2);
         s.fin = wt.kind == MGB_AGG_VARIANCE ? FIN_VAR : FIN_STD;
         s.ddof = wt.ddof;
+        break;
+      case MGB_AGG_M2:
+        // sum of squared deviations from the MEAN (cpp/src/groupby/sort/group_m2.cu:40-52; cudf 24.04
+        // serves it from its sort path only), null where MEAN is
+        s.field_off = pc.p2_offset(OP_M2);
+        s.fin = FIN_M2;
         break;
       case MGB_AGG_ARGMIN: case MGB_AGG_ARGMAX:
         s.field_off = pc.p2_offset(wt.kind == MGB_AGG_ARGMIN ? OP_ARGMIN_IDX : OP_ARGMAX_IDX);

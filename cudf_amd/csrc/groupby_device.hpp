@@ -1050,6 +1050,7 @@ __device__ __forceinline__ void finalize_record(const uint8_t* rec, int64_t pos,
         bits = (uint64_t)__double_as_longlong(r);
         break;
       }
+      case FIN_M2: bits = *reinterpret_cast<const uint64_t*>(f); break;
       case FIN_ARGIDX:
         // a group without valid values keeps the sentinel -1 (groupby.cu:231-240)
         bits = cnt ? *reinterpret_cast<const uint32_t*>(f) : 0xffffffffu;

@@ -53,7 +53,8 @@ enum FinKind : int32_t {
   FIN_MEAN_DURATION,  // i64 sum / u32 count -> integer of out_width bytes (truncating)
   FIN_VAR,            // f64 sum of squared deviations / (count - ddof) -> double, null unless count > ddof
   FIN_STD,            // square root of FIN_VAR's value
-  FIN_ARGIDX          // u32 row index -> int32 (-1 where the group has no valid value)
+  FIN_ARGIDX,         // u32 row index -> int32 (-1 where the group has no valid value)
+  FIN_M2              // f64 sum of squared deviations -> double, null where the group has no valid value
 };
 
 struct OutSpec {

@@ -610,8 +610,8 @@ extern "C" mgb_status mgb_server_submit(mgb_server* s, int32_t table_id, const i
       q.vals.push_back(t.cols[requests[r].column]);
       q.aggs.emplace_back(requests[r].aggs, requests[r].aggs + requests[r].num_aggs);
       for (auto& a : q.aggs.back())
-        q.plain_only |= a.kind == MGB_AGG_VARIANCE || a.kind == MGB_AGG_STD || a.kind == MGB_AGG_ARGMIN ||
-                        a.kind == MGB_AGG_ARGMAX;
+        q.plain_only |= a.kind == MGB_AGG_VARIANCE || a.kind == MGB_AGG_STD || a.kind == MGB_AGG_M2 ||
+                        a.kind == MGB_AGG_ARGMIN || a.kind == MGB_AGG_ARGMAX;
     }
     // null keys grouped as a key (null_policy INCLUDE): plain path too
     if (null_policy == MGB_NULL_INCLUDE)

@@ -94,6 +94,8 @@ class Plan:
         self.sort, self.dropna = bool(sort), bool(dropna)
         self.index = df.index
         self.rows = len(df)
+        if self.rows == 0:
+            raise NotServed("empty frame")  # pandas' empty results take dtypes from the spec's form
         self.keys = [_column(df[b]) for b in self.by]
         value_cols = [c for c in df.columns if c not in self.by]
         # (output label, column label, aggregation name)

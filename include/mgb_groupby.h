@@ -148,9 +148,10 @@ int32_t mgb_target_type(int32_t value_type_id, int32_t agg_kind);
  * waits for `stream` first. Errors: MGB_ERROR_LOGIC for a
  * size mismatch or an invalid type/aggregation pair (groupby.cu:208-212,
  * 173-190), MGB_ERROR_UNSUPPORTED for aggregations cudf serves only from its
- * sort-based path (M2 and the kinds the hash path does not list).
- * VARIANCE, STD, ARGMIN and ARGMAX take a second pass over the rows
- * (cpp/src/groupby/hash/groupby.cu:73-84 lists them as hash aggregations).
+ * sort-based path (median, quantile, nth element, nunique, collect, ...).
+ * VARIANCE, STD, ARGMIN and ARGMAX (hash aggregations in
+ * cpp/src/groupby/hash/groupby.cu:73-84) and M2 (sort-only in cudf,
+ * cpp/src/groupby/sort/group_m2.cu) take a second pass over the rows.
  */
 mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_keys, int32_t null_policy,
                                  const mgb_request* requests, int32_t num_requests,
@@ -246,7 +247,7 @@ typedef struct mgb_batch_stats {
 /* keys: num_queries * num_keys columns, query-major; requests:
  * num_queries * num_requests, query-major (request r of every query carries
  * the same aggregations). options may be NULL. MGB_ERROR_UNSUPPORTED for
- * VARIANCE, STD, ARGMIN, ARGMAX and for null keys grouped under
+ * VARIANCE, STD, M2, ARGMIN, ARGMAX and for null keys grouped under
  * MGB_NULL_INCLUDE: those queries run on mgb_groupby_aggregate. */
 mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys, int32_t null_policy,
                             const mgb_request* requests, int32_t num_requests, int32_t num_queries,

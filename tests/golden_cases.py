@@ -15,7 +15,7 @@ TYPES = {"INT8": O.INT8, "INT16": O.INT16, "INT32": O.INT32, "INT64": O.INT64, "
          "FLOAT64": O.FLOAT64, "BOOL8": O.BOOL8}
 KINDS = {"SUM": O.SUM, "PRODUCT": O.PRODUCT, "MIN": O.MIN, "MAX": O.MAX, "COUNT_VALID": O.COUNT_VALID,
          "COUNT_ALL": O.COUNT_ALL, "SUM_OF_SQUARES": O.SUM_OF_SQUARES, "MEAN": O.MEAN,
-         "VARIANCE": O.VARIANCE, "STD": O.STD, "ARGMAX": O.ARGMAX, "ARGMIN": O.ARGMIN}
+         "VARIANCE": O.VARIANCE, "STD": O.STD, "ARGMAX": O.ARGMAX, "ARGMIN": O.ARGMIN, "M2": O.M2}
 
 
 def _num(x):
@@ -32,6 +32,8 @@ def load_cases():
         kind = KINDS[c["agg"]]
         if c.get("typed"):
             combos = [("INT32", v) for v in doc["value_types_for_typed_cases"]]
+        elif c.get("typed_both"):  # keys and values of the same type (keys_col<T>, vals_col<T>)
+            combos = [(v, v) for v in doc["value_types_for_typed_cases"]]
         elif c.get("typed_keys"):
             combos = [(k, c["val_type"]) for k in ("INT8", "INT16", "INT32", "INT64", "FLOAT32", "FLOAT64")]
         elif "val_types" in c:
@@ -84,6 +86,7 @@ def check_case(case, key_cols, results):
         a, b = gv[evalid].astype(np.float64), ev[evalid].astype(np.float64)
         with np.errstate(invalid="ignore"):
             same = (a == b) | (np.abs(a - b) <= 4 * np.spacing(np.abs(b).astype(ev.dtype)).astype(np.float64))
+            same |= np.isnan(a) & np.isnan(b)
         assert same.all(), f"{case['id']}: {gv} vs {ev}"
     else:
         assert np.array_equal(gv[evalid], ev[evalid]), f"{case['id']}: {gv} vs {ev}"

@@ -1,4 +1,4 @@
-"""GPU parity of the second-pass aggregations (VARIANCE, STD, ARGMIN, ARGMAX)
+"""GPU parity of the second-pass aggregations (VARIANCE, STD, M2, ARGMIN, ARGMAX)
 through the C ABI: against the oracle on seeded inputs (every value type,
 nulls, ties, NaN and signed zeros, composite and null keys, several ddof),
 size-independent properties at 20M rows, and their routing (batches refuse
@@ -6,7 +6,7 @@ them; a batching server runs them alone on the plain path).
 
 Tolerances: ARGMIN / ARGMAX are row indices and must match exactly.
 VARIANCE is compared as |gpu - oracle| <= rtol * (sum of x^2) / (n - ddof)
-per group, rtol 1e-12 (1e-9 for float32 sources, whose mean comes from a
+per group (M2 against rtol * sum of x^2), rtol 1e-12 (1e-9 for float32 sources, whose mean comes from a
 float32 SUM): the deviation sum has no cancellation, so its error is relative
 to the second moment; STD takes the square root of that bound."""
 from __future__ import annotations
@@ -21,7 +21,7 @@ from tests.helpers import abs_sums, assert_same, gpu_groupby, oracle_groupby, so
 
 pytestmark = pytest.mark.gpu
 
-SECOND = [(O.VARIANCE, 1), (O.STD, 1), O.ARGMIN, O.ARGMAX]
+SECOND = [(O.VARIANCE, 1), (O.STD, 1), O.M2, O.ARGMIN, O.ARGMAX]
 
 
 def _kind(a):
@@ -60,7 +60,7 @@ def _compare(keys, reqs, include_null_keys=False):
         f32 = reqs[r][0][0] == O.FLOAT32
         for a, ((g, gvalid), (o, ovalid)) in zip(reqs[r][1], zip(gblk, oblk)):
             what = f"request {r} agg {a}"
-            if _kind(a) not in (O.VARIANCE, O.STD, O.ARGMIN, O.ARGMAX):
+            if _kind(a) not in (O.VARIANCE, O.STD, O.M2, O.ARGMIN, O.ARGMAX):
                 # the first-pass aggregations, to the bar of the other parity tests
                 assert_same((g, gvalid), (o, ovalid), None, abs_sum=s[O.sort_order(sk)],
                             rtol=1e-5 if f32 else None, what=what)
@@ -70,10 +70,10 @@ def _compare(keys, reqs, include_null_keys=False):
             assert np.array_equal(gm, om), f"{what}: validity differs at {np.flatnonzero(gm != om)[:5]}"
             g, o = np.asarray(g)[om], np.asarray(o)[om]
             k = _kind(a)
-            if k in (O.VARIANCE, O.STD):
+            if k in (O.VARIANCE, O.STD, O.M2):
                 ddof = a[1] if isinstance(a, tuple) else 1
                 rtol = 1e-9 if f32 else 1e-12
-                scale = sumsq[om] / np.maximum(cnt[om] - ddof, 1)
+                scale = sumsq[om] if k == O.M2 else sumsq[om] / np.maximum(cnt[om] - ddof, 1)
                 bound = rtol * scale if k == O.VARIANCE else np.sqrt(rtol * scale)
                 err = np.abs(g - o)
                 bad = ~((err <= bound) | (err <= 1e-12 * np.abs(o)))

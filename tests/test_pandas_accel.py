@@ -124,6 +124,17 @@ def test_calls_not_served(func):
         PA.Plan(df, "k", func, True, True)
 
 
+def test_empty_frame_not_served():
+    with pytest.raises(PA.NotServed):
+        PA.Plan(frame(1).iloc[:0], "k", "sum", True, True)
+
+
+def test_dropna_false_without_missing_keys():
+    df = frame(10).drop(columns=["k2", "ni"])
+    spec = {"f64": ["sum", "max"], "i8": "count"}
+    same(oracle_frame(df, "k", spec, dropna=False), df.groupby("k", dropna=False).agg(spec))
+
+
 def test_dtypes_not_served():
     df = frame(7)
     df["s"] = ["x"] * len(df)
