@@ -8,9 +8,9 @@ from .column import Column, Table, pack_validity, unpack_validity
 from .batch import GroupByBatch
 from .groupby import GroupBy, GroupByRequest
 from .serving import Server
-from .types import Kind, NullPolicy, Sorted, TypeId
+from .types import Interpolation, Kind, NullPolicy, Sorted, TypeId
 
 __all__ = [
-    "aggregation", "hashing", "Column", "Table", "GroupBy", "GroupByBatch", "GroupByRequest", "Server", "Kind", "NullPolicy",
+    "aggregation", "hashing", "Column", "Table", "GroupBy", "GroupByBatch", "GroupByRequest", "Server", "Interpolation", "Kind", "NullPolicy",
     "Sorted", "TypeId", "pack_validity", "unpack_validity",
 ]

@@ -32,7 +32,8 @@ class mgb_column(ctypes.Structure):
 
 
 class mgb_aggregation(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int32), ("ddof", ctypes.c_int32)]
+    _fields_ = [("kind", ctypes.c_int32), ("ddof", ctypes.c_int32), ("null_policy", ctypes.c_int32),
+                ("interpolation", ctypes.c_int32), ("quantile", ctypes.c_double)]
 
 
 class mgb_request(ctypes.Structure):

@@ -4,16 +4,25 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 
-from .types import Kind, NullPolicy
+from .types import Interpolation, Kind, NullPolicy
 
 
 @dataclass(frozen=True)
 class Aggregation:
     kind: Kind
-    ddof: int = 0
+    ddof: int = 0              # VARIANCE / STD: ddof; NTH_ELEMENT: n
+    null_policy: int = NullPolicy.EXCLUDE
+    interpolation: int = Interpolation.LINEAR
+    quantile: float = 0.0
 
     def __repr__(self) -> str:
         return f"Aggregation({self.kind.name})"
+
+    def to_c(self):
+        from . import _lib
+
+        return _lib.mgb_aggregation(int(self.kind), int(self.ddof), int(self.null_policy), int(self.interpolation),
+                                    float(self.quantile))
 
 
 def sum() -> Aggregation:  # noqa: A001 - mirrors the reference name
@@ -55,6 +64,23 @@ def variance(ddof: int = 1) -> Aggregation:
 
 def std(ddof: int = 1) -> Aggregation:
     return Aggregation(Kind.STD, ddof)
+
+
+def median() -> Aggregation:
+    return Aggregation(Kind.MEDIAN)
+
+
+def quantile(q: float, interp: Interpolation = Interpolation.LINEAR) -> Aggregation:
+    """One quantile per aggregation (pylibcudf takes a list: one aggregation per entry here)."""
+    return Aggregation(Kind.QUANTILE, interpolation=int(interp), quantile=float(q))
+
+
+def nunique(null_handling: NullPolicy = NullPolicy.EXCLUDE) -> Aggregation:
+    return Aggregation(Kind.NUNIQUE, null_policy=int(null_handling))
+
+
+def nth_element(n: int, null_handling: NullPolicy = NullPolicy.INCLUDE) -> Aggregation:
+    return Aggregation(Kind.NTH_ELEMENT, ddof=int(n), null_policy=int(null_handling))
 
 
 def argmax() -> Aggregation:

@@ -55,7 +55,7 @@ class GroupByBatch:
                 keys[q * nk + c] = _to_mgb(kt.columns[c])
             for r, req in enumerate(rq):
                 arr = (_lib.mgb_aggregation * max(len(req.aggregations), 1))(
-                    *[_lib.mgb_aggregation(int(a.kind), int(a.ddof)) for a in req.aggregations])
+                    *[a.to_c() for a in req.aggregations])
                 self._keep.append(arr)
                 reqs[q * nreq + r].values = _to_mgb(req.values)
                 reqs[q * nreq + r].aggs = ctypes.cast(arr, ctypes.POINTER(_lib.mgb_aggregation))

@@ -10,6 +10,8 @@
 
 #include <rmm/device_buffer.hpp>
 
+#include <hip/hip_runtime.h>
+
 #include "mgb_groupby.h"
 
 #include <string>
@@ -38,32 +40,75 @@ rmm::cuda_stream_view const get_default_stream() { return rmm::cuda_stream_view{
 namespace detail {
 
 // One concrete aggregation type serving every factory; `param` carries ddof
-// (VARIANCE / STD) or n (NTH_ELEMENT).
+// (VARIANCE / STD) or n (NTH_ELEMENT), `nulls` the null_policy of NUNIQUE /
+// NTH_ELEMENT, `quantiles` / `interp` those of QUANTILE.
 class simple_aggregation final : public groupby_aggregation,
                                  public groupby_scan_aggregation,
                                  public reduce_aggregation,
                                  public rolling_aggregation {
  public:
-  explicit simple_aggregation(aggregation::Kind k, size_type param = 0) : aggregation(k), _param{param} {}
+  explicit simple_aggregation(aggregation::Kind k, size_type param = 0, null_policy nulls = null_policy::EXCLUDE,
+                              std::vector<double> quantiles = {}, interpolation interp = interpolation::LINEAR)
+    : aggregation(k), _param{param}, _nulls{nulls}, _quantiles{std::move(quantiles)}, _interp{interp}
+  {
+  }
   [[nodiscard]] std::unique_ptr<aggregation> clone() const override
   {
-    return std::make_unique<simple_aggregation>(kind, _param);
+    return std::make_unique<simple_aggregation>(kind, _param, _nulls, _quantiles, _interp);
   }
   [[nodiscard]] bool is_equal(aggregation const& other) const override
   {
     auto const* o = dynamic_cast<simple_aggregation const*>(&other);
-    return o && o->kind == kind && o->_param == _param;
+    return o && o->kind == kind && o->_param == _param && o->_nulls == _nulls && o->_quantiles == _quantiles &&
+           o->_interp == _interp;
   }
   [[nodiscard]] size_type param() const { return _param; }
+  [[nodiscard]] null_policy nulls() const { return _nulls; }
+  [[nodiscard]] std::vector<double> const& quantiles() const { return _quantiles; }
+  [[nodiscard]] interpolation interp() const { return _interp; }
 
  private:
   size_type _param;
+  null_policy _nulls;
+  std::vector<double> _quantiles;
+  interpolation _interp;
 };
 
-size_type aggregation_param(aggregation const& a)
+// The C ABI aggregations of one cudf aggregation: one per quantile of a
+// QUANTILE, one otherwise.
+std::vector<mgb_aggregation> to_mgb_aggs(aggregation const& a)
 {
   auto const* s = dynamic_cast<simple_aggregation const*>(&a);
-  return s ? s->param() : 0;
+  mgb_aggregation m{};
+  m.kind = static_cast<int32_t>(a.kind);
+  if (!s) return {m};
+  m.ddof = s->param();
+  m.null_policy = s->nulls() == null_policy::INCLUDE ? MGB_NULL_INCLUDE : MGB_NULL_EXCLUDE;
+  m.interpolation = static_cast<int32_t>(s->interp());
+  if (a.kind != aggregation::QUANTILE) return {m};
+  std::vector<mgb_aggregation> out;
+  for (double q : s->quantiles()) {
+    m.quantile = q;
+    out.push_back(m);
+  }
+  return out;
+}
+
+// Validity of a flattened [groups x k] quantile result: bit g * k + j is bit g
+// of the groups' validity.
+__global__ void repeat_bits_kernel(const uint32_t* in, int64_t groups, int32_t k, uint32_t* out)
+{
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t words = (groups * k + 31) / 32;
+  if (w >= words) return;
+  uint32_t v = 0;
+  for (int b = 0; b < 32; ++b) {
+    const int64_t i = w * 32 + b;
+    if (i >= groups * k) break;
+    const int64_t g = i / k;
+    v |= ((in[g >> 5] >> (g & 31)) & 1u) << b;
+  }
+  out[w] = v;
 }
 
 [[noreturn]] void throw_status(mgb_status st)
@@ -140,6 +185,7 @@ MGB_FACTORY(make_max_aggregation, MAX)
 MGB_FACTORY(make_sum_of_squares_aggregation, SUM_OF_SQUARES)
 MGB_FACTORY(make_mean_aggregation, MEAN)
 MGB_FACTORY(make_m2_aggregation, M2)
+MGB_FACTORY(make_median_aggregation, MEDIAN)
 MGB_FACTORY(make_argmax_aggregation, ARGMAX)
 MGB_FACTORY(make_argmin_aggregation, ARGMIN)
 #undef MGB_FACTORY
@@ -174,21 +220,32 @@ template std::unique_ptr<aggregation> make_std_aggregation<aggregation>(size_typ
 template std::unique_ptr<groupby_aggregation> make_std_aggregation<groupby_aggregation>(size_type);
 
 template <typename Base>
-std::unique_ptr<Base> make_nunique_aggregation(null_policy)
+std::unique_ptr<Base> make_nunique_aggregation(null_policy null_handling)
 {
-  return std::make_unique<detail::simple_aggregation>(aggregation::NUNIQUE);
+  return std::make_unique<detail::simple_aggregation>(aggregation::NUNIQUE, 0, null_handling);
 }
 template std::unique_ptr<aggregation> make_nunique_aggregation<aggregation>(null_policy);
 template std::unique_ptr<groupby_aggregation> make_nunique_aggregation<groupby_aggregation>(null_policy);
 
 template <typename Base>
-std::unique_ptr<Base> make_nth_element_aggregation(size_type n, null_policy)
+std::unique_ptr<Base> make_nth_element_aggregation(size_type n, null_policy null_handling)
 {
-  return std::make_unique<detail::simple_aggregation>(aggregation::NTH_ELEMENT, n);
+  return std::make_unique<detail::simple_aggregation>(aggregation::NTH_ELEMENT, n, null_handling);
 }
 template std::unique_ptr<aggregation> make_nth_element_aggregation<aggregation>(size_type, null_policy);
 template std::unique_ptr<groupby_aggregation> make_nth_element_aggregation<groupby_aggregation>(size_type,
                                                                                                 null_policy);
+
+template <typename Base>
+std::unique_ptr<Base> make_quantile_aggregation(std::vector<double> const& quantiles, interpolation interp)
+{
+  return std::make_unique<detail::simple_aggregation>(aggregation::QUANTILE, 0, null_policy::EXCLUDE, quantiles,
+                                                      interp);
+}
+template std::unique_ptr<aggregation> make_quantile_aggregation<aggregation>(std::vector<double> const&,
+                                                                             interpolation);
+template std::unique_ptr<groupby_aggregation> make_quantile_aggregation<groupby_aggregation>(
+  std::vector<double> const&, interpolation);
 
 namespace groupby {
 
@@ -226,9 +283,14 @@ std::pair<std::unique_ptr<table>, std::vector<aggregation_result>> groupby::aggr
   std::vector<std::vector<mgb_aggregation>> aggs(requests.size());
   std::vector<mgb_request> reqs(requests.size());
   std::size_t n_out = 0;
+  // C ABI aggregations of each cudf aggregation (a QUANTILE of k quantiles is k of them)
+  std::vector<std::vector<std::size_t>> parts(requests.size());
   for (std::size_t i = 0; i < requests.size(); ++i) {
-    for (auto const& a : requests[i].aggregations)
-      aggs[i].push_back(mgb_aggregation{static_cast<int32_t>(a->kind), detail::aggregation_param(*a)});
+    for (auto const& a : requests[i].aggregations) {
+      auto m = detail::to_mgb_aggs(*a);
+      parts[i].push_back(m.size());
+      aggs[i].insert(aggs[i].end(), m.begin(), m.end());
+    }
     reqs[i].values = detail::to_mgb(requests[i].values);
     reqs[i].aggs = aggs[i].data();
     reqs[i].num_aggs = static_cast<int32_t>(aggs[i].size());
@@ -249,9 +311,35 @@ std::pair<std::unique_ptr<table>, std::vector<aggregation_result>> groupby::aggr
   for (auto const& o : out_keys) key_cols.push_back(detail::adopt(o, stream, mr));
   std::vector<aggregation_result> results(requests.size());
   std::size_t j = 0;
-  for (std::size_t i = 0; i < requests.size(); ++i)
-    for (std::size_t a = 0; a < aggs[i].size(); ++a, ++j)
-      results[i].results.push_back(detail::adopt(out_results[j], stream, mr));
+  for (std::size_t i = 0; i < requests.size(); ++i) {
+    for (std::size_t k : parts[i]) {
+      if (k == 1) {
+        results[i].results.push_back(detail::adopt(out_results[j++], stream, mr));
+        continue;
+      }
+      // k quantiles: cudf's flattened [groups x k] column (group_quantiles.cu:95-100)
+      std::vector<std::unique_ptr<column>> q;
+      for (std::size_t t = 0; t < k; ++t) q.push_back(detail::adopt(out_results[j++], stream, mr));
+      const int64_t n = groups * static_cast<int64_t>(k);
+      rmm::device_buffer data{static_cast<std::size_t>(n) * 8, stream, mr};
+      for (std::size_t t = 0; t < k && groups > 0; ++t)
+        if (hipMemcpy2DAsync(static_cast<uint8_t*>(data.data()) + t * 8, k * 8, q[t]->view().head(), 8, 8,
+                             static_cast<std::size_t>(groups), hipMemcpyDeviceToDevice, stream.value()) != hipSuccess)
+          throw cudf::cuda_error("CUDA error: interleaving quantiles");
+      rmm::device_buffer mask;
+      size_type nulls = 0;
+      if (q[0]->null_count() > 0) {
+        const int64_t words = (n + 31) / 32;
+        mask = rmm::device_buffer{static_cast<std::size_t>(((words * 4 + 63) / 64) * 64), stream, mr};
+        detail::repeat_bits_kernel<<<(unsigned)((words + 255) / 256), 256, 0, stream.value()>>>(
+          q[0]->view().null_mask(), groups, static_cast<int32_t>(k), static_cast<uint32_t*>(mask.data()));
+        if (hipGetLastError() != hipSuccess) throw cudf::cuda_error("CUDA error: quantile validity");
+        nulls = q[0]->null_count() * static_cast<size_type>(k);
+      }
+      results[i].results.push_back(std::make_unique<column>(data_type{type_id::FLOAT64}, static_cast<size_type>(n),
+                                                            std::move(data), std::move(mask), nulls));
+    }
+  }
   return {std::make_unique<table>(std::move(key_cols)), std::move(results)};
 }
 

@@ -37,9 +37,12 @@
 #include <tuple>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "groupby_host.hpp"
 #include "lds_agg.hpp"
 #include "partition.hpp"
+#include "sort_aggs.hpp"
 
 namespace mgb {
 
@@ -160,6 +163,7 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(const FinParams p)
   const uint64_t hdr = *reinterpret_cast<const uint64_t*>(rec);
   finalize_keys(hdr, r == p.capacity, p.key_mode, p.keys, p.num_keys, pos, *p.out);
   finalize_record(rec, pos, *p.out);
+  if (p.rec_pos) p.rec_pos[r] = (uint32_t)pos;
 }
 
 // Packs validity bytes into a cudf bitmask (words beyond n are zeroed) and
@@ -795,6 +799,13 @@ int32_t target_type(int32_t t, int32_t kind)
       if (is_duration_t(t)) return t;
       if (is_timestamp_t(t)) return -1;
       return MGB_TYPE_FLOAT64;
+    // sort-path aggregations (aggregation.hpp:1290-1330): quantiles of numeric
+    // sources are double, NUNIQUE is size_type, NTH_ELEMENT keeps the type
+    case MGB_AGG_MEDIAN: case MGB_AGG_QUANTILE:
+      if (!is_integral_t(t) && !is_float_t(t)) return -1;
+      return MGB_TYPE_FLOAT64;
+    case MGB_AGG_NUNIQUE: return MGB_TYPE_INT32;
+    case MGB_AGG_NTH_ELEMENT: return t;
     case MGB_AGG_M2: case MGB_AGG_VARIANCE: case MGB_AGG_STD:
       // numeric sources only (multi_pass_kernels.cuh:59-63)
       if (!is_integral_t(t) && !is_float_t(t)) return -1;
@@ -819,6 +830,7 @@ struct Workspace {
   DeviceBuffer outs;    // temporary output columns
   DeviceBuffer params;  // kernel parameter blocks (output tables, identity templates)
   DeviceBuffer fine;    // fine bucket counts per chunk of the last key pass (hll_hist_kernel)
+  DeviceBuffer sorts;   // row groups, sort keys and radix-sort space of the sort-path aggregations
   bool fine_valid = false;  // `fine` holds the counts of this call's keys
   int64_t fine_nw = 0, fine_chunk = 0;
   unsigned long long* pinned = nullptr;  // host readback area
@@ -830,6 +842,7 @@ struct Workspace {
     if (outs.ptr) (void)hipFree(outs.ptr);
     if (params.ptr) (void)hipFree(params.ptr);
     if (fine.ptr) (void)hipFree(fine.ptr);
+    if (sorts.ptr) (void)hipFree(sorts.ptr);
     if (pinned) (void)hipHostFree(pinned);
     if (staging) (void)hipHostFree(staging);
   }
@@ -907,6 +920,7 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
   std::map<ValKey, int> col_index;
   struct Want {
     int col, kind, ddof;
+    mgb_aggregation agg;
   };
   std::vector<Want> wants;
   for (int r = 0; r < nreq; ++r) {
@@ -919,6 +933,9 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
         case MGB_AGG_SUM: case MGB_AGG_PRODUCT: case MGB_AGG_MIN: case MGB_AGG_MAX: case MGB_AGG_COUNT_VALID:
         case MGB_AGG_COUNT_ALL: case MGB_AGG_SUM_OF_SQUARES: case MGB_AGG_MEAN: case MGB_AGG_VARIANCE:
         case MGB_AGG_STD: case MGB_AGG_M2: case MGB_AGG_ARGMIN: case MGB_AGG_ARGMAX: break;
+        case MGB_AGG_MEDIAN: case MGB_AGG_QUANTILE: case MGB_AGG_NUNIQUE: case MGB_AGG_NTH_ELEMENT:
+          plan.has_sort = true;
+          break;
         default:
           return fail(MGB_ERROR_UNSUPPORTED,
                       "aggregation kind " + std::to_string(kind) + " is not served by this hash groupby");
@@ -943,7 +960,8 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
     } else {
       ci = it->second;
     }
-    for (int a = 0; a < reqs[r].num_aggs; ++a) wants.push_back({ci, reqs[r].aggs[a].kind, reqs[r].aggs[a].ddof});
+    for (int a = 0; a < reqs[r].num_aggs; ++a)
+      wants.push_back({ci, reqs[r].aggs[a].kind, reqs[r].aggs[a].ddof, reqs[r].aggs[a]});
   }
 
   // ops per column (deduplicated by kind); the counts are added below
@@ -1035,6 +1053,10 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
         s.field_off = pc.p2_offset(OP_M2);
         s.fin = wt.kind == MGB_AGG_VARIANCE ? FIN_VAR : FIN_STD;
         s.ddof = wt.ddof;
+        break;
+      case MGB_AGG_MEDIAN: case MGB_AGG_QUANTILE: case MGB_AGG_NUNIQUE: case MGB_AGG_NTH_ELEMENT:
+        s.fin = FIN_NONE;  // the sort path writes it
+        o.agg = wt.agg;
         break;
       case MGB_AGG_M2:
         // sum of squared deviations from the MEAN (cpp/src/groupby/sort/group_m2.cu:40-52; cudf 24.04
@@ -1533,6 +1555,116 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
 
 // Global-table executions (global and direct paths): build the table, then
 // compact it into temporary outputs.
+// The sort-path aggregations (MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT) of a
+// call whose global table is complete and finalised: `rec_pos` gives every
+// record's output row. Every row's group is looked up once; then per output
+// the rows are radix-sorted by value and (stably) by group, or by group
+// alone for NTH_ELEMENT, and the results read off the sorted rows
+// (cpp/src/groupby/sort/sort_helper.cu orders the rows by key the same way).
+mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows, const uint32_t* rec_pos,
+                         int64_t groups, hipStream_t stream, TempOuts& temp)
+{
+  const int64_t R = rows.num_rows;
+  if (groups == 0 || R == 0) return MGB_SUCCESS;
+  if (R > (int64_t)0x7fffffff || groups >= (int64_t)0xffffffff)
+    return fail(MGB_ERROR_UNSUPPORTED, "sort-path aggregations of more than 2^31 rows");
+  const int n = (int)R;
+  const uint32_t G = (uint32_t)groups;
+  int gbits = 1;
+  while (gbits < 32 && (1ull << gbits) <= (unsigned long long)G) ++gbits;
+  // radix-sort space: the largest of the three uses
+  size_t t64 = 0, t32 = 0, tscan = 0;
+  MGB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t64, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, 64, stream));
+  MGB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t32, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, gbits, stream));
+  MGB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tscan, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)G,
+                                           stream));
+  const size_t tcub = align_up(std::max({t64, t32, tscan}), 256);
+  const size_t r4 = align_up((size_t)R * 4, 256), r8 = align_up((size_t)R * 8, 256), g4 = align_up((size_t)G * 4, 256);
+  // gid, gkey, idx, order1, gk2, sorted_gkey, order | code, code_out | count, offset, has_null | cub
+  const size_t need = 7 * r4 + 2 * r8 + 3 * g4 + tcub;
+  MGB_HIP(ws->sorts.reserve(need, stream));
+  uint8_t* b = static_cast<uint8_t*>(ws->sorts.ptr);
+  size_t off = 0;
+  auto take = [&](size_t bytes) { uint8_t* p = b + off; off += bytes; return p; };
+  uint32_t* gid = reinterpret_cast<uint32_t*>(take(r4));
+  uint32_t* gkey = reinterpret_cast<uint32_t*>(take(r4));
+  uint32_t* idx = reinterpret_cast<uint32_t*>(take(r4));
+  uint32_t* order1 = reinterpret_cast<uint32_t*>(take(r4));
+  uint32_t* gk2 = reinterpret_cast<uint32_t*>(take(r4));
+  uint32_t* sorted_gkey = reinterpret_cast<uint32_t*>(take(r4));
+  uint32_t* order = reinterpret_cast<uint32_t*>(take(r4));
+  uint64_t* code = reinterpret_cast<uint64_t*>(take(r8));
+  uint64_t* code_out = reinterpret_cast<uint64_t*>(take(r8));
+  uint32_t* count = reinterpret_cast<uint32_t*>(take(g4));
+  uint32_t* offset = reinterpret_cast<uint32_t*>(take(g4));
+  uint32_t* has_null = reinterpret_cast<uint32_t*>(take(g4));
+  void* cub_tmp = take(tcub);
+
+  const int cus = device_cu_count();
+  const unsigned grid_rows = (unsigned)std::max<int64_t>(1, std::min<int64_t>((R + kBlock - 1) / kBlock, (int64_t)cus * 16));
+  const unsigned grid_groups = (unsigned)((groups + kBlock - 1) / kBlock);
+  {
+    ProfScope prof("sort_aggs", stream);
+    row_group_kernel<<<grid_rows, kBlock, 0, stream>>>(rows, rec_pos, G, gid);
+    MGB_HIP(hipGetLastError());
+    for (size_t o = 0; o < plan.outs.size(); ++o) {
+      const Plan::Out& out = plan.outs[o];
+      if (out.spec.fin != FIN_NONE) continue;
+      SortAggParams s{};
+      s.v = plan.cols[out.col].col;
+      s.num_rows = R;
+      s.groups = G;
+      s.kind = out.kind;
+      s.nulls = out.agg.null_policy == MGB_NULL_INCLUDE ? 1 : 0;
+      s.interp = out.kind == MGB_AGG_MEDIAN ? 0 : out.agg.interpolation;
+      s.n = out.agg.ddof;
+      s.q = out.kind == MGB_AGG_MEDIAN ? 0.5 : out.agg.quantile;
+      s.gid = gid;
+      s.gkey = gkey;
+      s.code = code;
+      s.idx = idx;
+      s.count = count;
+      s.has_null = has_null;
+      s.order = order;
+      s.sorted_gkey = sorted_gkey;
+      s.offset = offset;
+      s.out = temp.out_data[o];
+      s.out_valid = temp.out_valid[o];
+      s.out_width = out.spec.out_width;
+      if (!s.out_valid) return fail(MGB_ERROR_HIP, "sort-path output without validity");
+      MGB_HIP(hipMemsetAsync(count, 0, (size_t)G * 4, stream));
+      MGB_HIP(hipMemsetAsync(has_null, 0, (size_t)G * 4, stream));
+      sort_prepare_kernel<<<grid_rows, kBlock, 0, stream>>>(s);
+      MGB_HIP(hipGetLastError());
+      size_t tb = tcub;
+      if (out.kind != MGB_AGG_NTH_ELEMENT) {
+        // by value, then stably by group: each group's rows in value order
+        MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, code, code_out, idx, order1, n, 0, 64, stream));
+        gather_u32_kernel<<<grid_rows, kBlock, 0, stream>>>(gkey, order1, R, gk2);
+        MGB_HIP(hipGetLastError());
+        tb = tcub;
+        MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, gk2, sorted_gkey, order1, order, n, 0, gbits, stream));
+      } else {
+        // by group alone: each group's rows in row order
+        MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, gkey, sorted_gkey, idx, order, n, 0, gbits, stream));
+      }
+      tb = tcub;
+      MGB_HIP(hipcub::DeviceScan::ExclusiveSum(cub_tmp, tb, count, offset, (int)G, stream));
+      if (out.kind == MGB_AGG_NUNIQUE) {
+        MGB_HIP(hipMemsetAsync(s.out, 0, (size_t)G * 4, stream));
+        nunique_count_kernel<<<grid_rows, kBlock, 0, stream>>>(s, R);
+        nunique_finish_kernel<<<grid_groups, kBlock, 0, stream>>>(s);
+      } else {
+        sort_select_kernel<<<grid_groups, kBlock, 0, stream>>>(s);
+      }
+      MGB_HIP(hipGetLastError());
+    }
+  }
+  return MGB_SUCCESS;
+}
+
 mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int64_t R, int64_t G_est, bool direct,
                      const LdsGeom& geom, hipStream_t stream, TempOuts& temp, int64_t& groups)
 {
@@ -1541,6 +1673,7 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
   capacity = std::min<int64_t>(capacity, std::max<int64_t>(2 * R + 64, 64));
   int64_t nblocks = 0;
   size_t block_off = 0;
+  size_t rec_pos_off = 0;  // per record: its output row (sort-path aggregations)
   LdsParams direct_lp{};  // the direct launch, for its second pass
   int64_t direct_items = 0;
   for (int attempt = 0;; ++attempt) {
@@ -1549,7 +1682,8 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
     const size_t table_bytes = align_up((size_t)(capacity + 1) * plan.stride, 256);
     const size_t counters_off = table_bytes;
     block_off = counters_off + 256;
-    const size_t need = block_off + align_up((size_t)nblocks * 8, 256);
+    rec_pos_off = block_off + align_up((size_t)nblocks * 8, 256);
+    const size_t need = rec_pos_off + (plan.has_sort ? align_up((size_t)(capacity + 1) * 4, 256) : 0);
     MGB_HIP(ws->main.reserve(need, stream));
     uint8_t* wsb = static_cast<uint8_t*>(ws->main.ptr);
 
@@ -1725,11 +1859,20 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
   f.block_offsets = reinterpret_cast<const int64_t*>(static_cast<uint8_t*>(ws->main.ptr) + block_off);
   f.out = d_ot;
   for (size_t c = 0; c < plan.keys.size(); ++c) f.keys[c] = plan.keys[c];
+  uint32_t* rec_pos = plan.has_sort ? reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws->main.ptr) + rec_pos_off)
+                                    : nullptr;
+  f.rec_pos = rec_pos;
   {
     ProfScope prof("finalize", stream);
     finalize_kernel<<<(unsigned)nblocks, kBlock, 0, stream>>>(f);
   }
   MGB_HIP(hipGetLastError());
+  if (plan.has_sort) {
+    AggParams rows = base;
+    rows.table = static_cast<uint8_t*>(ws->main.ptr);
+    rows.capacity = capacity;
+    return run_sort_aggs(ws, plan, rows, rec_pos, groups, stream, temp);
+  }
   return MGB_SUCCESS;
 }
 
@@ -2066,6 +2209,9 @@ mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mg
                       (int)plan.outs.size() <= kMaxOuts && pass2_ops(plan, probe, probe2);
   Path path = forced_path();
   if (!lds_ok) path = Path::Global;
+  // the sort-path aggregations map rows to groups through a global table,
+  // which the partitioned execution does not keep
+  if (plan.has_sort && path != Path::Direct) path = Path::Global;
   if (path == Path::Auto) path = (G_est <= geom.cap / 2) ? Path::Direct : Path::Partitioned;
 
   TempOuts temp;

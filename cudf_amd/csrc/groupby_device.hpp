@@ -999,6 +999,7 @@ __device__ __forceinline__ void finalize_record(const uint8_t* rec, int64_t pos,
 {
   for (int o = 0; o < ot.num_outs; ++o) {
     const OutSpec& s = ot.outs[o];
+    if (s.fin == FIN_NONE) continue;
     uint32_t cnt = *reinterpret_cast<const uint32_t*>(rec + s.count_off);
     if (s.nulls_off >= 0) cnt -= *reinterpret_cast<const uint32_t*>(rec + s.nulls_off);
     const uint8_t* f = rec + s.field_off;

@@ -145,11 +145,13 @@ struct Plan {
   int rows_off = -1;
   int stride = 8;
   bool has_pass2 = false;   // VARIANCE / STD / ARGMIN / ARGMAX: a second pass over the rows
+  bool has_sort = false;    // MEDIAN / QUANTILE / NUNIQUE / NTH_ELEMENT: sorts of the rows by group
   struct Out {
     int col;        // index into cols
     int kind;       // aggregation kind
     int out_type;
     OutSpec spec;
+    mgb_aggregation agg{};  // the request's parameters (sort-path aggregations)
   };
   std::vector<Out> outs;
 };

@@ -54,7 +54,8 @@ enum FinKind : int32_t {
   FIN_VAR,            // f64 sum of squared deviations / (count - ddof) -> double, null unless count > ddof
   FIN_STD,            // square root of FIN_VAR's value
   FIN_ARGIDX,         // u32 row index -> int32 (-1 where the group has no valid value)
-  FIN_M2              // f64 sum of squared deviations -> double, null where the group has no valid value
+  FIN_M2,             // f64 sum of squared deviations -> double, null where the group has no valid value
+  FIN_NONE            // written by the sort-path kernels after finalize (MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT)
 };
 
 struct OutSpec {
@@ -94,6 +95,7 @@ struct FinParams {
   const unsigned long long* table_groups;  // the table's group counter (0: nothing to write), or nullptr
   const OutTable* out;            // device memory
   KeyCol keys[kMaxKeys];          // representative-row gather source
+  uint32_t* rec_pos = nullptr;    // per record: its output row (the sort-path aggregations), or nullptr
 };
 
 // Hash partitioning of the input rows (histogram and scatter passes).

@@ -276,7 +276,9 @@ std::string signature_of(const Query& q)
   for (size_t r = 0; r < q.vals.size(); ++r) {
     const mgb_column& c = q.vals[r];
     sig += "v" + std::to_string(c.type_id) + (c.null_mask && c.null_count ? "n" : "") + ":";
-    for (auto& a : q.aggs[r]) sig += std::to_string(a.kind) + "." + std::to_string(a.ddof) + ",";
+    for (auto& a : q.aggs[r])
+      sig += std::to_string(a.kind) + "." + std::to_string(a.ddof) + "." + std::to_string(a.null_policy) + "." +
+             std::to_string(a.interpolation) + "." + std::to_string(a.quantile) + ",";
   }
   return sig;
 }
@@ -611,7 +613,8 @@ extern "C" mgb_status mgb_server_submit(mgb_server* s, int32_t table_id, const i
       q.aggs.emplace_back(requests[r].aggs, requests[r].aggs + requests[r].num_aggs);
       for (auto& a : q.aggs.back())
         q.plain_only |= a.kind == MGB_AGG_VARIANCE || a.kind == MGB_AGG_STD || a.kind == MGB_AGG_M2 ||
-                        a.kind == MGB_AGG_ARGMIN || a.kind == MGB_AGG_ARGMAX;
+                        a.kind == MGB_AGG_ARGMIN || a.kind == MGB_AGG_ARGMAX || a.kind == MGB_AGG_MEDIAN ||
+                        a.kind == MGB_AGG_QUANTILE || a.kind == MGB_AGG_NUNIQUE || a.kind == MGB_AGG_NTH_ELEMENT;
     }
     // null keys grouped as a key (null_policy INCLUDE): plain path too
     if (null_policy == MGB_NULL_INCLUDE)

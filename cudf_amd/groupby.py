@@ -113,7 +113,7 @@ class GroupBy:
         n_out = 0
         for i, r in enumerate(requests):
             arr = (_lib.mgb_aggregation * max(len(r.aggregations), 1))(
-                *[_lib.mgb_aggregation(int(a.kind), int(a.ddof)) for a in r.aggregations])
+                *[a.to_c() for a in r.aggregations])
             agg_arrays.append(arr)
             reqs[i].values = _to_mgb(r.values)
             reqs[i].aggs = ctypes.cast(arr, ctypes.POINTER(_lib.mgb_aggregation))

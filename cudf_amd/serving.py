@@ -65,7 +65,7 @@ class Server:
         keep = []
         for i, (col, aggs) in enumerate(requests):
             arr = (_lib.mgb_aggregation * max(len(aggs), 1))(
-                *[_lib.mgb_aggregation(int(a.kind), int(a.ddof)) for a in aggs])
+                *[a.to_c() for a in aggs])
             keep.append(arr)
             reqs[i].column = int(col)
             reqs[i].num_aggs = len(aggs)

@@ -37,6 +37,15 @@ class NullPolicy(enum.IntEnum):
     INCLUDE = 1
 
 
+class Interpolation(enum.IntEnum):
+    """cudf::interpolation (cpp/include/cudf/types.hpp)."""
+    LINEAR = 0
+    LOWER = 1
+    HIGHER = 2
+    MIDPOINT = 3
+    NEAREST = 4
+
+
 class Sorted(enum.IntEnum):
     NO = 0
     YES = 1

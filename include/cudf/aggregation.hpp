@@ -108,6 +108,11 @@ std::unique_ptr<Base> make_mean_aggregation();
 template <typename Base = aggregation>
 std::unique_ptr<Base> make_m2_aggregation();
 template <typename Base = aggregation>
+std::unique_ptr<Base> make_median_aggregation();
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_quantile_aggregation(std::vector<double> const& quantiles,
+                                                interpolation interp = interpolation::LINEAR);
+template <typename Base = aggregation>
 std::unique_ptr<Base> make_variance_aggregation(size_type ddof = 1);
 template <typename Base = aggregation>
 std::unique_ptr<Base> make_std_aggregation(size_type ddof = 1);

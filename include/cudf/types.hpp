@@ -18,6 +18,7 @@ class table_view;
 
 enum class order : bool { ASCENDING, DESCENDING };
 enum class null_policy : bool { EXCLUDE, INCLUDE };
+enum class interpolation : int32_t { LINEAR, LOWER, HIGHER, MIDPOINT, NEAREST };
 enum class nan_policy : bool { NAN_IS_NULL, NAN_IS_VALID };
 enum class null_equality : bool { EQUAL, UNEQUAL };
 enum class null_order : bool { AFTER, BEFORE };

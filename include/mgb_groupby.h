@@ -59,8 +59,13 @@ enum {
   MGB_AGG_SUM = 0, MGB_AGG_PRODUCT = 1, MGB_AGG_MIN = 2, MGB_AGG_MAX = 3,
   MGB_AGG_COUNT_VALID = 4, MGB_AGG_COUNT_ALL = 5, MGB_AGG_SUM_OF_SQUARES = 8,
   MGB_AGG_MEAN = 9, MGB_AGG_M2 = 10, MGB_AGG_VARIANCE = 11, MGB_AGG_STD = 12,
-  MGB_AGG_ARGMAX = 15, MGB_AGG_ARGMIN = 16
+  MGB_AGG_MEDIAN = 13, MGB_AGG_QUANTILE = 14, MGB_AGG_ARGMAX = 15, MGB_AGG_ARGMIN = 16,
+  MGB_AGG_NUNIQUE = 17, MGB_AGG_NTH_ELEMENT = 18
 };
+
+/* cudf::interpolation (cpp/include/cudf/types.hpp). */
+enum { MGB_INTERP_LINEAR = 0, MGB_INTERP_LOWER = 1, MGB_INTERP_HIGHER = 2, MGB_INTERP_MIDPOINT = 3,
+       MGB_INTERP_NEAREST = 4 };
 
 /* cudf::null_policy (cpp/include/cudf/types.hpp). */
 enum { MGB_NULL_EXCLUDE = 0, MGB_NULL_INCLUDE = 1 };
@@ -76,11 +81,16 @@ typedef struct mgb_column {
   int64_t null_count;        /* nulls in [offset, offset+size); -1 = unknown */
 } mgb_column;
 
-/* One aggregation: kind plus its parameter (ddof for VARIANCE / STD, the
- * null_policy for COUNT is expressed by COUNT_VALID vs COUNT_ALL). */
+/* One aggregation: kind plus its parameters (the null_policy of COUNT is
+ * expressed by COUNT_VALID vs COUNT_ALL). Zero-initialised fields are the
+ * defaults except where noted. */
 typedef struct mgb_aggregation {
   int32_t kind;
-  int32_t ddof;
+  int32_t ddof;           /* VARIANCE / STD: ddof; NTH_ELEMENT: n (negative: from the end) */
+  int32_t null_policy;    /* NUNIQUE, NTH_ELEMENT: MGB_NULL_EXCLUDE / MGB_NULL_INCLUDE (cudf's
+                             defaults are EXCLUDE for NUNIQUE and INCLUDE for NTH_ELEMENT) */
+  int32_t interpolation;  /* QUANTILE: MGB_INTERP_* */
+  double quantile;        /* QUANTILE: one quantile in [0, 1] (MEDIAN is 0.5, LINEAR) */
 } mgb_aggregation;
 
 /* cudf::groupby::aggregation_request (cpp/include/cudf/groupby.hpp:59-62). */

@@ -60,7 +60,7 @@ enum {
 enum {
   K_SUM = 0, K_PRODUCT = 1, K_MIN = 2, K_MAX = 3, K_COUNT_VALID = 4, K_COUNT_ALL = 5,
   K_SUM_OF_SQUARES = 8, K_MEAN = 9, K_M2 = 10, K_VARIANCE = 11, K_STD = 12,
-  K_ARGMAX = 15, K_ARGMIN = 16
+  K_MEDIAN = 13, K_QUANTILE = 14, K_ARGMAX = 15, K_ARGMIN = 16, K_NUNIQUE = 17, K_NTH_ELEMENT = 18
 };
 
 typedef struct {
@@ -276,6 +276,13 @@ int mgo_target_type(int t, int kind)
     case K_MIN: case K_MAX: return t;
     case K_COUNT_VALID: case K_COUNT_ALL: return T_INT32;
     case K_ARGMIN: case K_ARGMAX: return T_INT32;
+    /* sort-path aggregations (aggregation.hpp:1290-1330): quantiles of numeric
+     * sources are double, NUNIQUE is size_type, NTH_ELEMENT keeps the type */
+    case K_MEDIAN: case K_QUANTILE:
+      if (!is_integral(t) && !is_float(t)) return -1;
+      return T_FLOAT64;
+    case K_NUNIQUE: return T_INT32;
+    case K_NTH_ELEMENT: return t;
     case K_SUM:
       if (is_integral(t)) return T_INT64;
       if (is_float(t) || is_duration(t)) return t;
@@ -316,11 +323,122 @@ typedef struct {
  * first appearance; out_data[j] / out_valid[j] = request j's result per group
  * (capacity num_rows each). Returns 0, or 1 for an invalid type/aggregation pair.
  */
-int mgo_groupby_aggregate(const mgo_col* keys, int32_t n_keys, int64_t num_rows, int32_t include_null_keys,
-                          const mgo_col* values, int32_t n_values, const int32_t* req_col,
-                          const int32_t* req_kind, const int32_t* req_ddof, int32_t n_req,
-                          int64_t* out_rep_rows, int64_t* out_num_groups, void* const* out_data,
-                          uint8_t* const* out_valid)
+/* ------------------------------------------------ sort-path aggregations */
+
+/* Value order of cudf's sort of a column (ascending, NaN after every number;
+ * -0.0 and +0.0 equal): the context of the qsort comparator below. */
+static const mgo_col* g_sort_col;
+static int value_cmp(int64_t a, int64_t b)
+{
+  int t = g_sort_col->type_id;
+  if (is_float(t)) {
+    double x = load_f(t, g_sort_col->data, a), y = load_f(t, g_sort_col->data, b);
+    int nx = isnan(x), ny = isnan(y);
+    if (nx || ny) return nx - ny;
+    return x < y ? -1 : x > y ? 1 : 0;
+  }
+  if (is_unsigned_int(t)) {
+    uint64_t x = (uint64_t)load_i(t, g_sort_col->data, a), y = (uint64_t)load_i(t, g_sort_col->data, b);
+    return x < y ? -1 : x > y ? 1 : 0;
+  }
+  int64_t x = load_i(t, g_sort_col->data, a), y = load_i(t, g_sort_col->data, b);
+  return x < y ? -1 : x > y ? 1 : 0;
+}
+static int row_cmp(const void* pa, const void* pb)
+{
+  int64_t a = *(const int64_t*)pa, b = *(const int64_t*)pb;
+  int c = value_cmp(a, b);
+  return c ? c : (a < b ? -1 : a > b);
+}
+static double value_f64(const mgo_col* v, int64_t r)
+{
+  return is_float(v->type_id) ? load_f(v->type_id, v->data, r)
+         : is_unsigned_int(v->type_id) ? (double)(uint64_t)load_i(v->type_id, v->data, r)
+                                       : (double)load_i(v->type_id, v->data, r);
+}
+
+/* One quantile of the sorted values rows[0..n) (quantiles_util.hpp:90-105
+ * quantile_index, :161-181 select_quantile_data, :40-80 interpolation). */
+static double select_quantile(const mgo_col* v, const int64_t* rows, int64_t n, double q, int interp)
+{
+  if (q < 0.0) q = 0.0;
+  if (q > 1.0) q = 1.0;
+  double val = q * (double)(n - 1);
+  int64_t lo = (int64_t)floor(val), hi = (int64_t)ceil(val), ne = (int64_t)nearbyint(val);
+  double frac = val - (double)lo;
+  switch (interp) {
+    case 1: return value_f64(v, rows[lo]);   /* LOWER */
+    case 2: return value_f64(v, rows[hi]);   /* HIGHER */
+    case 4: return value_f64(v, rows[ne]);   /* NEAREST */
+    case 3:                                  /* MIDPOINT */
+      if (v->type_id == T_INT64) {
+        int64_t a = load_i(T_INT64, v->data, rows[lo]), b = load_i(T_INT64, v->data, rows[hi]);
+        int64_t half = a / 2 + b / 2, rest = a % 2 + b % 2;
+        return (double)half + (double)rest * 0.5;
+      }
+      return value_f64(v, rows[lo]) / 2 + value_f64(v, rows[hi]) / 2;
+    default: {                               /* LINEAR */
+      double a = value_f64(v, rows[lo]), b = value_f64(v, rows[hi]);
+      return (1.0 - frac) * a + frac * b;
+    }
+  }
+}
+
+/* MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT of every group: grp_rows[grp_off[g]..
+ * grp_off[g+1]) are group g's rows in row order. */
+static void sort_path_agg(const mgo_col* v, int kind, int ddof, int nulls, int interp, double q, int64_t ng,
+                          const int64_t* grp_off, const int64_t* grp_rows, void* out, uint8_t* valid)
+{
+  int64_t max_n = 1;
+  for (int64_t g = 0; g < ng; ++g)
+    if (grp_off[g + 1] - grp_off[g] > max_n) max_n = grp_off[g + 1] - grp_off[g];
+  int64_t* buf = (int64_t*)malloc(sizeof(int64_t) * (size_t)max_n);
+  int w = type_size(v->type_id);
+  for (int64_t g = 0; g < ng; ++g) {
+    const int64_t* rows = grp_rows + grp_off[g];
+    int64_t size = grp_off[g + 1] - grp_off[g], n = 0, has_null = 0;
+    if (kind == K_NTH_ELEMENT) {
+      /* the n-th row (n < 0: from the end) of the group, of all its rows
+       * (INCLUDE) or of its valid values (EXCLUDE) (group_nth_element.cu:60-120) */
+      for (int64_t i = 0; i < size; ++i)
+        if (nulls == 1 || bit_is_set(v->mask, rows[i])) buf[n++] = rows[i];
+      int64_t idx = ddof < 0 ? n + ddof : ddof;
+      int ok = idx >= 0 && idx < n && bit_is_set(v->mask, buf[idx]);
+      valid[g] = (uint8_t)ok;
+      memset((uint8_t*)out + g * w, 0, (size_t)w);
+      if (ok) memcpy((uint8_t*)out + g * w, (const uint8_t*)v->data + buf[idx] * w, (size_t)w);
+      continue;
+    }
+    for (int64_t i = 0; i < size; ++i) {
+      if (bit_is_set(v->mask, rows[i])) buf[n++] = rows[i];
+      else has_null = 1;
+    }
+    g_sort_col = v;
+    qsort(buf, (size_t)n, sizeof(int64_t), row_cmp);
+    if (kind == K_NUNIQUE) {
+      /* distinct values in sorted order, NaN == NaN; nulls one more value
+       * under INCLUDE (group_nunique.cu:60-70) */
+      int32_t c = 0;
+      for (int64_t i = 0; i < n; ++i) c += (i == 0 || value_cmp(buf[i - 1], buf[i]) != 0);
+      if (nulls == 1 && has_null) c += 1;
+      ((int32_t*)out)[g] = c;
+      valid[g] = 1;
+      continue;
+    }
+    /* MEDIAN (0.5, LINEAR) and QUANTILE: null without values (group_quantiles.cu:66-74) */
+    valid[g] = (uint8_t)(n > 0);
+    ((double*)out)[g] = n > 0 ? select_quantile(v, buf, n, kind == K_MEDIAN ? 0.5 : q, kind == K_MEDIAN ? 0 : interp)
+                              : 0.0;
+  }
+  free(buf);
+}
+
+int mgo_groupby_aggregate_ex(const mgo_col* keys, int32_t n_keys, int64_t num_rows, int32_t include_null_keys,
+                             const mgo_col* values, int32_t n_values, const int32_t* req_col,
+                             const int32_t* req_kind, const int32_t* req_ddof, const int32_t* req_nulls,
+                             const int32_t* req_interp, const double* req_q, int32_t n_req,
+                             int64_t* out_rep_rows, int64_t* out_num_groups, void* const* out_data,
+                             uint8_t* const* out_valid)
 {
   (void)n_values;
   for (int32_t j = 0; j < n_req; ++j)
@@ -364,11 +482,31 @@ int mgo_groupby_aggregate(const mgo_col* keys, int32_t n_keys, int64_t num_rows,
   a.m2 = (double*)malloc(sizeof(double) * gsz);
   a.valid = (uint8_t*)malloc(gsz);
 
+  /* each group's rows in row order (for the sort-path aggregations) */
+  int64_t* grp_off = (int64_t*)calloc((size_t)ng + 1, sizeof(int64_t));
+  int64_t* grp_rows = (int64_t*)malloc(sizeof(int64_t) * (size_t)(num_rows ? num_rows : 1));
+  for (int64_t r = 0; r < num_rows; ++r)
+    if (group_of_row[r] >= 0) grp_off[group_of_row[r] + 1]++;
+  for (int64_t g = 0; g < ng; ++g) grp_off[g + 1] += grp_off[g];
+  {
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ng ? ng : 1));
+    for (int64_t g = 0; g < ng; ++g) fill[g] = grp_off[g];
+    for (int64_t r = 0; r < num_rows; ++r)
+      if (group_of_row[r] >= 0) grp_rows[fill[group_of_row[r]]++] = r;
+    free(fill);
+  }
+
   for (int32_t j = 0; j < n_req; ++j) {
     const mgo_col* v = &values[req_col[j]];
     int t = v->type_id, kind = req_kind[j];
     int tt = mgo_target_type(t, kind);
     int fl = is_float(t);
+    if (kind == K_MEDIAN || kind == K_QUANTILE || kind == K_NUNIQUE || kind == K_NTH_ELEMENT) {
+      sort_path_agg(v, kind, req_ddof ? req_ddof[j] : 0, req_nulls ? req_nulls[j] : 0,
+                    req_interp ? req_interp[j] : 0, req_q ? req_q[j] : 0.5, ng, grp_off, grp_rows, out_data[j],
+                    out_valid[j]);
+      continue;
+    }
     for (int64_t g = 0; g < ng; ++g) {
       a.cnt[g] = 0; a.valid[g] = 0; a.m2[g] = 0.0;
       /* identities (device_operators.cuh:63-200) */
@@ -524,6 +662,19 @@ int mgo_groupby_aggregate(const mgo_col* keys, int32_t n_keys, int64_t num_rows,
     }
   }
   free(a.i); free(a.f); free(a.f32); free(a.cnt); free(a.m2); free(a.valid);
+  free(grp_off); free(grp_rows);
   free(group_of_row);
   return 0;
+}
+
+/* The hash-path entry point: no sort-path parameters (defaults). */
+int mgo_groupby_aggregate(const mgo_col* keys, int32_t n_keys, int64_t num_rows, int32_t include_null_keys,
+                          const mgo_col* values, int32_t n_values, const int32_t* req_col,
+                          const int32_t* req_kind, const int32_t* req_ddof, int32_t n_req,
+                          int64_t* out_rep_rows, int64_t* out_num_groups, void* const* out_data,
+                          uint8_t* const* out_valid)
+{
+  return mgo_groupby_aggregate_ex(keys, n_keys, num_rows, include_null_keys, values, n_values, req_col, req_kind,
+                                  req_ddof, NULL, NULL, NULL, n_req, out_rep_rows, out_num_groups, out_data,
+                                  out_valid);
 }

@@ -37,6 +37,7 @@ NP = {
 # cudf::aggregation::Kind values
 SUM, PRODUCT, MIN, MAX, COUNT_VALID, COUNT_ALL = 0, 1, 2, 3, 4, 5
 SUM_OF_SQUARES, MEAN, M2, VARIANCE, STD, ARGMAX, ARGMIN = 8, 9, 10, 11, 12, 15, 16
+MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT = 13, 14, 17, 18
 
 
 class _Col(ctypes.Structure):
@@ -71,11 +72,12 @@ def load():
                                                      ctypes.c_uint32, ctypes.c_void_p]
         lib.mgo_spark_murmurhash3_x86_32_table.restype = ctypes.c_int
         lib.mgo_spark_murmurhash3_x86_32_table.argtypes = lib.mgo_murmurhash3_x86_32_table.argtypes
-        lib.mgo_groupby_aggregate.restype = ctypes.c_int
-        lib.mgo_groupby_aggregate.argtypes = [
+        lib.mgo_groupby_aggregate_ex.restype = ctypes.c_int
+        lib.mgo_groupby_aggregate_ex.argtypes = [
             ctypes.POINTER(_Col), ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
             ctypes.POINTER(_Col), ctypes.c_int32,
             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+            ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
             ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
             ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
         ]
@@ -152,21 +154,27 @@ def groupby(keys: Sequence[HostColumn], requests: Sequence[Tuple[HostColumn, Seq
             include_null_keys: bool = False):
     """The C restatement of cudf's hash groupby.
 
-    requests: [(values, [kind or (kind, ddof), ...]), ...]
+    requests: [(values, [kind, (kind, ddof) or (kind, ddof, null_policy,
+    interpolation, quantile), ...]), ...]; ddof is n for NTH_ELEMENT
     Returns (key columns [(values, valid)], results [[(values, valid), ...] per request]),
     groups in order of first appearance."""
     lib = load()
     n = len(keys[0]) if keys else 0
     vals: List[HostColumn] = []
-    req_col, req_kind, req_ddof, shape = [], [], [], []
+    req_col, req_kind, req_ddof, req_nulls, req_interp, req_q, shape = [], [], [], [], [], [], []
     for v, aggs in requests:
         vals.append(v)
         shape.append(len(aggs))
         for a in aggs:
-            kind, ddof = (a, 1) if isinstance(a, int) else a
+            spec = (a, 1) if isinstance(a, int) else tuple(a)
+            kind, ddof = spec[0], spec[1]
+            nulls, interp, q = spec[2:5] if len(spec) >= 5 else (0, 0, 0.5)
             req_col.append(len(vals) - 1)
             req_kind.append(kind)
             req_ddof.append(ddof)
+            req_nulls.append(nulls)
+            req_interp.append(interp)
+            req_q.append(q)
     nreq = len(req_kind)
     out_data, out_valid = [], []
     for j in range(nreq):
@@ -182,8 +190,10 @@ def groupby(keys: Sequence[HostColumn], requests: Sequence[Tuple[HostColumn, Seq
     i32 = ctypes.c_int32 * max(nreq, 1)
     dptr = (ctypes.c_void_p * max(nreq, 1))(*[d.ctypes.data for d in out_data])
     vptr = (ctypes.c_void_p * max(nreq, 1))(*[d.ctypes.data for d in out_valid])
-    rc = lib.mgo_groupby_aggregate(karr, len(keys), n, 1 if include_null_keys else 0, varr, len(vals),
-                                   i32(*req_col), i32(*req_kind), i32(*req_ddof), nreq,
+    f64 = ctypes.c_double * max(nreq, 1)
+    rc = lib.mgo_groupby_aggregate_ex(karr, len(keys), n, 1 if include_null_keys else 0, varr, len(vals),
+                                   i32(*req_col), i32(*req_kind), i32(*req_ddof), i32(*req_nulls),
+                                   i32(*req_interp), f64(*req_q), nreq,
                                    reps.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(ng),
                                    dptr, vptr)
     if rc != 0:

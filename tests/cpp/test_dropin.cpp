@@ -313,6 +313,45 @@ void second_pass_tests()
                     {1, 2, 3, 4}, {3 / std::sqrt(2.), 7 / std::sqrt(3.), 3 * std::sqrt(2.), 0.}, {1, 1, 1, 0});
 }
 
+// quantile_tests.cpp:124-152 (two quantiles: a flattened [groups x 2]
+// column), median_tests.cpp:35-53, nunique_tests.cpp:173-198 (INCLUDE),
+// nth_element_tests.cpp:94-119
+template <typename V>
+void sort_path_tests()
+{
+  wrapper<K> keys(cudf::type_id::INT32, kKeys);
+  wrapper<V> vals(tid<V>(), iota_vals<V>(10));
+  std::vector<cudf::groupby::aggregation_request> r(1);
+  r[0].values = vals.view();
+  r[0].aggregations.push_back(
+    cudf::make_quantile_aggregation<cudf::groupby_aggregation>({0.25, 0.75}, cudf::interpolation::LINEAR));
+  r[0].aggregations.push_back(cudf::make_median_aggregation<cudf::groupby_aggregation>());
+  r[0].aggregations.push_back(cudf::make_nth_element_aggregation<cudf::groupby_aggregation>(-2));
+  cudf::groupby::groupby gb(cudf::table_view({keys.view()}));
+  auto [uk, res] = gb.aggregate(r);
+  auto k = to_host<K>(uk->get_column(0).view());
+  auto q = to_host<double>(res[0].results[0]->view());
+  auto m = to_host<double>(res[0].results[1]->view());
+  auto nth = to_host<V>(res[0].results[2]->view());
+  CHECK(res[0].results[0]->size() == 6 && q.size() == 6);
+  const std::map<K, std::vector<double>> want_q{{1, {1.5, 4.5}}, {2, {3.25, 6.}}, {3, {4.5, 7.5}}};
+  const std::map<K, double> want_m{{1, 3.}, {2, 4.5}, {3, 7.}};
+  const std::map<K, double> want_nth{{1, 3.}, {2, 5.}, {3, 7.}};
+  for (size_t g = 0; g < k.size() && q.size() == 6; ++g) {
+    CHECK(same_value(q[2 * g], want_q.at(k[g])[0]) && same_value(q[2 * g + 1], want_q.at(k[g])[1]));
+    CHECK(same_value(m[g], want_m.at(k[g])));
+    CHECK(same_value(static_cast<double>(nth[g]), want_nth.at(k[g])));
+  }
+  wrapper<K> k14(cudf::type_id::INT32, {1, 2, 3, 3, 1, 2, 2, 1, 3, 3, 2, 4, 4, 2},
+                 {1, 1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 1});
+  wrapper<V> v14(tid<V>(), std::vector<V>{0, 1, 2, 2, 3, 4, 5, 6, 7, 8, 9, 4, 4, 2},
+                 {0, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 0, 0, 0});
+  expect<K, int32_t>(run_single<K, int32_t>(k14.view(), v14.view(),
+                                            cudf::make_nunique_aggregation<cudf::groupby_aggregation>(
+                                              cudf::null_policy::INCLUDE)),
+                     {1, 2, 3, 4}, {3, 4, 2, 1});
+}
+
 void error_tests()
 {
   // groupby.cu:208-212: size mismatch; :176-189: invalid type/aggregation
@@ -372,6 +411,7 @@ int main()
     {"minmax_count<float>", minmax_count_tests<float>},   {"minmax_count<double>", minmax_count_tests<double>},
     {"second_pass<int32>", second_pass_tests<int32_t>},   {"second_pass<double>", second_pass_tests<double>},
     {"second_pass<float>", second_pass_tests<float>},
+    {"sort_path<int32>", sort_path_tests<int32_t>},      {"sort_path<double>", sort_path_tests<double>},
     {"keys", keys_tests},                        {"multi_request", multi_request_tests},
     {"errors", error_tests},                     {"hashing", hashing_tests},
   };

@@ -15,7 +15,19 @@ TYPES = {"INT8": O.INT8, "INT16": O.INT16, "INT32": O.INT32, "INT64": O.INT64, "
          "FLOAT64": O.FLOAT64, "BOOL8": O.BOOL8}
 KINDS = {"SUM": O.SUM, "PRODUCT": O.PRODUCT, "MIN": O.MIN, "MAX": O.MAX, "COUNT_VALID": O.COUNT_VALID,
          "COUNT_ALL": O.COUNT_ALL, "SUM_OF_SQUARES": O.SUM_OF_SQUARES, "MEAN": O.MEAN,
-         "VARIANCE": O.VARIANCE, "STD": O.STD, "ARGMAX": O.ARGMAX, "ARGMIN": O.ARGMIN, "M2": O.M2}
+         "VARIANCE": O.VARIANCE, "STD": O.STD, "ARGMAX": O.ARGMAX, "ARGMIN": O.ARGMIN, "M2": O.M2,
+         "MEDIAN": O.MEDIAN, "QUANTILE": O.QUANTILE, "NUNIQUE": O.NUNIQUE, "NTH_ELEMENT": O.NTH_ELEMENT}
+INTERP = {"LINEAR": 0, "LOWER": 1, "HIGHER": 2, "MIDPOINT": 3, "NEAREST": 4}
+NULLS = {"EXCLUDE": 0, "INCLUDE": 1}
+
+
+def agg_spec(c, kind):
+    """The oracle's aggregation spec of a case: kind, (kind, ddof) or
+    (kind, ddof or n, null_policy, interpolation, quantile)."""
+    if kind in (O.MEDIAN, O.QUANTILE, O.NUNIQUE, O.NTH_ELEMENT):
+        return (kind, c.get("n", 0), NULLS[c.get("nulls", "EXCLUDE")], INTERP[c.get("interp", "LINEAR")],
+                c.get("q", 0.5))
+    return (kind, c["ddof"]) if "ddof" in c else kind
 
 
 def _num(x):
@@ -52,7 +64,7 @@ def load_cases():
             exp_vals = np.array([_num(x) for x in c["expect_vals"]], dtype=np.float64).astype(O.NP[tt])
             exp_valid = np.array(c.get("expect_valid", [1] * len(c["expect_vals"])), bool)
             out.append(dict(
-                id=f"{c['name']}[{kt}->{vt}]", cite=c["cite"], kind=(kind, c["ddof"]) if "ddof" in c else kind,
+                id=f"{c['name']}[{kt}->{vt}]", cite=c["cite"], kind=agg_spec(c, kind),
                 include_null_keys=c.get("null_policy") == "INCLUDE",
                 keys=(k_t, keys, kvalid), vals=(v_t, vals, vvalid), target=tt,
                 expect_keys=(exp_keys, exp_keys_valid), expect=(exp_vals, exp_valid)))

@@ -18,6 +18,8 @@ TYPE_NAMES = {"INT8": O.INT8, "INT16": O.INT16, "INT32": O.INT32, "INT64": O.INT
 
 def to_agg(kind) -> ca.aggregation.Aggregation:
     """kind, or (kind, ddof) as the oracle takes it."""
+    if isinstance(kind, tuple) and len(kind) >= 5:
+        return ca.aggregation.Aggregation(ca.Kind(kind[0]), kind[1], kind[2], kind[3], kind[4])
     if isinstance(kind, tuple):
         return ca.aggregation.Aggregation(ca.Kind(kind[0]), kind[1])
     return ca.aggregation.Aggregation(ca.Kind(kind), 1 if kind in (O.VARIANCE, O.STD) else 0)

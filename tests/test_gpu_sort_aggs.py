@@ -1,0 +1,141 @@
+"""GPU parity of the aggregations cudf serves from its sort-based groupby
+(MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT) through the C ABI, against the
+oracle on seeded inputs: every value type, each interpolation, both null
+policies, NaN and ties, negative and out-of-range n, composite and null
+keys, each requested execution (the partitioned one gives way to the global
+one), and sizes up to 2M rows. Results must match exactly, except float
+quantiles between two values (within 1e-12 relative, the interpolation's
+rounding)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import gpu_groupby, oracle_groupby, sorted_result
+
+pytestmark = pytest.mark.gpu
+
+
+def Q(q, interp=0):
+    return (O.QUANTILE, 0, 0, interp, q)
+
+
+def NTH(n, include=True):
+    return (O.NTH_ELEMENT, n, 1 if include else 0, 0, 0.5)
+
+
+def NU(include=False):
+    return (O.NUNIQUE, 0, 1 if include else 0, 0, 0.5)
+
+
+MED = (O.MEDIAN, 0, 0, 0, 0.5)
+MIX = [MED, Q(0.25), Q(0.9, 1), Q(0.33, 2), Q(0.5, 3), Q(0.7, 4), NU(), NU(True), NTH(0), NTH(-1), NTH(2, False),
+       NTH(-3, False), NTH(50)]
+
+
+def _compare(keys, reqs, include_null_keys=False):
+    gk, gr = gpu_groupby(keys, reqs, include_null_keys)
+    ok, orr = oracle_groupby(keys, reqs, include_null_keys)
+    gk, gr = sorted_result(gk, gr)
+    ok, orr = sorted_result(ok, orr)
+    assert len(gk[0][0]) == len(ok[0][0]), "group count differs"
+    for (gv, gvalid), (ov, ovalid) in zip(gk, ok):
+        gm = np.ones(len(gv), bool) if gvalid is None else np.asarray(gvalid, bool)
+        om = np.ones(len(ov), bool) if ovalid is None else np.asarray(ovalid, bool)
+        assert np.array_equal(gm, om) and np.array_equal(gv[om], ov[om]), "keys differ"
+    for r, (gblk, oblk) in enumerate(zip(gr, orr)):
+        for a, ((g, gvalid), (o, ovalid)) in zip(reqs[r][1], zip(gblk, oblk)):
+            what = f"request {r} agg {a}"
+            gm = np.ones(len(g), bool) if gvalid is None else np.asarray(gvalid, bool)
+            om = np.ones(len(o), bool) if ovalid is None else np.asarray(ovalid, bool)
+            assert np.array_equal(gm, om), f"{what}: validity differs at {np.flatnonzero(gm != om)[:5]}"
+            g, o = np.asarray(g)[om], np.asarray(o)[om]
+            if g.dtype.kind == "f":
+                same = (g == o) | (np.isnan(g) & np.isnan(o)) | (np.abs(g - o) <= 1e-12 * np.abs(o))
+                assert same.all(), f"{what}: {int((~same).sum())} groups differ, e.g. {g[~same][:3]} vs {o[~same][:3]}"
+            else:
+                assert np.array_equal(g, o), f"{what}: differs at {np.flatnonzero(g != o)[:5]}"
+
+
+def _values(rng, vt, n, ties=False):
+    if vt in (O.FLOAT32, O.FLOAT64):
+        if ties:
+            pool = np.array([-np.inf, -1.5, -0.0, 0.0, 2.5, np.inf, np.nan], dtype=O.NP[vt])
+            return pool[rng.integers(0, len(pool), n)]
+        return (rng.standard_normal(n) * 50).astype(O.NP[vt])
+    info = np.iinfo(O.NP[vt])
+    if ties:
+        return rng.integers(max(info.min, -4), min(info.max, 4), n, endpoint=True).astype(O.NP[vt])
+    return rng.integers(max(info.min, -10 ** 9), min(info.max, 10 ** 9), n, endpoint=True).astype(O.NP[vt])
+
+
+@pytest.mark.parametrize("vt", [O.INT8, O.INT16, O.INT32, O.INT64, O.UINT8, O.UINT32, O.UINT64, O.FLOAT32,
+                                O.FLOAT64])
+def test_value_types(gpu, vt):
+    rng = np.random.default_rng(700 + vt)
+    n = 50_000
+    keys = rng.integers(0, 3000, n).astype(np.int32)
+    reqs = [((vt, _values(rng, vt, n), rng.random(n) > 0.1), MIX)]
+    _compare([(O.INT32, keys, None)], reqs)
+
+
+@pytest.mark.parametrize("vt", [O.INT16, O.FLOAT32, O.FLOAT64])
+def test_ties_and_nan(gpu, vt):
+    rng = np.random.default_rng(720 + vt)
+    n = 60_000
+    keys = rng.integers(0, 500, n).astype(np.int64)
+    _compare([(O.INT64, keys, None)], [((vt, _values(rng, vt, n, ties=True), rng.random(n) > 0.05), MIX)])
+
+
+PATHS = ["auto", "partitioned", "direct", "global"]
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("card", [7, 20_000])
+def test_every_execution(gpu, monkeypatch, path, card):
+    if path != "auto":
+        monkeypatch.setenv("MGB_GROUPBY_PATH", path)
+    rng = np.random.default_rng(740 + card)
+    n = 300_000
+    keys = rng.integers(-2 ** 40, 2 ** 40, card)[rng.integers(0, card, n)]
+    reqs = [((O.FLOAT64, _values(rng, O.FLOAT64, n), rng.random(n) > 0.02), [MED, Q(0.1), NU(True), NTH(-2)]),
+            ((O.INT32, _values(rng, O.INT32, n, ties=True), None), [NU(), NTH(1, False), Q(0.5, 3)])]
+    _compare([(O.INT64, keys, None)], reqs)
+
+
+def test_composite_and_null_keys(gpu):
+    rng = np.random.default_rng(760)
+    n = 80_000
+    a = rng.integers(0, 100, n).astype(np.int32)
+    b = rng.integers(0, 20, n).astype(np.float64) / 2
+    keys = [(O.INT32, a, rng.random(n) > 0.05), (O.FLOAT64, b, None)]
+    reqs = [((O.INT64, _values(rng, O.INT64, n), rng.random(n) > 0.1), [MED, Q(0.3, 4), NU(True), NTH(0, False)])]
+    _compare(keys, reqs)
+    _compare(keys, reqs, include_null_keys=True)
+
+
+def test_mixed_with_hash_aggregations(gpu):
+    rng = np.random.default_rng(770)
+    n = 100_000
+    keys = rng.integers(0, 8000, n).astype(np.int32)
+    v = rng.standard_normal(n)
+    valid = rng.random(n) > 0.1
+    reqs = [((O.FLOAT64, v, valid), [O.SUM, MED, (O.VARIANCE, 1), NU(), O.ARGMAX, NTH(-1), O.COUNT_VALID])]
+    gk, gr = gpu_groupby([(O.INT32, keys, None)], reqs)
+    ok_, orr = oracle_groupby([(O.INT32, keys, None)], reqs)
+    gk, gr = sorted_result(gk, gr)
+    ok_, orr = sorted_result(ok_, orr)
+    for (g, gv), (o, ov), a in zip(gr[0], orr[0], reqs[0][1]):
+        gm = np.ones(len(g), bool) if gv is None else np.asarray(gv, bool)
+        om = np.ones(len(o), bool) if ov is None else np.asarray(ov, bool)
+        assert np.array_equal(gm, om), a
+        assert np.allclose(np.asarray(g, np.float64)[om], np.asarray(o, np.float64)[om], rtol=1e-9, equal_nan=True), a
+
+
+def test_large(gpu):
+    rng = np.random.default_rng(780)
+    n = 2_000_000
+    keys = rng.integers(0, 200_000, n).astype(np.int64) * 31
+    reqs = [((O.FLOAT64, _values(rng, O.FLOAT64, n), rng.random(n) > 0.05), [MED, Q(0.95), NU(), NTH(3)])]
+    _compare([(O.INT64, keys, None)], reqs)

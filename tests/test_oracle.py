@@ -334,3 +334,68 @@ def test_oracle_float_min_max_nan_and_zero(vt):
             assert np.signbit(mn[g]) == np.signbit(x[x == 0]).any()
         if hi == 0:
             assert np.signbit(mx[g]) == np.signbit(x[x == 0]).all()
+
+
+# ---------------------------------------------------------------- sort-path aggregations
+
+NP_METHOD = {0: "linear", 1: "lower", 2: "higher", 3: "midpoint", 4: "nearest"}
+
+
+@pytest.mark.parametrize("vt", [O.INT16, O.INT64, O.UINT32, O.FLOAT32, O.FLOAT64])
+@pytest.mark.parametrize("interp", [0, 1, 2, 3, 4])
+def test_oracle_quantiles_against_numpy(vt, interp):
+    """QUANTILE / MEDIAN per group = numpy.quantile of the group's valid
+    values with the matching method; null for a group without values."""
+    rng = np.random.default_rng(90 + vt * 7 + interp)
+    n = 3000
+    keys = _tiny_and_large_groups(rng, n)
+    if vt in (O.FLOAT32, O.FLOAT64):
+        vals = (rng.standard_normal(n) * 100).astype(O.NP[vt])
+    else:
+        info = np.iinfo(O.NP[vt])
+        vals = rng.integers(max(info.min, -10 ** 6), min(info.max, 10 ** 6), n, endpoint=True).astype(O.NP[vt])
+    valid = rng.random(n) > 0.2
+    qs = [0.0, 0.13, 0.5, 0.77, 1.0]
+    aggs = [(O.QUANTILE, 0, 0, interp, q) for q in qs] + [(O.MEDIAN, 0, 0, 0, 0.5)]
+    ks, res = O.groupby([O.HostColumn(O.INT32, keys)], [(O.HostColumn(vt, vals, valid), aggs)])
+    for g, k in enumerate(ks[0][0]):
+        x = np.sort(vals[(keys == k) & valid].astype(np.float64))
+        for a, (out, ok) in zip(aggs, res[0]):
+            assert bool(ok[g]) == (len(x) > 0)
+            if not len(x):
+                continue
+            method = NP_METHOD[a[3]]
+            want = np.quantile(x, a[4], method=method)
+            assert abs(out[g] - want) <= 1e-9 * max(1.0, abs(want)), (k, a, out[g], want)
+
+
+@pytest.mark.parametrize("vt", [O.INT8, O.INT32, O.FLOAT64])
+def test_oracle_nunique_and_nth_against_python(vt):
+    rng = np.random.default_rng(95 + vt)
+    n = 3000
+    keys = _tiny_and_large_groups(rng, n)
+    if vt == O.FLOAT64:
+        vals = rng.choice(np.array([-0.0, 0.0, 1.5, np.nan, 2.5]), n)
+    else:
+        vals = rng.integers(-3, 4, n).astype(O.NP[vt])
+    valid = rng.random(n) > 0.25
+    aggs = [(O.NUNIQUE, 0, 0, 0, 0.5), (O.NUNIQUE, 0, 1, 0, 0.5)] + \
+           [(O.NTH_ELEMENT, m, p, 0, 0.5) for m in (0, 2, -1, -3) for p in (0, 1)]
+    ks, res = O.groupby([O.HostColumn(O.INT32, keys)], [(O.HostColumn(vt, vals, valid), aggs)])
+    for g, k in enumerate(ks[0][0]):
+        rows = np.flatnonzero(keys == k)
+        vv = [float(vals[r]) for r in rows if valid[r]]
+        distinct = set()
+        for x in vv:  # NaN == NaN, -0.0 == +0.0
+            distinct.add("nan" if np.isnan(x) else x + 0.0)
+        for a, (out, ok) in zip(aggs, res[0]):
+            if a[0] == O.NUNIQUE:
+                want = len(distinct) + (1 if a[2] == 1 and (~valid[rows]).any() else 0)
+                assert out[g] == want and ok[g], (k, a, out[g], want)
+            else:
+                pool = list(rows) if a[2] == 1 else [r for r in rows if valid[r]]
+                idx = a[1] if a[1] >= 0 else len(pool) + a[1]
+                hit = 0 <= idx < len(pool) and valid[pool[idx]]
+                assert bool(ok[g]) == hit, (k, a)
+                if hit:
+                    assert out[g] == vals[pool[idx]] or (np.isnan(out[g]) and np.isnan(vals[pool[idx]])), (k, a)
