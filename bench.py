@@ -35,10 +35,10 @@ HBM_PEAK_GBS = 8000.0
 
 # library profiling scopes (mgb_profiling_read) and the kernels they time
 PROFILED = ("aggregate", "lds_aggregate", "fill", "finalize", "partition_hist", "partition_scatter", "estimate",
-            "pass2")
+            "pass2", "sort_aggs")
 KERNEL_SYMBOL = {"aggregate": "mgb::aggregate_kernel", "lds_aggregate": "mgb::lds_agg_kernel<false>",
                  "partition_scatter": "mgb::part_scatter_tiled_kernel", "partition_hist": "mgb::part_hist_kernel",
-                 "pass2": "mgb::pass2_kernel"}
+                 "pass2": "mgb::pass2_kernel", "sort_aggs": "sort-path kernels and hipCUB radix sorts"}
 
 
 def pack_bits(valid):
@@ -123,6 +123,14 @@ def make_workload(name: str, rows: int, seed: int, device: str = "cuda"):
         return ([key_col], req, 8 + 4 + 8 + 4 + 3 / 8, 8 + 3 * (8 + 8 + 8 + 4 + 4) + 15 / 8,
                 f"groupby-aggregate, {rows:,} rows, int64 key over {distinct:,} distinct values, int32/f64/f32 "
                 "values 5% null, mean/variance/std/argmin/argmax per value column")
+    if name == "c2s":
+        # the headline data with aggregations cudf serves from its sort path
+        key_col, cols, distinct = make_c2(rows, 1_000_000, seed, device)
+        req = [ca.GroupByRequest(c, [agg.median(), agg.quantile(0.9), agg.nunique(), agg.nth_element(0)])
+               for c in cols]
+        return ([key_col], req, 8 + 4 + 8 + 4 + 3 / 8, 8 + 3 * (8 + 8 + 4) + 4 + 8 + 4 + 15 / 8,
+                f"groupby-aggregate, {rows:,} rows, int64 key over {distinct:,} distinct values, int32/f64/f32 "
+                "values 5% null, median/quantile(0.9)/nunique/nth_element(0) per value column")
     if name == "c1":
         keys = torch.randint(0, 1000, (rows,), generator=g, device=device, dtype=torch.int32)
         vals = torch.rand(rows, generator=g, device=device, dtype=torch.float64)
@@ -156,7 +164,7 @@ def make_workload(name: str, rows: int, seed: int, device: str = "cuda"):
     raise SystemExit(f"unknown workload {name}")
 
 
-DEFAULT_ROWS = {"c1": 1_000_000, "c2": 100_000_000, "c2v": 100_000_000, "c3a": 200_000_000, "c3b": 200_000_000}
+DEFAULT_ROWS = {"c1": 1_000_000, "c2": 100_000_000, "c2v": 100_000_000, "c2s": 100_000_000, "c3a": 200_000_000, "c3b": 200_000_000}
 
 
 def make_c4(num_queries: int, seed: int, device: str = "cuda", max_rows: float = 1e7):
@@ -446,7 +454,10 @@ def oracle_sample(name: str, sample_rows: int, distinct: int, seed: int):
     kinds5 = [O.SUM, O.COUNT_VALID, O.MIN, O.MAX, O.MEAN]
     if name == "c2v":
         kinds5 = [O.MEAN, (O.VARIANCE, 1), (O.STD, 1), O.ARGMIN, O.ARGMAX]
-    if name in ("c2", "c2v"):
+    if name == "c2s":
+        kinds5 = [(O.MEDIAN, 0, 0, 0, 0.5), (O.QUANTILE, 0, 0, 0, 0.9), (O.NUNIQUE, 0, 0, 0, 0.5),
+                  (O.NTH_ELEMENT, 0, 1, 0, 0.5)]
+    if name in ("c2", "c2v", "c2s"):
         uniq = np.unique(rng.integers(-2 ** 63, 2 ** 63 - 1, distinct, dtype=np.int64))
         keys = [O.HostColumn(O.INT64, uniq[rng.integers(0, uniq.size, n)])]
         reqs = [(O.HostColumn(O.INT32, rng.integers(-1_000_000, 1_000_001, n).astype(np.int32), rng.random(n) >= 0.05),
@@ -485,7 +496,9 @@ def cpu_baseline_oracle(name: str, sample_rows: int, distinct: int, seed: int, p
     t = time.perf_counter() - t0
     pd_line = (cpu_baseline_c2(pandas_rows, distinct, seed) if name == "c2"
                else cpu_baseline_c2(pandas_rows, distinct, seed, aggs=["mean", "var", "std", "idxmin", "idxmax"])
-               if name == "c2v" else cpu_baseline_generic(name, pandas_rows, seed))
+               if name == "c2v"
+               else cpu_baseline_c2(pandas_rows, distinct, seed, aggs=["median", "nunique", "first"])
+               if name == "c2s" else cpu_baseline_generic(name, pandas_rows, seed))
     return {"value": sample_rows / t, "unit": "rows/s", "cores": 1, "kind": "port",
             "sample": f"the oracle's C groupby (oracle/groupby_oracle.c, one thread) on {sample_rows:,} rows drawn "
                       f"like the workload, one call of {t:.1f} s; host os.cpu_count()={os.cpu_count()}",
@@ -630,7 +643,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c2v", "c3a", "c3b", "c4", "c5", "pandas"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c2v", "c2s", "c3a", "c3b", "c4", "c5", "pandas"])
     ap.add_argument("--tables", type=int, default=32, help="c5: resident tables")
     ap.add_argument("--max-table-rows", type=float, default=5e7, help="c5: largest table (rows)")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0, help="c5: HBM budget of the server (GiB)")
@@ -779,7 +792,7 @@ def main():
     out_bytes = groups * out_bytes_per_group
     # the dominant kernel: the longest of the passes that each read every row
     # once (its algorithmic bytes: the input columns, read once)
-    dom = max((k for k in ("aggregate", "lds_aggregate", "partition_scatter", "pass2") if k in kern),
+    dom = max((k for k in ("aggregate", "lds_aggregate", "partition_scatter", "pass2", "sort_aggs") if k in kern),
               key=lambda k: kern[k][1] / kern[k][0])
     agg_n, agg_ms = kern[dom]
     symbol = KERNEL_SYMBOL[dom]

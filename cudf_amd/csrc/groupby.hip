@@ -1553,14 +1553,19 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
   return hll_estimate(ws, base, R, stream, st);
 }
 
-// Global-table executions (global and direct paths): build the table, then
-// compact it into temporary outputs.
 // The sort-path aggregations (MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT) of a
 // call whose global table is complete and finalised: `rec_pos` gives every
-// record's output row. Every row's group is looked up once; then per output
-// the rows are radix-sorted by value and (stably) by group, or by group
-// alone for NTH_ELEMENT, and the results read off the sorted rows
-// (cpp/src/groupby/sort/sort_helper.cu orders the rows by key the same way).
+// record's output row. Every row's group is looked up once. The outputs then
+// share orderings: one per value column for MEDIAN, QUANTILE and NUNIQUE
+// (each group's valid values ascending), one per value column for
+// NTH_ELEMENT EXCLUDE and one for every NTH_ELEMENT INCLUDE (each group's
+// rows in row order). cpp/src/groupby/sort/sort_helper.cu orders the rows by
+// key the same way; the results are read off the sorted rows.
+//
+// Value orderings of columns of at most 4 bytes are one keys-only radix sort
+// of (group << 32 | value code) over 32 + log2(groups) bits; 8-byte columns
+// sort by the 64-bit code and then stably by group, with the code as payload.
+// Results are decoded from the sorted codes, so no value is gathered.
 mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows, const uint32_t* rec_pos,
                          int64_t groups, hipStream_t stream, TempOuts& temp)
 {
@@ -1572,92 +1577,143 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
   const uint32_t G = (uint32_t)groups;
   int gbits = 1;
   while (gbits < 32 && (1ull << gbits) <= (unsigned long long)G) ++gbits;
-  // radix-sort space: the largest of the three uses
-  size_t t64 = 0, t32 = 0, tscan = 0;
+
+  // the orderings the outputs need, in order of first use
+  struct Ordering { int col; int mode; };
+  std::vector<Ordering> orderings;
+  std::vector<int> out_ordering(plan.outs.size(), -1);
+  for (size_t o = 0; o < plan.outs.size(); ++o) {
+    const Plan::Out& out = plan.outs[o];
+    if (out.spec.fin != FIN_NONE) continue;
+    Ordering want{out.col, SORT_BY_VALUE};
+    if (out.kind == MGB_AGG_NTH_ELEMENT)
+      want = out.agg.null_policy == MGB_NULL_INCLUDE ? Ordering{-1, SORT_ROWS_ALL} : Ordering{out.col, SORT_ROWS_VALID};
+    int k = 0;
+    while (k < (int)orderings.size() && !(orderings[k].col == want.col && orderings[k].mode == want.mode)) ++k;
+    if (k == (int)orderings.size()) orderings.push_back(want);
+    out_ordering[o] = k;
+  }
+
+  // radix-sort and scan space: the largest of the uses
+  size_t tk = 0, t64 = 0, t64b = 0, t32 = 0, tscan = 0;
+  MGB_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tk, (const uint64_t*)nullptr, (uint64_t*)nullptr, n, 0,
+                                            32 + gbits, stream));
   MGB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t64, (const uint64_t*)nullptr, (uint64_t*)nullptr,
                                              (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, 64, stream));
+  MGB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t64b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (const uint64_t*)nullptr, (uint64_t*)nullptr, n, 0, gbits, stream));
   MGB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t32, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                              (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, gbits, stream));
-  MGB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tscan, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)G,
-                                           stream));
-  const size_t tcub = align_up(std::max({t64, t32, tscan}), 256);
+  MGB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tscan, (const uint32_t*)nullptr, (uint32_t*)nullptr, n, stream));
+  const size_t tcub = align_up(std::max({tk, t64, t64b, t32, tscan}), 256);
   const size_t r4 = align_up((size_t)R * 4, 256), r8 = align_up((size_t)R * 8, 256), g4 = align_up((size_t)G * 4, 256);
-  // gid, gkey, idx, order1, gk2, sorted_gkey, order | code, code_out | count, offset, has_null | cub
-  const size_t need = 7 * r4 + 2 * r8 + 3 * g4 + tcub;
+  const size_t o4 = align_up(((size_t)G + 1) * 4, 256);
+  // gid | A, B (8 bytes a row) | C, D, E (4 bytes a row) | offset, has_null | cub
+  const size_t need = 4 * r4 + 2 * r8 + o4 + g4 + tcub;
   MGB_HIP(ws->sorts.reserve(need, stream));
   uint8_t* b = static_cast<uint8_t*>(ws->sorts.ptr);
   size_t off = 0;
   auto take = [&](size_t bytes) { uint8_t* p = b + off; off += bytes; return p; };
   uint32_t* gid = reinterpret_cast<uint32_t*>(take(r4));
-  uint32_t* gkey = reinterpret_cast<uint32_t*>(take(r4));
-  uint32_t* idx = reinterpret_cast<uint32_t*>(take(r4));
-  uint32_t* order1 = reinterpret_cast<uint32_t*>(take(r4));
-  uint32_t* gk2 = reinterpret_cast<uint32_t*>(take(r4));
-  uint32_t* sorted_gkey = reinterpret_cast<uint32_t*>(take(r4));
-  uint32_t* order = reinterpret_cast<uint32_t*>(take(r4));
-  uint64_t* code = reinterpret_cast<uint64_t*>(take(r8));
-  uint64_t* code_out = reinterpret_cast<uint64_t*>(take(r8));
-  uint32_t* count = reinterpret_cast<uint32_t*>(take(g4));
-  uint32_t* offset = reinterpret_cast<uint32_t*>(take(g4));
+  uint8_t* A = take(r8);
+  uint8_t* B = take(r8);
+  uint32_t* C = reinterpret_cast<uint32_t*>(take(r4));
+  uint32_t* D = reinterpret_cast<uint32_t*>(take(r4));
+  uint32_t* E = reinterpret_cast<uint32_t*>(take(r4));
+  uint32_t* offset = reinterpret_cast<uint32_t*>(take(o4));
   uint32_t* has_null = reinterpret_cast<uint32_t*>(take(g4));
   void* cub_tmp = take(tcub);
 
   const int cus = device_cu_count();
   const unsigned grid_rows = (unsigned)std::max<int64_t>(1, std::min<int64_t>((R + kBlock - 1) / kBlock, (int64_t)cus * 16));
   const unsigned grid_groups = (unsigned)((groups + kBlock - 1) / kBlock);
-  {
-    ProfScope prof("sort_aggs", stream);
-    row_group_kernel<<<grid_rows, kBlock, 0, stream>>>(rows, rec_pos, G, gid);
-    MGB_HIP(hipGetLastError());
-    for (size_t o = 0; o < plan.outs.size(); ++o) {
-      const Plan::Out& out = plan.outs[o];
-      if (out.spec.fin != FIN_NONE) continue;
-      SortAggParams s{};
-      s.v = plan.cols[out.col].col;
-      s.num_rows = R;
-      s.groups = G;
-      s.kind = out.kind;
-      s.nulls = out.agg.null_policy == MGB_NULL_INCLUDE ? 1 : 0;
-      s.interp = out.kind == MGB_AGG_MEDIAN ? 0 : out.agg.interpolation;
-      s.n = out.agg.ddof;
-      s.q = out.kind == MGB_AGG_MEDIAN ? 0.5 : out.agg.quantile;
-      s.gid = gid;
-      s.gkey = gkey;
-      s.code = code;
-      s.idx = idx;
-      s.count = count;
-      s.has_null = has_null;
-      s.order = order;
-      s.sorted_gkey = sorted_gkey;
-      s.offset = offset;
-      s.out = temp.out_data[o];
-      s.out_valid = temp.out_valid[o];
-      s.out_width = out.spec.out_width;
-      if (!s.out_valid) return fail(MGB_ERROR_HIP, "sort-path output without validity");
-      MGB_HIP(hipMemsetAsync(count, 0, (size_t)G * 4, stream));
+  ProfScope prof("sort_aggs", stream);
+  row_group_kernel<<<grid_rows, kBlock, 0, stream>>>(rows, rec_pos, G, gid);
+  MGB_HIP(hipGetLastError());
+  for (int k = 0; k < (int)orderings.size(); ++k) {
+    const Ordering ord = orderings[k];
+    SortPrepParams sp{};
+    if (ord.col >= 0) sp.v = plan.cols[ord.col].col;
+    sp.num_rows = R;
+    sp.groups = G;
+    sp.mode = ord.mode;
+    sp.narrow = ord.mode == SORT_BY_VALUE && sp.v.width <= 4;
+    sp.gid = gid;
+    sp.key64 = reinterpret_cast<uint64_t*>(A);
+    sp.code = reinterpret_cast<uint64_t*>(A);
+    sp.gkey = C;
+    sp.idx = reinterpret_cast<uint32_t*>(A);
+    bool want_nulls = false;
+    for (size_t o = 0; o < plan.outs.size(); ++o)
+      if (out_ordering[o] == k && plan.outs[o].kind == MGB_AGG_NUNIQUE &&
+          plan.outs[o].agg.null_policy == MGB_NULL_INCLUDE)
+        want_nulls = true;
+    if (want_nulls) {
       MGB_HIP(hipMemsetAsync(has_null, 0, (size_t)G * 4, stream));
-      sort_prepare_kernel<<<grid_rows, kBlock, 0, stream>>>(s);
-      MGB_HIP(hipGetLastError());
-      size_t tb = tcub;
-      if (out.kind != MGB_AGG_NTH_ELEMENT) {
-        // by value, then stably by group: each group's rows in value order
-        MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, code, code_out, idx, order1, n, 0, 64, stream));
-        gather_u32_kernel<<<grid_rows, kBlock, 0, stream>>>(gkey, order1, R, gk2);
-        MGB_HIP(hipGetLastError());
-        tb = tcub;
-        MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, gk2, sorted_gkey, order1, order, n, 0, gbits, stream));
-      } else {
-        // by group alone: each group's rows in row order
-        MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, gkey, sorted_gkey, idx, order, n, 0, gbits, stream));
-      }
+      sp.has_null = has_null;
+    }
+    sort_prepare_kernel<<<grid_rows, kBlock, 0, stream>>>(sp);
+    MGB_HIP(hipGetLastError());
+    SortView view{};
+    view.n = R;
+    view.groups = G;
+    view.narrow = sp.narrow;
+    size_t tb = tcub;
+    if (ord.mode != SORT_BY_VALUE) {
+      // by group alone (stable: each group's rows stay in row order)
+      MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, C, D, reinterpret_cast<uint32_t*>(A),
+                                                 reinterpret_cast<uint32_t*>(B), n, 0, gbits, stream));
+      view.gkey = D;
+      view.order = reinterpret_cast<const uint32_t*>(B);
+    } else if (sp.narrow) {
+      MGB_HIP(hipcub::DeviceRadixSort::SortKeys(cub_tmp, tb, reinterpret_cast<const uint64_t*>(A),
+                                                reinterpret_cast<uint64_t*>(B), n, 0, 32 + gbits, stream));
+      view.key64 = reinterpret_cast<const uint64_t*>(B);
+    } else {
+      // by value, then stably by group, the code riding along
+      MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, reinterpret_cast<const uint64_t*>(A),
+                                                 reinterpret_cast<uint64_t*>(B), C, D, n, 0, 64, stream));
       tb = tcub;
-      MGB_HIP(hipcub::DeviceScan::ExclusiveSum(cub_tmp, tb, count, offset, (int)G, stream));
+      MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, D, E, reinterpret_cast<const uint64_t*>(B),
+                                                 reinterpret_cast<uint64_t*>(A), n, 0, gbits, stream));
+      view.gkey = E;
+      view.code = reinterpret_cast<const uint64_t*>(A);
+    }
+    view.offset = offset;
+    group_bounds_kernel<<<grid_rows, kBlock, 0, stream>>>(view, offset);
+    MGB_HIP(hipGetLastError());
+    bool scanned = false;  // NUNIQUE's prefix sums (in D) for this ordering
+    for (size_t o = 0; o < plan.outs.size(); ++o) {
+      if (out_ordering[o] != k) continue;
+      const Plan::Out& out = plan.outs[o];
+      SortOutParams p{};
+      p.v = plan.cols[out.col].col;
+      p.s = view;
+      p.kind = out.kind;
+      p.interp = out.kind == MGB_AGG_MEDIAN ? 0 : out.agg.interpolation;
+      p.n = out.agg.ddof;
+      p.nulls = out.agg.null_policy == MGB_NULL_INCLUDE ? 1 : 0;
+      p.q = out.kind == MGB_AGG_MEDIAN ? 0.5 : out.agg.quantile;
+      p.has_null = has_null;
+      p.out = temp.out_data[o];
+      p.out_valid = temp.out_valid[o];
+      p.out_width = out.spec.out_width;
+      if (!p.out_valid) return fail(MGB_ERROR_HIP, "sort-path output without validity");
       if (out.kind == MGB_AGG_NUNIQUE) {
-        MGB_HIP(hipMemsetAsync(s.out, 0, (size_t)G * 4, stream));
-        nunique_count_kernel<<<grid_rows, kBlock, 0, stream>>>(s, R);
-        nunique_finish_kernel<<<grid_groups, kBlock, 0, stream>>>(s);
+        if (!scanned) {
+          // C is free once the sorts are done; D holds sorted groups only in
+          // the row orderings, which NUNIQUE does not use
+          const int32_t is_float = p.v.vclass == VC_F32 || p.v.vclass == VC_F64;
+          distinct_start_kernel<<<grid_rows, kBlock, 0, stream>>>(view, is_float, C);
+          MGB_HIP(hipGetLastError());
+          tb = tcub;
+          MGB_HIP(hipcub::DeviceScan::InclusiveSum(cub_tmp, tb, C, D, n, stream));
+          scanned = true;
+        }
+        p.prefix = D;
+        nunique_finish_kernel<<<grid_groups, kBlock, 0, stream>>>(p);
       } else {
-        sort_select_kernel<<<grid_groups, kBlock, 0, stream>>>(s);
+        sort_select_kernel<<<grid_groups, kBlock, 0, stream>>>(p);
       }
       MGB_HIP(hipGetLastError());
     }
@@ -1665,6 +1721,8 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
   return MGB_SUCCESS;
 }
 
+// Global-table executions (global and direct paths): build the table, then
+// compact it into temporary outputs.
 mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int64_t R, int64_t G_est, bool direct,
                      const LdsGeom& geom, hipStream_t stream, TempOuts& temp, int64_t& groups)
 {

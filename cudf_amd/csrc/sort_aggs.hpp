@@ -1,9 +1,9 @@
 // Device routines of the aggregations cudf serves from its sort-based groupby
 // (MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT; cpp/src/groupby/sort/). The hash
-// table assigns every row its group (the group's output row); two stable
-// radix sorts (hipCUB) then order each value column's rows by group and, for
-// the value-ordered aggregations, by value within the group; small kernels
-// read the results off the sorted rows.
+// table assigns every row its group (the group's output row); radix sorts
+// (hipCUB) then order each value column's rows by group and, for the
+// value-ordered aggregations, by value within the group; small kernels read
+// the results off the sorted rows.
 #pragma once
 
 #include "groupby_host.hpp"
@@ -25,170 +25,243 @@ __global__ __launch_bounds__(kBlock) void row_group_kernel(const AggParams p, co
   }
 }
 
-// A value in the order of cudf's sort (ascending; NaN after every number):
-// an unsigned code whose order is the value order. -0.0 and +0.0 get adjacent
-// codes (equal values; their relative order does not change a result's value).
-__device__ __forceinline__ uint64_t sort_code(uint64_t raw, int width, int vclass)
+// Values in the order of cudf's sort (ascending; NaN after every number) as
+// unsigned codes whose order is the value order. Columns of at most 4 bytes
+// take 32-bit codes (the group goes above them in one 64-bit sort key),
+// 8-byte columns 64-bit codes. -0.0 and +0.0 get adjacent codes (equal
+// values; their relative order does not change a result's value).
+__device__ __forceinline__ uint32_t sort_code32(uint64_t raw, int width, int vclass)
 {
-  if (vclass == VC_F32 || vclass == VC_F64) {
-    const double x = widen_f64(raw, width, vclass);
-    if (x != x) return ~0ull;
-    const uint64_t b = (uint64_t)__double_as_longlong(x);
-    return (b >> 63) ? ~b : (b | (1ull << 63));
+  if (vclass == VC_F32) {
+    const float x = __uint_as_float((uint32_t)raw);
+    if (x != x) return ~0u;
+    const uint32_t b = (uint32_t)raw;
+    return (b >> 31) ? ~b : (b | 0x80000000u);
   }
-  if (vclass == VC_SIGNED) return (uint64_t)widen_int(raw, width, vclass) ^ (1ull << 63);
-  return (uint64_t)widen_int(raw, width, vclass);
+  if (vclass == VC_SIGNED) return (uint32_t)(int32_t)widen_int(raw, width, vclass) ^ 0x80000000u;
+  return (uint32_t)widen_int(raw, width, vclass);
 }
 
-struct SortAggParams {
-  ValCol v;               // the value column (input column, its own validity mask)
+__device__ __forceinline__ uint64_t sort_code64(uint64_t raw, int vclass)
+{
+  if (vclass == VC_F64) {
+    const double x = __longlong_as_double((long long)raw);
+    if (x != x) return ~0ull;
+    return (raw >> 63) ? ~raw : (raw | (1ull << 63));
+  }
+  if (vclass == VC_SIGNED) return raw ^ (1ull << 63);
+  return raw;
+}
+
+// The value a code stands for, as double (MEDIAN and QUANTILE results).
+__device__ __forceinline__ double code_value(uint64_t c, bool narrow, int vclass)
+{
+  if (narrow) {
+    const uint32_t c32 = (uint32_t)c;
+    if (vclass == VC_F32) return (double)__uint_as_float((c32 >> 31) ? (c32 & 0x7fffffffu) : ~c32);
+    if (vclass == VC_SIGNED) return (double)(int32_t)(c32 ^ 0x80000000u);
+    return (double)c32;
+  }
+  if (vclass == VC_F64) return __longlong_as_double((long long)((c >> 63) ? (c & ~(1ull << 63)) : ~c));
+  if (vclass == VC_SIGNED) return (double)(int64_t)(c ^ (1ull << 63));
+  return (double)c;
+}
+
+// The code of -0.0 (+0.0's is the next one up).
+__device__ __forceinline__ uint64_t neg_zero_code(bool narrow) { return narrow ? 0x7fffffffull : ~(1ull << 63); }
+
+// What to sort the rows of one value column by.
+enum SortMode : int32_t {
+  SORT_BY_VALUE = 0,   // valid values, by (group, value): MEDIAN, QUANTILE, NUNIQUE
+  SORT_ROWS_VALID = 1, // valid values, by group in row order: NTH_ELEMENT EXCLUDE
+  SORT_ROWS_ALL = 2,   // every row of a group, by group in row order: NTH_ELEMENT INCLUDE
+};
+
+struct SortPrepParams {
+  ValCol v;
   int64_t num_rows;
   uint32_t groups;
+  int32_t mode;
+  int32_t narrow;         // SORT_BY_VALUE on a column of at most 4 bytes
+  int32_t pad;
+  const uint32_t* gid;    // group of every row (row_group_kernel)
+  uint64_t* key64;        // narrow: group << 32 | 32-bit code
+  uint64_t* code;         // 8-byte values: 64-bit code
+  uint32_t* gkey;         // not narrow: the row's group if it takes part, else `groups`
+  uint32_t* idx;          // row order: the row's index
+  uint32_t* has_null;     // optional, per group: 1 when a row of the group has a null value
+};
+
+// Sort keys of every row; rows that take no part get group `groups` and sort last.
+__global__ __launch_bounds__(kBlock) void sort_prepare_kernel(const SortPrepParams s)
+{
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < s.num_rows; row += step) {
+    const uint32_t g = s.gid[row];
+    if (s.mode == SORT_ROWS_ALL) {
+      s.gkey[row] = g < s.groups ? g : s.groups;
+      s.idx[row] = (uint32_t)row;
+      continue;
+    }
+    const bool valid = bit_valid(s.v.mask, s.v.bit_off + row);
+    const bool part = g < s.groups && valid;
+    if (s.has_null && g < s.groups && !valid) s.has_null[g] = 1u;
+    if (s.mode == SORT_ROWS_VALID) {
+      s.gkey[row] = part ? g : s.groups;
+      s.idx[row] = (uint32_t)row;
+    } else if (s.narrow) {
+      const uint32_t c = part ? sort_code32(load_at(s.v.data, s.v.width, s.v.stride, row), s.v.width, s.v.vclass) : 0u;
+      s.key64[row] = ((uint64_t)(part ? g : s.groups) << 32) | c;
+    } else {
+      s.gkey[row] = part ? g : s.groups;
+      s.code[row] = part ? sort_code64(load_at(s.v.data, 8, s.v.stride, row), s.v.vclass) : 0ull;
+    }
+  }
+}
+
+// One ordering's sorted rows.
+struct SortView {
+  const uint64_t* key64;   // narrow: sorted group << 32 | code
+  const uint32_t* gkey;    // otherwise: sorted groups
+  const uint64_t* code;    // 8-byte values: sorted codes
+  const uint32_t* order;   // row order: the sorted rows' indices
+  const uint32_t* offset;  // groups + 1 entries: first sorted position of every group
+  int64_t n;
+  uint32_t groups;
+  int32_t narrow;
+};
+
+__device__ __forceinline__ uint32_t view_group(const SortView& s, int64_t i)
+{
+  return s.narrow ? (uint32_t)(s.key64[i] >> 32) : s.gkey[i];
+}
+
+__device__ __forceinline__ uint64_t view_code(const SortView& s, int64_t i)
+{
+  return s.narrow ? (s.key64[i] & 0xffffffffull) : s.code[i];
+}
+
+// offset[g] = first sorted position of group g (every position i from 1 to n
+// fills the groups between its predecessor's group and its own).
+__global__ __launch_bounds__(kBlock) void group_bounds_kernel(const SortView s, uint32_t* offset)
+{
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= s.n; i += step) {
+    const int64_t prev = i == 0 ? -1 : (int64_t)view_group(s, i - 1);
+    const int64_t cur = i == s.n ? (int64_t)s.groups : (int64_t)view_group(s, i);
+    for (int64_t g = prev + 1; g <= cur; ++g) offset[g] = (uint32_t)i;
+  }
+}
+
+struct SortOutParams {
+  ValCol v;               // the value column (NTH_ELEMENT reads its rows)
+  SortView s;
   int32_t kind;           // MGB_AGG_MEDIAN / QUANTILE / NUNIQUE / NTH_ELEMENT
-  int32_t nulls;          // NUNIQUE / NTH_ELEMENT: 1 = INCLUDE
   int32_t interp;         // QUANTILE: 0 LINEAR, 1 LOWER, 2 HIGHER, 3 MIDPOINT, 4 NEAREST
   int32_t n;              // NTH_ELEMENT
-  int32_t pad;
+  int32_t nulls;          // NUNIQUE: 1 = INCLUDE
   double q;               // QUANTILE
-  const uint32_t* gid;    // group of every row (row_group_kernel)
-  uint32_t* gkey;         // per row: the row's group if it takes part, else `groups`
-  uint64_t* code;         // per row: sort_code of its value (value-ordered kinds)
-  uint32_t* idx;          // per row: its index (the sorts' payload)
-  uint32_t* count;        // per group: rows taking part
-  uint32_t* has_null;     // per group: 1 when a row of the group has a null value (NUNIQUE INCLUDE)
-  const uint32_t* order;  // the rows sorted by (group, value) or by group (row order kept)
-  const uint32_t* sorted_gkey;  // gkey of order[i]
-  const uint32_t* offset; // per group: first position of its rows in `order`
+  const uint32_t* prefix; // NUNIQUE: inclusive sum of the distinct-value starts
+  const uint32_t* has_null;
   void* out;              // per group: result
   uint8_t* out_valid;     // per group: validity byte
   int32_t out_width;
 };
 
-__device__ __forceinline__ bool sort_value_valid(const ValCol& v, int64_t row)
-{
-  return bit_valid(v.mask, v.bit_off + row);
-}
-
-// Which rows take part, their sort codes, and the per-group counts.
-__global__ __launch_bounds__(kBlock) void sort_prepare_kernel(const SortAggParams s)
-{
-  const int64_t step = (int64_t)gridDim.x * blockDim.x;
-  const bool by_value = s.kind != MGB_AGG_NTH_ELEMENT;
-  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < s.num_rows; row += step) {
-    const uint32_t g = s.gid[row];
-    const bool valid = sort_value_valid(s.v, row);
-    // value-ordered kinds take the valid values; NTH_ELEMENT every row of
-    // the group (INCLUDE) or its valid values (EXCLUDE)
-    const bool takes_part = g < s.groups && (valid || (!by_value && s.nulls));
-    s.gkey[row] = takes_part ? g : s.groups;
-    s.idx[row] = (uint32_t)row;
-    if (by_value) s.code[row] = valid ? sort_code(load_at(s.v.data, s.v.width, s.v.stride, row), s.v.width, s.v.vclass) : 0ull;
-    if (takes_part) atomicAdd(&s.count[g], 1u);
-    if (g < s.groups && !valid && s.has_null) s.has_null[g] = 1u;
-  }
-}
-
-// gkey of the rows in `order` (between the value sort and the group sort).
-__global__ __launch_bounds__(kBlock) void gather_u32_kernel(const uint32_t* src, const uint32_t* order, int64_t n,
-                                                            uint32_t* dst)
-{
-  const int64_t step = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) dst[i] = src[order[i]];
-}
-
-__device__ __forceinline__ double sort_value_f64(const ValCol& v, uint32_t row)
-{
-  return widen_f64(load_at(v.data, v.width, v.stride, row), v.width, v.vclass);
-}
-
 // MEDIAN / QUANTILE / NTH_ELEMENT of every group (one thread per group).
-__global__ __launch_bounds__(kBlock) void sort_select_kernel(const SortAggParams s)
+__global__ __launch_bounds__(kBlock) void sort_select_kernel(const SortOutParams p)
 {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= s.groups) return;
-  const int64_t n = s.count[g], off = s.offset[g];
-  if (s.kind == MGB_AGG_NTH_ELEMENT) {
+  if (g >= p.s.groups) return;
+  const int64_t off = p.s.offset[g], n = (int64_t)p.s.offset[g + 1] - off;
+  if (p.kind == MGB_AGG_NTH_ELEMENT) {
     // group_nth_element.cu:60-120: the n-th row (from the end when n < 0)
-    const int64_t i = s.n < 0 ? n + s.n : s.n;
+    const int64_t i = p.n < 0 ? n + p.n : p.n;
     bool ok = i >= 0 && i < n;
     uint64_t bits = 0;
     if (ok) {
-      const uint32_t row = s.order[off + i];
-      ok = sort_value_valid(s.v, row);
-      if (ok) bits = load_at(s.v.data, s.v.width, s.v.stride, row);
+      const uint32_t row = p.s.order[off + i];
+      ok = bit_valid(p.v.mask, p.v.bit_off + row);
+      if (ok) bits = load_at(p.v.data, p.v.width, p.v.stride, row);
     }
-    store_width(s.out, g, s.out_width, bits);
-    s.out_valid[g] = ok;
+    store_width(p.out, g, p.out_width, bits);
+    p.out_valid[g] = ok;
     return;
   }
   // MEDIAN (0.5, LINEAR) and QUANTILE: quantiles_util.hpp:90-105 (index),
   // :161-181 (selection), :40-80 (interpolation); null without values
   if (n == 0) {
-    reinterpret_cast<double*>(s.out)[g] = 0.0;
-    s.out_valid[g] = 0;
+    reinterpret_cast<double*>(p.out)[g] = 0.0;
+    p.out_valid[g] = 0;
     return;
   }
-  const double q = s.q < 0.0 ? 0.0 : s.q > 1.0 ? 1.0 : s.q;
+  const bool narrow = p.s.narrow != 0;
+  const int vc = p.v.vclass;
+  const double q = p.q < 0.0 ? 0.0 : p.q > 1.0 ? 1.0 : p.q;
   const double val = q * (double)(n - 1);
   const int64_t lo = (int64_t)floor(val), hi = (int64_t)ceil(val), ne = (int64_t)nearbyint(val);
   const double frac = val - (double)lo;
   double r;
-  switch (s.interp) {
-    case 1: r = sort_value_f64(s.v, s.order[off + lo]); break;
-    case 2: r = sort_value_f64(s.v, s.order[off + hi]); break;
-    case 4: r = sort_value_f64(s.v, s.order[off + ne]); break;
+  switch (p.interp) {
+    case 1: r = code_value(view_code(p.s, off + lo), narrow, vc); break;
+    case 2: r = code_value(view_code(p.s, off + hi), narrow, vc); break;
+    case 4: r = code_value(view_code(p.s, off + ne), narrow, vc); break;
     case 3:
-      if (s.v.vclass == VC_SIGNED && s.v.width == 8) {
-        const int64_t a = (int64_t)load_at(s.v.data, 8, s.v.stride, s.order[off + lo]);
-        const int64_t b = (int64_t)load_at(s.v.data, 8, s.v.stride, s.order[off + hi]);
+      if (!narrow && vc == VC_SIGNED) {
+        // exact for int64: halves and the remainders' half
+        const int64_t a = (int64_t)(view_code(p.s, off + lo) ^ (1ull << 63));
+        const int64_t b = (int64_t)(view_code(p.s, off + hi) ^ (1ull << 63));
         const int64_t half = a / 2 + b / 2, rest = a % 2 + b % 2;
         r = (double)half + (double)rest * 0.5;
       } else {
-        r = sort_value_f64(s.v, s.order[off + lo]) / 2 + sort_value_f64(s.v, s.order[off + hi]) / 2;
+        r = code_value(view_code(p.s, off + lo), narrow, vc) / 2 + code_value(view_code(p.s, off + hi), narrow, vc) / 2;
       }
       break;
     default: {
-      const double a = sort_value_f64(s.v, s.order[off + lo]), b = sort_value_f64(s.v, s.order[off + hi]);
+      const double a = code_value(view_code(p.s, off + lo), narrow, vc);
+      const double b = code_value(view_code(p.s, off + hi), narrow, vc);
       r = (1.0 - frac) * a + frac * b;
       break;
     }
   }
-  reinterpret_cast<double*>(s.out)[g] = r;
-  s.out_valid[g] = 1;
+  reinterpret_cast<double*>(p.out)[g] = r;
+  p.out_valid[g] = 1;
 }
 
-// NUNIQUE: every sorted position that starts its group or differs from the
-// previous value counts one (group_nunique.cu:60-70; NaN == NaN, -0.0 ==
-// +0.0, as cudf's row equality); `total` positions take part.
-__global__ __launch_bounds__(kBlock) void nunique_count_kernel(const SortAggParams s, int64_t total)
+// NUNIQUE: 1 at every sorted position that starts its group or differs from
+// the previous value (group_nunique.cu:60-70; NaN == NaN, -0.0 == +0.0, as
+// cudf's row equality).
+__global__ __launch_bounds__(kBlock) void distinct_start_kernel(const SortView s, int32_t is_float, uint32_t* flag)
 {
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += step) {
-    const uint32_t g = s.sorted_gkey[i];
-    if (g >= s.groups) continue;  // rows that take no part sort last
-    bool first = i == (int64_t)s.offset[g];
-    if (!first) {
-      const uint64_t a = load_at(s.v.data, s.v.width, s.v.stride, s.order[i - 1]);
-      const uint64_t b = load_at(s.v.data, s.v.width, s.v.stride, s.order[i]);
-      if (s.v.vclass == VC_F32 || s.v.vclass == VC_F64) {
-        const double x = widen_f64(a, s.v.width, s.v.vclass), y = widen_f64(b, s.v.width, s.v.vclass);
-        first = !((x == y) || (x != x && y != y));
+  const uint64_t nz = neg_zero_code(s.narrow != 0);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += step) {
+    uint32_t f = 1;
+    if (i > 0) {
+      if (s.narrow) {
+        const uint64_t a = s.key64[i - 1], b = s.key64[i];
+        const bool zeros = is_float && (a >> 32) == (b >> 32) && (a & 0xffffffffull) == nz && (b & 0xffffffffull) == nz + 1;
+        f = (a != b && !zeros) ? 1u : 0u;
       } else {
-        first = widen_int(a, s.v.width, s.v.vclass) != widen_int(b, s.v.width, s.v.vclass);
+        const uint64_t a = s.code[i - 1], b = s.code[i];
+        const bool same = a == b || (is_float && a == nz && b == nz + 1);
+        f = (s.gkey[i - 1] != s.gkey[i] || !same) ? 1u : 0u;
       }
     }
-    if (first) atomicAdd(reinterpret_cast<uint32_t*>(s.out) + g, 1u);
+    flag[i] = f;
   }
 }
 
-// NUNIQUE INCLUDE: a group with a null value counts one more; every group's
-// count is valid.
-__global__ __launch_bounds__(kBlock) void nunique_finish_kernel(const SortAggParams s)
+// NUNIQUE per group from the prefix sums; INCLUDE counts a null value once.
+__global__ __launch_bounds__(kBlock) void nunique_finish_kernel(const SortOutParams p)
 {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= s.groups) return;
-  if (s.nulls && s.has_null[g]) reinterpret_cast<uint32_t*>(s.out)[g] += 1u;
-  s.out_valid[g] = 1;
+  if (g >= p.s.groups) return;
+  const uint32_t lo = p.s.offset[g], hi = p.s.offset[g + 1];
+  uint32_t c = hi > lo ? p.prefix[hi - 1] - (lo ? p.prefix[lo - 1] : 0u) : 0u;
+  if (p.nulls && p.has_null[g]) c += 1u;
+  reinterpret_cast<uint32_t*>(p.out)[g] = c;
+  p.out_valid[g] = 1;
 }
 
 }  // namespace mgb
