@@ -46,8 +46,9 @@ struct Query {
   std::string group_key;          // table and key columns: the cache key of its group-count estimate
   std::vector<std::vector<mgb_aggregation>> aggs;
   std::string signature;  // queries of one batch share it
-  // a second-pass aggregation (VARIANCE, STD, ARGMIN, ARGMAX): batches do
-  // not serve it, so the query runs alone on the plain path
+  // a second-pass or sort-path aggregation (VARIANCE, STD, M2, ARGMIN,
+  // ARGMAX, MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT) or null keys grouped under
+  // INCLUDE: batches do not serve it, so the query runs alone on the plain path
   bool plain_only = false;
   Clock::time_point arrival;
 };
