@@ -195,9 +195,13 @@ __global__ __launch_bounds__(kLdsBlock) void batch_scatter_kernel(const BatchQue
 // thread.
 __device__ __forceinline__ void global_rows(const AggParams& p, int64_t begin, int64_t end)
 {
+  uint32_t iter = 0;
   for (int64_t row = begin + threadIdx.x; row < end; row += kLdsBlock) {
+    // the overflow flag every 32 rows rather than at every probe
+    if ((++iter & 31u) == 0u && __hip_atomic_load(&p.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return;
     if (!key_row_valid(p, row)) continue;
-    uint8_t* g = global_find_or_insert(p, row_key(p, row), row);
+    uint8_t* g = global_find_or_insert(p, row_key(p, row), row, false);
     if (g) apply_row(p, g, row);
   }
 }

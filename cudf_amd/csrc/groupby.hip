@@ -67,10 +67,14 @@ __global__ __launch_bounds__(kBlock) void fill_table_kernel(const FillParams p)
 __global__ __launch_bounds__(kBlock) void aggregate_kernel(const AggParams p)
 {
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  uint32_t iter = 0;
   for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < p.num_rows; row += step) {
+    // an overflowed attempt is repeated with a larger table: stop early
+    if ((++iter & 31u) == 0u && __hip_atomic_load(&p.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return;
     if (!key_row_valid(p, row)) continue;
     const RowKey rk = row_key(p, row);
-    uint8_t* rec = global_find_or_insert(p, rk, row);
+    uint8_t* rec = global_find_or_insert(p, rk, row, false);
     if (rec) apply_row(p, rec, row);
   }
 }
@@ -1003,7 +1007,11 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
       default: break;
     }
     // COUNT_VALID = rows - nulls: one shared row count per group plus a null
-    // count per nullable column (updated only for null values)
+    // count per nullable column (updated only for null values); the
+    // sort-path kinds read neither
+    if (wt.kind == MGB_AGG_MEDIAN || wt.kind == MGB_AGG_QUANTILE || wt.kind == MGB_AGG_NUNIQUE ||
+        wt.kind == MGB_AGG_NTH_ELEMENT)
+      continue;
     need_rows = true;
     if (wt.kind != MGB_AGG_COUNT_ALL && pc.nullable) add_op(pc, OP_NULLS);
   }
@@ -2268,8 +2276,12 @@ mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mg
   Path path = forced_path();
   if (!lds_ok) path = Path::Global;
   // the sort-path aggregations map rows to groups through a global table,
-  // which the partitioned execution does not keep
-  if (plan.has_sort && path != Path::Direct) path = Path::Global;
+  // which the partitioned execution does not keep; a call of sort-path
+  // aggregations only has records without fields, which only the global
+  // execution builds
+  bool no_fields = plan.rows_off < 0;
+  for (const PlanCol& pc : plan.cols) no_fields = no_fields && pc.ops.empty() && pc.p2.empty();
+  if (plan.has_sort && (path != Path::Direct || no_fields)) path = Path::Global;
   if (path == Path::Auto) path = (G_est <= geom.cap / 2) ? Path::Direct : Path::Partitioned;
 
   TempOuts temp;

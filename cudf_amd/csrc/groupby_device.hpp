@@ -105,7 +105,11 @@ __device__ __forceinline__ void flag_overflow(unsigned long long* counters) { at
 
 // Record of the group of (rk, row) in the global table, inserting it if
 // needed; nullptr when the table is over its group budget (overflow flag set).
-__device__ __forceinline__ uint8_t* global_find_or_insert(const AggParams& p, const RowKey& rk, int64_t row)
+// `entry_check`: test the overflow flag first (a caller that probes every row
+// tests it itself, every few rows: one flag read by every row of a launch
+// queues on a single L2 channel).
+__device__ __forceinline__ uint8_t* global_find_or_insert(const AggParams& p, const RowKey& rk, int64_t row,
+                                                         bool entry_check = true)
 {
   if (rk.special) {
     uint8_t* rec = p.table + p.capacity * (int64_t)p.stride;
@@ -115,7 +119,7 @@ __device__ __forceinline__ uint8_t* global_find_or_insert(const AggParams& p, co
   }
   // once the table has overflowed the call is repeated with a larger one:
   // stop work on this attempt at once
-  if (__hip_atomic_load(&p.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return nullptr;
+  if (entry_check && __hip_atomic_load(&p.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return nullptr;
   const uint64_t empty = p.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
   int64_t idx = reduce_range(rk.hash, p.capacity);
   for (int64_t probe = 0; probe < p.capacity; ++probe) {
@@ -184,6 +188,7 @@ __device__ __forceinline__ void apply_row(const AggParams& p, uint8_t* rec, int6
   if (p.rows_op >= 0) apply_op(rec, OP_ROWS, p.ops[p.rows_op].off, 0, 4, VC_UNSIGNED);
   for (int c = 0; c < p.num_vals; ++c) {
     const ValCol& v = p.vals[c];
+    if (v.op_begin == v.op_end && v.nulls_off < 0) continue;  // a column of sort-path aggregations only
     if (!value_valid(p, v, row, flags)) {
       if (v.nulls_off >= 0) atom_add(reinterpret_cast<uint32_t*>(rec + v.nulls_off), 1u);
       continue;

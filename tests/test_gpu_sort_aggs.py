@@ -139,3 +139,47 @@ def test_large(gpu):
     keys = rng.integers(0, 200_000, n).astype(np.int64) * 31
     reqs = [((O.FLOAT64, _values(rng, O.FLOAT64, n), rng.random(n) > 0.05), [MED, Q(0.95), NU(), NTH(3)])]
     _compare([(O.INT64, keys, None)], reqs)
+
+
+def test_twenty_million_rows_against_numpy(gpu):
+    """At 20M rows over 1M int64 keys (about 20 rows a group), each group's
+    sorted valid values from numpy: MEDIAN (linear), QUANTILE
+    0.9 LOWER, NUNIQUE, the first row (INCLUDE) and the last valid value
+    (EXCLUDE)."""
+    rng = np.random.default_rng(790)
+    n, card = 20_000_000, 1_000_000
+    keys = rng.integers(-2 ** 62, 2 ** 62, card)[rng.integers(0, card, n)]
+    v = np.round(rng.standard_normal(n) * 100, 1)  # repeats, so NUNIQUE < count
+    valid = rng.random(n) > 0.05
+    reqs = [((O.FLOAT64, v, valid), [MED, Q(0.9, 1), NU(), NTH(0), NTH(-1, False)])]
+    gk, gr = gpu_groupby([(O.INT64, keys, None)], reqs)
+    order = np.argsort(gk[0][0], kind="stable")
+    got = [(np.asarray(a)[order], np.ones(len(order), bool) if m is None else np.asarray(m, bool)[order])
+           for a, m in gr[0]]
+
+    ukeys, first, inv = np.unique(keys, return_index=True, return_inverse=True)
+    assert np.array_equal(np.asarray(gk[0][0])[order], ukeys)
+    G = len(ukeys)
+    vi = np.flatnonzero(valid)
+    srt = np.lexsort((v[vi], inv[vi]))
+    sg, sv = inv[vi][srt], v[vi][srt]
+    starts = np.searchsorted(sg, np.arange(G))
+    cnt = np.searchsorted(sg, np.arange(G), "right") - starts
+    has = cnt > 0
+    pos = 0.5 * (cnt[has] - 1)
+    lo, hi = np.floor(pos).astype(np.int64), np.ceil(pos).astype(np.int64)
+    f = pos - lo
+    med = (1.0 - f) * sv[starts[has] + lo] + f * sv[starts[has] + hi]
+    (gm, gmv), (gq, gqv), (gu, guv), (g0, g0v), (gl, glv) = got
+    assert np.array_equal(gmv, has)
+    assert np.all(np.abs(gm[has] - med) <= 1e-12 * np.abs(med)), "MEDIAN"
+    q_lo = np.floor(0.9 * (cnt[has] - 1)).astype(np.int64)
+    assert np.array_equal(gqv, has) and np.array_equal(gq[has], sv[starts[has] + q_lo]), "QUANTILE LOWER"
+    d = np.ones(len(sv), bool)
+    d[1:] = (sg[1:] != sg[:-1]) | (sv[1:] != sv[:-1])
+    assert np.array_equal(gu, np.bincount(sg, weights=d, minlength=G).astype(np.int64)), "NUNIQUE"
+    assert np.array_equal(g0v, valid[first]) and np.array_equal(g0[g0v], v[first][g0v]), "NTH_ELEMENT 0"
+    last = np.full(G, -1)
+    rev_g, rev_i = np.unique(inv[vi][::-1], return_index=True)
+    last[rev_g] = vi[len(vi) - 1 - rev_i]
+    assert np.array_equal(glv, last >= 0) and np.array_equal(gl[glv], v[last[last >= 0]]), "NTH_ELEMENT -1"
