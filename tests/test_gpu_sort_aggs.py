@@ -52,7 +52,8 @@ def _compare(keys, reqs, include_null_keys=False):
             assert np.array_equal(gm, om), f"{what}: validity differs at {np.flatnonzero(gm != om)[:5]}"
             g, o = np.asarray(g)[om], np.asarray(o)[om]
             if g.dtype.kind == "f":
-                same = (g == o) | (np.isnan(g) & np.isnan(o)) | (np.abs(g - o) <= 1e-12 * np.abs(o))
+                with np.errstate(invalid="ignore"):  # inf - inf where both are inf
+                    same = (g == o) | (np.isnan(g) & np.isnan(o)) | (np.abs(g - o) <= 1e-12 * np.abs(o))
                 assert same.all(), f"{what}: {int((~same).sum())} groups differ, e.g. {g[~same][:3]} vs {o[~same][:3]}"
             else:
                 assert np.array_equal(g, o), f"{what}: differs at {np.flatnonzero(g != o)[:5]}"
