@@ -17,11 +17,13 @@ for the one operation this library implements, ``DataFrame.groupby(by).agg``:
 pandas' conventions, applied to the library's cudf-typed results (as cudf's
 own Python layer does, python/cudf/cudf/core/groupby/groupby.py:627-655):
 NaN values are skipped (they enter as nulls) and NaN keys are missing keys;
-``sum`` / ``prod`` of a group without values is 0 / 1, other aggregations
-give NaN (or ``<NA>`` for nullable dtypes); ``sum``, ``prod``, ``min`` and
-``max`` keep the column's dtype, ``count`` and ``size`` are int64, ``mean``,
-``var`` and ``std`` are float64 except for float32 columns; ``idxmin`` /
-``idxmax`` return index labels; ``sort=True`` orders the groups by key,
+``sum`` / ``prod`` of a group without values is 0 / 1 and ``nunique`` 0,
+other aggregations give NaN (or ``<NA>`` for nullable dtypes); ``sum``,
+``prod``, ``min``, ``max``, ``first`` and ``last`` keep the column's dtype,
+``count``, ``size`` and ``nunique`` are int64, ``mean``, ``median``, ``var``
+and ``std`` are float64 except for float32 columns; ``idxmin`` / ``idxmax``
+return index labels; ``first`` / ``last`` are the first / last value that is
+not missing, ``nunique`` counts distinct values that are not missing; ``sort=True`` orders the groups by key,
 ``sort=False`` by first appearance.
 
 Nothing here computes an aggregate on the host: when the library is not
@@ -42,6 +44,9 @@ AGGS = {
     "sum": (Kind.SUM, 0), "prod": (Kind.PRODUCT, 0), "min": (Kind.MIN, 0), "max": (Kind.MAX, 0),
     "count": (Kind.COUNT_VALID, 0), "size": (Kind.COUNT_ALL, 0), "mean": (Kind.MEAN, 0),
     "var": (Kind.VARIANCE, 1), "std": (Kind.STD, 1), "idxmin": (Kind.ARGMIN, 0), "idxmax": (Kind.ARGMAX, 0),
+    # the sort-path kinds (missing values excluded, as pandas' defaults)
+    "median": (Kind.MEDIAN, 0), "nunique": (Kind.NUNIQUE, 0), "first": (Kind.NTH_ELEMENT, 0),
+    "last": (Kind.NTH_ELEMENT, -1),
 }
 
 _NUMPY_TYPES = {
@@ -138,7 +143,7 @@ class Plan:
                 req[1].append(a)
         for c, names in self.requests:
             tid = self.columns[c][2]
-            if tid == TypeId.BOOL8 and any(a in ("mean", "var", "std", "sum", "prod") for a in names):
+            if tid == TypeId.BOOL8 and any(a in ("mean", "median", "var", "std", "sum", "prod") for a in names):
                 raise NotServed("arithmetic on bool")
         self.pd = pd
 
@@ -176,6 +181,8 @@ def _as_pandas(plan: Plan, name: str, dtype, values: np.ndarray, valid):
     ok = np.ones(len(values), bool) if valid is None else np.asarray(valid, bool)
     nullable = isinstance(dtype, pd.api.extensions.ExtensionDtype)
     np_dt = np.dtype(getattr(dtype, "numpy_dtype", dtype))
+    if name == "nunique":
+        return values.astype(np.int64)  # int64 for nullable columns too
     if name in ("count", "size"):
         out = values.astype(np.int64)
         return pd.array(out, dtype="Int64") if nullable else out
@@ -188,13 +195,13 @@ def _as_pandas(plan: Plan, name: str, dtype, values: np.ndarray, valid):
         empty = 0 if name == "sum" else 1
         out = np.where(ok, values, empty).astype(np_dt) if np_dt.kind != "b" else values
         return pd.array(out, dtype=dtype) if nullable else out
-    if name in ("min", "max"):
+    if name in ("min", "max", "first", "last"):
         if nullable:
             return _masked(pd, np.where(ok, values, 0).astype(np_dt), ok, dtype)
         if np_dt.kind == "f":
             return np.where(ok, values, np.nan).astype(np_dt)
         return values.astype(np_dt)  # numpy ints: every group has a value
-    # mean, var, std: float64 (float32 for float32 columns)
+    # mean, median, var, std: float64 (float32 for float32 columns)
     out_dt = np.float32 if np_dt == np.float32 else np.float64
     out = np.where(ok, values, np.nan).astype(out_dt)
     if nullable:

@@ -47,6 +47,23 @@ def test_large_frames(gpu, rows, card):
     same(PA.groupby_agg(df, "key", spec), df.groupby("key").agg(spec))
 
 
+def test_sort_path_aggregations(gpu):
+    df = frame(11)
+    sub = df[["k", "f64", "f32", "i8", "ni"]]
+    spec = ["median", "nunique", "first", "last"]
+    same(PA.groupby_agg(sub, "k", spec), sub.groupby("k").agg(spec))
+    spec2 = {"f64": ["median", "sum"], "ni": ["first", "nunique"], "i8": "last"}
+    same(PA.groupby_agg(df, ["k", "k2"], spec2), df.groupby(["k", "k2"]).agg(spec2))
+    same(PA.groupby_agg(sub, "k", spec, sort=False), sub.groupby("k", sort=False).agg(spec))
+    rng = np.random.default_rng(12)
+    n = 1_000_000
+    big = pd.DataFrame({"key": rng.integers(0, 50_000, n), "x": np.round(rng.standard_normal(n), 2),
+                        "y": rng.integers(-20, 20, n).astype(np.int32)})
+    big.loc[rng.random(n) < 0.05, "x"] = np.nan
+    spec3 = {"x": ["median", "nunique", "first", "last"], "y": ["median", "nunique"]}
+    same(PA.groupby_agg(big, "key", spec3), big.groupby("key").agg(spec3))
+
+
 def test_installed_hook_runs_on_the_library(gpu):
     df = frame(8)
     want = df.groupby("k").agg({"f64": ["sum", "var"], "i8": "max"})
@@ -54,10 +71,10 @@ def test_installed_hook_runs_on_the_library(gpu):
     try:
         before = PA.stats()
         got = df.groupby("k").agg({"f64": ["sum", "var"], "i8": "max"})
-        median = df.groupby("k").agg({"f64": "median"})  # not served: pandas runs it
+        sem = df.groupby("k").agg({"f64": "sem"})  # not served: pandas runs it
         after = PA.stats()
     finally:
         PA.uninstall()
     same(got, want)
-    pd.testing.assert_frame_equal(median, df.groupby("k").agg({"f64": "median"}))
+    pd.testing.assert_frame_equal(sem, df.groupby("k").agg({"f64": "sem"}))
     assert after["gpu"] - before["gpu"] == 1 and after["pandas"] - before["pandas"] == 1

@@ -117,7 +117,17 @@ def test_two_keys():
     same(oracle_frame(df, ["k", "k2"], spec), df.groupby(["k", "k2"]).agg(spec))
 
 
-@pytest.mark.parametrize("func", ["median", ["sum", "median"], {"f64": "first"}, {"f64": lambda x: x.sum()}, {}])
+def test_sort_path_aggregations():
+    df = frame(11)
+    sub = df[["k", "f64", "f32", "i8", "ni"]]
+    spec = ["median", "nunique", "first", "last"]
+    same(oracle_frame(sub, "k", spec), sub.groupby("k").agg(spec))
+    spec2 = {"f64": ["median", "sum"], "ni": ["first", "nunique"], "i8": "last"}
+    same(oracle_frame(df, ["k", "k2"], spec2), df.groupby(["k", "k2"]).agg(spec2))
+    same(oracle_frame(sub, "k", spec, sort=False), sub.groupby("k", sort=False).agg(spec))
+
+
+@pytest.mark.parametrize("func", ["sem", ["sum", "skew"], {"f64": "ohlc"}, {"f64": lambda x: x.sum()}, {}])
 def test_calls_not_served(func):
     df = frame(6)
     with pytest.raises(PA.NotServed):
@@ -153,8 +163,8 @@ def test_install_routes_only_served_calls(monkeypatch):
     try:
         df = frame(8)
         before = PA.stats()
-        want = df.groupby("k").agg({"f64": "median"})
-        got = df.groupby("k").agg({"f64": "median"})
+        want = df.groupby("k").agg({"f64": "sem"})
+        got = df.groupby("k").agg({"f64": "sem"})
         pd.testing.assert_frame_equal(got, want)
         assert df.groupby("k").agg({"f64": "sum"}) == "gpu" and calls == [{"f64": "sum"}]
         assert df.groupby("k")[["f64", "i8"]].aggregate("max") == "gpu"
