@@ -271,36 +271,64 @@ def stats() -> Dict[str, int]:
         return dict(_counts)
 
 
-def _patched(self, func=None, *args, engine=None, engine_kwargs=None, **kwargs):
-    orig = _saved["aggregate"]
-    served = (not args and not kwargs and engine is None and engine_kwargs is None and self.as_index
-              and self.level is None and func is not None)
-    if served:
-        try:
-            df = self.obj
-            sel = getattr(self, "_selection", None)
-            if sel is not None:
-                cols = [sel] if not isinstance(sel, (list, tuple)) else list(sel)
-                keys = [self.keys] if not isinstance(self.keys, (list, tuple)) else list(self.keys)
-                df = df[keys + [c for c in cols if c not in keys]]
-            plan = Plan(df, self.keys, func, self.sort, self.dropna)
-        except NotServed:
-            served = False
-        except Exception:
-            served = False
-    if not served:
-        with _lock:
-            _counts["pandas"] += 1
-        return orig(self, func, *args, engine=engine, engine_kwargs=engine_kwargs, **kwargs)
-    out = groupby_agg(df, self.keys, func, self.sort, self.dropna)
+def _served_frame(gb, func):
+    """The frame to aggregate when the library serves ``gb.agg(func)``, else None."""
+    if not gb.as_index or gb.level is not None or func is None:
+        return None
+    try:
+        df = gb.obj
+        sel = getattr(gb, "_selection", None)
+        if sel is not None:
+            cols = [sel] if not isinstance(sel, (list, tuple)) else list(sel)
+            keys = [gb.keys] if not isinstance(gb.keys, (list, tuple)) else list(gb.keys)
+            df = df[keys + [c for c in cols if c not in keys]]
+        Plan(df, gb.keys, func, gb.sort, gb.dropna)
+        return df
+    except Exception:  # NotServed, or a grouping this module does not read
+        return None
+
+
+def _run(gb, df, func):
+    out = groupby_agg(df, gb.keys, func, gb.sort, gb.dropna)
     with _lock:
         _counts["gpu"] += 1
     return out
 
 
+def _patched(self, func=None, *args, engine=None, engine_kwargs=None, **kwargs):
+    df = None
+    if not args and not kwargs and engine is None and engine_kwargs is None:
+        df = _served_frame(self, func)
+    if df is None:
+        with _lock:
+            _counts["pandas"] += 1
+        return _saved["aggregate"](self, func, *args, engine=engine, engine_kwargs=engine_kwargs, **kwargs)
+    return _run(self, df, func)
+
+
+# DataFrameGroupBy methods served when called without arguments (the same
+# result as ``agg(name)``); with any argument pandas runs them
+METHODS = ("sum", "prod", "min", "max", "count", "mean", "median", "var", "std", "nunique", "first", "last",
+           "idxmin", "idxmax")
+
+
+def _method(name):
+    def patched(self, *args, **kwargs):
+        df = None if (args or kwargs) else _served_frame(self, name)
+        if df is None:
+            with _lock:
+                _counts["pandas"] += 1
+            return _saved[name](self, *args, **kwargs)
+        return _run(self, df, name)
+
+    patched.__name__ = name
+    return patched
+
+
 def install() -> None:
-    """Routes served ``DataFrameGroupBy.agg`` calls to the MI355X path.
-    Raises when the library is not built."""
+    """Routes served ``DataFrameGroupBy.agg`` calls, and the argument-free
+    calls of the methods in METHODS, to the MI355X path. Raises when the
+    library is not built."""
     from . import _lib
     from pandas.core.groupby.generic import DataFrameGroupBy
 
@@ -312,6 +340,9 @@ def install() -> None:
         _saved["agg"] = DataFrameGroupBy.agg
         DataFrameGroupBy.aggregate = _patched
         DataFrameGroupBy.agg = _patched
+        for name in METHODS:
+            _saved[name] = getattr(DataFrameGroupBy, name)
+            setattr(DataFrameGroupBy, name, _method(name))
 
 
 def uninstall() -> None:
@@ -322,6 +353,8 @@ def uninstall() -> None:
             return
         DataFrameGroupBy.aggregate = _saved.pop("aggregate")
         DataFrameGroupBy.agg = _saved.pop("agg")
+        for name in METHODS:
+            setattr(DataFrameGroupBy, name, _saved.pop(name))
 
 
 __all__: Sequence[str] = ["groupby_agg", "install", "uninstall", "stats", "NotServed", "Plan"]

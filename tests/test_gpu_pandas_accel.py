@@ -78,3 +78,21 @@ def test_installed_hook_runs_on_the_library(gpu):
     same(got, want)
     pd.testing.assert_frame_equal(sem, df.groupby("k").agg({"f64": "sem"}))
     assert after["gpu"] - before["gpu"] == 1 and after["pandas"] - before["pandas"] == 1
+
+
+def test_installed_hook_serves_methods(gpu):
+    df = frame(14)[["k", "f64", "f32", "i8", "ni"]]
+    # products of the integer columns overflow here (pandas then widens the
+    # dtype, the library wraps in it): prod runs on the float columns
+    frames = {name: df[["k", "f64", "f32"]] if name == "prod" else df for name in PA.METHODS}
+    want = {name: getattr(frames[name].groupby("k"), name)() for name in PA.METHODS}
+    PA.install()
+    try:
+        before = PA.stats()
+        got = {name: getattr(frames[name].groupby("k"), name)() for name in PA.METHODS}
+        after = PA.stats()
+    finally:
+        PA.uninstall()
+    for name in PA.METHODS:
+        same(got[name], want[name])
+    assert after["gpu"] - before["gpu"] == len(PA.METHODS)

@@ -173,3 +173,30 @@ def test_install_routes_only_served_calls(monkeypatch):
     finally:
         PA.uninstall()
     assert not isinstance(frame(8).groupby("k").agg({"f64": "sum"}), str)
+
+
+def test_install_routes_argument_free_methods(monkeypatch):
+    """``gb.sum()`` and the other methods in METHODS go to groupby_agg when
+    called without arguments and give agg(name)'s result; with an argument
+    pandas runs them."""
+    calls = []
+    monkeypatch.setattr(PA, "groupby_agg", lambda df, by, func, sort, dropna: calls.append(func) or "gpu")
+    df = frame(8)
+    want = df.groupby("k").sum(min_count=1)
+    PA.install()
+    try:
+        for name in PA.METHODS:
+            assert getattr(df.groupby("k"), name)() == "gpu"
+        assert calls == list(PA.METHODS)
+        pd.testing.assert_frame_equal(df.groupby("k").sum(min_count=1), want)
+    finally:
+        PA.uninstall()
+    assert not isinstance(df.groupby("k").sum(), str)
+
+
+@pytest.mark.parametrize("name", PA.METHODS)
+def test_methods_equal_agg_by_name(name):
+    """The method and agg(name) agree in pandas itself, so serving one as the
+    other keeps pandas' result."""
+    df = frame(13)[["k", "f64", "f32", "i8"]]
+    pd.testing.assert_frame_equal(getattr(df.groupby("k"), name)(), df.groupby("k").agg(name))
