@@ -288,6 +288,35 @@ def _served_frame(gb, func):
         return None
 
 
+def _series_frame(gb, func):
+    """For ``SeriesGroupBy.agg(func)`` (``df.groupby(keys)[name]``) the
+    library serves: the frame of the key columns and the series, else None."""
+    if not gb.as_index or gb.level is not None or func is None or isinstance(func, dict):
+        return None
+    try:
+        import pandas as pd
+
+        groupings = gb._grouper.groupings
+        keys = [gb.keys] if not isinstance(gb.keys, (list, tuple)) else list(gb.keys)
+        name = gb.obj.name
+        if ([g.name for g in groupings] != keys or not all(getattr(g, "in_axis", False) for g in groupings)
+                or name is None or name in keys):
+            return None
+        data = {k: g.grouping_vector for k, g in zip(keys, groupings)}
+        data[name] = gb.obj.array
+        df = pd.DataFrame(data, index=gb.obj.index)
+        Plan(df, keys, func, gb.sort, gb.dropna)
+        return df
+    except Exception:  # NotServed, or a grouping this module does not read
+        return None
+
+
+def _run_series(gb, df, func):
+    """A SeriesGroupBy result: the series' column of the frame's result (a
+    Series for one aggregation name, a frame of the names for a list)."""
+    return _run(gb, df, func)[gb.obj.name]
+
+
 def _run(gb, df, func):
     out = groupby_agg(df, gb.keys, func, gb.sort, gb.dropna)
     with _lock:
@@ -297,12 +326,10 @@ def _run(gb, df, func):
 
 def _patched(self, func=None, *args, engine=None, engine_kwargs=None, **kwargs):
     df = None
-    if not args and not kwargs and engine is None and engine_kwargs is None:
+    if not args and not kwargs and engine is None and engine_kwargs is None and not getattr(_tls, "inside", False):
         df = _served_frame(self, func)
     if df is None:
-        with _lock:
-            _counts["pandas"] += 1
-        return _saved["aggregate"](self, func, *args, engine=engine, engine_kwargs=engine_kwargs, **kwargs)
+        return _in_pandas(_saved["aggregate"], self, func, *args, engine=engine, engine_kwargs=engine_kwargs, **kwargs)
     return _run(self, df, func)
 
 
@@ -312,25 +339,54 @@ METHODS = ("sum", "prod", "min", "max", "count", "mean", "median", "var", "std",
            "idxmin", "idxmax")
 
 
-def _method(name):
+_tls = threading.local()  # .inside: within a call that pandas runs
+
+
+def _in_pandas(orig, *args, **kwargs):
+    """Runs a call in pandas; the patched methods that pandas calls inside it
+    run in pandas too and are not counted."""
+    if getattr(_tls, "inside", False):
+        return orig(*args, **kwargs)
+    with _lock:
+        _counts["pandas"] += 1
+    _tls.inside = True
+    try:
+        return orig(*args, **kwargs)
+    finally:
+        _tls.inside = False
+
+
+def _series_patched(self, func=None, *args, engine=None, engine_kwargs=None, **kwargs):
+    df = None
+    if not args and not kwargs and engine is None and engine_kwargs is None and not getattr(_tls, "inside", False):
+        df = _series_frame(self, func)
+    if df is None:
+        return _in_pandas(_saved["series.aggregate"], self, func, *args, engine=engine, engine_kwargs=engine_kwargs,
+                          **kwargs)
+    return _run_series(self, df, func)
+
+
+def _method(name, series=False):
+    key = ("series." if series else "") + name
+
     def patched(self, *args, **kwargs):
-        df = None if (args or kwargs) else _served_frame(self, name)
+        df = None
+        if not (args or kwargs or getattr(_tls, "inside", False)):
+            df = _series_frame(self, name) if series else _served_frame(self, name)
         if df is None:
-            with _lock:
-                _counts["pandas"] += 1
-            return _saved[name](self, *args, **kwargs)
-        return _run(self, df, name)
+            return _in_pandas(_saved[key], self, *args, **kwargs)
+        return _run_series(self, df, name) if series else _run(self, df, name)
 
     patched.__name__ = name
     return patched
 
 
 def install() -> None:
-    """Routes served ``DataFrameGroupBy.agg`` calls, and the argument-free
-    calls of the methods in METHODS, to the MI355X path. Raises when the
-    library is not built."""
+    """Routes served ``DataFrameGroupBy.agg`` / ``SeriesGroupBy.agg`` calls,
+    and the argument-free calls of the methods in METHODS on either, to the
+    MI355X path. Raises when the library is not built."""
     from . import _lib
-    from pandas.core.groupby.generic import DataFrameGroupBy
+    from pandas.core.groupby.generic import DataFrameGroupBy, SeriesGroupBy
 
     _lib.load()
     with _lock:
@@ -340,21 +396,30 @@ def install() -> None:
         _saved["agg"] = DataFrameGroupBy.agg
         DataFrameGroupBy.aggregate = _patched
         DataFrameGroupBy.agg = _patched
+        _saved["series.aggregate"] = SeriesGroupBy.aggregate
+        _saved["series.agg"] = SeriesGroupBy.agg
+        SeriesGroupBy.aggregate = _series_patched
+        SeriesGroupBy.agg = _series_patched
         for name in METHODS:
             _saved[name] = getattr(DataFrameGroupBy, name)
             setattr(DataFrameGroupBy, name, _method(name))
+            _saved["series." + name] = getattr(SeriesGroupBy, name)
+            setattr(SeriesGroupBy, name, _method(name, series=True))
 
 
 def uninstall() -> None:
-    from pandas.core.groupby.generic import DataFrameGroupBy
+    from pandas.core.groupby.generic import DataFrameGroupBy, SeriesGroupBy
 
     with _lock:
         if "aggregate" not in _saved:
             return
         DataFrameGroupBy.aggregate = _saved.pop("aggregate")
         DataFrameGroupBy.agg = _saved.pop("agg")
+        SeriesGroupBy.aggregate = _saved.pop("series.aggregate")
+        SeriesGroupBy.agg = _saved.pop("series.agg")
         for name in METHODS:
             setattr(DataFrameGroupBy, name, _saved.pop(name))
+            setattr(SeriesGroupBy, name, _saved.pop("series." + name))
 
 
 __all__: Sequence[str] = ["groupby_agg", "install", "uninstall", "stats", "NotServed", "Plan"]

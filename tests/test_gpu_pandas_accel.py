@@ -96,3 +96,24 @@ def test_installed_hook_serves_methods(gpu):
     for name in PA.METHODS:
         same(got[name], want[name])
     assert after["gpu"] - before["gpu"] == len(PA.METHODS)
+
+
+def test_installed_hook_serves_series_groupby(gpu):
+    df = frame(16)
+    calls = [("f64", "sum"), ("f64", "median"), ("f32", "nunique"), ("i8", "first"), ("ni", "mean"),
+             ("ni", "last"), ("f64", "idxmax")]
+    want = [getattr(df.groupby("k")[c], m)() for c, m in calls]
+    want_list = df.groupby(["k", "k2"])["f64"].agg(["sum", "median", "std"])
+    PA.install()
+    try:
+        before = PA.stats()
+        got = [getattr(df.groupby("k")[c], m)() for c, m in calls]
+        got_list = df.groupby(["k", "k2"])["f64"].agg(["sum", "median", "std"])
+        after = PA.stats()
+    finally:
+        PA.uninstall()
+    for (c, m), g, w in zip(calls, got, want):
+        pd.testing.assert_series_equal(g, w, check_exact=False, rtol=1e-5, atol=1e-4, check_dtype=False)
+        assert g.dtype.kind == w.dtype.kind, (c, m, g.dtype, w.dtype)
+    same(got_list, want_list)
+    assert after["gpu"] - before["gpu"] == len(calls) + 1

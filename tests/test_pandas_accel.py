@@ -200,3 +200,41 @@ def test_methods_equal_agg_by_name(name):
     other keeps pandas' result."""
     df = frame(13)[["k", "f64", "f32", "i8"]]
     pd.testing.assert_frame_equal(getattr(df.groupby("k"), name)(), df.groupby("k").agg(name))
+
+
+def test_series_groupby_through_the_hook(monkeypatch):
+    """``df.groupby(k)[x]`` calls: agg with a name or a list and the
+    argument-free methods, with the oracle standing in for the device, equal
+    pandas' results; a dict spec, an argument, or a key that is not a column
+    stays in pandas."""
+    monkeypatch.setattr(PA, "groupby_agg", lambda df, by, func, sort, dropna: oracle_frame(df, by, func, sort, dropna))
+    df = frame(15)
+    calls = [("f64", "agg", "sum"), ("f64", "agg", ["median", "nunique", "first"]), ("i8", "agg", ["mean", "max"])]
+    calls += [(c, m, None) for c in ("f64", "ni") for m in PA.METHODS if m != "prod"]
+    want = [(getattr(df.groupby("k")[c], m)(f) if f is not None else getattr(df.groupby("k")[c], m)())
+            for c, m, f in calls]
+    want_two = df.groupby(["k", "k2"])["f32"].median()
+    want_unsorted = df.groupby("k", sort=False)["f64"].sum()
+    PA.install()
+    try:
+        before = PA.stats()
+        got = [(getattr(df.groupby("k")[c], m)(f) if f is not None else getattr(df.groupby("k")[c], m)())
+               for c, m, f in calls]
+        got_two = df.groupby(["k", "k2"])["f32"].median()
+        got_unsorted = df.groupby("k", sort=False)["f64"].sum()
+        mid = PA.stats()
+        df.groupby("k")["f64"].sum(min_count=2)
+        df.groupby(df["k"].to_numpy())["f64"].sum()
+        after = PA.stats()
+    finally:
+        PA.uninstall()
+    for (c, m, f), g, w in zip(calls, got, want):
+        if isinstance(w, pd.Series):
+            pd.testing.assert_series_equal(g, w, check_exact=False, rtol=1e-5, atol=1e-4, check_dtype=False)
+            assert g.dtype == w.dtype or (g.dtype.kind == w.dtype.kind), (c, m, g.dtype, w.dtype)
+        else:
+            same(g, w)
+    pd.testing.assert_series_equal(got_two, want_two, check_exact=False, rtol=1e-5)
+    pd.testing.assert_series_equal(got_unsorted, want_unsorted, check_exact=False, rtol=1e-9)
+    assert mid["gpu"] - before["gpu"] == len(calls) + 2
+    assert after["pandas"] - mid["pandas"] >= 2 and after["gpu"] == mid["gpu"]
