@@ -10,9 +10,12 @@ for the one operation this library implements, ``DataFrame.groupby(by).agg``:
   returns for it, or raises ``NotServed`` for a call the library does not
   serve (a key or value dtype it has no kernel for, an aggregation it does
   not implement, keyword arguments, ...).
-* ``install()`` patches ``DataFrameGroupBy.agg`` / ``aggregate`` so that
-  served calls take that path and every other call runs in pandas as before;
-  ``stats()`` counts the calls each side took.
+* ``install()`` patches ``agg`` / ``aggregate`` and the argument-free forms
+  of the methods in ``METHODS`` (``sum()``, ``median()``, ``nunique()``, ...)
+  on ``DataFrameGroupBy`` and ``SeriesGroupBy``, so that served calls take
+  that path and every other call runs in pandas as before (calls pandas
+  makes inside such a call included); ``stats()`` counts the calls each side
+  took.
 
 pandas' conventions, applied to the library's cudf-typed results (as cudf's
 own Python layer does, python/cudf/cudf/core/groupby/groupby.py:627-655):
