@@ -351,8 +351,8 @@ def bench_pandas(args):
                        "i32": rng.integers(-1_000_000, 1_000_001, rows).astype(np.int32)})
     for c in ("f64", "f32"):
         df.loc[rng.random(rows) < 0.05, c] = np.nan
-    spec = {"f64": ["sum", "count", "min", "max", "mean"], "f32": ["sum", "count", "min", "max", "mean"],
-            "i32": ["sum", "count", "min", "max", "mean"]}
+    names = args.pandas_aggs.split(",")
+    spec = {"f64": names, "f32": names, "i32": names}
     for _ in range(max(args.warmup, 1)):
         PA.groupby_agg(df, "key", spec)
     t_gpu = []
@@ -666,6 +666,8 @@ def main():
     ap.add_argument("--pandas-sample-rows", type=int, default=10_000_000,
                     help="rows of the reference CPU path's (pandas) timed sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pandas-aggs", default="sum,count,min,max,mean",
+                    help="pandas workload: the aggregations of each value column (pandas names)")
     ap.add_argument("--summarize-trace", nargs=3, metavar=("RESULTS_DB", "TITLE", "COMMAND"),
                     help="print a text summary of a rocprofv3 --kernel-trace --stats database and exit")
     ap.add_argument("--summarize-pmc", nargs=6,
@@ -716,7 +718,8 @@ def main():
             "data": "synthetic (seeded host DataFrame)",
             "config": {"workload": f"pandas df.groupby('key').agg(...) through the hook, {res['rows']:,} rows, "
                                    f"{args.distinct:,} distinct int64 keys, f64/f32 values 5% NaN, int32 values, "
-                                   "sum/count/min/max/mean each; host DataFrame in and out (PCIe inclusive)",
+                                   f"{'/'.join(args.pandas_aggs.split(','))} each; host DataFrame in and out "
+                                   "(PCIe inclusive)",
                        "parallelism": "single GPU"},
             "cpu_baseline": {"value": res["pandas_rows_per_s"], "unit": "rows/s", "cores": 1, "kind": "port",
                              "sample": f"pandas groupby-agg of the same DataFrame, median of 3"},
