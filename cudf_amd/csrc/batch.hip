@@ -584,6 +584,7 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
     for (int r = 0; r < num_requests; ++r)
       if (b->values[(size_t)q * num_requests + r].size != R)
         return fail(MGB_ERROR_LOGIC, "Size mismatch between request values and groupby keys.");
+    if (R < 0 || R > kMaxRows) return fail(MGB_ERROR_UNSUPPORTED, "more rows than cudf::size_type holds (2^31 - 1)");
   }
   // the batch plan: query 0's
   {

@@ -2265,8 +2265,14 @@ mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mg
 
   mgb_status st = MGB_SUCCESS;
   ws->fine_valid = false;
-  const int64_t G_est = sample_estimate(ws, base, R, stream, st);
+  int64_t G_est = sample_estimate(ws, base, R, stream, st);
   if (st != MGB_SUCCESS) return st;
+  if (const char* e = std::getenv("MGB_ESTIMATE_SCALE")) {
+    // test hook: a scaled estimate (below 1 forces the overflow-and-retry
+    // paths of the table and partitioned executions)
+    const double f = std::atof(e);
+    if (f > 0.0) G_est = std::max<int64_t>(1, (int64_t)((double)G_est * f));
+  }
 
   const LdsGeom geom = lds_geometry(plan.stride);
   AggParams probe = base;
@@ -2351,6 +2357,9 @@ extern "C" mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_
   for (int r = 0; r < num_requests; ++r)
     if (requests[r].values.size != R)
       return fail(MGB_ERROR_LOGIC, "Size mismatch between request values and groupby keys.");
+  // cudf::size_type is int32: rows, groups, per-group counts and ARGMIN /
+  // ARGMAX indices are 32-bit throughout (cpp/include/cudf/types.hpp:80)
+  if (R < 0 || R > kMaxRows) return fail(MGB_ERROR_UNSUPPORTED, "more rows than cudf::size_type holds (2^31 - 1)");
   for (int r = 0; r < num_requests; ++r)
     for (int a = 0; a < requests[r].num_aggs; ++a)
       if (target_type(requests[r].values.type_id, requests[r].aggs[a].kind) < 0)
@@ -2387,6 +2396,7 @@ extern "C" mgb_status mgb_estimate_groups(const mgb_column* keys, int32_t num_ke
   const int64_t R = keys[0].size;
   for (int c = 1; c < num_keys; ++c)
     if (keys[c].size != R) return fail(MGB_ERROR_LOGIC, "key columns differ in size");
+  if (R < 0 || R > kMaxRows) return fail(MGB_ERROR_UNSUPPORTED, "more rows than cudf::size_type holds (2^31 - 1)");
   Plan plan;
   mgb_status st = build_plan(keys, num_keys, null_policy, nullptr, 0, plan);
   if (st != MGB_SUCCESS) return st;

@@ -23,6 +23,7 @@ constexpr int kLdsBlock = 1024;     // threads of an LDS aggregation workgroup
 constexpr int kScanItems = 16;      // u32 scan: items per thread (x256 threads)
 constexpr int kScanTile = 256 * kScanItems;
 constexpr int kMaxRecordWords = 64;
+constexpr int64_t kMaxRows = 0x7fffffff;  // cudf::size_type: rows of one call
 
 // Writes `tmpl` (header + identities) into every record; record `capacity`
 // gets `special_hdr` as header.

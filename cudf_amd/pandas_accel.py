@@ -196,6 +196,18 @@ def _as_pandas(plan: Plan, name: str, dtype, values: np.ndarray, valid):
         return np.where(ok, np.asarray(labels, dtype=np.float64), np.nan)
     if name in ("sum", "prod"):
         empty = 0 if name == "sum" else 1
+        if np_dt.kind in "iu":
+            # pandas accumulates integers in 64 bits and casts the result back
+            # to the column's dtype only when every group's value survives the
+            # cast (its maybe_downcast_to_dtype); otherwise it keeps int64 /
+            # uint64 (Int64 / UInt64 for nullable columns)
+            wide = np.dtype(np.uint64 if np_dt.kind == "u" else np.int64)
+            out = np.where(ok, values, empty).astype(wide)
+            if np_dt.itemsize < 8 and np.array_equal(out.astype(np_dt).astype(wide), out):
+                out, out_dt = out.astype(np_dt), dtype
+            else:
+                out_dt = ("UInt64" if np_dt.kind == "u" else "Int64") if nullable else wide
+            return pd.array(out, dtype=out_dt) if nullable else out
         out = np.where(ok, values, empty).astype(np_dt) if np_dt.kind != "b" else values
         return pd.array(out, dtype=dtype) if nullable else out
     if name in ("min", "max", "first", "last"):

@@ -94,6 +94,20 @@ def test_empty_input_returns_typed_empty_columns_without_allocating():
     assert [orr[i].type_id for i in range(3)] == [9, 3, 10]  # SUM f32 -> f32, COUNT -> int32, MEAN -> f64
 
 
+@pytest.mark.parametrize("rows", [2 ** 31, 2 ** 32, 2 ** 40])
+def test_more_rows_than_size_type_refused_before_device_work(rows):
+    # cudf::size_type is int32 (cpp/include/cudf/types.hpp:80): a call whose
+    # columns hold 2^31 rows or more is refused, never run with 32-bit row
+    # references that would wrap (the data pointer is never dereferenced)
+    st, g, ok, orr, calls = _call([_col(3, rows)], [(_col(3, rows), [0, 4])], 2)
+    assert st == _lib.MGB_ERROR_UNSUPPORTED and calls == []
+    assert b"size_type" in _lib.load().mgb_last_error_message()
+    lib = _lib.load()
+    est = ctypes.c_int64(-1)
+    key = (_lib.mgb_column * 1)(_col(4, rows))
+    assert lib.mgb_estimate_groups(key, 1, 0, None, ctypes.byref(est)) == _lib.MGB_ERROR_UNSUPPORTED
+
+
 def test_null_arguments_rejected():
     lib = _lib.load()
     g = ctypes.c_int64()

@@ -254,3 +254,31 @@ def test_low_cardinality_16_keys(gpu, path):
     keys = uniq[rng.integers(0, 16, n)]
     reqs = [((O.FLOAT64, rng.standard_normal(n), None), [O.SUM, O.COUNT_VALID, O.MAX])]
     _compare([(O.INT32, keys, None)], reqs)
+
+
+@pytest.mark.parametrize("scale", ["0.001", "0.05"])
+def test_underestimate_retries(gpu, path, scale, monkeypatch):
+    """The estimate scaled far below the true group count: the direct and
+    global executions' tables overflow and are rebuilt larger, the
+    partitioned execution's spill table overflows and the rows are
+    repartitioned finer (falling back to the global execution after four
+    attempts). Every attempt ends (bounded probing, overflow flag) and the
+    result is the oracle's."""
+    monkeypatch.setenv("MGB_ESTIMATE_SCALE", scale)
+    rng = np.random.default_rng(35)
+    keys, reqs = _random_case(rng, 400_000, O.INT64, 100_000, [O.INT32, O.FLOAT64], 0.05,
+                              [O.SUM, O.COUNT_VALID, O.MIN, O.MAX, O.MEAN])
+    _compare(keys, reqs)
+
+
+def test_underestimate_retries_zipf_hot_key(gpu, path, monkeypatch):
+    """The composite-key Zipf shape with a 1000x underestimate: the hot key's
+    partition is split and the spill table overflows on the first attempts."""
+    monkeypatch.setenv("MGB_ESTIMATE_SCALE", "0.001")
+    rng = np.random.default_rng(36)
+    n, n_keys = 1_000_000, 200_000
+    ta = rng.integers(-2 ** 31, 2 ** 31 - 1, n_keys).astype(np.int32)
+    tb = rng.integers(-2 ** 62, 2 ** 62, n_keys).astype(np.int64)
+    idx = rng.permutation(n_keys)[_zipf_ranks(rng, n, n_keys, 1.1)]
+    _compare([(O.INT32, ta[idx], None), (O.INT64, tb[idx], None)],
+             [((O.FLOAT64, rng.standard_normal(n), None), [O.SUM, O.COUNT_VALID, O.MAX])])

@@ -87,6 +87,20 @@ def test_prod():
     same(oracle_frame(df, "k", {"s": "prod", "f": "prod"}), df.groupby("k").agg({"s": "prod", "f": "prod"}))
 
 
+@pytest.mark.parametrize("dtype", ["int8", "int16", "int32", "uint8", "uint32", "Int8", "Int32", "UInt16"])
+@pytest.mark.parametrize("func", ["sum", "prod"])
+def test_narrow_integer_results_widen_like_pandas(dtype, func):
+    """pandas keeps a narrow integer dtype for sum / prod only when every
+    group's result fits it; otherwise int64 / uint64 (Int64 / UInt64)."""
+    info = np.iinfo(pd.api.types.pandas_dtype(dtype).numpy_dtype if dtype[0] in "IU" else np.dtype(dtype))
+    big = info.max // 2 + 1
+    for vals in ([big, big, 1, 2], [1, 2, 1, 1]):
+        df = pd.DataFrame({"k": np.array([1, 1, 2, 2], np.int32), "v": pd.array(vals, dtype=dtype)})
+        got = oracle_frame(df, "k", [func])
+        want = df.groupby("k").agg([func])
+        pd.testing.assert_frame_equal(got, want, check_dtype=True)
+
+
 def test_dict_spec_and_single_string():
     df = frame(2)
     spec = {"f64": ["mean", "idxmin", "idxmax"], "i8": "sum", "ni": ["min", "count"]}
