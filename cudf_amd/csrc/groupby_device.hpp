@@ -221,11 +221,20 @@ __device__ __forceinline__ void merge_record(const AggParams& p, uint8_t* dst, c
         unsigned long long* dp = reinterpret_cast<unsigned long long*>(d);
         unsigned long long x = *reinterpret_cast<const unsigned long long*>(s);
         if (x == op_identity(kind)) break;
+        // a float32 column's partial products are float32 values held in a
+        // double: they multiply at float32 precision, as each row's update
+        // does (apply_op, lds_op_rows)
+        bool f32 = false;
+        for (int c = 0; c < p.num_vals; ++c)
+          f32 |= p.vals[c].vclass == VC_F32 && o >= p.vals[c].op_begin && o < p.vals[c].op_end;
         unsigned long long old = __hip_atomic_load(dp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         for (;;) {
           unsigned long long nxt;
           if (kind == OP_PROD_I64) {
             nxt = old * x;
+          } else if (f32) {
+            nxt = (unsigned long long)__double_as_longlong(
+                (double)((float)__longlong_as_double((long long)old) * (float)__longlong_as_double((long long)x)));
           } else {
             nxt = (unsigned long long)__double_as_longlong(__longlong_as_double((long long)old) *
                                                           __longlong_as_double((long long)x));

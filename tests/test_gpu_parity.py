@@ -282,3 +282,43 @@ def test_underestimate_retries_zipf_hot_key(gpu, path, monkeypatch):
     idx = rng.permutation(n_keys)[_zipf_ranks(rng, n, n_keys, 1.1)]
     _compare([(O.INT32, ta[idx], None), (O.INT64, tb[idx], None)],
              [((O.FLOAT64, rng.standard_normal(n), None), [O.SUM, O.COUNT_VALID, O.MAX])])
+
+
+def test_product_against_oracle(gpu, path):
+    """PRODUCT of integer and float columns (integer products wrap modulo
+    2^64 in any order, so they are bit-exact; float32 products are held at
+    float32 precision in every fold)."""
+    rng = np.random.default_rng(37)
+    n, card = 400_000, 2000
+    pool = rng.integers(-2 ** 62, 2 ** 62, card, dtype=np.int64)
+    keys = pool[rng.integers(0, card, n)]
+    reqs = [((O.INT8, rng.integers(-3, 4, n).astype(np.int8), rng.random(n) > 0.1), [O.PRODUCT, O.SUM]),
+            ((O.INT32, rng.integers(-1_000_000, 1_000_000, n).astype(np.int32), rng.random(n) > 0.05), [O.PRODUCT]),
+            ((O.INT64, rng.integers(-2 ** 40, 2 ** 40, n, dtype=np.int64), None), [O.PRODUCT, O.MAX]),
+            ((O.UINT16, rng.integers(0, 2 ** 16, n).astype(np.uint16), rng.random(n) > 0.05), [O.PRODUCT]),
+            ((O.FLOAT32, rng.uniform(0.95, 1.05, n).astype(np.float32), rng.random(n) > 0.05), [O.PRODUCT]),
+            ((O.FLOAT64, rng.uniform(0.9, 1.1, n), rng.random(n) > 0.05), [O.PRODUCT, O.SUM])]
+    _compare([(O.INT64, keys, None)], reqs)
+
+
+def test_product_hot_key_split(gpu, path):
+    """PRODUCT where one key holds 40% of 1M rows: its partition is split
+    over work items whose partial products meet in the global table. Float32
+    factors are +-1 with rare 2 and 0.5, so every order gives the exact
+    product."""
+    rng = np.random.default_rng(38)
+    n = 1_000_000
+    keys = rng.integers(0, 30_000, n).astype(np.int64)
+    keys[rng.random(n) < 0.4] = 77
+    f32 = rng.choice(np.array([1.0, -1.0, 2.0, 0.5], np.float32), n, p=[0.4995, 0.4995, 0.0005, 0.0005])
+    reqs = [((O.FLOAT32, f32, None), [O.PRODUCT, O.COUNT_VALID]),
+            ((O.INT64, rng.integers(-9, 10, n, dtype=np.int64), rng.random(n) > 0.2), [O.PRODUCT]),
+            ((O.FLOAT64, rng.uniform(0.999, 1.001, n), None), [O.PRODUCT])]
+    gk, gr = gpu_groupby([(O.INT64, keys, None)], reqs)
+    ok, orr = oracle_groupby([(O.INT64, keys, None)], reqs)
+    gk, gr = sorted_result(gk, gr)
+    ok, orr = sorted_result(ok, orr)
+    assert np.array_equal(gk[0][0], ok[0][0])
+    assert np.array_equal(gr[0][0][0], orr[0][0][0]), "float32 products of +-1, 2, 0.5 are exact in any order"
+    assert np.array_equal(gr[1][0][0], orr[1][0][0]) and np.array_equal(gr[1][0][1], orr[1][0][1])
+    assert_same(gr[2][0], orr[2][0], None, what="float64 product")
