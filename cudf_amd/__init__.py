@@ -7,10 +7,11 @@ from . import aggregation, hashing
 from .column import Column, Table, pack_validity, unpack_validity
 from .batch import GroupByBatch
 from .groupby import GroupBy, GroupByRequest
+from .pool import DevicePool
 from .serving import Server
 from .types import Interpolation, Kind, NullPolicy, Sorted, TypeId
 
 __all__ = [
-    "aggregation", "hashing", "Column", "Table", "GroupBy", "GroupByBatch", "GroupByRequest", "Server", "Interpolation", "Kind", "NullPolicy",
+    "aggregation", "hashing", "Column", "Table", "GroupBy", "GroupByBatch", "GroupByRequest", "DevicePool", "Server", "Interpolation", "Kind", "NullPolicy",
     "Sorted", "TypeId", "pack_validity", "unpack_validity",
 ]

@@ -54,8 +54,16 @@ ALLOC_MANY_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32
 
 
 class mgb_allocator(ctypes.Structure):
-    _fields_ = [("allocate", ALLOC_FN), ("deallocate", DEALLOC_FN), ("ctx", ctypes.c_void_p),
-                ("allocate_many", ALLOC_MANY_FN)]
+    pass
+
+
+mgb_allocator._fields_ = [("allocate", ALLOC_FN), ("deallocate", DEALLOC_FN), ("ctx", ctypes.c_void_p),
+                          ("allocate_many", ALLOC_MANY_FN), ("temporaries", ctypes.POINTER(mgb_allocator))]
+
+
+class mgb_pool_stats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "limit_bytes", "held_bytes", "in_use_bytes", "peak_in_use_bytes", "chunks", "stream_waits")]
 
 
 class mgb_out_column(ctypes.Structure):
@@ -130,7 +138,8 @@ SIGNATURES = {
     ),
     "mgb_estimate_groups": (
         ctypes.c_int,
-        [ctypes.POINTER(mgb_column), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)],
+        [ctypes.POINTER(mgb_column), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(mgb_allocator),
+         ctypes.POINTER(ctypes.c_int64)],
     ),
     "mgb_murmurhash3_x86_32": (
         ctypes.c_int,
@@ -145,6 +154,11 @@ SIGNATURES = {
         ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)],
     ),
     "mgb_release_workspace": (ctypes.c_int, [ctypes.c_void_p]),
+    "mgb_pool_create": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
+    "mgb_pool_allocator": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(mgb_allocator)]),
+    "mgb_pool_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(mgb_pool_stats)]),
+    "mgb_pool_trim": (ctypes.c_int, [ctypes.c_void_p]),
+    "mgb_pool_destroy": (None, [ctypes.c_void_p]),
     "mgb_batch_create": (
         ctypes.c_int,
         [

@@ -486,6 +486,29 @@ void* fb_alloc(void*, size_t bytes, void*)
 }
 void fb_free(void*, void* p, size_t, void*) { (void)hipFree(p); }
 
+// Temporaries of the plain-path re-runs: from the batch's allocator, given
+// back only after the stream has been waited for (the re-runs' last kernels
+// may still read them when a call returns, and the allocator may hand a
+// returned block to another stream at once).
+struct DeferredTemps {
+  const mgb_allocator* base;
+  std::vector<std::pair<void*, size_t>> pending;
+  static void* allocate(void* ctx, size_t bytes, void* stream)
+  {
+    const DeferredTemps* d = static_cast<const DeferredTemps*>(ctx);
+    return d->base->allocate(d->base->ctx, bytes, stream);
+  }
+  static void deallocate(void* ctx, void* p, size_t bytes, void*)
+  {
+    static_cast<DeferredTemps*>(ctx)->pending.push_back({p, bytes});
+  }
+  void flush(hipStream_t stream)
+  {
+    for (auto& b : pending) base->deallocate(base->ctx, b.first, b.second, (void*)stream);
+    pending.clear();
+  }
+};
+
 // AggParams of one query's columns with the batch plan's ops and record.
 mgb_status query_params(const mgb_batch& b, int q, AggParams& ap)
 {
@@ -1122,6 +1145,10 @@ extern "C" mgb_status mgb_batch_finish(mgb_batch* b, void* stream_v)
                          stream));
   MGB_HIP(hipStreamSynchronize(stream));
   int64_t reruns = 0;
+  const mgb_allocator fb{fb_alloc, fb_free, nullptr, nullptr, nullptr};
+  const mgb_allocator& out_alloc = b->own_alloc ? b->alloc : fb;
+  DeferredTemps temps{out_alloc.temporaries ? out_alloc.temporaries : &out_alloc, {}};
+  const mgb_allocator temp_alloc{DeferredTemps::allocate, DeferredTemps::deallocate, &temps, nullptr, nullptr};
   for (int q = 0; q < Q; ++q) {
     const unsigned long long* c = b->host_counters + 4 * (size_t)q;
     b->groups[q] = (int64_t)c[2];
@@ -1137,16 +1164,25 @@ extern "C" mgb_status mgb_batch_finish(mgb_batch* b, void* stream_v)
     }
     b->fb_keys[q].assign(b->num_keys, mgb_out_column{});
     b->fb_results[q].assign(b->num_outs, mgb_out_column{});
-    const mgb_allocator al = b->own_alloc ? b->alloc : mgb_allocator{fb_alloc, fb_free, nullptr};
+    mgb_allocator al = out_alloc;
+    al.temporaries = &temp_alloc;
     int64_t g = 0;
     mgb_status st = mgb_groupby_aggregate(&b->keys[(size_t)q * b->num_keys], b->num_keys, b->null_policy,
                                           reqs.data(), b->num_requests, b->fb_keys[q].data(), b->fb_results[q].data(),
                                           &g, stream, &al);
-    if (st != MGB_SUCCESS) return st;
+    if (st != MGB_SUCCESS) {
+      (void)hipStreamSynchronize(stream);
+      temps.flush(stream);
+      return st;
+    }
     b->groups[q] = g;
   }
-  // re-run results are read from any stream afterwards
-  if (reruns) MGB_HIP(hipStreamSynchronize(stream));
+  // re-run results are read from any stream afterwards; their temporaries go back
+  if (reruns) {
+    hipError_t e = hipStreamSynchronize(stream);
+    temps.flush(stream);
+    MGB_HIP(e);
+  }
   b->stats.reruns = reruns;
   return MGB_SUCCESS;
 }

@@ -140,7 +140,7 @@ struct mr_allocator {
   }
   static mgb_allocator make(rmm::mr::device_memory_resource* mr)
   {
-    return mgb_allocator{&mr_allocator::allocate, &mr_allocator::deallocate, mr};
+    return mgb_allocator{&mr_allocator::allocate, &mr_allocator::deallocate, mr, nullptr, nullptr};
   }
 };
 
@@ -299,7 +299,11 @@ std::pair<std::unique_ptr<table>, std::vector<aggregation_result>> groupby::aggr
   std::vector<mgb_out_column> out_keys(keys.size());
   std::vector<mgb_out_column> out_results(n_out);
   int64_t groups = 0;
+  // results from `mr`, temporaries from the current device resource (as
+  // cudf's hash groupby takes them, cpp/src/groupby/hash/groupby.cu:404,478,484)
+  mgb_allocator temps = detail::mr_allocator::make(rmm::mr::get_current_device_resource());
   mgb_allocator alloc = detail::mr_allocator::make(mr);
+  alloc.temporaries = &temps;
   mgb_status st = mgb_groupby_aggregate(keys.data(), static_cast<int32_t>(keys.size()),
                                         _include_null_keys == null_policy::INCLUDE ? MGB_NULL_INCLUDE
                                                                                    : MGB_NULL_EXCLUDE,

@@ -825,46 +825,56 @@ std::vector<ProfRecord> g_prof;
 
 
 
-constexpr size_t kStagingBytes = 64 * 1024;  // pinned parameter staging (kUploadSlots slots)
+constexpr size_t kStagingBytes = 64 * 1024;  // parameter staging (kUploadSlots slots)
 constexpr int kUploadSlots = 8;
 
-struct Workspace {
-  std::mutex mu;
-  DeviceBuffer main;    // tables, partitioned columns, histograms, counters, sample table
-  DeviceBuffer outs;    // temporary output columns
-  DeviceBuffer params;  // kernel parameter blocks (output tables, identity templates)
-  DeviceBuffer fine;    // fine bucket counts per chunk of the last key pass (hll_hist_kernel)
-  DeviceBuffer sorts;   // row groups, sort keys and radix-sort space of the sort-path aggregations
-  bool fine_valid = false;  // `fine` holds the counts of this call's keys
-  int64_t fine_nw = 0, fine_chunk = 0;
-  unsigned long long* pinned = nullptr;  // host readback area
-  uint8_t* staging = nullptr;            // pinned source of parameter uploads
-  int upload_slot = 0;                   // next staging / params slot of the current call
-  ~Workspace()
+// Host memory a stream's calls share: pinned readback slots and the pinned
+// source of parameter uploads, taken at a stream's first call and kept.
+struct HostStaging {
+  std::mutex mu;  // one call at a time per stream
+  unsigned long long* pinned = nullptr;
+  uint8_t* staging = nullptr;
+  ~HostStaging()
   {
-    if (main.ptr) (void)hipFree(main.ptr);
-    if (outs.ptr) (void)hipFree(outs.ptr);
-    if (params.ptr) (void)hipFree(params.ptr);
-    if (fine.ptr) (void)hipFree(fine.ptr);
-    if (sorts.ptr) (void)hipFree(sorts.ptr);
     if (pinned) (void)hipHostFree(pinned);
     if (staging) (void)hipHostFree(staging);
   }
 };
 
 std::mutex g_ws_mu;
-std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<HostStaging>> g_ws;
 
-Workspace* workspace_for(hipStream_t s)
+HostStaging* staging_for(hipStream_t s)
 {
   int dev = 0;
   (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lk(g_ws_mu);
   auto& w = g_ws[{dev, s}];
-  if (!w) w.reset(new Workspace());
+  if (!w) w.reset(new HostStaging());
   return w.get();
 }
 
+// One call's working memory: device temporaries (from the call's allocator
+// for temporaries, or a fixed region) and the host staging it uses.
+struct Workspace {
+  CallMem mem;
+  DeviceBuffer main{&mem};    // tables, partitioned columns, histograms, counters, sample table
+  DeviceBuffer outs{&mem};    // temporary output columns
+  DeviceBuffer params{&mem};  // kernel parameter blocks (output tables, identity templates)
+  DeviceBuffer fine{&mem};    // fine bucket counts per chunk of the last key pass (hll_hist_kernel)
+  DeviceBuffer sorts{&mem};   // row groups, sort keys and radix-sort space of the sort-path aggregations
+  bool fine_valid = false;  // `fine` holds the counts of this call's keys
+  int64_t fine_nw = 0, fine_chunk = 0;
+  unsigned long long* pinned = nullptr;  // host readback area (kMaxPinned slots)
+  uint8_t* staging = nullptr;            // host source of parameter uploads (kStagingBytes)
+  int upload_slot = 0;                   // next staging / params slot of the call
+};
+
+#define MGB_RESERVE(buf, n)                                                                              \
+  do {                                                                                                   \
+    if ((buf).reserve((n), stream) != hipSuccess)                                                        \
+      return fail(MGB_ERROR_ALLOC, "allocation of " + std::to_string((size_t)(n)) + " bytes of temporaries failed"); \
+  } while (0)
 
 constexpr int kMaxPinned = 1024;  // host readback slots (counters, null counts)
 constexpr int kPinnedNulls = 128; // first slot of null counts read back with the group count
@@ -1170,7 +1180,7 @@ mgb_status layout_temp(Workspace* ws, const Plan& plan, int64_t rows, hipStream_
   }
   size_t nc_off = off;
   off += align_up((size_t)(nk + no) * 8, 256);
-  MGB_HIP(ws->outs.reserve(off, stream));
+  MGB_RESERVE(ws->outs, off);
   uint8_t* b = static_cast<uint8_t*>(ws->outs.ptr);
   t = TempOuts{};
   t.rows = rows;
@@ -1204,7 +1214,7 @@ mgb_status upload_params(Workspace* ws, const OutTable& ot, const uint64_t* tmpl
                 "parameter slot too small");
   if (ws->upload_slot >= kUploadSlots) return fail(MGB_ERROR_UNSUPPORTED, "too many parameter uploads in one call");
   const int slot = ws->upload_slot++;
-  MGB_HIP(ws->params.reserve(kStagingBytes, stream));
+  MGB_RESERVE(ws->params, kStagingBytes);
   uint8_t* host = ws->staging + slot * slot_bytes;
   uint8_t* dev = static_cast<uint8_t*>(ws->params.ptr) + slot * slot_bytes;
   std::memcpy(host, &ot, sizeof(OutTable));
@@ -1222,7 +1232,7 @@ mgb_status upload_pass2(Workspace* ws, const Pass2Ops& q, hipStream_t stream, co
   static_assert(sizeof(Pass2Ops) <= kStagingBytes / kUploadSlots, "parameter slot too small");
   if (ws->upload_slot >= kUploadSlots) return fail(MGB_ERROR_UNSUPPORTED, "too many parameter uploads in one call");
   const int slot = ws->upload_slot++;
-  MGB_HIP(ws->params.reserve(kStagingBytes, stream));
+  MGB_RESERVE(ws->params, kStagingBytes);
   uint8_t* host = ws->staging + slot * slot_bytes;
   uint8_t* dev = static_cast<uint8_t*>(ws->params.ptr) + slot * slot_bytes;
   std::memcpy(host, &q, sizeof(Pass2Ops));
@@ -1231,13 +1241,105 @@ mgb_status upload_pass2(Workspace* ws, const Pass2Ops& q, hipStream_t stream, co
   return MGB_SUCCESS;
 }
 
-// Allocates the caller's columns for `groups` rows, copies the temporary
-// outputs into them and builds their null masks.
-mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int64_t groups, mgb_out_column* out_keys,
-                        mgb_out_column* out_results, hipStream_t stream, const mgb_allocator* alloc)
+// Where the result columns go: buffers from the caller's allocator, or
+// caller-provided buffers of `capacity` rows each (the entry point that
+// allocates nothing).
+struct OutputSink {
+  const mgb_allocator* alloc = nullptr;
+  int64_t capacity = -1;  // >= 0: the caller's buffers (data and null_mask of every output)
+};
+
+// Copies the temporary outputs into the result columns (data and null_mask
+// set, null_mask NULL where the column has no validity) and builds their
+// null masks and null counts.
+mgb_status copy_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int64_t groups, mgb_out_column* out_keys,
+                        mgb_out_column* out_results, hipStream_t stream)
 {
-  Allocs al{alloc, stream, {}};
   const int nk = (int)plan.keys.size(), no = (int)plan.outs.size();
+  if (groups == 0) return MGB_SUCCESS;
+  if (!t.nulls_known) MGB_HIP(hipMemsetAsync(t.null_counts, 0, (size_t)(nk + no) * 8, stream));
+  const int64_t words = bitmask_bytes(groups) / 4;
+  // every column's copy and validity in one launch (per kEmitCols columns)
+  for (int j0 = 0; j0 < no + nk; j0 += kEmitCols) {
+    EmitParams ep{};
+    ep.rows = groups;
+    const int n = std::min(kEmitCols, no + nk - j0);
+    int64_t units = words;
+    for (int i = 0; i < n; ++i) {
+      const int j = j0 + i;
+      const mgb_out_column& oc = j < no ? out_results[j] : out_keys[j - no];
+      const int width = j < no ? plan.outs[j].spec.out_width : plan.keys[j - no].width;
+      EmitCol& c = ep.cols[i];
+      c.src = j < no ? t.out_data[j] : t.key_data[j - no];
+      c.dst = static_cast<uint8_t*>(oc.data);
+      c.bytes = groups * width;
+      c.valid = j < no ? t.out_valid[j] : t.key_valid[j - no];
+      c.mask = oc.null_mask;
+      c.words = oc.null_mask ? words : 0;
+      c.nulls = t.nulls_known ? nullptr : t.null_counts + j;
+      units = std::max(units, (c.bytes + 15) / 16);
+    }
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((units + kBlock - 1) / kBlock, 1024));
+    emit_columns_kernel<<<dim3(gx, (unsigned)n), kBlock, 0, stream>>>(ep);
+  }
+  MGB_HIP(hipGetLastError());
+  const int64_t* counts = nullptr;
+  if (t.nulls_known) {
+    counts = t.host_nulls.data();
+  } else {
+    MGB_HIP(hipMemcpyAsync(ws->pinned, t.null_counts, (size_t)(no + nk) * 8, hipMemcpyDeviceToHost, stream));
+    MGB_HIP(hipStreamSynchronize(stream));
+    counts = reinterpret_cast<const int64_t*>(ws->pinned);
+  }
+  for (int o = 0; o < no; ++o) out_results[o].null_count = out_results[o].null_mask ? counts[o] : 0;
+  for (int c = 0; c < nk; ++c) out_keys[c].null_count = out_keys[c].null_mask ? counts[no + c] : 0;
+  return MGB_SUCCESS;
+}
+
+// The result columns in the caller's buffers: every output's data buffer
+// must hold `capacity` rows (data_bytes), and its null_mask buffer
+// bitmask_bytes(capacity) bytes (mask_bytes) where the column has validity.
+mgb_status emit_into_caller_buffers(Workspace* ws, const Plan& plan, const TempOuts& t, int64_t groups,
+                                    mgb_out_column* out_keys, mgb_out_column* out_results, hipStream_t stream,
+                                    int64_t capacity)
+{
+  const int nk = (int)plan.keys.size(), no = (int)plan.outs.size();
+  if (groups > capacity)
+    return fail(MGB_ERROR_ALLOC, "the result has " + std::to_string(groups) + " groups, more than the " +
+                                     std::to_string(capacity) + " rows the output buffers hold");
+  auto check_col = [&](mgb_out_column& oc, int type_id, int width, const uint8_t* valid, const char* what) {
+    oc.type_id = type_id;
+    oc.size = groups;
+    oc.null_count = 0;
+    if (groups > 0 && (!oc.data || oc.data_bytes < (size_t)(groups * width)))
+      return fail(MGB_ERROR_LOGIC, std::string(what) + ": data buffer missing or too small");
+    if (valid && groups > 0 && (!oc.null_mask || oc.mask_bytes < (size_t)bitmask_bytes(groups)))
+      return fail(MGB_ERROR_LOGIC, std::string(what) + ": null_mask buffer missing or too small");
+    if (!valid) oc.null_mask = nullptr;
+    return MGB_SUCCESS;
+  };
+  for (int c = 0; c < nk; ++c) {
+    mgb_status st = check_col(out_keys[c], plan.keys[c].type_id, plan.keys[c].width, t.key_valid[c], "key column");
+    if (st != MGB_SUCCESS) return st;
+  }
+  for (int o = 0; o < no; ++o) {
+    mgb_status st = check_col(out_results[o], plan.outs[o].out_type, plan.outs[o].spec.out_width, t.out_valid[o],
+                              "result column");
+    if (st != MGB_SUCCESS) return st;
+  }
+  return copy_outputs(ws, plan, t, groups, out_keys, out_results, stream);
+}
+
+// Gives the caller's columns `groups` rows (allocated, or checked against the
+// caller's buffers), copies the temporary outputs into them and builds their
+// null masks.
+mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int64_t groups, mgb_out_column* out_keys,
+                        mgb_out_column* out_results, hipStream_t stream, const OutputSink& sink)
+{
+  if (sink.capacity >= 0)
+    return emit_into_caller_buffers(ws, plan, t, groups, out_keys, out_results, stream, sink.capacity);
+  const int nk = (int)plan.keys.size(), no = (int)plan.outs.size();
+  Allocs al{sink.alloc, stream, {}};
   // the columns' sizes, then every buffer in one request
   std::vector<size_t> sizes;
   std::vector<void**> dests;
@@ -1266,44 +1368,13 @@ mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int6
     return fail(MGB_ERROR_ALLOC, "allocation of the output columns failed");
   }
   for (size_t i = 0; i < ptrs.size(); ++i) *dests[i] = ptrs[i];
-  if (groups > 0) {
-    if (!t.nulls_known) MGB_HIP(hipMemsetAsync(t.null_counts, 0, (size_t)(nk + no) * 8, stream));
-    const int64_t words = bitmask_bytes(groups) / 4;
-    // every column's copy and validity in one launch (per kEmitCols columns)
-    for (int j0 = 0; j0 < no + nk; j0 += kEmitCols) {
-      EmitParams ep{};
-      ep.rows = groups;
-      const int n = std::min(kEmitCols, no + nk - j0);
-      int64_t units = words;
-      for (int i = 0; i < n; ++i) {
-        const int j = j0 + i;
-        const mgb_out_column& oc = j < no ? out_results[j] : out_keys[j - no];
-        EmitCol& c = ep.cols[i];
-        c.src = j < no ? t.out_data[j] : t.key_data[j - no];
-        c.dst = static_cast<uint8_t*>(oc.data);
-        c.bytes = (int64_t)oc.data_bytes;
-        c.valid = j < no ? t.out_valid[j] : t.key_valid[j - no];
-        c.mask = oc.null_mask;
-        c.words = oc.null_mask ? words : 0;
-        c.nulls = t.nulls_known ? nullptr : t.null_counts + j;
-        units = std::max(units, (c.bytes + 15) / 16);
-      }
-      const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((units + kBlock - 1) / kBlock, 1024));
-      emit_columns_kernel<<<dim3(gx, (unsigned)n), kBlock, 0, stream>>>(ep);
-    }
-    MGB_HIP(hipGetLastError());
-    const int64_t* counts = nullptr;
-    if (t.nulls_known) {
-      counts = t.host_nulls.data();
-    } else {
-      MGB_HIP(hipMemcpyAsync(ws->pinned, t.null_counts, (size_t)(no + nk) * 8, hipMemcpyDeviceToHost, stream));
-      MGB_HIP(hipStreamSynchronize(stream));
-      counts = reinterpret_cast<const int64_t*>(ws->pinned);
-    }
-    for (int o = 0; o < no; ++o) out_results[o].null_count = out_results[o].null_mask ? counts[o] : 0;
-    for (int c = 0; c < nk; ++c) out_keys[c].null_count = out_keys[c].null_mask ? counts[no + c] : 0;
+  mgb_status st = copy_outputs(ws, plan, t, groups, out_keys, out_results, stream);
+  if (st != MGB_SUCCESS) {
+    al.release_all();
+    for (int c = 0; c < nk; ++c) out_keys[c].data = out_keys[c].null_mask = nullptr;
+    for (int o = 0; o < no; ++o) out_results[o].data = out_results[o].null_mask = nullptr;
   }
-  return MGB_SUCCESS;
+  return st;
 }
 
 // The LDS path's view of a column's ops (ValCol::sum_off ... lds_generic).
@@ -1475,12 +1546,14 @@ int64_t hll_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_
   const size_t regs_bytes = align_up((size_t)nb * kHllRegs, 256);
   const size_t need = regs_bytes + align_up((size_t)nred * 8, 256) + align_up((size_t)nred * 4, 256);
   const size_t lds = (size_t)(kHllRegs + kFineBuckets) * 4;
-  hipError_t e = ws->main.reserve(need, stream);
-  if (e == hipSuccess) e = ws->fine.reserve((size_t)kFineBuckets * nb * 4, stream);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)hll_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (ws->main.reserve(need, stream) != hipSuccess ||
+      ws->fine.reserve((size_t)kFineBuckets * nb * 4, stream) != hipSuccess) {
+    st = fail(MGB_ERROR_ALLOC, "allocation of the cardinality estimate's temporaries failed");
+    return R;
+  }
+  hipError_t e = hipFuncSetAttribute((const void*)hll_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) {
-    st = fail(MGB_ERROR_HIP, std::string("workspace: ") + hipGetErrorString(e));
+    st = fail(MGB_ERROR_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
     return R;
   }
   uint8_t* regs = static_cast<uint8_t*>(ws->main.ptr);
@@ -1526,11 +1599,11 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
   // small inputs are counted exactly; larger ones are sampled
   const int64_t samples = R <= scap / 2 ? R : kSample;
   size_t need = align_up((size_t)(scap + 1) * 8, 256) + align_up((size_t)(scap + 1) * 4, 256) + 256;
-  hipError_t e = ws->main.reserve(need, stream);
-  if (e != hipSuccess) {
-    st = fail(MGB_ERROR_HIP, std::string("workspace: ") + hipGetErrorString(e));
+  if (ws->main.reserve(need, stream) != hipSuccess) {
+    st = fail(MGB_ERROR_ALLOC, "allocation of the cardinality sample's temporaries failed");
     return R;
   }
+  hipError_t e = hipSuccess;
   SampleParams sp{};
   sp.agg = base;
   sp.hdr = static_cast<unsigned long long*>(ws->main.ptr);
@@ -1618,7 +1691,7 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
   const size_t o4 = align_up(((size_t)G + 1) * 4, 256);
   // gid | A, B (8 bytes a row) | C, D, E (4 bytes a row) | offset, has_null | cub
   const size_t need = 4 * r4 + 2 * r8 + o4 + g4 + tcub;
-  MGB_HIP(ws->sorts.reserve(need, stream));
+  MGB_RESERVE(ws->sorts, need);
   uint8_t* b = static_cast<uint8_t*>(ws->sorts.ptr);
   size_t off = 0;
   auto take = [&](size_t bytes) { uint8_t* p = b + off; off += bytes; return p; };
@@ -1729,8 +1802,92 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
   return MGB_SUCCESS;
 }
 
-// Global-table executions (global and direct paths): build the table, then
-// compact it into temporary outputs.
+// Global-table executions (global and direct paths): build the table, run
+// the second pass, compact it into temporary outputs (null results counted
+// as they are written) and read the group count, the overflow flag and the
+// null counts back with one wait. A table that overflowed is rebuilt larger.
+// The sort-path aggregations need the group count on the host first, so
+// their calls wait before compacting.
+mgb_status second_pass_table(Workspace* ws, const Plan& plan, const AggParams& base, int64_t R, bool direct,
+                             const LdsParams& direct_lp, int64_t direct_items, const LdsGeom& geom,
+                             int64_t capacity, hipStream_t stream)
+{
+  const int cus = device_cu_count();
+  if (direct) {
+    // the table is complete: every workgroup loads it into LDS for the
+    // second pass over its slice
+    LdsParams lp = direct_lp;
+    Pass2Ops q{};
+    if (!pass2_ops(plan, lp.src, q)) return fail(MGB_ERROR_UNSUPPORTED, "too many second-pass aggregations");
+    q.rows = lp.src;
+    mgb_status st = upload_pass2(ws, q, stream, &lp.p2);
+    if (st != MGB_SUCCESS) return st;
+    MGB_HIP(hipFuncSetAttribute((const void*)pass2_direct_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)geom.bytes));
+    {
+      ProfScope prof("pass2", stream);
+      pass2_direct_kernel<<<(unsigned)direct_items, kLdsBlock, geom.bytes, stream>>>(lp);
+    }
+    MGB_HIP(hipGetLastError());
+    return MGB_SUCCESS;
+  }
+  // the table is complete: each row finds its group and updates the
+  // second-pass fields, in launches of up to kMaxPass2 ops
+  AggParams src = base;
+  src.table = static_cast<uint8_t*>(ws->main.ptr);
+  src.capacity = capacity;
+  src.flags = nullptr;
+  std::vector<Pass2Op> all;
+  for (size_t c = 0; c < plan.cols.size(); ++c) {
+    ValCol v = plan.cols[c].col;
+    v.flag_bit = -1;
+    for (auto& o : plan.cols[c].p2) all.push_back(pass2_op(plan, plan.cols[c], o, v, (int)c));
+  }
+  for (size_t first = 0; first < all.size(); first += kMaxPass2) {
+    Pass2Ops q{};
+    q.num_ops = (int)std::min<size_t>(kMaxPass2, all.size() - first);
+    for (int i = 0; i < q.num_ops; ++i) q.ops[i] = all[first + i];
+    const Pass2Ops* d_q = nullptr;
+    mgb_status st = upload_pass2(ws, q, stream, &d_q);
+    if (st != MGB_SUCCESS) return st;
+    const int64_t grid = std::min<int64_t>((R + kBlock - 1) / kBlock, (int64_t)cus * 16);
+    {
+      ProfScope prof("pass2", stream);
+      pass2_kernel<<<(unsigned)std::max<int64_t>(grid, 1), kBlock, 0, stream>>>(src, d_q);
+    }
+    MGB_HIP(hipGetLastError());
+  }
+  return MGB_SUCCESS;
+}
+
+// Compacts the table's groups into the temporary outputs (`temp`, laid out
+// for at least the table's groups), counting null results when `nulls`.
+mgb_status finalize_table(Workspace* ws, const Plan& plan, int64_t capacity, int64_t nblocks, size_t block_off,
+                          uint32_t* rec_pos, hipStream_t stream, const OutTable& ot)
+{
+  uint64_t tmpl[kMaxRecordWords] = {};
+  const OutTable* d_ot = nullptr;
+  const uint64_t* d_tmpl = nullptr;
+  mgb_status st = upload_params(ws, ot, tmpl, 1, stream, &d_ot, &d_tmpl);
+  if (st != MGB_SUCCESS) return st;
+  FinParams f{};
+  f.table = static_cast<const uint8_t*>(ws->main.ptr);
+  f.capacity = capacity;
+  f.stride = plan.stride;
+  f.key_mode = plan.key_mode;
+  f.num_keys = (int)plan.keys.size();
+  f.block_offsets = reinterpret_cast<const int64_t*>(static_cast<uint8_t*>(ws->main.ptr) + block_off);
+  f.out = d_ot;
+  for (size_t c = 0; c < plan.keys.size(); ++c) f.keys[c] = plan.keys[c];
+  f.rec_pos = rec_pos;
+  {
+    ProfScope prof("finalize", stream);
+    finalize_kernel<<<(unsigned)nblocks, kBlock, 0, stream>>>(f);
+  }
+  MGB_HIP(hipGetLastError());
+  return MGB_SUCCESS;
+}
+
 mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int64_t R, int64_t G_est, bool direct,
                      const LdsGeom& geom, hipStream_t stream, TempOuts& temp, int64_t& groups)
 {
@@ -1740,17 +1897,23 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
   int64_t nblocks = 0;
   size_t block_off = 0;
   size_t rec_pos_off = 0;  // per record: its output row (sort-path aggregations)
-  LdsParams direct_lp{};  // the direct launch, for its second pass
-  int64_t direct_items = 0;
+  // without sort-path aggregations the groups are compacted before the one
+  // wait of the call
+  const bool early = !plan.has_sort;
+  const int n_nulls = (int)plan.outs.size() + (int)plan.keys.size();
+  const int first_slot = ws->upload_slot;
   for (int attempt = 0;; ++attempt) {
     if (capacity > (int64_t)0x7fffffff) return fail(MGB_ERROR_UNSUPPORTED, "groupby table would exceed 2^31 records");
+    // the previous attempt's kernels have completed (its counters were
+    // read): its parameter slots are free again
+    ws->upload_slot = first_slot;
     nblocks = (capacity + 1 + kBlock - 1) / kBlock;
     const size_t table_bytes = align_up((size_t)(capacity + 1) * plan.stride, 256);
     const size_t counters_off = table_bytes;
     block_off = counters_off + 256;
     rec_pos_off = block_off + align_up((size_t)nblocks * 8, 256);
     const size_t need = rec_pos_off + (plan.has_sort ? align_up((size_t)(capacity + 1) * 4, 256) : 0);
-    MGB_HIP(ws->main.reserve(need, stream));
+    MGB_RESERVE(ws->main, need);
     uint8_t* wsb = static_cast<uint8_t*>(ws->main.ptr);
 
     FillParams fp{};
@@ -1773,6 +1936,8 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
     ap.capacity = capacity;
     ap.counters = counters;
     ap.max_groups = std::max<int64_t>(1, capacity * 3 / 4);
+    LdsParams direct_lp{};  // the direct launch, for its second pass
+    int64_t direct_items = 0;
     if (direct) {
       LdsParams lp{};
       lp.src = ap;
@@ -1843,103 +2008,61 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
         first = false;
       } while (ci < plan.cols.size());
     }
+    if (plan.has_pass2 && early) {
+      // a table that overflowed is missing groups: the second pass skips
+      // their rows, and the attempt is discarded below
+      mgb_status st = second_pass_table(ws, plan, base, R, direct, direct_lp, direct_items, geom, capacity, stream);
+      if (st != MGB_SUCCESS) return st;
+    }
 
     int64_t* block_counts = reinterpret_cast<int64_t*>(wsb + block_off);
     count_occupied_kernel<<<(unsigned)nblocks, kBlock, 0, stream>>>(wsb, capacity, plan.stride, plan.key_mode,
                                                                      block_counts, nullptr);
     scan_blocks_kernel<<<1, 1024, 0, stream>>>(block_counts, nblocks, counters + 2, nullptr);
     MGB_HIP(hipGetLastError());
+    if (early) {
+      // temporary outputs for every record the table can hold
+      OutTable ot{};
+      mgb_status st = layout_temp(ws, plan, std::min<int64_t>(capacity + 1, R), stream, temp, ot);
+      if (st != MGB_SUCCESS) return st;
+      ot.nulls = temp.null_counts;
+      MGB_HIP(hipMemsetAsync(temp.null_counts, 0, (size_t)n_nulls * 8, stream));
+      st = finalize_table(ws, plan, capacity, nblocks, block_off, nullptr, stream, ot);
+      if (st != MGB_SUCCESS) return st;
+      MGB_HIP(hipMemcpyAsync(ws->pinned + kPinnedNulls, temp.null_counts, (size_t)n_nulls * 8, hipMemcpyDeviceToHost,
+                             stream));
+    }
     MGB_HIP(hipMemcpyAsync(ws->pinned, counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     MGB_HIP(hipStreamSynchronize(stream));
     if (ws->pinned[1] == 0) {
       groups = (int64_t)ws->pinned[2];
+      if (early) {
+        temp.nulls_known = true;
+        temp.host_nulls.assign(ws->pinned + kPinnedNulls, ws->pinned + kPinnedNulls + n_nulls);
+        return MGB_SUCCESS;
+      }
+      // sort-path aggregations: second pass, compaction and the sorts after the wait
+      if (plan.has_pass2 && groups > 0) {
+        mgb_status st = second_pass_table(ws, plan, base, R, direct, direct_lp, direct_items, geom, capacity, stream);
+        if (st != MGB_SUCCESS) return st;
+      }
       break;
     }
     if (attempt >= 6) return fail(MGB_ERROR_HIP, "groupby table kept overflowing");
     capacity = std::min<int64_t>(capacity * 4, std::max<int64_t>(2 * R + 64, 64));
-    // the attempt's kernels have completed (the counters were read): its
-    // parameter slot is free again
-    if (direct) ws->upload_slot = std::max(0, ws->upload_slot - 1);
-  }
-
-  if (plan.has_pass2 && groups > 0 && direct) {
-    // the table is complete: every workgroup loads it into LDS for the
-    // second pass over its slice
-    LdsParams lp = direct_lp;
-    Pass2Ops q{};
-    if (!pass2_ops(plan, lp.src, q)) return fail(MGB_ERROR_UNSUPPORTED, "too many second-pass aggregations");
-    q.rows = lp.src;
-    mgb_status st = upload_pass2(ws, q, stream, &lp.p2);
-    if (st != MGB_SUCCESS) return st;
-    MGB_HIP(hipFuncSetAttribute((const void*)pass2_direct_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)geom.bytes));
-    {
-      ProfScope prof("pass2", stream);
-      pass2_direct_kernel<<<(unsigned)direct_items, kLdsBlock, geom.bytes, stream>>>(lp);
-    }
-    MGB_HIP(hipGetLastError());
-  } else if (plan.has_pass2 && groups > 0) {
-    // the table is complete: each row finds its group and updates the
-    // second-pass fields, in launches of up to kMaxPass2 ops
-    AggParams src = base;
-    src.table = static_cast<uint8_t*>(ws->main.ptr);
-    src.capacity = capacity;
-    src.flags = nullptr;
-    std::vector<Pass2Op> all;
-    for (size_t c = 0; c < plan.cols.size(); ++c) {
-      ValCol v = plan.cols[c].col;
-      v.flag_bit = -1;
-      for (auto& o : plan.cols[c].p2) all.push_back(pass2_op(plan, plan.cols[c], o, v, (int)c));
-    }
-    for (size_t first = 0; first < all.size(); first += kMaxPass2) {
-      Pass2Ops q{};
-      q.num_ops = (int)std::min<size_t>(kMaxPass2, all.size() - first);
-      for (int i = 0; i < q.num_ops; ++i) q.ops[i] = all[first + i];
-      const Pass2Ops* d_q = nullptr;
-      mgb_status st = upload_pass2(ws, q, stream, &d_q);
-      if (st != MGB_SUCCESS) return st;
-      const int64_t grid = std::min<int64_t>((R + kBlock - 1) / kBlock, (int64_t)cus * 16);
-      {
-        ProfScope prof("pass2", stream);
-        pass2_kernel<<<(unsigned)std::max<int64_t>(grid, 1), kBlock, 0, stream>>>(src, d_q);
-      }
-      MGB_HIP(hipGetLastError());
-    }
   }
 
   OutTable ot{};
   mgb_status st = layout_temp(ws, plan, groups, stream, temp, ot);
   if (st != MGB_SUCCESS) return st;
   if (groups == 0) return MGB_SUCCESS;
-  uint64_t tmpl[kMaxRecordWords] = {};
-  const OutTable* d_ot = nullptr;
-  const uint64_t* d_tmpl = nullptr;
-  st = upload_params(ws, ot, tmpl, 1, stream, &d_ot, &d_tmpl);
+  uint32_t* rec_pos = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws->main.ptr) + rec_pos_off);
+  st = finalize_table(ws, plan, capacity, nblocks, block_off, rec_pos, stream, ot);
   if (st != MGB_SUCCESS) return st;
-  FinParams f{};
-  f.table = static_cast<const uint8_t*>(ws->main.ptr);
-  f.capacity = capacity;
-  f.stride = plan.stride;
-  f.key_mode = plan.key_mode;
-  f.num_keys = (int)plan.keys.size();
-  f.block_offsets = reinterpret_cast<const int64_t*>(static_cast<uint8_t*>(ws->main.ptr) + block_off);
-  f.out = d_ot;
-  for (size_t c = 0; c < plan.keys.size(); ++c) f.keys[c] = plan.keys[c];
-  uint32_t* rec_pos = plan.has_sort ? reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws->main.ptr) + rec_pos_off)
-                                    : nullptr;
-  f.rec_pos = rec_pos;
-  {
-    ProfScope prof("finalize", stream);
-    finalize_kernel<<<(unsigned)nblocks, kBlock, 0, stream>>>(f);
-  }
-  MGB_HIP(hipGetLastError());
-  if (plan.has_sort) {
-    AggParams rows = base;
-    rows.table = static_cast<uint8_t*>(ws->main.ptr);
-    rows.capacity = capacity;
-    return run_sort_aggs(ws, plan, rows, rec_pos, groups, stream, temp);
-  }
-  return MGB_SUCCESS;
+  AggParams rows = base;
+  rows.table = static_cast<uint8_t*>(ws->main.ptr);
+  rows.capacity = capacity;
+  return run_sort_aggs(ws, plan, rows, rec_pos, groups, stream, temp);
 }
 
 // Partitioned execution: hash-partition the rows, then aggregate every
@@ -2033,7 +2156,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     off += align_up((size_t)gblocks * 8, 256);
     const size_t ctr_off = off;
     off += 256;
-    MGB_HIP(ws->main.reserve(off, stream));
+    MGB_RESERVE(ws->main, off);
     uint8_t* wsb = static_cast<uint8_t*>(ws->main.ptr);
     // ctr: [0] partitioned rows [1] groups written by items [2] unused [3] items
     //      [4] global groups [5] global overflow [6] global occupied
@@ -2242,16 +2365,10 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
 }
 
 
-mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mgb_out_column* out_results,
-                       int64_t* out_num_groups, hipStream_t stream, const mgb_allocator* alloc)
+mgb_status run_groupby(const Plan& plan, int64_t R, Workspace* ws, mgb_out_column* out_keys,
+                       mgb_out_column* out_results, int64_t* out_num_groups, hipStream_t stream,
+                       const OutputSink& sink)
 {
-  Workspace* ws = workspace_for(stream);
-  std::lock_guard<std::mutex> lk(ws->mu);
-  if (!ws->pinned) {
-    MGB_HIP(hipHostMalloc((void**)&ws->pinned, kMaxPinned * sizeof(unsigned long long), hipHostMallocDefault));
-    MGB_HIP(hipHostMalloc((void**)&ws->staging, kStagingBytes, hipHostMallocDefault));
-  }
-
   ws->upload_slot = 0;
 
   AggParams base{};
@@ -2303,7 +2420,19 @@ mgb_status run_groupby(const Plan& plan, int64_t R, mgb_out_column* out_keys, mg
     if (st != MGB_SUCCESS) return st;
   }
   *out_num_groups = groups;
-  return emit_outputs(ws, plan, temp, groups, out_keys, out_results, stream, alloc);
+  return emit_outputs(ws, plan, temp, groups, out_keys, out_results, stream, sink);
+}
+
+// The host staging of `stream`, taken at its first call (pinned memory).
+mgb_status lock_staging(hipStream_t stream, HostStaging*& hs, std::unique_lock<std::mutex>& lk)
+{
+  hs = staging_for(stream);
+  lk = std::unique_lock<std::mutex>(hs->mu);
+  if (!hs->pinned) {
+    MGB_HIP(hipHostMalloc((void**)&hs->pinned, kMaxPinned * sizeof(unsigned long long), hipHostMallocDefault));
+    MGB_HIP(hipHostMalloc((void**)&hs->staging, kStagingBytes, hipHostMallocDefault));
+  }
+  return MGB_SUCCESS;
 }
 
 mgb_status make_hash_params(const mgb_column* cols, int32_t n, HashParams& hp, int64_t& rows)
@@ -2385,13 +2514,31 @@ extern "C" mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_
   }
   mgb_status st = build_plan(keys, num_keys, null_policy, requests, num_requests, plan);
   if (st != MGB_SUCCESS) return st;
-  return run_groupby(plan, R, out_keys, out_results, out_num_groups, static_cast<hipStream_t>(stream), alloc);
+  const mgb_allocator* temps = alloc->temporaries ? alloc->temporaries : alloc;
+  if (!temps->allocate || !temps->deallocate) return fail(MGB_ERROR_LOGIC, "allocator of temporaries without callbacks");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HostStaging* hs = nullptr;
+  std::unique_lock<std::mutex> lk;
+  st = lock_staging(s, hs, lk);
+  if (st != MGB_SUCCESS) return st;
+  Workspace ws;  // its temporaries go back to `temps` when the call ends, also on failure
+  ws.mem.alloc = temps;
+  ws.mem.stream = s;
+  ws.pinned = hs->pinned;
+  ws.staging = hs->staging;
+  OutputSink sink;
+  sink.alloc = alloc;
+  return run_groupby(plan, R, &ws, out_keys, out_results, out_num_groups, s, sink);
 }
 
 extern "C" mgb_status mgb_estimate_groups(const mgb_column* keys, int32_t num_keys, int32_t null_policy, void* stream_v,
-                                          int64_t* estimate)
+                                          const mgb_allocator* alloc, int64_t* estimate)
 {
+  g_last_error.clear();
   if (!estimate) return fail(MGB_ERROR_LOGIC, "estimate is NULL");
+  if (!alloc) return fail(MGB_ERROR_LOGIC, "allocator is NULL");
+  const mgb_allocator* temps = alloc->temporaries ? alloc->temporaries : alloc;
+  if (!temps->allocate || !temps->deallocate) return fail(MGB_ERROR_LOGIC, "allocator without callbacks");
   if (num_keys <= 0 || !keys) return fail(MGB_ERROR_LOGIC, "no key columns");
   const int64_t R = keys[0].size;
   for (int c = 1; c < num_keys; ++c)
@@ -2405,12 +2552,16 @@ extern "C" mgb_status mgb_estimate_groups(const mgb_column* keys, int32_t num_ke
     return MGB_SUCCESS;
   }
   hipStream_t stream = static_cast<hipStream_t>(stream_v);
-  Workspace* ws = workspace_for(stream);
-  std::lock_guard<std::mutex> lk(ws->mu);
-  if (!ws->pinned) {
-    MGB_HIP(hipHostMalloc((void**)&ws->pinned, kMaxPinned * sizeof(unsigned long long), hipHostMallocDefault));
-    MGB_HIP(hipHostMalloc((void**)&ws->staging, kStagingBytes, hipHostMallocDefault));
-  }
+  HostStaging* hs = nullptr;
+  std::unique_lock<std::mutex> lk;
+  st = lock_staging(stream, hs, lk);
+  if (st != MGB_SUCCESS) return st;
+  Workspace call;
+  Workspace* ws = &call;
+  call.mem.alloc = temps;
+  call.mem.stream = stream;
+  call.pinned = hs->pinned;
+  call.staging = hs->staging;
   AggParams base{};
   base.num_rows = R;
   base.key_mode = plan.key_mode;
@@ -2494,7 +2645,7 @@ extern "C" mgb_status mgb_release_workspace(void* stream)
   g_last_error.clear();
   int dev = 0;
   (void)hipGetDevice(&dev);
-  std::unique_ptr<Workspace> w;
+  std::unique_ptr<HostStaging> w;
   {
     std::lock_guard<std::mutex> lk(g_ws_mu);
     auto it = g_ws.find({dev, static_cast<hipStream_t>(stream)});
@@ -2502,6 +2653,9 @@ extern "C" mgb_status mgb_release_workspace(void* stream)
     w = std::move(it->second);
     g_ws.erase(it);
   }
+  // a call still running on the stream holds the staging's lock: wait for it
+  // before the staging goes, and for the stream's copies from it
+  std::lock_guard<std::mutex> lk(w->mu);
   MGB_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   return MGB_SUCCESS;
 }

@@ -70,29 +70,75 @@ struct ProfScope {
   }
 };
 
-// ------------------------------------------------------------ workspace
+// ------------------------------------------------------------ call memory
 
+inline size_t align_up(size_t x, size_t a);
+
+// Device memory of one call. Every temporary comes from the caller's
+// allocator for temporaries (the role of cudf's current device resource,
+// cpp/src/groupby/hash/groupby.cu:404,478,484) and goes back to it,
+// stream-ordered on the call's stream, when the call ends; or, for the
+// entry point that allocates nothing, it is carved from a fixed region
+// (bump allocation; the block on top can grow in place).
+struct CallMem {
+  const mgb_allocator* alloc = nullptr;
+  uint8_t* region = nullptr;
+  size_t region_bytes = 0, top = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::pair<void*, size_t>> live;
+  size_t held = 0, peak = 0;  // bytes of the live temporaries, and their peak
+
+  void* get(size_t bytes)
+  {
+    bytes = align_up(std::max<size_t>(bytes, 1), 256);
+    void* p = nullptr;
+    if (region) {
+      if (top + bytes > region_bytes) return nullptr;
+      p = region + top;
+      top += bytes;
+    } else {
+      p = alloc->allocate(alloc->ctx, bytes, (void*)stream);
+      if (!p) return nullptr;
+    }
+    live.push_back({p, bytes});
+    held += bytes;
+    peak = std::max(peak, held);
+    return p;
+  }
+  void put(void* p)
+  {
+    for (size_t i = 0; i < live.size(); ++i) {
+      if (live[i].first != p) continue;
+      const size_t bytes = live[i].second;
+      if (region) {
+        if (static_cast<uint8_t*>(p) + bytes == region + top) top -= bytes;  // the top block: reused at once
+      } else {
+        alloc->deallocate(alloc->ctx, p, bytes, (void*)stream);
+      }
+      held -= bytes;
+      live.erase(live.begin() + (std::ptrdiff_t)i);
+      return;
+    }
+  }
+  void release_all()
+  {
+    while (!live.empty()) put(live.back().first);
+  }
+  ~CallMem() { release_all(); }
+};
+
+// A temporary that grows on demand (its contents are not kept when it grows).
 struct DeviceBuffer {
+  CallMem* mem = nullptr;
   void* ptr = nullptr;
   size_t bytes = 0;
-  hipError_t reserve(size_t need, hipStream_t stream)
+  hipError_t reserve(size_t need, hipStream_t)
   {
     if (need <= bytes) return hipSuccess;
-    if (ptr) {
-      hipError_t e = hipStreamSynchronize(stream);
-      if (e != hipSuccess) return e;
-      (void)hipFree(ptr);
-      ptr = nullptr;
-      bytes = 0;
-    }
-    size_t want = std::max(need, bytes + bytes / 4);
-    hipError_t e = hipMalloc(&ptr, want);
-    if (e != hipSuccess) {
-      ptr = nullptr;
-      return e;
-    }
-    bytes = want;
-    return hipSuccess;
+    if (ptr) mem->put(ptr);
+    ptr = mem->get(need);
+    bytes = ptr ? align_up(need, 256) : 0;
+    return ptr ? hipSuccess : hipErrorOutOfMemory;
   }
 };
 

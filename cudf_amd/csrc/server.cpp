@@ -212,7 +212,21 @@ struct Slot {
   std::shared_ptr<HeldBatch> batch;
   std::vector<Query> queries;
   struct mgb_server* server = nullptr;
+  // temporaries of the plain-path calls run on this slot's stream: carved
+  // from the server's arena (so they count against the HBM budget) and
+  // returned to it only after the slot has waited for its stream
+  Arena* arena = nullptr;
+  std::vector<void*> deferred;
+  mgb_allocator temps{};
+  void flush_deferred()
+  {
+    for (void* p : deferred) arena->deallocate(p);
+    deferred.clear();
+  }
 };
+
+void* slot_temp_alloc(void* ctx, size_t bytes, void*) { return static_cast<Slot*>(ctx)->arena->allocate(bytes); }
+void slot_temp_free(void* ctx, void* p, size_t, void*) { static_cast<Slot*>(ctx)->deferred.push_back(p); }
 
 // Released after the query's work and its copies have completed.
 void free_out(Arena* arena, std::vector<mgb_out_column>& cols)
@@ -366,12 +380,28 @@ bool mgb_server::launch(Slot& s)
     Done d;
     d.keys.assign(keys.size(), mgb_out_column{});
     d.results.assign(nout, mgb_out_column{});
+    mgb_allocator call_alloc = alloc;
+    call_alloc.temporaries = &s.temps;
     d.status = mgb_groupby_aggregate(keys.data(), (int32_t)keys.size(), q.null_policy, reqs.data(),
-                                     (int32_t)reqs.size(), d.keys.data(), d.results.data(), &d.groups, s.stream, &alloc);
+                                     (int32_t)reqs.size(), d.keys.data(), d.results.data(), &d.groups, s.stream,
+                                     &call_alloc);
     // the call returns with its last writes still queued on the stream: the
-    // result is complete (and may be read from any stream) after this wait
-    if (d.status == MGB_SUCCESS && hipStreamSynchronize(s.stream) != hipSuccess)
+    // result is complete (and may be read from any stream) after this wait,
+    // and the call's temporaries may go back to the arena
+    if (hipStreamSynchronize(s.stream) != hipSuccess && d.status == MGB_SUCCESS)
       d.status = fail(MGB_ERROR_HIP, "stream sync failed");
+    s.flush_deferred();
+    if (d.status == MGB_ERROR_ALLOC) {
+      // no memory for its temporaries now: wait for batches to release theirs
+      // unless nothing else holds any
+      std::lock_guard<std::mutex> lk(mu);
+      if (arena->in_use()) {
+        ++stats.deferred;
+        pending.push_front(std::move(q));
+        wait_memory = true;
+        return false;
+      }
+    }
     if (d.status != MGB_SUCCESS) d.error = mgb_last_error_message();
     d.latency_ms = ms_since(q.arrival, Clock::now());
     for (auto& c : d.keys) d.bytes += (int64_t)(c.data_bytes + c.mask_bytes);
@@ -410,8 +440,10 @@ bool mgb_server::launch(Slot& s)
       if (it == estimates.end()) {
         int64_t e = 0;
         if (mgb_estimate_groups(take[q].keys.data(), (int32_t)take[q].keys.size(), take[q].null_policy, s.stream,
-                                &e) != MGB_SUCCESS)
+                                &s.temps, &e) != MGB_SUCCESS)
           e = take[q].keys.empty() ? 1 : take[q].keys[0].size;
+        (void)hipStreamSynchronize(s.stream);
+        s.flush_deferred();
         it = estimates.emplace(take[q].group_key, e).first;
       }
       est[q] = it->second;
@@ -553,6 +585,8 @@ extern "C" mgb_status mgb_server_create(const mgb_server_options* options, mgb_s
   for (int i = 0; i < s->opt.max_inflight; ++i) {
     auto slot = std::make_unique<Slot>();
     slot->server = s.get();
+    slot->arena = s->arena.get();
+    slot->temps = mgb_allocator{slot_temp_alloc, slot_temp_free, slot.get(), nullptr, nullptr};
     MGB_HIP(hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking));
     s->slots.push_back(std::move(slot));
   }
@@ -802,6 +836,9 @@ extern "C" void mgb_server_destroy(mgb_server* s)
   for (auto& slot : s->slots) {
     if (slot->stream) {
       (void)hipStreamSynchronize(slot->stream);
+      slot->flush_deferred();
+      // the host staging the plain path keeps for the stream goes with it
+      (void)mgb_release_workspace(slot->stream);
       (void)hipStreamDestroy(slot->stream);
     }
   }

@@ -13,6 +13,7 @@ import torch
 from . import _lib
 from .aggregation import Aggregation
 from .column import Column, Table
+from .pool import DevicePool, default_pool
 from .types import NullPolicy, Sorted
 
 
@@ -90,10 +91,12 @@ class GroupBy:
     """Group values by keys and compute aggregations (cudf::groupby::groupby)."""
 
     def __init__(self, keys: Table, null_handling: NullPolicy = NullPolicy.EXCLUDE,
-                 keys_are_sorted: Sorted = Sorted.NO):
+                 keys_are_sorted: Sorted = Sorted.NO, temporaries: "DevicePool" = None):
         self.keys = keys
         self.null_handling = NullPolicy(null_handling)
         self.keys_are_sorted = Sorted(keys_are_sorted)
+        # the pool of every device temporary of a call (None: the device's default pool)
+        self.temporaries = temporaries
 
     def aggregate(self, requests: List[GroupByRequest], stream: torch.cuda.Stream = None):
         lib = _lib.load()
@@ -123,6 +126,8 @@ class GroupBy:
         out_res = (_lib.mgb_out_column * max(n_out, 1))()
         groups = ctypes.c_int64(0)
         alloc = _TorchAllocator(dev)
+        pool = self.temporaries if self.temporaries is not None else default_pool(dev)
+        alloc.struct.temporaries = ctypes.pointer(pool.allocator)
         st = lib.mgb_groupby_aggregate(keys_arr, nk, int(self.null_handling), reqs, len(requests),
                                        out_keys, out_res, ctypes.byref(groups),
                                        ctypes.c_void_p(stream.cuda_stream), ctypes.byref(alloc.struct))

@@ -101,18 +101,29 @@ typedef struct mgb_request {
   int32_t reserved;
 } mgb_request;
 
-/* Device allocator for the returned buffers (the role of the `mr` argument of
- * cudf::groupby::groupby::aggregate). Buffers come back owned by the caller,
- * who releases them with `deallocate`, one buffer at a time.
+/* Device allocator (the role of the `mr` argument of
+ * cudf::groupby::groupby::aggregate, and of cudf's current device resource
+ * for temporaries). Result buffers come back owned by the caller, who
+ * releases them with `deallocate`, one buffer at a time.
  * `allocate_many` (optional, may be NULL; no counterpart in the reference) is
  * asked for all result buffers of one call at once: n sizes in, n pointers out,
  * 0 on success. A caller whose allocator is costly per call (an interpreter
- * callback) sets it; each buffer is still released by its own `deallocate`. */
+ * callback) sets it; each buffer is still released by its own `deallocate`.
+ * `temporaries` (optional, may be NULL: this allocator) serves every device
+ * temporary of a call (hash tables, partitioned rows, histograms, staged
+ * results, sort space): each is allocated during the call and returned with
+ * `deallocate` on the call's stream before the call returns. The return is
+ * stream-ordered, as rmm's resources are: work already queued on that stream
+ * may still use the buffer, so the allocator hands it to other streams only
+ * after that work (an allocator that frees at once must wait for the stream
+ * first). Nothing else of a call touches device memory the caller did not
+ * provide. */
 typedef struct mgb_allocator {
   void* (*allocate)(void* ctx, size_t bytes, void* stream);
   void (*deallocate)(void* ctx, void* ptr, size_t bytes, void* stream);
   void* ctx;
   int32_t (*allocate_many)(void* ctx, int32_t n, const size_t* bytes, void** out, void* stream);
+  const struct mgb_allocator* temporaries;
 } mgb_allocator;
 
 /* An owned result column written by the library. */
@@ -176,15 +187,16 @@ mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_keys, int32
  * (cpp/include/cudf/hashing.hpp:90-94, cpp/src/hash/murmurhash3_x86_32.cu).
  * `out` is a device array of num_rows uint32.
  */
-/* Estimated number of distinct key rows (groups) of a key table: exact up
- * to 131072 rows, otherwise from a sample, or HyperLogLog over every row when
- * the sample does not settle it (the plain path's own estimate). Synchronises
- * `stream`. No counterpart in the reference. */
-mgb_status mgb_estimate_groups(const mgb_column* keys, int32_t num_keys, int32_t null_policy, void* stream,
-                               int64_t* estimate);
-
 mgb_status mgb_murmurhash3_x86_32(const mgb_column* columns, int32_t num_columns, uint32_t seed,
                                   uint32_t* out, void* stream);
+
+/* Estimated number of distinct key rows (groups) of a key table: exact up
+ * to 131072 rows, otherwise from a sample, or HyperLogLog over every row when
+ * the sample does not settle it (the plain path's own estimate). Its
+ * temporaries (about 40 MB at most) come from `alloc` (its `temporaries` if
+ * set). Synchronises `stream`. No counterpart in the reference. */
+mgb_status mgb_estimate_groups(const mgb_column* keys, int32_t num_keys, int32_t null_policy, void* stream,
+                               const mgb_allocator* alloc, int64_t* estimate);
 
 /*
  * Spark-compatible row hash: columns hashed in order with each result seeding
@@ -206,9 +218,40 @@ mgb_status mgb_spark_murmurhash3_x86_32(const mgb_column* columns, int32_t num_c
 mgb_status mgb_profiling_enable(int32_t enable);
 mgb_status mgb_profiling_read(const char* kernel, int64_t* launches, double* total_ms);
 
-/* Releases the device workspace cached for `stream` (the library keeps one per
- * stream so that repeated calls do not allocate). */
+/* Releases the pinned host staging the library keeps for `stream` (readback
+ * slots and parameter uploads, about 72 KB, taken at the stream's first call).
+ * Call after the stream's last groupby call, e.g. before destroying it. */
 mgb_status mgb_release_workspace(void* stream);
+
+/* ------------------------------------------------------------------------
+ * Device memory pool (no counterpart in cudf; the role of rmm's
+ * pool_memory_resource). Chunks are taken from the device as needed, never
+ * more than `limit_bytes` in all, and kept until the pool is trimmed or
+ * destroyed; blocks are carved first-fit. A block returned on a stream is
+ * reused at once by that stream and by other streams only after the pool has
+ * waited for it (which it does only when it would otherwise pass its limit).
+ * mgb_pool_allocator fills an mgb_allocator over the pool, e.g. as the
+ * `temporaries` of a groupby call's allocator.
+ * ------------------------------------------------------------------------ */
+typedef struct mgb_pool mgb_pool;
+
+typedef struct mgb_pool_stats {
+  int64_t limit_bytes;
+  int64_t held_bytes;         /* chunks taken from the device */
+  int64_t in_use_bytes;       /* blocks handed out and not returned */
+  int64_t peak_in_use_bytes;
+  int64_t chunks;
+  int64_t stream_waits;       /* streams waited for to reuse their blocks elsewhere */
+} mgb_pool_stats;
+
+mgb_status mgb_pool_create(int64_t limit_bytes, mgb_pool** out);
+mgb_status mgb_pool_allocator(mgb_pool* pool, mgb_allocator* out);
+mgb_status mgb_pool_get_stats(mgb_pool* pool, mgb_pool_stats* out);
+/* Waits for the streams that returned blocks and frees the chunks no block uses. */
+mgb_status mgb_pool_trim(mgb_pool* pool);
+/* Waits for the streams that returned blocks, then frees every chunk. Every
+ * block must have been returned. */
+void mgb_pool_destroy(mgb_pool* pool);
 
 
 /* ------------------------------------------------------------------------

@@ -5,6 +5,7 @@
 #include <rmm/cuda_stream_view.hpp>
 
 #include <cstddef>
+#include <cstdint>
 #include <new>
 #include <string>
 
@@ -43,8 +44,21 @@ class device_memory_resource {
   }
 };
 
-// Device memory from hipMallocAsync / hipFreeAsync on the given stream.
+// Device memory from hipMallocAsync / hipFreeAsync on the given stream (the
+// device's default memory pool, which keeps freed memory for reuse instead
+// of returning it to the device at every synchronisation).
 class cuda_memory_resource final : public device_memory_resource {
+ public:
+  cuda_memory_resource()
+  {
+    int dev = 0;
+    hipMemPool_t pool = nullptr;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+      uint64_t keep = ~0ull;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+  }
+
  private:
   void* do_allocate(std::size_t bytes, cuda_stream_view stream) override
   {

@@ -105,7 +105,9 @@ def test_more_rows_than_size_type_refused_before_device_work(rows):
     lib = _lib.load()
     est = ctypes.c_int64(-1)
     key = (_lib.mgb_column * 1)(_col(4, rows))
-    assert lib.mgb_estimate_groups(key, 1, 0, None, ctypes.byref(est)) == _lib.MGB_ERROR_UNSUPPORTED
+    alloc = _lib.mgb_allocator(_lib.ALLOC_FN(lambda c, n, s: None), _lib.DEALLOC_FN(lambda c, p, n, s: None), None)
+    assert lib.mgb_estimate_groups(key, 1, 0, None, ctypes.byref(alloc), ctypes.byref(est)) == \
+        _lib.MGB_ERROR_UNSUPPORTED
 
 
 def test_null_arguments_rejected():

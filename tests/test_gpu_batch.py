@@ -145,11 +145,13 @@ def test_batch_with_given_estimates(gpu):
     from cudf_amd import _lib
     host, dev = _batch_inputs(19)
     lib = _lib.load()
+    pool = ca.DevicePool(1 << 30)
     ests = []
     for kt, _ in dev:
         arr = (_lib.mgb_column * 1)(ca.groupby._to_mgb(kt.columns[0]))
         e = ctypes.c_int64()
-        _lib.check(lib.mgb_estimate_groups(arr, 1, 0, ctypes.c_void_p(0), ctypes.byref(e)))
+        _lib.check(lib.mgb_estimate_groups(arr, 1, 0, ctypes.c_void_p(0), ctypes.byref(pool.allocator),
+                                           ctypes.byref(e)))
         ests.append(e.value)
     for (keys, _), e in zip(host, ests):
         exact = len(np.unique(keys[0][1]))
