@@ -2471,16 +2471,15 @@ extern "C" const char* mgb_last_error_message(void) { return g_last_error.c_str(
 
 extern "C" int32_t mgb_target_type(int32_t value_type_id, int32_t agg_kind) { return target_type(value_type_id, agg_kind); }
 
-extern "C" mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_keys, int32_t null_policy,
-                                            const mgb_request* requests, int32_t num_requests,
-                                            mgb_out_column* out_keys, mgb_out_column* out_results,
-                                            int64_t* out_num_groups, void* stream, const mgb_allocator* alloc)
+namespace mgb {
+
+// Argument checks shared by the groupby entry points (groupby.cu:173-216).
+mgb_status check_call(const mgb_column* keys, int32_t num_keys, int32_t null_policy, const mgb_request* requests,
+                      int32_t num_requests, int64_t& R)
 {
-  g_last_error.clear();
-  if (num_keys < 0 || num_requests < 0 || (num_keys > 0 && !keys) || (num_requests > 0 && !requests) ||
-      !out_num_groups || !alloc || !alloc->allocate || !alloc->deallocate)
+  if (num_keys < 0 || num_requests < 0 || (num_keys > 0 && !keys) || (num_requests > 0 && !requests))
     return fail(MGB_ERROR_LOGIC, "null argument");
-  const int64_t R = num_keys > 0 ? keys[0].size : 0;
+  R = num_keys > 0 ? keys[0].size : 0;
   for (int c = 0; c < num_keys; ++c)
     if (keys[c].size != R) return fail(MGB_ERROR_LOGIC, "key columns must have equal sizes");
   for (int r = 0; r < num_requests; ++r)
@@ -2495,24 +2494,50 @@ extern "C" mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_
         return fail(MGB_ERROR_LOGIC, "Invalid type/aggregation combination.");
   if (null_policy != MGB_NULL_EXCLUDE && null_policy != MGB_NULL_INCLUDE)
     return fail(MGB_ERROR_LOGIC, "invalid null_policy");
+  return MGB_SUCCESS;
+}
 
-  Plan plan;
-  if (R == 0) {
-    // empty input: empty typed columns (groupby.cu:147-170,216)
-    *out_num_groups = 0;
-    for (int c = 0; c < num_keys; ++c) {
-      std::memset(&out_keys[c], 0, sizeof(mgb_out_column));
-      out_keys[c].type_id = keys[c].type_id;
+// Empty input: empty typed columns (groupby.cu:147-170,216). Buffers the
+// caller provided stay the caller's (`keep_buffers`).
+void empty_result(const mgb_column* keys, int32_t num_keys, const mgb_request* requests, int32_t num_requests,
+                  mgb_out_column* out_keys, mgb_out_column* out_results, bool keep_buffers)
+{
+  auto reset = [&](mgb_out_column& o, int type_id) {
+    if (keep_buffers) {
+      o.size = 0;
+      o.null_count = 0;
+      o.null_mask = nullptr;
+    } else {
+      std::memset(&o, 0, sizeof(mgb_out_column));
     }
-    int j = 0;
-    for (int r = 0; r < num_requests; ++r)
-      for (int a = 0; a < requests[r].num_aggs; ++a, ++j) {
-        std::memset(&out_results[j], 0, sizeof(mgb_out_column));
-        out_results[j].type_id = target_type(requests[r].values.type_id, requests[r].aggs[a].kind);
-      }
+    o.type_id = type_id;
+  };
+  for (int c = 0; c < num_keys; ++c) reset(out_keys[c], keys[c].type_id);
+  int j = 0;
+  for (int r = 0; r < num_requests; ++r)
+    for (int a = 0; a < requests[r].num_aggs; ++a, ++j)
+      reset(out_results[j], target_type(requests[r].values.type_id, requests[r].aggs[a].kind));
+}
+
+}  // namespace mgb
+
+extern "C" mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_keys, int32_t null_policy,
+                                            const mgb_request* requests, int32_t num_requests,
+                                            mgb_out_column* out_keys, mgb_out_column* out_results,
+                                            int64_t* out_num_groups, void* stream, const mgb_allocator* alloc)
+{
+  g_last_error.clear();
+  if (!out_num_groups || !alloc || !alloc->allocate || !alloc->deallocate) return fail(MGB_ERROR_LOGIC, "null argument");
+  int64_t R = 0;
+  mgb_status st = check_call(keys, num_keys, null_policy, requests, num_requests, R);
+  if (st != MGB_SUCCESS) return st;
+  if (R == 0) {
+    *out_num_groups = 0;
+    empty_result(keys, num_keys, requests, num_requests, out_keys, out_results, false);
     return MGB_SUCCESS;
   }
-  mgb_status st = build_plan(keys, num_keys, null_policy, requests, num_requests, plan);
+  Plan plan;
+  st = build_plan(keys, num_keys, null_policy, requests, num_requests, plan);
   if (st != MGB_SUCCESS) return st;
   const mgb_allocator* temps = alloc->temporaries ? alloc->temporaries : alloc;
   if (!temps->allocate || !temps->deallocate) return fail(MGB_ERROR_LOGIC, "allocator of temporaries without callbacks");
