@@ -136,6 +136,15 @@ SIGNATURES = {
             ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.POINTER(mgb_allocator),
         ],
     ),
+    "mgb_groupby_aggregate_noalloc": (
+        ctypes.c_int,
+        [
+            ctypes.POINTER(mgb_column), ctypes.c_int32, ctypes.c_int32,
+            ctypes.POINTER(mgb_request), ctypes.c_int32,
+            ctypes.POINTER(mgb_out_column), ctypes.POINTER(mgb_out_column),
+            ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p,
+        ],
+    ),
     "mgb_estimate_groups": (
         ctypes.c_int,
         [ctypes.POINTER(mgb_column), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(mgb_allocator),

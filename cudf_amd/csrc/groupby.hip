@@ -878,6 +878,7 @@ struct Workspace {
 
 constexpr int kMaxPinned = 1024;  // host readback slots (counters, null counts)
 constexpr int kPinnedNulls = 128; // first slot of null counts read back with the group count
+constexpr int64_t kNoAllocMaxRows = 1 << 23;  // rows of one mgb_groupby_aggregate_noalloc call
 
 int device_cu_count()
 {
@@ -2554,6 +2555,88 @@ extern "C" mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_
   OutputSink sink;
   sink.alloc = alloc;
   return run_groupby(plan, R, &ws, out_keys, out_results, out_num_groups, s, sink);
+}
+
+// ---- the entry point that allocates nothing
+
+namespace mgb {
+
+// Device memory of mgb_groupby_aggregate_noalloc: every temporary of such a
+// call is carved from this array (one copy on each device the library runs
+// on, held until the process ends). In host code `scratch` only names the
+// symbol; its device address comes from hipGetSymbolAddress, per device.
+constexpr size_t kScratchBytes = 256ull << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;                  // one allocation-free call at a time in the process
+std::map<int, uint8_t*> g_scratch_addr;   // device -> address of its `scratch`
+unsigned long long g_noalloc_readback[kMaxPinned];  // host readback of those calls (pageable)
+uint8_t g_noalloc_staging[kStagingBytes];           // host source of their parameter uploads
+
+// The address of the current device's `scratch`, looked up at the device's
+// first call (with that device current) and kept.
+mgb_status scratch_address(uint8_t*& out)
+{
+  int dev = 0;
+  MGB_HIP(hipGetDevice(&dev));
+  auto it = g_scratch_addr.find(dev);
+  if (it != g_scratch_addr.end()) {
+    out = it->second;
+    return MGB_SUCCESS;
+  }
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(MGB_ERROR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_addr[dev] = out = static_cast<uint8_t*>(p);
+  return MGB_SUCCESS;
+}
+
+}  // namespace mgb
+
+extern "C" mgb_status mgb_groupby_aggregate_noalloc(const mgb_column* keys, int32_t num_keys, int32_t null_policy,
+                                                    const mgb_request* requests, int32_t num_requests,
+                                                    mgb_out_column* out_keys, mgb_out_column* out_results,
+                                                    int64_t max_groups, int64_t* out_num_groups, void* stream)
+{
+  g_last_error.clear();
+  if (!out_num_groups || max_groups < 0 || (num_keys > 0 && !out_keys)) return fail(MGB_ERROR_LOGIC, "null argument");
+  int64_t R = 0;
+  mgb_status st = check_call(keys, num_keys, null_policy, requests, num_requests, R);
+  if (st != MGB_SUCCESS) return st;
+  if (R > (int64_t)kNoAllocMaxRows)
+    return fail(MGB_ERROR_UNSUPPORTED, "mgb_groupby_aggregate_noalloc takes at most " +
+                                           std::to_string(kNoAllocMaxRows) + " rows (its scratch memory is " +
+                                           std::to_string(kScratchBytes >> 20) + " MiB)");
+  if (R == 0) {
+    *out_num_groups = 0;
+    empty_result(keys, num_keys, requests, num_requests, out_keys, out_results, true);
+    return MGB_SUCCESS;
+  }
+  Plan plan;
+  st = build_plan(keys, num_keys, null_policy, requests, num_requests, plan);
+  if (st != MGB_SUCCESS) return st;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  uint8_t* region = nullptr;
+  st = scratch_address(region);
+  if (st != MGB_SUCCESS) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Workspace ws;
+  ws.mem.region = region;
+  ws.mem.region_bytes = kScratchBytes;
+  ws.mem.stream = s;
+  ws.pinned = g_noalloc_readback;
+  ws.staging = g_noalloc_staging;
+  OutputSink sink;
+  sink.capacity = max_groups;
+  st = run_groupby(plan, R, &ws, out_keys, out_results, out_num_groups, s, sink);
+  if (st == MGB_ERROR_ALLOC && ws.mem.region && g_last_error.find("temporaries") != std::string::npos)
+    return fail(MGB_ERROR_ALLOC, g_last_error + " (beyond the " + std::to_string(kScratchBytes >> 20) +
+                                     " MiB scratch memory of mgb_groupby_aggregate_noalloc)");
+  // the scratch is reused by the next call: its work must be complete
+  // before another call may start (the copies into the caller's columns)
+  if (st == MGB_SUCCESS) MGB_HIP(hipStreamSynchronize(s));
+  return st;
 }
 
 extern "C" mgb_status mgb_estimate_groups(const mgb_column* keys, int32_t num_keys, int32_t null_policy, void* stream_v,

@@ -187,6 +187,34 @@ mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_keys, int32
  * (cpp/include/cudf/hashing.hpp:90-94, cpp/src/hash/murmurhash3_x86_32.cu).
  * `out` is a device array of num_rows uint32.
  */
+/*
+ * mgb_groupby_aggregate for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). No counterpart in the reference. Same inputs, semantics and
+ * errors as mgb_groupby_aggregate, but the call allocates nothing: its
+ * temporaries come from a fixed 256 MiB scratch array the library holds on
+ * each device it runs on (for the life of the process), and the results go
+ * into buffers the caller provides:
+ *   out_keys[c] / out_results[j]   data: a device buffer of data_bytes bytes
+ *                                  holding max_groups values of the column's
+ *                                  type (mgb_target_type for results);
+ *                                  null_mask: a device buffer of mask_bytes
+ *                                  bytes holding max_groups bits (rounded up
+ *                                  to 64 bytes), needed where the column can
+ *                                  hold nulls and set to NULL on return where
+ *                                  it has none. type_id, size and null_count
+ *                                  are written by the call.
+ * Inputs above 8,388,608 rows are refused (MGB_ERROR_UNSUPPORTED) before any
+ * device work; a call whose temporaries would pass the scratch array, or
+ * whose group count passes max_groups, fails with MGB_ERROR_ALLOC. Calls of
+ * one process run one at a time (they share the scratch array) and return
+ * with their work on `stream` complete.
+ */
+mgb_status mgb_groupby_aggregate_noalloc(const mgb_column* keys, int32_t num_keys, int32_t null_policy,
+                                         const mgb_request* requests, int32_t num_requests,
+                                         mgb_out_column* out_keys, mgb_out_column* out_results,
+                                         int64_t max_groups, int64_t* out_num_groups, void* stream);
+
 mgb_status mgb_murmurhash3_x86_32(const mgb_column* columns, int32_t num_columns, uint32_t seed,
                                   uint32_t* out, void* stream);
 
