@@ -8,10 +8,12 @@ cudf::groupby::groupby::aggregate call through the C ABI on device-resident
 inputs (output allocation included).
 
 Prints one JSON line. `value` is rows/s over the timed steps; `roofline`
-prices the dominant kernel (aggregate) with its algorithmic bytes (the input
-columns read once) over its own average duration, measured with HIP events on
-its launch stream; `cpu_baseline` times pandas (the cudf.pandas CPU fallback)
-on a bounded sample of the same data on this host.
+prices the whole call (input columns read once and results written once, over
+the measured time per step) and, in `dominant_kernel`, the longest kernel with
+its algorithmic bytes over its own average duration, measured with HIP events
+on its launch stream in a separate pass; `cpu_baseline` times pandas (what
+the reference runs on the CPU: the cudf.pandas fallback) on a bounded sample of
+the same data on this host, with the oracle's C port beside it.
 """
 from __future__ import annotations
 
@@ -482,31 +484,45 @@ def oracle_sample(name: str, sample_rows: int, distinct: int, seed: int):
     return keys, reqs
 
 
-def cpu_baseline_oracle(name: str, sample_rows: int, distinct: int, seed: int, pandas_rows: int):
-    """The oracle (the C restatement of the reference's hash groupby,
-    oracle/groupby_oracle.c, one thread) timed on a bounded sample of the
-    workload, with the reference's own CPU path (pandas, the cudf.pandas
-    fallback) timed beside it on a smaller sample."""
+def cpu_baseline(name: str, pandas_rows: int, oracle_rows: int, distinct: int, seed: int):
+    """The reported CPU baseline: pandas' groupby-agg, which is what the
+    reference runs on the CPU (the cudf.pandas fallback executes pandas,
+    python/cudf/cudf/pandas/module_accelerator.py:79-92), on a bounded sample
+    drawn like the workload: default sort, 2 warm-ups, median of 7 runs, one
+    core. Beside it, the oracle (the C restatement of the reference's hash
+    groupby, oracle/groupby_oracle.c, one thread) on its own sample."""
     from oracle import oracle as O
 
+    if name in ("c2", "c2v", "c2s"):
+        aggs = {"c2": ("sum", "count", "min", "max", "mean"), "c2v": ("mean", "var", "std", "idxmin", "idxmax"),
+                "c2s": ("median", "nunique", "first")}[name]
+        pd_line = cpu_baseline_c2(pandas_rows, distinct, seed, aggs=aggs)
+    else:
+        pd_line = cpu_baseline_generic(name, pandas_rows, seed)
     O.load()
-    keys, reqs = oracle_sample(name, sample_rows, distinct, seed)
+    keys, reqs = oracle_sample(name, oracle_rows, distinct, seed)
     t0 = time.perf_counter()
     O.groupby(keys, reqs)
     t = time.perf_counter() - t0
-    pd_line = (cpu_baseline_c2(pandas_rows, distinct, seed) if name == "c2"
-               else cpu_baseline_c2(pandas_rows, distinct, seed, aggs=["mean", "var", "std", "idxmin", "idxmax"])
-               if name == "c2v"
-               else cpu_baseline_c2(pandas_rows, distinct, seed, aggs=["median", "nunique", "first"])
-               if name == "c2s" else cpu_baseline_generic(name, pandas_rows, seed))
-    return {"value": sample_rows / t, "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": f"the oracle's C groupby (oracle/groupby_oracle.c, one thread) on {sample_rows:,} rows drawn "
-                      f"like the workload, one call of {t:.1f} s; host os.cpu_count()={os.cpu_count()}",
-            "reference_cpu_path": {"value": pd_line["value"], "unit": "rows/s", "cores": 1,
-                                   "sample": pd_line["sample"]}}
+    pd_line["oracle_port"] = {
+        "value": oracle_rows / t, "unit": "rows/s", "cores": 1, "kind": "port",
+        "sample": f"the oracle's C groupby (oracle/groupby_oracle.c, one thread) on {oracle_rows:,} rows drawn like "
+                  f"the workload, one call of {t:.1f} s"}
+    return pd_line
 
 
-def cpu_baseline_generic(name: str, sample_rows: int, seed: int, reps: int = 3):
+def _time_pandas(run, warmups: int = 2, reps: int = 7) -> float:
+    for _ in range(warmups):
+        run()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times)
+
+
+def cpu_baseline_generic(name: str, sample_rows: int, seed: int):
     """pandas groupby-agg on a bounded sample drawn like the workload."""
     import numpy as np
     import pandas as pd
@@ -514,7 +530,8 @@ def cpu_baseline_generic(name: str, sample_rows: int, seed: int, reps: int = 3):
     rng = np.random.default_rng(seed)
     if name == "c1":
         df = pd.DataFrame({"key": rng.integers(0, 1000, sample_rows).astype(np.int32), "val": rng.random(sample_rows)})
-        run = lambda: df.groupby("key", sort=False)["val"].agg(["sum", "count", "mean", "min", "max"])
+        run = lambda: df.groupby("key")["val"].agg(["sum", "count", "mean", "min", "max"])
+        what = "int32 key in [0, 1000), float64 U[0, 1); sum/count/mean/min/max"
     elif name == "c3a":
         n_keys = 50_000_000
         u = rng.random(sample_rows)
@@ -522,25 +539,24 @@ def cpu_baseline_generic(name: str, sample_rows: int, seed: int, reps: int = 3):
         df = pd.DataFrame({"a": (r * 2654435761 % 2 ** 31).astype(np.int32),
                            "b": (r.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15) % np.uint64(2 ** 61)).astype(np.int64),
                            "val": rng.standard_normal(sample_rows)})
-        run = lambda: df.groupby(["a", "b"], sort=False)["val"].agg(["sum", "count", "max"])
+        run = lambda: df.groupby(["a", "b"])["val"].agg(["sum", "count", "max"])
+        what = "(int32, int64) Zipf s=1.1 keys, float64 value; sum/count/max"
     else:  # c3b
         uniq = rng.integers(-2 ** 31, 2 ** 31 - 1, 16).astype(np.int32)
         df = pd.DataFrame({"key": uniq[rng.integers(0, 16, sample_rows)], "val": rng.standard_normal(sample_rows)})
-        run = lambda: df.groupby("key", sort=False)["val"].agg(["sum", "count", "max"])
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        run()
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
-    return {"value": sample_rows / t, "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": f"pandas {pd.__version__} groupby-agg on {sample_rows:,} rows drawn like the workload, "
-                      f"median of {reps}, single-threaded; host os.cpu_count()={os.cpu_count()}"}
+        run = lambda: df.groupby("key")["val"].agg(["sum", "count", "max"])
+        what = "int32 key over 16 values, float64 value; sum/count/max"
+    t = _time_pandas(run)
+    return {"value": sample_rows / t, "unit": "rows/s", "cores": 1, "kind": "reference",
+            "sample": f"pandas {pd.__version__} groupby-agg (the reference's CPU path: cudf.pandas falls back to "
+                      f"pandas) on {sample_rows:,} rows drawn like the workload ({what}), default sort, 2 warm-ups, "
+                      f"median of 7, single-threaded; host os.cpu_count()={os.cpu_count()}"}
 
 
-def cpu_baseline_c2(sample_rows: int, distinct: int, seed: int, reps: int = 3,
-                    aggs=("sum", "count", "min", "max", "mean")):
-    """pandas groupby-agg on a bounded sample of the config-2 distribution."""
+def cpu_baseline_c2(sample_rows: int, distinct: int, seed: int, aggs=("sum", "count", "min", "max", "mean")):
+    """pandas groupby-agg on a bounded sample of the 100M-row workload's
+    distribution (nullable columns; int32 values summed as Int64, as cudf
+    widens them)."""
     import numpy as np
     import pandas as pd
 
@@ -556,21 +572,16 @@ def cpu_baseline_c2(sample_rows: int, distinct: int, seed: int, reps: int = 3,
     for c in ("i32", "f64", "f32"):
         df.loc[rng.random(sample_rows) < 0.05, c] = pd.NA
     aggs = list(aggs)
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        df.groupby("key", sort=False).agg({"i32": aggs, "f64": aggs, "f32": aggs})
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
+    t = _time_pandas(lambda: df.groupby("key").agg({"i32": aggs, "f64": aggs, "f32": aggs}))
     return {
         "value": sample_rows / t,
         "unit": "rows/s",
         "cores": 1,
-        "kind": "port",
-        "sample": (f"pandas {pd.__version__} groupby-agg (the cudf.pandas CPU fallback) on {sample_rows:,} rows "
-                   f"of the same distribution ({distinct:,} distinct int64 keys, 3 value columns 5% null, "
-                   f"{'/'.join(aggs)} each), median of {reps}, single-threaded; "
-                   f"host os.cpu_count()={os.cpu_count()}"),
+        "kind": "reference",
+        "sample": (f"pandas {pd.__version__} groupby-agg (the reference's CPU path: cudf.pandas falls back to pandas) "
+                   f"on {sample_rows:,} rows of the same distribution ({distinct:,} distinct int64 keys, 3 value "
+                   f"columns 5% null, {'/'.join(aggs)} each), default sort, 2 warm-ups, median of 7, "
+                   f"single-threaded; host os.cpu_count()={os.cpu_count()}"),
     }
 
 
@@ -638,6 +649,21 @@ def read_traffic(path: str, kernel: str):
     return None
 
 
+def read_call_traffic(path: str, kernel: str):
+    """HBM bytes of one whole call from a summary written by summarize_pmc:
+    every kernel's bytes per dispatch times its dispatches per call (the
+    dispatches of `kernel`, which runs once per call, count the calls)."""
+    try:
+        rows = [line.split() for line in open(path)]
+    except OSError:
+        return None
+    rows = [r for r in rows if len(r) == 4 and r[0].startswith("mgb::") and r[1].isdigit()]
+    calls = next((int(r[1]) for r in rows if r[0] == kernel), 0)
+    if not calls:
+        return None
+    return sum((float(r[2]) + float(r[3])) * 1e6 * int(r[1]) for r in rows) / calls
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -661,10 +687,13 @@ def main():
     ap.add_argument("--rows", type=int, default=None)
     ap.add_argument("--distinct", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--cpu-sample-rows", type=int, default=24_000_000,
-                    help="rows of the oracle's timed CPU sample (about 10 s for the 100M-row workload)")
+    ap.add_argument("--cpu-sample-rows", type=int, default=10_000_000,
+                    help="rows of the oracle's timed CPU sample (about 2 s for the 100M-row workload)")
     ap.add_argument("--pandas-sample-rows", type=int, default=10_000_000,
-                    help="rows of the reference CPU path's (pandas) timed sample")
+                    help="rows of the reference CPU path's (pandas) timed sample (9 runs, about 12 s at 10M rows)")
+    ap.add_argument("--settle-s", type=float, default=1.0,
+                    help="seconds of untimed calls before the warm-up steps (a fresh process's first calls "
+                         "include one-time host stalls)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pandas-aggs", default="sum,count,min,max,mean",
                     help="pandas workload: the aggregations of each value column (pandas names)")
@@ -754,9 +783,18 @@ def main():
     log(f"generating workload {args.workload} ({rows:,} rows)")
     key_cols, reqs, in_bytes_per_row, out_bytes_per_group, description = make_workload(args.workload, rows, args.seed)
     torch.cuda.synchronize()
-    log("warmup")
     gb = ca.GroupBy(ca.Table(key_cols))
     groups = 0
+    log(f"settling ({args.settle_s:.1f} s of untimed calls)")
+    t_end = time.perf_counter() + args.settle_s
+    while True:
+        k, r = gb.aggregate(reqs)
+        groups = k.num_rows()
+        del k, r
+        torch.cuda.synchronize()
+        if time.perf_counter() >= t_end:
+            break
+    log("warmup")
     for _ in range(args.warmup):
         k, r = gb.aggregate(reqs)
         groups = k.num_rows()
@@ -764,10 +802,6 @@ def main():
     torch.cuda.synchronize()
 
     log(f"timing {args.steps} steps")
-    lib.mgb_profiling_enable(1)
-    for name in PROFILED:
-        n, ms = ctypes.c_int64(), ctypes.c_double()
-        lib.mgb_profiling_read(name.encode(), ctypes.byref(n), ctypes.byref(ms))  # drop stale records
     stream = torch.cuda.current_stream()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -779,6 +813,18 @@ def main():
     stop.record(stream)
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
+
+    # per-kernel durations from a separate pass of the same steps, with HIP
+    # events around every launch on its stream (not in the timed region)
+    log("kernel timings")
+    lib.mgb_profiling_enable(1)
+    for name in PROFILED:
+        n, ms = ctypes.c_int64(), ctypes.c_double()
+        lib.mgb_profiling_read(name.encode(), ctypes.byref(n), ctypes.byref(ms))  # drop stale records
+    for _ in range(args.steps):
+        k, r = gb.aggregate(reqs)
+        del k, r
+    torch.cuda.synchronize()
     kern = {}
     for name in PROFILED:
         n, ms = ctypes.c_int64(), ctypes.c_double()
@@ -809,8 +855,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        cpu = cpu_baseline_oracle(args.workload, min(args.cpu_sample_rows, R), args.distinct, args.seed,
-                                  min(args.pandas_sample_rows, R))
+        cpu = cpu_baseline(args.workload, min(args.pandas_sample_rows, R), min(args.cpu_sample_rows, R),
+                           args.distinct, args.seed)
 
     line = {
         "metric": METRIC,
@@ -826,24 +872,30 @@ def main():
         "dtype": "int key(s); int/float values; int64 and f64 accumulation",
         "data": "synthetic (seeded on device)",
         "config": {"workload": description, "rows": R, "groups": groups, "parallelism": "single GPU"},
+        # the whole call: the algorithmic bytes (input columns read once,
+        # results written once) over the measured time per step
         "roofline": {
             "bound": "hbm",
-            "kernel": symbol,
-            "achieved": achieved,
+            "scope": "whole call (inputs read once + results written once, per step)",
+            "achieved": call_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "traffic_source": os.path.relpath(args.traffic_file, ROOT) if traffic is not None else None,
-            "algorithmic_bytes_per_launch": in_bytes,
-            "avg_launch_ms": agg_avg_ms,
-        },
-        "whole_call": {
-            "algorithmic_bytes": in_bytes + out_bytes,
-            "achieved_GBs": call_gbs,
             "frac": call_gbs / HBM_PEAK_GBS,
-            "kernel_ms": {k: (v[1] / max(v[0], 1)) for k, v in kern.items()},
+            "traffic": read_call_traffic(args.traffic_file, symbol) if args.workload == "c2" else None,
+            "algorithmic_bytes_per_step": in_bytes + out_bytes,
+            # the dominant kernel: its algorithmic bytes per launch (the input
+            # columns, read once) over its average launch duration
+            "dominant_kernel": {
+                "kernel": symbol,
+                "achieved": achieved,
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic,
+                "traffic_source": os.path.relpath(args.traffic_file, ROOT) if traffic is not None else None,
+                "algorithmic_bytes_per_launch": in_bytes,
+                "avg_launch_ms": agg_avg_ms,
+            },
         },
+        "kernel_ms": {k: (v[1] / max(v[0], 1)) for k, v in kern.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
