@@ -1,7 +1,9 @@
 // cudf::groupby::groupby -- the drop-in for cpp/include/cudf/groupby.hpp
 // (cudf 24.04): same class, request/result structs and `aggregate`
-// signatures. Only the hash-based path exists here; aggregations that cudf
-// serves from its sort-based path raise cudf::logic_error.
+// signatures. Every call groups through a hash table; the aggregations cudf
+// serves only from its sort-based path (MEDIAN, QUANTILE, NUNIQUE,
+// NTH_ELEMENT, M2) are computed from sorts of each value column's rows by
+// group; list-valued and nested aggregations raise cudf::logic_error.
 #pragma once
 
 #include <cudf/aggregation.hpp>

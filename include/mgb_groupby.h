@@ -168,25 +168,21 @@ int32_t mgb_target_type(int32_t value_type_id, int32_t agg_kind);
  * when it returns (stream order, as in cudf): a reader on another stream
  * waits for `stream` first. Errors: MGB_ERROR_LOGIC for a
  * size mismatch or an invalid type/aggregation pair (groupby.cu:208-212,
- * 173-190), MGB_ERROR_UNSUPPORTED for aggregations cudf serves only from its
- * sort-based path (median, quantile, nth element, nunique, collect, ...).
- * VARIANCE, STD, ARGMIN and ARGMAX (hash aggregations in
+ * 173-190); MGB_ERROR_UNSUPPORTED for more rows than cudf::size_type holds
+ * and for the list-valued and nested aggregations (COLLECT_LIST,
+ * COLLECT_SET, MERGE_*, TDIGEST, ...); MGB_ERROR_ALLOC when the allocator
+ * returns NULL. VARIANCE, STD, ARGMIN and ARGMAX (hash aggregations in
  * cpp/src/groupby/hash/groupby.cu:73-84) and M2 (sort-only in cudf,
- * cpp/src/groupby/sort/group_m2.cu) take a second pass over the rows.
+ * cpp/src/groupby/sort/group_m2.cu) take a second pass over the rows;
+ * MEDIAN, QUANTILE, NUNIQUE and NTH_ELEMENT (sort-only in cudf,
+ * cpp/src/groupby/sort/aggregate.cpp) sort each value column's rows by
+ * group.
  */
 mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_keys, int32_t null_policy,
                                  const mgb_request* requests, int32_t num_requests,
                                  mgb_out_column* out_keys, mgb_out_column* out_results,
                                  int64_t* out_num_groups, void* stream, const mgb_allocator* alloc);
 
-/*
- * Row hash of a table with cudf's MurmurHash3_x86_32 row hasher: per column
- * murmur3 of the element (floats with -0.0 -> +0.0 and NaN -> canonical NaN,
- * null -> 0xFFFFFFFF), folded with hash_combine from `seed`. Replaces
- * cudf::hashing::murmurhash3_x86_32(table_view, seed) for fixed-width columns
- * (cpp/include/cudf/hashing.hpp:90-94, cpp/src/hash/murmurhash3_x86_32.cu).
- * `out` is a device array of num_rows uint32.
- */
 /*
  * mgb_groupby_aggregate for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
@@ -215,6 +211,14 @@ mgb_status mgb_groupby_aggregate_noalloc(const mgb_column* keys, int32_t num_key
                                          mgb_out_column* out_keys, mgb_out_column* out_results,
                                          int64_t max_groups, int64_t* out_num_groups, void* stream);
 
+/*
+ * Row hash of a table with cudf's MurmurHash3_x86_32 row hasher: per column
+ * murmur3 of the element (floats with -0.0 -> +0.0 and NaN -> canonical NaN,
+ * null -> 0xFFFFFFFF), folded with hash_combine from `seed`. Replaces
+ * cudf::hashing::murmurhash3_x86_32(table_view, seed) for fixed-width columns
+ * (cpp/include/cudf/hashing.hpp:90-94, cpp/src/hash/murmurhash3_x86_32.cu).
+ * `out` is a device array of num_rows uint32.
+ */
 mgb_status mgb_murmurhash3_x86_32(const mgb_column* columns, int32_t num_columns, uint32_t seed,
                                   uint32_t* out, void* stream);
 
