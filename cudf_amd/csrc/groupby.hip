@@ -41,6 +41,7 @@
 
 #include "groupby_host.hpp"
 #include "lds_agg.hpp"
+#include "tiny_agg.hpp"
 #include "partition.hpp"
 #include "sort_aggs.hpp"
 
@@ -265,7 +266,7 @@ struct SampleParams {
   uint32_t* cnt;              // occurrences per slot; cnt[scap] = the special key
   int64_t scap;
   int64_t samples;
-  unsigned long long* stats;  // [0] distinct, [1] f1, [2] f2, [3] sampled rows
+  unsigned long long* stats;  // [0] distinct, [1] f1, [2] f2, [3] sampled rows, [4] keys found, [5..] keys
 };
 
 __global__ __launch_bounds__(kBlock) void sample_insert_kernel(const SampleParams s)
@@ -322,6 +323,18 @@ __global__ __launch_bounds__(kBlock) void sample_stats_kernel(const uint32_t* cn
     if (f1) atom_add(&stats[1], (unsigned long long)f1);
     if (f2) atom_add(&stats[2], (unsigned long long)f2);
   }
+}
+
+// The sample's distinct keys (integer keys: the slot headers), up to
+// kTinyKeys of them, into keys[0..]; *count: how many there are.
+__global__ __launch_bounds__(kBlock) void sample_keys_kernel(const unsigned long long* hdr, const uint32_t* cnt,
+                                                             int64_t scap, unsigned long long* count,
+                                                             unsigned long long* keys)
+{
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= scap || cnt[i] == 0) return;
+  const unsigned long long j = atomicAdd(count, 1ull);
+  if (j < (unsigned long long)kTinyKeys) keys[j] = hdr[i];
 }
 
 // ---- cardinality estimate: HyperLogLog over every row's key hash
@@ -649,6 +662,23 @@ __global__ __launch_bounds__(kLdsBlock) void lds_agg_kernel(const LdsParams lp)
   }
   if (fold) lds_fold_to_global(p, t, fold_part, fold_parts);
   else lds_write_groups(p, t, &lp.out_counters[0], 0, *lp.out);
+}
+
+// The few-groups direct execution: each workgroup aggregates its slice of
+// rows against the candidate groups (tiny_aggregate) and folds its table
+// into the global table.
+template <int NC>
+__global__ __launch_bounds__(kLdsBlock) void tiny_agg_kernel(const LdsParams lp, const TinyParams* __restrict__ tp,
+                                                             int32_t tiny_off)
+{
+  extern __shared__ __align__(16) uint8_t lds[];
+  const AggParams& p = lp.src;
+  const int64_t begin = (int64_t)blockIdx.x * lp.rows_per_item;
+  const int64_t end = begin + lp.rows_per_item < p.num_rows ? begin + lp.rows_per_item : p.num_rows;
+  const LdsTable t = lds_table_at(lds, lp.slots, lp.words, lp.cap);
+  lds_table_init(t, lp.tmpl, lp.words);
+  tiny_aggregate<NC>(p, t, lds, (uint32_t)tiny_off, lp.tmpl, lp.words, tp, begin, end);
+  lds_fold_to_global(p, t);
 }
 
 // Second pass of the partitioned items whose groups went to the global
@@ -1242,6 +1272,22 @@ mgb_status upload_pass2(Workspace* ws, const Pass2Ops& q, hipStream_t stream, co
   return MGB_SUCCESS;
 }
 
+// Copies the few-groups parameters into a parameter slot of device memory.
+mgb_status upload_tiny(Workspace* ws, const TinyParams& tp, hipStream_t stream, const TinyParams** d)
+{
+  const size_t slot_bytes = kStagingBytes / kUploadSlots;
+  static_assert(sizeof(TinyParams) <= kStagingBytes / kUploadSlots, "parameter slot too small");
+  if (ws->upload_slot >= kUploadSlots) return fail(MGB_ERROR_UNSUPPORTED, "too many parameter uploads in one call");
+  const int slot = ws->upload_slot++;
+  MGB_RESERVE(ws->params, kStagingBytes);
+  uint8_t* host = ws->staging + slot * slot_bytes;
+  uint8_t* dev = static_cast<uint8_t*>(ws->params.ptr) + slot * slot_bytes;
+  std::memcpy(host, &tp, sizeof(TinyParams));
+  MGB_HIP(hipMemcpyAsync(dev, host, sizeof(TinyParams), hipMemcpyHostToDevice, stream));
+  *d = reinterpret_cast<const TinyParams*>(dev);
+  return MGB_SUCCESS;
+}
+
 // Where the result columns go: buffers from the caller's allocator, or
 // caller-provided buffers of `capacity` rows each (the entry point that
 // allocates nothing).
@@ -1592,14 +1638,24 @@ int64_t hll_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_
   return std::max<int64_t>(1, std::min<int64_t>(R, (int64_t)std::ceil(est)));
 }
 
-int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_t stream, mgb_status& st)
+// The sample's distinct integer keys when it holds every group of the call
+// and at most kTinyKeys of them.
+struct SampleKeys {
+  bool complete = false;
+  int n = 0;
+  uint64_t keys[kTinyKeys] = {};
+};
+
+int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_t stream, mgb_status& st,
+                        SampleKeys* sk = nullptr)
 {
   st = MGB_SUCCESS;
   const int64_t kSample = 1 << 16;
   const int64_t scap = 4 * kSample;
   // small inputs are counted exactly; larger ones are sampled
   const int64_t samples = R <= scap / 2 ? R : kSample;
-  size_t need = align_up((size_t)(scap + 1) * 8, 256) + align_up((size_t)(scap + 1) * 4, 256) + 256;
+  size_t need = align_up((size_t)(scap + 1) * 8, 256) + align_up((size_t)(scap + 1) * 4, 256) +
+                align_up((5 + kTinyKeys) * 8, 256);
   if (ws->main.reserve(need, stream) != hipSuccess) {
     st = fail(MGB_ERROR_ALLOC, "allocation of the cardinality sample's temporaries failed");
     return R;
@@ -1616,22 +1672,33 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
   const uint64_t empty = base.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
   fill_u64_kernel<<<256, kBlock, 0, stream>>>(sp.hdr, scap, empty);
   (void)hipMemsetAsync(sp.cnt, 0, (size_t)(scap + 1) * 4, stream);
-  (void)hipMemsetAsync(sp.stats, 0, 4 * sizeof(unsigned long long), stream);
+  (void)hipMemsetAsync(sp.stats, 0, 5 * sizeof(unsigned long long), stream);
   sample_insert_kernel<<<(unsigned)((samples + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp);
   sample_stats_kernel<<<(unsigned)((scap + 1 + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp.cnt, scap + 1, sp.stats);
+  const bool want_keys = sk && base.key_mode == KEY_INT;
+  if (want_keys)
+    sample_keys_kernel<<<(unsigned)((scap + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp.hdr, sp.cnt, scap,
+                                                                                      sp.stats + 4, sp.stats + 5);
   e = hipGetLastError();
   if (e == hipSuccess)
-    e = hipMemcpyAsync(ws->pinned, sp.stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream);
+    e = hipMemcpyAsync(ws->pinned, sp.stats, (want_keys ? 5 + kTinyKeys : 4) * sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) {
     st = fail(MGB_ERROR_HIP, std::string("cardinality sample: ") + hipGetErrorString(e));
     return R;
   }
   const uint64_t d = ws->pinned[0], f1 = ws->pinned[1], sampled = ws->pinned[3];
-  if ((int64_t)sampled >= R || samples == R) return (int64_t)d;
-  // a saturated sample (every sampled key seen at least twice, few distinct
-  // keys) has found all groups; otherwise count distinct keys over all rows
-  if (f1 == 0 && d * 8 <= sampled) return (int64_t)d;
+  // every row sampled, or a saturated sample (every sampled key seen at least
+  // twice, few distinct keys): all groups found
+  const bool complete = (int64_t)sampled >= R || samples == R || (f1 == 0 && d * 8 <= sampled);
+  if (want_keys && complete && ws->pinned[4] <= (uint64_t)kTinyKeys) {
+    sk->complete = true;
+    sk->n = (int)ws->pinned[4];
+    for (int i = 0; i < sk->n; ++i) sk->keys[i] = ws->pinned[5 + i];
+  }
+  if (complete) return (int64_t)d;
+  // otherwise count distinct keys over all rows
   return hll_estimate(ws, base, R, stream, st);
 }
 
@@ -1803,6 +1870,52 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
   return MGB_SUCCESS;
 }
 
+// Whether a call takes the few-groups direct execution, and its parameters:
+// a single integer key, every group among the sample's distinct keys (at
+// most kTinyKeys), one or two value columns whose ops are SUM / MIN / MAX
+// and counts, no second pass or sort. MGB_FEW_GROUPS=off turns it off,
+// =collapse makes every wave collapse each group's lanes first.
+bool tiny_plan(const Plan& plan, const AggParams& ap, const SampleKeys& sk, TinyParams& tp)
+{
+  const char* env = std::getenv("MGB_FEW_GROUPS");
+  const std::string mode = env ? env : "";
+  if (mode == "off") return false;
+  if (!sk.complete || sk.n < 1 || plan.key_mode != KEY_INT || plan.compare_key_nulls || plan.has_pass2 ||
+      plan.has_sort || ap.num_vals < 1 || ap.num_vals > kTinyCols)
+    return false;
+  tp = TinyParams{};
+  for (int c = 0; c < ap.num_vals; ++c) {
+    ValCol v = ap.vals[c];
+    if (v.lds_generic) return false;
+    v.flag_bit = -1;  // validity from the column's mask
+    tp.prog[c] = pack_program(v, 0);
+  }
+  tp.collapse = mode == "collapse" ? 1 : 0;
+  // candidates (the key equal to the empty header takes the probe)
+  int n = 0;
+  for (int i = 0; i < sk.n; ++i)
+    if (sk.keys[i] != kEmptyIntKey) tp.keys[n++] = sk.keys[i];
+  if (n == 0) return false;
+  tp.num_keys = n;
+  // a lookup multiplier under which the candidates fall into distinct entries
+  for (uint32_t trial = 0; trial < 4096; ++trial) {
+    const uint32_t mult = (0x9e3779b1u + trial * 0x6a09e668u) | 1u;
+    bool used[1 << kTinyLutBits] = {};
+    bool ok = true;
+    for (int i = 0; i < n && ok; ++i) {
+      const uint32_t h = tiny_hash(tp.keys[i], mult);
+      ok = !used[h];
+      used[h] = true;
+    }
+    if (!ok) continue;
+    tp.mult = mult;
+    std::memset(tp.lut, kTinyNone, sizeof(tp.lut));
+    for (int i = 0; i < n; ++i) tp.lut[tiny_hash(tp.keys[i], mult)] = (uint8_t)i;
+    return true;
+  }
+  return false;
+}
+
 // Global-table executions (global and direct paths): build the table, run
 // the second pass, compact it into temporary outputs (null results counted
 // as they are written) and read the group count, the overflow flag and the
@@ -1890,7 +2003,8 @@ mgb_status finalize_table(Workspace* ws, const Plan& plan, int64_t capacity, int
 }
 
 mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int64_t R, int64_t G_est, bool direct,
-                     const LdsGeom& geom, hipStream_t stream, TempOuts& temp, int64_t& groups)
+                     const LdsGeom& geom, hipStream_t stream, TempOuts& temp, int64_t& groups,
+                     const TinyParams* tiny = nullptr)
 {
   const int cus = device_cu_count();
   int64_t capacity = std::max<int64_t>(64, 2 * G_est + 1024);
@@ -1961,13 +2075,38 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
       lp.words = lds_stride_of(plan.stride) / 8;
       lp.direct = 1;
       lp.log2p = 0;
-      MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)geom.bytes));
-      {
-        ProfScope prof("lds_aggregate", stream);
-        lds_agg_kernel<false><<<(unsigned)items, kLdsBlock, geom.bytes, stream>>>(lp);
+      if (tiny) {
+        // the few-groups execution: a small table (for keys the sample
+        // missed) and the striped copies of the candidates' records
+        const int words = lds_stride_of(plan.stride) / 8;
+        const size_t tiny_bytes = tiny_lds_bytes(words);
+        const size_t avail = lds_limit() - tiny_bytes - 64;
+        const int slots = (int)std::min<int64_t>(1023, (int64_t)((avail - 16 - (size_t)kLdsBlock * 4) / ((size_t)words * 8)) - 1);
+        if (slots < 64) return fail(MGB_ERROR_UNSUPPORTED, "few-groups execution does not fit LDS");
+        lp.slots = slots;
+        lp.cap = (int)(slots * 0.62);
+        const size_t table_bytes = align_up(lds_table_bytes(slots, words), 16);
+        const size_t total = table_bytes + tiny_bytes;
+        const TinyParams* d_tp = nullptr;
+        st = upload_tiny(ws, *tiny, stream, &d_tp);
+        if (st != MGB_SUCCESS) return st;
+        const void* fn = lp.src.num_vals == 1 ? (const void*)tiny_agg_kernel<1> : (const void*)tiny_agg_kernel<2>;
+        MGB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)total));
+        {
+          ProfScope prof("lds_aggregate", stream);
+          if (lp.src.num_vals == 1) tiny_agg_kernel<1><<<(unsigned)items, kLdsBlock, total, stream>>>(lp, d_tp, (int32_t)table_bytes);
+          else tiny_agg_kernel<2><<<(unsigned)items, kLdsBlock, total, stream>>>(lp, d_tp, (int32_t)table_bytes);
+        }
+        MGB_HIP(hipGetLastError());
+      } else {
+        MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)geom.bytes));
+        {
+          ProfScope prof("lds_aggregate", stream);
+          lds_agg_kernel<false><<<(unsigned)items, kLdsBlock, geom.bytes, stream>>>(lp);
+        }
+        MGB_HIP(hipGetLastError());
       }
-      MGB_HIP(hipGetLastError());
       direct_lp = lp;
       direct_items = items;
     } else {
@@ -2383,7 +2522,8 @@ mgb_status run_groupby(const Plan& plan, int64_t R, Workspace* ws, mgb_out_colum
 
   mgb_status st = MGB_SUCCESS;
   ws->fine_valid = false;
-  int64_t G_est = sample_estimate(ws, base, R, stream, st);
+  SampleKeys sk;
+  int64_t G_est = sample_estimate(ws, base, R, stream, st, &sk);
   if (st != MGB_SUCCESS) return st;
   if (const char* e = std::getenv("MGB_ESTIMATE_SCALE")) {
     // test hook: a scaled estimate (below 1 forces the overflow-and-retry
@@ -2417,7 +2557,9 @@ mgb_status run_groupby(const Plan& plan, int64_t R, Workspace* ws, mgb_out_colum
     if (!done) path = Path::Global;
   }
   if (!done) {
-    st = run_table(ws, plan, base, R, G_est, path == Path::Direct, geom, stream, temp, groups);
+    TinyParams tp;
+    const bool tiny = path == Path::Direct && !std::getenv("MGB_ESTIMATE_SCALE") && tiny_plan(plan, probe, sk, tp);
+    st = run_table(ws, plan, base, R, G_est, path == Path::Direct, geom, stream, temp, groups, tiny ? &tp : nullptr);
     if (st != MGB_SUCCESS) return st;
   }
   *out_num_groups = groups;
