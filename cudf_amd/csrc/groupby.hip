@@ -667,7 +667,7 @@ __global__ __launch_bounds__(kLdsBlock) void lds_agg_kernel(const LdsParams lp)
 // The few-groups direct execution: each workgroup aggregates its slice of
 // rows against the candidate groups (tiny_aggregate) and folds its table
 // into the global table.
-template <int NC>
+template <int NC, int KW, int TV>
 __global__ __launch_bounds__(kLdsBlock) void tiny_agg_kernel(const LdsParams lp, const TinyParams* __restrict__ tp,
                                                              int32_t tiny_off)
 {
@@ -677,7 +677,7 @@ __global__ __launch_bounds__(kLdsBlock) void tiny_agg_kernel(const LdsParams lp,
   const int64_t end = begin + lp.rows_per_item < p.num_rows ? begin + lp.rows_per_item : p.num_rows;
   const LdsTable t = lds_table_at(lds, lp.slots, lp.words, lp.cap);
   lds_table_init(t, lp.tmpl, lp.words);
-  tiny_aggregate<NC>(p, t, lds, (uint32_t)tiny_off, lp.tmpl, lp.words, tp, begin, end);
+  tiny_aggregate<NC, KW, TV>(p, t, lds, (uint32_t)tiny_off, lp.tmpl, lp.words, tp, begin, end);
   lds_fold_to_global(p, t);
 }
 
@@ -2090,12 +2090,27 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
         const TinyParams* d_tp = nullptr;
         st = upload_tiny(ws, *tiny, stream, &d_tp);
         if (st != MGB_SUCCESS) return st;
-        const void* fn = lp.src.num_vals == 1 ? (const void*)tiny_agg_kernel<1> : (const void*)tiny_agg_kernel<2>;
-        MGB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)total));
+        // compiled for a single value column of a common type and a 4- or
+        // 8-byte key; the general instance otherwise
+        using TinyFn = void (*)(const LdsParams, const TinyParams*, int32_t);
+        TinyFn fn = lp.src.num_vals == 1 ? tiny_agg_kernel<1, 0, TV_ANY> : tiny_agg_kernel<2, 0, TV_ANY>;
+        if (lp.src.num_vals == 1) {
+          const int kw = lp.src.keys[0].stride == lp.src.keys[0].width ? lp.src.keys[0].width : 0;
+          const int tv = lp.src.vals[0].stride == lp.src.vals[0].width
+                             ? tiny_val_type(lp.src.vals[0].width, lp.src.vals[0].vclass)
+                             : TV_ANY;
+          if (kw == 4) {
+            fn = tv == TV_I32 ? tiny_agg_kernel<1, 4, TV_I32> : tv == TV_I64 ? tiny_agg_kernel<1, 4, TV_I64>
+                 : tv == TV_F32 ? tiny_agg_kernel<1, 4, TV_F32> : tv == TV_F64 ? tiny_agg_kernel<1, 4, TV_F64> : fn;
+          } else if (kw == 8) {
+            fn = tv == TV_I32 ? tiny_agg_kernel<1, 8, TV_I32> : tv == TV_I64 ? tiny_agg_kernel<1, 8, TV_I64>
+                 : tv == TV_F32 ? tiny_agg_kernel<1, 8, TV_F32> : tv == TV_F64 ? tiny_agg_kernel<1, 8, TV_F64> : fn;
+          }
+        }
+        MGB_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)total));
         {
           ProfScope prof("lds_aggregate", stream);
-          if (lp.src.num_vals == 1) tiny_agg_kernel<1><<<(unsigned)items, kLdsBlock, total, stream>>>(lp, d_tp, (int32_t)table_bytes);
-          else tiny_agg_kernel<2><<<(unsigned)items, kLdsBlock, total, stream>>>(lp, d_tp, (int32_t)table_bytes);
+          hipLaunchKernelGGL(fn, dim3((unsigned)items), dim3(kLdsBlock), total, stream, lp, d_tp, (int32_t)table_bytes);
         }
         MGB_HIP(hipGetLastError());
       } else {

@@ -60,6 +60,63 @@ __device__ __forceinline__ void tiny_miss_row(const AggParams& p, const LdsTable
   else spill_row(p, rk, row);
 }
 
+// Value column types the row loop is compiled for (TV_ANY: read from the
+// column's description at run time).
+enum TinyValType : int { TV_ANY = 0, TV_I32, TV_I64, TV_F32, TV_F64 };
+
+__host__ __device__ constexpr int tiny_val_type(int width, int vclass)
+{
+  return vclass == VC_SIGNED && width == 4 ? TV_I32
+         : vclass == VC_SIGNED && width == 8 ? TV_I64
+         : vclass == VC_F32 ? TV_F32
+         : vclass == VC_F64 ? TV_F64
+                            : TV_ANY;
+}
+
+// SUM / MIN / MAX of N rows of a column of known type on their record
+// copies (the column program's updates with the dispatch resolved at
+// compile time).
+template <int TV, int N>
+__device__ __forceinline__ void tiny_column_fixed(uint8_t* lds, const Program& g, const uint32_t (&lo)[N],
+                                                  const uint64_t (&x)[N], const bool (&ok)[N], uint32_t dum)
+{
+  if (g.sum_off >= 0) {
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      uint8_t* a = lds + (ok[u] ? lo[u] + g.sum_off : dum);
+      if constexpr (TV == TV_F64 || TV == TV_F32)
+        lds_add(reinterpret_cast<double*>(a), canon_f64(x[u], TV == TV_F32 ? VC_F32 : VC_F64));
+      else
+        lds_add(reinterpret_cast<unsigned long long*>(a), (unsigned long long)x[u]);
+    }
+  }
+  for (int which = 0; which < 2; ++which) {
+    const int off = which == 0 ? g.min_off : g.max_off;
+    if (off < 0) continue;
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      uint8_t* a = lds + (ok[u] ? lo[u] + off : dum);
+      if constexpr (TV == TV_F64) {
+        const unsigned long long b = x[u];
+        const bool nan = (b & 0x7ff0000000000000ull) == 0x7ff0000000000000ull && (b & 0x000fffffffffffffull);
+        if (which == 0) lds_min(reinterpret_cast<unsigned long long*>(a), nan ? kF64MinNaN : f64_code(b));
+        else lds_max(reinterpret_cast<unsigned long long*>(a), nan ? kF64MaxNaN : f64_code(b));
+      } else if constexpr (TV == TV_F32) {
+        const uint32_t b = (uint32_t)x[u];
+        const bool nan = (b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu);
+        if (which == 0) lds_min(reinterpret_cast<uint32_t*>(a), nan ? kF32MinNaN : f32_code(b));
+        else lds_max(reinterpret_cast<uint32_t*>(a), nan ? kF32MaxNaN : f32_code(b));
+      } else if constexpr (TV == TV_I32) {
+        if (which == 0) lds_min(reinterpret_cast<int32_t*>(a), (int32_t)x[u]);
+        else lds_max(reinterpret_cast<int32_t*>(a), (int32_t)x[u]);
+      } else {
+        if (which == 0) lds_min(reinterpret_cast<long long*>(a), (long long)x[u]);
+        else lds_max(reinterpret_cast<long long*>(a), (long long)x[u]);
+      }
+    }
+  }
+}
+
 // Wave-wide combination of one field (every lane ends with the result).
 template <typename T, typename F>
 __device__ __forceinline__ T tiny_wave_reduce(T v, F op)
@@ -130,10 +187,12 @@ __device__ __forceinline__ void tiny_collapse(const Program& g, const uint32_t (
   }
 }
 
-// Rows [begin, end) of a direct work item with NC value columns. `tp`
+// Rows [begin, end) of a direct work item with NC value columns (KW: the
+// key's width and TV: the first value column's type when compiled for them,
+// 0 otherwise). `tp`
 // (device memory): the candidates and programs. LDS: the item's table, then
 // at `tiny` the striped records and the lookup data.
-template <int NC>
+template <int NC, int KW, int TV>
 __device__ __forceinline__ void tiny_aggregate(const AggParams& p, const LdsTable& t, uint8_t* lds, uint32_t tiny_off,
                                                const uint64_t* tmpl, int words, const TinyParams* __restrict__ tp,
                                                int64_t begin, int64_t end)
@@ -193,12 +252,37 @@ __device__ __forceinline__ void tiny_aggregate(const AggParams& p, const LdsTabl
     return (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v) |
            (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32)) << 32;
   };
+  // the keys and values of the rows starting at `from` (nothing past the end)
+  auto fetch = [&](int64_t from, uint64_t (&key)[kRows], uint64_t (&x)[NC][kRows]) {
+    const uint32_t kp = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_k);
+    bool pr[kRows];
+#pragma unroll
+    for (int u = 0; u < kRows; ++u) pr[u] = from + threadIdx.x + u * step < end;
+    load_rows(reinterpret_cast<const uint8_t*>(sread64(pin_kdata)), KW ? KW : (int)(kp & 15),
+              KW ? KW : (int)((kp >> 8) & 0xffff), VC_UNSIGNED, from + threadIdx.x, step, pr, key);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const uint32_t vp = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_v[c]);
+      if (c == 0 && TV != TV_ANY) {
+        constexpr int w = TV == TV_I32 || TV == TV_F32 ? 4 : 8;
+        constexpr int vc = TV == TV_I32 || TV == TV_I64 ? VC_SIGNED : TV == TV_F32 ? VC_F32 : VC_F64;
+        load_rows(reinterpret_cast<const uint8_t*>(sread64(pin_vdata[c])), w, w, vc, from + threadIdx.x, step, pr,
+                  x[c]);
+      } else {
+        load_rows(reinterpret_cast<const uint8_t*>(sread64(pin_vdata[c])), (int)(vp & 15), (int)((vp >> 8) & 0xffff),
+                  (int)((vp >> 4) & 7), from + threadIdx.x, step, pr, x[c]);
+      }
+    }
+  };
+  uint64_t nkey[kRows], nx[NC][kRows];
+  if (begin < end) fetch(begin, nkey, nx);
   // the loop runs alike on every lane of the workgroup (the collapse's
-  // cross-lane reductions need the whole wave); lanes past the end hold no row
+  // cross-lane reductions need the whole wave); lanes past the end hold no
+  // row. The next rows' loads are issued before this iteration's updates.
   for (int64_t first = begin; first < end; first += (int64_t)kRows * kLdsBlock) {
     const int64_t base = first + threadIdx.x;
     const uint32_t kp = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_k);
-    const int kw = (int)(kp & 15), kvc = (int)((kp >> 4) & 7), kstride = (int)((kp >> 8) & 0xffff);
+    const int kw = KW ? KW : (int)(kp & 15), kvc = (int)((kp >> 4) & 7);
     const bool collapse = (kp >> 25) & 1u;
     const uint32_t mult = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_mult);
     const int rows_off = (int)(uint32_t)__builtin_amdgcn_readfirstlane((int)pin_rows) - 1;
@@ -206,15 +290,14 @@ __device__ __forceinline__ void tiny_aggregate(const AggParams& p, const LdsTabl
 #pragma unroll
     for (int u = 0; u < kRows; ++u) present[u] = base + u * step < end;
     uint64_t key[kRows];
-    load_rows(reinterpret_cast<const uint8_t*>(sread64(pin_kdata)), kw, kstride, VC_UNSIGNED, base, step, present,
-              key);
     uint64_t x[NC][kRows];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const uint32_t vp = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_v[c]);
-      load_rows(reinterpret_cast<const uint8_t*>(sread64(pin_vdata[c])), (int)(vp & 15), (int)((vp >> 8) & 0xffff),
-                (int)((vp >> 4) & 7), base, step, present, x[c]);
+    for (int u = 0; u < kRows; ++u) {
+      key[u] = nkey[u];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) x[c][u] = nx[c][u];
     }
+    if (first + (int64_t)kRows * kLdsBlock < end) fetch(first + (int64_t)kRows * kLdsBlock, nkey, nx);
     if ((kp >> 24) & 1u) {
 #pragma unroll
       for (int u = 0; u < kRows; ++u)
@@ -276,7 +359,8 @@ __device__ __forceinline__ void tiny_aggregate(const AggParams& p, const LdsTabl
         uint32_t lo[kRows];
 #pragma unroll
         for (int u = 0; u < kRows; ++u) lo[u] = hit[u] ? loff[u] : dum;
-        lds_column_program(lds, g, lo, x[c], ok, dum);
+        if (c == 0 && TV != TV_ANY) tiny_column_fixed<TV>(lds, g, lo, x[c], ok, dum);
+        else lds_column_program(lds, g, lo, x[c], ok, dum);
       } else {
         bool h[kRows];
         uint32_t n[kRows];

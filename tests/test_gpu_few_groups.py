@@ -108,3 +108,16 @@ def test_few_groups_shape_stress_16_keys(gpu, few):
     uniq = rng.integers(-2 ** 31, 2 ** 31 - 1, 16).astype(np.int32)
     keys = uniq[rng.integers(0, 16, n)]
     _compare([(O.INT32, keys, None)], [((O.FLOAT64, rng.standard_normal(n), None), [O.SUM, O.COUNT_VALID, O.MAX])])
+
+
+@pytest.mark.parametrize("key_type", [O.INT32, O.INT64])
+@pytest.mark.parametrize("val_type", [O.INT32, O.INT64, O.FLOAT32, O.FLOAT64])
+def test_few_groups_compiled_shapes(gpu, few, key_type, val_type):
+    """One value column of each type the row loop is compiled for, under 4-
+    and 8-byte keys, with nulls, NaN, infinities and signed zeros."""
+    rng = np.random.default_rng(700 + 10 * key_type + val_type)
+    n = 2_000_000
+    pool = np.unique(rng.integers(-2 ** 31, 2 ** 31 - 1, 20)).astype(O.NP[key_type])[:12]
+    keys = pool[rng.integers(0, pool.size, n)]
+    reqs = [((val_type, _vals(rng, val_type, n), rng.random(n) > 0.03), [O.SUM, O.MIN, O.MAX, O.COUNT_VALID, O.MEAN])]
+    _compare([(key_type, keys, None)], reqs)
