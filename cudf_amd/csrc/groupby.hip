@@ -62,6 +62,8 @@ __global__ __launch_bounds__(kBlock) void fill_table_kernel(const FillParams p)
     if (w == 0 && r == p.num_records - 1) v = p.special_hdr;
     p.table[i] = v;
   }
+  for (int z = 0; z < 2; ++z)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.zero_words[z]; i += step) p.zero[z][i] = 0ull;
 }
 
 // Per-row insert-or-find and atomic update in the global table.
@@ -269,18 +271,28 @@ struct SampleParams {
   unsigned long long* stats;  // [0] distinct, [1] f1, [2] f2, [3] sampled rows, [4] keys found, [5..] keys
 };
 
+// Occurrence counts are only told apart up to 3 (distinct keys, keys seen
+// once, keys seen twice): a count already at 3 is not incremented, so that
+// the few keys of a low-cardinality input do not serialise the sample on
+// their counters.
+__device__ __forceinline__ void sample_count(uint32_t* c)
+{
+  if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 3u) atom_add(c, 1u);
+}
+
 __global__ __launch_bounds__(kBlock) void sample_insert_kernel(const SampleParams s)
 {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (j >= s.samples) return;
   const AggParams& p = s.agg;
   const int64_t span = p.num_rows / s.samples;
   int64_t row = j * span + (span > 1 ? (int64_t)(fmix32((uint32_t)j * 0x9e3779b9u + 1u) % (uint32_t)span) : 0);
-  if (row >= p.num_rows || !key_row_valid(p, row)) return;
-  atom_add(&s.stats[3], 1ull);
+  const bool valid = j < s.samples && row < p.num_rows && key_row_valid(p, row);
+  const int n = __syncthreads_count(valid);  // sampled rows: one add per workgroup
+  if (threadIdx.x == 0 && n) atom_add(&s.stats[3], (unsigned long long)n);
+  if (!valid) return;
   const RowKey rk = row_key(p, row);
   if (rk.special) {
-    atom_add(&s.cnt[s.scap], 1u);
+    sample_count(&s.cnt[s.scap]);
     return;
   }
   const uint64_t empty = p.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
@@ -291,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void sample_insert_kernel(const SampleParam
       if (cur == empty) {
         uint64_t seen = atom_cas(&s.hdr[idx], empty, rk.key);
         if (seen == empty) {
-          atom_add(&s.cnt[idx], 1u);
+          sample_count(&s.cnt[idx]);
           return;
         }
         cur = seen;
@@ -302,12 +314,25 @@ __global__ __launch_bounds__(kBlock) void sample_insert_kernel(const SampleParam
                   : ((uint32_t)(cur >> 32) == rk.hash &&
                      rows_equal(p.keys, p.num_keys, p.compare_key_nulls != 0, (int64_t)(cur & 0xffffffffull), row));
       if (eq) {
-        atom_add(&s.cnt[idx], 1u);
+        sample_count(&s.cnt[idx]);
         return;
       }
       break;
     }
     idx = (idx + 1 == s.scap) ? 0 : idx + 1;
+  }
+}
+
+// Clears the sample table: empty headers, zero counts and statistics.
+__global__ __launch_bounds__(kBlock) void sample_clear_kernel(unsigned long long* hdr, uint32_t* cnt, int64_t scap,
+                                                              unsigned long long empty, unsigned long long* stats,
+                                                              int nstats)
+{
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i <= scap; i += step) {
+    if (i < scap) hdr[i] = empty;
+    cnt[i] = 0u;
+    if (i < nstats) stats[i] = 0ull;
   }
 }
 
@@ -1670,9 +1695,7 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
   sp.scap = scap;
   sp.samples = samples;
   const uint64_t empty = base.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
-  fill_u64_kernel<<<256, kBlock, 0, stream>>>(sp.hdr, scap, empty);
-  (void)hipMemsetAsync(sp.cnt, 0, (size_t)(scap + 1) * 4, stream);
-  (void)hipMemsetAsync(sp.stats, 0, 5 * sizeof(unsigned long long), stream);
+  sample_clear_kernel<<<256, kBlock, 0, stream>>>(sp.hdr, sp.cnt, scap, empty, sp.stats, 5);
   sample_insert_kernel<<<(unsigned)((samples + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp);
   sample_stats_kernel<<<(unsigned)((scap + 1 + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp.cnt, scap + 1, sp.stats);
   const bool want_keys = sk && base.key_mode == KEY_INT;
@@ -2039,12 +2062,22 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
     record_template(plan, plan.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef, fp.tmpl);
     const int64_t fill_words = fp.num_records * fp.words;
     const unsigned fill_grid = (unsigned)std::min<int64_t>((fill_words + kBlock - 1) / kBlock, (int64_t)cus * 32);
+    unsigned long long* counters = reinterpret_cast<unsigned long long*>(wsb + counters_off);
+    fp.zero[0] = counters;  // cleared with the table
+    fp.zero_words[0] = 32;
+    if (early) {
+      // the temporary outputs (and their null counters, cleared with the
+      // table) are laid out before the aggregation
+      OutTable ot{};
+      mgb_status st = layout_temp(ws, plan, std::min<int64_t>(capacity + 1, R), stream, temp, ot);
+      if (st != MGB_SUCCESS) return st;
+      fp.zero[1] = temp.null_counts;
+      fp.zero_words[1] = n_nulls;
+    }
     {
       ProfScope prof("fill", stream);
       fill_table_kernel<<<fill_grid, kBlock, 0, stream>>>(fp);
     }
-    unsigned long long* counters = reinterpret_cast<unsigned long long*>(wsb + counters_off);
-    MGB_HIP(hipMemsetAsync(counters, 0, 256, stream));
 
     AggParams ap = base;
     ap.table = wsb;
@@ -2176,12 +2209,12 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
     scan_blocks_kernel<<<1, 1024, 0, stream>>>(block_counts, nblocks, counters + 2, nullptr);
     MGB_HIP(hipGetLastError());
     if (early) {
-      // temporary outputs for every record the table can hold
+      // the temporary outputs for every record the table can hold (laid out
+      // above; layout_temp gives the same buffers for the same size)
       OutTable ot{};
       mgb_status st = layout_temp(ws, plan, std::min<int64_t>(capacity + 1, R), stream, temp, ot);
       if (st != MGB_SUCCESS) return st;
       ot.nulls = temp.null_counts;
-      MGB_HIP(hipMemsetAsync(temp.null_counts, 0, (size_t)n_nulls * 8, stream));
       st = finalize_table(ws, plan, capacity, nblocks, block_off, nullptr, stream, ot);
       if (st != MGB_SUCCESS) return st;
       MGB_HIP(hipMemcpyAsync(ws->pinned + kPinnedNulls, temp.null_counts, (size_t)n_nulls * 8, hipMemcpyDeviceToHost,

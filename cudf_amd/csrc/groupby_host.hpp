@@ -26,7 +26,8 @@ constexpr int kMaxRecordWords = 64;
 constexpr int64_t kMaxRows = 0x7fffffff;  // cudf::size_type: rows of one call
 
 // Writes `tmpl` (header + identities) into every record; record `capacity`
-// gets `special_hdr` as header.
+// gets `special_hdr` as header. Also zeroes up to two ranges of 8-byte
+// words (the call's counters), saving their separate clears.
 struct FillParams {
   uint64_t* table;
   int64_t num_records;
@@ -34,6 +35,8 @@ struct FillParams {
   int32_t pad;
   uint64_t special_hdr;
   uint64_t tmpl[kMaxRecordWords];
+  unsigned long long* zero[2];
+  int64_t zero_words[2];
 };
 
 // ------------------------------------------------------------ profiling
