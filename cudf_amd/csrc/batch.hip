@@ -608,6 +608,14 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
       if (b->values[(size_t)q * num_requests + r].size != R)
         return fail(MGB_ERROR_LOGIC, "Size mismatch between request values and groupby keys.");
     if (R < 0 || R > kMaxRows) return fail(MGB_ERROR_UNSUPPORTED, "more rows than cudf::size_type holds (2^31 - 1)");
+    if (R > 0) {
+      // the kernels read row 0 of a column in place of an absent row
+      for (int c = 0; c < num_keys; ++c)
+        if (!b->keys[(size_t)q * num_keys + c].data) return fail(MGB_ERROR_LOGIC, "a key column with rows has no data");
+      for (int r = 0; r < num_requests; ++r)
+        if (!b->values[(size_t)q * num_requests + r].data)
+          return fail(MGB_ERROR_LOGIC, "a value column with rows has no data");
+    }
   }
   // the batch plan: query 0's
   {

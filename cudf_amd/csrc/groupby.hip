@@ -2679,6 +2679,13 @@ mgb_status check_call(const mgb_column* keys, int32_t num_keys, int32_t null_pol
   // cudf::size_type is int32: rows, groups, per-group counts and ARGMIN /
   // ARGMAX indices are 32-bit throughout (cpp/include/cudf/types.hpp:80)
   if (R < 0 || R > kMaxRows) return fail(MGB_ERROR_UNSUPPORTED, "more rows than cudf::size_type holds (2^31 - 1)");
+  // the kernels read row 0 of a column in place of an absent row
+  if (R > 0) {
+    for (int c = 0; c < num_keys; ++c)
+      if (!keys[c].data) return fail(MGB_ERROR_LOGIC, "a key column with rows has no data");
+    for (int r = 0; r < num_requests; ++r)
+      if (!requests[r].values.data) return fail(MGB_ERROR_LOGIC, "a value column with rows has no data");
+  }
   for (int r = 0; r < num_requests; ++r)
     for (int a = 0; a < requests[r].num_aggs; ++a)
       if (target_type(requests[r].values.type_id, requests[r].aggs[a].kind) < 0)

@@ -452,40 +452,82 @@ __device__ __forceinline__ void pass2_update(const Pass2Ops* __restrict__ q, uin
 // Elements row0 + u * step (u < N, where present[u]) of a column, in the
 // canonical form the ops take: integers widened to 64 bits (sign- or
 // zero-extended, bool -> 0/1), floats as their raw bits. Absent rows read 0.
+// Every lane issues its loads: an absent row loads row 0 (in bounds for any
+// column with rows) and its value is replaced by 0 afterwards, so no load
+// sits under a branch (a narrow load under a branch was waited for before
+// the branch closed, serialising a thread's loads).
 template <int N>
 __device__ __forceinline__ void load_rows(const uint8_t* data, int width, int stride, int vclass, int64_t row0,
                                           int64_t step, const bool (&present)[N], uint64_t (&x)[N])
 {
   const bool sgn = vclass == VC_SIGNED;
   const global_t<const uint8_t>* g = to_global(data);
+  const global_t<const uint8_t>* a[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) a[u] = present[u] ? g + (row0 + u * step) * stride : g;
   switch (width) {
     case 1:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
-        const uint8_t b = present[u] ? g[(row0 + u * step) * stride] : 0;
+        const uint8_t v = *a[u];
+        const uint8_t b = present[u] ? v : (uint8_t)0;
         x[u] = sgn ? (uint64_t)(int64_t)(int8_t)b : vclass == VC_BOOL ? (uint64_t)(b != 0) : (uint64_t)b;
       }
       break;
     case 2:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
-        const uint16_t b =
-            present[u] ? *reinterpret_cast<const global_t<uint16_t>*>(g + (row0 + u * step) * stride) : 0;
+        const uint16_t v = *reinterpret_cast<const global_t<uint16_t>*>(a[u]);
+        const uint16_t b = present[u] ? v : (uint16_t)0;
         x[u] = sgn ? (uint64_t)(int64_t)(int16_t)b : (uint64_t)b;
       }
       break;
     case 4:
 #pragma unroll
       for (int u = 0; u < N; ++u) {
-        const uint32_t b =
-            present[u] ? *reinterpret_cast<const global_t<uint32_t>*>(g + (row0 + u * step) * stride) : 0u;
+        const uint32_t v = *reinterpret_cast<const global_t<uint32_t>*>(a[u]);
+        const uint32_t b = present[u] ? v : 0u;
         x[u] = sgn ? (uint64_t)(int64_t)(int32_t)b : (uint64_t)b;
       }
       break;
     default:
 #pragma unroll
-      for (int u = 0; u < N; ++u)
-        x[u] = present[u] ? *reinterpret_cast<const global_t<uint64_t>*>(g + (row0 + u * step) * stride) : 0ull;
+      for (int u = 0; u < N; ++u) {
+        const uint64_t v = *reinterpret_cast<const global_t<uint64_t>*>(a[u]);
+        x[u] = present[u] ? v : 0ull;
+      }
+      break;
+  }
+}
+
+// Elements row0 + u * step of a column as raw bits, zero-extended (an
+// absent row loads row 0: its bits are arbitrary). Nothing is computed from
+// the loaded values, so the loads stay in flight until the bits are used;
+// widen_raw gives their canonical form.
+template <int N>
+__device__ __forceinline__ void load_rows_raw(const uint8_t* data, int width, int stride, int64_t row0, int64_t step,
+                                              const bool (&present)[N], uint64_t (&x)[N])
+{
+  const global_t<const uint8_t>* g = to_global(data);
+  const global_t<const uint8_t>* a[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) a[u] = present[u] ? g + (row0 + u * step) * stride : g;
+  switch (width) {
+    case 1:
+#pragma unroll
+      for (int u = 0; u < N; ++u) x[u] = (uint64_t)*a[u];
+      break;
+    case 2:
+#pragma unroll
+      for (int u = 0; u < N; ++u) x[u] = (uint64_t)*reinterpret_cast<const global_t<uint16_t>*>(a[u]);
+      break;
+    case 4:
+#pragma unroll
+      for (int u = 0; u < N; ++u) x[u] = (uint64_t)*reinterpret_cast<const global_t<uint32_t>*>(a[u]);
+      break;
+    default:
+#pragma unroll
+      for (int u = 0; u < N; ++u) x[u] = *reinterpret_cast<const global_t<uint64_t>*>(a[u]);
       break;
   }
 }

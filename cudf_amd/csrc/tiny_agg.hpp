@@ -252,33 +252,35 @@ __device__ __forceinline__ void tiny_aggregate(const AggParams& p, const LdsTabl
     return (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v) |
            (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32)) << 32;
   };
-  // the keys and values of the rows starting at `from` (nothing past the end)
+  // the raw bits of the keys and values of the rows starting at `from`
+  // (rows past the end load row 0); nothing is computed from them here, so
+  // the loads stay in flight through the current rows' updates
   auto fetch = [&](int64_t from, uint64_t (&key)[kRows], uint64_t (&x)[NC][kRows]) {
     const uint32_t kp = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_k);
     bool pr[kRows];
 #pragma unroll
     for (int u = 0; u < kRows; ++u) pr[u] = from + threadIdx.x + u * step < end;
-    load_rows(reinterpret_cast<const uint8_t*>(sread64(pin_kdata)), KW ? KW : (int)(kp & 15),
-              KW ? KW : (int)((kp >> 8) & 0xffff), VC_UNSIGNED, from + threadIdx.x, step, pr, key);
+    load_rows_raw(reinterpret_cast<const uint8_t*>(sread64(pin_kdata)), KW ? KW : (int)(kp & 15),
+                  KW ? KW : (int)((kp >> 8) & 0xffff), from + threadIdx.x, step, pr, key);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const uint32_t vp = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_v[c]);
       if (c == 0 && TV != TV_ANY) {
         constexpr int w = TV == TV_I32 || TV == TV_F32 ? 4 : 8;
-        constexpr int vc = TV == TV_I32 || TV == TV_I64 ? VC_SIGNED : TV == TV_F32 ? VC_F32 : VC_F64;
-        load_rows(reinterpret_cast<const uint8_t*>(sread64(pin_vdata[c])), w, w, vc, from + threadIdx.x, step, pr,
-                  x[c]);
+        load_rows_raw(reinterpret_cast<const uint8_t*>(sread64(pin_vdata[c])), w, w, from + threadIdx.x, step, pr,
+                      x[c]);
       } else {
-        load_rows(reinterpret_cast<const uint8_t*>(sread64(pin_vdata[c])), (int)(vp & 15), (int)((vp >> 8) & 0xffff),
-                  (int)((vp >> 4) & 7), from + threadIdx.x, step, pr, x[c]);
+        load_rows_raw(reinterpret_cast<const uint8_t*>(sread64(pin_vdata[c])), (int)(vp & 15),
+                      (int)((vp >> 8) & 0xffff), from + threadIdx.x, step, pr, x[c]);
       }
     }
   };
   uint64_t nkey[kRows], nx[NC][kRows];
-  if (begin < end) fetch(begin, nkey, nx);
+  fetch(begin, nkey, nx);
   // the loop runs alike on every lane of the workgroup (the collapse's
   // cross-lane reductions need the whole wave); lanes past the end hold no
-  // row. The next rows' loads are issued before this iteration's updates.
+  // row. The next rows' loads are issued before this iteration's updates
+  // (past the end: loads of row 0, unused).
   for (int64_t first = begin; first < end; first += (int64_t)kRows * kLdsBlock) {
     const int64_t base = first + threadIdx.x;
     const uint32_t kp = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_k);
@@ -297,7 +299,21 @@ __device__ __forceinline__ void tiny_aggregate(const AggParams& p, const LdsTabl
 #pragma unroll
       for (int c = 0; c < NC; ++c) x[c][u] = nx[c][u];
     }
-    if (first + (int64_t)kRows * kLdsBlock < end) fetch(first + (int64_t)kRows * kLdsBlock, nkey, nx);
+    fetch(first + (int64_t)kRows * kLdsBlock, nkey, nx);
+    // the values' canonical form (integers widened, floats as raw bits)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c == 0 && TV != TV_ANY) {
+        if constexpr (TV == TV_I32) {
+#pragma unroll
+          for (int u = 0; u < kRows; ++u) x[c][u] = (uint64_t)(int64_t)(int32_t)(uint32_t)x[c][u];
+        }
+      } else {
+        const uint32_t vp = (uint32_t)__builtin_amdgcn_readfirstlane((int)pin_v[c]);
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) x[c][u] = widen_raw(x[c][u], (int)(vp & 15), (int)((vp >> 4) & 7));
+      }
+    }
     if ((kp >> 24) & 1u) {
 #pragma unroll
       for (int u = 0; u < kRows; ++u)

@@ -110,6 +110,17 @@ def test_more_rows_than_size_type_refused_before_device_work(rows):
         _lib.MGB_ERROR_UNSUPPORTED
 
 
+@pytest.mark.parametrize("which", ["key", "value"])
+def test_column_with_rows_and_no_data_is_a_logic_error(which):
+    # the kernels read row 0 of a column in place of an absent row, so a
+    # column with rows must have data (refused before any device work)
+    key = _col(3, 10, 0 if which == "key" else 0x1000)
+    val = _col(3, 10, 0 if which == "value" else 0x1000)
+    st, g, ok, orr, calls = _call([key], [(val, [0])], 1)
+    assert st == _lib.MGB_ERROR_LOGIC and calls == []
+    assert b"has no data" in _lib.load().mgb_last_error_message()
+
+
 def test_null_arguments_rejected():
     lib = _lib.load()
     g = ctypes.c_int64()

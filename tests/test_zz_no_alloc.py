@@ -166,7 +166,8 @@ def test_noalloc_limits(gpu):
     st, got = _noalloc(keys, vals, [agg.sum(), agg.count()], max_groups=5000)
     assert st == _lib.MGB_SUCCESS
     # more rows than the entry point takes: refused before any device work
-    big = ca.Column(ca.TypeId.INT64, (1 << 23) + 1, None)
+    n_big = (1 << 23) + 1
+    big = ca.Column(ca.TypeId.INT64, n_big, torch.empty(n_big * 8, dtype=torch.uint8, device=gpu))
     st, _ = _noalloc([big], [], [], max_groups=10)
     assert st == _lib.MGB_ERROR_UNSUPPORTED
     assert b"noalloc" in _lib.load().mgb_last_error_message()
