@@ -637,6 +637,29 @@ def summarize_pmc(fetch_csv: str, write_csv: str, cal_fetch_csv: str, cal_write_
     return "\n".join(out) + "\n"
 
 
+def summarize_counters(csv_paths, title: str, commands) -> str:
+    """Per-kernel mean of every counter of one or more rocprofv3 --pmc CSV
+    files (one per counter pass), for the library's kernels."""
+    import collections
+    import csv
+
+    acc = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in csv_paths:
+        for r in csv.DictReader(open(path)):
+            name = r["Kernel_Name"].split("(")[0].removeprefix("void ")
+            if name.startswith("mgb::"):
+                acc[name][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    counters = sorted({c for k in acc.values() for c in k})
+    out = [title, "Commands (from the repo root, TMPDIR=/tmp):"] + ["  " + c for c in commands] + [
+        "Mean value per dispatch of each counter (summed over the device's shader engines by rocprofv3).", ""]
+    for k in sorted(acc):
+        out.append(f"{k}  (dispatches: {max(len(v) for v in acc[k].values())})")
+        for c in counters:
+            if c in acc[k]:
+                out.append(f"    {c:28s} {statistics.mean(acc[k][c]):18.0f}")
+    return "\n".join(out) + "\n"
+
+
 def read_traffic(path: str, kernel: str):
     """HBM bytes per dispatch of `kernel` from a summary written by summarize_pmc, or None."""
     try:
@@ -703,6 +726,9 @@ def main():
                     metavar=("FETCH_CSV", "WRITE_CSV", "CAL_FETCH_CSV", "CAL_WRITE_CSV", "TITLE", "COMMANDS"),
                     help="print per-kernel HBM bytes from FETCH_SIZE / WRITE_SIZE passes and exit "
                          "(COMMANDS: the profiling commands, separated by ';;')")
+    ap.add_argument("--summarize-counters", nargs=3, metavar=("CSV_LIST", "TITLE", "COMMANDS"),
+                    help="print per-kernel means of the counters in rocprofv3 --pmc CSV files and exit "
+                         "(CSV_LIST and COMMANDS separated by ';;')")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "hbm_traffic_groupby_c2_100M_rows.txt"))
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB buffer 3 times (reference kernel for FETCH_SIZE / WRITE_SIZE scaling)")
@@ -713,6 +739,10 @@ def main():
     if args.summarize_pmc:
         a = args.summarize_pmc
         print(summarize_pmc(a[0], a[1], a[2], a[3], a[4], a[5].split(";;")), end="")
+        return
+    if args.summarize_counters:
+        a = args.summarize_counters
+        print(summarize_counters(a[0].split(";;"), a[1], a[2].split(";;")), end="")
         return
     if args.calibrate_copy:
         import torch

@@ -1297,6 +1297,23 @@ mgb_status upload_pass2(Workspace* ws, const Pass2Ops& q, hipStream_t stream, co
   return MGB_SUCCESS;
 }
 
+// Copies the LDS aggregation's parameters into a parameter slot of device
+// memory (the compact execution's kernels read them there).
+mgb_status upload_lds(Workspace* ws, const LdsParams& lp, hipStream_t stream, const LdsParams** d)
+{
+  const size_t slot_bytes = kStagingBytes / kUploadSlots;
+  static_assert(sizeof(LdsParams) <= kStagingBytes / kUploadSlots, "parameter slot too small");
+  if (ws->upload_slot >= kUploadSlots) return fail(MGB_ERROR_UNSUPPORTED, "too many parameter uploads in one call");
+  const int slot = ws->upload_slot++;
+  MGB_RESERVE(ws->params, kStagingBytes);
+  uint8_t* host = ws->staging + slot * slot_bytes;
+  uint8_t* dev = static_cast<uint8_t*>(ws->params.ptr) + slot * slot_bytes;
+  std::memcpy(host, &lp, sizeof(LdsParams));
+  MGB_HIP(hipMemcpyAsync(dev, host, sizeof(LdsParams), hipMemcpyHostToDevice, stream));
+  *d = reinterpret_cast<const LdsParams*>(dev);
+  return MGB_SUCCESS;
+}
+
 // Copies the few-groups parameters into a parameter slot of device memory.
 mgb_status upload_tiny(Workspace* ws, const TinyParams& tp, hipStream_t stream, const TinyParams** d)
 {
@@ -2259,6 +2276,60 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
 // of an item that outgrows its LDS table). Returns MGB_SUCCESS with `done`
 // false when the estimate asks for more partitions than supported (the
 // caller then uses the global path).
+// The compact partitioned execution of a call (lean.hip), when it has one: a
+// single 4- or 8-byte integer key, at most kLeanCols value columns of 4 or 8
+// bytes whose ops the column programs apply, no second pass or sort, at most
+// 4096 partitions, null masks starting at a word boundary.
+// MGB_LEAN=off turns the execution off.
+struct LeanPlan {
+  bool on = false;
+  int kw = 0;
+  int w[kLeanCols] = {};
+  LeanOffsets lo{};
+  int rec_bytes = 0;
+  int tile = 0;        // rows staged per scatter tile
+  size_t lds = 0;      // the scatter's LDS bytes
+  int flag_bits = 0;   // LeanScatterArgs::flag_bits
+};
+
+LeanPlan lean_plan(const Plan& plan, const AggParams& src, int log2p)
+{
+  LeanPlan lp;
+  const char* e = std::getenv("MGB_LEAN");
+  if (e && std::string(e) == "off") return lp;
+  if (log2p > 12 || plan.key_mode != KEY_INT || plan.keys.size() != 1 || plan.compare_key_nulls ||
+      plan.has_pass2 || plan.has_sort || src.num_vals > kLeanCols || !src.program_cols)
+    return lp;
+  const KeyCol& k = plan.keys[0];
+  if ((k.width != 4 && k.width != 8) || k.stride != k.width || (k.mask && (k.bit_off & 31))) return lp;
+  lp.kw = k.width;
+  lp.flag_bits = 0xfff;
+  for (int c = 0; c < src.num_vals; ++c) {
+    const ValCol& v = src.vals[c];
+    if (v.lds_generic || (v.width != 4 && v.width != 8) || v.stride != v.width || (v.mask && (v.bit_off & 31)) ||
+        v.flag_bit >= 8)
+      return lp;
+    lp.w[c] = v.width;
+    if (v.mask && v.flag_bit >= 0) lp.flag_bits = (lp.flag_bits & ~(15 << (4 * c))) | v.flag_bit << (4 * c);
+  }
+  if (!lean_supported(lp.kw, lp.w)) return lp;
+  lp.lo = lean_offsets(lp.kw, lp.w[0], lp.w[1], lp.w[2]);
+  for (int c = 0; c < src.num_vals; ++c)
+    if (lp.lo.off[c] < 0) return lp;
+  lp.rec_bytes = lp.lo.bytes <= 16 ? 16 : 32;
+  const int P = 1 << log2p;
+  const size_t fixed = align_up((size_t)(3 * P + 16) * 4, 16);
+  const size_t per_row = 4 + (size_t)lp.rec_bytes;
+  for (int tt = kScatterTile; tt >= kLdsBlock; tt -= kLdsBlock)
+    if (fixed + (size_t)tt * per_row <= lds_limit()) {
+      lp.tile = tt;
+      break;
+    }
+  lp.lds = fixed + (size_t)lp.tile * per_row;
+  lp.on = lp.tile > 0;
+  return lp;
+}
+
 mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& base, int64_t R, int64_t G_est,
                            const LdsGeom& geom, hipStream_t stream, TempOuts& temp, int64_t& groups, bool& done)
 {
@@ -2272,6 +2343,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
   const double m_target = std::max(geom.cap * 0.5, geom.cap - 4.5 * std::sqrt((double)geom.cap));
   int64_t est = std::max<int64_t>(G_est, 1);
   for (int attempt = 0; attempt < 4; ++attempt) {
+    const int slot0 = ws->upload_slot;  // parameter slots of an attempt are reused by the next
     int log2p = 1;
     while ((double)(1ll << log2p) * m_target < (double)est) ++log2p;
     if (log2p > 15) return MGB_SUCCESS;
@@ -2316,9 +2388,17 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
         if (src.vals[c].width == w) { voff[c] = rec; rec += w; }
       if (w == 4 && need_rowid) { rowid_off = rec; rec += 4; }
     }
-    const int flags_rec_off = (int)align_up((size_t)rec, (size_t)std::max(flag_bytes, 1));
+    int flags_rec_off = (int)align_up((size_t)rec, (size_t)std::max(flag_bytes, 1));
     rec = flags_rec_off + flag_bytes;
-    const int rec_bytes = (int)align_up((size_t)rec, 8);
+    int rec_bytes = (int)align_up((size_t)rec, 8);
+    // the compact records, when the call has them
+    const LeanPlan lean = lean_plan(plan, src, log2p);
+    if (lean.on) {
+      koff[0] = 0;
+      for (int c = 0; c < src.num_vals; ++c) voff[c] = lean.lo.off[c];
+      flags_rec_off = lean.kw;
+      rec_bytes = lean.rec_bytes;
+    }
 
     // workspace: records, hist, offsets, block sums, item counts/offsets, global table, counters
     size_t off = 0;
@@ -2411,7 +2491,27 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       if (fixed_lds + (size_t)tt * rec_bytes <= lds_limit()) { tile = tt; break; }
     pp.tile_rows = tile;
     const size_t tiled_lds = fixed_lds + (size_t)tile * rec_bytes;
-    if (tile > 0) {
+    if (lean.on) {
+      LeanScatterArgs sa{};
+      const KeyCol& kc = plan.keys[0];
+      sa.key = kc.data;
+      sa.key_mask = src.skip_null_keys && kc.mask ? kc.mask + (kc.bit_off >> 5) : nullptr;
+      for (int c = 0; c < src.num_vals; ++c) {
+        sa.val[c] = src.vals[c].data;
+        sa.mask[c] = src.vals[c].mask ? src.vals[c].mask + (src.vals[c].bit_off >> 5) : nullptr;
+      }
+      sa.offsets = pp.offsets;
+      sa.out = pp.out;
+      sa.rows = R;
+      sa.chunk = chunk;
+      sa.log2p = log2p;
+      sa.nblocks = (int32_t)nw;
+      sa.tile_rows = lean.tile;
+      sa.flag_bits = lean.flag_bits;
+      ProfScope prof("partition_scatter", stream);
+      st = lean_scatter(lean.kw, lean.w, sa, (unsigned)nw, lean.lds, stream);
+      if (st != MGB_SUCCESS) return st;
+    } else if (tile > 0) {
       MGB_HIP(hipFuncSetAttribute((const void*)part_scatter_tiled_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)tiled_lds));
       ProfScope prof("partition_scatter", stream);
@@ -2497,9 +2597,37 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       st = upload_pass2(ws, q, stream, &lp.p2);
       if (st != MGB_SUCCESS) return st;
     }
-    const void* agg_fn = plan.has_pass2 ? (const void*)lds_agg_kernel<true> : (const void*)lds_agg_kernel<false>;
-    MGB_HIP(hipFuncSetAttribute(agg_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)geom.bytes));
-    {
+    if (lean.on) {
+      const LdsParams* d_lp = nullptr;
+      st = upload_lds(ws, lp, stream, &d_lp);
+      if (st != MGB_SUCCESS) return st;
+      LeanAggArgs aa{};
+      aa.recs = recs;
+      aa.offsets = lp.offsets;
+      aa.item_offsets = lp.item_offsets;
+      aa.total_rows = lp.total_rows;
+      aa.tmpl = lp.tmpl;
+      aa.lp = d_lp;
+      aa.max_item_rows = lp.max_item_rows;
+      aa.nblocks = lp.nblocks;
+      aa.log2p = lp.log2p;
+      aa.slots = lp.slots;
+      aa.cap = lp.cap;
+      aa.words = lp.words;
+      aa.rows_off = lp.src.rows_op >= 0 ? lp.src.ops[lp.src.rows_op].off : -1;
+      aa.key_signed = plan.keys[0].vclass == VC_SIGNED;
+      for (int c = 0; c < src.num_vals; ++c) {
+        const ValCol& v = lp.src.vals[c];
+        aa.cols[c] = LeanCol{v.sum_off, v.min_off, v.max_off, v.nulls_off,
+                             v.vclass == VC_SIGNED ? LEAN_SIGNED : v.vclass == VC_UNSIGNED ? LEAN_UNSIGNED : LEAN_FLOAT,
+                             src.vals[c].mask && v.flag_bit >= 0 ? v.flag_bit : -1};
+      }
+      ProfScope prof("lds_aggregate", stream);
+      st = lean_aggregate(lean.kw, lean.w, aa, d_lp, (unsigned)item_grid, geom.bytes, stream);
+      if (st != MGB_SUCCESS) return st;
+    } else {
+      const void* agg_fn = plan.has_pass2 ? (const void*)lds_agg_kernel<true> : (const void*)lds_agg_kernel<false>;
+      MGB_HIP(hipFuncSetAttribute(agg_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)geom.bytes));
       ProfScope prof("lds_aggregate", stream);
       if (plan.has_pass2) lds_agg_kernel<true><<<(unsigned)item_grid, kLdsBlock, geom.bytes, stream>>>(lp);
       else lds_agg_kernel<false><<<(unsigned)item_grid, kLdsBlock, geom.bytes, stream>>>(lp);
@@ -2547,7 +2675,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       return MGB_SUCCESS;
     }
     est *= 4;  // the spill table overflowed: repartition finer
-    ws->upload_slot = std::max(0, ws->upload_slot - 1);
+    ws->upload_slot = slot0;
   }
   return MGB_SUCCESS;
 }

@@ -118,6 +118,95 @@ struct PartParams {
   int32_t pad = 0;
 };
 
+// Partitioned records of the compact partitioned execution (a single 4- or
+// 8-byte integer key; at most kLeanCols value columns of 4 or 8 bytes whose
+// ops are SUM / MIN / MAX and counts): the key at byte 0, the row hash at
+// byte `key width` with the validity bits of the nullable columns in its low
+// byte, and each value column at the lowest free offset aligned to its
+// width, widest first (lean_offsets); 16 or 32 bytes a record. The
+// aggregation takes the LDS slot from the stored hash instead of hashing the
+// key again. Widths and offsets are compile-time constants of the kernels.
+constexpr int kLeanCols = 3;
+
+struct LeanOffsets {
+  int off[kLeanCols];  // byte offset of each value column (-1: absent, or no room)
+  int bytes;           // bytes used
+};
+
+__host__ __device__ constexpr LeanOffsets lean_offsets(int kw, int w0, int w1, int w2)
+{
+  LeanOffsets r{{-1, -1, -1}, kw + 4};
+  const int ws[kLeanCols] = {w0, w1, w2};
+  uint32_t used = (1u << (kw + 4)) - 1u;
+  for (int w = 8; w >= 1; w >>= 1) {
+    for (int c = 0; c < kLeanCols; ++c) {
+      if (ws[c] != w) continue;
+      for (int o = 0; o + w <= 32; o += w) {
+        const uint32_t bits = ((1u << w) - 1u) << o;
+        if ((used & bits) == 0) {
+          used |= bits;
+          r.off[c] = o;
+          if (o + w > r.bytes) r.bytes = o + w;
+          break;
+        }
+      }
+    }
+  }
+  return r;
+}
+
+// Arguments of the compact scatter. Masks point at the word holding row 0's
+// bit (the columns' bit offsets are multiples of 32).
+struct LeanScatterArgs {
+  const uint8_t* key;               // key column, 4 or 8 bytes a row
+  const uint32_t* key_mask;         // rows whose key is null are dropped; nullptr: none dropped
+  const uint8_t* val[kLeanCols];    // value columns
+  const uint32_t* mask[kLeanCols];  // their null masks, nullptr: never null
+  const uint32_t* offsets;          // [P][nblocks]: first destination row of partition p for chunk b
+  uint8_t* out;                     // the records
+  int64_t rows;
+  int64_t chunk;                    // rows per workgroup
+  int32_t log2p;
+  int32_t nblocks;
+  int32_t tile_rows;                // rows staged per tile: a multiple of 1024, at most 4096
+  int32_t flag_bits;                // bits 4c..4c+3: column c's validity bit in the hash word (15: never null)
+};
+
+struct LdsParams;
+
+// A value column's updates in the compact aggregation: LDS record field
+// offsets (-1: none) of its SUM, MIN, MAX and null count, its class and its
+// validity bit in the hash word (-1: never null).
+enum LeanClass : int32_t { LEAN_SIGNED = 0, LEAN_UNSIGNED = 1, LEAN_FLOAT = 2 };
+struct LeanCol {
+  int32_t sum_off;
+  int32_t min_off;
+  int32_t max_off;
+  int32_t nulls_off;
+  int32_t cls;       // LeanClass: f32 / f64 by the column's width
+  int32_t flag_bit;
+};
+
+// Arguments of the compact aggregation (one work item per workgroup).
+struct LeanAggArgs {
+  const uint8_t* recs;
+  const uint32_t* offsets;          // as LdsParams::offsets
+  const uint32_t* item_offsets;     // as LdsParams::item_offsets
+  const unsigned long long* total_rows;
+  const uint64_t* tmpl;             // identity record (device memory)
+  const LdsParams* lp;              // every parameter (device memory): spilled rows, writing or folding the groups
+  int64_t max_item_rows;
+  int32_t nblocks;
+  int32_t log2p;
+  int32_t slots;
+  int32_t cap;
+  int32_t words;
+  int32_t rows_off;                 // row-count field, or -1
+  int32_t key_signed;               // 4-byte keys: sign-extended into the header
+  int32_t pad;
+  LeanCol cols[kLeanCols];
+};
+
 // Second pass over the rows (VARIANCE / STD / ARGMIN / ARGMAX): every row
 // finds its group's record in a completed table (LDS or global) and updates
 // the second-pass fields from the first pass's SUM / MIN / MAX fields.

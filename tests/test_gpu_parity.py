@@ -46,14 +46,19 @@ def _random_case(rng, n, key_type, card, val_types, null_frac, kinds, key_null_f
     return [(key_type, keys, kvalid)], reqs
 
 
-PATHS = ["auto", "global", "direct", "partitioned"]
+PATHS = ["auto", "global", "direct", "partitioned", "partitioned-general"]
 
 
 @pytest.fixture(params=PATHS)
 def path(request, monkeypatch):
     """Runs a test once per execution of the groupby (the library reads
-    MGB_GROUPBY_PATH on every call; "auto" picks by the cardinality estimate)."""
-    if request.param != "auto":
+    MGB_GROUPBY_PATH on every call; "auto" picks by the cardinality estimate;
+    "partitioned-general" turns the compact records off, MGB_LEAN=off)."""
+    monkeypatch.delenv("MGB_LEAN", raising=False)
+    if request.param == "partitioned-general":
+        monkeypatch.setenv("MGB_GROUPBY_PATH", "partitioned")
+        monkeypatch.setenv("MGB_LEAN", "off")
+    elif request.param != "auto":
         monkeypatch.setenv("MGB_GROUPBY_PATH", request.param)
     else:
         monkeypatch.delenv("MGB_GROUPBY_PATH", raising=False)
