@@ -103,7 +103,7 @@ def zipf_ranks(n_rows: int, n_keys: int, s: float, g, device: str = "cuda"):
     return torch.clamp(x.floor().to(torch.int64) - 1, 0, n_keys - 1)
 
 
-def make_workload(name: str, rows: int, seed: int, device: str = "cuda"):
+def make_workload(name: str, rows: int, seed: int, device: str = "cuda", distinct: int = 1_000_000):
     """(key columns, requests, input bytes per row, output bytes per group, description)."""
     import torch
 
@@ -113,13 +113,13 @@ def make_workload(name: str, rows: int, seed: int, device: str = "cuda"):
     g = torch.Generator(device=device)
     g.manual_seed(seed)
     if name == "c2":
-        key_col, cols, distinct = make_c2(rows, 1_000_000, seed, device)
+        key_col, cols, distinct = make_c2(rows, distinct, seed, device)
         return ([key_col], c2_requests(cols), 8 + 4 + 8 + 4 + 3 / 8, 96 + 12 / 8,
                 f"groupby-aggregate, {rows:,} rows, int64 key over {distinct:,} distinct values, int32/f64/f32 "
                 "values 5% null, sum/count/min/max/mean per value column")
     if name == "c2v":
         # the headline data with the second-pass aggregations on every value column
-        key_col, cols, distinct = make_c2(rows, 1_000_000, seed, device)
+        key_col, cols, distinct = make_c2(rows, distinct, seed, device)
         req = [ca.GroupByRequest(c, [agg.mean(), agg.variance(), agg.std(), agg.argmin(), agg.argmax()])
                for c in cols]
         return ([key_col], req, 8 + 4 + 8 + 4 + 3 / 8, 8 + 3 * (8 + 8 + 8 + 4 + 4) + 15 / 8,
@@ -127,7 +127,7 @@ def make_workload(name: str, rows: int, seed: int, device: str = "cuda"):
                 "values 5% null, mean/variance/std/argmin/argmax per value column")
     if name == "c2s":
         # the headline data with aggregations cudf serves from its sort path
-        key_col, cols, distinct = make_c2(rows, 1_000_000, seed, device)
+        key_col, cols, distinct = make_c2(rows, distinct, seed, device)
         req = [ca.GroupByRequest(c, [agg.median(), agg.quantile(0.9), agg.nunique(), agg.nth_element(0)])
                for c in cols]
         return ([key_col], req, 8 + 4 + 8 + 4 + 3 / 8, 8 + 3 * (8 + 8 + 4) + 4 + 8 + 4 + 15 / 8,
@@ -811,7 +811,7 @@ def main():
     rows = args.rows or DEFAULT_ROWS[args.workload]
     args.rows = rows
     log(f"generating workload {args.workload} ({rows:,} rows)")
-    key_cols, reqs, in_bytes_per_row, out_bytes_per_group, description = make_workload(args.workload, rows, args.seed)
+    key_cols, reqs, in_bytes_per_row, out_bytes_per_group, description = make_workload(args.workload, rows, args.seed, distinct=args.distinct)
     torch.cuda.synchronize()
     gb = ca.GroupBy(ca.Table(key_cols))
     groups = 0

@@ -369,12 +369,63 @@ constexpr int kHllRegs = 1 << kHllBits;
 
 // Per-workgroup registers: register = top 14 bits of the row hash, value =
 // leading zeros + 1 of a bijective remix of the hash. The same pass counts
-// each chunk's rows per fine bucket (top kFineBits bits of the hash):
-// partitions of up to 2^kFineBits take their histogram from these counts
-// (hist_coarsen_kernel) instead of a second pass over the keys. Workgroup b
-// covers rows [b * chunk, +chunk), the chunks of the partitioned execution.
+// each chunk's rows per fine bucket (top kFineBits bits of the hash),
+// fine[chunk * kFineBuckets + bucket]: partitions of up to 2^kFineBits take
+// their histogram from these counts (hist_coarsen_kernel) instead of a
+// second pass over the keys. Workgroup b covers rows [b * chunk, +chunk), the
+// chunks of the partitioned execution.
 constexpr int kFineBits = 12;
 constexpr int kFineBuckets = 1 << kFineBits;
+
+// One row's hash into the registers and the fine counts.
+__device__ __forceinline__ void hll_add(uint32_t* regs, uint32_t* counts, uint32_t h)
+{
+  const uint32_t rank = (uint32_t)__clz(fmix32(h ^ 0x9e3779b9u)) + 1u;
+  uint32_t* r = &regs[h >> (32 - kHllBits)];
+  // registers saturate quickly: most rows only read
+  if (rank > __hip_atomic_load(r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMax(r, rank);
+  atomicAdd(&counts[h >> (32 - kFineBits)], 1u);
+}
+
+// The rows of a single key column of W bytes (4 or 8) without null
+// grouping: kItems rows per thread per step, the next step's keys (and
+// null-mask words) loaded before the current ones are hashed.
+template <int W>
+__device__ __forceinline__ void hll_rows_single(const AggParams& p, uint32_t* regs, uint32_t* counts, int64_t begin,
+                                                int64_t end)
+{
+  constexpr int kItems = 8;
+  const KeyCol& k = p.keys[0];
+  const bool nulls = p.skip_null_keys && k.mask;
+  uint64_t cur[kItems], nxt[kItems];
+  uint32_t cm[kItems], nm[kItems];
+  auto load = [&](int64_t row0, uint64_t (&kv)[kItems], uint32_t (&mv)[kItems]) {
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+      int64_t r = row0 + (int64_t)u * kLdsBlock;
+      r = r < end ? r : end - 1;
+      if constexpr (W == 8) kv[u] = to_global(reinterpret_cast<const uint64_t*>(k.data))[r];
+      else kv[u] = to_global(reinterpret_cast<const uint32_t*>(k.data))[r];
+      mv[u] = nulls ? to_global(k.mask)[(k.bit_off + r) >> 5] : ~0u;
+    }
+  };
+  if (begin < end) load(begin + threadIdx.x, cur, cm);
+  for (int64_t row0 = begin + threadIdx.x; row0 < end; row0 += (int64_t)kItems * kLdsBlock) {
+    load(row0 + (int64_t)kItems * kLdsBlock, nxt, nm);
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+      const int64_t r = row0 + (int64_t)u * kLdsBlock;
+      if (r < end && ((cm[u] >> ((k.bit_off + r) & 31)) & 1u))
+        hll_add(regs, counts, hash_combine(0u, murmur3_bits(normalize_float_bits(cur[u], k.vclass), W, 0)));
+    }
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+      cur[u] = nxt[u];
+      cm[u] = nm[u];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kLdsBlock) void hll_hist_kernel(const AggParams p, uint8_t* block_regs, uint32_t* fine,
                                                             int64_t chunk, int32_t nw)
 {
@@ -384,45 +435,58 @@ __global__ __launch_bounds__(kLdsBlock) void hll_hist_kernel(const AggParams p, 
   __syncthreads();
   const int64_t begin = (int64_t)blockIdx.x * chunk;
   const int64_t end = begin + chunk < p.num_rows ? begin + chunk : p.num_rows;
-  constexpr int kItems = 4;  // rows per thread per step, hashed column by column
-  for (int64_t row0 = begin + threadIdx.x; row0 < end; row0 += (int64_t)kItems * kLdsBlock) {
-    bool present[kItems];
+  const bool single = p.num_keys == 1 && !(p.key_mode != KEY_INT && p.compare_key_nulls) && p.keys[0].stride == p.keys[0].width;
+  if (single && p.keys[0].width == 8) {
+    hll_rows_single<8>(p, regs, counts, begin, end);
+  } else if (single && p.keys[0].width == 4) {
+    hll_rows_single<4>(p, regs, counts, begin, end);
+  } else {
+    constexpr int kItems = 4;  // rows per thread per step, hashed column by column
+    for (int64_t row0 = begin + threadIdx.x; row0 < end; row0 += (int64_t)kItems * kLdsBlock) {
+      bool present[kItems];
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-      present[k] = row0 + k * kLdsBlock < end;
-      if (present[k] && p.skip_null_keys) present[k] = key_row_valid(p, row0 + k * kLdsBlock);
-    }
-    uint32_t h[kItems];
-    uint64_t key0[kItems];
-    hash_rows(p, row0, kLdsBlock, present, h, key0);
+      for (int k = 0; k < kItems; ++k) {
+        present[k] = row0 + k * kLdsBlock < end;
+        if (present[k] && p.skip_null_keys) present[k] = key_row_valid(p, row0 + k * kLdsBlock);
+      }
+      uint32_t h[kItems];
+      uint64_t key0[kItems];
+      hash_rows(p, row0, kLdsBlock, present, h, key0);
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-      if (!present[k]) continue;
-      const uint32_t rank = (uint32_t)__clz(fmix32(h[k] ^ 0x9e3779b9u)) + 1u;
-      uint32_t* r = &regs[h[k] >> (32 - kHllBits)];
-      // registers saturate quickly: most rows only read
-      if (rank > __hip_atomic_load(r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMax(r, rank);
-      atomicAdd(&counts[h[k] >> (32 - kFineBits)], 1u);
+      for (int k = 0; k < kItems; ++k)
+        if (present[k]) hll_add(regs, counts, h[k]);
     }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kHllRegs; i += kLdsBlock)
     block_regs[(int64_t)blockIdx.x * kHllRegs + i] = (uint8_t)regs[i];
-  for (int i = threadIdx.x; i < kFineBuckets; i += kLdsBlock) fine[(int64_t)i * nw + blockIdx.x] = counts[i];
+  for (int i = threadIdx.x; i < kFineBuckets; i += kLdsBlock) fine[(int64_t)blockIdx.x * kFineBuckets + i] = counts[i];
 }
 
 // Histogram of 2^log2p partitions (log2p <= kFineBits) from the fine bucket
-// counts: hist[p * nw + c] = sum of the partition's buckets in chunk c.
+// counts: hist[p * nw + c] = sum of the partition's buckets in chunk c. A
+// workgroup sums a tile of 64 chunks by 64 partitions (each chunk's buckets
+// read contiguously) and writes it transposed through LDS.
 __global__ __launch_bounds__(kBlock) void hist_coarsen_kernel(const uint32_t* fine, int32_t nw, int32_t log2p,
                                                              uint32_t* hist)
 {
-  const int64_t n = ((int64_t)1 << log2p) * nw;
-  const int per = 1 << (kFineBits - log2p);
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const int64_t part = i / nw, c = i - part * nw;
+  __shared__ uint32_t tile[64][65];
+  const int P = 1 << log2p;
+  const int per = kFineBuckets >> log2p;
+  const int c0 = blockIdx.x * 64, p0 = blockIdx.y * 64;
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += kBlock) {
+    const int ci = idx >> 6, pi = idx & 63;
+    const int c = c0 + ci, part = p0 + pi;
     uint32_t sum = 0;
-    for (int j = 0; j < per; ++j) sum += fine[(part * per + j) * nw + c];
-    hist[i] = sum;
+    if (c < nw && part < P)
+      for (int j = 0; j < per; ++j) sum += fine[(int64_t)c * kFineBuckets + part * per + j];
+    tile[pi][ci] = sum;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += kBlock) {
+    const int pi = idx >> 6, ci = idx & 63;
+    const int c = c0 + ci, part = p0 + pi;
+    if (c < nw && part < P) hist[(int64_t)part * nw + c] = tile[pi][ci];
   }
 }
 
@@ -2318,14 +2382,12 @@ LeanPlan lean_plan(const Plan& plan, const AggParams& src, int log2p)
     if (lp.lo.off[c] < 0) return lp;
   lp.rec_bytes = lp.lo.bytes <= 16 ? 16 : 32;
   const int P = 1 << log2p;
-  const size_t fixed = align_up((size_t)(3 * P + 16) * 4, 16);
-  const size_t per_row = 4 + (size_t)lp.rec_bytes;
   for (int tt = kScatterTile; tt >= kLdsBlock; tt -= kLdsBlock)
-    if (fixed + (size_t)tt * per_row <= lds_limit()) {
+    if (lean_scatter_lds(P, tt, lp.rec_bytes).total <= lds_limit()) {
       lp.tile = tt;
       break;
     }
-  lp.lds = fixed + (size_t)lp.tile * per_row;
+  lp.lds = lean_scatter_lds(P, lp.tile, lp.rec_bytes).total;
   lp.on = lp.tile > 0;
   return lp;
 }
@@ -2340,7 +2402,9 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
   int nullable = 0;
   for (int c = 0; c < src.num_vals; ++c) nullable += src.vals[c].flag_bit >= 0;
   const int flag_bytes = nullable == 0 ? 0 : nullable <= 8 ? 1 : nullable <= 16 ? 2 : 4;
-  const double m_target = std::max(geom.cap * 0.5, geom.cap - 4.5 * std::sqrt((double)geom.cap));
+  // groups per partition: the table's capacity less four standard deviations
+  // of a partition's group count (a partition over it spills the rest)
+  const double m_target = std::max(geom.cap * 0.5, geom.cap - 4.0 * std::sqrt((double)geom.cap));
   int64_t est = std::max<int64_t>(G_est, 1);
   for (int attempt = 0; attempt < 4; ++attempt) {
     const int slot0 = ws->upload_slot;  // parameter slots of an attempt are reused by the next
@@ -2470,7 +2534,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     if (ws->fine_valid && log2p <= kFineBits && ws->fine_nw == nw && ws->fine_chunk == chunk) {
       // the estimate's pass counted the rows per fine bucket of these chunks
       ProfScope prof("partition_hist", stream);
-      const unsigned grid = (unsigned)std::min<int64_t>((nh + kBlock - 1) / kBlock, (int64_t)cus * 8);
+      const dim3 grid((unsigned)((nw + 63) / 64), (unsigned)((P + 63) / 64));
       hist_coarsen_kernel<<<grid, kBlock, 0, stream>>>(static_cast<const uint32_t*>(ws->fine.ptr), (int32_t)nw, log2p,
                                                       pp.hist);
     } else {

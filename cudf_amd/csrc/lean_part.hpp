@@ -32,6 +32,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // 16 bytes of a re
 // held in registers: the current step's and the next step's)
 template <int RB16>
 constexpr int kLeanRows = RB16 == 1 ? 4 : 3;
+constexpr int kLeanProbe = 4;  // slots of a row's probe read before the full probe
 
 // Exclusive scan of the P counts `cnt` (P a power of two, at most
 // 4 * kLdsBlock) into `start`; returns the total. `tmp`: 16 words of LDS.
@@ -75,8 +76,9 @@ __device__ __forceinline__ void lean_put(uint32_t (&w)[8], uint64_t v)
 }
 
 // Scatter of chunk blockIdx.x (rows [chunk * a.chunk, +a.chunk)) into
-// records. LDS: cursor, tile count and tile start of every partition, 16
-// scan words, the destination row of every staged record, the stage.
+// records. LDS (lean_scatter_lds): cursor, tile count and tile start of every
+// partition, 16 scan words, the destination row of every staged record, the
+// stage.
 template <int KW, int W0, int W1, int W2>
 __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint8_t* smem)
 {
@@ -90,8 +92,9 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
   uint32_t* tcount = cursor + P;
   uint32_t* tstart = tcount + P;
   uint32_t* tmp = tstart + P;
-  uint32_t* dstrow = tmp + 16;
-  u32x4* stage = reinterpret_cast<u32x4*>(smem + (((3 * P + 16 + T) * 4 + 15) & ~15));
+  const LeanScatterLds lay = lean_scatter_lds(P, T, RB16 * 16);
+  uint32_t* dstrow = reinterpret_cast<uint32_t*>(smem + lay.dst);
+  u32x4* stage = reinterpret_cast<u32x4*>(smem + lay.stage);
   const int64_t chunk = blockIdx.x;
   for (int i = threadIdx.x; i < P; i += kLdsBlock) {
     cursor[i] = a.offsets[(int64_t)i * a.nblocks + chunk];
@@ -396,8 +399,8 @@ __device__ __forceinline__ void lean_aggregate_rows(const LeanAggArgs& a, const 
     asm volatile("" : "+s"(rows_off), "+s"(log2p));
     load(base + (int64_t)R * kLdsBlock, nxt);
     uint64_t kk[R];
-    uint32_t hf[R], sh[R], loff[R], i0[R], i1[R];
-    uint64_t h0[R], h1[R];
+    uint32_t hf[R], sh[R], loff[R], i0[R];
+    uint64_t hv[R][kLeanProbe];
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       if constexpr (KW == 8) kk[u] = (uint64_t)cur[u][0][0] | (uint64_t)cur[u][0][1] << 32;
@@ -405,22 +408,30 @@ __device__ __forceinline__ void lean_aggregate_rows(const LeanAggArgs& a, const 
       hf[u] = cur[u][0][KW / 4];
       sh[u] = (hf[u] & 0xffffff00u) << log2p;
       i0[u] = (uint32_t)reduce_range(sh[u], t.slots);
-      i1[u] = i0[u] + 1 == (uint32_t)t.slots ? 0u : i0[u] + 1;
-      h0[u] = __hip_atomic_load(reinterpret_cast<unsigned long long*>(t.rec + i0[u] * stride), __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_WORKGROUP);
-      h1[u] = __hip_atomic_load(reinterpret_cast<unsigned long long*>(t.rec + i1[u] * stride), __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_WORKGROUP);
+      uint32_t i = i0[u];
+#pragma unroll
+      for (int j = 0; j < kLeanProbe; ++j) {
+        hv[u][j] = __hip_atomic_load(reinterpret_cast<unsigned long long*>(t.rec + i * stride), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+        i = i + 1 == (uint32_t)t.slots ? 0u : i + 1;
+      }
     }
     uint32_t miss = 0;
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const bool present = base + (int64_t)u * kLdsBlock < end;
-      const bool hit0 = h0[u] == kk[u], hit1 = h0[u] != kEmptyIntKey && h1[u] == kk[u];
-      loff[u] = !present                ? kRecNone
-                : kk[u] == kEmptyIntKey ? kRecSpill
-                : hit0                  ? i0[u] * stride
-                : hit1                  ? i1[u] * stride
-                                        : kRecSpill;
+      // the key's record among the first kLeanProbe slots of its probe
+      // sequence, before any empty one (the full probe otherwise)
+      uint32_t found = kRecSpill;
+      bool open = true;
+      uint32_t i = i0[u];
+#pragma unroll
+      for (int j = 0; j < kLeanProbe; ++j) {
+        if (open && found == kRecSpill && hv[u][j] == kk[u]) found = i * stride;
+        open = open && hv[u][j] != kEmptyIntKey;
+        i = i + 1 == (uint32_t)t.slots ? 0u : i + 1;
+      }
+      loff[u] = !present ? kRecNone : kk[u] == kEmptyIntKey ? kRecSpill : found;
       if (present && loff[u] == kRecSpill) miss |= 1u << u;
     }
 #pragma unroll 1
