@@ -336,30 +336,55 @@ __global__ __launch_bounds__(kBlock) void sample_clear_kernel(unsigned long long
   }
 }
 
-__global__ __launch_bounds__(kBlock) void sample_stats_kernel(const uint32_t* cnt, int64_t n, unsigned long long* stats)
+// The sample table's summary: distinct keys (occupied slots, the special
+// record included: n = scap + 1 counters), keys seen once and twice, into
+// stats[0..2]; with `keys`, the distinct integer keys (slot headers), up to
+// kTinyKeys of them, into keys[0..] and their number into *count. Each
+// workgroup strides over the counters, reduces its counts and keeps its
+// first kTinyKeys keys in LDS, then adds to the totals once (the same
+// counters written by every workgroup are the slow atomics here).
+constexpr int kSampleSummaryBlocks = 64;
+__global__ __launch_bounds__(kBlock) void sample_summary_kernel(const unsigned long long* hdr, const uint32_t* cnt,
+                                                               int64_t scap, unsigned long long* stats,
+                                                               unsigned long long* count, unsigned long long* keys)
 {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  uint32_t c = i < n ? cnt[i] : 0u;
-  int d = __syncthreads_count(c > 0);
-  int f1 = __syncthreads_count(c == 1);
-  int f2 = __syncthreads_count(c == 2);
-  if (threadIdx.x == 0) {
-    if (d) atom_add(&stats[0], (unsigned long long)d);
-    if (f1) atom_add(&stats[1], (unsigned long long)f1);
-    if (f2) atom_add(&stats[2], (unsigned long long)f2);
+  __shared__ uint32_t red[3][kBlock / 64];
+  __shared__ uint32_t nkeys;
+  __shared__ unsigned long long kbuf[kTinyKeys];
+  if (threadIdx.x == 0) nkeys = 0;
+  __syncthreads();
+  uint32_t d = 0, f1 = 0, f2 = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i <= scap; i += (int64_t)gridDim.x * kBlock) {
+    const uint32_t c = cnt[i];
+    d += c > 0;
+    f1 += c == 1;
+    f2 += c == 2;
+    if (keys && c > 0 && i < scap) {
+      const uint32_t j = atomicAdd(&nkeys, 1u);
+      if (j < (uint32_t)kTinyKeys) kbuf[j] = hdr[i];
+    }
   }
-}
-
-// The sample's distinct keys (integer keys: the slot headers), up to
-// kTinyKeys of them, into keys[0..]; *count: how many there are.
-__global__ __launch_bounds__(kBlock) void sample_keys_kernel(const unsigned long long* hdr, const uint32_t* cnt,
-                                                             int64_t scap, unsigned long long* count,
-                                                             unsigned long long* keys)
-{
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= scap || cnt[i] == 0) return;
-  const unsigned long long j = atomicAdd(count, 1ull);
-  if (j < (unsigned long long)kTinyKeys) keys[j] = hdr[i];
+  for (int o = 32; o > 0; o >>= 1) {
+    d += __shfl_down(d, o, 64);
+    f1 += __shfl_down(f1, o, 64);
+    f2 += __shfl_down(f2, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = d;
+    red[1][threadIdx.x >> 6] = f1;
+    red[2][threadIdx.x >> 6] = f2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    unsigned long long v = 0;
+    for (int w = 0; w < kBlock / 64; ++w) v += red[threadIdx.x][w];
+    if (v) atom_add(&stats[threadIdx.x], v);
+  }
+  if (keys && threadIdx.x == 0 && nkeys) {
+    const unsigned long long base = atomicAdd(count, (unsigned long long)nkeys);
+    const uint32_t kept = nkeys < (uint32_t)kTinyKeys ? nkeys : (uint32_t)kTinyKeys;
+    for (uint32_t j = 0; j < kept && base + j < (unsigned long long)kTinyKeys; ++j) keys[base + j] = kbuf[j];
+  }
 }
 
 // ---- cardinality estimate: HyperLogLog over every row's key hash
@@ -426,41 +451,59 @@ __device__ __forceinline__ void hll_rows_single(const AggParams& p, uint32_t* re
   }
 }
 
-__global__ __launch_bounds__(kLdsBlock) void hll_hist_kernel(const AggParams p, uint8_t* block_regs, uint32_t* fine,
-                                                            int64_t chunk, int32_t nw)
+// Whether a sample (SampleParams::stats) holds every group of its call: every
+// sampled key seen at least twice, and few distinct keys. The estimate pass
+// is then not needed; it is launched beside the sample and returns at once.
+__device__ __forceinline__ bool sample_saturated(const unsigned long long* stats)
 {
+  return stats && stats[1] == 0 && stats[0] * 8 <= stats[3];
+}
+
+// Workgroup w takes chunks w, w + gridDim.x, ... (nw chunks of `chunk`
+// rows): each chunk's fine counts are written when it is done, the
+// registers once at the end (block_regs[w]).
+__global__ __launch_bounds__(kLdsBlock) void hll_hist_kernel(const AggParams p, uint8_t* block_regs, uint32_t* fine,
+                                                            int64_t chunk, int32_t nw, const unsigned long long* gate)
+{
+  if (sample_saturated(gate)) return;
   extern __shared__ uint32_t regs[];  // kHllRegs registers, then kFineBuckets counts
   uint32_t* counts = regs + kHllRegs;
   for (int i = threadIdx.x; i < kHllRegs + kFineBuckets; i += kLdsBlock) regs[i] = 0;
   __syncthreads();
-  const int64_t begin = (int64_t)blockIdx.x * chunk;
-  const int64_t end = begin + chunk < p.num_rows ? begin + chunk : p.num_rows;
   const bool single = p.num_keys == 1 && !(p.key_mode != KEY_INT && p.compare_key_nulls) && p.keys[0].stride == p.keys[0].width;
-  if (single && p.keys[0].width == 8) {
-    hll_rows_single<8>(p, regs, counts, begin, end);
-  } else if (single && p.keys[0].width == 4) {
-    hll_rows_single<4>(p, regs, counts, begin, end);
-  } else {
-    constexpr int kItems = 4;  // rows per thread per step, hashed column by column
-    for (int64_t row0 = begin + threadIdx.x; row0 < end; row0 += (int64_t)kItems * kLdsBlock) {
-      bool present[kItems];
+  for (int c = blockIdx.x; c < nw; c += gridDim.x) {
+    const int64_t begin = (int64_t)c * chunk;
+    const int64_t end = begin + chunk < p.num_rows ? begin + chunk : p.num_rows;
+    if (single && p.keys[0].width == 8) {
+      hll_rows_single<8>(p, regs, counts, begin, end);
+    } else if (single && p.keys[0].width == 4) {
+      hll_rows_single<4>(p, regs, counts, begin, end);
+    } else {
+      constexpr int kItems = 4;  // rows per thread per step, hashed column by column
+      for (int64_t row0 = begin + threadIdx.x; row0 < end; row0 += (int64_t)kItems * kLdsBlock) {
+        bool present[kItems];
 #pragma unroll
-      for (int k = 0; k < kItems; ++k) {
-        present[k] = row0 + k * kLdsBlock < end;
-        if (present[k] && p.skip_null_keys) present[k] = key_row_valid(p, row0 + k * kLdsBlock);
+        for (int k = 0; k < kItems; ++k) {
+          present[k] = row0 + k * kLdsBlock < end;
+          if (present[k] && p.skip_null_keys) present[k] = key_row_valid(p, row0 + k * kLdsBlock);
+        }
+        uint32_t h[kItems];
+        uint64_t key0[kItems];
+        hash_rows(p, row0, kLdsBlock, present, h, key0);
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+          if (present[k]) hll_add(regs, counts, h[k]);
       }
-      uint32_t h[kItems];
-      uint64_t key0[kItems];
-      hash_rows(p, row0, kLdsBlock, present, h, key0);
-#pragma unroll
-      for (int k = 0; k < kItems; ++k)
-        if (present[k]) hll_add(regs, counts, h[k]);
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kFineBuckets; i += kLdsBlock) {
+      fine[(int64_t)c * kFineBuckets + i] = counts[i];
+      counts[i] = 0;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   for (int i = threadIdx.x; i < kHllRegs; i += kLdsBlock)
     block_regs[(int64_t)blockIdx.x * kHllRegs + i] = (uint8_t)regs[i];
-  for (int i = threadIdx.x; i < kFineBuckets; i += kLdsBlock) fine[(int64_t)blockIdx.x * kFineBuckets + i] = counts[i];
 }
 
 // Histogram of 2^log2p partitions (log2p <= kFineBits) from the fine bucket
@@ -496,8 +539,9 @@ __global__ __launch_bounds__(kBlock) void hist_coarsen_kernel(const uint32_t* fi
 // 32nd workgroup's words, and the 32 partial maxima meet in LDS.
 constexpr int kHllReduceRegs = 128;
 __global__ __launch_bounds__(1024) void hll_reduce_kernel(const uint8_t* block_regs, int nb, double* sums,
-                                                         uint32_t* zeros)
+                                                         uint32_t* zeros, const unsigned long long* gate)
 {
+  if (sample_saturated(gate)) return;
   __shared__ uint32_t part[32][32][4];
   __shared__ double s[32];
   __shared__ uint32_t z[32];
@@ -1649,6 +1693,23 @@ int lds_stride_of(int stride)
   return words * 8;
 }
 
+// Raises a kernel's dynamic LDS limit to `bytes` when it is below: the
+// attribute is set once per kernel, device and size, not at every launch.
+hipError_t set_dynamic_lds(const void* fn, size_t bytes)
+{
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, size_t> done;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& cur = done[{dev, fn}];
+  if (bytes <= cur) return hipSuccess;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) cur = bytes;
+  return e;
+}
+
 LdsGeom lds_geometry(int stride)
 {
   LdsGeom g;
@@ -1686,64 +1747,6 @@ int64_t partition_chunks(int64_t R, int log2p, int cus)
   return std::max<int64_t>(1, std::min<int64_t>(nw, (R + 4095) / 4096));
 }
 
-// HyperLogLog estimate of the number of distinct key rows (standard error
-// ~1.04/sqrt(2^14) = 0.8%), with linear counting for small cardinalities.
-int64_t hll_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_t stream, mgb_status& st)
-{
-  st = MGB_SUCCESS;
-  const int cus = device_cu_count();
-  // one workgroup per chunk of the partitioned execution (partition_chunks)
-  const int64_t nb = partition_chunks(R, kFineBits, cus);
-  const int64_t chunk = (R + nb - 1) / nb;
-  const int nred = kHllRegs / kHllReduceRegs;
-  const size_t regs_bytes = align_up((size_t)nb * kHllRegs, 256);
-  const size_t need = regs_bytes + align_up((size_t)nred * 8, 256) + align_up((size_t)nred * 4, 256);
-  const size_t lds = (size_t)(kHllRegs + kFineBuckets) * 4;
-  if (ws->main.reserve(need, stream) != hipSuccess ||
-      ws->fine.reserve((size_t)kFineBuckets * nb * 4, stream) != hipSuccess) {
-    st = fail(MGB_ERROR_ALLOC, "allocation of the cardinality estimate's temporaries failed");
-    return R;
-  }
-  hipError_t e = hipFuncSetAttribute((const void*)hll_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) {
-    st = fail(MGB_ERROR_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-    return R;
-  }
-  uint8_t* regs = static_cast<uint8_t*>(ws->main.ptr);
-  double* sums = reinterpret_cast<double*>(regs + regs_bytes);
-  uint32_t* zeros = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(sums) + align_up((size_t)nred * 8, 256));
-  {
-    ProfScope prof("estimate", stream);
-    hll_hist_kernel<<<(unsigned)nb, kLdsBlock, lds, stream>>>(base, regs, static_cast<uint32_t*>(ws->fine.ptr), chunk,
-                                                              (int32_t)nb);
-    hll_reduce_kernel<<<(unsigned)nred, 1024, 0, stream>>>(regs, (int)nb, sums, zeros);
-  }
-  ws->fine_valid = true;
-  ws->fine_nw = nb;
-  ws->fine_chunk = chunk;
-  e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(ws->pinned, sums, (size_t)nred * 8, hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(ws->pinned + nred, zeros, (size_t)nred * 4, hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  if (e != hipSuccess) {
-    st = fail(MGB_ERROR_HIP, std::string("cardinality estimate: ") + hipGetErrorString(e));
-    return R;
-  }
-  double sum = 0.0;
-  uint64_t z = 0;
-  const double* hs = reinterpret_cast<const double*>(ws->pinned);
-  const uint32_t* hz = reinterpret_cast<const uint32_t*>(ws->pinned + nred);
-  for (int i = 0; i < nred; ++i) {
-    sum += hs[i];
-    z += hz[i];
-  }
-  const double m = (double)kHllRegs;
-  double est = (0.7213 / (1.0 + 1.079 / m)) * m * m / sum;
-  if (est <= 2.5 * m && z > 0) est = m * std::log(m / (double)z);
-  return std::max<int64_t>(1, std::min<int64_t>(R, (int64_t)std::ceil(est)));
-}
-
 // The sample's distinct integer keys when it holds every group of the call
 // and at most kTinyKeys of them.
 struct SampleKeys {
@@ -1752,49 +1755,96 @@ struct SampleKeys {
   uint64_t keys[kTinyKeys] = {};
 };
 
+constexpr int kHllReduceBlocks = kHllRegs / kHllReduceRegs;
+constexpr int kPinnedHll = 256;  // first host readback slot of the HyperLogLog sums (then the zero counts)
+static_assert(kPinnedHll + kHllReduceBlocks * 3 / 2 <= kMaxPinned, "readback slots past the end");
+
+// Estimate of the number of distinct key rows. A sample of 65536 rows
+// (every row of a small input) is hashed into a table first; when it holds
+// every group (all rows sampled, or every sampled key seen at least twice
+// and few distinct keys) its count is exact. Otherwise a HyperLogLog pass
+// over every row's key hash (standard error ~1.04/sqrt(2^14) = 0.8%, linear
+// counting for small cardinalities) gives the estimate; that pass also
+// counts each chunk's rows per fine bucket (ws->fine). The pass is launched
+// right after the sample and reads the sample's counts on the device (it
+// returns at once when the sample is complete), so one host wait serves
+// both.
 int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_t stream, mgb_status& st,
                         SampleKeys* sk = nullptr)
 {
   st = MGB_SUCCESS;
+  ws->fine_valid = false;
   const int64_t kSample = 1 << 16;
   const int64_t scap = 4 * kSample;
   // small inputs are counted exactly; larger ones are sampled
   const int64_t samples = R <= scap / 2 ? R : kSample;
-  size_t need = align_up((size_t)(scap + 1) * 8, 256) + align_up((size_t)(scap + 1) * 4, 256) +
-                align_up((5 + kTinyKeys) * 8, 256);
-  if (ws->main.reserve(need, stream) != hipSuccess) {
-    st = fail(MGB_ERROR_ALLOC, "allocation of the cardinality sample's temporaries failed");
+  const bool may_need_hll = samples < R;
+  const size_t sample_bytes = align_up((size_t)(scap + 1) * 8, 256) + align_up((size_t)(scap + 1) * 4, 256) +
+                              align_up((5 + kTinyKeys) * 8, 256);
+  // the HyperLogLog pass: one workgroup per chunk of the partitioned execution
+  const int cus = device_cu_count();
+  const int64_t nb = partition_chunks(R, kFineBits, cus);
+  const int64_t chunk = (R + nb - 1) / nb;
+  // workgroups of the pass: each takes several chunks when they are many
+  // (fewer workgroups to start and to return at once after a complete sample)
+  const int64_t hll_wgs = std::min<int64_t>(nb, 2 * (int64_t)cus);
+  const size_t regs_bytes = align_up((size_t)hll_wgs * kHllRegs, 256);
+  const size_t hll_bytes = regs_bytes + align_up((size_t)kHllReduceBlocks * 8, 256) +
+                           align_up((size_t)kHllReduceBlocks * 4, 256);
+  if (ws->main.reserve(sample_bytes + (may_need_hll ? hll_bytes : 0), stream) != hipSuccess ||
+      (may_need_hll && ws->fine.reserve((size_t)kFineBuckets * nb * 4, stream) != hipSuccess)) {
+    st = fail(MGB_ERROR_ALLOC, "allocation of the cardinality estimate's temporaries failed");
     return R;
   }
-  hipError_t e = hipSuccess;
+  uint8_t* mem = static_cast<uint8_t*>(ws->main.ptr);
   SampleParams sp{};
   sp.agg = base;
-  sp.hdr = static_cast<unsigned long long*>(ws->main.ptr);
-  sp.cnt = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws->main.ptr) + align_up((size_t)(scap + 1) * 8, 256));
+  sp.hdr = reinterpret_cast<unsigned long long*>(mem);
+  sp.cnt = reinterpret_cast<uint32_t*>(mem + align_up((size_t)(scap + 1) * 8, 256));
   sp.stats = reinterpret_cast<unsigned long long*>(reinterpret_cast<uint8_t*>(sp.cnt) +
                                                    align_up((size_t)(scap + 1) * 4, 256));
   sp.scap = scap;
   sp.samples = samples;
   const uint64_t empty = base.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef;
-  sample_clear_kernel<<<256, kBlock, 0, stream>>>(sp.hdr, sp.cnt, scap, empty, sp.stats, 5);
-  sample_insert_kernel<<<(unsigned)((samples + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp);
-  sample_stats_kernel<<<(unsigned)((scap + 1 + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp.cnt, scap + 1, sp.stats);
   const bool want_keys = sk && base.key_mode == KEY_INT;
-  if (want_keys)
-    sample_keys_kernel<<<(unsigned)((scap + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp.hdr, sp.cnt, scap,
-                                                                                      sp.stats + 4, sp.stats + 5);
-  e = hipGetLastError();
+  uint8_t* regs = mem + sample_bytes;
+  double* sums = reinterpret_cast<double*>(regs + regs_bytes);
+  uint32_t* zeros = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(sums) +
+                                                align_up((size_t)kHllReduceBlocks * 8, 256));
+  hipError_t e = hipSuccess;
+  {
+    ProfScope prof("estimate", stream);
+    sample_clear_kernel<<<256, kBlock, 0, stream>>>(sp.hdr, sp.cnt, scap, empty, sp.stats, 5);
+    sample_insert_kernel<<<(unsigned)((samples + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp);
+    sample_summary_kernel<<<kSampleSummaryBlocks, kBlock, 0, stream>>>(sp.hdr, sp.cnt, scap, sp.stats, sp.stats + 4,
+                                                                        want_keys ? sp.stats + 5 : nullptr);
+    if (may_need_hll) {
+      const size_t lds = (size_t)(kHllRegs + kFineBuckets) * 4;
+      e = set_dynamic_lds((const void*)hll_hist_kernel, lds);
+      if (e == hipSuccess) {
+        hll_hist_kernel<<<(unsigned)hll_wgs, kLdsBlock, lds, stream>>>(base, regs,
+                                                                       static_cast<uint32_t*>(ws->fine.ptr), chunk,
+                                                                       (int32_t)nb, sp.stats);
+        hll_reduce_kernel<<<(unsigned)kHllReduceBlocks, 1024, 0, stream>>>(regs, (int)hll_wgs, sums, zeros, sp.stats);
+      }
+    }
+  }
+  if (e == hipSuccess) e = hipGetLastError();
   if (e == hipSuccess)
     e = hipMemcpyAsync(ws->pinned, sp.stats, (want_keys ? 5 + kTinyKeys : 4) * sizeof(unsigned long long),
                        hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess && may_need_hll)
+    e = hipMemcpyAsync(ws->pinned + kPinnedHll, sums, (size_t)kHllReduceBlocks * 8, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess && may_need_hll)
+    e = hipMemcpyAsync(ws->pinned + kPinnedHll + kHllReduceBlocks, zeros, (size_t)kHllReduceBlocks * 4,
+                       hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) {
-    st = fail(MGB_ERROR_HIP, std::string("cardinality sample: ") + hipGetErrorString(e));
+    st = fail(MGB_ERROR_HIP, std::string("cardinality estimate: ") + hipGetErrorString(e));
     return R;
   }
   const uint64_t d = ws->pinned[0], f1 = ws->pinned[1], sampled = ws->pinned[3];
-  // every row sampled, or a saturated sample (every sampled key seen at least
-  // twice, few distinct keys): all groups found
+  // every row sampled, or a saturated sample (as sample_saturated): all groups found
   const bool complete = (int64_t)sampled >= R || samples == R || (f1 == 0 && d * 8 <= sampled);
   if (want_keys && complete && ws->pinned[4] <= (uint64_t)kTinyKeys) {
     sk->complete = true;
@@ -1802,8 +1852,22 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
     for (int i = 0; i < sk->n; ++i) sk->keys[i] = ws->pinned[5 + i];
   }
   if (complete) return (int64_t)d;
-  // otherwise count distinct keys over all rows
-  return hll_estimate(ws, base, R, stream, st);
+  // the HyperLogLog estimate over all rows
+  ws->fine_valid = true;
+  ws->fine_nw = nb;
+  ws->fine_chunk = chunk;
+  double sum = 0.0;
+  uint64_t z = 0;
+  const double* hs = reinterpret_cast<const double*>(ws->pinned + kPinnedHll);
+  const uint32_t* hz = reinterpret_cast<const uint32_t*>(ws->pinned + kPinnedHll + kHllReduceBlocks);
+  for (int i = 0; i < kHllReduceBlocks; ++i) {
+    sum += hs[i];
+    z += hz[i];
+  }
+  const double m = (double)kHllRegs;
+  double est = (0.7213 / (1.0 + 1.079 / m)) * m * m / sum;
+  if (est <= 2.5 * m && z > 0) est = m * std::log(m / (double)z);
+  return std::max<int64_t>(1, std::min<int64_t>(R, (int64_t)std::ceil(est)));
 }
 
 // The sort-path aggregations (MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT) of a
@@ -2040,8 +2104,7 @@ mgb_status second_pass_table(Workspace* ws, const Plan& plan, const AggParams& b
     q.rows = lp.src;
     mgb_status st = upload_pass2(ws, q, stream, &lp.p2);
     if (st != MGB_SUCCESS) return st;
-    MGB_HIP(hipFuncSetAttribute((const void*)pass2_direct_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)geom.bytes));
+    MGB_HIP(set_dynamic_lds((const void*)pass2_direct_kernel, (int)geom.bytes));
     {
       ProfScope prof("pass2", stream);
       pass2_direct_kernel<<<(unsigned)direct_items, kLdsBlock, geom.bytes, stream>>>(lp);
@@ -2221,15 +2284,14 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
                  : tv == TV_F32 ? tiny_agg_kernel<1, 8, TV_F32> : tv == TV_F64 ? tiny_agg_kernel<1, 8, TV_F64> : fn;
           }
         }
-        MGB_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)total));
+        MGB_HIP(set_dynamic_lds((const void*)fn, (int)total));
         {
           ProfScope prof("lds_aggregate", stream);
           hipLaunchKernelGGL(fn, dim3((unsigned)items), dim3(kLdsBlock), total, stream, lp, d_tp, (int32_t)table_bytes);
         }
         MGB_HIP(hipGetLastError());
       } else {
-        MGB_HIP(hipFuncSetAttribute((const void*)lds_agg_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)geom.bytes));
+        MGB_HIP(set_dynamic_lds((const void*)lds_agg_kernel<false>, (int)geom.bytes));
         {
           ProfScope prof("lds_aggregate", stream);
           lds_agg_kernel<false><<<(unsigned)items, kLdsBlock, geom.bytes, stream>>>(lp);
@@ -2527,10 +2589,8 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     for (int c = 0; c < src.num_vals; ++c) pp.val_off[c] = voff[c];
     pp.rowid_off = rowid_off;
     const size_t part_lds = (size_t)P * 4;
-    MGB_HIP(hipFuncSetAttribute((const void*)part_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)part_lds));
-    MGB_HIP(hipFuncSetAttribute((const void*)part_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)part_lds));
+    MGB_HIP(set_dynamic_lds((const void*)part_hist_kernel, (int)part_lds));
+    MGB_HIP(set_dynamic_lds((const void*)part_scatter_kernel, (int)part_lds));
     if (ws->fine_valid && log2p <= kFineBits && ws->fine_nw == nw && ws->fine_chunk == chunk) {
       // the estimate's pass counted the rows per fine bucket of these chunks
       ProfScope prof("partition_hist", stream);
@@ -2576,8 +2636,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       st = lean_scatter(lean.kw, lean.w, sa, (unsigned)nw, lean.lds, stream);
       if (st != MGB_SUCCESS) return st;
     } else if (tile > 0) {
-      MGB_HIP(hipFuncSetAttribute((const void*)part_scatter_tiled_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)tiled_lds));
+      MGB_HIP(set_dynamic_lds((const void*)part_scatter_tiled_kernel, (int)tiled_lds));
       ProfScope prof("partition_scatter", stream);
       part_scatter_tiled_kernel<<<(unsigned)nw, kLdsBlock, tiled_lds, stream>>>(pp);
     } else {
@@ -2691,7 +2750,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       if (st != MGB_SUCCESS) return st;
     } else {
       const void* agg_fn = plan.has_pass2 ? (const void*)lds_agg_kernel<true> : (const void*)lds_agg_kernel<false>;
-      MGB_HIP(hipFuncSetAttribute(agg_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)geom.bytes));
+      MGB_HIP(set_dynamic_lds(agg_fn, (int)geom.bytes));
       ProfScope prof("lds_aggregate", stream);
       if (plan.has_pass2) lds_agg_kernel<true><<<(unsigned)item_grid, kLdsBlock, geom.bytes, stream>>>(lp);
       else lds_agg_kernel<false><<<(unsigned)item_grid, kLdsBlock, geom.bytes, stream>>>(lp);

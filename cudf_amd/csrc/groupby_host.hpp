@@ -276,6 +276,8 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
 void record_template(const Plan& plan, uint64_t header, uint64_t* words);
 bool single_launch_ops(const Plan& plan, AggParams& ap);
 size_t lds_limit();
+// Raises a kernel's dynamic LDS limit to `bytes` (once per kernel, device and size).
+hipError_t set_dynamic_lds(const void* fn, size_t bytes);
 int lds_stride_of(int stride);
 LdsGeom lds_geometry(int stride);
 

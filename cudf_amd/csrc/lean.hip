@@ -88,7 +88,7 @@ mgb_status lean_scatter(int kw, const int (&w)[kLeanCols], const LeanScatterArgs
 {
   const LeanFns* f = lean_find(kw, w);
   if (!f) return fail(MGB_ERROR_UNSUPPORTED, "compact partitioned execution: unsupported shape");
-  MGB_HIP(hipFuncSetAttribute(f->scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  MGB_HIP(set_dynamic_lds(f->scatter, lds));
   LeanScatterArgs args = a;
   void* kargs[] = {&args};
   MGB_HIP(hipLaunchKernel(f->scatter, dim3(grid), dim3(kLdsBlock), kargs, lds, stream));
@@ -100,7 +100,7 @@ mgb_status lean_aggregate(int kw, const int (&w)[kLeanCols], const LeanAggArgs& 
 {
   const LeanFns* f = lean_find(kw, w);
   if (!f) return fail(MGB_ERROR_UNSUPPORTED, "compact partitioned execution: unsupported shape");
-  MGB_HIP(hipFuncSetAttribute(f->agg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  MGB_HIP(set_dynamic_lds(f->agg, lds));
   LeanAggArgs args = a;
   const LdsParams* p = lp;
   void* kargs[] = {&args, &p};
