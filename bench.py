@@ -41,6 +41,15 @@ PROFILED = ("aggregate", "lds_aggregate", "fill", "finalize", "partition_hist", 
 KERNEL_SYMBOL = {"aggregate": "mgb::aggregate_kernel", "lds_aggregate": "mgb::lds_agg_kernel<false>",
                  "partition_scatter": "mgb::part_scatter_tiled_kernel", "partition_hist": "mgb::part_hist_kernel",
                  "pass2": "mgb::pass2_kernel", "sort_aggs": "sort-path kernels and hipCUB radix sorts"}
+# kernels of the compact partitioned execution (single integer key, up to three
+# 4/8-byte value columns), compiled per shape: a scope's kernel is found by
+# prefix in the traffic summary
+COMPACT_PREFIX = {"lds_aggregate": "mgb::lean_agg_kernel<", "partition_scatter": "mgb::lean_scatter_kernel<"}
+
+
+def kernel_name(raw: str) -> str:
+    """A rocprofv3 kernel name without 'void', arguments or spaces."""
+    return raw.split("(")[0].removeprefix("void ").replace(" ", "")
 
 
 def pack_bits(valid):
@@ -595,7 +604,7 @@ def summarize_trace(db_path: str, title: str, command: str) -> str:
            "Durations in microseconds (top_kernels view of the rocprofv3 database; torch kernels are data generation).",
            "", f"{'kernel':60s} {'calls':>7s} {'total_us':>12s} {'avg_us':>10s} {'pct':>6s}"]
     for name, calls, total, avg, pct in rows[:30]:
-        out.append(f"{name.split('(')[0].removeprefix('void ')[:60]:60s} {int(calls):7d} {float(total):12.1f} {float(avg):10.1f} "
+        out.append(f"{kernel_name(name)[:60]:60s} {int(calls):7d} {float(total):12.1f} {float(avg):10.1f} "
                    f"{float(pct):6.2f}")
     return "\n".join(out) + "\n"
 
@@ -612,7 +621,7 @@ def summarize_pmc(fetch_csv: str, write_csv: str, cal_fetch_csv: str, cal_write_
         acc = collections.defaultdict(list)
         for r in csv.DictReader(open(path)):
             if r["Counter_Name"] == counter:
-                acc[r["Kernel_Name"].split("(")[0].removeprefix("void ")].append(float(r["Counter_Value"]))
+                acc[kernel_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
         return acc
 
     cal_f = per_kernel(cal_fetch_csv, "FETCH_SIZE")
@@ -646,7 +655,7 @@ def summarize_counters(csv_paths, title: str, commands) -> str:
     acc = collections.defaultdict(lambda: collections.defaultdict(list))
     for path in csv_paths:
         for r in csv.DictReader(open(path)):
-            name = r["Kernel_Name"].split("(")[0].removeprefix("void ")
+            name = kernel_name(r["Kernel_Name"])
             if name.startswith("mgb::"):
                 acc[name][r["Counter_Name"]].append(float(r["Counter_Value"]))
     counters = sorted({c for k in acc.values() for c in k})
@@ -658,6 +667,20 @@ def summarize_counters(csv_paths, title: str, commands) -> str:
             if c in acc[k]:
                 out.append(f"    {c:28s} {statistics.mean(acc[k][c]):18.0f}")
     return "\n".join(out) + "\n"
+
+
+def find_kernel(path: str, scope: str, default: str) -> str:
+    """The kernel that a profiling scope launched: the compact execution's
+    instance named in the traffic summary when there is one, else `default`."""
+    prefix = COMPACT_PREFIX.get(scope)
+    try:
+        for line in open(path):
+            parts = line.split()
+            if prefix and parts and parts[0].startswith(prefix):
+                return parts[0]
+    except OSError:
+        pass
+    return default
 
 
 def read_traffic(path: str, kernel: str):
@@ -874,7 +897,7 @@ def main():
     dom = max((k for k in ("aggregate", "lds_aggregate", "partition_scatter", "pass2", "sort_aggs") if k in kern),
               key=lambda k: kern[k][1] / kern[k][0])
     agg_n, agg_ms = kern[dom]
-    symbol = KERNEL_SYMBOL[dom]
+    symbol = find_kernel(args.traffic_file, dom, KERNEL_SYMBOL[dom]) if args.workload == "c2" else KERNEL_SYMBOL[dom]
     if dom == "lds_aggregate" and args.workload == "c2v":
         symbol = "mgb::lds_agg_kernel<true>"  # the instance with the second pass
     agg_avg_ms = agg_ms / agg_n
