@@ -10,7 +10,8 @@ import ctypes
 import os
 import threading
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libcudf_amd.so")
+# MGB_LIB_PATH: another build of the library (the host sanitizer build)
+LIB_PATH = os.environ.get("MGB_LIB_PATH") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libcudf_amd.so")
 
 MGB_SUCCESS = 0
 MGB_ERROR_LOGIC = 1

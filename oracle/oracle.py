@@ -25,7 +25,8 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# MGB_ORACLE_LIB: another build of the same source (the sanitizer build)
+LIB_PATH = os.environ.get("MGB_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 # cudf::type_id values and their numpy storage types
 INT8, INT16, INT32, INT64, UINT8, UINT16, UINT32, UINT64, FLOAT32, FLOAT64, BOOL8 = range(1, 12)

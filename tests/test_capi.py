@@ -138,3 +138,59 @@ def test_validity_pack_roundtrip(n):
     # LSB-first: bit i of word i//32
     for i in range(min(n, 40)):
         assert ((int(words[i // 32]) >> (i % 32)) & 1) == int(v[i])
+
+
+def _no_device() -> bool:
+    import torch
+
+    return not torch.cuda.is_available()
+
+
+HASH_KINDS = [0, 1, 2, 3, 4, 5, 8, 9, 10, 11, 12, 15, 16]  # every kind of the hash path (and M2)
+SORT_KINDS = [13, 14, 17, 18]                             # MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT
+
+
+@pytest.mark.skipif(not _no_device(), reason="fake device pointers: run only where no GPU can read them")
+@pytest.mark.parametrize("keys", [[3], [4], [9], [3, 4], [1, 10, 6]], ids=["i32", "i64", "f32", "i32_i64", "3keys"])
+@pytest.mark.parametrize("vt", [1, 3, 4, 6, 9, 10, 11])
+def test_every_request_is_planned_before_device_work(keys, vt):
+    """On a machine without a GPU a valid call builds its plan (record
+    layout, ops, outputs) and stops at the first device call; an invalid
+    type/aggregation pair stops before it. The host sanitizer build runs
+    this planning code (tests/test_sanitizers.py)."""
+    lib = _lib.load()
+    n = 1000
+    kcols = []
+    for t in keys:
+        c = _col(t, n)
+        c.null_mask = 0x2000
+        c.null_count = 7
+        kcols.append(c)
+    vals = _col(vt, n)
+    vals.null_mask = 0x3000
+    vals.null_count = 5
+    for kinds in (HASH_KINDS, SORT_KINDS, HASH_KINDS + SORT_KINDS):
+        valid = [k for k in kinds if O.target_type(vt, k) != 0]
+        for policy in (0, 1):
+            ka = (_lib.mgb_column * len(kcols))(*kcols)
+            ra = (_lib.mgb_request * 2)()
+            aggs = (_lib.mgb_aggregation * len(valid))(*[_lib.mgb_aggregation(k, 1, 0, 0, 0.5) for k in valid])
+            bad = (_lib.mgb_aggregation * 1)(_lib.mgb_aggregation(kinds[0], 1, 0, 0, 0.5))
+            ra[0].values = vals
+            ra[0].aggs = ctypes.cast(aggs, ctypes.POINTER(_lib.mgb_aggregation))
+            ra[0].num_aggs = len(valid)
+            ra[1].values = vals
+            ra[1].aggs = ctypes.cast(bad, ctypes.POINTER(_lib.mgb_aggregation))
+            ra[1].num_aggs = 1
+            ok = (_lib.mgb_out_column * len(kcols))()
+            orr = (_lib.mgb_out_column * (len(valid) + 1))()
+            g = ctypes.c_int64(-1)
+            alloc = _lib.mgb_allocator(_lib.ALLOC_FN(lambda c, b, s: None), _lib.DEALLOC_FN(lambda c, p, b, s: None),
+                                       None)
+            st = lib.mgb_groupby_aggregate(ka, len(kcols), policy, ra, 1, ok, orr, ctypes.byref(g), None,
+                                           ctypes.byref(alloc))
+            assert st in (_lib.MGB_ERROR_HIP, _lib.MGB_ERROR_ALLOC), lib.mgb_last_error_message()
+            if O.target_type(vt, kinds[0]) == 0:
+                st = lib.mgb_groupby_aggregate(ka, len(kcols), policy, ra, 2, ok, orr, ctypes.byref(g), None,
+                                               ctypes.byref(alloc))
+                assert st == _lib.MGB_ERROR_LOGIC
