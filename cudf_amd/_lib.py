@@ -77,6 +77,14 @@ class mgb_out_column(ctypes.Structure):
         ("null_count", ctypes.c_int64),
         ("data_bytes", ctypes.c_size_t),
         ("mask_bytes", ctypes.c_size_t),
+        ("child_type_id", ctypes.c_int32),
+        ("child_reserved", ctypes.c_int32),
+        ("child_size", ctypes.c_int64),
+        ("child_data", ctypes.c_void_p),
+        ("child_null_mask", ctypes.c_void_p),
+        ("child_null_count", ctypes.c_int64),
+        ("child_data_bytes", ctypes.c_size_t),
+        ("child_mask_bytes", ctypes.c_size_t),
     ]
 
 

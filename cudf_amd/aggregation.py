@@ -4,15 +4,15 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 
-from .types import Interpolation, Kind, NullPolicy
+from .types import Interpolation, Kind, NanEquality, NullEquality, NullPolicy
 
 
 @dataclass(frozen=True)
 class Aggregation:
     kind: Kind
-    ddof: int = 0              # VARIANCE / STD: ddof; NTH_ELEMENT: n
+    ddof: int = 0              # VARIANCE / STD: ddof; NTH_ELEMENT: n; COLLECT_SET: null equality
     null_policy: int = NullPolicy.EXCLUDE
-    interpolation: int = Interpolation.LINEAR
+    interpolation: int = Interpolation.LINEAR  # QUANTILE; COLLECT_SET: NaN equality
     quantile: float = 0.0
 
     def __repr__(self) -> str:
@@ -89,3 +89,16 @@ def argmax() -> Aggregation:
 
 def argmin() -> Aggregation:
     return Aggregation(Kind.ARGMIN)
+
+
+def collect_list(null_handling: NullPolicy = NullPolicy.INCLUDE) -> Aggregation:
+    """Each group's values in row order, as a list (nulls kept as null elements under INCLUDE)."""
+    return Aggregation(Kind.COLLECT_LIST, null_policy=int(null_handling))
+
+
+def collect_set(null_handling: NullPolicy = NullPolicy.INCLUDE, nulls_equal: NullEquality = NullEquality.EQUAL,
+                nans_equal: NanEquality = NanEquality.ALL_EQUAL) -> Aggregation:
+    """Each group's distinct values, ascending, then its nulls (one when nulls are equal, each otherwise,
+    none under EXCLUDE)."""
+    return Aggregation(Kind.COLLECT_SET, ddof=int(nulls_equal), null_policy=int(null_handling),
+                       interpolation=int(nans_equal))

@@ -111,6 +111,43 @@ class Column:
         return f"Column({self.type_id.name}, size={self.size}, null_count={self.null_count})"
 
 
+class ListColumn:
+    """A LIST column (the result of COLLECT_LIST / COLLECT_SET): `size` lists,
+    list i holding elements offsets[i]..offsets[i+1] of `child`; lists are
+    never null themselves (cudf's lists column)."""
+
+    type_id = TypeId.LIST
+
+    def __init__(self, size: int, offsets: Column, child: Column):
+        self.size = int(size)
+        self.offsets = offsets
+        self.child = child
+        self.mask = None
+        self.null_count = 0
+
+    def nullable(self) -> bool:
+        return False
+
+    def to_numpy(self):
+        """(offsets int32[size + 1], (child values, child valid-or-None)) copied to the host."""
+        off, _ = self.offsets.to_numpy()
+        if off.size == 0:
+            off = np.zeros(1, dtype=np.int32)
+        return off, self.child.to_numpy()
+
+    def to_lists(self):
+        """Python lists of every row's elements (None for a null element)."""
+        off, (vals, valid) = self.to_numpy()
+        out = []
+        for i in range(self.size):
+            a, b = int(off[i]), int(off[i + 1])
+            out.append([None if (valid is not None and not valid[j]) else vals[j].item() for j in range(a, b)])
+        return out
+
+    def __repr__(self) -> str:
+        return f"ListColumn({self.child.type_id.name}, size={self.size}, elements={self.child.size})"
+
+
 class Table:
     """An ordered set of equal-length columns."""
 

@@ -633,8 +633,8 @@ extern "C" mgb_status mgb_batch_create(const mgb_column* keys, int32_t num_keys,
     return fail(MGB_ERROR_UNSUPPORTED, "batched groupby does not group null keys (null_policy INCLUDE with null keys)");
   if (plan.has_pass2 || plan.has_sort)
     return fail(MGB_ERROR_UNSUPPORTED, "batched groupby does not serve VARIANCE, STD, M2, ARGMIN, ARGMAX, MEDIAN, "
-                                       "QUANTILE, NUNIQUE or NTH_ELEMENT (a second pass over the rows): use "
-                                       "mgb_groupby_aggregate");
+                                       "QUANTILE, NUNIQUE, NTH_ELEMENT, COLLECT_LIST or COLLECT_SET (a second "
+                                       "pass over the rows): use mgb_groupby_aggregate");
   if ((int)plan.outs.size() > kMaxOuts) return fail(MGB_ERROR_UNSUPPORTED, "too many aggregations for a batch");
   b->num_outs = (int)plan.outs.size();
   const int Q = num_queries;

@@ -41,37 +41,46 @@ namespace detail {
 
 // One concrete aggregation type serving every factory; `param` carries ddof
 // (VARIANCE / STD) or n (NTH_ELEMENT), `nulls` the null_policy of NUNIQUE /
-// NTH_ELEMENT, `quantiles` / `interp` those of QUANTILE.
+// NTH_ELEMENT / COLLECT_LIST / COLLECT_SET, `quantiles` / `interp` those of
+// QUANTILE, `nulls_equal` / `nans_equal` those of COLLECT_SET.
 class simple_aggregation final : public groupby_aggregation,
                                  public groupby_scan_aggregation,
                                  public reduce_aggregation,
                                  public rolling_aggregation {
  public:
   explicit simple_aggregation(aggregation::Kind k, size_type param = 0, null_policy nulls = null_policy::EXCLUDE,
-                              std::vector<double> quantiles = {}, interpolation interp = interpolation::LINEAR)
-    : aggregation(k), _param{param}, _nulls{nulls}, _quantiles{std::move(quantiles)}, _interp{interp}
+                              std::vector<double> quantiles = {}, interpolation interp = interpolation::LINEAR,
+                              null_equality nulls_equal = null_equality::EQUAL,
+                              nan_equality nans_equal = nan_equality::ALL_EQUAL)
+    : aggregation(k), _param{param}, _nulls{nulls}, _quantiles{std::move(quantiles)}, _interp{interp},
+      _nulls_equal{nulls_equal}, _nans_equal{nans_equal}
   {
   }
   [[nodiscard]] std::unique_ptr<aggregation> clone() const override
   {
-    return std::make_unique<simple_aggregation>(kind, _param, _nulls, _quantiles, _interp);
+    return std::make_unique<simple_aggregation>(kind, _param, _nulls, _quantiles, _interp, _nulls_equal,
+                                                _nans_equal);
   }
   [[nodiscard]] bool is_equal(aggregation const& other) const override
   {
     auto const* o = dynamic_cast<simple_aggregation const*>(&other);
     return o && o->kind == kind && o->_param == _param && o->_nulls == _nulls && o->_quantiles == _quantiles &&
-           o->_interp == _interp;
+           o->_interp == _interp && o->_nulls_equal == _nulls_equal && o->_nans_equal == _nans_equal;
   }
   [[nodiscard]] size_type param() const { return _param; }
   [[nodiscard]] null_policy nulls() const { return _nulls; }
   [[nodiscard]] std::vector<double> const& quantiles() const { return _quantiles; }
   [[nodiscard]] interpolation interp() const { return _interp; }
+  [[nodiscard]] null_equality nulls_equal() const { return _nulls_equal; }
+  [[nodiscard]] nan_equality nans_equal() const { return _nans_equal; }
 
  private:
   size_type _param;
   null_policy _nulls;
   std::vector<double> _quantiles;
   interpolation _interp;
+  null_equality _nulls_equal;
+  nan_equality _nans_equal;
 };
 
 // The C ABI aggregations of one cudf aggregation: one per quantile of a
@@ -85,6 +94,10 @@ std::vector<mgb_aggregation> to_mgb_aggs(aggregation const& a)
   m.ddof = s->param();
   m.null_policy = s->nulls() == null_policy::INCLUDE ? MGB_NULL_INCLUDE : MGB_NULL_EXCLUDE;
   m.interpolation = static_cast<int32_t>(s->interp());
+  if (a.kind == aggregation::COLLECT_SET) {
+    m.ddof = s->nulls_equal() == null_equality::EQUAL ? MGB_NULL_EQUAL : MGB_NULL_UNEQUAL;
+    m.interpolation = s->nans_equal() == nan_equality::ALL_EQUAL ? MGB_NAN_ALL_EQUAL : MGB_NAN_UNEQUAL;
+  }
   if (a.kind != aggregation::QUANTILE) return {m};
   std::vector<mgb_aggregation> out;
   for (double q : s->quantiles()) {
@@ -161,6 +174,21 @@ std::unique_ptr<column> adopt(mgb_out_column const& o, rmm::cuda_stream_view str
 {
   rmm::device_buffer data{o.data, o.data ? o.data_bytes : 0, stream, mr};
   rmm::device_buffer mask{o.null_mask, o.null_mask ? o.mask_bytes : 0, stream, mr};
+  if (o.type_id == MGB_TYPE_LIST) {
+    // cudf's lists column: no data of its own, children {INT32 offsets, values}
+    // (cpp/include/cudf/lists/lists_column_view.hpp:63-64)
+    std::vector<std::unique_ptr<column>> children;
+    children.push_back(std::make_unique<column>(data_type{type_id::INT32},
+                                                static_cast<size_type>(o.size ? o.size + 1 : 0), std::move(data),
+                                                rmm::device_buffer{}, 0));
+    rmm::device_buffer cdata{o.child_data, o.child_data ? o.child_data_bytes : 0, stream, mr};
+    rmm::device_buffer cmask{o.child_null_mask, o.child_null_mask ? o.child_mask_bytes : 0, stream, mr};
+    children.push_back(std::make_unique<column>(data_type{static_cast<type_id>(o.child_type_id)},
+                                                static_cast<size_type>(o.child_size), std::move(cdata),
+                                                std::move(cmask), static_cast<size_type>(o.child_null_count)));
+    return std::make_unique<column>(data_type{type_id::LIST}, static_cast<size_type>(o.size), rmm::device_buffer{},
+                                    std::move(mask), static_cast<size_type>(o.null_count), std::move(children));
+  }
   return std::make_unique<column>(data_type{static_cast<type_id>(o.type_id)}, static_cast<size_type>(o.size),
                                   std::move(data), std::move(mask), static_cast<size_type>(o.null_count));
 }
@@ -235,6 +263,28 @@ std::unique_ptr<Base> make_nth_element_aggregation(size_type n, null_policy null
 template std::unique_ptr<aggregation> make_nth_element_aggregation<aggregation>(size_type, null_policy);
 template std::unique_ptr<groupby_aggregation> make_nth_element_aggregation<groupby_aggregation>(size_type,
                                                                                                 null_policy);
+
+template <typename Base>
+std::unique_ptr<Base> make_collect_list_aggregation(null_policy null_handling)
+{
+  return std::make_unique<detail::simple_aggregation>(aggregation::COLLECT_LIST, 0, null_handling);
+}
+template std::unique_ptr<aggregation> make_collect_list_aggregation<aggregation>(null_policy);
+template std::unique_ptr<groupby_aggregation> make_collect_list_aggregation<groupby_aggregation>(null_policy);
+
+template <typename Base>
+std::unique_ptr<Base> make_collect_set_aggregation(null_policy null_handling, null_equality nulls_equal,
+                                                   nan_equality nans_equal)
+{
+  return std::make_unique<detail::simple_aggregation>(aggregation::COLLECT_SET, 0, null_handling,
+                                                      std::vector<double>{}, interpolation::LINEAR, nulls_equal,
+                                                      nans_equal);
+}
+template std::unique_ptr<aggregation> make_collect_set_aggregation<aggregation>(null_policy, null_equality,
+                                                                               nan_equality);
+template std::unique_ptr<groupby_aggregation> make_collect_set_aggregation<groupby_aggregation>(null_policy,
+                                                                                               null_equality,
+                                                                                               nan_equality);
 
 template <typename Base>
 std::unique_ptr<Base> make_quantile_aggregation(std::vector<double> const& quantiles, interpolation interp)

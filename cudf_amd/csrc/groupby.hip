@@ -973,6 +973,8 @@ int32_t target_type(int32_t t, int32_t kind)
       return MGB_TYPE_FLOAT64;
     case MGB_AGG_NUNIQUE: return MGB_TYPE_INT32;
     case MGB_AGG_NTH_ELEMENT: return t;
+    // COLLECT_LIST / COLLECT_SET: a list of the source type (aggregation.hpp:1335-1350)
+    case MGB_AGG_COLLECT_LIST: case MGB_AGG_COLLECT_SET: return MGB_TYPE_LIST;
     case MGB_AGG_M2: case MGB_AGG_VARIANCE: case MGB_AGG_STD:
       // numeric sources only (multi_pass_kernels.cuh:59-63)
       if (!is_integral_t(t) && !is_float_t(t)) return -1;
@@ -1026,6 +1028,7 @@ struct Workspace {
   DeviceBuffer params{&mem};  // kernel parameter blocks (output tables, identity templates)
   DeviceBuffer fine{&mem};    // fine bucket counts per chunk of the last key pass (hll_hist_kernel)
   DeviceBuffer sorts{&mem};   // row groups, sort keys and radix-sort space of the sort-path aggregations
+  DeviceBuffer lists{&mem};   // COLLECT_LIST / COLLECT_SET results: offsets and elements
   bool fine_valid = false;  // `fine` holds the counts of this call's keys
   int64_t fine_nw = 0, fine_chunk = 0;
   unsigned long long* pinned = nullptr;  // host readback area (kMaxPinned slots)
@@ -1112,6 +1115,7 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
         case MGB_AGG_COUNT_ALL: case MGB_AGG_SUM_OF_SQUARES: case MGB_AGG_MEAN: case MGB_AGG_VARIANCE:
         case MGB_AGG_STD: case MGB_AGG_M2: case MGB_AGG_ARGMIN: case MGB_AGG_ARGMAX: break;
         case MGB_AGG_MEDIAN: case MGB_AGG_QUANTILE: case MGB_AGG_NUNIQUE: case MGB_AGG_NTH_ELEMENT:
+        case MGB_AGG_COLLECT_LIST: case MGB_AGG_COLLECT_SET:
           plan.has_sort = true;
           break;
         default:
@@ -1184,7 +1188,7 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
     // count per nullable column (updated only for null values); the
     // sort-path kinds read neither
     if (wt.kind == MGB_AGG_MEDIAN || wt.kind == MGB_AGG_QUANTILE || wt.kind == MGB_AGG_NUNIQUE ||
-        wt.kind == MGB_AGG_NTH_ELEMENT)
+        wt.kind == MGB_AGG_NTH_ELEMENT || wt.kind == MGB_AGG_COLLECT_LIST || wt.kind == MGB_AGG_COLLECT_SET)
       continue;
     need_rows = true;
     if (wt.kind != MGB_AGG_COUNT_ALL && pc.nullable) add_op(pc, OP_NULLS);
@@ -1238,6 +1242,14 @@ mgb_status build_plan(const mgb_column* keys, int32_t num_keys, int32_t null_pol
         break;
       case MGB_AGG_MEDIAN: case MGB_AGG_QUANTILE: case MGB_AGG_NUNIQUE: case MGB_AGG_NTH_ELEMENT:
         s.fin = FIN_NONE;  // the sort path writes it
+        o.agg = wt.agg;
+        break;
+      case MGB_AGG_COLLECT_LIST: case MGB_AGG_COLLECT_SET:
+        // a list column: the sort path writes its offsets and elements
+        // (TempOuts::lists); the temporary output is unused
+        s.fin = FIN_NONE;
+        s.out_width = 4;
+        s.valid = nullptr;
         o.agg = wt.agg;
         break;
       case MGB_AGG_M2:
@@ -1308,7 +1320,19 @@ void record_template(const Plan& plan, uint64_t header, uint64_t* words)
 
 // Temporary output columns in the workspace (filled by finalize / LDS
 // kernels, then copied into the caller's allocations).
+// A COLLECT_LIST / COLLECT_SET result as the sort path leaves it.
+struct ListTemp {
+  int out = -1;                  // index into plan.outs
+  uint32_t* offsets = nullptr;   // groups + 1 entries
+  uint8_t* child = nullptr;      // elements, the value column's width each
+  uint8_t* child_valid = nullptr;
+  bool child_nullable = false;   // the elements may be null
+  int64_t total = 0;             // elements (read back)
+  int64_t child_nulls = 0;       // null elements (read back)
+};
+
 struct TempOuts {
+  std::vector<ListTemp> lists;
   std::vector<uint8_t*> key_data, key_valid, out_data, out_valid;
   unsigned long long* null_counts = nullptr;
   int64_t rows = 0;  // capacity in groups
@@ -1449,6 +1473,14 @@ struct OutputSink {
 // Copies the temporary outputs into the result columns (data and null_mask
 // set, null_mask NULL where the column has no validity) and builds their
 // null masks and null counts.
+// The list temporaries of output o, or nullptr when it is not a list.
+const ListTemp* list_of(const TempOuts& t, int o)
+{
+  for (const ListTemp& lt : t.lists)
+    if (lt.out == o) return &lt;
+  return nullptr;
+}
+
 mgb_status copy_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int64_t groups, mgb_out_column* out_keys,
                         mgb_out_column* out_results, hipStream_t stream)
 {
@@ -1471,6 +1503,11 @@ mgb_status copy_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int6
       c.dst = static_cast<uint8_t*>(oc.data);
       c.bytes = groups * width;
       c.valid = j < no ? t.out_valid[j] : t.key_valid[j - no];
+      if (const ListTemp* lt = j < no ? list_of(t, j) : nullptr) {
+        c.src = reinterpret_cast<const uint8_t*>(lt->offsets);
+        c.bytes = (groups + 1) * 4;
+        c.valid = nullptr;
+      }
       c.mask = oc.null_mask;
       c.words = oc.null_mask ? words : 0;
       c.nulls = t.nulls_known ? nullptr : t.null_counts + j;
@@ -1478,6 +1515,24 @@ mgb_status copy_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int6
     }
     const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((units + kBlock - 1) / kBlock, 1024));
     emit_columns_kernel<<<dim3(gx, (unsigned)n), kBlock, 0, stream>>>(ep);
+  }
+  // the lists' elements (rows of their own)
+  for (const ListTemp& lt : t.lists) {
+    const mgb_out_column& oc = out_results[lt.out];
+    if (lt.total == 0) continue;
+    EmitParams ep{};
+    ep.rows = lt.total;
+    EmitCol& c = ep.cols[0];
+    c.src = lt.child;
+    c.dst = static_cast<uint8_t*>(oc.child_data);
+    c.bytes = lt.total * (int64_t)plan.cols[plan.outs[lt.out].col].col.width;
+    c.valid = oc.child_null_mask ? lt.child_valid : nullptr;
+    c.mask = oc.child_null_mask;
+    c.words = oc.child_null_mask ? bitmask_bytes(lt.total) / 4 : 0;
+    c.nulls = nullptr;
+    const int64_t units = std::max<int64_t>(c.words, (c.bytes + 15) / 16);
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((units + kBlock - 1) / kBlock, 1024));
+    emit_columns_kernel<<<dim3(gx, 1u), kBlock, 0, stream>>>(ep);
   }
   MGB_HIP(hipGetLastError());
   const int64_t* counts = nullptr;
@@ -1490,6 +1545,8 @@ mgb_status copy_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int6
   }
   for (int o = 0; o < no; ++o) out_results[o].null_count = out_results[o].null_mask ? counts[o] : 0;
   for (int c = 0; c < nk; ++c) out_keys[c].null_count = out_keys[c].null_mask ? counts[no + c] : 0;
+  for (const ListTemp& lt : t.lists)
+    out_results[lt.out].child_null_count = out_results[lt.out].child_null_mask ? lt.child_nulls : 0;
   return MGB_SUCCESS;
 }
 
@@ -1501,6 +1558,8 @@ mgb_status emit_into_caller_buffers(Workspace* ws, const Plan& plan, const TempO
                                     int64_t capacity)
 {
   const int nk = (int)plan.keys.size(), no = (int)plan.outs.size();
+  if (!t.lists.empty())
+    return fail(MGB_ERROR_UNSUPPORTED, "COLLECT_LIST / COLLECT_SET results need allocated outputs");
   if (groups > capacity)
     return fail(MGB_ERROR_ALLOC, "the result has " + std::to_string(groups) + " groups, more than the " +
                                      std::to_string(capacity) + " rows the output buffers hold");
@@ -1555,21 +1614,53 @@ mgb_status emit_outputs(Workspace* ws, const Plan& plan, const TempOuts& t, int6
     }
   };
   for (int c = 0; c < nk; ++c) fill_col(out_keys[c], plan.keys[c].type_id, plan.keys[c].width, t.key_valid[c]);
-  for (int o = 0; o < no; ++o)
-    fill_col(out_results[o], plan.outs[o].out_type, plan.outs[o].spec.out_width, t.out_valid[o]);
+  for (int o = 0; o < no; ++o) {
+    mgb_out_column& oc = out_results[o];
+    const ListTemp* lt = list_of(t, o);
+    if (!lt) {
+      fill_col(oc, plan.outs[o].out_type, plan.outs[o].spec.out_width, t.out_valid[o]);
+      if (oc.type_id == MGB_TYPE_LIST) oc.child_type_id = plan.cols[plan.outs[o].col].type;  // no groups
+      continue;
+    }
+    // a list column: groups + 1 offsets, then its elements
+    const ValCol& v = plan.cols[plan.outs[o].col].col;
+    std::memset(&oc, 0, sizeof(oc));
+    oc.type_id = MGB_TYPE_LIST;
+    oc.size = groups;
+    oc.child_type_id = plan.cols[plan.outs[o].col].type;
+    oc.child_size = lt->total;
+    if (groups == 0) continue;
+    oc.data_bytes = (size_t)(groups + 1) * 4;
+    sizes.push_back(oc.data_bytes);
+    dests.push_back(&oc.data);
+    if (lt->total == 0) continue;
+    oc.child_data_bytes = (size_t)lt->total * v.width;
+    sizes.push_back(oc.child_data_bytes);
+    dests.push_back(&oc.child_data);
+    if (lt->child_nullable) {
+      oc.child_mask_bytes = (size_t)bitmask_bytes(lt->total);
+      sizes.push_back(oc.child_mask_bytes);
+      dests.push_back(reinterpret_cast<void**>(&oc.child_null_mask));
+    }
+  }
+  auto clear_ptrs = [&] {
+    for (int c = 0; c < nk; ++c) out_keys[c].data = out_keys[c].null_mask = nullptr;
+    for (int o = 0; o < no; ++o) {
+      out_results[o].data = out_results[o].null_mask = nullptr;
+      out_results[o].child_data = out_results[o].child_null_mask = nullptr;
+    }
+  };
   std::vector<void*> ptrs(sizes.size(), nullptr);
   if (!al.get_many((int)sizes.size(), sizes.data(), ptrs.data())) {
     al.release_all();
-    for (int c = 0; c < nk; ++c) out_keys[c].data = out_keys[c].null_mask = nullptr;
-    for (int o = 0; o < no; ++o) out_results[o].data = out_results[o].null_mask = nullptr;
+    clear_ptrs();
     return fail(MGB_ERROR_ALLOC, "allocation of the output columns failed");
   }
   for (size_t i = 0; i < ptrs.size(); ++i) *dests[i] = ptrs[i];
   mgb_status st = copy_outputs(ws, plan, t, groups, out_keys, out_results, stream);
   if (st != MGB_SUCCESS) {
     al.release_all();
-    for (int c = 0; c < nk; ++c) out_keys[c].data = out_keys[c].null_mask = nullptr;
-    for (int o = 0; o < no; ++o) out_results[o].data = out_results[o].null_mask = nullptr;
+    clear_ptrs();
   }
   return st;
 }
@@ -1903,7 +1994,7 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
     const Plan::Out& out = plan.outs[o];
     if (out.spec.fin != FIN_NONE) continue;
     Ordering want{out.col, SORT_BY_VALUE};
-    if (out.kind == MGB_AGG_NTH_ELEMENT)
+    if (out.kind == MGB_AGG_NTH_ELEMENT || out.kind == MGB_AGG_COLLECT_LIST)
       want = out.agg.null_policy == MGB_NULL_INCLUDE ? Ordering{-1, SORT_ROWS_ALL} : Ordering{out.col, SORT_ROWS_VALID};
     int k = 0;
     while (k < (int)orderings.size() && !(orderings[k].col == want.col && orderings[k].mode == want.mode)) ++k;
@@ -1925,8 +2016,8 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
   const size_t tcub = align_up(std::max({tk, t64, t64b, t32, tscan}), 256);
   const size_t r4 = align_up((size_t)R * 4, 256), r8 = align_up((size_t)R * 8, 256), g4 = align_up((size_t)G * 4, 256);
   const size_t o4 = align_up(((size_t)G + 1) * 4, 256);
-  // gid | A, B (8 bytes a row) | C, D, E (4 bytes a row) | offset, has_null | cub
-  const size_t need = 4 * r4 + 2 * r8 + o4 + g4 + tcub;
+  // gid | A, B (8 bytes a row) | C, D, E (4 bytes a row) | offset, has_null, group nulls, list lengths | cub
+  const size_t need = 4 * r4 + 2 * r8 + o4 + 3 * g4 + tcub;
   MGB_RESERVE(ws->sorts, need);
   uint8_t* b = static_cast<uint8_t*>(ws->sorts.ptr);
   size_t off = 0;
@@ -1939,7 +2030,42 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
   uint32_t* E = reinterpret_cast<uint32_t*>(take(r4));
   uint32_t* offset = reinterpret_cast<uint32_t*>(take(o4));
   uint32_t* has_null = reinterpret_cast<uint32_t*>(take(g4));
+  uint32_t* gnulls = reinterpret_cast<uint32_t*>(take(g4));
+  uint32_t* lens = reinterpret_cast<uint32_t*>(take(g4));
   void* cub_tmp = take(tcub);
+
+  // COLLECT_LIST / COLLECT_SET results: offsets, elements and their validity
+  // bytes per list output, and two counters (elements, null elements)
+  size_t list_bytes = 0;
+  for (const Plan::Out& out : plan.outs)
+    if (out.kind == MGB_AGG_COLLECT_LIST || out.kind == MGB_AGG_COLLECT_SET)
+      list_bytes += o4 + align_up((size_t)R * plan.cols[out.col].col.width, 256) + align_up((size_t)R, 256) + 256;
+  temp.lists.clear();
+  if (list_bytes) {
+    MGB_RESERVE(ws->lists, list_bytes);
+    uint8_t* lb = static_cast<uint8_t*>(ws->lists.ptr);
+    size_t loff = 0;
+    for (size_t o = 0; o < plan.outs.size(); ++o) {
+      const Plan::Out& out = plan.outs[o];
+      if (out.kind != MGB_AGG_COLLECT_LIST && out.kind != MGB_AGG_COLLECT_SET) continue;
+      const ValCol& v = plan.cols[out.col].col;
+      ListTemp lt;
+      lt.out = (int)o;
+      lt.offsets = reinterpret_cast<uint32_t*>(lb + loff);
+      loff += o4;
+      lt.child = lb + loff;
+      loff += align_up((size_t)R * v.width, 256);
+      lt.child_valid = lb + loff;
+      loff += align_up((size_t)R, 256);
+      loff += 256;  // the counters (elements, null elements) sit before the next output
+      lt.child_nullable = out.agg.null_policy == MGB_NULL_INCLUDE && v.mask != nullptr;
+      temp.lists.push_back(lt);
+    }
+  }
+  auto list_counters = [&](const ListTemp& lt) {
+    return reinterpret_cast<unsigned long long*>(lt.child_valid + align_up((size_t)R, 256));
+  };
+  for (const ListTemp& lt : temp.lists) MGB_HIP(hipMemsetAsync(list_counters(lt), 0, 16, stream));
 
   const int cus = device_cu_count();
   const unsigned grid_rows = (unsigned)std::max<int64_t>(1, std::min<int64_t>((R + kBlock - 1) / kBlock, (int64_t)cus * 16));
@@ -2015,7 +2141,51 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
       p.out = temp.out_data[o];
       p.out_valid = temp.out_valid[o];
       p.out_width = out.spec.out_width;
-      if (!p.out_valid) return fail(MGB_ERROR_HIP, "sort-path output without validity");
+      const bool is_list = out.kind == MGB_AGG_COLLECT_LIST || out.kind == MGB_AGG_COLLECT_SET;
+      if (!p.out_valid && !is_list) return fail(MGB_ERROR_HIP, "sort-path output without validity");
+      if (is_list) {
+        const ListTemp* lt = nullptr;
+        for (const ListTemp& l : temp.lists)
+          if (l.out == (int)o) lt = &l;
+        if (!lt) return fail(MGB_ERROR_HIP, "list output without its temporaries");
+        CollectParams cp{};
+        cp.v = p.v;
+        cp.s = view;
+        cp.list_off = lt->offsets;
+        cp.child = lt->child;
+        cp.child_valid = lt->child_valid;
+        cp.include_nulls = out.agg.null_policy == MGB_NULL_INCLUDE ? 1 : 0;
+        cp.nulls_unequal = out.agg.ddof == MGB_NULL_UNEQUAL ? 1 : 0;
+        if (out.kind == MGB_AGG_COLLECT_LIST) {
+          // the ordering's group bounds are the lists' offsets
+          MGB_HIP(hipMemcpyAsync(lt->offsets, offset, (size_t)(G + 1) * 4, hipMemcpyDeviceToDevice, stream));
+          collect_gather_kernel<<<grid_rows, kBlock, 0, stream>>>(cp);
+          if (lt->child_nullable)
+            list_nulls_kernel<<<grid_rows, kBlock, 0, stream>>>(p.v, gid, R, G, list_counters(*lt) + 1);
+        } else {
+          const int32_t is_float = p.v.vclass == VC_F32 || p.v.vclass == VC_F64;
+          distinct_start_kernel<<<grid_rows, kBlock, 0, stream>>>(view, is_float, C,
+                                                                  out.agg.interpolation == MGB_NAN_UNEQUAL ? 1 : 0);
+          MGB_HIP(hipGetLastError());
+          tb = tcub;
+          MGB_HIP(hipcub::DeviceScan::InclusiveSum(cub_tmp, tb, C, D, n, stream));
+          scanned = false;  // D no longer holds NUNIQUE's sums
+          MGB_HIP(hipMemsetAsync(gnulls, 0, (size_t)G * 4, stream));
+          if (p.v.mask) group_nulls_kernel<<<grid_rows, kBlock, 0, stream>>>(p.v, gid, R, G, gnulls);
+          cp.prefix = D;
+          cp.flag = C;
+          cp.nulls = gnulls;
+          set_lengths_kernel<<<grid_groups, kBlock, 0, stream>>>(cp);
+          MGB_HIP(hipGetLastError());
+          tb = tcub;
+          MGB_HIP(hipcub::DeviceScan::InclusiveSum(cub_tmp, tb, lt->offsets + 1, lens, (int)G, stream));
+          MGB_HIP(hipMemcpyAsync(lt->offsets + 1, lens, (size_t)G * 4, hipMemcpyDeviceToDevice, stream));
+          set_values_kernel<<<grid_rows, kBlock, 0, stream>>>(cp);
+          set_nulls_kernel<<<grid_groups, kBlock, 0, stream>>>(cp, list_counters(*lt) + 1);
+        }
+        MGB_HIP(hipGetLastError());
+        continue;
+      }
       if (out.kind == MGB_AGG_NUNIQUE) {
         if (!scanned) {
           // C is free once the sorts are done; D holds sorted groups only in
@@ -2033,6 +2203,22 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
         sort_select_kernel<<<grid_groups, kBlock, 0, stream>>>(p);
       }
       MGB_HIP(hipGetLastError());
+    }
+  }
+  if (!temp.lists.empty()) {
+    // the lists' element and null-element counts: one more wait
+    const int nl = (int)temp.lists.size();
+    if (2 * nl > kPinnedNulls) return fail(MGB_ERROR_UNSUPPORTED, "too many list aggregations in one call");
+    uint32_t* totals = reinterpret_cast<uint32_t*>(ws->pinned);
+    for (int l = 0; l < nl; ++l) {
+      const ListTemp& lt = temp.lists[l];
+      MGB_HIP(hipMemcpyAsync(totals + l, lt.offsets + G, 4, hipMemcpyDeviceToHost, stream));
+      MGB_HIP(hipMemcpyAsync(ws->pinned + nl + l, list_counters(lt) + 1, 8, hipMemcpyDeviceToHost, stream));
+    }
+    MGB_HIP(hipStreamSynchronize(stream));
+    for (int l = 0; l < nl; ++l) {
+      temp.lists[l].total = totals[l];
+      temp.lists[l].child_nulls = (int64_t)ws->pinned[nl + l];
     }
   }
   return MGB_SUCCESS;
@@ -2964,8 +3150,11 @@ void empty_result(const mgb_column* keys, int32_t num_keys, const mgb_request* r
   for (int c = 0; c < num_keys; ++c) reset(out_keys[c], keys[c].type_id);
   int j = 0;
   for (int r = 0; r < num_requests; ++r)
-    for (int a = 0; a < requests[r].num_aggs; ++a, ++j)
-      reset(out_results[j], target_type(requests[r].values.type_id, requests[r].aggs[a].kind));
+    for (int a = 0; a < requests[r].num_aggs; ++a, ++j) {
+      const int tt = target_type(requests[r].values.type_id, requests[r].aggs[a].kind);
+      reset(out_results[j], tt);
+      if (tt == MGB_TYPE_LIST) out_results[j].child_type_id = requests[r].values.type_id;
+    }
 }
 
 }  // namespace mgb
@@ -3064,6 +3253,10 @@ extern "C" mgb_status mgb_groupby_aggregate_noalloc(const mgb_column* keys, int3
   Plan plan;
   st = build_plan(keys, num_keys, null_policy, requests, num_requests, plan);
   if (st != MGB_SUCCESS) return st;
+  for (const Plan::Out& o : plan.outs)
+    if (o.kind == MGB_AGG_COLLECT_LIST || o.kind == MGB_AGG_COLLECT_SET)
+      return fail(MGB_ERROR_UNSUPPORTED, "mgb_groupby_aggregate_noalloc: COLLECT_LIST / COLLECT_SET results need "
+                                         "allocated outputs");
   std::lock_guard<std::mutex> lk(g_scratch_mu);
   uint8_t* region = nullptr;
   st = scratch_address(region);

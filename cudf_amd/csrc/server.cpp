@@ -234,8 +234,12 @@ void free_out(Arena* arena, std::vector<mgb_out_column>& cols)
   for (auto& c : cols) {
     arena->deallocate(c.data);
     arena->deallocate(c.null_mask);
+    arena->deallocate(c.child_data);
+    arena->deallocate(c.child_null_mask);
     c.data = nullptr;
     c.null_mask = nullptr;
+    c.child_data = nullptr;
+    c.child_null_mask = nullptr;
   }
 }
 
@@ -405,7 +409,8 @@ bool mgb_server::launch(Slot& s)
     if (d.status != MGB_SUCCESS) d.error = mgb_last_error_message();
     d.latency_ms = ms_since(q.arrival, Clock::now());
     for (auto& c : d.keys) d.bytes += (int64_t)(c.data_bytes + c.mask_bytes);
-    for (auto& c : d.results) d.bytes += (int64_t)(c.data_bytes + c.mask_bytes);
+    for (auto& c : d.results)
+      d.bytes += (int64_t)(c.data_bytes + c.mask_bytes + c.child_data_bytes + c.child_mask_bytes);
     std::lock_guard<std::mutex> lk(mu);
     held_bytes += d.bytes;
     ++stats.completed;
@@ -649,7 +654,8 @@ extern "C" mgb_status mgb_server_submit(mgb_server* s, int32_t table_id, const i
       for (auto& a : q.aggs.back())
         q.plain_only |= a.kind == MGB_AGG_VARIANCE || a.kind == MGB_AGG_STD || a.kind == MGB_AGG_M2 ||
                         a.kind == MGB_AGG_ARGMIN || a.kind == MGB_AGG_ARGMAX || a.kind == MGB_AGG_MEDIAN ||
-                        a.kind == MGB_AGG_QUANTILE || a.kind == MGB_AGG_NUNIQUE || a.kind == MGB_AGG_NTH_ELEMENT;
+                        a.kind == MGB_AGG_QUANTILE || a.kind == MGB_AGG_NUNIQUE || a.kind == MGB_AGG_NTH_ELEMENT ||
+                        a.kind == MGB_AGG_COLLECT_LIST || a.kind == MGB_AGG_COLLECT_SET;
     }
     // null keys grouped as a key (null_policy INCLUDE): plain path too
     if (null_policy == MGB_NULL_INCLUDE)
@@ -763,6 +769,19 @@ extern "C" mgb_status mgb_server_take_result(mgb_server* s, int64_t ticket, mgb_
       dst = src;
       dst.data = nullptr;
       dst.null_mask = nullptr;
+      dst.child_data = nullptr;
+      dst.child_null_mask = nullptr;
+      if (src.child_data) {
+        dst.child_data = alloc->allocate(alloc->ctx, src.child_data_bytes, stream);
+        if (!dst.child_data) return fail(MGB_ERROR_ALLOC, "allocation of a list result's values failed");
+        MGB_HIP(hipMemcpyAsync(dst.child_data, src.child_data, src.child_data_bytes, hipMemcpyDeviceToDevice, hs));
+      }
+      if (src.child_null_mask) {
+        dst.child_null_mask = static_cast<uint32_t*>(alloc->allocate(alloc->ctx, src.child_mask_bytes, stream));
+        if (!dst.child_null_mask) return fail(MGB_ERROR_ALLOC, "allocation of a list result's null mask failed");
+        MGB_HIP(hipMemcpyAsync(dst.child_null_mask, src.child_null_mask, src.child_mask_bytes,
+                               hipMemcpyDeviceToDevice, hs));
+      }
       if (src.data) {
         dst.data = alloc->allocate(alloc->ctx, src.data_bytes, stream);
         if (!dst.data) return fail(MGB_ERROR_ALLOC, "allocation of a result column failed");

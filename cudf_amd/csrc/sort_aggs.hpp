@@ -231,13 +231,17 @@ __global__ __launch_bounds__(kBlock) void sort_select_kernel(const SortOutParams
 // NUNIQUE: 1 at every sorted position that starts its group or differs from
 // the previous value (group_nunique.cu:60-70; NaN == NaN, -0.0 == +0.0, as
 // cudf's row equality).
-__global__ __launch_bounds__(kBlock) void distinct_start_kernel(const SortView s, int32_t is_float, uint32_t* flag)
+// nan_distinct (COLLECT_SET under nan_equality UNEQUAL): every NaN starts an
+// element of its own.
+__global__ __launch_bounds__(kBlock) void distinct_start_kernel(const SortView s, int32_t is_float, uint32_t* flag,
+                                                               int32_t nan_distinct = 0)
 {
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
   const uint64_t nz = neg_zero_code(s.narrow != 0);
+  const uint64_t nan_code = s.narrow ? 0xffffffffull : ~0ull;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += step) {
     uint32_t f = 1;
-    if (i > 0) {
+    if (i > 0 && !(nan_distinct && is_float && view_code(s, i) == nan_code)) {
       if (s.narrow) {
         const uint64_t a = s.key64[i - 1], b = s.key64[i];
         const bool zeros = is_float && (a >> 32) == (b >> 32) && (a & 0xffffffffull) == nz && (b & 0xffffffffull) == nz + 1;
@@ -262,6 +266,133 @@ __global__ __launch_bounds__(kBlock) void nunique_finish_kernel(const SortOutPar
   if (p.nulls && p.has_null[g]) c += 1u;
   reinterpret_cast<uint32_t*>(p.out)[g] = c;
   p.out_valid[g] = 1;
+}
+
+// ---- COLLECT_LIST and COLLECT_SET (cpp/src/groupby/sort/group_collect.cu,
+// cpp/src/groupby/sort/aggregate.cpp collect_set: the lists' elements then
+// lists::distinct)
+
+// Null values per group (gid: every row's group, `groups` when dropped).
+__global__ __launch_bounds__(kBlock) void group_nulls_kernel(const ValCol v, const uint32_t* gid, int64_t n,
+                                                            uint32_t groups, uint32_t* nulls)
+{
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n; row += step) {
+    const uint32_t g = gid[row];
+    if (g < groups && !bit_valid(v.mask, v.bit_off + row)) atomicAdd(&nulls[g], 1u);
+  }
+}
+
+// The raw bits of a value from its sort code (sort_code32 / sort_code64
+// inverted; a NaN's code gives a quiet NaN).
+__device__ __forceinline__ uint64_t code_bits(uint64_t c, bool narrow, int vclass)
+{
+  if (narrow) {
+    const uint32_t c32 = (uint32_t)c;
+    if (vclass == VC_F32) return (c32 >> 31) ? (c32 & 0x7fffffffu) : ~c32;
+    if (vclass == VC_SIGNED) return (uint64_t)(uint32_t)(c32 ^ 0x80000000u);
+    return c32;
+  }
+  if (vclass == VC_F64) return (c >> 63) ? (c & ~(1ull << 63)) : ~c;
+  if (vclass == VC_SIGNED) return c ^ (1ull << 63);
+  return c;
+}
+
+struct CollectParams {
+  ValCol v;
+  SortView s;
+  const uint32_t* prefix;   // COLLECT_SET: inclusive sum of the distinct-value starts
+  const uint32_t* flag;     // COLLECT_SET: the starts
+  const uint32_t* nulls;    // COLLECT_SET: null values per group
+  uint32_t* list_off;       // groups + 1 entries: first element of every list
+  uint8_t* child;           // elements (v.width bytes each)
+  uint8_t* child_valid;     // validity byte per element
+  int32_t include_nulls;    // null_policy INCLUDE
+  int32_t nulls_unequal;    // COLLECT_SET: null_equality UNEQUAL (each null an element)
+};
+
+// COLLECT_LIST: element i is the value of sorted row i (the group's rows in
+// row order, or its valid rows): a gather up to the last group's end.
+__global__ __launch_bounds__(kBlock) void collect_gather_kernel(const CollectParams p)
+{
+  const int64_t total = p.s.offset[p.s.groups];
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += step) {
+    const uint32_t row = p.s.order[i];
+    const bool ok = bit_valid(p.v.mask, p.v.bit_off + row);
+    store_width(p.child, i, p.v.width, ok ? load_at(p.v.data, p.v.width, p.v.stride, row) : 0ull);
+    p.child_valid[i] = ok;
+  }
+}
+
+// The group's distinct values and the elements its nulls take.
+__device__ __forceinline__ uint32_t set_distinct(const CollectParams& p, int64_t g)
+{
+  const uint32_t lo = p.s.offset[g], hi = p.s.offset[g + 1];
+  return hi > lo ? p.prefix[hi - 1] - (lo ? p.prefix[lo - 1] : 0u) : 0u;
+}
+__device__ __forceinline__ uint32_t set_null_elems(const CollectParams& p, int64_t g)
+{
+  const uint32_t n = p.nulls[g];
+  return !p.include_nulls ? 0u : p.nulls_unequal ? n : (n ? 1u : 0u);
+}
+
+// COLLECT_SET: each list's length into list_off[g + 1] (list_off[0] = 0); an
+// inclusive scan of list_off[1..] then gives the offsets.
+__global__ __launch_bounds__(kBlock) void set_lengths_kernel(const CollectParams p)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0) p.list_off[0] = 0;
+  if (g >= p.s.groups) return;
+  p.list_off[g + 1] = set_distinct(p, g) + set_null_elems(p, g);
+}
+
+// COLLECT_SET: every distinct value (the start of a run of equal sorted
+// values) at its place in its group's list.
+__global__ __launch_bounds__(kBlock) void set_values_kernel(const CollectParams p)
+{
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  const bool narrow = p.s.narrow != 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.s.n; i += step) {
+    const uint32_t g = view_group(p.s, i);
+    if (g >= p.s.groups || !p.flag[i]) continue;
+    const uint32_t lo = p.s.offset[g];
+    const uint32_t k = p.prefix[i] - 1u - (lo ? p.prefix[lo - 1] : 0u);
+    const int64_t e = (int64_t)p.list_off[g] + k;
+    store_width(p.child, e, p.v.width, code_bits(view_code(p.s, i), narrow, p.v.vclass));
+    p.child_valid[e] = 1;
+  }
+}
+
+// COLLECT_SET: the null elements at the end of every list (one thread per group).
+// The null elements are counted into *child_nulls.
+__global__ __launch_bounds__(kBlock) void set_nulls_kernel(const CollectParams p, unsigned long long* child_nulls)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t k = 0;
+  if (g < p.s.groups) {
+    const int64_t first = (int64_t)p.list_off[g] + set_distinct(p, g);
+    k = set_null_elems(p, g);
+    for (uint32_t j = 0; j < k; ++j) {
+      store_width(p.child, first + j, p.v.width, 0ull);
+      p.child_valid[first + j] = 0;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) k += __shfl_down(k, o, 64);
+  if ((threadIdx.x & 63) == 0 && k) atomicAdd(child_nulls, (unsigned long long)k);
+}
+
+// COLLECT_LIST under INCLUDE: the null elements (null values of rows whose
+// key is kept) counted into *child_nulls.
+__global__ __launch_bounds__(kBlock) void list_nulls_kernel(const ValCol v, const uint32_t* gid, int64_t n,
+                                                           uint32_t groups, unsigned long long* child_nulls)
+{
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  uint32_t k = 0;
+  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n; row += step)
+    k += gid[row] < groups && !bit_valid(v.mask, v.bit_off + row);
+  for (int o = 32; o > 0; o >>= 1) k += __shfl_down(k, o, 64);
+  if ((threadIdx.x & 63) == 0 && k) atomicAdd(child_nulls, (unsigned long long)k);
 }
 
 }  // namespace mgb

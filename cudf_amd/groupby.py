@@ -12,9 +12,9 @@ import torch
 
 from . import _lib
 from .aggregation import Aggregation
-from .column import Column, Table
+from .column import Column, ListColumn, Table
 from .pool import DevicePool, default_pool
-from .types import NullPolicy, Sorted
+from .types import NullPolicy, Sorted, TypeId
 
 
 def _to_mgb(c: Column) -> _lib.mgb_column:
@@ -79,6 +79,21 @@ class _TorchAllocator:
             return self.live.pop(ptr)
 
 
+def wrap_out_column(alloc, o: _lib.mgb_out_column):
+    """Column (or ListColumn) over the device buffers of one result column
+    that the library allocated through `alloc`."""
+    data = alloc.take(o.data)
+    mask_t = alloc.take(o.null_mask)
+    mask = mask_t.view(torch.int32) if mask_t is not None else None
+    if o.type_id == TypeId.LIST:
+        offsets = Column(TypeId.INT32, o.size + 1 if o.size else 0, data)
+        cmask_t = alloc.take(o.child_null_mask)
+        child = Column(o.child_type_id, o.child_size, alloc.take(o.child_data),
+                       cmask_t.view(torch.int32) if cmask_t is not None else None, o.child_null_count)
+        return ListColumn(o.size, offsets, child)
+    return Column(o.type_id, o.size, data, mask, o.null_count)
+
+
 class GroupByRequest:
     """Values to aggregate and the aggregations to compute on them."""
 
@@ -133,11 +148,8 @@ class GroupBy:
                                        ctypes.c_void_p(stream.cuda_stream), ctypes.byref(alloc.struct))
         _lib.check(st)
 
-        def wrap(o: _lib.mgb_out_column) -> Column:
-            data = alloc.take(o.data)
-            mask_t = alloc.take(o.null_mask)
-            mask = mask_t.view(torch.int32) if mask_t is not None else None
-            return Column(o.type_id, o.size, data, mask, o.null_count)
+        def wrap(o: _lib.mgb_out_column):
+            return wrap_out_column(alloc, o)
 
         key_table = Table([wrap(out_keys[i]) for i in range(nk)])
         results = []

@@ -20,7 +20,7 @@ import torch
 from . import _lib
 from .aggregation import Aggregation
 from .column import Column, Table
-from .groupby import _TorchAllocator, _to_mgb
+from .groupby import _TorchAllocator, _to_mgb, wrap_out_column
 from .types import NullPolicy
 
 
@@ -93,10 +93,7 @@ class Server:
                                                     ctypes.c_void_p(s.cuda_stream), ctypes.byref(alloc.struct)))
 
         def wrap(o):
-            data = alloc.take(o.data)
-            mask_t = alloc.take(o.null_mask)
-            return Column(o.type_id, o.size, data, mask_t.view(torch.int32) if mask_t is not None else None,
-                          o.null_count)
+            return wrap_out_column(alloc, o)
 
         keys = Table([wrap(out_keys[i]) for i in range(nk)])
         results, j = [], 0

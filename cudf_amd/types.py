@@ -30,6 +30,7 @@ class TypeId(enum.IntEnum):
     DURATION_MILLISECONDS = 19
     DURATION_MICROSECONDS = 20
     DURATION_NANOSECONDS = 21
+    LIST = 24  # results of COLLECT_LIST / COLLECT_SET (ListColumn)
 
 
 class NullPolicy(enum.IntEnum):
@@ -44,6 +45,18 @@ class Interpolation(enum.IntEnum):
     HIGHER = 2
     MIDPOINT = 3
     NEAREST = 4
+
+
+class NullEquality(enum.IntEnum):
+    """cudf::null_equality (cpp/include/cudf/types.hpp)."""
+    EQUAL = 0
+    UNEQUAL = 1
+
+
+class NanEquality(enum.IntEnum):
+    """cudf::nan_equality (cpp/include/cudf/types.hpp)."""
+    ALL_EQUAL = 0
+    UNEQUAL = 1
 
 
 class Sorted(enum.IntEnum):
@@ -71,6 +84,10 @@ class Kind(enum.IntEnum):
     ARGMIN = 16
     NUNIQUE = 17
     NTH_ELEMENT = 18
+    ROW_NUMBER = 19
+    RANK = 20
+    COLLECT_LIST = 21
+    COLLECT_SET = 22
 
 
 # numpy storage dtype of each fixed-width type

@@ -1,5 +1,5 @@
 // Aggregation objects and factories used by cudf::groupby (subset of
-// cpp/include/cudf/aggregation.hpp:80-306 in cudf 24.04: same class names,
+// cpp/include/cudf/aggregation.hpp:80-306 and 505-528 in cudf 24.04: same class names,
 // Kind numbering and factory signatures).
 #pragma once
 
@@ -124,5 +124,11 @@ template <typename Base = aggregation>
 std::unique_ptr<Base> make_nunique_aggregation(null_policy null_handling = null_policy::EXCLUDE);
 template <typename Base = aggregation>
 std::unique_ptr<Base> make_nth_element_aggregation(size_type n, null_policy null_handling = null_policy::INCLUDE);
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_collect_list_aggregation(null_policy null_handling = null_policy::INCLUDE);
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_collect_set_aggregation(null_policy null_handling = null_policy::INCLUDE,
+                                                   null_equality nulls_equal = null_equality::EQUAL,
+                                                   nan_equality nans_equal = nan_equality::ALL_EQUAL);
 
 }  // namespace cudf

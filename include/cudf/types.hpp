@@ -21,6 +21,7 @@ enum class null_policy : bool { EXCLUDE, INCLUDE };
 enum class interpolation : int32_t { LINEAR, LOWER, HIGHER, MIDPOINT, NEAREST };
 enum class nan_policy : bool { NAN_IS_NULL, NAN_IS_VALID };
 enum class null_equality : bool { EQUAL, UNEQUAL };
+enum class nan_equality { ALL_EQUAL, UNEQUAL };
 enum class null_order : bool { AFTER, BEFORE };
 enum class sorted : bool { NO, YES };
 enum class mask_state : int32_t { UNALLOCATED, UNINITIALIZED, ALL_VALID, ALL_NULL };

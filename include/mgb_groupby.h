@@ -51,7 +51,8 @@ enum {
   MGB_TYPE_TIMESTAMP_DAYS = 12, MGB_TYPE_TIMESTAMP_SECONDS = 13, MGB_TYPE_TIMESTAMP_MILLISECONDS = 14,
   MGB_TYPE_TIMESTAMP_MICROSECONDS = 15, MGB_TYPE_TIMESTAMP_NANOSECONDS = 16,
   MGB_TYPE_DURATION_DAYS = 17, MGB_TYPE_DURATION_SECONDS = 18, MGB_TYPE_DURATION_MILLISECONDS = 19,
-  MGB_TYPE_DURATION_MICROSECONDS = 20, MGB_TYPE_DURATION_NANOSECONDS = 21
+  MGB_TYPE_DURATION_MICROSECONDS = 20, MGB_TYPE_DURATION_NANOSECONDS = 21,
+  MGB_TYPE_LIST = 24  /* results only: COLLECT_LIST / COLLECT_SET (offsets + child, see mgb_out_column) */
 };
 
 /* cudf::aggregation::Kind subset handled here (same numeric values). */
@@ -60,8 +61,12 @@ enum {
   MGB_AGG_COUNT_VALID = 4, MGB_AGG_COUNT_ALL = 5, MGB_AGG_SUM_OF_SQUARES = 8,
   MGB_AGG_MEAN = 9, MGB_AGG_M2 = 10, MGB_AGG_VARIANCE = 11, MGB_AGG_STD = 12,
   MGB_AGG_MEDIAN = 13, MGB_AGG_QUANTILE = 14, MGB_AGG_ARGMAX = 15, MGB_AGG_ARGMIN = 16,
-  MGB_AGG_NUNIQUE = 17, MGB_AGG_NTH_ELEMENT = 18
+  MGB_AGG_NUNIQUE = 17, MGB_AGG_NTH_ELEMENT = 18, MGB_AGG_COLLECT_LIST = 21, MGB_AGG_COLLECT_SET = 22
 };
+
+/* cudf::null_equality and cudf::nan_equality (cpp/include/cudf/types.hpp), for COLLECT_SET. */
+enum { MGB_NULL_EQUAL = 0, MGB_NULL_UNEQUAL = 1 };
+enum { MGB_NAN_ALL_EQUAL = 0, MGB_NAN_UNEQUAL = 1 };
 
 /* cudf::interpolation (cpp/include/cudf/types.hpp). */
 enum { MGB_INTERP_LINEAR = 0, MGB_INTERP_LOWER = 1, MGB_INTERP_HIGHER = 2, MGB_INTERP_MIDPOINT = 3,
@@ -86,10 +91,12 @@ typedef struct mgb_column {
  * defaults except where noted. */
 typedef struct mgb_aggregation {
   int32_t kind;
-  int32_t ddof;           /* VARIANCE / STD: ddof; NTH_ELEMENT: n (negative: from the end) */
-  int32_t null_policy;    /* NUNIQUE, NTH_ELEMENT: MGB_NULL_EXCLUDE / MGB_NULL_INCLUDE (cudf's
-                             defaults are EXCLUDE for NUNIQUE and INCLUDE for NTH_ELEMENT) */
-  int32_t interpolation;  /* QUANTILE: MGB_INTERP_* */
+  int32_t ddof;           /* VARIANCE / STD: ddof; NTH_ELEMENT: n (negative: from the end);
+                             COLLECT_SET: MGB_NULL_EQUAL / MGB_NULL_UNEQUAL */
+  int32_t null_policy;    /* NUNIQUE, NTH_ELEMENT, COLLECT_LIST, COLLECT_SET: MGB_NULL_EXCLUDE /
+                             MGB_NULL_INCLUDE (cudf's defaults are EXCLUDE for NUNIQUE and INCLUDE for
+                             NTH_ELEMENT, COLLECT_LIST and COLLECT_SET) */
+  int32_t interpolation;  /* QUANTILE: MGB_INTERP_*; COLLECT_SET: MGB_NAN_ALL_EQUAL / MGB_NAN_UNEQUAL */
   double quantile;        /* QUANTILE: one quantile in [0, 1] (MEDIAN is 0.5, LINEAR) */
 } mgb_aggregation;
 
@@ -126,7 +133,11 @@ typedef struct mgb_allocator {
   const struct mgb_allocator* temporaries;
 } mgb_allocator;
 
-/* An owned result column written by the library. */
+/* An owned result column written by the library. A LIST column
+ * (MGB_TYPE_LIST, the results of COLLECT_LIST and COLLECT_SET; cudf's lists
+ * column, cpp/include/cudf/lists/lists_column_view.hpp) keeps its size + 1
+ * int32 offsets in `data` and never has nulls itself; its elements are the
+ * child column described by the child_* fields (zero for other columns). */
 typedef struct mgb_out_column {
   int32_t type_id;
   int32_t reserved;
@@ -136,6 +147,14 @@ typedef struct mgb_out_column {
   int64_t null_count;
   size_t data_bytes;   /* bytes allocated for data (for deallocate) */
   size_t mask_bytes;   /* bytes allocated for null_mask */
+  int32_t child_type_id;
+  int32_t child_reserved;
+  int64_t child_size;
+  void* child_data;          /* allocated with the caller's allocator; NULL when child_size == 0 */
+  uint32_t* child_null_mask; /* allocated with the caller's allocator, or NULL */
+  int64_t child_null_count;
+  size_t child_data_bytes;
+  size_t child_mask_bytes;
 } mgb_out_column;
 
 /* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
@@ -169,14 +188,22 @@ int32_t mgb_target_type(int32_t value_type_id, int32_t agg_kind);
  * waits for `stream` first. Errors: MGB_ERROR_LOGIC for a
  * size mismatch or an invalid type/aggregation pair (groupby.cu:208-212,
  * 173-190); MGB_ERROR_UNSUPPORTED for more rows than cudf::size_type holds
- * and for the list-valued and nested aggregations (COLLECT_LIST,
- * COLLECT_SET, MERGE_*, TDIGEST, ...); MGB_ERROR_ALLOC when the allocator
- * returns NULL. VARIANCE, STD, ARGMIN and ARGMAX (hash aggregations in
- * cpp/src/groupby/hash/groupby.cu:73-84) and M2 (sort-only in cudf,
- * cpp/src/groupby/sort/group_m2.cu) take a second pass over the rows;
- * MEDIAN, QUANTILE, NUNIQUE and NTH_ELEMENT (sort-only in cudf,
- * cpp/src/groupby/sort/aggregate.cpp) sort each value column's rows by
- * group.
+ * and for the aggregations of nested values or nested results other than
+ * COLLECT_LIST / COLLECT_SET (MERGE_*, TDIGEST, HISTOGRAM, ...);
+ * MGB_ERROR_ALLOC when the allocator returns NULL. VARIANCE, STD, ARGMIN and
+ * ARGMAX (hash aggregations in cpp/src/groupby/hash/groupby.cu:73-84) and M2
+ * (sort-only in cudf, cpp/src/groupby/sort/group_m2.cu) take a second pass
+ * over the rows; MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT, COLLECT_LIST and
+ * COLLECT_SET (sort-only in cudf, cpp/src/groupby/sort/aggregate.cpp) sort
+ * each value column's rows by group. COLLECT_LIST gives each group's values
+ * in row order (null values as null elements, or dropped under
+ * MGB_NULL_EXCLUDE); COLLECT_SET its distinct values in ascending order
+ * (NaN after every number, -0.0 == +0.0 with -0.0 kept when present, NaNs
+ * one element or each its own under MGB_NAN_UNEQUAL), then its nulls (one
+ * under MGB_NULL_EQUAL, each under MGB_NULL_UNEQUAL, none under
+ * MGB_NULL_EXCLUDE). Both are LIST columns of the value type (see
+ * mgb_out_column); calls with them wait on `stream` once more, for their
+ * element counts.
  */
 mgb_status mgb_groupby_aggregate(const mgb_column* keys, int32_t num_keys, int32_t null_policy,
                                  const mgb_request* requests, int32_t num_requests,

@@ -32,6 +32,11 @@
  *       cpp/include/cudf/detail/utilities/device_operators.cuh:63-200
  *       cpp/include/cudf/fixed_point/temporary.hpp:62-72 (min / max ops)
  *   - target types: cpp/include/cudf/detail/aggregation/aggregation.hpp:1168-1424
+ *   - the sort-path aggregations (MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT,
+ *     COLLECT_LIST, COLLECT_SET) over each group's rows in row order:
+ *       cpp/src/groupby/sort/aggregate.cpp, group_quantiles.cu,
+ *       group_nunique.cu, group_nth_element.cu, group_collect.cu,
+ *       cpp/src/lists/stream_compaction/distinct.cu
  *
  * The rows are processed in order, so float sums are the naive left-to-right
  * sums and float MIN/MAX follow the CAS-loop operator `old < x ? old : x`
@@ -42,7 +47,10 @@
  * Spark golden vectors in cpp/tests/hashing/spark_murmurhash3_x86_32_test.cpp;
  * hash_combine against the list-column vectors of
  * cpp/tests/hashing/murmurhash3_x86_32_test.cpp:123-160; the groupby against
- * the literal fixtures of the cpp/tests/groupby test files, brute force and pandas.
+ * the literal fixtures of the cpp/tests/groupby test files, brute force and pandas;
+ * COLLECT_LIST / COLLECT_SET against the fixtures of
+ * cpp/tests/groupby/collect_list_tests.cpp and collect_set_tests.cpp and
+ * against set and list identities on random inputs.
  */
 #include <math.h>
 #include <stdint.h>
@@ -60,8 +68,10 @@ enum {
 enum {
   K_SUM = 0, K_PRODUCT = 1, K_MIN = 2, K_MAX = 3, K_COUNT_VALID = 4, K_COUNT_ALL = 5,
   K_SUM_OF_SQUARES = 8, K_MEAN = 9, K_M2 = 10, K_VARIANCE = 11, K_STD = 12,
-  K_MEDIAN = 13, K_QUANTILE = 14, K_ARGMAX = 15, K_ARGMIN = 16, K_NUNIQUE = 17, K_NTH_ELEMENT = 18
+  K_MEDIAN = 13, K_QUANTILE = 14, K_ARGMAX = 15, K_ARGMIN = 16, K_NUNIQUE = 17, K_NTH_ELEMENT = 18,
+  K_COLLECT_LIST = 21, K_COLLECT_SET = 22
 };
+enum { T_LIST = 24 };
 
 typedef struct {
   int32_t type_id;
@@ -283,6 +293,8 @@ int mgo_target_type(int t, int kind)
       return T_FLOAT64;
     case K_NUNIQUE: return T_INT32;
     case K_NTH_ELEMENT: return t;
+    /* a list of the source type (aggregation.hpp:1335-1350) */
+    case K_COLLECT_LIST: case K_COLLECT_SET: return T_LIST;
     case K_SUM:
       if (is_integral(t)) return T_INT64;
       if (is_float(t) || is_duration(t)) return t;
@@ -433,26 +445,19 @@ static void sort_path_agg(const mgo_col* v, int kind, int ddof, int nulls, int i
   free(buf);
 }
 
-int mgo_groupby_aggregate_ex(const mgo_col* keys, int32_t n_keys, int64_t num_rows, int32_t include_null_keys,
-                             const mgo_col* values, int32_t n_values, const int32_t* req_col,
-                             const int32_t* req_kind, const int32_t* req_ddof, const int32_t* req_nulls,
-                             const int32_t* req_interp, const double* req_q, int32_t n_req,
-                             int64_t* out_rep_rows, int64_t* out_num_groups, void* const* out_data,
-                             uint8_t* const* out_valid)
+/* Groups of the key rows in order of first appearance: group_of_row[i] (-1
+ * for a row dropped for a null key), out_rep_rows[g] = first row of group g;
+ * returns the group count. Open addressing over row indices, linear probing. */
+static int64_t form_groups(const mgo_col* keys, int32_t n_keys, int64_t num_rows, int32_t include_null_keys,
+                           int64_t* out_rep_rows, int64_t* group_of_row)
 {
-  (void)n_values;
-  for (int32_t j = 0; j < n_req; ++j)
-    if (mgo_target_type(values[req_col[j]].type_id, req_kind[j]) < 0) return 1;
-
   int keys_have_nulls = 0;
   for (int32_t c = 0; c < n_keys; ++c) keys_have_nulls |= keys[c].mask != NULL;
   int skip_null_rows = keys_have_nulls && !include_null_keys;
 
-  /* open addressing over row indices, linear probing, empty = -1 */
   int64_t cap = 16;
   while (cap < 2 * num_rows) cap <<= 1;
   int64_t* slots = (int64_t*)malloc(sizeof(int64_t) * cap);
-  int64_t* group_of_row = (int64_t*)malloc(sizeof(int64_t) * (num_rows ? num_rows : 1));
   for (int64_t s = 0; s < cap; ++s) slots[s] = -1;
 
   int64_t ng = 0;
@@ -471,6 +476,37 @@ int mgo_groupby_aggregate_ex(const mgo_col* keys, int32_t n_keys, int64_t num_ro
     }
   }
   free(slots);
+  return ng;
+}
+
+/* Each group's rows in row order: grp_rows[grp_off[g]..grp_off[g+1]). */
+static void group_rows(const int64_t* group_of_row, int64_t num_rows, int64_t ng, int64_t* grp_off,
+                       int64_t* grp_rows)
+{
+  for (int64_t g = 0; g <= ng; ++g) grp_off[g] = 0;
+  for (int64_t r = 0; r < num_rows; ++r)
+    if (group_of_row[r] >= 0) grp_off[group_of_row[r] + 1]++;
+  for (int64_t g = 0; g < ng; ++g) grp_off[g + 1] += grp_off[g];
+  int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ng ? ng : 1));
+  for (int64_t g = 0; g < ng; ++g) fill[g] = grp_off[g];
+  for (int64_t r = 0; r < num_rows; ++r)
+    if (group_of_row[r] >= 0) grp_rows[fill[group_of_row[r]]++] = r;
+  free(fill);
+}
+
+int mgo_groupby_aggregate_ex(const mgo_col* keys, int32_t n_keys, int64_t num_rows, int32_t include_null_keys,
+                             const mgo_col* values, int32_t n_values, const int32_t* req_col,
+                             const int32_t* req_kind, const int32_t* req_ddof, const int32_t* req_nulls,
+                             const int32_t* req_interp, const double* req_q, int32_t n_req,
+                             int64_t* out_rep_rows, int64_t* out_num_groups, void* const* out_data,
+                             uint8_t* const* out_valid)
+{
+  (void)n_values;
+  for (int32_t j = 0; j < n_req; ++j)
+    if (mgo_target_type(values[req_col[j]].type_id, req_kind[j]) < 0) return 1;
+
+  int64_t* group_of_row = (int64_t*)malloc(sizeof(int64_t) * (num_rows ? num_rows : 1));
+  int64_t ng = form_groups(keys, n_keys, num_rows, include_null_keys, out_rep_rows, group_of_row);
   *out_num_groups = ng;
 
   size_t gsz = (size_t)(ng ? ng : 1);
@@ -485,16 +521,7 @@ int mgo_groupby_aggregate_ex(const mgo_col* keys, int32_t n_keys, int64_t num_ro
   /* each group's rows in row order (for the sort-path aggregations) */
   int64_t* grp_off = (int64_t*)calloc((size_t)ng + 1, sizeof(int64_t));
   int64_t* grp_rows = (int64_t*)malloc(sizeof(int64_t) * (size_t)(num_rows ? num_rows : 1));
-  for (int64_t r = 0; r < num_rows; ++r)
-    if (group_of_row[r] >= 0) grp_off[group_of_row[r] + 1]++;
-  for (int64_t g = 0; g < ng; ++g) grp_off[g + 1] += grp_off[g];
-  {
-    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ng ? ng : 1));
-    for (int64_t g = 0; g < ng; ++g) fill[g] = grp_off[g];
-    for (int64_t r = 0; r < num_rows; ++r)
-      if (group_of_row[r] >= 0) grp_rows[fill[group_of_row[r]]++] = r;
-    free(fill);
-  }
+  group_rows(group_of_row, num_rows, ng, grp_off, grp_rows);
 
   for (int32_t j = 0; j < n_req; ++j) {
     const mgo_col* v = &values[req_col[j]];
@@ -677,4 +704,82 @@ int mgo_groupby_aggregate(const mgo_col* keys, int32_t n_keys, int64_t num_rows,
   return mgo_groupby_aggregate_ex(keys, n_keys, num_rows, include_null_keys, values, n_values, req_col, req_kind,
                                   req_ddof, NULL, NULL, NULL, n_req, out_rep_rows, out_num_groups, out_data,
                                   out_valid);
+}
+
+/* ------------------------------------------------ COLLECT_LIST / COLLECT_SET */
+
+/*
+ * COLLECT_LIST (kind 21) or COLLECT_SET (kind 22) of one value column, with
+ * the groups of mgo_groupby_aggregate_ex (order of first appearance,
+ * out_rep_rows / out_num_groups as there). Group g's list holds the source
+ * rows out_rows[out_offsets[g] .. out_offsets[g+1]) (out_offsets: ng + 1
+ * entries; out_rows: capacity num_rows): an element's value and validity are
+ * those of its row.
+ *
+ * COLLECT_LIST: the group's rows in row order, null values kept
+ * (null_policy 1, INCLUDE) or dropped (0, EXCLUDE)
+ * (cpp/src/groupby/sort/group_collect.cu:36-115: the grouped values are the
+ * stable sort by key, so row order within a group).
+ * COLLECT_SET: that list, then its distinct elements
+ * (cpp/src/groupby/sort/aggregate.cpp:426-447 -> lists::distinct,
+ * cpp/src/lists/stream_compaction/distinct.cu:36-70: stable_distinct over
+ * {list label, element} with KEEP_ANY): values equal under cudf's row
+ * equality (-0.0 == +0.0; NaNs all equal under nans_equal 0 (ALL_EQUAL),
+ * each its own element under 1 (UNEQUAL)), nulls one element under
+ * nulls_equal 0 (EQUAL), each its own under 1 (UNEQUAL). Of the choices
+ * KEEP_ANY leaves open, this oracle keeps the first row of each run of equal
+ * elements, and the elements in order of first appearance; callers compare
+ * sets after sorting each list, as cpp/tests/groupby/collect_set_tests.cpp:
+ * 50-65 does.
+ * Returns 0, or 1 for a kind other than 21 / 22 or a type without a width.
+ */
+int mgo_groupby_collect(const mgo_col* keys, int32_t n_keys, int64_t num_rows, int32_t include_null_keys,
+                        const mgo_col* value, int32_t kind, int32_t null_policy, int32_t nulls_equal,
+                        int32_t nans_equal, int64_t* out_rep_rows, int64_t* out_num_groups, int64_t* out_offsets,
+                        int64_t* out_rows)
+{
+  if ((kind != K_COLLECT_LIST && kind != K_COLLECT_SET) || mgo_target_type(value->type_id, kind) < 0) return 1;
+  int64_t* group_of_row = (int64_t*)malloc(sizeof(int64_t) * (num_rows ? num_rows : 1));
+  int64_t ng = form_groups(keys, n_keys, num_rows, include_null_keys, out_rep_rows, group_of_row);
+  *out_num_groups = ng;
+  int64_t* grp_off = (int64_t*)calloc((size_t)ng + 1, sizeof(int64_t));
+  int64_t* grp_rows = (int64_t*)malloc(sizeof(int64_t) * (size_t)(num_rows ? num_rows : 1));
+  group_rows(group_of_row, num_rows, ng, grp_off, grp_rows);
+  int64_t* buf = (int64_t*)malloc(sizeof(int64_t) * (size_t)(num_rows ? num_rows : 1));
+  uint8_t* keep = (uint8_t*)malloc((size_t)(num_rows ? num_rows : 1));
+  const int fl = is_float(value->type_id);
+
+  int64_t e = 0;
+  for (int64_t g = 0; g < ng; ++g) {
+    const int64_t* rows = grp_rows + grp_off[g];
+    const int64_t size = grp_off[g + 1] - grp_off[g];
+    out_offsets[g] = e;
+    if (kind == K_COLLECT_LIST) {
+      for (int64_t i = 0; i < size; ++i)
+        if (null_policy == 1 || bit_is_set(value->mask, rows[i])) out_rows[e++] = rows[i];
+      continue;
+    }
+    /* valid values sorted into runs of equal values, the first row of each kept */
+    int64_t n = 0, first_null = -1;
+    for (int64_t i = 0; i < size; ++i) {
+      keep[rows[i]] = 0;
+      if (bit_is_set(value->mask, rows[i])) buf[n++] = rows[i];
+      else if (null_policy == 1 && (nulls_equal == 1 || first_null < 0)) {
+        keep[rows[i]] = 1;
+        if (first_null < 0) first_null = rows[i];
+      }
+    }
+    g_sort_col = value;
+    qsort(buf, (size_t)n, sizeof(int64_t), row_cmp); /* value order, ties by row */
+    for (int64_t i = 0; i < n; ++i) {
+      int starts = i == 0 || value_cmp(buf[i - 1], buf[i]) != 0;
+      if (!starts && fl && nans_equal == 1 && isnan(load_f(value->type_id, value->data, buf[i]))) starts = 1;
+      keep[buf[i]] = (uint8_t)starts;
+    }
+    for (int64_t i = 0; i < size; ++i)
+      if (keep[rows[i]]) out_rows[e++] = rows[i];
+  }
+  out_offsets[ng] = e;
+  free(buf); free(keep); free(grp_off); free(grp_rows); free(group_of_row);
+  return 0;
 }

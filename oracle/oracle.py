@@ -39,6 +39,8 @@ NP = {
 SUM, PRODUCT, MIN, MAX, COUNT_VALID, COUNT_ALL = 0, 1, 2, 3, 4, 5
 SUM_OF_SQUARES, MEAN, M2, VARIANCE, STD, ARGMAX, ARGMIN = 8, 9, 10, 11, 12, 15, 16
 MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT = 13, 14, 17, 18
+COLLECT_LIST, COLLECT_SET = 21, 22
+LIST = 24
 
 
 class _Col(ctypes.Structure):
@@ -81,6 +83,12 @@ def load():
             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
             ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
             ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+        ]
+        lib.mgo_groupby_collect.restype = ctypes.c_int
+        lib.mgo_groupby_collect.argtypes = [
+            ctypes.POINTER(_Col), ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_Col),
+            ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+            ctypes.c_void_p, ctypes.c_void_p,
         ]
         _lib = lib
     return _lib
@@ -214,6 +222,55 @@ def groupby(keys: Sequence[HostColumn], requests: Sequence[Tuple[HostColumn, Seq
             j += 1
         results.append(block)
     return out_keys, results
+
+
+def collect(keys: Sequence[HostColumn], values: HostColumn, kind: int, null_policy: int = 1,
+            nulls_equal: int = 0, nans_equal: int = 0, include_null_keys: bool = False):
+    """The C restatement of cudf's groupby COLLECT_LIST / COLLECT_SET.
+
+    null_policy 1 = INCLUDE, 0 = EXCLUDE; nulls_equal 0 = EQUAL, 1 = UNEQUAL;
+    nans_equal 0 = ALL_EQUAL, 1 = UNEQUAL. Returns (key columns [(values,
+    valid)], lists [(values, valid) per group]), groups in order of first
+    appearance; a set's elements are in order of first appearance."""
+    lib = load()
+    n = len(keys[0]) if keys else 0
+    reps = np.zeros(max(n, 1), dtype=np.int64)
+    offsets = np.zeros(max(n, 1) + 1, dtype=np.int64)
+    rows = np.zeros(max(n, 1), dtype=np.int64)
+    ng = ctypes.c_int64(0)
+    karr = (_Col * max(len(keys), 1))(*[k.c() for k in keys])
+    v = values.c()
+    rc = lib.mgo_groupby_collect(karr, len(keys), n, 1 if include_null_keys else 0, ctypes.byref(v), kind,
+                                 null_policy, nulls_equal, nans_equal, reps.ctypes.data, ctypes.byref(ng),
+                                 offsets.ctypes.data, rows.ctypes.data)
+    if rc != 0:
+        raise ValueError("Invalid type/aggregation combination.")
+    g = ng.value
+    reps = reps[:g]
+    out_keys = [(k.values[reps], k.valid[reps] if k.valid is not None else None) for k in keys]
+    valid = values.valid if values.valid is not None else np.ones(len(values), bool)
+    lists = []
+    for i in range(g):
+        r = rows[offsets[i]:offsets[i + 1]]
+        lists.append((values.values[r].copy(), valid[r].copy()))
+    return out_keys, lists
+
+
+def sort_list(vals: np.ndarray, valid: np.ndarray):
+    """A list's elements ascending, NaN after every number, nulls last: the
+    order cudf::lists::sort_lists(ASCENDING, null_order::AFTER) gives and the
+    reference's COLLECT_SET tests compare in
+    (cpp/tests/groupby/collect_set_tests.cpp:58-63)."""
+    vals = np.asarray(vals)
+    valid = np.asarray(valid, bool)
+    v = vals[valid]
+    if v.dtype.kind == "f":
+        order = np.lexsort((v, np.isnan(v)))  # numbers ascending, then NaNs
+    else:
+        order = np.argsort(v, kind="stable")
+    nn = int((~valid).sum())
+    out = np.concatenate([v[order], np.zeros(nn, dtype=vals.dtype)])
+    return out, np.concatenate([np.ones(v.size, bool), np.zeros(nn, bool)])
 
 
 def sort_order(key_cols) -> np.ndarray:

@@ -4,6 +4,7 @@
 // cpp/tests/groupby/groupby_test_util.cpp:33-95). Needs a GPU.
 #include <cudf/aggregation.hpp>
 #include <cudf/groupby.hpp>
+#include <cudf/lists/lists_column_view.hpp>
 #include <cudf/hashing.hpp>
 #include <cudf/utilities/error.hpp>
 
@@ -352,6 +353,77 @@ void sort_path_tests()
                      {1, 2, 3, 4}, {3, 4, 2, 1});
 }
 
+// One list of a lists column: (values, validity) of every element.
+template <typename V>
+std::vector<std::pair<std::vector<V>, std::vector<int>>> lists_host(cudf::column_view const& c)
+{
+  cudf::lists_column_view lv(c);
+  auto off = to_host<int32_t>(lv.offsets());
+  auto vals = to_host<V>(lv.child());
+  auto valid = valid_host(lv.child());
+  std::vector<std::pair<std::vector<V>, std::vector<int>>> out;
+  for (int i = 0; i < lv.size(); ++i)
+    out.emplace_back(std::vector<V>(vals.begin() + off[i], vals.begin() + off[i + 1]),
+                     std::vector<int>(valid.begin() + off[i], valid.begin() + off[i + 1]));
+  return out;
+}
+
+// collect_list_tests.cpp:48-65 and collect_set_tests.cpp:357-398 (keys
+// {4, 1, 2, 4, 3, 3, 3, 3, 2, 1}): lists in row order; sets ascending, nulls last
+template <typename V>
+void collect_tests()
+{
+  wrapper<K> keys(cudf::type_id::INT32, {1, 1, 2, 2, 3, 3});
+  wrapper<V> vals(tid<V>(), std::vector<V>{1, 2, 3, 4, 5, 6}, {1, 0, 1, 0, 1, 0});
+  std::vector<cudf::groupby::aggregation_request> r(1);
+  r[0].values = vals.view();
+  r[0].aggregations.push_back(cudf::make_collect_list_aggregation<cudf::groupby_aggregation>());
+  r[0].aggregations.push_back(
+    cudf::make_collect_list_aggregation<cudf::groupby_aggregation>(cudf::null_policy::EXCLUDE));
+  cudf::groupby::groupby gb(cudf::table_view({keys.view()}));
+  auto [uk, res] = gb.aggregate(r);
+  auto k = to_host<K>(uk->get_column(0).view());
+  CHECK(res[0].results[0]->type().id() == cudf::type_id::LIST && res[0].results[0]->num_children() == 2);
+  CHECK(res[0].results[0]->child(1).type().id() == tid<V>());
+  auto inc = lists_host<V>(res[0].results[0]->view());
+  auto exc = lists_host<V>(res[0].results[1]->view());
+  CHECK(inc.size() == 3 && exc.size() == 3);
+  for (size_t g = 0; g < k.size() && inc.size() == 3 && exc.size() == 3; ++g) {
+    const V first = static_cast<V>(2 * k[g] - 1);
+    CHECK(inc[g].first.size() == 2 && inc[g].first[0] == first && inc[g].second == std::vector<int>({1, 0}));
+    CHECK(exc[g].first == std::vector<V>({first}) && exc[g].second == std::vector<int>({1}));
+  }
+
+  wrapper<K> k10(cudf::type_id::INT32, {4, 1, 2, 4, 3, 3, 3, 3, 2, 1});
+  wrapper<V> v10(tid<V>(), std::vector<V>{40, 10, 20, 40, 0, 0, 0, 0, 21, 0}, {1, 1, 1, 1, 0, 0, 0, 0, 1, 0});
+  std::vector<cudf::groupby::aggregation_request> r2(1);
+  r2[0].values = v10.view();
+  r2[0].aggregations.push_back(cudf::make_collect_set_aggregation<cudf::groupby_aggregation>());
+  r2[0].aggregations.push_back(cudf::make_collect_set_aggregation<cudf::groupby_aggregation>(
+    cudf::null_policy::INCLUDE, cudf::null_equality::UNEQUAL));
+  r2[0].aggregations.push_back(
+    cudf::make_collect_set_aggregation<cudf::groupby_aggregation>(cudf::null_policy::EXCLUDE));
+  cudf::groupby::groupby gb2(cudf::table_view({k10.view()}));
+  auto [uk2, res2] = gb2.aggregate(r2);
+  auto kk = to_host<K>(uk2->get_column(0).view());
+  using L = std::pair<std::vector<V>, std::vector<int>>;
+  const std::map<K, L> want_eq{{1, {{10, 0}, {1, 0}}}, {2, {{20, 21}, {1, 1}}}, {3, {{0}, {0}}}, {4, {{40}, {1}}}};
+  const std::map<K, L> want_ne{
+    {1, {{10, 0}, {1, 0}}}, {2, {{20, 21}, {1, 1}}}, {3, {{0, 0, 0, 0}, {0, 0, 0, 0}}}, {4, {{40}, {1}}}};
+  const std::map<K, L> want_ex{{1, {{10}, {1}}}, {2, {{20, 21}, {1, 1}}}, {3, {{}, {}}}, {4, {{40}, {1}}}};
+  const std::map<K, L>* wants[3] = {&want_eq, &want_ne, &want_ex};
+  for (int a = 0; a < 3; ++a) {
+    auto got = lists_host<V>(res2[0].results[a]->view());
+    CHECK(got.size() == 4);
+    for (size_t g = 0; g < kk.size() && got.size() == 4; ++g) {
+      const L& w = wants[a]->at(kk[g]);
+      CHECK(got[g].second == w.second);
+      for (size_t e = 0; e < w.first.size() && got[g].first.size() == w.first.size(); ++e)
+        if (w.second[e]) CHECK(got[g].first[e] == w.first[e]);
+    }
+  }
+}
+
 void error_tests()
 {
   // groupby.cu:208-212: size mismatch; :176-189: invalid type/aggregation
@@ -412,6 +484,8 @@ int main()
     {"second_pass<int32>", second_pass_tests<int32_t>},   {"second_pass<double>", second_pass_tests<double>},
     {"second_pass<float>", second_pass_tests<float>},
     {"sort_path<int32>", sort_path_tests<int32_t>},      {"sort_path<double>", sort_path_tests<double>},
+    {"collect<int32>", collect_tests<int32_t>},          {"collect<int64>", collect_tests<int64_t>},
+    {"collect<double>", collect_tests<double>},
     {"keys", keys_tests},                        {"multi_request", multi_request_tests},
     {"errors", error_tests},                     {"hashing", hashing_tests},
   };

@@ -102,3 +102,67 @@ def check_case(case, key_cols, results):
         assert same.all(), f"{case['id']}: {gv} vs {ev}"
     else:
         assert np.array_equal(gv[evalid], ev[evalid]), f"{case['id']}: {gv} vs {ev}"
+
+
+COLLECT_TYPES = dict(TYPES, TIMESTAMP_DAYS=12, TIMESTAMP_SECONDS=13, TIMESTAMP_NANOSECONDS=16)
+COLLECT_KINDS = {"COLLECT_LIST": O.COLLECT_LIST, "COLLECT_SET": O.COLLECT_SET}
+EQUALITY = {"EQUAL": 0, "UNEQUAL": 1, "ALL_EQUAL": 0}
+
+
+def load_collect_cases():
+    """tests/golden/reference_collect_cases.json expanded into typed cases:
+    keys / values already sliced, the aggregation as (kind, null_policy,
+    nulls_equal, nans_equal), expected lists as [(values, valid)] in the
+    reference's sorted key order."""
+    with open(os.path.join(HERE, "golden", "reference_collect_cases.json")) as f:
+        doc = json.load(f)
+    out = []
+    for c in doc["cases"]:
+        vts = doc["value_types_for_typed_cases"] if c.get("typed") else [c["val_type"]]
+        lo, hi = c.get("slice", [0, len(c["keys"])])
+        for vt in vts:
+            v_t = COLLECT_TYPES[vt]
+            dt = O.NP[v_t]
+            keys = np.array(c["keys"], dtype=np.int32)[lo:hi]
+            vals = np.array([_num(x) for x in c["vals"]], dtype=np.float64).astype(dt)[lo:hi]
+            vvalid = np.array(c["vals_valid"], bool)[lo:hi] if "vals_valid" in c else None
+            lists = []
+            for lst in c["expect_lists"]:
+                valid = np.array([x is not None for x in lst], bool)
+                v = np.array([0 if x is None else _num(x) for x in lst], dtype=np.float64).astype(dt)
+                lists.append((v, valid))
+            out.append(dict(
+                id=f"{c['name']}[{vt}]", cite=c["cite"],
+                agg=(COLLECT_KINDS[c["agg"]], NULLS[c.get("nulls", "INCLUDE")],
+                     EQUALITY[c.get("nulls_equal", "EQUAL")], EQUALITY[c.get("nans_equal", "ALL_EQUAL")]),
+                keys=(O.INT32, keys, None), vals=(v_t, vals, vvalid),
+                expect_keys=np.array(c["expect_keys"], dtype=np.int32), expect_lists=lists))
+    return out
+
+
+def same_list(got, want) -> bool:
+    """Element-wise equality of two lists (values, valid): same validity, equal
+    valid values (NaN equal to NaN, -0.0 to +0.0)."""
+    gv, gvalid = np.asarray(got[0]), np.asarray(got[1], bool)
+    wv, wvalid = np.asarray(want[0]), np.asarray(want[1], bool)
+    if gv.size != wv.size or not np.array_equal(gvalid, wvalid):
+        return False
+    a, b = gv[wvalid], wv[wvalid]
+    if a.dtype.kind == "f":
+        return bool(((a == b) | (np.isnan(a) & np.isnan(b))).all())
+    return bool(np.array_equal(a, b))
+
+
+def check_collect_case(case, key_col, lists, sort_sets=True):
+    """Compares one COLLECT output (groups unsorted) with a case: keys sorted,
+    and each list as is (COLLECT_LIST, or COLLECT_SET with sort_sets False)
+    or after sorting its elements (COLLECT_SET with sort_sets True)."""
+    kv = np.asarray(key_col[0])
+    order = np.argsort(kv, kind="stable")
+    assert np.array_equal(kv[order], case["expect_keys"]), f"{case['id']}: keys {kv[order]}"
+    for pos, g in enumerate(order):
+        got = lists[g]
+        if case["agg"][0] == O.COLLECT_SET and sort_sets:
+            got = O.sort_list(*got)
+        want = case["expect_lists"][pos]
+        assert same_list(got, want), f"{case['id']}: group {kv[g]}: {got} vs {want}"

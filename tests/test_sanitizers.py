@@ -41,7 +41,8 @@ def test_oracle_under_asan_and_ubsan():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-B", "liboracle_san.so"], check=True)
     lib = os.path.join(ROOT, "oracle", "liboracle_san.so")
     r = _run_child({"LD_PRELOAD": f"{asan}:{ubsan}", "MGB_ORACLE_LIB": lib},
-                   ["tests/test_oracle.py", "tests/test_sanitizers.py::test_sanitized_oracle_is_loaded"])
+                   ["tests/test_oracle.py", "tests/test_oracle_collect.py",
+                    "tests/test_sanitizers.py::test_sanitized_oracle_is_loaded"])
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
 
 
@@ -59,11 +60,12 @@ def test_sanitized_oracle_is_loaded():
 
 def test_library_host_code_under_asan_and_ubsan():
     lib = os.path.join(ROOT, "cudf_amd", "lib", "san", "libcudf_amd.so")
-    if not os.path.exists(lib):
-        pytest.skip("host sanitizer build absent (make -C cudf_amd/csrc sanitize)")
     rts = sorted(glob.glob("/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so"))
-    if not rts:
-        pytest.skip("clang sanitizer runtime not installed")
+    if not rts or not os.path.exists("/opt/rocm/bin/hipcc"):
+        pytest.skip("hipcc or clang's sanitizer runtime not installed")
+    # incremental: rebuilds only what changed since the last sanitizer build
+    subprocess.run(["make", "-s", "-j8", "-C", os.path.join(ROOT, "cudf_amd", "csrc"), "sanitize"], check=True,
+                   capture_output=True, timeout=1800)
     r = _run_child({"LD_PRELOAD": rts[-1], "MGB_LIB_PATH": lib},
                    ["tests/test_capi.py", "tests/test_pandas_accel.py",
                     "tests/test_sanitizers.py::test_sanitized_library_is_loaded"])
