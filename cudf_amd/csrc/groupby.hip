@@ -2595,8 +2595,11 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
 // MGB_LEAN=off turns the execution off.
 struct LeanPlan {
   bool on = false;
-  int kw = 0;
+  int k0 = 0, k1 = 0;  // key column widths (k1: a second key column, else 0)
+  int kb = 0;          // key bytes of a record
+  int tail = 0;        // two-column keys: LDS tail bytes of a record (lean_tail_bytes)
   int w[kLeanCols] = {};
+  LeanKeyOffsets ko{};
   LeanOffsets lo{};
   int rec_bytes = 0;
   int tile = 0;        // rows staged per scatter tile
@@ -2604,31 +2607,56 @@ struct LeanPlan {
   int flag_bits = 0;   // LeanScatterArgs::flag_bits
 };
 
-LeanPlan lean_plan(const Plan& plan, const AggParams& src, int log2p)
+// Whether a call fits the compact partitioned execution, and its record
+// shape (everything but the partition count). MGB_LEAN=off turns it off.
+bool lean_shape_of(const Plan& plan, const AggParams& src, LeanPlan& lp)
 {
-  LeanPlan lp;
+  lp = LeanPlan{};
   const char* e = std::getenv("MGB_LEAN");
-  if (e && std::string(e) == "off") return lp;
-  if (log2p > 12 || plan.key_mode != KEY_INT || plan.keys.size() != 1 || plan.compare_key_nulls ||
-      plan.has_pass2 || plan.has_sort || src.num_vals > kLeanCols || !src.program_cols)
-    return lp;
-  const KeyCol& k = plan.keys[0];
-  if ((k.width != 4 && k.width != 8) || k.stride != k.width || (k.mask && (k.bit_off & 31))) return lp;
-  lp.kw = k.width;
+  if (e && std::string(e) == "off") return false;
+  if (plan.compare_key_nulls || plan.has_pass2 || plan.has_sort || src.num_vals > kLeanCols || !src.program_cols)
+    return false;
+  auto plain = [](const KeyCol& k) {
+    return (k.width == 4 || k.width == 8) && k.stride == k.width && !(k.mask && (k.bit_off & 31));
+  };
+  if (plan.key_mode == KEY_INT && plan.keys.size() == 1) {
+    if (!plain(plan.keys[0])) return false;
+    lp.k0 = plan.keys[0].width;
+  } else if (plan.key_mode == KEY_ROWREF && plan.keys.size() == 2) {
+    // two integer columns: compared as raw bits (no float normalisation)
+    for (const KeyCol& k : plan.keys)
+      if (!plain(k) || (k.vclass != VC_SIGNED && k.vclass != VC_UNSIGNED)) return false;
+    lp.k0 = plan.keys[0].width;
+    lp.k1 = plan.keys[1].width;
+  } else {
+    return false;
+  }
+  lp.ko = lean_key_offsets(lp.k0, lp.k1);
+  lp.kb = lp.ko.bytes;
+  lp.tail = lp.k1 ? lean_tail_bytes(lp.kb) : 0;
   lp.flag_bits = 0xfff;
   for (int c = 0; c < src.num_vals; ++c) {
     const ValCol& v = src.vals[c];
     if (v.lds_generic || (v.width != 4 && v.width != 8) || v.stride != v.width || (v.mask && (v.bit_off & 31)) ||
         v.flag_bit >= 8)
-      return lp;
+      return false;
     lp.w[c] = v.width;
     if (v.mask && v.flag_bit >= 0) lp.flag_bits = (lp.flag_bits & ~(15 << (4 * c))) | v.flag_bit << (4 * c);
   }
-  if (!lean_supported(lp.kw, lp.w)) return lp;
-  lp.lo = lean_offsets(lp.kw, lp.w[0], lp.w[1], lp.w[2]);
+  if (!lean_supported(lp.k0, lp.k1, lp.w)) return false;
+  lp.lo = lean_offsets(lp.kb, lp.w[0], lp.w[1], lp.w[2]);
   for (int c = 0; c < src.num_vals; ++c)
-    if (lp.lo.off[c] < 0) return lp;
+    if (lp.lo.off[c] < 0) return false;
   lp.rec_bytes = lp.lo.bytes <= 16 ? 16 : 32;
+  return true;
+}
+
+// The compact scatter's tile for 2^log2p partitions; lp.on when it fits LDS.
+void lean_tile(LeanPlan& lp, int log2p)
+{
+  lp.on = false;
+  lp.tile = 0;
+  if (lp.rec_bytes == 0 || log2p > kLeanMaxLog2P) return;
   const int P = 1 << log2p;
   for (int tt = kScatterTile; tt >= kLdsBlock; tt -= kLdsBlock)
     if (lean_scatter_lds(P, tt, lp.rec_bytes).total <= lds_limit()) {
@@ -2637,11 +2665,10 @@ LeanPlan lean_plan(const Plan& plan, const AggParams& src, int log2p)
     }
   lp.lds = lean_scatter_lds(P, lp.tile, lp.rec_bytes).total;
   lp.on = lp.tile > 0;
-  return lp;
 }
 
 mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& base, int64_t R, int64_t G_est,
-                           const LdsGeom& geom, hipStream_t stream, TempOuts& temp, int64_t& groups, bool& done)
+                           const LdsGeom& plan_geom, hipStream_t stream, TempOuts& temp, int64_t& groups, bool& done)
 {
   done = false;
   const int cus = device_cu_count();
@@ -2650,14 +2677,35 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
   int nullable = 0;
   for (int c = 0; c < src.num_vals; ++c) nullable += src.vals[c].flag_bit >= 0;
   const int flag_bytes = nullable == 0 ? 0 : nullable <= 8 ? 1 : nullable <= 16 ? 2 : 4;
-  // groups per partition: the table's capacity less four standard deviations
-  // of a partition's group count (a partition over it spills the rest)
-  const double m_target = std::max(geom.cap * 0.5, geom.cap - 4.0 * std::sqrt((double)geom.cap));
+  // the compact execution's record shape; a two-column key's LDS records
+  // carry a tail after the plan's fields (8-byte aligned), so fewer fit
+  LeanPlan lean_shape;
+  bool lean_ok = lean_shape_of(plan, src, lean_shape);
+  const int tail_off = (int)align_up((size_t)plan.stride, 8);
+  LdsGeom geom = plan_geom;
+  if (lean_ok && lean_shape.tail) {
+    const LdsGeom g2 = lds_geometry(tail_off + lean_shape.tail);
+    if (g2.ok) geom = g2;
+    else lean_ok = false;
+  }
   int64_t est = std::max<int64_t>(G_est, 1);
   for (int attempt = 0; attempt < 4; ++attempt) {
     const int slot0 = ws->upload_slot;  // parameter slots of an attempt are reused by the next
-    int log2p = 1;
-    while ((double)(1ll << log2p) * m_target < (double)est) ++log2p;
+    // groups per partition: the table's capacity less four standard deviations
+    // of a partition's group count (a partition over it spills the rest)
+    auto partitions_for = [&](const LdsGeom& g) {
+      const double m_target = std::max(g.cap * 0.5, g.cap - 4.0 * std::sqrt((double)g.cap));
+      int l = 1;
+      while ((double)(1ll << l) * m_target < (double)est) ++l;
+      return l;
+    };
+    int log2p = partitions_for(geom);
+    if (lean_ok && lean_shape.tail && log2p > kLeanMaxLog2P) {
+      // more partitions than the compact scatter takes: the general records
+      lean_ok = false;
+      geom = plan_geom;
+      log2p = partitions_for(geom);
+    }
     if (log2p > 15) return MGB_SUCCESS;
     // more partitions than the groups need when the rows are many, so that a
     // partition of mean size is one work item (the items of a partition cut
@@ -2704,11 +2752,16 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     rec = flags_rec_off + flag_bytes;
     int rec_bytes = (int)align_up((size_t)rec, 8);
     // the compact records, when the call has them
-    const LeanPlan lean = lean_plan(plan, src, log2p);
+    LeanPlan lean = lean_shape;
+    if (lean_ok) lean_tile(lean, log2p);
+    else lean.on = false;
+    // LDS record words (two-column compact keys: the plan's fields, then the tail)
+    const int lds_words = lds_stride_of(lean.on && lean.tail ? tail_off + lean.tail : plan.stride) / 8;
     if (lean.on) {
-      koff[0] = 0;
+      koff[0] = lean.ko.off0;
+      if (lean.k1) koff[1] = lean.ko.off1;
       for (int c = 0; c < src.num_vals; ++c) voff[c] = lean.lo.off[c];
-      flags_rec_off = lean.kw;
+      flags_rec_off = lean.kb;
       rec_bytes = lean.rec_bytes;
     }
 
@@ -2754,9 +2807,15 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     MGB_HIP(hipMemsetAsync(temp.null_counts, 0, (size_t)n_nulls * 8, stream));
     uint64_t tmpl[kMaxRecordWords] = {};
     record_template(plan, plan.key_mode == KEY_INT ? kEmptyIntKey : kEmptyRowRef, tmpl);
+    if (lean.on && lean.tail) {
+      // two-column compact keys: an empty header is kEmptyIntKey (key part
+      // A), and the tail reads as not yet written
+      tmpl[0] = kEmptyIntKey;
+      for (int w = tail_off / 8; w < (tail_off + lean.tail) / 8; ++w) tmpl[w] = ~0ull;
+    }
     const OutTable* d_ot = nullptr;
     const uint64_t* d_tmpl = nullptr;
-    st = upload_params(ws, ot, tmpl, lds_stride_of(plan.stride) / 8, stream, &d_ot, &d_tmpl);
+    st = upload_params(ws, ot, tmpl, lds_words, stream, &d_ot, &d_tmpl);
     if (st != MGB_SUCCESS) return st;
 
     PartParams pp{};
@@ -2806,6 +2865,11 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       const KeyCol& kc = plan.keys[0];
       sa.key = kc.data;
       sa.key_mask = src.skip_null_keys && kc.mask ? kc.mask + (kc.bit_off >> 5) : nullptr;
+      if (lean.k1) {
+        const KeyCol& kc1 = plan.keys[1];
+        sa.key1 = kc1.data;
+        sa.key1_mask = src.skip_null_keys && kc1.mask ? kc1.mask + (kc1.bit_off >> 5) : nullptr;
+      }
       for (int c = 0; c < src.num_vals; ++c) {
         sa.val[c] = src.vals[c].data;
         sa.mask[c] = src.vals[c].mask ? src.vals[c].mask + (src.vals[c].bit_off >> 5) : nullptr;
@@ -2819,7 +2883,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       sa.tile_rows = lean.tile;
       sa.flag_bits = lean.flag_bits;
       ProfScope prof("partition_scatter", stream);
-      st = lean_scatter(lean.kw, lean.w, sa, (unsigned)nw, lean.lds, stream);
+      st = lean_scatter(lean.k0, lean.k1, lean.w, sa, (unsigned)nw, lean.lds, stream);
       if (st != MGB_SUCCESS) return st;
     } else if (tile > 0) {
       MGB_HIP(set_dynamic_lds((const void*)part_scatter_tiled_kernel, (int)tiled_lds));
@@ -2892,7 +2956,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     lp.log2p = log2p;
     lp.slots = geom.slots;
     lp.cap = geom.cap;
-    lp.words = lds_stride_of(plan.stride) / 8;
+    lp.words = lds_words;
     lp.direct = 0;
     lp.item_offsets = ioff;
     lp.max_item_rows = max_item_rows;
@@ -2924,7 +2988,8 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       aa.cap = lp.cap;
       aa.words = lp.words;
       aa.rows_off = lp.src.rows_op >= 0 ? lp.src.ops[lp.src.rows_op].off : -1;
-      aa.key_signed = plan.keys[0].vclass == VC_SIGNED;
+      aa.key_signed = !lean.k1 && plan.keys[0].vclass == VC_SIGNED;
+      aa.tail_off = tail_off;
       for (int c = 0; c < src.num_vals; ++c) {
         const ValCol& v = lp.src.vals[c];
         aa.cols[c] = LeanCol{v.sum_off, v.min_off, v.max_off, v.nulls_off,
@@ -2932,7 +2997,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
                              src.vals[c].mask && v.flag_bit >= 0 ? v.flag_bit : -1};
       }
       ProfScope prof("lds_aggregate", stream);
-      st = lean_aggregate(lean.kw, lean.w, aa, d_lp, (unsigned)item_grid, geom.bytes, stream);
+      st = lean_aggregate(lean.k0, lean.k1, lean.w, aa, d_lp, (unsigned)item_grid, geom.bytes, stream);
       if (st != MGB_SUCCESS) return st;
     } else {
       const void* agg_fn = plan.has_pass2 ? (const void*)lds_agg_kernel<true> : (const void*)lds_agg_kernel<false>;

@@ -107,14 +107,21 @@ __device__ __forceinline__ void flag_overflow(unsigned long long* counters) { at
 // needed; nullptr when the table is over its group budget (overflow flag set).
 // `entry_check`: test the overflow flag first (a caller that probes every row
 // tests it itself, every few rows: one flag read by every row of a launch
-// queues on a single L2 channel).
+// queues on a single L2 channel). `local_count`: the caller checked the
+// budget for all of its inserts beforehand; inserts are counted there (LDS)
+// instead of in the table's group counter, which the caller adds them to
+// once (a counter and flag taken by every insert of a launch are one L2
+// channel's traffic, and stall every access that maps there).
 __device__ __forceinline__ uint8_t* global_find_or_insert(const AggParams& p, const RowKey& rk, int64_t row,
-                                                         bool entry_check = true)
+                                                         bool entry_check = true, uint32_t* local_count = nullptr)
 {
   if (rk.special) {
     uint8_t* rec = p.table + p.capacity * (int64_t)p.stride;
     unsigned long long* hp = reinterpret_cast<unsigned long long*>(rec);
-    if (*hp == 0ull && atom_cas(hp, 0ull, 1ull) == 0ull) atom_add(&p.counters[0], 1ull);
+    if (*hp == 0ull && atom_cas(hp, 0ull, 1ull) == 0ull) {
+      if (local_count) atomicAdd(local_count, 1u);
+      else atom_add(&p.counters[0], 1ull);
+    }
     return rec;
   }
   // once the table has overflowed the call is repeated with a larger one:
@@ -131,14 +138,15 @@ __device__ __forceinline__ uint8_t* global_find_or_insert(const AggParams& p, co
     uint64_t cur = *hp;  // a stale read can only be EMPTY; the CAS below resolves it
     for (;;) {
       if (cur == empty) {
-        if (__hip_atomic_load(&p.counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
-            (unsigned long long)p.max_groups) {
+        if (!local_count && __hip_atomic_load(&p.counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+                                (unsigned long long)p.max_groups) {
           flag_overflow(p.counters);
           return nullptr;
         }
         uint64_t seen = atom_cas(hp, empty, rk.key);
         if (seen == empty) {
-          atom_add(&p.counters[0], 1ull);
+          if (local_count) atomicAdd(local_count, 1u);
+          else atom_add(&p.counters[0], 1ull);
           return rec;
         }
         cur = seen;

@@ -283,15 +283,15 @@ LdsGeom lds_geometry(int stride);
 
 // ------------------------------------------------------------ compact partitioned execution (lean.hip)
 
-// Whether the compact kernels are compiled for a key width and value widths
-// (absent columns 0, last).
-bool lean_supported(int kw, const int (&w)[kLeanCols]);
+// Whether the compact kernels are compiled for key column widths (k0, and k1
+// for a second key column, else 0) and value widths (absent columns 0, last).
+bool lean_supported(int k0, int k1, const int (&w)[kLeanCols]);
 // The compact scatter over `grid` chunks, and the compact aggregation over
 // `grid` work items (lp: the LDS parameters in device memory).
-mgb_status lean_scatter(int kw, const int (&w)[kLeanCols], const LeanScatterArgs& a, unsigned grid, size_t lds,
-                        hipStream_t stream);
-mgb_status lean_aggregate(int kw, const int (&w)[kLeanCols], const LeanAggArgs& a, const LdsParams* lp, unsigned grid,
-                          size_t lds, hipStream_t stream);
+mgb_status lean_scatter(int k0, int k1, const int (&w)[kLeanCols], const LeanScatterArgs& a, unsigned grid,
+                        size_t lds, hipStream_t stream);
+mgb_status lean_aggregate(int k0, int k1, const int (&w)[kLeanCols], const LeanAggArgs& a, const LdsParams* lp,
+                          unsigned grid, size_t lds, hipStream_t stream);
 
 // ------------------------------------------------------------ kernels shared with batch.hip
 

@@ -118,15 +118,36 @@ struct PartParams {
   int32_t pad = 0;
 };
 
-// Partitioned records of the compact partitioned execution (a single 4- or
-// 8-byte integer key; at most kLeanCols value columns of 4 or 8 bytes whose
-// ops are SUM / MIN / MAX and counts): the key at byte 0, the row hash at
-// byte `key width` with the validity bits of the nullable columns in its low
-// byte, and each value column at the lowest free offset aligned to its
-// width, widest first (lean_offsets); 16 or 32 bytes a record. The
+// Partitioned records of the compact partitioned execution (a 4- or 8-byte
+// integer key, or two such columns; at most kLeanCols value columns of 4 or 8
+// bytes whose ops are SUM / MIN / MAX and counts): the key at byte 0, the row
+// hash at byte `key bytes` with the validity bits of the nullable columns in
+// its low byte, and each value column at the lowest free offset aligned to
+// its width, widest first (lean_offsets); 16 or 32 bytes a record. The
 // aggregation takes the LDS slot from the stored hash instead of hashing the
 // key again. Widths and offsets are compile-time constants of the kernels.
 constexpr int kLeanCols = 3;
+constexpr int kLeanMaxLog2P = 13;  // partitions of the compact scatter: at most 2^13
+
+// Two-column integer keys (column widths K0, K1 of 4 or 8 bytes): the
+// record's key region holds both columns, the 8-byte one first (column 0
+// when both are as wide), so that its first 8 bytes are one 8-byte value
+// ("key part A") and the rest (none, 4 or 8 bytes) "key part B". The LDS
+// record keeps part A in its header and, in a tail after the plan's fields,
+// part B and the index of the group's first record (lean_tail_bytes).
+struct LeanKeyOffsets {
+  int off0, off1, bytes;  // byte offsets of key columns 0 and 1 (-1: absent), key bytes
+};
+__host__ __device__ constexpr LeanKeyOffsets lean_key_offsets(int k0, int k1)
+{
+  return k1 == 0 ? LeanKeyOffsets{0, -1, k0}
+                 : (k0 >= k1 ? LeanKeyOffsets{0, k0, k0 + k1} : LeanKeyOffsets{k1, 0, k0 + k1});
+}
+// LDS tail of a two-column key's record: 8 bytes (the first record's index,
+// and part B when it is 4 bytes) or 16 (part B of 8 bytes, then the index).
+__host__ __device__ constexpr int lean_tail_bytes(int key_bytes) { return key_bytes <= 12 ? 8 : 16; }
+// A tail word whose record index is this is not written yet.
+constexpr uint32_t kLeanNotReady = 0xffffffffu;
 
 struct LeanOffsets {
   int off[kLeanCols];  // byte offset of each value column (-1: absent, or no room)
@@ -172,8 +193,10 @@ __host__ __device__ constexpr LeanScatterLds lean_scatter_lds(int P, int T, int 
 // Arguments of the compact scatter. Masks point at the word holding row 0's
 // bit (the columns' bit offsets are multiples of 32).
 struct LeanScatterArgs {
-  const uint8_t* key;               // key column, 4 or 8 bytes a row
+  const uint8_t* key;               // key column (column 0 of two), 4 or 8 bytes a row
   const uint32_t* key_mask;         // rows whose key is null are dropped; nullptr: none dropped
+  const uint8_t* key1;              // key column 1 of a two-column key
+  const uint32_t* key1_mask;
   const uint8_t* val[kLeanCols];    // value columns
   const uint32_t* mask[kLeanCols];  // their null masks, nullptr: never null
   const uint32_t* offsets;          // [P][nblocks]: first destination row of partition p for chunk b
@@ -183,6 +206,7 @@ struct LeanScatterArgs {
   int32_t log2p;
   int32_t nblocks;
   int32_t tile_rows;                // rows staged per tile: a multiple of 1024, at most 4096
+                                    // (partitions: at most 2^kLeanMaxLog2P)
   int32_t flag_bits;                // bits 4c..4c+3: column c's validity bit in the hash word (15: never null)
 };
 
@@ -217,7 +241,7 @@ struct LeanAggArgs {
   int32_t words;
   int32_t rows_off;                 // row-count field, or -1
   int32_t key_signed;               // 4-byte keys: sign-extended into the header
-  int32_t pad;
+  int32_t tail_off;                 // two-column keys: byte offset of the LDS record's tail
   LeanCol cols[kLeanCols];
 };
 

@@ -323,10 +323,26 @@ __device__ __forceinline__ void lds_aggregate_rows(const AggParams& p, const Lds
 // `part` of `nparts`: items that cover parts of one partition hold mostly the
 // same groups in the same slots and tend to finish together; each starts its
 // pass at a different slot, so that their atomics on one record do not meet.
+//
+// The group budget is checked once for the whole table (its groups at most
+// all new): within it, inserts are counted in LDS and added to the table's
+// group counter once at the end. Tables folded together may then overshoot
+// the budget by at most one table each, which the global table's slack past
+// its budget holds; past the budget, every insert checks it as usual.
 __device__ __forceinline__ void lds_fold_to_global(const AggParams& p, const LdsTable& t, int part = 0, int nparts = 1)
 {
   const int n = t.slots + 1;
   const int first = nparts > 1 ? (int)(((int64_t)n * part) / nparts) : 0;
+  uint32_t* inserted = reinterpret_cast<uint32_t*>(t.count + 2);  // free until groups are written out
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long used = __hip_atomic_load(&p.counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long over = __hip_atomic_load(&p.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    inserted[0] = 0u;
+    inserted[1] = over == 0ull && used + (unsigned long long)*t.count <= (unsigned long long)p.max_groups ? 1u : 0u;
+  }
+  __syncthreads();
+  uint32_t* local = inserted[1] ? inserted : nullptr;
   for (int j = threadIdx.x; j < n; j += kLdsBlock) {
     const int i = j + first < n ? j + first : j + first - n;
     if (!lds_occupied(t, i, p.key_mode)) continue;
@@ -344,8 +360,12 @@ __device__ __forceinline__ void lds_fold_to_global(const AggParams& p, const Lds
       rep = (int64_t)(hdr & 0xffffffffull);
       rk = row_key(p, rep);
     }
-    uint8_t* g = global_find_or_insert(p, rk, rep);
+    uint8_t* g = global_find_or_insert(p, rk, rep, local == nullptr, local);
     if (g) merge_record(p, g, rec);
+  }
+  if (local) {
+    __syncthreads();
+    if (threadIdx.x == 0 && inserted[0]) atom_add(&p.counters[0], (unsigned long long)inserted[0]);
   }
 }
 

@@ -1,7 +1,7 @@
 // The compact partitioned execution: the partitioned execution's two passes
-// for calls with a single 4- or 8-byte integer key and at most kLeanCols
-// value columns of 4 or 8 bytes whose ops are SUM / MIN / MAX and counts.
-// Each pass is compiled per key width and value widths, so that record
+// for calls with an integer key of one or two 4- or 8-byte columns and at
+// most kLeanCols value columns of 4 or 8 bytes whose ops are SUM / MIN / MAX
+// and counts. Each pass is compiled per key and value widths, so that record
 // offsets are constants, and takes a small argument block (LeanScatterArgs,
 // LeanAggArgs): the row loops hold few uniform values, all in scalar
 // registers, and the rare paths (a spilled row, the groups' output) read the
@@ -19,6 +19,14 @@
 // thread takes a few records at a time (kLeanRows), issuing the loads of its
 // next ones before it updates the table with the current ones. The LDS slot
 // comes from the hash stored in the record (its bits below the partition's).
+// A single-column key is the LDS record's header. A two-column key puts its
+// first 8 bytes (part A) in the header and the rest with the group's first
+// record index in a tail: a lane claims an empty slot by writing part A,
+// then writes the tail; a lane that meets part A before the tail is written
+// retries on its next pass of the miss loop (never waits inside it, so a
+// claim by a lane of its own wave always completes first). Before the groups
+// leave the table, headers become the row-reference headers the general
+// routines read (the first record's index).
 #pragma once
 
 #include "lds_agg.hpp"
@@ -35,24 +43,22 @@ constexpr int kLeanRows = RB16 == 1 ? 4 : 3;
 constexpr int kLeanProbe = 4;  // slots of a row's probe read before the full probe
 
 // Exclusive scan of the P counts `cnt` (P a power of two, at most
-// 4 * kLdsBlock) into `start`; returns the total. `tmp`: 16 words of LDS.
+// 8 * kLdsBlock) into `start`; returns the total. `tmp`: 16 words of LDS.
 // Ends with a workgroup barrier.
 __device__ __forceinline__ uint32_t lean_scan_counts(const uint32_t* cnt, uint32_t* start, int P, uint32_t* tmp)
 {
   const int per = P > kLdsBlock ? P / kLdsBlock : 1;
   const int i0 = (int)threadIdx.x * per;
-  uint32_t loc[4];
   uint32_t sum = 0;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    loc[e] = sum;
-    if (e < per && i0 + e < P) sum += cnt[i0 + e];
-  }
+  for (int e = 0; e < per; ++e)
+    if (i0 + e < P) sum += cnt[i0 + e];
   uint32_t total;
-  const uint32_t ex = block_exclusive_scan_1024(sum, tmp, &total);
-#pragma unroll
-  for (int e = 0; e < 4; ++e)
-    if (e < per && i0 + e < P) start[i0 + e] = ex + loc[e];
+  uint32_t run = block_exclusive_scan_1024(sum, tmp, &total);
+  for (int e = 0; e < per; ++e)
+    if (i0 + e < P) {
+      start[i0 + e] = run;
+      run += cnt[i0 + e];
+    }
   __syncthreads();
   return total;
 }
@@ -78,11 +84,13 @@ __device__ __forceinline__ void lean_put(uint32_t (&w)[8], uint64_t v)
 // Scatter of chunk blockIdx.x (rows [chunk * a.chunk, +a.chunk)) into
 // records. LDS (lean_scatter_lds): cursor, tile count and tile start of every
 // partition, 16 scan words, the destination row of every staged record, the
-// stage.
-template <int KW, int W0, int W1, int W2>
+// stage. K1 = 0: a single key column of K0 bytes.
+template <int K0, int K1, int W0, int W1, int W2>
 __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint8_t* smem)
 {
-  constexpr LeanOffsets LO = lean_offsets(KW, W0, W1, W2);
+  constexpr LeanKeyOffsets KO = lean_key_offsets(K0, K1);
+  constexpr int KB = KO.bytes;
+  constexpr LeanOffsets LO = lean_offsets(KB, W0, W1, W2);
   constexpr int RB16 = LO.bytes <= 16 ? 1 : 2;
   constexpr int I = kScatterItems;
   const int P = 1 << a.log2p;
@@ -108,7 +116,7 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
 
   // the thread's rows of one tile, as loaded (rows past the end load the
   // last row and are not staged)
-  uint64_t key[I], v0[I], v1[I], v2[I];
+  uint64_t key[I], key1[I], v0[I], v1[I], v2[I];
   uint32_t km[I], m0[I], m1[I], m2[I];
   auto fetch = [&](int64_t tile) {
 #pragma unroll
@@ -116,8 +124,12 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
       if (k >= items) continue;
       int64_t r = tile + threadIdx.x + (int64_t)k * kLdsBlock;
       r = r < end ? r : end - 1;
-      key[k] = lean_load<KW>(a.key, r);
+      key[k] = lean_load<K0>(a.key, r);
       km[k] = a.key_mask ? to_global(a.key_mask)[r >> 5] : ~0u;
+      if constexpr (K1 != 0) {
+        key1[k] = lean_load<K1>(a.key1, r);
+        if (a.key1_mask) km[k] &= to_global(a.key1_mask)[r >> 5];
+      }
       if constexpr (W0 != 0) {
         v0[k] = lean_load<W0>(a.val[0], r);
         m0[k] = a.mask[0] ? to_global(a.mask[0])[r >> 5] : ~0u;
@@ -143,7 +155,9 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
       const int64_t r = tile + threadIdx.x + (int64_t)k * kLdsBlock;
       const uint32_t bit = (uint32_t)r & 31u;
       present[k] = k < items && r < end && ((km[k] >> bit) & 1u);
-      const uint32_t h = hash_combine(0u, murmur3_bits(key[k], KW, 0));
+      // cudf's row hash: hash_combine over the key columns from seed 0
+      uint32_t h = hash_combine(0u, murmur3_bits(key[k], K0, 0));
+      if constexpr (K1 != 0) h = hash_combine(h, murmur3_bits(key1[k], K1, 0));
       uint32_t f = 0;
       if constexpr (W0 != 0) f |= fb0 < 8 ? ((m0[k] >> bit) & 1u) << fb0 : 0u;
       if constexpr (W1 != 0) f |= fb1 < 8 ? ((m1[k] >> bit) & 1u) << fb1 : 0u;
@@ -160,8 +174,9 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
       const uint32_t pos = tstart[part[k]] + rank[k];
       dstrow[pos] = cursor[part[k]] + rank[k];
       uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-      lean_put<KW, 0>(w, key[k]);
-      w[KW / 4] = hf[k];
+      lean_put<K0, KO.off0>(w, key[k]);
+      if constexpr (K1 != 0) lean_put<K1, KO.off1>(w, key1[k]);
+      w[KB / 4] = hf[k];
       if constexpr (W0 != 0) lean_put<W0, LO.off[0]>(w, v0[k]);
       if constexpr (W1 != 0) lean_put<W1, LO.off[1]>(w, v1[k]);
       if constexpr (W2 != 0) lean_put<W2, LO.off[2]>(w, v2[k]);
@@ -219,6 +234,81 @@ __device__ __forceinline__ uint8_t* lean_find_or_insert(const LdsTable& t, uint6
   }
   *t.overflow = 1;
   return nullptr;
+}
+
+constexpr uint32_t kLeanRetry = 0xfffffffdu;  // the probe met a slot being claimed: try again
+
+// Byte offset of the record of a two-column key (part A `ka`, part B `kb`:
+// none for KB = 8, its low 4 bytes for KB = 12) in the LDS table, inserting it
+// with `rowref` (the row's record index) when absent; kRecSpill when the
+// table is full or part A equals the empty header (such rows always take the
+// global table); kLeanRetry when a slot holding part A has no tail yet.
+template <int KB>
+__device__ __forceinline__ uint32_t lean2_find_or_insert(const LdsTable& t, uint8_t* lds, uint64_t ka, uint64_t kb,
+                                                         uint32_t rowref, uint32_t slot_hash, uint32_t tail_off)
+{
+  if (ka == kEmptyIntKey) return kRecSpill;
+  int32_t idx = (int32_t)reduce_range(slot_hash, t.slots);
+  for (int32_t probe = 0; probe < t.slots; ++probe) {
+    uint8_t* rec = lds_record(t, idx);
+    unsigned long long* hp = reinterpret_cast<unsigned long long*>(rec);
+    unsigned long long* tail = reinterpret_cast<unsigned long long*>(rec + tail_off);
+    uint64_t cur = __hip_atomic_load(hp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == kEmptyIntKey) {
+      if (__hip_atomic_load(t.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= t.cap) {
+        *t.overflow = 1;
+        return kRecSpill;
+      }
+      unsigned long long expect = kEmptyIntKey;
+      if (__hip_atomic_compare_exchange_strong(hp, &expect, (unsigned long long)ka, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+        if constexpr (KB <= 12) {
+          // one word: record index in the high half, part B (if any) in the low
+          __hip_atomic_store(tail, (unsigned long long)rowref << 32 | (uint32_t)kb, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+          __hip_atomic_store(tail, (unsigned long long)kb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_store(tail + 1, (unsigned long long)rowref << 32, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        atomicAdd(t.count, 1);
+        return (uint32_t)(rec - lds);
+      }
+      cur = expect;
+    }
+    if (cur == ka) {
+      if constexpr (KB == 8) {
+        return (uint32_t)(rec - lds);
+      } else if constexpr (KB == 12) {
+        const uint64_t w = __hip_atomic_load(tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if ((uint32_t)(w >> 32) == kLeanNotReady) return kLeanRetry;
+        if ((uint32_t)w == (uint32_t)kb) return (uint32_t)(rec - lds);
+      } else {
+        const uint64_t w1 = __hip_atomic_load(tail + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if ((uint32_t)(w1 >> 32) == kLeanNotReady) return kLeanRetry;
+        if (__hip_atomic_load(tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == kb)
+          return (uint32_t)(rec - lds);
+      }
+    }
+    idx = (idx + 1 == t.slots) ? 0 : idx + 1;
+  }
+  *t.overflow = 1;
+  return kRecSpill;
+}
+
+// Two-column keys: every record's header becomes the row-reference header of
+// the general routines (the group's first record index; kEmptyRowRef where
+// empty), for the groups' output or fold. Ends with a workgroup barrier.
+template <int KB>
+__device__ __forceinline__ void lean2_headers_to_rowref(const LdsTable& t, uint32_t tail_off)
+{
+  for (int i = threadIdx.x; i <= t.slots; i += kLdsBlock) {
+    uint8_t* rec = lds_record(t, i);
+    uint64_t* hp = reinterpret_cast<uint64_t*>(rec);
+    const uint64_t* tail = reinterpret_cast<const uint64_t*>(rec + tail_off);
+    const bool used = i < t.slots && *hp != kEmptyIntKey;
+    *hp = used ? (tail[KB <= 12 ? 0 : 1] >> 32) : kEmptyRowRef;
+  }
+  __syncthreads();
 }
 
 // A row whose group did not fit the LDS table: the global table of the full
@@ -363,11 +453,13 @@ __device__ __forceinline__ void lean_column(uint8_t* lds, const LeanCol& c, cons
 
 // Aggregates the records [begin, end) into the LDS table. Rows whose group
 // does not fit go to the global table. Ends with a workgroup barrier.
-template <int KW, int W0, int W1, int W2>
+template <int K0, int K1, int W0, int W1, int W2>
 __device__ __forceinline__ void lean_aggregate_rows(const LeanAggArgs& a, const LdsTable& t, uint8_t* lds,
                                                     int64_t begin, int64_t end)
 {
-  constexpr LeanOffsets LO = lean_offsets(KW, W0, W1, W2);
+  constexpr int KB = lean_key_offsets(K0, K1).bytes;
+  constexpr int KW = K1 == 0 ? K0 : 8;  // bytes of the key's header part
+  constexpr LeanOffsets LO = lean_offsets(KB, W0, W1, W2);
   constexpr int RB16 = LO.bytes <= 16 ? 1 : 2;
   constexpr int R = kLeanRows<RB16>;
   const global_t<const u32x4>* recs = to_global(reinterpret_cast<const u32x4*>(a.recs));
@@ -388,6 +480,7 @@ __device__ __forceinline__ void lean_aggregate_rows(const LeanAggArgs& a, const 
 #pragma unroll
   for (int c = 0; c < kLeanCols; ++c) cols[c] = a.cols[c];
   int rows_off = a.rows_off, log2p = a.log2p;
+  const uint32_t tail_off = (uint32_t)a.tail_off;
   for (int64_t base = begin + threadIdx.x; base < end; base += (int64_t)R * kLdsBlock) {
     // the uniform values are redefined (in scalar registers) every step, so
     // that the compiler tests them where they are used instead of keeping a
@@ -398,14 +491,17 @@ __device__ __forceinline__ void lean_aggregate_rows(const LeanAggArgs& a, const 
                    "+s"(cols[c].cls), "+s"(cols[c].flag_bit));
     asm volatile("" : "+s"(rows_off), "+s"(log2p));
     load(base + (int64_t)R * kLdsBlock, nxt);
-    uint64_t kk[R];
+    uint64_t kk[R], kb[R];
     uint32_t hf[R], sh[R], loff[R], i0[R];
     uint64_t hv[R][kLeanProbe];
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       if constexpr (KW == 8) kk[u] = (uint64_t)cur[u][0][0] | (uint64_t)cur[u][0][1] << 32;
       else kk[u] = a.key_signed ? (uint64_t)(int64_t)(int32_t)cur[u][0][0] : (uint64_t)cur[u][0][0];
-      hf[u] = cur[u][0][KW / 4];
+      kb[u] = 0;
+      if constexpr (KB == 12) kb[u] = cur[u][0][2];
+      if constexpr (KB == 16) kb[u] = (uint64_t)cur[u][0][2] | (uint64_t)cur[u][0][3] << 32;
+      hf[u] = lean_word<KB / 4, RB16>(cur[u]);
       sh[u] = (hf[u] & 0xffffff00u) << log2p;
       i0[u] = (uint32_t)reduce_range(sh[u], t.slots);
       uint32_t i = i0[u];
@@ -432,22 +528,50 @@ __device__ __forceinline__ void lean_aggregate_rows(const LeanAggArgs& a, const 
         i = i + 1 == (uint32_t)t.slots ? 0u : i + 1;
       }
       loff[u] = !present ? kRecNone : kk[u] == kEmptyIntKey ? kRecSpill : found;
-      if (present && loff[u] == kRecSpill) miss |= 1u << u;
     }
+    if constexpr (K1 != 0 && KB > 8) {
+      // part B of the slot whose header is part A: read once the tail is
+      // written (a slot still being claimed, or another part B: the full probe)
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        if (loff[u] >= kRecSpill) continue;
+        const unsigned long long* tl = reinterpret_cast<const unsigned long long*>(t.rec + loff[u] + tail_off);
+        if constexpr (KB == 12) {
+          const uint64_t w = __hip_atomic_load(tl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if ((uint32_t)(w >> 32) == kLeanNotReady || (uint32_t)w != (uint32_t)kb[u]) loff[u] = kRecSpill;
+        } else {
+          const uint64_t w1 = __hip_atomic_load(tl + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if ((uint32_t)(w1 >> 32) == kLeanNotReady ||
+              __hip_atomic_load(tl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != kb[u])
+            loff[u] = kRecSpill;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+      if (loff[u] == kRecSpill) miss |= 1u << u;
 #pragma unroll 1
     while (miss) {
       const int u = __builtin_ctz(miss);
-      miss &= miss - 1;
-      uint64_t key = 0;
+      uint64_t key = 0, part_b = 0;
       uint32_t slot_hash = 0;
 #pragma unroll
       for (int q = 0; q < R; ++q)
         if (q == u) {
           key = kk[q];
+          part_b = kb[q];
           slot_hash = sh[q];
         }
-      uint8_t* r = lean_find_or_insert(t, key, slot_hash);
-      const uint32_t v = r ? (uint32_t)(r - lds) : kRecSpill;
+      uint32_t v;
+      if constexpr (K1 == 0) {
+        uint8_t* r = lean_find_or_insert(t, key, slot_hash);
+        v = r ? (uint32_t)(r - lds) : kRecSpill;
+      } else {
+        v = lean2_find_or_insert<KB>(t, lds, key, part_b, (uint32_t)(base + (int64_t)u * kLdsBlock), slot_hash,
+                                     tail_off);
+        if (v == kLeanRetry) continue;  // the slot's claim completes before the next pass
+      }
+      miss &= miss - 1;
 #pragma unroll
       for (int q = 0; q < R; ++q)
         if (q == u) loff[q] = v;

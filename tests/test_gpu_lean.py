@@ -1,8 +1,9 @@
-"""GPU parity of the compact partitioned execution (a single 4- or 8-byte
-integer key, up to three value columns of SUM / MIN / MAX / counts, records
-of 16 or 32 bytes holding the row hash and validity bits): every key and
-value width, null values and null keys, the empty-header key, spills into the
-global table and split partitions, against the oracle."""
+"""GPU parity of the compact partitioned execution (an integer key of one or
+two 4- or 8-byte columns, up to three value columns of SUM / MIN / MAX /
+counts, records of 16 or 32 bytes holding the row hash and validity bits):
+every key and value width, two-column keys whose 8-byte parts are shared by
+many groups, null values and null keys, the empty-header key, spills into the
+global table, split partitions and 8192 partitions, against the oracle."""
 from __future__ import annotations
 
 import numpy as np
@@ -138,6 +139,133 @@ def test_same_result_as_general_records(gpu, monkeypatch):
     monkeypatch.setenv("MGB_LEAN", "off")
     b = sorted_result(*gpu_groupby(keys, reqs))
     assert np.array_equal(a[0][0][0], b[0][0][0])
+    for blk_a, blk_b in zip(a[1], b[1]):
+        for (va, ma), (vb, mb) in zip(blk_a, blk_b):
+            assert np.array_equal(va, vb)
+            assert (ma is None and mb is None) or np.array_equal(ma, mb)
+
+
+# ---- two-column integer keys: part A (the first 8 key bytes) in the LDS
+# header, part B and the first record's index in the record's tail
+
+def _pair_keys(rng, n, types, card, shared_a=0, key_null_frac=0.0):
+    """Two key columns drawn as `card` distinct tuples; with shared_a > 0 the
+    8-byte part of the key takes only that many values, so many groups share
+    their header and differ in the tail."""
+    cols = []
+    for c, t in enumerate(types):
+        info = np.iinfo(O.NP[t])
+        lo, hi = max(info.min, -2 ** 62), min(info.max, 2 ** 62)
+        cols.append(rng.integers(lo, hi, card, dtype=np.int64, endpoint=True).astype(O.NP[t]))
+    if shared_a:
+        wide = 0 if O.NP[types[0]]().itemsize >= O.NP[types[1]]().itemsize else 1
+        cols[wide] = cols[wide][rng.integers(0, shared_a, card)]
+    idx = rng.integers(0, card, n)
+    out = []
+    for c, t in enumerate(types):
+        valid = (rng.random(n) >= key_null_frac) if key_null_frac > 0 else None
+        out.append((t, cols[c][idx], valid))
+    return out
+
+
+PAIR_LAYOUTS = [
+    ((O.INT32, O.INT64), [O.FLOAT64]),
+    ((O.INT64, O.INT32), [O.INT32, O.FLOAT32]),
+    ((O.INT64, O.INT64), [O.INT32]),
+    ((O.INT32, O.INT32), [O.FLOAT64, O.INT64]),
+    ((O.UINT32, O.UINT64), [O.INT32, O.INT16, O.UINT32]),
+    ((O.UINT64, O.INT64), []),
+    ((O.INT32, O.UINT32), [O.INT8, O.UINT16, O.FLOAT32]),
+]
+
+
+@pytest.mark.parametrize("layout", PAIR_LAYOUTS,
+                         ids=[f"k{a}.{b}-" + "-".join(map(str, v)) for (a, b), v in PAIR_LAYOUTS])
+@pytest.mark.parametrize("shared_a", [0, 300])
+def test_two_column_layouts(gpu, layout, shared_a):
+    key_types, val_types = layout
+    rng = np.random.default_rng(600 + 7 * sum(key_types) + sum(val_types) + shared_a)
+    n = 300_000
+    keys = _pair_keys(rng, n, key_types, 60_000, shared_a)
+    _, reqs = _random_case(rng, n, O.INT32, 10, val_types or [O.INT32], 0.1, KINDS)
+    if not val_types:
+        reqs = [(reqs[0][0], [O.COUNT_ALL])]
+    _compare(keys, reqs)
+
+
+def test_two_column_null_keys_dropped(gpu):
+    rng = np.random.default_rng(610)
+    n = 300_000
+    keys = _pair_keys(rng, n, (O.INT32, O.INT64), 50_000, key_null_frac=0.05)
+    _, reqs = _random_case(rng, n, O.INT32, 10, [O.FLOAT64, O.INT32], 0.05, KINDS)
+    _compare(keys, reqs, include_null_keys=False)
+
+
+def test_two_column_empty_header_part(gpu):
+    """Tuples whose 8-byte part equals the empty header (INT64_MIN, and
+    (0, INT32_MIN) for two 4-byte columns) take the global table; others
+    sharing their second part do not."""
+    rng = np.random.default_rng(611)
+    n = 200_000
+    a = rng.integers(-1000, 1000, n).astype(np.int64)
+    b = rng.integers(-50, 50, n).astype(np.int32)
+    a[rng.random(n) < 0.05] = np.iinfo(np.int64).min
+    v = rng.standard_normal(n)
+    _compare([(O.INT64, a, None), (O.INT32, b, None)], [((O.FLOAT64, v, rng.random(n) > 0.1), KINDS)])
+    c = rng.integers(-1000, 1000, n).astype(np.int32)
+    d = rng.integers(-1000, 1000, n).astype(np.int32)
+    hit = rng.random(n) < 0.05
+    c[hit] = 0
+    d[hit] = np.iinfo(np.int32).min
+    _compare([(O.INT32, c, None), (O.INT32, d, None)], [((O.FLOAT64, v, None), KINDS)])
+
+
+@pytest.mark.parametrize("scale", ["0.05", "0.001"])
+def test_two_column_underestimate(gpu, scale, monkeypatch):
+    monkeypatch.setenv("MGB_ESTIMATE_SCALE", scale)
+    rng = np.random.default_rng(612)
+    n = 500_000
+    keys = _pair_keys(rng, n, (O.INT32, O.INT64), 200_000, shared_a=5000)
+    _, reqs = _random_case(rng, n, O.INT32, 10, [O.FLOAT64, O.INT32], 0.05, [O.SUM, O.COUNT_VALID, O.MAX, O.MIN])
+    _compare(keys, reqs)
+
+
+def test_two_column_zipf(gpu):
+    """The shape-stress distribution at 4M rows: (int32, int64) tuples, Zipf
+    s=1.1 over 1M of them; hot partitions cut into several items."""
+    rng = np.random.default_rng(613)
+    n, n_keys = 4_000_000, 1_000_000
+    ta = rng.integers(-2 ** 31, 2 ** 31 - 1, n_keys).astype(np.int32)
+    tb = rng.integers(-2 ** 62, 2 ** 62, n_keys).astype(np.int64)
+    idx = rng.permutation(n_keys)[_zipf_ranks(rng, n, n_keys, 1.1)]
+    reqs = [((O.FLOAT64, rng.standard_normal(n), None), [O.SUM, O.COUNT_VALID, O.MAX])]
+    _compare([(O.INT32, ta[idx], None), (O.INT64, tb[idx], None)], reqs)
+
+
+def test_eight_thousand_partitions(gpu):
+    """Enough groups for 8192 partitions (the most the compact scatter
+    takes), with one key column and with two."""
+    rng = np.random.default_rng(614)
+    keys, reqs = _random_case(rng, 7_000_000, O.INT64, 5_500_000, [O.INT32], 0.05, [O.SUM, O.COUNT_VALID, O.MAX])
+    _compare(keys, reqs)
+    n = 12_000_000
+    keys = _pair_keys(rng, n, (O.INT32, O.INT64), 11_000_000)
+    _compare(keys, [((O.FLOAT64, rng.standard_normal(n), None), [O.SUM, O.COUNT_VALID, O.MAX])])
+
+
+def test_two_column_same_result_as_general_records(gpu, monkeypatch):
+    from tests.helpers import gpu_groupby, sorted_result
+
+    rng = np.random.default_rng(615)
+    n = 1_000_000
+    keys = _pair_keys(rng, n, (O.INT64, O.INT32), 150_000, shared_a=2000)
+    _, reqs = _random_case(rng, n, O.INT32, 10, [O.INT32, O.INT64], 0.05,
+                           [O.SUM, O.COUNT_VALID, O.MIN, O.MAX, O.COUNT_ALL])
+    a = sorted_result(*gpu_groupby(keys, reqs))
+    monkeypatch.setenv("MGB_LEAN", "off")
+    b = sorted_result(*gpu_groupby(keys, reqs))
+    for (ka, _), (kb, _) in zip(a[0], b[0]):
+        assert np.array_equal(ka, kb)
     for blk_a, blk_b in zip(a[1], b[1]):
         for (va, ma), (vb, mb) in zip(blk_a, blk_b):
             assert np.array_equal(va, vb)
