@@ -399,7 +399,7 @@ constexpr int kHllRegs = 1 << kHllBits;
 // their histogram from these counts (hist_coarsen_kernel) instead of a
 // second pass over the keys. Workgroup b covers rows [b * chunk, +chunk), the
 // chunks of the partitioned execution.
-constexpr int kFineBits = 12;
+constexpr int kFineBits = 13;  // up to 8192 partitions (kLeanMaxLog2P)
 constexpr int kFineBuckets = 1 << kFineBits;
 
 // One row's hash into the registers and the fine counts.
@@ -1834,7 +1834,9 @@ Path forced_path()
 // (the histogram holds 2^log2p counts per chunk), at least 4096 rows each.
 int64_t partition_chunks(int64_t R, int log2p, int cus)
 {
-  int64_t nw = std::min<int64_t>((int64_t)cus * 4, std::max<int64_t>(64, (1ll << 22) >> log2p));
+  // 4 chunks per CU up to 2^13 partitions (the same chunks as the estimate's
+  // fine histogram), fewer beyond (the histogram is P x chunks words)
+  int64_t nw = std::min<int64_t>((int64_t)cus * 4, std::max<int64_t>(64, (1ll << 23) >> log2p));
   return std::max<int64_t>(1, std::min<int64_t>(nw, (R + 4095) / 4096));
 }
 

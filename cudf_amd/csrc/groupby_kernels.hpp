@@ -176,16 +176,17 @@ __host__ __device__ constexpr LeanOffsets lean_offsets(int kw, int w0, int w1, i
   return r;
 }
 
-// LDS layout of the compact tiled scatter: per partition its cursor, tile
-// count and tile start, 16 scan words, then per staged row its destination,
-// and the stage of records. Offsets in bytes.
+// LDS layout of the compact tiled scatter: per partition its cursor and its
+// tile count (scanned in place into its tile start; one more word for the
+// tile's total), 16 scan words, then per staged row its destination, and the
+// stage of records. Offsets in bytes.
 struct LeanScatterLds {
   size_t dst, stage, total;
 };
 __host__ __device__ constexpr size_t lean_align16(size_t x) { return (x + 15) & ~(size_t)15; }
 __host__ __device__ constexpr LeanScatterLds lean_scatter_lds(int P, int T, int rec_bytes)
 {
-  const size_t d = (size_t)(3 * P + 16) * 4;
+  const size_t d = (size_t)(2 * P + 1 + 16) * 4;
   const size_t st = lean_align16(d + (size_t)T * 4);
   return LeanScatterLds{d, st, st + (size_t)T * rec_bytes};
 }

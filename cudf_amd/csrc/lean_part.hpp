@@ -42,10 +42,10 @@ template <int RB16>
 constexpr int kLeanRows = RB16 == 1 ? 4 : 3;
 constexpr int kLeanProbe = 4;  // slots of a row's probe read before the full probe
 
-// Exclusive scan of the P counts `cnt` (P a power of two, at most
-// 8 * kLdsBlock) into `start`; returns the total. `tmp`: 16 words of LDS.
-// Ends with a workgroup barrier.
-__device__ __forceinline__ uint32_t lean_scan_counts(const uint32_t* cnt, uint32_t* start, int P, uint32_t* tmp)
+// Exclusive scan, in place, of the P counts `cnt` (P a power of two, at most
+// 8 * kLdsBlock) into tile starts; cnt[P] = the total, also returned. `tmp`:
+// 16 words of LDS. Ends with a workgroup barrier.
+__device__ __forceinline__ uint32_t lean_scan_counts(uint32_t* cnt, int P, uint32_t* tmp)
 {
   const int per = P > kLdsBlock ? P / kLdsBlock : 1;
   const int i0 = (int)threadIdx.x * per;
@@ -56,9 +56,11 @@ __device__ __forceinline__ uint32_t lean_scan_counts(const uint32_t* cnt, uint32
   uint32_t run = block_exclusive_scan_1024(sum, tmp, &total);
   for (int e = 0; e < per; ++e)
     if (i0 + e < P) {
-      start[i0 + e] = run;
-      run += cnt[i0 + e];
+      const uint32_t c = cnt[i0 + e];
+      cnt[i0 + e] = run;
+      run += c;
     }
+  if (threadIdx.x == 0) cnt[P] = total;
   __syncthreads();
   return total;
 }
@@ -82,9 +84,9 @@ __device__ __forceinline__ void lean_put(uint32_t (&w)[8], uint64_t v)
 }
 
 // Scatter of chunk blockIdx.x (rows [chunk * a.chunk, +a.chunk)) into
-// records. LDS (lean_scatter_lds): cursor, tile count and tile start of every
-// partition, 16 scan words, the destination row of every staged record, the
-// stage. K1 = 0: a single key column of K0 bytes.
+// records. LDS (lean_scatter_lds): cursor and tile count (then tile start)
+// of every partition, 16 scan words, the destination row of every staged
+// record, the stage. K1 = 0: a single key column of K0 bytes.
 template <int K0, int K1, int W0, int W1, int W2>
 __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint8_t* smem)
 {
@@ -97,9 +99,8 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
   const int T = a.tile_rows;
   const int items = T / kLdsBlock;
   uint32_t* cursor = reinterpret_cast<uint32_t*>(smem);
-  uint32_t* tcount = cursor + P;
-  uint32_t* tstart = tcount + P;
-  uint32_t* tmp = tstart + P;
+  uint32_t* tcount = cursor + P;  // counts of the tile's rows, then their starts (P + 1 words)
+  uint32_t* tmp = tcount + P + 1;
   const LeanScatterLds lay = lean_scatter_lds(P, T, RB16 * 16);
   uint32_t* dstrow = reinterpret_cast<uint32_t*>(smem + lay.dst);
   u32x4* stage = reinterpret_cast<u32x4*>(smem + lay.stage);
@@ -167,7 +168,8 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
       rank[k] = present[k] ? atomicAdd(&tcount[part[k]], 1u) : 0u;
     }
     __syncthreads();
-    const uint32_t tile_rows = lean_scan_counts(tcount, tstart, P, tmp);
+    const uint32_t tile_rows = lean_scan_counts(tcount, P, tmp);
+    const uint32_t* tstart = tcount;
 #pragma unroll
     for (int k = 0; k < I; ++k) {
       if (!present[k]) continue;
@@ -191,10 +193,9 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
       const uint32_t r = RB16 == 2 ? u >> 1 : u;
       out[(uint64_t)dstrow[r] * RB16 + (u & (uint32_t)(RB16 - 1))] = stage[u];
     }
-    for (int i = threadIdx.x; i < P; i += kLdsBlock) {
-      cursor[i] += tcount[i];
-      tcount[i] = 0;
-    }
+    for (int i = threadIdx.x; i < P; i += kLdsBlock) cursor[i] += tcount[i + 1] - tcount[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < P; i += kLdsBlock) tcount[i] = 0;
     __syncthreads();
   }
 }
