@@ -40,6 +40,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "groupby_host.hpp"
+#include "hot_keys.hpp"
 #include "lds_agg.hpp"
 #include "tiny_agg.hpp"
 #include "partition.hpp"
@@ -271,13 +272,13 @@ struct SampleParams {
   unsigned long long* stats;  // [0] distinct, [1] f1, [2] f2, [3] sampled rows, [4] keys found, [5..] keys
 };
 
-// Occurrence counts are only told apart up to 3 (distinct keys, keys seen
-// once, keys seen twice): a count already at 3 is not incremented, so that
-// the few keys of a low-cardinality input do not serialise the sample on
-// their counters.
+// Occurrence counts are only told apart up to kHotMin (distinct keys, keys
+// seen once and twice, hot keys): a count already there is not incremented,
+// so that the few keys of a low-cardinality input do not serialise the
+// sample on their counters.
 __device__ __forceinline__ void sample_count(uint32_t* c)
 {
-  if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 3u) atom_add(c, 1u);
+  if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < kHotMin) atom_add(c, 1u);
 }
 
 __global__ __launch_bounds__(kBlock) void sample_insert_kernel(const SampleParams s)
@@ -323,18 +324,28 @@ __global__ __launch_bounds__(kBlock) void sample_insert_kernel(const SampleParam
   }
 }
 
-// Clears the sample table: empty headers, zero counts and statistics.
+// Clears the sample table: empty headers, zero counts and statistics (and
+// the hot set's count, when there is one).
 __global__ __launch_bounds__(kBlock) void sample_clear_kernel(unsigned long long* hdr, uint32_t* cnt, int64_t scap,
                                                               unsigned long long empty, unsigned long long* stats,
-                                                              int nstats)
+                                                              int nstats, HotSet* hot)
 {
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i <= scap; i += step) {
     if (i < scap) hdr[i] = empty;
     cnt[i] = 0u;
     if (i < nstats) stats[i] = 0ull;
+    if (i == 0 && hot) hot->n = 0u;
   }
 }
+
+// The key columns of a compact call whose hot keys are collected: widths
+// (k1 = 0: one column) and, for two columns, the columns themselves (the
+// sample table then holds row references).
+struct HotSpec {
+  int32_t k0 = 0, k1 = 0;
+  KeyCol c0{}, c1{};
+};
 
 // The sample table's summary: distinct keys (occupied slots, the special
 // record included: n = scap + 1 counters), keys seen once and twice, into
@@ -343,10 +354,14 @@ __global__ __launch_bounds__(kBlock) void sample_clear_kernel(unsigned long long
 // workgroup strides over the counters, reduces its counts and keeps its
 // first kTinyKeys keys in LDS, then adds to the totals once (the same
 // counters written by every workgroup are the slow atomics here).
+// With `hot`, keys counted kHotMin times are added to the hot set (their raw
+// column values: from the header for one column, from the sampled row for
+// two); keys past kHotMax are left out.
 constexpr int kSampleSummaryBlocks = 64;
 __global__ __launch_bounds__(kBlock) void sample_summary_kernel(const unsigned long long* hdr, const uint32_t* cnt,
                                                                int64_t scap, unsigned long long* stats,
-                                                               unsigned long long* count, unsigned long long* keys)
+                                                               unsigned long long* count, unsigned long long* keys,
+                                                               HotSet* hot, const HotSpec hs)
 {
   __shared__ uint32_t red[3][kBlock / 64];
   __shared__ uint32_t nkeys;
@@ -362,6 +377,20 @@ __global__ __launch_bounds__(kBlock) void sample_summary_kernel(const unsigned l
     if (keys && c > 0 && i < scap) {
       const uint32_t j = atomicAdd(&nkeys, 1u);
       if (j < (uint32_t)kTinyKeys) kbuf[j] = hdr[i];
+    }
+    if (hot && c >= kHotMin && i < scap) {
+      const uint64_t h = hdr[i];
+      HotKey k;
+      if (hs.k1 == 0) {
+        k.c0 = hs.k0 == 8 ? h : (h & 0xffffffffull);
+        k.c1 = 0;
+      } else {
+        const int64_t row = (int64_t)(h & 0xffffffffull);
+        k.c0 = load_at(hs.c0.data, hs.k0, hs.c0.stride, row);
+        k.c1 = load_at(hs.c1.data, hs.k1, hs.c1.stride, row);
+      }
+      const uint32_t j = atomicAdd(&hot->n, 1u);
+      if (j < (uint32_t)kHotMax) hot->keys[j] = k;
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -417,7 +446,7 @@ __device__ __forceinline__ void hll_add(uint32_t* regs, uint32_t* counts, uint32
 // null-mask words) loaded before the current ones are hashed.
 template <int W>
 __device__ __forceinline__ void hll_rows_single(const AggParams& p, uint32_t* regs, uint32_t* counts, int64_t begin,
-                                                int64_t end)
+                                                int64_t end, const HotLds& hl)
 {
   constexpr int kItems = 8;
   const KeyCol& k = p.keys[0];
@@ -437,16 +466,88 @@ __device__ __forceinline__ void hll_rows_single(const AggParams& p, uint32_t* re
   if (begin < end) load(begin + threadIdx.x, cur, cm);
   for (int64_t row0 = begin + threadIdx.x; row0 < end; row0 += (int64_t)kItems * kLdsBlock) {
     load(row0 + (int64_t)kItems * kLdsBlock, nxt, nm);
+    uint32_t h[kItems];
+    bool in[kItems];
 #pragma unroll
     for (int u = 0; u < kItems; ++u) {
       const int64_t r = row0 + (int64_t)u * kLdsBlock;
-      if (r < end && ((cm[u] >> ((k.bit_off + r) & 31)) & 1u))
-        hll_add(regs, counts, hash_combine(0u, murmur3_bits(normalize_float_bits(cur[u], k.vclass), W, 0)));
+      in[u] = r < end && ((cm[u] >> ((k.bit_off + r) & 31)) & 1u);
+      h[u] = hash_combine(0u, murmur3_bits(normalize_float_bits(cur[u], k.vclass), W, 0));
     }
+    if (hl.n != 0) {
+      // hot rows: not partitioned
+      const uint64_t zero[kItems] = {};
+      int hi[kItems];
+      hot_find_n<kItems>(hl, h, cur, zero, in, hi);
+#pragma unroll
+      for (int u = 0; u < kItems; ++u) in[u] = in[u] && hi[u] < 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kItems; ++u)
+      if (in[u]) hll_add(regs, counts, h[u]);
 #pragma unroll
     for (int u = 0; u < kItems; ++u) {
       cur[u] = nxt[u];
       cm[u] = nm[u];
+    }
+  }
+}
+
+// The rows of two key columns of W0 and W1 bytes (4 or 8 each) without null
+// grouping, as hll_rows_single: both columns' values (and null-mask words)
+// of the next step loaded before the current ones are hashed; the hot check
+// takes the values already in registers.
+template <int W0, int W1>
+__device__ __forceinline__ void hll_rows_pair(const AggParams& p, uint32_t* regs, uint32_t* counts, int64_t begin,
+                                              int64_t end, const HotLds& hl)
+{
+  constexpr int kItems = 4;
+  const KeyCol& k0 = p.keys[0];
+  const KeyCol& k1 = p.keys[1];
+  const bool n0 = p.skip_null_keys && k0.mask, n1 = p.skip_null_keys && k1.mask;
+  uint64_t a[kItems], b[kItems], na[kItems], nb[kItems];
+  uint32_t ma[kItems], mb[kItems], nma[kItems], nmb[kItems];
+  auto load = [&](int64_t row0, uint64_t (&va)[kItems], uint64_t (&vb)[kItems], uint32_t (&wa)[kItems],
+                  uint32_t (&wb)[kItems]) {
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+      int64_t r = row0 + (int64_t)u * kLdsBlock;
+      r = r < end ? r : end - 1;
+      if constexpr (W0 == 8) va[u] = to_global(reinterpret_cast<const uint64_t*>(k0.data))[r];
+      else va[u] = to_global(reinterpret_cast<const uint32_t*>(k0.data))[r];
+      if constexpr (W1 == 8) vb[u] = to_global(reinterpret_cast<const uint64_t*>(k1.data))[r];
+      else vb[u] = to_global(reinterpret_cast<const uint32_t*>(k1.data))[r];
+      wa[u] = n0 ? to_global(k0.mask)[(k0.bit_off + r) >> 5] : ~0u;
+      wb[u] = n1 ? to_global(k1.mask)[(k1.bit_off + r) >> 5] : ~0u;
+    }
+  };
+  if (begin < end) load(begin + threadIdx.x, a, b, ma, mb);
+  for (int64_t row0 = begin + threadIdx.x; row0 < end; row0 += (int64_t)kItems * kLdsBlock) {
+    load(row0 + (int64_t)kItems * kLdsBlock, na, nb, nma, nmb);
+    uint32_t h[kItems];
+    bool in[kItems];
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+      const int64_t r = row0 + (int64_t)u * kLdsBlock;
+      in[u] = r < end && ((ma[u] >> ((k0.bit_off + r) & 31)) & 1u) && ((mb[u] >> ((k1.bit_off + r) & 31)) & 1u);
+      h[u] = hash_combine(hash_combine(0u, murmur3_bits(normalize_float_bits(a[u], k0.vclass), W0, 0)),
+                          murmur3_bits(normalize_float_bits(b[u], k1.vclass), W1, 0));
+    }
+    if (hl.n != 0) {
+      int hi[kItems];
+      hot_find_n<kItems>(hl, h, a, b, in, hi);
+#pragma unroll
+      for (int u = 0; u < kItems; ++u) in[u] = in[u] && hi[u] < 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kItems; ++u)
+      if (in[u]) hll_add(regs, counts, h[u]);
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+      a[u] = na[u];
+      b[u] = nb[u];
+      ma[u] = nma[u];
+      mb[u] = nmb[u];
     }
   }
 }
@@ -461,23 +562,43 @@ __device__ __forceinline__ bool sample_saturated(const unsigned long long* stats
 
 // Workgroup w takes chunks w, w + gridDim.x, ... (nw chunks of `chunk`
 // rows): each chunk's fine counts are written when it is done, the
-// registers once at the end (block_regs[w]).
+// registers once at the end (block_regs[w]). With `hot`, rows of hot keys
+// are left out of both (the compact scatter aggregates them apart).
 __global__ __launch_bounds__(kLdsBlock) void hll_hist_kernel(const AggParams p, uint8_t* block_regs, uint32_t* fine,
-                                                            int64_t chunk, int32_t nw, const unsigned long long* gate)
+                                                            int64_t chunk, int32_t nw, const unsigned long long* gate,
+                                                            const HotSet* hot, const HotSpec hs)
 {
   if (sample_saturated(gate)) return;
-  extern __shared__ uint32_t regs[];  // kHllRegs registers, then kFineBuckets counts
+  extern __shared__ uint32_t regs[];  // kHllRegs registers, kFineBuckets counts, then the hot keys' lookup
   uint32_t* counts = regs + kHllRegs;
   for (int i = threadIdx.x; i < kHllRegs + kFineBuckets; i += kLdsBlock) regs[i] = 0;
+  HotLds hl{nullptr, nullptr, 0};
+  if (hot) {
+    HotKey* hk = reinterpret_cast<HotKey*>(counts + kFineBuckets);
+    hl = hot_load(hot, hk, reinterpret_cast<uint32_t*>(hk + kHotMax), hs.k0, hs.k1);
+  }
   __syncthreads();
   const bool single = p.num_keys == 1 && !(p.key_mode != KEY_INT && p.compare_key_nulls) && p.keys[0].stride == p.keys[0].width;
+  // two 4- or 8-byte columns, nulls not grouped: their own row loop
+  const bool pair = p.num_keys == 2 && !p.compare_key_nulls && p.keys[0].stride == p.keys[0].width &&
+                    p.keys[1].stride == p.keys[1].width && (p.keys[0].width == 4 || p.keys[0].width == 8) &&
+                    (p.keys[1].width == 4 || p.keys[1].width == 8);
+  const int pair_shape = pair ? (p.keys[0].width == 8) * 2 + (p.keys[1].width == 8) : -1;
   for (int c = blockIdx.x; c < nw; c += gridDim.x) {
     const int64_t begin = (int64_t)c * chunk;
     const int64_t end = begin + chunk < p.num_rows ? begin + chunk : p.num_rows;
     if (single && p.keys[0].width == 8) {
-      hll_rows_single<8>(p, regs, counts, begin, end);
+      hll_rows_single<8>(p, regs, counts, begin, end, hl);
     } else if (single && p.keys[0].width == 4) {
-      hll_rows_single<4>(p, regs, counts, begin, end);
+      hll_rows_single<4>(p, regs, counts, begin, end, hl);
+    } else if (pair_shape == 0) {
+      hll_rows_pair<4, 4>(p, regs, counts, begin, end, hl);
+    } else if (pair_shape == 1) {
+      hll_rows_pair<4, 8>(p, regs, counts, begin, end, hl);
+    } else if (pair_shape == 2) {
+      hll_rows_pair<8, 4>(p, regs, counts, begin, end, hl);
+    } else if (pair_shape == 3) {
+      hll_rows_pair<8, 8>(p, regs, counts, begin, end, hl);
     } else {
       constexpr int kItems = 4;  // rows per thread per step, hashed column by column
       for (int64_t row0 = begin + threadIdx.x; row0 < end; row0 += (int64_t)kItems * kLdsBlock) {
@@ -490,6 +611,20 @@ __global__ __launch_bounds__(kLdsBlock) void hll_hist_kernel(const AggParams p, 
         uint32_t h[kItems];
         uint64_t key0[kItems];
         hash_rows(p, row0, kLdsBlock, present, h, key0);
+        if (hl.n != 0) {
+          // two-column compact keys: hot rows are not partitioned
+          uint64_t c0[kItems], c1[kItems];
+#pragma unroll
+          for (int k = 0; k < kItems; ++k) {
+            const int64_t r = present[k] ? row0 + k * kLdsBlock : 0;
+            c0[k] = load_at(hs.c0.data, hs.k0, hs.c0.stride, r);
+            c1[k] = hs.k1 ? load_at(hs.c1.data, hs.k1, hs.c1.stride, r) : 0ull;
+          }
+          int hi[kItems];
+          hot_find_n<kItems>(hl, h, c0, c1, present, hi);
+#pragma unroll
+          for (int k = 0; k < kItems; ++k) present[k] = present[k] && hi[k] < 0;
+        }
 #pragma unroll
         for (int k = 0; k < kItems; ++k)
           if (present[k]) hll_add(regs, counts, h[k]);
@@ -1029,7 +1164,9 @@ struct Workspace {
   DeviceBuffer fine{&mem};    // fine bucket counts per chunk of the last key pass (hll_hist_kernel)
   DeviceBuffer sorts{&mem};   // row groups, sort keys and radix-sort space of the sort-path aggregations
   DeviceBuffer lists{&mem};   // COLLECT_LIST / COLLECT_SET results: offsets and elements
+  DeviceBuffer hot{&mem};     // the call's hot keys (HotSet), when the compact execution may take it
   bool fine_valid = false;  // `fine` holds the counts of this call's keys
+  bool fine_hot = false;    // ... without the rows of the hot keys in `hot`
   int64_t fine_nw = 0, fine_chunk = 0;
   unsigned long long* pinned = nullptr;  // host readback area (kMaxPinned slots)
   uint8_t* staging = nullptr;            // host source of parameter uploads (kStagingBytes)
@@ -1850,6 +1987,7 @@ struct SampleKeys {
 
 constexpr int kHllReduceBlocks = kHllRegs / kHllReduceRegs;
 constexpr int kPinnedHll = 256;  // first host readback slot of the HyperLogLog sums (then the zero counts)
+constexpr int kPinnedHot = kMaxPinned - 8;  // readback slot of the hot keys' count
 static_assert(kPinnedHll + kHllReduceBlocks * 3 / 2 <= kMaxPinned, "readback slots past the end");
 
 // Estimate of the number of distinct key rows. A sample of 65536 rows
@@ -1861,12 +1999,15 @@ static_assert(kPinnedHll + kHllReduceBlocks * 3 / 2 <= kMaxPinned, "readback slo
 // counts each chunk's rows per fine bucket (ws->fine). The pass is launched
 // right after the sample and reads the sample's counts on the device (it
 // returns at once when the sample is complete), so one host wait serves
-// both.
+// both. With `hot` (a call the compact execution may take), the sample's
+// hot keys are collected into ws->hot and the pass leaves their rows out
+// of the fine counts (ws->fine_hot).
 int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStream_t stream, mgb_status& st,
-                        SampleKeys* sk = nullptr)
+                        SampleKeys* sk = nullptr, const HotSpec* hot = nullptr)
 {
   st = MGB_SUCCESS;
   ws->fine_valid = false;
+  ws->fine_hot = false;
   const int64_t kSample = 1 << 16;
   const int64_t scap = 4 * kSample;
   // small inputs are counted exactly; larger ones are sampled
@@ -1884,8 +2025,12 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
   const size_t regs_bytes = align_up((size_t)hll_wgs * kHllRegs, 256);
   const size_t hll_bytes = regs_bytes + align_up((size_t)kHllReduceBlocks * 8, 256) +
                            align_up((size_t)kHllReduceBlocks * 4, 256);
+  // MGB_HOT_KEYS=off: no hot keys (every row partitioned)
+  const char* hot_env = std::getenv("MGB_HOT_KEYS");
+  if (!may_need_hll || (hot_env && std::string(hot_env) == "off")) hot = nullptr;
   if (ws->main.reserve(sample_bytes + (may_need_hll ? hll_bytes : 0), stream) != hipSuccess ||
-      (may_need_hll && ws->fine.reserve((size_t)kFineBuckets * nb * 4, stream) != hipSuccess)) {
+      (may_need_hll && ws->fine.reserve((size_t)kFineBuckets * nb * 4, stream) != hipSuccess) ||
+      (hot && ws->hot.reserve(sizeof(HotSet), stream) != hipSuccess)) {
     st = fail(MGB_ERROR_ALLOC, "allocation of the cardinality estimate's temporaries failed");
     return R;
   }
@@ -1907,17 +2052,20 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
   hipError_t e = hipSuccess;
   {
     ProfScope prof("estimate", stream);
-    sample_clear_kernel<<<256, kBlock, 0, stream>>>(sp.hdr, sp.cnt, scap, empty, sp.stats, 5);
+    HotSet* hot_set = hot ? static_cast<HotSet*>(ws->hot.ptr) : nullptr;
+    const HotSpec hs = hot ? *hot : HotSpec{};
+    sample_clear_kernel<<<256, kBlock, 0, stream>>>(sp.hdr, sp.cnt, scap, empty, sp.stats, 5, hot_set);
     sample_insert_kernel<<<(unsigned)((samples + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sp);
     sample_summary_kernel<<<kSampleSummaryBlocks, kBlock, 0, stream>>>(sp.hdr, sp.cnt, scap, sp.stats, sp.stats + 4,
-                                                                        want_keys ? sp.stats + 5 : nullptr);
+                                                                        want_keys ? sp.stats + 5 : nullptr, hot_set, hs);
     if (may_need_hll) {
-      const size_t lds = (size_t)(kHllRegs + kFineBuckets) * 4;
+      const size_t lds = (size_t)(kHllRegs + kFineBuckets) * 4 +
+                         (hot ? (size_t)kHotMax * sizeof(HotKey) + (size_t)kHotSlots * 4 : 0);
       e = set_dynamic_lds((const void*)hll_hist_kernel, lds);
       if (e == hipSuccess) {
         hll_hist_kernel<<<(unsigned)hll_wgs, kLdsBlock, lds, stream>>>(base, regs,
                                                                        static_cast<uint32_t*>(ws->fine.ptr), chunk,
-                                                                       (int32_t)nb, sp.stats);
+                                                                       (int32_t)nb, sp.stats, hot_set, hs);
         hll_reduce_kernel<<<(unsigned)kHllReduceBlocks, 1024, 0, stream>>>(regs, (int)hll_wgs, sums, zeros, sp.stats);
       }
     }
@@ -1926,6 +2074,9 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
   if (e == hipSuccess)
     e = hipMemcpyAsync(ws->pinned, sp.stats, (want_keys ? 5 + kTinyKeys : 4) * sizeof(unsigned long long),
                        hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess && hot)
+    e = hipMemcpyAsync(ws->pinned + kPinnedHot, &static_cast<HotSet*>(ws->hot.ptr)->n, 4, hipMemcpyDeviceToHost,
+                       stream);
   if (e == hipSuccess && may_need_hll)
     e = hipMemcpyAsync(ws->pinned + kPinnedHll, sums, (size_t)kHllReduceBlocks * 8, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess && may_need_hll)
@@ -1936,6 +2087,8 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
     st = fail(MGB_ERROR_HIP, std::string("cardinality estimate: ") + hipGetErrorString(e));
     return R;
   }
+  // hot keys found: the fine counts left their rows out
+  ws->fine_hot = hot && (*reinterpret_cast<const uint32_t*>(ws->pinned + kPinnedHot)) > 0u;
   const uint64_t d = ws->pinned[0], f1 = ws->pinned[1], sampled = ws->pinned[3];
   // every row sampled, or a saturated sample (as sample_saturated): all groups found
   const bool complete = (int64_t)sampled >= R || samples == R || (f1 == 0 && d * 8 <= sampled);
@@ -2653,20 +2806,41 @@ bool lean_shape_of(const Plan& plan, const AggParams& src, LeanPlan& lp)
   return true;
 }
 
-// The compact scatter's tile for 2^log2p partitions; lp.on when it fits LDS.
-void lean_tile(LeanPlan& lp, int log2p)
+// The compact scatter's tile for 2^log2p partitions (hot_stride: the LDS
+// stride of hot-key records, 0 without hot keys); lp.on when it fits LDS.
+void lean_tile(LeanPlan& lp, int log2p, int hot_stride)
 {
   lp.on = false;
   lp.tile = 0;
   if (lp.rec_bytes == 0 || log2p > kLeanMaxLog2P) return;
   const int P = 1 << log2p;
   for (int tt = kScatterTile; tt >= kLdsBlock; tt -= kLdsBlock)
-    if (lean_scatter_lds(P, tt, lp.rec_bytes).total <= lds_limit()) {
+    if (lean_scatter_lds(P, tt, lp.rec_bytes, hot_stride).total <= lds_limit()) {
       lp.tile = tt;
       break;
     }
-  lp.lds = lean_scatter_lds(P, lp.tile, lp.rec_bytes).total;
+  lp.lds = lean_scatter_lds(P, lp.tile, lp.rec_bytes, hot_stride).total;
   lp.on = lp.tile > 0;
+}
+
+// The representative records of two-column hot keys (records rep_base + i):
+// their key columns at the record's key offsets, the rest zero. Global-table
+// groups of hot keys refer to them.
+__global__ __launch_bounds__(kHotMax) void hot_reps_kernel(const HotSet* hot, uint8_t* recs, int64_t rep_base,
+                                                          int32_t rec_bytes, int32_t off0, int32_t k0, int32_t off1,
+                                                          int32_t k1)
+{
+  const int i = threadIdx.x;
+  const int n = (int)min(hot->n, (uint32_t)kHotMax);
+  if (i >= n) return;
+  uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  const uint64_t c0 = hot->keys[i].c0, c1 = hot->keys[i].c1;
+  w[off0 / 4] = (uint32_t)c0;
+  if (k0 == 8) w[off0 / 4 + 1] = (uint32_t)(c0 >> 32);
+  w[off1 / 4] = (uint32_t)c1;
+  if (k1 == 8) w[off1 / 4 + 1] = (uint32_t)(c1 >> 32);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(recs + (rep_base + i) * (int64_t)rec_bytes);
+  for (int j = 0; j < rec_bytes / 4; ++j) dst[j] = w[j];
 }
 
 mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& base, int64_t R, int64_t G_est,
@@ -2731,8 +2905,9 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     // global table for the groups of split or spilled items: no more than the
     // items could hold, nor than the groups there are
     const int64_t split_bound = (R + max_item_rows - 1) / max_item_rows + P / 16 + 1;
-    int64_t gcap = 2 * std::min<int64_t>({R, split_bound * (int64_t)(geom.slots + 1), 2 * est}) + 1024;
-    gcap = std::min<int64_t>(gcap, 2 * R + 64);
+    // (and the hot keys' groups)
+    int64_t gcap = 2 * std::min<int64_t>({R, split_bound * (int64_t)(geom.slots + 1), 2 * est}) + 1024 + kHotMax;
+    gcap = std::min<int64_t>(gcap, 2 * R + 64 + kHotMax);
     const int64_t gblocks = (gcap + 1 + kBlock - 1) / kBlock;
 
     // record layout of a partitioned row: 8-byte fields, then 4, 2 and 1-byte
@@ -2753,10 +2928,23 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     int flags_rec_off = (int)align_up((size_t)rec, (size_t)std::max(flag_bytes, 1));
     rec = flags_rec_off + flag_bytes;
     int rec_bytes = (int)align_up((size_t)rec, 8);
-    // the compact records, when the call has them
+    // the compact records, when the call has them. The estimate's fine
+    // counts serve as the histogram when they cover these partitions and
+    // chunks; when they leave the hot keys' rows out, only the compact
+    // scatter (which aggregates those rows apart) may take them.
+    const bool fine_ok = ws->fine_valid && log2p <= kFineBits && ws->fine_nw == nw && ws->fine_chunk == chunk;
+    const int hot_words = lds_stride_of(plan.stride) / 8;
     LeanPlan lean = lean_shape;
-    if (lean_ok) lean_tile(lean, log2p);
-    else lean.on = false;
+    lean.on = false;
+    bool use_hot = false;
+    if (lean_ok) {
+      if (fine_ok && ws->fine_hot) {
+        lean_tile(lean, log2p, hot_words * 8);
+        use_hot = lean.on;
+      }
+      if (!lean.on) lean_tile(lean, log2p, 0);  // without hot keys (then a second pass counts every row)
+    }
+    const bool coarsen = fine_ok && (!ws->fine_hot || use_hot);
     // LDS record words (two-column compact keys: the plan's fields, then the tail)
     const int lds_words = lds_stride_of(lean.on && lean.tail ? tail_off + lean.tail : plan.stride) / 8;
     if (lean.on) {
@@ -2769,8 +2957,8 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
 
     // workspace: records, hist, offsets, block sums, item counts/offsets, global table, counters
     size_t off = 0;
-    const size_t recs_off = off;
-    off += align_up((size_t)R * rec_bytes, 256);
+    const size_t recs_off = off;  // the partitioned records, then kHotMax representative records of hot keys
+    off += align_up((size_t)(R + kHotMax) * rec_bytes, 256);
     const size_t hist_off = off;
     off += align_up((size_t)nh * 4, 256);
     const size_t offs_off = off;
@@ -2838,7 +3026,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     const size_t part_lds = (size_t)P * 4;
     MGB_HIP(set_dynamic_lds((const void*)part_hist_kernel, (int)part_lds));
     MGB_HIP(set_dynamic_lds((const void*)part_scatter_kernel, (int)part_lds));
-    if (ws->fine_valid && log2p <= kFineBits && ws->fine_nw == nw && ws->fine_chunk == chunk) {
+    if (coarsen) {
       // the estimate's pass counted the rows per fine bucket of these chunks
       ProfScope prof("partition_hist", stream);
       const dim3 grid((unsigned)((nw + 63) / 64), (unsigned)((P + 63) / 64));
@@ -2862,40 +3050,6 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       if (fixed_lds + (size_t)tt * rec_bytes <= lds_limit()) { tile = tt; break; }
     pp.tile_rows = tile;
     const size_t tiled_lds = fixed_lds + (size_t)tile * rec_bytes;
-    if (lean.on) {
-      LeanScatterArgs sa{};
-      const KeyCol& kc = plan.keys[0];
-      sa.key = kc.data;
-      sa.key_mask = src.skip_null_keys && kc.mask ? kc.mask + (kc.bit_off >> 5) : nullptr;
-      if (lean.k1) {
-        const KeyCol& kc1 = plan.keys[1];
-        sa.key1 = kc1.data;
-        sa.key1_mask = src.skip_null_keys && kc1.mask ? kc1.mask + (kc1.bit_off >> 5) : nullptr;
-      }
-      for (int c = 0; c < src.num_vals; ++c) {
-        sa.val[c] = src.vals[c].data;
-        sa.mask[c] = src.vals[c].mask ? src.vals[c].mask + (src.vals[c].bit_off >> 5) : nullptr;
-      }
-      sa.offsets = pp.offsets;
-      sa.out = pp.out;
-      sa.rows = R;
-      sa.chunk = chunk;
-      sa.log2p = log2p;
-      sa.nblocks = (int32_t)nw;
-      sa.tile_rows = lean.tile;
-      sa.flag_bits = lean.flag_bits;
-      ProfScope prof("partition_scatter", stream);
-      st = lean_scatter(lean.k0, lean.k1, lean.w, sa, (unsigned)nw, lean.lds, stream);
-      if (st != MGB_SUCCESS) return st;
-    } else if (tile > 0) {
-      MGB_HIP(set_dynamic_lds((const void*)part_scatter_tiled_kernel, (int)tiled_lds));
-      ProfScope prof("partition_scatter", stream);
-      part_scatter_tiled_kernel<<<(unsigned)nw, kLdsBlock, tiled_lds, stream>>>(pp);
-    } else {
-      ProfScope prof("partition_scatter", stream);
-      part_scatter_kernel<<<(unsigned)nw, kBlock, part_lds, stream>>>(pp);
-    }
-    MGB_HIP(hipGetLastError());
 
     // work items per partition
     uint32_t* icnt = reinterpret_cast<uint32_t*>(wsb + icnt_off);
@@ -2972,10 +3126,68 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       st = upload_pass2(ws, q, stream, &lp.p2);
       if (st != MGB_SUCCESS) return st;
     }
+    // the LDS parameters in device memory: the compact kernels' full view
+    // (spilled rows, the groups' output, the hot keys' fold)
+    const LdsParams* d_lp = nullptr;
     if (lean.on) {
-      const LdsParams* d_lp = nullptr;
       st = upload_lds(ws, lp, stream, &d_lp);
       if (st != MGB_SUCCESS) return st;
+    }
+    if (lean.on) {
+      LeanScatterArgs sa{};
+      const KeyCol& kc = plan.keys[0];
+      sa.key = kc.data;
+      sa.key_mask = src.skip_null_keys && kc.mask ? kc.mask + (kc.bit_off >> 5) : nullptr;
+      if (lean.k1) {
+        const KeyCol& kc1 = plan.keys[1];
+        sa.key1 = kc1.data;
+        sa.key1_mask = src.skip_null_keys && kc1.mask ? kc1.mask + (kc1.bit_off >> 5) : nullptr;
+      }
+      for (int c = 0; c < src.num_vals; ++c) {
+        sa.val[c] = src.vals[c].data;
+        sa.mask[c] = src.vals[c].mask ? src.vals[c].mask + (src.vals[c].bit_off >> 5) : nullptr;
+      }
+      sa.offsets = pp.offsets;
+      sa.out = pp.out;
+      sa.rows = R;
+      sa.chunk = chunk;
+      sa.log2p = log2p;
+      sa.nblocks = (int32_t)nw;
+      sa.tile_rows = lean.tile;
+      sa.flag_bits = lean.flag_bits;
+      if (use_hot) {
+        sa.hot = static_cast<const HotSet*>(ws->hot.ptr);
+        sa.lp = d_lp;
+        sa.tmpl = d_tmpl;
+        sa.hot_words = hot_words;
+        sa.rows_off = lp.src.rows_op >= 0 ? lp.src.ops[lp.src.rows_op].off : -1;
+        sa.key_signed = !lean.k1 && plan.keys[0].vclass == VC_SIGNED;
+        sa.rowref = lean.k1 != 0;
+        sa.rep_base = R;
+        for (int c = 0; c < src.num_vals; ++c) {
+          const ValCol& v = lp.src.vals[c];
+          sa.cols[c] = LeanCol{v.sum_off, v.min_off, v.max_off, v.nulls_off,
+                               v.vclass == VC_SIGNED ? LEAN_SIGNED : v.vclass == VC_UNSIGNED ? LEAN_UNSIGNED : LEAN_FLOAT,
+                               -1};
+        }
+        if (lean.k1)
+          hot_reps_kernel<<<1, kHotMax, 0, stream>>>(sa.hot, recs, R, rec_bytes, lean.ko.off0, lean.k0, lean.ko.off1,
+                                                     lean.k1);
+      }
+      ProfScope prof("partition_scatter", stream);
+      st = lean_scatter(lean.k0, lean.k1, lean.w, sa, (unsigned)nw, lean.lds, stream);
+      if (st != MGB_SUCCESS) return st;
+    } else if (tile > 0) {
+      MGB_HIP(set_dynamic_lds((const void*)part_scatter_tiled_kernel, (int)tiled_lds));
+      ProfScope prof("partition_scatter", stream);
+      part_scatter_tiled_kernel<<<(unsigned)nw, kLdsBlock, tiled_lds, stream>>>(pp);
+    } else {
+      ProfScope prof("partition_scatter", stream);
+      part_scatter_kernel<<<(unsigned)nw, kBlock, part_lds, stream>>>(pp);
+    }
+    MGB_HIP(hipGetLastError());
+
+    if (lean.on) {
       LeanAggArgs aa{};
       aa.recs = recs;
       aa.offsets = lp.offsets;
@@ -3075,7 +3287,21 @@ mgb_status run_groupby(const Plan& plan, int64_t R, Workspace* ws, mgb_out_colum
   mgb_status st = MGB_SUCCESS;
   ws->fine_valid = false;
   SampleKeys sk;
-  int64_t G_est = sample_estimate(ws, base, R, stream, st, &sk);
+  // a call the compact execution may take collects its hot keys
+  HotSpec hot_spec;
+  const HotSpec* hot = nullptr;
+  {
+    AggParams src = base;
+    LeanPlan shape;
+    if (single_launch_ops(plan, src) && lean_shape_of(plan, src, shape)) {
+      hot_spec.k0 = shape.k0;
+      hot_spec.k1 = shape.k1;
+      hot_spec.c0 = plan.keys[0];
+      if (shape.k1) hot_spec.c1 = plan.keys[1];
+      hot = &hot_spec;
+    }
+  }
+  int64_t G_est = sample_estimate(ws, base, R, stream, st, &sk, hot);
   if (st != MGB_SUCCESS) return st;
   if (const char* e = std::getenv("MGB_ESTIMATE_SCALE")) {
     // test hook: a scaled estimate (below 1 forces the overflow-and-retry

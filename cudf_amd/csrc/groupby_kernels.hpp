@@ -176,20 +176,63 @@ __host__ __device__ constexpr LeanOffsets lean_offsets(int kw, int w0, int w1, i
   return r;
 }
 
+// Hot keys of a compact call: keys seen kHotMin times or more in the
+// estimate's sample (at most kHotMax of them), as their raw key column
+// values (zero-extended). The estimate's fine histogram leaves their rows
+// out, and the compact scatter aggregates those rows in LDS records of its
+// own, folded into the global table when the chunk is done, instead of
+// writing them as partitioned records: under skew (a few keys holding much
+// of the rows) the records and their aggregation shrink accordingly.
+constexpr int kHotMax = 256;
+constexpr uint32_t kHotMin = 16;
+constexpr int kHotSlots = 2048;  // LDS lookup slots (a power of two; few occupied, so a miss ends at its home slot)
+struct HotKey {
+  uint64_t c0, c1;  // key column 0 and 1 (0 for a single column)
+};
+struct HotSet {
+  uint32_t n;
+  uint32_t pad[3];
+  HotKey keys[kHotMax];
+};
+
 // LDS layout of the compact tiled scatter: per partition its cursor and its
 // tile count (scanned in place into its tile start; one more word for the
 // tile's total), 16 scan words, then per staged row its destination, and the
 // stage of records. Offsets in bytes.
+// With hot keys (hot_stride: their LDS record stride, else 0) the hot keys,
+// their lookup slots, a counter block (count, overflow, 2 words for the
+// fold, kHotMax bits of reached records, 64 dummy 8-byte words) and
+// kHotMax + 1 records follow.
 struct LeanScatterLds {
-  size_t dst, stage, total;
+  size_t dst, stage, hot_keys, hot_idx, hot_ctr, hot_rec, total;
 };
 __host__ __device__ constexpr size_t lean_align16(size_t x) { return (x + 15) & ~(size_t)15; }
-__host__ __device__ constexpr LeanScatterLds lean_scatter_lds(int P, int T, int rec_bytes)
+__host__ __device__ constexpr LeanScatterLds lean_scatter_lds(int P, int T, int rec_bytes, int hot_stride = 0)
 {
   const size_t d = (size_t)(2 * P + 1 + 16) * 4;
   const size_t st = lean_align16(d + (size_t)T * 4);
-  return LeanScatterLds{d, st, st + (size_t)T * rec_bytes};
+  const size_t hk = st + (size_t)T * rec_bytes;
+  if (hot_stride == 0) return LeanScatterLds{d, st, hk, hk, hk, hk, hk};
+  const size_t hi = hk + (size_t)kHotMax * 16;
+  const size_t hc = hi + (size_t)kHotSlots * 4;
+  const size_t hr = hc + 16 + kHotMax / 8 + 64 * 8;
+  return LeanScatterLds{d, st, hk, hi, hc, hr, hr + (size_t)(kHotMax + 1) * hot_stride};
 }
+
+// A value column's updates in the compact aggregation: LDS record field
+// offsets (-1: none) of its SUM, MIN, MAX and null count, its class and its
+// validity bit in the hash word (-1: never null).
+enum LeanClass : int32_t { LEAN_SIGNED = 0, LEAN_UNSIGNED = 1, LEAN_FLOAT = 2 };
+struct LeanCol {
+  int32_t sum_off;
+  int32_t min_off;
+  int32_t max_off;
+  int32_t nulls_off;
+  int32_t cls;       // LeanClass: f32 / f64 by the column's width
+  int32_t flag_bit;
+};
+
+struct LdsParams;
 
 // Arguments of the compact scatter. Masks point at the word holding row 0's
 // bit (the columns' bit offsets are multiples of 32).
@@ -209,21 +252,16 @@ struct LeanScatterArgs {
   int32_t tile_rows;                // rows staged per tile: a multiple of 1024, at most 4096
                                     // (partitions: at most 2^kLeanMaxLog2P)
   int32_t flag_bits;                // bits 4c..4c+3: column c's validity bit in the hash word (15: never null)
-};
-
-struct LdsParams;
-
-// A value column's updates in the compact aggregation: LDS record field
-// offsets (-1: none) of its SUM, MIN, MAX and null count, its class and its
-// validity bit in the hash word (-1: never null).
-enum LeanClass : int32_t { LEAN_SIGNED = 0, LEAN_UNSIGNED = 1, LEAN_FLOAT = 2 };
-struct LeanCol {
-  int32_t sum_off;
-  int32_t min_off;
-  int32_t max_off;
-  int32_t nulls_off;
-  int32_t cls;       // LeanClass: f32 / f64 by the column's width
-  int32_t flag_bit;
+  // hot keys (hot == nullptr: none)
+  const HotSet* hot;
+  const LdsParams* lp;              // the global table (device memory), for the hot records' fold
+  const uint64_t* tmpl;             // identity record (device memory)
+  int32_t hot_words;                // 8-byte words of a hot record
+  int32_t rows_off;                 // row-count field of a record, or -1
+  int32_t key_signed;               // a single 4-byte key sign-extended into the header
+  int32_t rowref;                   // two-column keys: headers are record indices
+  int64_t rep_base;                 // two-column keys: record index of hot key 0's record
+  LeanCol cols[kLeanCols];           // the value columns' updates of a hot record (flag_bit: -1)
 };
 
 // Arguments of the compact aggregation (one work item per workgroup).

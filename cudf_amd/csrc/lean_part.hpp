@@ -29,6 +29,7 @@
 // routines read (the first record's index).
 #pragma once
 
+#include "hot_keys.hpp"
 #include "lds_agg.hpp"
 #include "partition.hpp"
 
@@ -83,10 +84,22 @@ __device__ __forceinline__ void lean_put(uint32_t (&w)[8], uint64_t v)
   }
 }
 
+template <int W, int N>
+__device__ __forceinline__ void lean_update(uint8_t* lds, const LeanCol& c, const uint64_t (&x)[N],
+                                            const bool (&valid)[N], const uint32_t (&loff)[N], uint32_t dum);
+
 // Scatter of chunk blockIdx.x (rows [chunk * a.chunk, +a.chunk)) into
 // records. LDS (lean_scatter_lds): cursor and tile count (then tile start)
 // of every partition, 16 scan words, the destination row of every staged
-// record, the stage. K1 = 0: a single key column of K0 bytes.
+// record, the stage; with hot keys (a.hot), their lookup and records. K1 =
+// 0: a single key column of K0 bytes.
+//
+// A hot key's rows are not staged: they update the chunk's record of the
+// key (LDS, in the layout of the plan's records), and when the chunk is
+// done the records that rows reached fold into the global table. Their
+// headers are the key (one column), or the index of the key's
+// representative record (two columns: records past the partitioned ones,
+// written beforehand), as the global table's row references expect.
 template <int K0, int K1, int W0, int W1, int W2>
 __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint8_t* smem)
 {
@@ -101,7 +114,8 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
   uint32_t* cursor = reinterpret_cast<uint32_t*>(smem);
   uint32_t* tcount = cursor + P;  // counts of the tile's rows, then their starts (P + 1 words)
   uint32_t* tmp = tcount + P + 1;
-  const LeanScatterLds lay = lean_scatter_lds(P, T, RB16 * 16);
+  const bool use_hot = a.hot != nullptr;
+  const LeanScatterLds lay = lean_scatter_lds(P, T, RB16 * 16, use_hot ? a.hot_words * 8 : 0);
   uint32_t* dstrow = reinterpret_cast<uint32_t*>(smem + lay.dst);
   u32x4* stage = reinterpret_cast<u32x4*>(smem + lay.stage);
   const int64_t chunk = blockIdx.x;
@@ -114,6 +128,39 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
   const int64_t end = begin + a.chunk < a.rows ? begin + a.chunk : a.rows;
   if (begin >= end) return;  // the whole workgroup
   global_t<u32x4>* out = to_global(reinterpret_cast<u32x4*>(a.out));
+
+  HotLds hl{nullptr, nullptr, 0};
+  LdsTable ht{};
+  uint32_t* touched = nullptr;  // hot records that rows reached (kHotMax bits)
+  uint32_t hdum = 0;            // the thread's dummy word for hot updates
+  if (use_hot) {
+    hl = hot_load(a.hot, reinterpret_cast<HotKey*>(smem + lay.hot_keys), reinterpret_cast<uint32_t*>(smem + lay.hot_idx),
+                  K0, K1);
+    int32_t* ctr = reinterpret_cast<int32_t*>(smem + lay.hot_ctr);  // count, overflow, 2 fold words
+    touched = reinterpret_cast<uint32_t*>(ctr + 4);
+    hdum = (uint32_t)(lay.hot_ctr + 48) + (threadIdx.x & 63u) * 8u;
+    ht.rec = smem + lay.hot_rec;
+    ht.slots = hl.n;
+    ht.stride = a.hot_words * 8;
+    ht.cap = kHotMax;
+    ht.count = ctr;
+    ht.overflow = ctr + 1;
+    const int words = a.hot_words;
+    uint64_t* w64 = reinterpret_cast<uint64_t*>(ht.rec);
+    for (int i = threadIdx.x; i < (hl.n + 1) * words; i += kLdsBlock) {
+      const int r = i / words, w = i - r * words;
+      uint64_t v = a.tmpl[w];
+      if (w == 0) {
+        if (r == hl.n) v = 0ull;  // the table's special record: never occupied
+        else if constexpr (K1 != 0) v = (uint64_t)(a.rep_base + r);
+        else v = (K0 == 4 && a.key_signed) ? (uint64_t)(int64_t)(int32_t)(uint32_t)hl.keys[r].c0 : hl.keys[r].c0;
+      }
+      w64[i] = v;
+    }
+    if (threadIdx.x < 4) ctr[threadIdx.x] = 0;
+    if (threadIdx.x < kHotMax / 32) touched[threadIdx.x] = 0u;
+    __syncthreads();
+  }
 
   // the thread's rows of one tile, as loaded (rows past the end load the
   // last row and are not staged)
@@ -150,22 +197,66 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
   fetch(begin);
   for (int64_t tile = begin; tile < end; tile += T) {
     bool present[I];
-    uint32_t part[I], rank[I], hf[I];
+    uint32_t part[I], rank[I], hf[I], bitk[I], hk[I];
+    int hidx[I];
 #pragma unroll
     for (int k = 0; k < I; ++k) {
       const int64_t r = tile + threadIdx.x + (int64_t)k * kLdsBlock;
       const uint32_t bit = (uint32_t)r & 31u;
+      bitk[k] = bit;
       present[k] = k < items && r < end && ((km[k] >> bit) & 1u);
       // cudf's row hash: hash_combine over the key columns from seed 0
       uint32_t h = hash_combine(0u, murmur3_bits(key[k], K0, 0));
       if constexpr (K1 != 0) h = hash_combine(h, murmur3_bits(key1[k], K1, 0));
+      hk[k] = h;
       uint32_t f = 0;
       if constexpr (W0 != 0) f |= fb0 < 8 ? ((m0[k] >> bit) & 1u) << fb0 : 0u;
       if constexpr (W1 != 0) f |= fb1 < 8 ? ((m1[k] >> bit) & 1u) << fb1 : 0u;
       if constexpr (W2 != 0) f |= fb2 < 8 ? ((m2[k] >> bit) & 1u) << fb2 : 0u;
       hf[k] = (h & 0xffffff00u) | f;
       part[k] = partition_of(h, a.log2p);
-      rank[k] = present[k] ? atomicAdd(&tcount[part[k]], 1u) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < I; ++k) hidx[k] = -1;
+    if (use_hot) {
+      uint64_t c1[I];
+#pragma unroll
+      for (int k = 0; k < I; ++k) c1[k] = K1 != 0 ? key1[k] : 0ull;
+      hot_find_n<I>(hl, hk, key, c1, present, hidx);
+#pragma unroll
+      for (int k = 0; k < I; ++k) present[k] = present[k] && hidx[k] < 0;
+    }
+#pragma unroll
+    for (int k = 0; k < I; ++k) rank[k] = present[k] ? atomicAdd(&tcount[part[k]], 1u) : 0u;
+    if (use_hot) {
+      // hot rows into their records
+      uint32_t hoff[I];
+#pragma unroll
+      for (int k = 0; k < I; ++k) {
+        hoff[k] = hidx[k] >= 0 ? (uint32_t)(lay.hot_rec + (size_t)hidx[k] * ht.stride) : kRecNone;
+        if (hidx[k] >= 0) atomicOr(&touched[hidx[k] >> 5], 1u << (hidx[k] & 31));
+      }
+      if (a.rows_off >= 0) {
+#pragma unroll
+        for (int k = 0; k < I; ++k)
+          lds_add(reinterpret_cast<uint32_t*>(smem + (hoff[k] < kRecSpill ? hoff[k] + a.rows_off : hdum)), 1u);
+      }
+      bool valid[I];
+      if constexpr (W0 != 0) {
+#pragma unroll
+        for (int k = 0; k < I; ++k) valid[k] = (m0[k] >> bitk[k]) & 1u;
+        lean_update<W0, I>(smem, a.cols[0], v0, valid, hoff, hdum);
+      }
+      if constexpr (W1 != 0) {
+#pragma unroll
+        for (int k = 0; k < I; ++k) valid[k] = (m1[k] >> bitk[k]) & 1u;
+        lean_update<W1, I>(smem, a.cols[1], v1, valid, hoff, hdum);
+      }
+      if constexpr (W2 != 0) {
+#pragma unroll
+        for (int k = 0; k < I; ++k) valid[k] = (m2[k] >> bitk[k]) & 1u;
+        lean_update<W2, I>(smem, a.cols[2], v2, valid, hoff, hdum);
+      }
     }
     __syncthreads();
     const uint32_t tile_rows = lean_scan_counts(tcount, P, tmp);
@@ -197,6 +288,18 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
     __syncthreads();
     for (int i = threadIdx.x; i < P; i += kLdsBlock) tcount[i] = 0;
     __syncthreads();
+  }
+  if (use_hot) {
+    // the records that rows reached fold into the global table
+    for (int i = threadIdx.x; i < hl.n; i += kLdsBlock)
+      if (!((touched[i >> 5] >> (i & 31)) & 1u))
+        *reinterpret_cast<uint64_t*>(ht.rec + (int64_t)i * ht.stride) = K1 != 0 ? kEmptyRowRef : kEmptyIntKey;
+    if (threadIdx.x == 0) {
+      int n = 0;
+      for (int j = 0; j < kHotMax / 32; ++j) n += __popc(touched[j]);
+      ht.count[0] = n;
+    }
+    lds_fold_to_global(a.lp->src, ht);
   }
 }
 
@@ -328,24 +431,18 @@ __device__ __forceinline__ uint32_t lean_word(const u32x4 (&r)[RB16])
   return r[D / 4][D % 4];
 }
 
-// SUM / MIN / MAX and the null count of one value column (W bytes at OFF)
-// over N records. The column's fields and class are uniform: every branch
-// on them is scalar. Rows without a record or a valid value update the
-// thread's dummy word `dum` instead.
-template <int W, int OFF, int N, int RB16>
-__device__ __forceinline__ void lean_column(uint8_t* lds, const LeanCol& c, const u32x4 (&rec)[N][RB16],
-                                            const uint32_t (&loff)[N], const uint32_t (&hf)[N], uint32_t dum)
+// SUM / MIN / MAX and the null count of one value column of W bytes over N
+// rows: values x (zero-extended), their validity, the byte offsets of their
+// LDS records (kRecSpill / kRecNone: none). The column's fields and class
+// are uniform: every branch on them is scalar. Rows without a record or a
+// valid value update the thread's dummy word `dum` instead.
+template <int W, int N>
+__device__ __forceinline__ void lean_update(uint8_t* lds, const LeanCol& c, const uint64_t (&x)[N],
+                                            const bool (&valid)[N], const uint32_t (&loff)[N], uint32_t dum)
 {
-  uint64_t x[N];
-#pragma unroll
-  for (int u = 0; u < N; ++u) {
-    const uint32_t lo = lean_word<OFF / 4, RB16>(rec[u]);
-    if constexpr (W == 8) x[u] = (uint64_t)lo | (uint64_t)lean_word<OFF / 4 + 1, RB16>(rec[u]) << 32;
-    else x[u] = lo;
-  }
   bool ok[N];
 #pragma unroll
-  for (int u = 0; u < N; ++u) ok[u] = loff[u] < kRecSpill && (c.flag_bit < 0 || ((hf[u] >> c.flag_bit) & 1u));
+  for (int u = 0; u < N; ++u) ok[u] = loff[u] < kRecSpill && valid[u];
   if (c.nulls_off >= 0) {
 #pragma unroll
     for (int u = 0; u < N; ++u)
@@ -450,6 +547,24 @@ __device__ __forceinline__ void lean_column(uint8_t* lds, const LeanCol& c, cons
       }
     }
   }
+}
+
+// lean_update of the value column at byte OFF of N records (RB16 16-byte
+// units each), its validity from bit c.flag_bit of the records' hash words.
+template <int W, int OFF, int N, int RB16>
+__device__ __forceinline__ void lean_column(uint8_t* lds, const LeanCol& c, const u32x4 (&rec)[N][RB16],
+                                            const uint32_t (&loff)[N], const uint32_t (&hf)[N], uint32_t dum)
+{
+  uint64_t x[N];
+  bool valid[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    const uint32_t lo = lean_word<OFF / 4, RB16>(rec[u]);
+    if constexpr (W == 8) x[u] = (uint64_t)lo | (uint64_t)lean_word<OFF / 4 + 1, RB16>(rec[u]) << 32;
+    else x[u] = lo;
+    valid[u] = c.flag_bit < 0 || ((hf[u] >> c.flag_bit) & 1u);
+  }
+  lean_update<W, N>(lds, c, x, valid, loff, dum);
 }
 
 // Aggregates the records [begin, end) into the LDS table. Rows whose group

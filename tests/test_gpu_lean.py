@@ -22,6 +22,7 @@ def partitioned(monkeypatch):
     monkeypatch.setenv("MGB_GROUPBY_PATH", "partitioned")
     monkeypatch.delenv("MGB_LEAN", raising=False)
     monkeypatch.delenv("MGB_ESTIMATE_SCALE", raising=False)
+    monkeypatch.delenv("MGB_HOT_KEYS", raising=False)
 
 
 # (key type, value types): records of 16 bytes (4-byte key + hash + 8 bytes
@@ -266,6 +267,79 @@ def test_two_column_same_result_as_general_records(gpu, monkeypatch):
     b = sorted_result(*gpu_groupby(keys, reqs))
     for (ka, _), (kb, _) in zip(a[0], b[0]):
         assert np.array_equal(ka, kb)
+    for blk_a, blk_b in zip(a[1], b[1]):
+        for (va, ma), (vb, mb) in zip(blk_a, blk_b):
+            assert np.array_equal(va, vb)
+            assert (ma is None and mb is None) or np.array_equal(ma, mb)
+
+
+# ---- hot keys: keys seen 16 times or more in the estimate's sample skip the
+# partitioned records and are aggregated by the scatter
+
+def _skewed_keys(rng, n, pool, hot_frac, n_hot):
+    """Keys from `pool`: hot_frac of the rows on its first n_hot keys (a few
+    of them heavier), the rest uniform over the pool."""
+    hot = rng.random(n) < hot_frac
+    w = 1.0 / np.arange(1, n_hot + 1) ** 0.7
+    idx = np.where(hot, rng.choice(n_hot, n, p=w / w.sum()), rng.integers(0, len(pool), n))
+    return pool[idx]
+
+
+@pytest.mark.parametrize("kt", [O.INT64, O.INT32, O.UINT32])
+def test_hot_keys_single_column(gpu, kt):
+    rng = np.random.default_rng(620 + kt)
+    n = 2_000_000
+    info = np.iinfo(O.NP[kt])
+    pool = rng.integers(max(info.min, -2 ** 62), min(info.max, 2 ** 62), 400_000, dtype=np.int64).astype(O.NP[kt])
+    keys = _skewed_keys(rng, n, pool, 0.6, 120)
+    v = rng.standard_normal(n)
+    v[rng.random(n) < 0.02] = -0.0
+    i32 = rng.integers(-1000, 1000, n).astype(np.int32)
+    reqs = [((O.FLOAT64, v, rng.random(n) > 0.05), KINDS), ((O.INT32, i32, rng.random(n) > 0.3), KINDS)]
+    _compare([(kt, keys, None)], reqs)
+
+
+@pytest.mark.parametrize("types", [(O.INT32, O.INT64), (O.INT64, O.INT64), (O.INT32, O.INT32)])
+def test_hot_keys_two_columns(gpu, types):
+    rng = np.random.default_rng(630 + sum(types))
+    n = 2_000_000
+    card = 300_000
+    cols = [rng.integers(np.iinfo(O.NP[t]).min // 2, np.iinfo(O.NP[t]).max // 2, card, dtype=np.int64).astype(O.NP[t])
+            for t in types]
+    hot = rng.random(n) < 0.5
+    w = 1.0 / np.arange(1, 201) ** 0.8
+    idx = np.where(hot, rng.choice(200, n, p=w / w.sum()), rng.integers(0, card, n))
+    keys = [(types[0], cols[0][idx], None), (types[1], cols[1][idx], None)]
+    reqs = [((O.FLOAT64, rng.standard_normal(n), None), [O.SUM, O.COUNT_VALID, O.MAX, O.MIN]),
+            ((O.INT64, rng.integers(-10 ** 9, 10 ** 9, n), rng.random(n) > 0.1), [O.SUM, O.MIN, O.COUNT_ALL])]
+    _compare(keys, reqs)
+
+
+def test_hot_keys_more_than_the_cap_and_null_keys(gpu):
+    """1000 keys each about 0.05% of the rows: more hot candidates than the
+    256 taken (the rest are partitioned as usual); null keys dropped."""
+    rng = np.random.default_rng(640)
+    n = 3_000_000
+    pool = rng.integers(-2 ** 40, 2 ** 40, 500_000)
+    keys = _skewed_keys(rng, n, pool, 0.5, 1000)
+    kvalid = rng.random(n) > 0.05
+    _, reqs = _random_case(rng, n, O.INT64, 10, [O.INT32, O.FLOAT32], 0.05, KINDS)
+    _compare([(O.INT64, keys, kvalid)], reqs)
+
+
+def test_hot_keys_same_result_as_without(gpu, monkeypatch):
+    from tests.helpers import gpu_groupby, sorted_result
+
+    rng = np.random.default_rng(650)
+    n = 2_000_000
+    pool = rng.integers(-2 ** 50, 2 ** 50, 300_000)
+    keys = [(O.INT64, _skewed_keys(rng, n, pool, 0.7, 64), None)]
+    _, reqs = _random_case(rng, n, O.INT32, 10, [O.INT32, O.INT64], 0.05,
+                           [O.SUM, O.COUNT_VALID, O.MIN, O.MAX, O.COUNT_ALL])
+    a = sorted_result(*gpu_groupby(keys, reqs))
+    monkeypatch.setenv("MGB_HOT_KEYS", "off")
+    b = sorted_result(*gpu_groupby(keys, reqs))
+    assert np.array_equal(a[0][0][0], b[0][0][0])
     for blk_a, blk_b in zip(a[1], b[1]):
         for (va, ma), (vb, mb) in zip(blk_a, blk_b):
             assert np.array_equal(va, vb)
