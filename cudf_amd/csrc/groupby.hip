@@ -2750,6 +2750,7 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
 // MGB_LEAN=off turns the execution off.
 struct LeanPlan {
   bool on = false;
+  bool direct = false;  // the scatter writes rows straight out (partitions far outnumber a tile's rows)
   int k0 = 0, k1 = 0;  // key column widths (k1: a second key column, else 0)
   int kb = 0;          // key bytes of a record
   int tail = 0;        // two-column keys: LDS tail bytes of a record (lean_tail_bytes)
@@ -2811,6 +2812,7 @@ bool lean_shape_of(const Plan& plan, const AggParams& src, LeanPlan& lp)
 void lean_tile(LeanPlan& lp, int log2p, int hot_stride)
 {
   lp.on = false;
+  lp.direct = false;
   lp.tile = 0;
   if (lp.rec_bytes == 0 || log2p > kLeanMaxLog2P) return;
   const int P = 1 << log2p;
@@ -2821,6 +2823,17 @@ void lean_tile(LeanPlan& lp, int log2p, int hot_stride)
     }
   lp.lds = lean_scatter_lds(P, lp.tile, lp.rec_bytes, hot_stride).total;
   lp.on = lp.tile > 0;
+  // a tile of fewer than two rows per partition makes no runs to copy out
+  // together, while every tile scans all partitions: rows go straight out
+  // (MGB_LEAN_DIRECT=on / off: always / never)
+  const char* e = std::getenv("MGB_LEAN_DIRECT");
+  const std::string mode = e ? e : "";
+  lp.direct = mode == "on" || (mode != "off" && (!lp.on || 2 * lp.tile <= P));
+  if (lp.direct) {
+    lp.tile = kScatterItems * kLdsBlock;
+    lp.lds = lean_scatter_lds(P, 0, lp.rec_bytes, hot_stride).total;
+    lp.on = lp.lds <= lds_limit();
+  }
 }
 
 // The representative records of two-column hot keys (records rep_base + i):
@@ -3155,6 +3168,7 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
       sa.nblocks = (int32_t)nw;
       sa.tile_rows = lean.tile;
       sa.flag_bits = lean.flag_bits;
+      sa.direct = lean.direct ? 1 : 0;
       if (use_hot) {
         sa.hot = static_cast<const HotSet*>(ws->hot.ptr);
         sa.lp = d_lp;

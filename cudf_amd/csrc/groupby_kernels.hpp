@@ -252,6 +252,8 @@ struct LeanScatterArgs {
   int32_t tile_rows;                // rows staged per tile: a multiple of 1024, at most 4096
                                     // (partitions: at most 2^kLeanMaxLog2P)
   int32_t flag_bits;                // bits 4c..4c+3: column c's validity bit in the hash word (15: never null)
+  int32_t direct;                   // rows written straight to their records (no staging; tile_rows: rows
+                                    // per step, no stage in LDS)
   // hot keys (hot == nullptr: none)
   const HotSet* hot;
   const LdsParams* lp;              // the global table (device memory), for the hot records' fold

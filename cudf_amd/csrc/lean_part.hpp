@@ -100,7 +100,7 @@ __device__ __forceinline__ void lean_update(uint8_t* lds, const LeanCol& c, cons
 // headers are the key (one column), or the index of the key's
 // representative record (two columns: records past the partitioned ones,
 // written beforehand), as the global table's row references expect.
-template <int K0, int K1, int W0, int W1, int W2>
+template <int K0, int K1, int W0, int W1, int W2, bool HOT, bool DIRECT>
 __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint8_t* smem)
 {
   constexpr LeanKeyOffsets KO = lean_key_offsets(K0, K1);
@@ -114,8 +114,8 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
   uint32_t* cursor = reinterpret_cast<uint32_t*>(smem);
   uint32_t* tcount = cursor + P;  // counts of the tile's rows, then their starts (P + 1 words)
   uint32_t* tmp = tcount + P + 1;
-  const bool use_hot = a.hot != nullptr;
-  const LeanScatterLds lay = lean_scatter_lds(P, T, RB16 * 16, use_hot ? a.hot_words * 8 : 0);
+  constexpr bool use_hot = HOT;  // a.hot set (the variant compiled for calls with hot keys)
+  const LeanScatterLds lay = lean_scatter_lds(P, DIRECT ? 0 : T, RB16 * 16, use_hot ? a.hot_words * 8 : 0);
   uint32_t* dstrow = reinterpret_cast<uint32_t*>(smem + lay.dst);
   u32x4* stage = reinterpret_cast<u32x4*>(smem + lay.stage);
   const int64_t chunk = blockIdx.x;
@@ -133,7 +133,7 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
   LdsTable ht{};
   uint32_t* touched = nullptr;  // hot records that rows reached (kHotMax bits)
   uint32_t hdum = 0;            // the thread's dummy word for hot updates
-  if (use_hot) {
+  if constexpr (use_hot) {
     hl = hot_load(a.hot, reinterpret_cast<HotKey*>(smem + lay.hot_keys), reinterpret_cast<uint32_t*>(smem + lay.hot_idx),
                   K0, K1);
     int32_t* ctr = reinterpret_cast<int32_t*>(smem + lay.hot_ctr);  // count, overflow, 2 fold words
@@ -218,7 +218,7 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
     }
 #pragma unroll
     for (int k = 0; k < I; ++k) hidx[k] = -1;
-    if (use_hot) {
+    if constexpr (use_hot) {
       uint64_t c1[I];
 #pragma unroll
       for (int k = 0; k < I; ++k) c1[k] = K1 != 0 ? key1[k] : 0ull;
@@ -226,9 +226,16 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
 #pragma unroll
       for (int k = 0; k < I; ++k) present[k] = present[k] && hidx[k] < 0;
     }
+    if constexpr (DIRECT) {
+      // many more partitions than rows a tile holds: each row takes its
+      // destination from its partition's cursor and is written at once
 #pragma unroll
-    for (int k = 0; k < I; ++k) rank[k] = present[k] ? atomicAdd(&tcount[part[k]], 1u) : 0u;
-    if (use_hot) {
+      for (int k = 0; k < I; ++k) rank[k] = present[k] ? atomicAdd(&cursor[part[k]], 1u) : 0u;
+    } else {
+#pragma unroll
+      for (int k = 0; k < I; ++k) rank[k] = present[k] ? atomicAdd(&tcount[part[k]], 1u) : 0u;
+    }
+    if constexpr (use_hot) {
       // hot rows into their records
       uint32_t hoff[I];
 #pragma unroll
@@ -257,6 +264,23 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
         for (int k = 0; k < I; ++k) valid[k] = (m2[k] >> bitk[k]) & 1u;
         lean_update<W2, I>(smem, a.cols[2], v2, valid, hoff, hdum);
       }
+    }
+    if constexpr (DIRECT) {
+#pragma unroll
+      for (int k = 0; k < I; ++k) {
+        if (!present[k]) continue;
+        uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+        lean_put<K0, KO.off0>(w, key[k]);
+        if constexpr (K1 != 0) lean_put<K1, KO.off1>(w, key1[k]);
+        w[KB / 4] = hf[k];
+        if constexpr (W0 != 0) lean_put<W0, LO.off[0]>(w, v0[k]);
+        if constexpr (W1 != 0) lean_put<W1, LO.off[1]>(w, v1[k]);
+        if constexpr (W2 != 0) lean_put<W2, LO.off[2]>(w, v2[k]);
+        out[(uint64_t)rank[k] * RB16] = u32x4{w[0], w[1], w[2], w[3]};
+        if constexpr (RB16 == 2) out[(uint64_t)rank[k] * 2 + 1] = u32x4{w[4], w[5], w[6], w[7]};
+      }
+      if (tile + T < end) fetch(tile + T);
+      continue;
     }
     __syncthreads();
     const uint32_t tile_rows = lean_scan_counts(tcount, P, tmp);
@@ -289,7 +313,8 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
     for (int i = threadIdx.x; i < P; i += kLdsBlock) tcount[i] = 0;
     __syncthreads();
   }
-  if (use_hot) {
+  if constexpr (use_hot) {
+    __syncthreads();  // every hot update done (the direct mode has no barrier per tile)
     // the records that rows reached fold into the global table
     for (int i = threadIdx.x; i < hl.n; i += kLdsBlock)
       if (!((touched[i >> 5] >> (i & 31)) & 1u))
