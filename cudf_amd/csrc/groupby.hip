@@ -68,19 +68,42 @@ __global__ __launch_bounds__(kBlock) void fill_table_kernel(const FillParams p)
 }
 
 // Per-row insert-or-find and atomic update in the global table.
+// New groups are counted per workgroup in LDS and added to the table's
+// group counter every 32 rows of a thread and at the end, not at every
+// insert (one counter taken by every insert of the launch is one L2
+// channel's traffic, which stalls every access that maps there); the
+// budget is tested against the counter as last added to. With p.row_rec,
+// every row's record index (kNoRecord: none) is written for the sort path.
+__device__ __forceinline__ void add_inserted(const AggParams& p, uint32_t n)
+{
+  if (n == 0) return;
+  const unsigned long long total = atomicAdd(&p.counters[0], (unsigned long long)n) + n;
+  if (total >= (unsigned long long)p.max_groups) flag_overflow(p.counters);
+}
+
 __global__ __launch_bounds__(kBlock) void aggregate_kernel(const AggParams p)
 {
+  __shared__ uint32_t inserted;
+  if (threadIdx.x == 0) inserted = 0u;
+  __syncthreads();
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
   uint32_t iter = 0;
   for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < p.num_rows; row += step) {
-    // an overflowed attempt is repeated with a larger table: stop early
-    if ((++iter & 31u) == 0u && __hip_atomic_load(&p.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      return;
-    if (!key_row_valid(p, row)) continue;
-    const RowKey rk = row_key(p, row);
-    uint8_t* rec = global_find_or_insert(p, rk, row, false);
-    if (rec) apply_row(p, rec, row);
+    if ((++iter & 31u) == 0u) {
+      add_inserted(p, atomicExch(&inserted, 0u));
+      // an overflowed attempt is repeated with a larger table: stop early
+      if (__hip_atomic_load(&p.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    }
+    uint8_t* rec = nullptr;
+    if (key_row_valid(p, row)) {
+      const RowKey rk = row_key(p, row);
+      rec = global_find_or_insert(p, rk, row, false, &inserted, true);
+      if (rec) apply_row(p, rec, row);
+    }
+    if (p.row_rec) p.row_rec[row] = rec ? (uint32_t)((rec - p.table) / p.stride) : kNoRecord;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) add_inserted(p, inserted);
 }
 
 // Second pass over the rows (VARIANCE / STD / ARGMIN / ARGMAX fields) on
@@ -1165,6 +1188,7 @@ struct Workspace {
   DeviceBuffer sorts{&mem};   // row groups, sort keys and radix-sort space of the sort-path aggregations
   DeviceBuffer lists{&mem};   // COLLECT_LIST / COLLECT_SET results: offsets and elements
   DeviceBuffer hot{&mem};     // the call's hot keys (HotSet), when the compact execution may take it
+  DeviceBuffer rowrec{&mem};  // sort path: every row's record in the global table (aggregate_kernel)
   bool fine_valid = false;  // `fine` holds the counts of this call's keys
   bool fine_hot = false;    // ... without the rows of the hot keys in `hot`
   int64_t fine_nw = 0, fine_chunk = 0;
@@ -2514,6 +2538,7 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
                      const LdsGeom& geom, hipStream_t stream, TempOuts& temp, int64_t& groups,
                      const TinyParams* tiny = nullptr)
 {
+  bool row_rec_written = false;  // the global build wrote the sort path's row-to-record map
   const int cus = device_cu_count();
   int64_t capacity = std::max<int64_t>(64, 2 * G_est + 1024);
   capacity = std::min<int64_t>(capacity, std::max<int64_t>(2 * R + 64, 64));
@@ -2671,6 +2696,12 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
         }
         if (lp.num_vals == 0 && ci < plan.cols.size())
           return fail(MGB_ERROR_UNSUPPORTED, "a value column needs more accumulator ops than one launch holds");
+        // the sort path's row-to-record map, from the first launch
+        if (first && plan.has_sort) {
+          MGB_RESERVE(ws->rowrec, (size_t)R * 4);
+          lp.row_rec = static_cast<uint32_t*>(ws->rowrec.ptr);
+          row_rec_written = true;
+        }
         const int64_t grid = std::min<int64_t>((R + kBlock - 1) / kBlock, (int64_t)cus * 16);
         {
           ProfScope prof("aggregate", stream);
@@ -2734,6 +2765,7 @@ mgb_status run_table(Workspace* ws, const Plan& plan, const AggParams& base, int
   AggParams rows = base;
   rows.table = static_cast<uint8_t*>(ws->main.ptr);
   rows.capacity = capacity;
+  rows.row_rec = row_rec_written ? static_cast<uint32_t*>(ws->rowrec.ptr) : nullptr;
   return run_sort_aggs(ws, plan, rows, rec_pos, groups, stream, temp);
 }
 

@@ -112,8 +112,11 @@ __device__ __forceinline__ void flag_overflow(unsigned long long* counters) { at
 // instead of in the table's group counter, which the caller adds them to
 // once (a counter and flag taken by every insert of a launch are one L2
 // channel's traffic, and stall every access that maps there).
+// `budget_read`: with local_count, still test the budget (the counter as last
+// flushed) before each insert.
 __device__ __forceinline__ uint8_t* global_find_or_insert(const AggParams& p, const RowKey& rk, int64_t row,
-                                                         bool entry_check = true, uint32_t* local_count = nullptr)
+                                                         bool entry_check = true, uint32_t* local_count = nullptr,
+                                                         bool budget_read = false)
 {
   if (rk.special) {
     uint8_t* rec = p.table + p.capacity * (int64_t)p.stride;
@@ -138,8 +141,9 @@ __device__ __forceinline__ uint8_t* global_find_or_insert(const AggParams& p, co
     uint64_t cur = *hp;  // a stale read can only be EMPTY; the CAS below resolves it
     for (;;) {
       if (cur == empty) {
-        if (!local_count && __hip_atomic_load(&p.counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
-                                (unsigned long long)p.max_groups) {
+        if ((!local_count || budget_read) &&
+            __hip_atomic_load(&p.counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+              (unsigned long long)p.max_groups) {
           flag_overflow(p.counters);
           return nullptr;
         }

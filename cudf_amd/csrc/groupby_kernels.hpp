@@ -5,6 +5,8 @@
 
 namespace mgb {
 
+constexpr uint32_t kNoRecord = 0xffffffffu;
+
 // A set of rows to aggregate (the input columns, or the partitioned copy of
 // them), the accumulator ops, and the global record table.
 struct AggParams {
@@ -34,6 +36,7 @@ struct AggParams {
   uint64_t prog[kProgramCols] = {};
 
   const uint8_t* rec_base = nullptr;
+  uint32_t* row_rec = nullptr;    // global build of the sort path: each row's record index (kNoRecord: none)
   KeyCol keys[kMaxKeys];
   ValCol vals[kMaxVals];
   OpDesc ops[kMaxOps + kMaxColOps];  // the last kMaxColOps unused: a column's window of ops is read whole

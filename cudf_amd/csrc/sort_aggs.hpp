@@ -11,9 +11,19 @@
 namespace mgb {
 
 // The output row of every row's group (`groups` for rows whose key is dropped).
+// With p.row_rec (the build wrote each row's record index), the group is read
+// through it instead of probing the table again.
 __global__ __launch_bounds__(kBlock) void row_group_kernel(const AggParams p, const uint32_t* rec_pos, uint32_t groups,
                                                            uint32_t* gid)
 {
+  if (p.row_rec) {
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < p.num_rows; row += step) {
+      const uint32_t r = p.row_rec[row];
+      gid[row] = r == kNoRecord ? groups : rec_pos[r];
+    }
+    return;
+  }
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
   for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < p.num_rows; row += step) {
     uint32_t g = groups;
