@@ -192,18 +192,24 @@ __global__ __launch_bounds__(kLdsBlock) void batch_scatter_kernel(const BatchQue
 }
 
 // Rows [begin, end) into p's global table with device atomics, one row per
-// thread.
-__device__ __forceinline__ void global_rows(const AggParams& p, int64_t begin, int64_t end)
+// thread. New groups are counted in `inserted` (LDS, zero on entry and on
+// return) and added to the table's group counter every 32 rows of a thread
+// and at the end: the budget is tested against the counter as last added to.
+__device__ __forceinline__ void global_rows(const AggParams& p, int64_t begin, int64_t end, uint32_t* inserted)
 {
   uint32_t iter = 0;
   for (int64_t row = begin + threadIdx.x; row < end; row += kLdsBlock) {
-    // the overflow flag every 32 rows rather than at every probe
-    if ((++iter & 31u) == 0u && __hip_atomic_load(&p.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      return;
+    if ((++iter & 31u) == 0u) {
+      add_inserted(p, atomicExch(inserted, 0u));
+      // the overflow flag every 32 rows rather than at every probe
+      if (__hip_atomic_load(&p.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    }
     if (!key_row_valid(p, row)) continue;
-    uint8_t* g = global_find_or_insert(p, row_key(p, row), row, false);
+    uint8_t* g = global_find_or_insert(p, row_key(p, row), row, false, inserted, true);
     if (g) apply_row(p, g, row);
   }
+  __syncthreads();
+  if (threadIdx.x == 0) add_inserted(p, atomicExch(inserted, 0u));
 }
 
 // Persistent workgroups take items from the queue until it is empty.
@@ -212,6 +218,8 @@ __global__ __launch_bounds__(kLdsBlock) void batch_items_kernel(const BatchParam
 {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ int64_t next;
+  __shared__ uint32_t inserted;
+  if (threadIdx.x == 0) inserted = 0u;
   for (;;) {
     if (threadIdx.x == 0) next = (int64_t)std::min<unsigned long long>(atomicAdd(bp.queue, 1ull), 0x7fffffffull);
     __syncthreads();
@@ -222,7 +230,7 @@ __global__ __launch_bounds__(kLdsBlock) void batch_items_kernel(const BatchParam
     const BatchItem item = items[it];
     const BatchQuery& Q = queries[item.query];
     if (item.kind == BI_GLOBAL) {
-      global_rows(Q.ap, item.begin, item.end);
+      global_rows(Q.ap, item.begin, item.end, &inserted);
     } else {
       // one LDS aggregation for both kinds (a single inlined copy of the row loop)
       const bool part = item.kind == BI_PART;

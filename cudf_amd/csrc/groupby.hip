@@ -37,13 +37,13 @@
 #include <tuple>
 #include <vector>
 
-#include <hipcub/hipcub.hpp>
 
 #include "groupby_host.hpp"
 #include "hot_keys.hpp"
 #include "lds_agg.hpp"
 #include "tiny_agg.hpp"
 #include "partition.hpp"
+#include "seg_sort.hpp"
 #include "sort_aggs.hpp"
 
 namespace mgb {
@@ -74,13 +74,6 @@ __global__ __launch_bounds__(kBlock) void fill_table_kernel(const FillParams p)
 // channel's traffic, which stalls every access that maps there); the
 // budget is tested against the counter as last added to. With p.row_rec,
 // every row's record index (kNoRecord: none) is written for the sort path.
-__device__ __forceinline__ void add_inserted(const AggParams& p, uint32_t n)
-{
-  if (n == 0) return;
-  const unsigned long long total = atomicAdd(&p.counters[0], (unsigned long long)n) + n;
-  if (total >= (unsigned long long)p.max_groups) flag_overflow(p.counters);
-}
-
 __global__ __launch_bounds__(kBlock) void aggregate_kernel(const AggParams p)
 {
   __shared__ uint32_t inserted;
@@ -2140,19 +2133,21 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
   return std::max<int64_t>(1, std::min<int64_t>(R, (int64_t)std::ceil(est)));
 }
 
-// The sort-path aggregations (MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT) of a
-// call whose global table is complete and finalised: `rec_pos` gives every
-// record's output row. Every row's group is looked up once. The outputs then
-// share orderings: one per value column for MEDIAN, QUANTILE and NUNIQUE
-// (each group's valid values ascending), one per value column for
-// NTH_ELEMENT EXCLUDE and one for every NTH_ELEMENT INCLUDE (each group's
-// rows in row order). cpp/src/groupby/sort/sort_helper.cu orders the rows by
-// key the same way; the results are read off the sorted rows.
+// The sort-path aggregations (MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT,
+// COLLECT_LIST, COLLECT_SET) of a call whose global table is complete and
+// finalised: `rec_pos` gives every record's output row. Every row's group is
+// looked up once. The outputs then share orderings: one per value column for
+// MEDIAN, QUANTILE, NUNIQUE and COLLECT_SET (each group's valid values
+// ascending), one per value column for NTH_ELEMENT and COLLECT_LIST EXCLUDE
+// and one for every INCLUDE of them (each group's rows in row order).
+// cpp/src/groupby/sort/sort_helper.cu orders the rows by key the same way;
+// the results are read off the sorted positions.
 //
-// Value orderings of columns of at most 4 bytes are one keys-only radix sort
-// of (group << 32 | value code) over 32 + log2(groups) bits; 8-byte columns
-// sort by the 64-bit code and then stably by group, with the code as payload.
-// Results are decoded from the sorted codes, so no value is gathered.
+// An ordering is a segmented sort (seg_sort.hpp): each group's keys (32-bit
+// value codes for columns of at most 4 bytes, 64-bit codes for 8-byte
+// columns, row indices for the row orderings) bucketed into the group's
+// segment and sorted there. Results are decoded from the sorted codes, so no
+// value is gathered.
 mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows, const uint32_t* rec_pos,
                          int64_t groups, hipStream_t stream, TempOuts& temp)
 {
@@ -2160,10 +2155,7 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
   if (groups == 0 || R == 0) return MGB_SUCCESS;
   if (R > (int64_t)0x7fffffff || groups >= (int64_t)0xffffffff)
     return fail(MGB_ERROR_UNSUPPORTED, "sort-path aggregations of more than 2^31 rows");
-  const int n = (int)R;
   const uint32_t G = (uint32_t)groups;
-  int gbits = 1;
-  while (gbits < 32 && (1ull << gbits) <= (unsigned long long)G) ++gbits;
 
   // the orderings the outputs need, in order of first use
   struct Ordering { int col; int mode; };
@@ -2181,22 +2173,20 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
     out_ordering[o] = k;
   }
 
-  // radix-sort and scan space: the largest of the uses
-  size_t tk = 0, t64 = 0, t64b = 0, t32 = 0, tscan = 0;
-  MGB_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tk, (const uint64_t*)nullptr, (uint64_t*)nullptr, n, 0,
-                                            32 + gbits, stream));
-  MGB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t64, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, 64, stream));
-  MGB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t64b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (const uint64_t*)nullptr, (uint64_t*)nullptr, n, 0, gbits, stream));
-  MGB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t32, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, gbits, stream));
-  MGB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tscan, (const uint32_t*)nullptr, (uint32_t*)nullptr, n, stream));
-  const size_t tcub = align_up(std::max({tk, t64, t64b, t32, tscan}), 256);
-  const size_t r4 = align_up((size_t)R * 4, 256), r8 = align_up((size_t)R * 8, 256), g4 = align_up((size_t)G * 4, 256);
-  const size_t o4 = align_up(((size_t)G + 1) * 4, 256);
-  // gid | A, B (8 bytes a row) | C, D, E (4 bytes a row) | offset, has_null, group nulls, list lengths | cub
-  const size_t need = 4 * r4 + 2 * r8 + o4 + 3 * g4 + tcub;
+  // (R + 1)-entry u32 buffers: rank and group buffers, the distinct-value
+  // flags and their exclusive sums
+  const size_t r4 = align_up((size_t)(R + 1) * 4, 256), r8 = align_up((size_t)R * 8, 256);
+  const size_t g4 = align_up((size_t)G * 4, 256), o4 = align_up(((size_t)G + 1) * 4, 256);
+  const int64_t tiles = (R + kRadixTile - 1) / kRadixTile;
+  const size_t count_bytes = align_up((size_t)kRadixBins * (size_t)tiles * 4, 256);
+  const int64_t scan_n = std::max<int64_t>({R, (int64_t)G, (int64_t)kRadixBins * tiles}) + 1;
+  const int64_t scan_blocks = (scan_n + kScanTile - 1) / kScanTile;
+  const size_t scan_bytes = align_up((size_t)scan_blocks * 4, 256) + 256;
+  const size_t list_bytes_seg = align_up(((size_t)R / kWaveSeg + 2) * 8, 256) + 256;
+  // gid | A, B (8 bytes a row) | C, D, E (4 bytes a row) | offset, has_null,
+  // group nulls, distinct values, list lengths | scan block sums |
+  // long-segment list | radix counts
+  const size_t need = 4 * r4 + 2 * r8 + 2 * o4 + 3 * g4 + scan_bytes + list_bytes_seg + count_bytes;
   MGB_RESERVE(ws->sorts, need);
   uint8_t* b = static_cast<uint8_t*>(ws->sorts.ptr);
   size_t off = 0;
@@ -2210,8 +2200,23 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
   uint32_t* offset = reinterpret_cast<uint32_t*>(take(o4));
   uint32_t* has_null = reinterpret_cast<uint32_t*>(take(g4));
   uint32_t* gnulls = reinterpret_cast<uint32_t*>(take(g4));
-  uint32_t* lens = reinterpret_cast<uint32_t*>(take(g4));
-  void* cub_tmp = take(tcub);
+  uint32_t* distinct = reinterpret_cast<uint32_t*>(take(g4));
+  uint32_t* lens = reinterpret_cast<uint32_t*>(take(o4));
+  uint32_t* bsum = reinterpret_cast<uint32_t*>(take(scan_bytes));
+  unsigned long long* scan_total = reinterpret_cast<unsigned long long*>(
+    reinterpret_cast<uint8_t*>(bsum) + align_up((size_t)scan_blocks * 4, 256));
+  unsigned long long* seg_list = reinterpret_cast<unsigned long long*>(take(list_bytes_seg));
+  uint32_t* seg_count = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(seg_list) + list_bytes_seg - 256);
+  uint32_t* radix_counts = reinterpret_cast<uint32_t*>(take(count_bytes));
+  // exclusive sum of in[0, n) into out (in place allowed)
+  auto excl_scan = [&](const uint32_t* in, uint32_t* out, int64_t n) -> mgb_status {
+    const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    scan_reduce_u32_kernel<<<(unsigned)nb, 256, 0, stream>>>(in, n, bsum);
+    scan_bsums_u32_kernel<<<1, 1024, 0, stream>>>(bsum, nb, scan_total);
+    scan_apply_u32_kernel<<<(unsigned)nb, 256, 0, stream>>>(in, n, bsum, out);
+    MGB_HIP(hipGetLastError());
+    return MGB_SUCCESS;
+  };
 
   // COLLECT_LIST / COLLECT_SET results: offsets, elements and their validity
   // bytes per list output, and two counters (elements, null elements)
@@ -2252,19 +2257,23 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
   ProfScope prof("sort_aggs", stream);
   row_group_kernel<<<grid_rows, kBlock, 0, stream>>>(rows, rec_pos, G, gid);
   MGB_HIP(hipGetLastError());
+  const unsigned grid_small = (unsigned)std::max<int64_t>(1, std::min<int64_t>(((int64_t)G + 3) / 4, (int64_t)cus * 32));
+  const unsigned grid_long = (unsigned)std::max<int64_t>(1, std::min<int64_t>(R / kWaveSeg + 1, (int64_t)cus * 4));
+  const int max_rounds = seg_rounds((uint64_t)R);
+  // radix passes over the group keys 0..G (G: rows that take no part)
+  int gbits = 1;
+  while (gbits < 32 && (1ull << gbits) <= (unsigned long long)G) ++gbits;
+  const int passes = (gbits + 7) / 8;
+  const int pass_bits = (gbits + passes - 1) / passes;
   for (int k = 0; k < (int)orderings.size(); ++k) {
     const Ordering ord = orderings[k];
-    SortPrepParams sp{};
+    SegParams sp{};
     if (ord.col >= 0) sp.v = plan.cols[ord.col].col;
     sp.num_rows = R;
     sp.groups = G;
     sp.mode = ord.mode;
     sp.narrow = ord.mode == SORT_BY_VALUE && sp.v.width <= 4;
-    sp.gid = gid;
-    sp.key64 = reinterpret_cast<uint64_t*>(A);
-    sp.code = reinterpret_cast<uint64_t*>(A);
-    sp.gkey = C;
-    sp.idx = reinterpret_cast<uint32_t*>(A);
+    const bool wide = ord.mode == SORT_BY_VALUE && !sp.narrow;
     bool want_nulls = false;
     for (size_t o = 0; o < plan.outs.size(); ++o)
       if (out_ordering[o] == k && plan.outs[o].kind == MGB_AGG_NUNIQUE &&
@@ -2274,37 +2283,85 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
       MGB_HIP(hipMemsetAsync(has_null, 0, (size_t)G * 4, stream));
       sp.has_null = has_null;
     }
-    sort_prepare_kernel<<<grid_rows, kBlock, 0, stream>>>(sp);
+    sp.gid = gid;
+    // group keys C <-> E; payloads: 64-bit codes A <-> B, 32-bit keys D <-> B
+    // (the first pass reads the rows themselves and writes gk_b / pl_b)
+    uint32_t* gk_a = E;
+    uint32_t* gk_b = C;
+    void* pl_a = B;
+    void* pl_b = wide ? static_cast<void*>(A) : static_cast<void*>(D);
+    for (int ps = 0; ps < passes; ++ps) {
+      RadixPass rp{};
+      rp.n = R;
+      rp.tiles = tiles;
+      rp.shift = ps * pass_bits;
+      rp.bits = std::min(pass_bits, gbits - ps * pass_bits);
+      rp.gk_in = gk_a;
+      rp.gk_out = gk_b;
+      rp.pl_in = pl_a;
+      rp.pl_out = pl_b;
+      rp.counts = radix_counts;
+      if (ps == 0) radix_hist_kernel<true><<<(unsigned)tiles, kRadixBlock, 0, stream>>>(rp, sp);
+      else radix_hist_kernel<false><<<(unsigned)tiles, kRadixBlock, 0, stream>>>(rp, sp);
+      MGB_HIP(hipGetLastError());
+      const mgb_status st = excl_scan(radix_counts, radix_counts, ((int64_t)1 << rp.bits) * tiles);
+      if (st != MGB_SUCCESS) return st;
+      if (wide) {
+        if (ps == 0) radix_scatter_kernel<uint64_t, true><<<(unsigned)tiles, kRadixBlock, 0, stream>>>(rp, sp);
+        else radix_scatter_kernel<uint64_t, false><<<(unsigned)tiles, kRadixBlock, 0, stream>>>(rp, sp);
+      } else {
+        if (ps == 0) radix_scatter_kernel<uint32_t, true><<<(unsigned)tiles, kRadixBlock, 0, stream>>>(rp, sp);
+        else radix_scatter_kernel<uint32_t, false><<<(unsigned)tiles, kRadixBlock, 0, stream>>>(rp, sp);
+      }
+      MGB_HIP(hipGetLastError());
+      std::swap(gk_a, gk_b);
+      std::swap(pl_a, pl_b);
+    }
+    seg_bounds_kernel<<<grid_rows, kBlock, 0, stream>>>(gk_a, R, G, offset);
     MGB_HIP(hipGetLastError());
+    if (ord.mode == SORT_BY_VALUE) {
+      // each group's codes sorted in place; narrow results packed into A
+      sp.off = offset;
+      sp.keys = pl_a;
+      sp.alt = pl_b;
+      sp.key64 = sp.narrow ? reinterpret_cast<uint64_t*>(A) : nullptr;
+      sp.gkey = nullptr;  // the sorted group keys are in place already
+      sp.list = seg_list;
+      sp.list_count = seg_count;
+      MGB_HIP(hipMemsetAsync(seg_count, 0, 4, stream));
+      if (wide) {
+        seg_sort_small_kernel<uint64_t><<<grid_small, 256, 0, stream>>>(sp);
+        seg_sort_block_kernel<uint64_t><<<grid_long, kSegBlock, 0, stream>>>(sp);
+        for (int r = 0; r < max_rounds; ++r) seg_merge_kernel<uint64_t><<<grid_long, kSegBlock, 0, stream>>>(sp, r);
+        if (max_rounds) seg_emit_long_kernel<uint64_t><<<grid_long, kSegBlock, 0, stream>>>(sp);
+      } else {
+        seg_sort_small_kernel<uint32_t><<<grid_small, 256, 0, stream>>>(sp);
+        seg_sort_block_kernel<uint32_t><<<grid_long, kSegBlock, 0, stream>>>(sp);
+        for (int r = 0; r < max_rounds; ++r) seg_merge_kernel<uint32_t><<<grid_long, kSegBlock, 0, stream>>>(sp, r);
+        if (max_rounds) seg_emit_long_kernel<uint32_t><<<grid_long, kSegBlock, 0, stream>>>(sp);
+      }
+      MGB_HIP(hipGetLastError());
+    }
     SortView view{};
     view.n = R;
     view.groups = G;
     view.narrow = sp.narrow;
-    size_t tb = tcub;
-    if (ord.mode != SORT_BY_VALUE) {
-      // by group alone (stable: each group's rows stay in row order)
-      MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, C, D, reinterpret_cast<uint32_t*>(A),
-                                                 reinterpret_cast<uint32_t*>(B), n, 0, gbits, stream));
-      view.gkey = D;
-      view.order = reinterpret_cast<const uint32_t*>(B);
-    } else if (sp.narrow) {
-      MGB_HIP(hipcub::DeviceRadixSort::SortKeys(cub_tmp, tb, reinterpret_cast<const uint64_t*>(A),
-                                                reinterpret_cast<uint64_t*>(B), n, 0, 32 + gbits, stream));
-      view.key64 = reinterpret_cast<const uint64_t*>(B);
-    } else {
-      // by value, then stably by group, the code riding along
-      MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, reinterpret_cast<const uint64_t*>(A),
-                                                 reinterpret_cast<uint64_t*>(B), C, D, n, 0, 64, stream));
-      tb = tcub;
-      MGB_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp, tb, D, E, reinterpret_cast<const uint64_t*>(B),
-                                                 reinterpret_cast<uint64_t*>(A), n, 0, gbits, stream));
-      view.gkey = E;
-      view.code = reinterpret_cast<const uint64_t*>(A);
-    }
     view.offset = offset;
-    group_bounds_kernel<<<grid_rows, kBlock, 0, stream>>>(view, offset);
-    MGB_HIP(hipGetLastError());
-    bool scanned = false;  // NUNIQUE's prefix sums (in D) for this ordering
+    view.total = offset + G;
+    if (sp.narrow) {
+      view.key64 = reinterpret_cast<const uint64_t*>(A);
+    } else if (wide) {
+      view.gkey = gk_a;
+      view.code = static_cast<const uint64_t*>(pl_a);
+    } else {
+      view.gkey = gk_a;
+      view.order = static_cast<const uint32_t*>(pl_a);
+    }
+    // COLLECT_SET's distinct-value flags and their sums (R + 1 entries each,
+    // in buffers the view does not use)
+    uint32_t* flags = gk_b;
+    uint32_t* sums = sp.narrow ? static_cast<uint32_t*>(pl_b) : D;
+    bool counted = false;  // NUNIQUE's distinct values (in `distinct`) for this ordering
     for (size_t o = 0; o < plan.outs.size(); ++o) {
       if (out_ordering[o] != k) continue;
       const Plan::Out& out = plan.outs[o];
@@ -2343,22 +2400,20 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
             list_nulls_kernel<<<grid_rows, kBlock, 0, stream>>>(p.v, gid, R, G, list_counters(*lt) + 1);
         } else {
           const int32_t is_float = p.v.vclass == VC_F32 || p.v.vclass == VC_F64;
-          distinct_start_kernel<<<grid_rows, kBlock, 0, stream>>>(view, is_float, C,
+          distinct_start_kernel<<<grid_rows, kBlock, 0, stream>>>(view, is_float, flags,
                                                                   out.agg.interpolation == MGB_NAN_UNEQUAL ? 1 : 0);
           MGB_HIP(hipGetLastError());
-          tb = tcub;
-          MGB_HIP(hipcub::DeviceScan::InclusiveSum(cub_tmp, tb, C, D, n, stream));
-          scanned = false;  // D no longer holds NUNIQUE's sums
+          mgb_status st = excl_scan(flags, sums, R + 1);
+          if (st != MGB_SUCCESS) return st;
           MGB_HIP(hipMemsetAsync(gnulls, 0, (size_t)G * 4, stream));
           if (p.v.mask) group_nulls_kernel<<<grid_rows, kBlock, 0, stream>>>(p.v, gid, R, G, gnulls);
-          cp.prefix = D;
-          cp.flag = C;
+          cp.prefix = sums;
+          cp.flag = flags;
           cp.nulls = gnulls;
-          set_lengths_kernel<<<grid_groups, kBlock, 0, stream>>>(cp);
+          set_lengths_kernel<<<(unsigned)((G + kBlock) / kBlock), kBlock, 0, stream>>>(cp, lens);
           MGB_HIP(hipGetLastError());
-          tb = tcub;
-          MGB_HIP(hipcub::DeviceScan::InclusiveSum(cub_tmp, tb, lt->offsets + 1, lens, (int)G, stream));
-          MGB_HIP(hipMemcpyAsync(lt->offsets + 1, lens, (size_t)G * 4, hipMemcpyDeviceToDevice, stream));
+          st = excl_scan(lens, lt->offsets, (int64_t)G + 1);
+          if (st != MGB_SUCCESS) return st;
           set_values_kernel<<<grid_rows, kBlock, 0, stream>>>(cp);
           set_nulls_kernel<<<grid_groups, kBlock, 0, stream>>>(cp, list_counters(*lt) + 1);
         }
@@ -2366,17 +2421,14 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
         continue;
       }
       if (out.kind == MGB_AGG_NUNIQUE) {
-        if (!scanned) {
-          // C is free once the sorts are done; D holds sorted groups only in
-          // the row orderings, which NUNIQUE does not use
+        if (!counted) {
           const int32_t is_float = p.v.vclass == VC_F32 || p.v.vclass == VC_F64;
-          distinct_start_kernel<<<grid_rows, kBlock, 0, stream>>>(view, is_float, C);
+          MGB_HIP(hipMemsetAsync(distinct, 0, (size_t)G * 4, stream));
+          nunique_count_kernel<<<grid_rows, kBlock, 0, stream>>>(view, is_float, distinct);
           MGB_HIP(hipGetLastError());
-          tb = tcub;
-          MGB_HIP(hipcub::DeviceScan::InclusiveSum(cub_tmp, tb, C, D, n, stream));
-          scanned = true;
+          counted = true;
         }
-        p.prefix = D;
+        p.distinct = distinct;
         nunique_finish_kernel<<<grid_groups, kBlock, 0, stream>>>(p);
       } else {
         sort_select_kernel<<<grid_groups, kBlock, 0, stream>>>(p);

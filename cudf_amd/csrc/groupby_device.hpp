@@ -103,6 +103,15 @@ __device__ __forceinline__ int64_t reduce_range(uint32_t h, int64_t n) { return 
 
 __device__ __forceinline__ void flag_overflow(unsigned long long* counters) { atom_max(&counters[1], 1ull); }
 
+// n inserts counted apart (in LDS) added to the table's group counter; the
+// overflow flag set if they took it to the budget.
+__device__ __forceinline__ void add_inserted(const AggParams& p, uint32_t n)
+{
+  if (n == 0) return;
+  const unsigned long long total = atomicAdd(&p.counters[0], (unsigned long long)n) + n;
+  if (total >= (unsigned long long)p.max_groups) flag_overflow(p.counters);
+}
+
 // Record of the group of (rk, row) in the global table, inserting it if
 // needed; nullptr when the table is over its group budget (overflow flag set).
 // `entry_check`: test the overflow flag first (a caller that probes every row

@@ -1,9 +1,9 @@
 // Device routines of the aggregations cudf serves from its sort-based groupby
-// (MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT; cpp/src/groupby/sort/). The hash
-// table assigns every row its group (the group's output row); radix sorts
-// (hipCUB) then order each value column's rows by group and, for the
-// value-ordered aggregations, by value within the group; small kernels read
-// the results off the sorted rows.
+// (MEDIAN, QUANTILE, NUNIQUE, NTH_ELEMENT, COLLECT_LIST, COLLECT_SET;
+// cpp/src/groupby/sort/). The hash table assigns every row its group (the
+// group's output row); segmented sorts (seg_sort.hpp) then order each group's
+// values, or its rows, and small kernels read the results off the sorted
+// positions.
 #pragma once
 
 #include "groupby_host.hpp"
@@ -87,48 +87,6 @@ enum SortMode : int32_t {
   SORT_ROWS_ALL = 2,   // every row of a group, by group in row order: NTH_ELEMENT INCLUDE
 };
 
-struct SortPrepParams {
-  ValCol v;
-  int64_t num_rows;
-  uint32_t groups;
-  int32_t mode;
-  int32_t narrow;         // SORT_BY_VALUE on a column of at most 4 bytes
-  int32_t pad;
-  const uint32_t* gid;    // group of every row (row_group_kernel)
-  uint64_t* key64;        // narrow: group << 32 | 32-bit code
-  uint64_t* code;         // 8-byte values: 64-bit code
-  uint32_t* gkey;         // not narrow: the row's group if it takes part, else `groups`
-  uint32_t* idx;          // row order: the row's index
-  uint32_t* has_null;     // optional, per group: 1 when a row of the group has a null value
-};
-
-// Sort keys of every row; rows that take no part get group `groups` and sort last.
-__global__ __launch_bounds__(kBlock) void sort_prepare_kernel(const SortPrepParams s)
-{
-  const int64_t step = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < s.num_rows; row += step) {
-    const uint32_t g = s.gid[row];
-    if (s.mode == SORT_ROWS_ALL) {
-      s.gkey[row] = g < s.groups ? g : s.groups;
-      s.idx[row] = (uint32_t)row;
-      continue;
-    }
-    const bool valid = bit_valid(s.v.mask, s.v.bit_off + row);
-    const bool part = g < s.groups && valid;
-    if (s.has_null && g < s.groups && !valid) s.has_null[g] = 1u;
-    if (s.mode == SORT_ROWS_VALID) {
-      s.gkey[row] = part ? g : s.groups;
-      s.idx[row] = (uint32_t)row;
-    } else if (s.narrow) {
-      const uint32_t c = part ? sort_code32(load_at(s.v.data, s.v.width, s.v.stride, row), s.v.width, s.v.vclass) : 0u;
-      s.key64[row] = ((uint64_t)(part ? g : s.groups) << 32) | c;
-    } else {
-      s.gkey[row] = part ? g : s.groups;
-      s.code[row] = part ? sort_code64(load_at(s.v.data, 8, s.v.stride, row), s.v.vclass) : 0ull;
-    }
-  }
-}
-
 // One ordering's sorted rows.
 struct SortView {
   const uint64_t* key64;   // narrow: sorted group << 32 | code
@@ -136,7 +94,8 @@ struct SortView {
   const uint64_t* code;    // 8-byte values: sorted codes
   const uint32_t* order;   // row order: the sorted rows' indices
   const uint32_t* offset;  // groups + 1 entries: first sorted position of every group
-  int64_t n;
+  const uint32_t* total;   // sorted positions in use (offset[groups]; device memory)
+  int64_t n;               // positions the buffers hold (at least *total)
   uint32_t groups;
   int32_t narrow;
 };
@@ -151,18 +110,6 @@ __device__ __forceinline__ uint64_t view_code(const SortView& s, int64_t i)
   return s.narrow ? (s.key64[i] & 0xffffffffull) : s.code[i];
 }
 
-// offset[g] = first sorted position of group g (every position i from 1 to n
-// fills the groups between its predecessor's group and its own).
-__global__ __launch_bounds__(kBlock) void group_bounds_kernel(const SortView s, uint32_t* offset)
-{
-  const int64_t step = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= s.n; i += step) {
-    const int64_t prev = i == 0 ? -1 : (int64_t)view_group(s, i - 1);
-    const int64_t cur = i == s.n ? (int64_t)s.groups : (int64_t)view_group(s, i);
-    for (int64_t g = prev + 1; g <= cur; ++g) offset[g] = (uint32_t)i;
-  }
-}
-
 struct SortOutParams {
   ValCol v;               // the value column (NTH_ELEMENT reads its rows)
   SortView s;
@@ -171,7 +118,7 @@ struct SortOutParams {
   int32_t n;              // NTH_ELEMENT
   int32_t nulls;          // NUNIQUE: 1 = INCLUDE
   double q;               // QUANTILE
-  const uint32_t* prefix; // NUNIQUE: inclusive sum of the distinct-value starts
+  const uint32_t* distinct; // NUNIQUE: distinct values per group
   const uint32_t* has_null;
   void* out;              // per group: result
   uint8_t* out_valid;     // per group: validity byte
@@ -238,41 +185,74 @@ __global__ __launch_bounds__(kBlock) void sort_select_kernel(const SortOutParams
   p.out_valid[g] = 1;
 }
 
-// NUNIQUE: 1 at every sorted position that starts its group or differs from
-// the previous value (group_nunique.cu:60-70; NaN == NaN, -0.0 == +0.0, as
-// cudf's row equality).
-// nan_distinct (COLLECT_SET under nan_equality UNEQUAL): every NaN starts an
-// element of its own.
+// Whether sorted position i (0 < i < *total) starts its group or differs
+// from the previous value (group_nunique.cu:60-70; NaN == NaN, -0.0 ==
+// +0.0, as cudf's row equality). nan_distinct (COLLECT_SET under
+// nan_equality UNEQUAL): every NaN starts an element of its own.
+__device__ __forceinline__ bool distinct_at(const SortView& s, int64_t i, int32_t is_float, int32_t nan_distinct)
+{
+  const uint64_t nz = neg_zero_code(s.narrow != 0);
+  const uint64_t nan_code = s.narrow ? 0xffffffffull : ~0ull;
+  if (nan_distinct && is_float && view_code(s, i) == nan_code) return true;
+  if (s.narrow) {
+    const uint64_t a = s.key64[i - 1], b = s.key64[i];
+    const bool zeros = is_float && (a >> 32) == (b >> 32) && (a & 0xffffffffull) == nz && (b & 0xffffffffull) == nz + 1;
+    return a != b && !zeros;
+  }
+  const uint64_t a = s.code[i - 1], b = s.code[i];
+  const bool same = a == b || (is_float && a == nz && b == nz + 1);
+  return s.gkey[i - 1] != s.gkey[i] || !same;
+}
+
+// 1 at every sorted position distinct_at marks (and at position 0), 0 at the
+// positions from *total to n (so that the exclusive sum of n + 1 flags holds
+// every group's count).
 __global__ __launch_bounds__(kBlock) void distinct_start_kernel(const SortView s, int32_t is_float, uint32_t* flag,
                                                                int32_t nan_distinct = 0)
 {
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
-  const uint64_t nz = neg_zero_code(s.narrow != 0);
-  const uint64_t nan_code = s.narrow ? 0xffffffffull : ~0ull;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += step) {
-    uint32_t f = 1;
-    if (i > 0 && !(nan_distinct && is_float && view_code(s, i) == nan_code)) {
-      if (s.narrow) {
-        const uint64_t a = s.key64[i - 1], b = s.key64[i];
-        const bool zeros = is_float && (a >> 32) == (b >> 32) && (a & 0xffffffffull) == nz && (b & 0xffffffffull) == nz + 1;
-        f = (a != b && !zeros) ? 1u : 0u;
-      } else {
-        const uint64_t a = s.code[i - 1], b = s.code[i];
-        const bool same = a == b || (is_float && a == nz && b == nz + 1);
-        f = (s.gkey[i - 1] != s.gkey[i] || !same) ? 1u : 0u;
-      }
+  const int64_t used = *s.total;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= s.n; i += step)
+    flag[i] = i < used && (i == 0 || distinct_at(s, i, is_float, nan_distinct)) ? 1u : 0u;
+}
+
+// NUNIQUE's distinct values of every group, added into cnt[group] (zero on
+// entry): each wave takes 64 consecutive sorted positions, and each run of
+// one group's positions there adds its count of distinct_at positions once.
+__global__ __launch_bounds__(kBlock) void nunique_count_kernel(const SortView s, int32_t is_float, uint32_t* cnt)
+{
+  const int64_t used = *s.total;
+  const int lane = threadIdx.x & 63;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < used; i0 += step) {
+    const int64_t i = i0 + threadIdx.x;
+    const bool ok = i < used;
+    uint32_t g = 0xffffffffu;
+    bool f = false;
+    if (ok) {
+      g = view_group(s, i);
+      f = i == 0 || distinct_at(s, i, is_float, 0);
     }
-    flag[i] = f;
+    const uint32_t gp = __shfl_up(g, 1, 64);
+    const bool start = ok && (lane == 0 || gp != g);
+    const uint64_t starts = __ballot(start);
+    const uint64_t flags = __ballot(f);
+    if (start) {
+      const uint64_t after = starts & ~((2ull << lane) - 1ull);  // lane 63: 2 << 63 wraps to 0, none after
+      const uint64_t below_next = after ? (after & (~after + 1ull)) - 1ull : ~0ull;
+      const uint64_t run = below_next & ~((1ull << lane) - 1ull);
+      const uint32_t c = (uint32_t)__popcll(flags & run);
+      if (c) atomicAdd(&cnt[g], c);
+    }
   }
 }
 
-// NUNIQUE per group from the prefix sums; INCLUDE counts a null value once.
+// NUNIQUE per group from its distinct values; INCLUDE counts a null value once.
 __global__ __launch_bounds__(kBlock) void nunique_finish_kernel(const SortOutParams p)
 {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= p.s.groups) return;
-  const uint32_t lo = p.s.offset[g], hi = p.s.offset[g + 1];
-  uint32_t c = hi > lo ? p.prefix[hi - 1] - (lo ? p.prefix[lo - 1] : 0u) : 0u;
+  uint32_t c = p.distinct[g];
   if (p.nulls && p.has_null[g]) c += 1u;
   reinterpret_cast<uint32_t*>(p.out)[g] = c;
   p.out_valid[g] = 1;
@@ -311,7 +291,7 @@ __device__ __forceinline__ uint64_t code_bits(uint64_t c, bool narrow, int vclas
 struct CollectParams {
   ValCol v;
   SortView s;
-  const uint32_t* prefix;   // COLLECT_SET: inclusive sum of the distinct-value starts
+  const uint32_t* prefix;   // COLLECT_SET: exclusive sum of the distinct-value starts
   const uint32_t* flag;     // COLLECT_SET: the starts
   const uint32_t* nulls;    // COLLECT_SET: null values per group
   uint32_t* list_off;       // groups + 1 entries: first element of every list
@@ -338,8 +318,7 @@ __global__ __launch_bounds__(kBlock) void collect_gather_kernel(const CollectPar
 // The group's distinct values and the elements its nulls take.
 __device__ __forceinline__ uint32_t set_distinct(const CollectParams& p, int64_t g)
 {
-  const uint32_t lo = p.s.offset[g], hi = p.s.offset[g + 1];
-  return hi > lo ? p.prefix[hi - 1] - (lo ? p.prefix[lo - 1] : 0u) : 0u;
+  return p.prefix[p.s.offset[g + 1]] - p.prefix[p.s.offset[g]];
 }
 __device__ __forceinline__ uint32_t set_null_elems(const CollectParams& p, int64_t g)
 {
@@ -347,14 +326,14 @@ __device__ __forceinline__ uint32_t set_null_elems(const CollectParams& p, int64
   return !p.include_nulls ? 0u : p.nulls_unequal ? n : (n ? 1u : 0u);
 }
 
-// COLLECT_SET: each list's length into list_off[g + 1] (list_off[0] = 0); an
-// inclusive scan of list_off[1..] then gives the offsets.
-__global__ __launch_bounds__(kBlock) void set_lengths_kernel(const CollectParams p)
+// COLLECT_SET: each list's length into lens[g] (lens[groups] = 0); their
+// exclusive sum is the lists' offsets.
+__global__ __launch_bounds__(kBlock) void set_lengths_kernel(const CollectParams p, uint32_t* lens)
 {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g == 0) p.list_off[0] = 0;
+  if (g == p.s.groups) lens[g] = 0;
   if (g >= p.s.groups) return;
-  p.list_off[g + 1] = set_distinct(p, g) + set_null_elems(p, g);
+  lens[g] = set_distinct(p, g) + set_null_elems(p, g);
 }
 
 // COLLECT_SET: every distinct value (the start of a run of equal sorted
@@ -363,11 +342,11 @@ __global__ __launch_bounds__(kBlock) void set_values_kernel(const CollectParams 
 {
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
   const bool narrow = p.s.narrow != 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.s.n; i += step) {
+  const int64_t used = *p.s.total;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < used; i += step) {
+    if (!p.flag[i]) continue;
     const uint32_t g = view_group(p.s, i);
-    if (g >= p.s.groups || !p.flag[i]) continue;
-    const uint32_t lo = p.s.offset[g];
-    const uint32_t k = p.prefix[i] - 1u - (lo ? p.prefix[lo - 1] : 0u);
+    const uint32_t k = p.prefix[i] - p.prefix[p.s.offset[g]];
     const int64_t e = (int64_t)p.list_off[g] + k;
     store_width(p.child, e, p.v.width, code_bits(view_code(p.s, i), narrow, p.v.vclass));
     p.child_valid[e] = 1;

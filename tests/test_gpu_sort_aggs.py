@@ -134,6 +134,28 @@ def test_mixed_with_hash_aggregations(gpu):
         assert np.allclose(np.asarray(g, np.float64)[om], np.asarray(o, np.float64)[om], rtol=1e-9, equal_nan=True), a
 
 
+SEGMENT_SIZES = [1, 2, 63, 64, 65, 127, 128, 129, 255, 256, 257, 1000, 4095, 4096, 4097, 8192, 8193, 12_289, 40_000,
+                 1_500_000]
+
+
+@pytest.mark.parametrize("vt", [O.INT32, O.FLOAT32, O.FLOAT64, O.INT64])
+def test_segment_sizes(gpu, vt):
+    """Groups at every size boundary of the segmented sort: one wave (up to
+    64, 128 and 256 keys), one workgroup (up to 4096) and chunked merges (one
+    to nine rounds), rows shuffled; value orderings (narrow and wide codes)
+    and both row orderings."""
+    rng = np.random.default_rng(800 + vt)
+    sizes = np.array(SEGMENT_SIZES)
+    keys = np.repeat(np.arange(len(sizes), dtype=np.int64) * 7919 - 50_000, sizes)
+    rng.shuffle(keys)
+    n = keys.size
+    for ties in (False, True):
+        v = _values(rng, vt, n, ties=ties)
+        reqs = [((vt, v, rng.random(n) > 0.03), [MED, Q(0.75, 1), Q(0.2), NU(), NU(True), NTH(0), NTH(-1), NTH(-5, False),
+                                                 NTH(4000, False)])]
+        _compare([(O.INT64, keys, None)], reqs)
+
+
 def test_large(gpu):
     rng = np.random.default_rng(780)
     n = 2_000_000
