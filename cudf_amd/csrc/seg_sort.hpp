@@ -140,8 +140,9 @@ __global__ __launch_bounds__(kRadixBlock) void radix_hist_kernel(const RadixPass
 // [w, w + 1) * kRadixWaveKeys of the tile 64 at a time: lanes of one digit
 // find each other with one ballot per digit bit, and a running count per
 // wave and digit in LDS orders the rounds. The waves' counts then give each
-// key its place in the tile's digit-sorted order, where it is staged in LDS;
-// consecutive threads then write consecutive places of each digit's run.
+// key its place in the tile's digit-sorted order, where it is staged in LDS
+// (the keys, then the payloads in the same space); consecutive threads then
+// write consecutive places of each digit's run.
 template <typename K, bool FIRST>
 __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const RadixPass p, const SegParams s)
 {
@@ -149,11 +150,13 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const RadixP
   __shared__ uint32_t tstart[kRadixBins];
   __shared__ uint32_t gbase[kRadixBins];
   __shared__ uint32_t wsum[kRadixBlock / 64];
-  __shared__ uint32_t sg[kRadixTile];
-  __shared__ K sp[kRadixTile];
+  __shared__ uint8_t dig[kRadixTile];
+  __shared__ K stage[kRadixTile];  // the keys (as 32-bit words), then the payloads
   const uint32_t bins = 1u << p.bits, mask = bins - 1u;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (uint32_t b = threadIdx.x; b < (uint32_t)(kRadixBlock / 64 * kRadixBins); b += kRadixBlock) (&wcnt[0][0])[b] = 0u;
+  // the tile's first place per digit, read while the keys load (bins <= kRadixBlock)
+  const uint32_t my_base = threadIdx.x < bins ? p.counts[(int64_t)threadIdx.x * p.tiles + blockIdx.x] : 0u;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kRadixTile;
   const int64_t tile_n = std::min<int64_t>(kRadixTile, p.n - base);
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const RadixP
       run += c;
     }
     tot = run;  // bins <= kRadixBlock: one digit a thread
-    gbase[b] = p.counts[(int64_t)b * p.tiles + blockIdx.x];
+    gbase[b] = my_base;
   }
   // exclusive sum of the digits' totals over the workgroup (thread b holds digit b)
   uint32_t x = tot;
@@ -213,24 +216,36 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const RadixP
   for (int ww = 0; ww < w; ++ww) wb += wsum[ww];
   if (threadIdx.x < bins) tstart[threadIdx.x] = wb + x - tot;
   __syncthreads();
+  // places in the tile's digit-sorted order; keys staged and written first,
+  // then the payloads through the same LDS
+  uint32_t* stage32 = reinterpret_cast<uint32_t*>(stage);
 #pragma unroll
   for (int k = 0; k < kRadixItems; ++k) {
     const int64_t li = (int64_t)w * kRadixWaveKeys + k * 64 + lane;
     if (li < tile_n) {
       const uint32_t d = (g[k] >> p.shift) & mask;
-      const uint32_t at = tstart[d] + wcnt[w][d] + rk[k];
-      sg[at] = g[k];
-      sp[at] = v[k];
+      rk[k] += tstart[d] + wcnt[w][d];
+      stage32[rk[k]] = g[k];
     }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < tile_n; i += kRadixBlock) {
+    const uint32_t key = stage32[i];
+    const uint32_t d = (key >> p.shift) & mask;
+    dig[i] = (uint8_t)d;
+    p.gk_out[gbase[d] + ((uint32_t)i - tstart[d])] = key;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kRadixItems; ++k) {
+    const int64_t li = (int64_t)w * kRadixWaveKeys + k * 64 + lane;
+    if (li < tile_n) stage[rk[k]] = v[k];
   }
   __syncthreads();
   K* pout = static_cast<K*>(p.pl_out);
   for (int i = threadIdx.x; i < tile_n; i += kRadixBlock) {
-    const uint32_t key = sg[i];
-    const uint32_t d = (key >> p.shift) & mask;
-    const uint32_t pos = gbase[d] + ((uint32_t)i - tstart[d]);
-    p.gk_out[pos] = key;
-    pout[pos] = sp[i];
+    const uint32_t d = dig[i];
+    pout[gbase[d] + ((uint32_t)i - tstart[d])] = stage[i];
   }
 }
 
@@ -268,10 +283,12 @@ __device__ __forceinline__ uint64_t shfl_xor_k(uint64_t v, int m)
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Bitonic sort of the wave's 64*E keys, key q*64 + lane in v[q] of `lane`:
-// partners 64 or more apart sit in the same lane, nearer ones one shuffle away.
-template <typename K, int E>
-__device__ __forceinline__ void wave_bitonic(K (&v)[E], int lane)
+// Bitonic sort of N independent sets of the wave's 64*E keys (key q*64 +
+// lane of set n in v[n][q] of `lane`; the sets' networks interleave, which
+// hides the shuffles' latency): partners 64 or more apart sit in the same
+// lane, nearer ones one shuffle away.
+template <typename K, int E, int N>
+__device__ __forceinline__ void wave_bitonic(K (&v)[N][E], int lane)
 {
   constexpr int P = 64 * E;
 #pragma unroll
@@ -284,42 +301,54 @@ __device__ __forceinline__ void wave_bitonic(K (&v)[E], int lane)
           const int qp = q ^ (j >> 6);
           if (q < qp) {
             const bool up = ((((unsigned)q << 6) | (unsigned)lane) & (unsigned)k) == 0;
-            const K a = v[q], b = v[qp];
-            const bool sw = (a > b) == up;
-            v[q] = sw ? b : a;
-            v[qp] = sw ? a : b;
+#pragma unroll
+            for (int n = 0; n < N; ++n) {
+              const K a = v[n][q], b = v[n][qp];
+              const bool sw = (a > b) == up;
+              v[n][q] = sw ? b : a;
+              v[n][qp] = sw ? a : b;
+            }
           }
         }
       } else {
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-          const K o = shfl_xor_k(v[q], j);
           const bool up = ((((unsigned)q << 6) | (unsigned)lane) & (unsigned)k) == 0;
           const bool keep_min = ((lane & j) == 0) == up;
-          v[q] = keep_min ? (o < v[q] ? o : v[q]) : (o > v[q] ? o : v[q]);
+#pragma unroll
+          for (int n = 0; n < N; ++n) {
+            const K o = shfl_xor_k(v[n][q], j);
+            v[n][q] = keep_min ? (o < v[n][q] ? o : v[n][q]) : (o > v[n][q] ? o : v[n][q]);
+          }
         }
       }
     }
   }
 }
 
-// One segment of m <= 64*E keys sorted by one wave (padding keys are the
-// largest key: the first m sorted keys are the segment's).
+// Two segments of at most 64*E keys each (m = 0: none) sorted by one wave
+// (padding keys are the largest key: the first m sorted keys are the
+// segment's).
 template <typename K, int E>
-__device__ __forceinline__ void seg_wave(const SegParams& s, K* keys, uint32_t g, uint32_t lo, uint32_t m, int lane)
+__device__ __forceinline__ void seg_wave2(const SegParams& s, K* keys, const uint32_t (&g)[2], const uint32_t (&lo)[2],
+                                          const uint32_t (&m)[2], int lane)
 {
-  K v[E];
+  K v[2][E];
 #pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const uint32_t i = (uint32_t)(q * 64 + lane);
-    v[q] = i < m ? keys[lo + i] : (K)~(K)0;
-  }
-  wave_bitonic<K, E>(v, lane);
+  for (int n = 0; n < 2; ++n)
 #pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const uint32_t i = (uint32_t)(q * 64 + lane);
-    if (i < m) seg_emit(s, keys, g, (int64_t)lo + i, v[q], false);
-  }
+    for (int q = 0; q < E; ++q) {
+      const uint32_t i = (uint32_t)(q * 64 + lane);
+      v[n][q] = i < m[n] ? keys[lo[n] + i] : (K)~(K)0;
+    }
+  wave_bitonic<K, E, 2>(v, lane);
+#pragma unroll
+  for (int n = 0; n < 2; ++n)
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const uint32_t i = (uint32_t)(q * 64 + lane);
+      if (i < m[n]) seg_emit(s, keys, g[n], (int64_t)lo[n] + i, v[n][q], false);
+    }
 }
 
 // Step 4a: a wave per group; segments longer than kWaveSeg are listed, one
@@ -331,20 +360,33 @@ __global__ __launch_bounds__(256) void seg_sort_small_kernel(const SegParams s)
   const int lane = threadIdx.x & 63;
   const int64_t wid = (int64_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   const int64_t nw = (int64_t)gridDim.x * 4;
-  for (int64_t g = wid; g < (int64_t)s.groups; g += nw) {
-    const uint32_t lo = s.off[g], m = s.off[g + 1] - lo;
-    if (m == 0) continue;
-    if (m > (uint32_t)kWaveSeg) {
-      const uint32_t chunks = (m + kSegChunk - 1) / kSegChunk;
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(s.list_count, chunks);
-      base = (uint32_t)__shfl((int)base, 0, 64);
-      for (uint32_t c = lane; c < chunks; c += 64) s.list[base + c] = ((unsigned long long)g << 32) | c;
-      continue;
+  // two groups an iteration: wid + 2*i*nw and the one nw after it
+  for (int64_t g0 = wid; g0 < (int64_t)s.groups; g0 += 2 * nw) {
+    uint32_t g[2], lo[2], m[2];
+    uint32_t mmax = 0;
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+      const int64_t gg = g0 + n * nw;
+      g[n] = (uint32_t)gg;
+      lo[n] = 0;
+      m[n] = 0;
+      if (gg >= (int64_t)s.groups) continue;
+      lo[n] = s.off[gg];
+      m[n] = s.off[gg + 1] - lo[n];
+      if (m[n] > (uint32_t)kWaveSeg) {
+        const uint32_t chunks = (m[n] + kSegChunk - 1) / kSegChunk;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(s.list_count, chunks);
+        base = (uint32_t)__shfl((int)base, 0, 64);
+        for (uint32_t c = lane; c < chunks; c += 64) s.list[base + c] = ((unsigned long long)gg << 32) | c;
+        m[n] = 0;  // sorted by the workgroup kernels
+      }
+      mmax = m[n] > mmax ? m[n] : mmax;
     }
-    if (m <= 64) seg_wave<K, 1>(s, keys, (uint32_t)g, lo, m, lane);
-    else if (m <= 128) seg_wave<K, 2>(s, keys, (uint32_t)g, lo, m, lane);
-    else seg_wave<K, 4>(s, keys, (uint32_t)g, lo, m, lane);
+    if (mmax == 0) continue;
+    if (mmax <= 64) seg_wave2<K, 1>(s, keys, g, lo, m, lane);
+    else if (mmax <= 128) seg_wave2<K, 2>(s, keys, g, lo, m, lane);
+    else seg_wave2<K, 4>(s, keys, g, lo, m, lane);
   }
 }
 
