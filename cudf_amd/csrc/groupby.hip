@@ -137,31 +137,46 @@ __global__ __launch_bounds__(kBlock) void count_occupied_kernel(const uint8_t* t
 }
 
 // Exclusive scan of n block counts in place by one workgroup; total -> *total.
+// Each pass takes 16 consecutive counts a thread: the threads' sums are
+// scanned within each wave by shuffles and across the 16 waves in LDS.
+constexpr int kScanBlocksItems = 16;
 __global__ __launch_bounds__(1024) void scan_blocks_kernel(int64_t* counts, int64_t n, unsigned long long* total,
                                                           const unsigned long long* table_groups)
 {
-  __shared__ int64_t buf[1024];
-  __shared__ int64_t carry;
+  __shared__ long long wsum[16];
+  __shared__ long long carry;
   if (table_groups && *table_groups == 0) {  // every count is 0
     if (threadIdx.x == 0) *total = 0ull;
     return;
   }
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
-  for (int64_t base = 0; base < n; base += 1024) {
-    int64_t i = base + threadIdx.x;
-    int64_t v = i < n ? counts[i] : 0;
-    buf[threadIdx.x] = v;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-      int64_t t = threadIdx.x >= (unsigned)d ? buf[threadIdx.x - d] : 0;
-      __syncthreads();
-      buf[threadIdx.x] += t;
-      __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int64_t base = 0; base < n; base += 1024 * kScanBlocksItems) {
+    const int64_t first = base + (int64_t)threadIdx.x * kScanBlocksItems;
+    long long v[kScanBlocksItems];
+    long long sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanBlocksItems; ++k) {
+      v[k] = first + k < n ? (long long)counts[first + k] : 0ll;
+      sum += v[k];
     }
-    if (i < n) counts[i] = carry + buf[threadIdx.x] - v;
+    long long x = sum;
+    for (int d = 1; d < 64; d <<= 1) {
+      const long long y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
     __syncthreads();
-    if (threadIdx.x == 1023) carry += buf[1023];
+    long long run = carry + x - sum;
+    for (int ww = 0; ww < w; ++ww) run += wsum[ww];
+#pragma unroll
+    for (int k = 0; k < kScanBlocksItems; ++k) {
+      if (first + k < n) counts[first + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = run;  // the counts so far
     __syncthreads();
   }
   if (threadIdx.x == 0) *total = (unsigned long long)carry;
@@ -682,6 +697,31 @@ __global__ __launch_bounds__(kBlock) void hist_coarsen_kernel(const uint32_t* fi
     const int c = c0 + ci, part = p0 + pi;
     if (c < nw && part < P) hist[(int64_t)part * nw + c] = tile[pi][ci];
   }
+}
+
+// The same histogram when each partition spans at least 64 fine buckets
+// (log2p <= kFineBits - 6, so at most kBlock partitions): one workgroup per
+// chunk; each wave reads 64 consecutive buckets (one partition's), adds them
+// across the wave and one lane adds the sum to the partition's LDS count.
+// (The tiled kernel's threads would each sum thousands of buckets in turn.)
+__global__ __launch_bounds__(kBlock) void hist_coarsen_few_kernel(const uint32_t* fine, int32_t nw, int32_t log2p,
+                                                                 uint32_t* hist)
+{
+  __shared__ uint32_t s[kBlock];
+  const int P = 1 << log2p;
+  const int shift = kFineBits - log2p;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < P; i += kBlock) s[i] = 0u;
+  __syncthreads();
+  const int c = blockIdx.x;
+  const uint32_t* f = fine + (int64_t)c * kFineBuckets;
+  for (int g = w; g < kFineBuckets / 64; g += kBlock / 64) {
+    uint32_t v = f[g * 64 + lane];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0 && v) atomicAdd(&s[(g * 64) >> shift], v);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < P; i += kBlock) hist[(int64_t)i * nw + c] = s[i];
 }
 
 // Max over workgroups per register, then per block the sum of 2^-M and the
@@ -3126,9 +3166,14 @@ mgb_status run_partitioned(Workspace* ws, const Plan& plan, const AggParams& bas
     if (coarsen) {
       // the estimate's pass counted the rows per fine bucket of these chunks
       ProfScope prof("partition_hist", stream);
-      const dim3 grid((unsigned)((nw + 63) / 64), (unsigned)((P + 63) / 64));
-      hist_coarsen_kernel<<<grid, kBlock, 0, stream>>>(static_cast<const uint32_t*>(ws->fine.ptr), (int32_t)nw, log2p,
-                                                      pp.hist);
+      if (log2p <= kFineBits - 6) {
+        hist_coarsen_few_kernel<<<(unsigned)nw, kBlock, 0, stream>>>(static_cast<const uint32_t*>(ws->fine.ptr),
+                                                                    (int32_t)nw, log2p, pp.hist);
+      } else {
+        const dim3 grid((unsigned)((nw + 63) / 64), (unsigned)((P + 63) / 64));
+        hist_coarsen_kernel<<<grid, kBlock, 0, stream>>>(static_cast<const uint32_t*>(ws->fine.ptr), (int32_t)nw, log2p,
+                                                        pp.hist);
+      }
     } else {
       ProfScope prof("partition_hist", stream);
       part_hist_kernel<<<(unsigned)nw, kBlock, part_lds, stream>>>(pp);
