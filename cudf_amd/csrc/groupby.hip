@@ -2346,12 +2346,20 @@ mgb_status run_sort_aggs(Workspace* ws, const Plan& plan, const AggParams& rows,
       MGB_HIP(hipGetLastError());
       const mgb_status st = excl_scan(radix_counts, radix_counts, ((int64_t)1 << rp.bits) * tiles);
       if (st != MGB_SUCCESS) return st;
+      const unsigned gt = (unsigned)tiles;
       if (wide) {
-        if (ps == 0) radix_scatter_kernel<uint64_t, true><<<(unsigned)tiles, kRadixBlock, 0, stream>>>(rp, sp);
-        else radix_scatter_kernel<uint64_t, false><<<(unsigned)tiles, kRadixBlock, 0, stream>>>(rp, sp);
+        if (ps == 0) radix_scatter_kernel<uint64_t, true, 8><<<gt, kRadixBlock, 0, stream>>>(rp, sp);
+        else radix_scatter_kernel<uint64_t, false, 0><<<gt, kRadixBlock, 0, stream>>>(rp, sp);
+      } else if (ps > 0) {
+        radix_scatter_kernel<uint32_t, false, 0><<<gt, kRadixBlock, 0, stream>>>(rp, sp);
       } else {
-        if (ps == 0) radix_scatter_kernel<uint32_t, true><<<(unsigned)tiles, kRadixBlock, 0, stream>>>(rp, sp);
-        else radix_scatter_kernel<uint32_t, false><<<(unsigned)tiles, kRadixBlock, 0, stream>>>(rp, sp);
+        // the first pass loads the value at its width (none for the row orderings)
+        switch (ord.mode == SORT_BY_VALUE ? sp.v.width : 0) {
+          case 1: radix_scatter_kernel<uint32_t, true, 1><<<gt, kRadixBlock, 0, stream>>>(rp, sp); break;
+          case 2: radix_scatter_kernel<uint32_t, true, 2><<<gt, kRadixBlock, 0, stream>>>(rp, sp); break;
+          case 4: radix_scatter_kernel<uint32_t, true, 4><<<gt, kRadixBlock, 0, stream>>>(rp, sp); break;
+          default: radix_scatter_kernel<uint32_t, true, 0><<<gt, kRadixBlock, 0, stream>>>(rp, sp); break;
+        }
       }
       MGB_HIP(hipGetLastError());
       std::swap(gk_a, gk_b);

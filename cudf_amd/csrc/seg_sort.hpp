@@ -3,9 +3,10 @@
 // table already assigned. cpp/src/groupby/sort/sort_helper.cu sorts every
 // row by key and then by value; here every row already carries its group (a
 // dense index below 2^32), so:
-//   1. seg_source: every row's group key (`groups` for a row that takes no
-//      part, which sorts last) and payload (value code, or the row index for
-//      the row orderings), computed as the first radix pass reads the row;
+//   1. seg_load / seg_key: every row's group key (`groups` for a row that
+//      takes no part, which sorts last) and payload (value code, or the row
+//      index for the row orderings), computed as the first radix pass reads
+//      the row;
 //   2. a stable LSD radix sort of (group key, payload) over only the bits
 //      the group count needs (radix_hist_kernel, a scan, radix_scatter_kernel
 //      per pass of at most 8 bits): the rows come out group-contiguous and in
@@ -72,20 +73,53 @@ __device__ __forceinline__ bool seg_takes_part(const SegParams& s, int64_t row, 
 // which sorts last) and its payload (never read for such a row). The first
 // radix pass reads the rows through this; later passes read the previous
 // pass's output.
-template <typename K>
-__device__ __forceinline__ uint32_t seg_source(const SegParams& s, int64_t row, K* pl)
+// The first pass's loads of one row, without branches, so that all of a
+// thread's loads are in flight together: its group, the word of its
+// validity bit (a word of the group array when no mask applies: any in-range
+// word, never used), and VW bytes of its value (VW = 0: none).
+struct SegRaw {
+  uint32_t g;
+  uint32_t mword;
+  uint64_t raw;
+};
+
+__device__ __forceinline__ bool seg_masked(const SegParams& s) { return s.v.mask != nullptr && s.mode != SORT_ROWS_ALL; }
+
+template <int VW>
+__device__ __forceinline__ SegRaw seg_load(const SegParams& s, int64_t row)
 {
-  const uint32_t g = s.gid[row];
-  const bool part = seg_takes_part(s, row, g);
-  if (pl) {
-    K k = 0;
-    if (s.mode != SORT_BY_VALUE) k = (K)row;
-    else if (!part) k = 0;
-    else if constexpr (sizeof(K) == 4) k = sort_code32(load_at(s.v.data, s.v.width, s.v.stride, row), s.v.width, s.v.vclass);
-    else k = sort_code64(load_at(s.v.data, 8, s.v.stride, row), s.v.vclass);
-    *pl = k;
-  }
-  return part ? g : s.groups;
+  SegRaw r;
+  r.g = s.gid[row];
+  const bool masked = seg_masked(s);
+  const uint32_t* mp = masked ? s.v.mask : s.gid;
+  const int64_t bit = masked ? s.v.bit_off + row : row;
+  r.mword = to_global(mp)[bit >> 5];
+  const global_t<const uint8_t>* q = to_global(s.v.data) + (VW ? row * (int64_t)s.v.stride : 0);
+  if constexpr (VW == 1) r.raw = *q;
+  else if constexpr (VW == 2) r.raw = *reinterpret_cast<const global_t<uint16_t>*>(q);
+  else if constexpr (VW == 4) r.raw = *reinterpret_cast<const global_t<uint32_t>*>(q);
+  else if constexpr (VW == 8) r.raw = *reinterpret_cast<const global_t<uint64_t>*>(q);
+  else r.raw = 0;
+  return r;
+}
+
+// Whether the row takes part: its key's group is kept and (value orderings,
+// the valid-row ordering) its value is valid.
+__device__ __forceinline__ bool seg_valid(const SegParams& s, int64_t row, const SegRaw& r)
+{
+  const bool masked = seg_masked(s);
+  const int64_t bit = masked ? s.v.bit_off + row : row;
+  return !masked || ((r.mword >> (bit & 31)) & 1u);
+}
+
+// The group key (`groups` for a row that takes no part) and the payload.
+template <typename K, int VW>
+__device__ __forceinline__ uint32_t seg_key(const SegParams& s, int64_t row, const SegRaw& r, K& pl)
+{
+  if (s.mode != SORT_BY_VALUE) pl = (K)row;
+  else if constexpr (sizeof(K) == 4) pl = sort_code32(r.raw, VW, s.v.vclass);
+  else pl = sort_code64(r.raw, s.v.vclass);
+  return r.g < s.groups && seg_valid(s, row, r) ? r.g : s.groups;
 }
 
 // One pass of the radix sort: digit (gk >> shift) & (2^bits - 1); tiles of
@@ -104,7 +138,7 @@ struct RadixPass {
 
 // Step 2a: every tile's digit counts, digit-major (so that their exclusive
 // sum is, per digit and tile, the tile's first position for the digit).
-// FIRST: the keys come from the rows (seg_source), and groups with a null
+// FIRST: the keys come from the rows (seg_load, seg_key), and groups with a null
 // value are flagged in s.has_null when it is set.
 template <bool FIRST>
 __global__ __launch_bounds__(kRadixBlock) void radix_hist_kernel(const RadixPass p, const SegParams s)
@@ -115,19 +149,31 @@ __global__ __launch_bounds__(kRadixBlock) void radix_hist_kernel(const RadixPass
   for (uint32_t b = threadIdx.x; b < (uint32_t)(kRadixBlock / 64 * kRadixBins); b += kRadixBlock) (&h[0][0])[b] = 0u;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kRadixTile;
-#pragma unroll 4
-  for (int k = 0; k < kRadixItems; ++k) {
-    const int64_t i = base + k * kRadixBlock + threadIdx.x;
-    if (i >= p.n) continue;
-    uint32_t key;
-    if constexpr (FIRST) {
-      key = seg_source<uint32_t>(s, i, nullptr);
-      if (s.has_null && s.gid[i] < s.groups && !bit_valid(s.v.mask, s.v.bit_off + i)) s.has_null[s.gid[i]] = 1u;
-    } else {
-      key = p.gk_in[i];
+  uint32_t key[kRadixItems];
+  if constexpr (FIRST) {
+    SegRaw raw[kRadixItems];
+#pragma unroll
+    for (int k = 0; k < kRadixItems; ++k) {
+      const int64_t i = base + k * kRadixBlock + threadIdx.x;
+      raw[k] = seg_load<0>(s, i < p.n ? i : p.n - 1);
     }
-    atomicAdd(&h[w][(key >> p.shift) & mask], 1u);
+#pragma unroll
+    for (int k = 0; k < kRadixItems; ++k) {
+      const int64_t i = base + k * kRadixBlock + threadIdx.x;
+      uint32_t dummy;
+      key[k] = seg_key<uint32_t, 0>(s, i, raw[k], dummy);
+      if (i < p.n && s.has_null && raw[k].g < s.groups && !seg_valid(s, i, raw[k])) s.has_null[raw[k].g] = 1u;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kRadixItems; ++k) {
+      const int64_t i = base + k * kRadixBlock + threadIdx.x;
+      key[k] = i < p.n ? p.gk_in[i] : 0u;
+    }
   }
+#pragma unroll
+  for (int k = 0; k < kRadixItems; ++k)
+    if (base + k * kRadixBlock + threadIdx.x < p.n) atomicAdd(&h[w][(key[k] >> p.shift) & mask], 1u);
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < bins; b += kRadixBlock) {
     uint32_t c = 0;
@@ -143,7 +189,7 @@ __global__ __launch_bounds__(kRadixBlock) void radix_hist_kernel(const RadixPass
 // key its place in the tile's digit-sorted order, where it is staged in LDS
 // (the keys, then the payloads in the same space); consecutive threads then
 // write consecutive places of each digit's run.
-template <typename K, bool FIRST>
+template <typename K, bool FIRST, int VW>
 __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const RadixPass p, const SegParams s)
 {
   __shared__ uint32_t wcnt[kRadixBlock / 64][kRadixBins];
@@ -166,17 +212,32 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const RadixP
   K v[kRadixItems];
   uint32_t rk[kRadixItems];
   const uint64_t lt = (1ull << lane) - 1ull;
+  // every load issued before the ranking rounds (which wait on LDS)
+  if constexpr (FIRST) {
+    SegRaw raw[kRadixItems];
+#pragma unroll
+    for (int k = 0; k < kRadixItems; ++k) {
+      const int64_t li = (int64_t)w * kRadixWaveKeys + k * 64 + lane;
+      raw[k] = seg_load<VW>(s, base + (li < tile_n ? li : tile_n - 1));
+    }
+#pragma unroll
+    for (int k = 0; k < kRadixItems; ++k) {
+      const int64_t li = (int64_t)w * kRadixWaveKeys + k * 64 + lane;
+      g[k] = seg_key<K, VW>(s, base + li, raw[k], v[k]);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kRadixItems; ++k) {
+      const int64_t li = (int64_t)w * kRadixWaveKeys + k * 64 + lane;
+      const bool ok = li < tile_n;
+      g[k] = ok ? gin[base + li] : 0u;
+      v[k] = ok ? pin[base + li] : (K)0;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < kRadixItems; ++k) {
     const int64_t li = (int64_t)w * kRadixWaveKeys + k * 64 + lane;
     const bool ok = li < tile_n;
-    if constexpr (FIRST) {
-      v[k] = (K)0;
-      g[k] = ok ? seg_source<K>(s, base + li, &v[k]) : 0u;
-    } else {
-      g[k] = ok ? gin[base + li] : 0u;
-      v[k] = ok ? pin[base + li] : (K)0;
-    }
     const uint32_t d = (g[k] >> p.shift) & mask;
     uint64_t peers = __ballot(ok);
     for (int b = 0; b < p.bits; ++b) {
