@@ -165,7 +165,10 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
   // the thread's rows of one tile, as loaded (rows past the end load the
   // last row and are not staged)
   uint64_t key[I], key1[I], v0[I], v1[I], v2[I];
-  uint32_t km[I], m0[I], m1[I], m2[I];
+  // km1: the second key column's mask word, kept apart from km until the
+  // row is tested (combining them here would wait for km's load inside the
+  // prefetch of the next tile)
+  uint32_t km[I], km1[I], m0[I], m1[I], m2[I];
   auto fetch = [&](int64_t tile) {
 #pragma unroll
     for (int k = 0; k < I; ++k) {
@@ -176,7 +179,7 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
       km[k] = a.key_mask ? to_global(a.key_mask)[r >> 5] : ~0u;
       if constexpr (K1 != 0) {
         key1[k] = lean_load<K1>(a.key1, r);
-        if (a.key1_mask) km[k] &= to_global(a.key1_mask)[r >> 5];
+        km1[k] = a.key1_mask ? to_global(a.key1_mask)[r >> 5] : ~0u;
       }
       if constexpr (W0 != 0) {
         v0[k] = lean_load<W0>(a.val[0], r);
@@ -204,7 +207,9 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
       const int64_t r = tile + threadIdx.x + (int64_t)k * kLdsBlock;
       const uint32_t bit = (uint32_t)r & 31u;
       bitk[k] = bit;
-      present[k] = k < items && r < end && ((km[k] >> bit) & 1u);
+      uint32_t kmk = km[k];
+      if constexpr (K1 != 0) kmk &= km1[k];
+      present[k] = k < items && r < end && ((kmk >> bit) & 1u);
       // cudf's row hash: hash_combine over the key columns from seed 0
       uint32_t h = hash_combine(0u, murmur3_bits(key[k], K0, 0));
       if constexpr (K1 != 0) h = hash_combine(h, murmur3_bits(key1[k], K1, 0));

@@ -315,11 +315,23 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const RadixP
 // predecessor's group and its own).
 __global__ __launch_bounds__(kBlock) void seg_bounds_kernel(const uint32_t* gk, int64_t n, uint32_t groups, uint32_t* off)
 {
-  const int64_t step = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += step) {
-    const int64_t prev = i == 0 ? -1 : (int64_t)gk[i - 1];
-    const int64_t cur = i == n ? (int64_t)groups : (int64_t)gk[i];
-    for (int64_t g = prev + 1; g <= cur; ++g) off[g] = (uint32_t)i;
+  // 4 positions a thread at a time, their loads in flight together
+  constexpr int U = 4;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x * U;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x * U + threadIdx.x; base <= n; base += step) {
+    int64_t prev[U], cur[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = base + (int64_t)u * blockDim.x;
+      prev[u] = i == 0 || i > n ? -1 : (int64_t)gk[i - 1];
+      cur[u] = i >= n ? (int64_t)groups : (int64_t)gk[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = base + (int64_t)u * blockDim.x;
+      if (i > n) continue;
+      for (int64_t g = prev[u] + 1; g <= cur[u]; ++g) off[g] = (uint32_t)i;
+    }
   }
 }
 

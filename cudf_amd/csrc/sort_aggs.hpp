@@ -17,10 +17,24 @@ __global__ __launch_bounds__(kBlock) void row_group_kernel(const AggParams p, co
                                                            uint32_t* gid)
 {
   if (p.row_rec) {
-    const int64_t step = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < p.num_rows; row += step) {
-      const uint32_t r = p.row_rec[row];
-      gid[row] = r == kNoRecord ? groups : rec_pos[r];
+    // 8 rows a thread at a time: their record loads, then their output-row
+    // loads, each batch in flight together
+    constexpr int U = 8;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x * U;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x * U + threadIdx.x; base < p.num_rows; base += step) {
+      uint32_t r[U], g[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t row = base + (int64_t)u * blockDim.x;
+        r[u] = row < p.num_rows ? p.row_rec[row] : kNoRecord;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) g[u] = r[u] == kNoRecord ? groups : rec_pos[r[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t row = base + (int64_t)u * blockDim.x;
+        if (row < p.num_rows) gid[row] = g[u];
+      }
     }
     return;
   }
