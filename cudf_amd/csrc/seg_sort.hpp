@@ -348,11 +348,26 @@ __device__ __forceinline__ void seg_emit(const SegParams& s, K* keys, uint32_t g
   }
 }
 
-__device__ __forceinline__ uint32_t shfl_xor_k(uint32_t v, int m) { return (uint32_t)__shfl_xor((int)v, m, 64); }
+// The value of lane (lane ^ m). Partners within a row of 16 lanes move by
+// DPP (no LDS crossbar): m = 1 and 2 by quad permutes, 8 by a row rotate of
+// 8, 4 by a half-row mirror (lane i of 8 from 7 - i) then a quad mirror
+// (lane i of 4 from 3 - i); 16 and 32 by ds_bpermute. m is a constant once
+// the networks are unrolled.
+__device__ __forceinline__ uint32_t shfl_xor_k(uint32_t v, int m)
+{
+  switch (m) {
+    case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);
+    case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);
+    case 4: return (uint32_t)__builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, true), 0x1B, 0xF,
+                                                      0xF, true);
+    case 8: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, true);
+    default: return (uint32_t)__shfl_xor((int)v, m, 64);
+  }
+}
 __device__ __forceinline__ uint64_t shfl_xor_k(uint64_t v, int m)
 {
-  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
-  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+  const uint32_t lo = shfl_xor_k((uint32_t)v, m);
+  const uint32_t hi = shfl_xor_k((uint32_t)(v >> 32), m);
   return ((uint64_t)hi << 32) | lo;
 }
 
