@@ -156,6 +156,20 @@ def test_segment_sizes(gpu, vt):
         _compare([(O.INT64, keys, None)], reqs)
 
 
+@pytest.mark.parametrize("card", [1, 2, 127, 128, 255, 256, 16_383, 16_384, 65_536])
+def test_radix_pass_boundaries(gpu, card):
+    """Group counts at the edges of the radix passes over the group bits
+    (one pass of up to 8 bits, two, three with uneven digit widths), every
+    group present, value and row orderings."""
+    rng = np.random.default_rng(860 + card)
+    n = max(4 * card, 50_000)
+    keys = np.concatenate([np.arange(card), rng.integers(0, card, n - card)]).astype(np.int64) * 3 - 7
+    rng.shuffle(keys)
+    reqs = [((O.FLOAT64, _values(rng, O.FLOAT64, n), rng.random(n) > 0.05), [MED, Q(0.9, 1), NU(True), NTH(0), NTH(-1, False)]),
+            ((O.INT32, _values(rng, O.INT32, n, ties=True), rng.random(n) > 0.05), [MED, NU(), NTH(1)])]
+    _compare([(O.INT64, keys, None)], reqs)
+
+
 def test_large(gpu):
     rng = np.random.default_rng(780)
     n = 2_000_000
