@@ -2065,7 +2065,9 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
   st = MGB_SUCCESS;
   ws->fine_valid = false;
   ws->fine_hot = false;
-  const int64_t kSample = 1 << 16;
+  // a call that collects hot keys samples twice the rows, so that keys of
+  // about 1 row in 16K reach kHotMin sightings
+  const int64_t kSample = hot ? 1 << 17 : 1 << 16;
   const int64_t scap = 4 * kSample;
   // small inputs are counted exactly; larger ones are sampled
   const int64_t samples = R <= scap / 2 ? R : kSample;

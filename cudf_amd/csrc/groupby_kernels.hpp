@@ -186,9 +186,9 @@ __host__ __device__ constexpr LeanOffsets lean_offsets(int kw, int w0, int w1, i
 // own, folded into the global table when the chunk is done, instead of
 // writing them as partitioned records: under skew (a few keys holding much
 // of the rows) the records and their aggregation shrink accordingly.
-constexpr int kHotMax = 256;
-constexpr uint32_t kHotMin = 16;
-constexpr int kHotSlots = 2048;  // LDS lookup slots (a power of two; few occupied, so a miss ends at its home slot)
+constexpr int kHotMax = 1024;
+constexpr uint32_t kHotMin = 8;
+constexpr int kHotSlots = 4096;  // LDS lookup slots (a power of two; few occupied, so a miss ends at its home slot)
 struct HotKey {
   uint64_t c0, c1;  // key column 0 and 1 (0 for a single column)
 };

@@ -138,7 +138,7 @@ __device__ __forceinline__ void lean_scatter_rows(const LeanScatterArgs& a, uint
                   K0, K1);
     int32_t* ctr = reinterpret_cast<int32_t*>(smem + lay.hot_ctr);  // count, overflow, 2 fold words
     touched = reinterpret_cast<uint32_t*>(ctr + 4);
-    hdum = (uint32_t)(lay.hot_ctr + 48) + (threadIdx.x & 63u) * 8u;
+    hdum = (uint32_t)(lay.hot_ctr + 16 + kHotMax / 8) + (threadIdx.x & 63u) * 8u;  // past the touched bits
     ht.rec = smem + lay.hot_rec;
     ht.slots = hl.n;
     ht.stride = a.hot_words * 8;

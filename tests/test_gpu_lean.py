@@ -273,7 +273,7 @@ def test_two_column_same_result_as_general_records(gpu, monkeypatch):
             assert (ma is None and mb is None) or np.array_equal(ma, mb)
 
 
-# ---- hot keys: keys seen 16 times or more in the estimate's sample skip the
+# ---- hot keys: keys seen 8 times or more in the estimate's sample skip the
 # partitioned records and are aggregated by the scatter
 
 def _skewed_keys(rng, n, pool, hot_frac, n_hot):
@@ -316,12 +316,13 @@ def test_hot_keys_two_columns(gpu, types):
 
 
 def test_hot_keys_more_than_the_cap_and_null_keys(gpu):
-    """1000 keys each about 0.05% of the rows: more hot candidates than the
-    256 taken (the rest are partitioned as usual); null keys dropped."""
+    """Half the rows on 3000 keys (about 2000 of them seen 8 times or more in
+    the estimate's sample): more hot candidates than the 1024 taken (the rest
+    are partitioned as usual); null keys dropped."""
     rng = np.random.default_rng(640)
     n = 3_000_000
     pool = rng.integers(-2 ** 40, 2 ** 40, 500_000)
-    keys = _skewed_keys(rng, n, pool, 0.5, 1000)
+    keys = _skewed_keys(rng, n, pool, 0.5, 3000)
     kvalid = rng.random(n) > 0.05
     _, reqs = _random_case(rng, n, O.INT64, 10, [O.INT32, O.FLOAT32], 0.05, KINDS)
     _compare([(O.INT64, keys, kvalid)], reqs)
