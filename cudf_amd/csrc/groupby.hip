@@ -2065,10 +2065,11 @@ int64_t sample_estimate(Workspace* ws, const AggParams& base, int64_t R, hipStre
   st = MGB_SUCCESS;
   ws->fine_valid = false;
   ws->fine_hot = false;
-  // a call that collects hot keys samples twice the rows, so that keys of
-  // about 1 row in 16K reach kHotMin sightings
-  const int64_t kSample = hot ? 1 << 17 : 1 << 16;
-  const int64_t scap = 4 * kSample;
+  // a call of 8M rows or more that collects hot keys samples twice the
+  // rows, so that keys of about 1 row in 16K reach kHotMin sightings (the
+  // sample table keeps its 2^18 slots: at most half of them taken)
+  const int64_t scap = 4 << 16;
+  const int64_t kSample = hot && R >= (1ll << 23) ? 1 << 17 : 1 << 16;
   // small inputs are counted exactly; larger ones are sampled
   const int64_t samples = R <= scap / 2 ? R : kSample;
   const bool may_need_hll = samples < R;
